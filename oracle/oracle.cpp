@@ -1,0 +1,693 @@
+// CPU oracle: FP64 restatement of the reference's H-matrix hot path.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h): the product library never links or
+// calls this file. It restates, step by step, the reference files cited at each
+// function (paths relative to the reference root, `H/` = include/htool/).
+//
+// Deliberate restatement choices, kept identical in the HIP library so that
+// results can be compared bitwise where the arithmetic allows it:
+//   * kernel entries: squared distance summed x, y, z from zero without FMA,
+//     then sqrt, exactly as the reference's generators (H/testing/generator_test.hpp:97)
+//   * ACA residual updates use fused multiply-add, as an FMA-capable BLAS daxpy does
+//   * the ACA Frobenius dot products are sequential sums, like the reference's
+//     hand-written Blas::dot (H/wrappers/wrapper_blas.hpp:111-117)
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <utility>
+#include <vector>
+
+namespace {
+
+// ----------------------------------------------------------------------------
+// Kernel entries (H/testing/generator_test.hpp:93-103, 135-145;
+// examples/smallest_example.cpp:16-27 for the smooth kernel).
+struct Kernel {
+    int kind;
+    double delta, diag;
+    bool same;
+    const double *xt, *xs;
+    double operator()(int i, int j) const {
+        const double *a = xt + 3 * i, *b = xs + 3 * j;
+        double r2 = 0;
+        for (int d = 0; d < 3; d++) {
+            double u = a[d] - b[d];
+            r2       = r2 + u * u;
+        }
+        switch (kind) {
+        case 0:
+            if (same && i == j)
+                return diag;
+            return 1. / (4 * M_PI * std::sqrt(r2));
+        case 1:
+            return 1. / (delta + std::sqrt(r2));
+        case 2:
+            return 1. / (delta + 4 * M_PI * std::sqrt(r2));
+        default:
+            return 1. / (4 * M_PI * std::sqrt(delta + r2));
+        }
+    }
+};
+
+// ----------------------------------------------------------------------------
+// Cluster tree: PCA with geometric splitting (H/clustering/pca.hpp:15-128,
+// H/clustering/splitting.hpp:63-120, H/misc/evp.hpp:43-135). One MPI rank, so
+// the root has exactly one son (pca.hpp:28) and the root rank is -1
+// (cluster.hpp:81).
+struct Node {
+    int offset = 0, size = 0, depth = 0, rank = -1;
+    double ctr[3] = {0, 0, 0};
+    double rad    = 0;
+    std::vector<int> sons;
+};
+
+double dot3(const double *a, const double *b) { // std::inner_product from zero
+    double s = 0;
+    for (int d = 0; d < 3; d++)
+        s = s + a[d] * b[d];
+    return s;
+}
+
+// Largest-eigenvalue direction of a symmetric 3x3 matrix (H/misc/evp.hpp:43-133).
+// cov is column-major.
+void evp3(const double cov[9], double dir[3]) {
+    auto C = [&](int i, int j) { return cov[i + 3 * j]; };
+    dir[0] = dir[1] = dir[2] = 0;
+    double p1 = std::pow(C(0, 1), 2) + std::pow(C(0, 2), 2) + std::pow(C(1, 2), 2);
+    double eigs[3];
+    if (p1 < 1e-16) {
+        eigs[0] = C(0, 0);
+        eigs[1] = C(1, 1);
+        eigs[2] = C(2, 2);
+        dir[0]  = 1;
+        if (eigs[0] < eigs[1]) {
+            std::swap(eigs[0], eigs[1]);
+            dir[0] = 0, dir[1] = 1, dir[2] = 0;
+        }
+        if (eigs[0] < eigs[2]) {
+            std::swap(eigs[0], eigs[2]);
+            dir[0] = 0, dir[1] = 0, dir[2] = 1;
+        }
+        return;
+    }
+    double q  = (C(0, 0) + C(1, 1) + C(2, 2)) / 3.;
+    double p2 = std::pow(C(0, 0) - q, 2) + std::pow(C(1, 1) - q, 2) + std::pow(C(2, 2) - q, 2) + 2. * p1;
+    double p  = std::sqrt(p2 / 6.);
+    double B[9];
+    for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++)
+            B[i + 3 * j] = (C(i, j) - (i == j ? 1.0 : 0.0) * q) * (1. / p);
+    auto Bm     = [&](int i, int j) { return B[i + 3 * j]; };
+    double detB = Bm(0, 0) * (Bm(1, 1) * Bm(2, 2) - Bm(1, 2) * Bm(2, 1)) - Bm(0, 1) * (Bm(1, 0) * Bm(2, 2) - Bm(1, 2) * Bm(2, 0)) + Bm(0, 2) * (Bm(1, 0) * Bm(2, 1) - Bm(1, 1) * Bm(2, 0));
+    double r    = detB / 2.;
+    double phi;
+    if (r <= -1)
+        phi = 3.14159265358979323846 / 3.;
+    else if (r >= 1)
+        phi = 0;
+    else
+        phi = std::acos(r) / 3.;
+    eigs[0] = q + 2. * p * std::cos(phi);
+    eigs[2] = q + 2. * p * std::cos(phi + (2. * 3.14159265358979323846 / 3.));
+    eigs[1] = 3. * q - eigs[0] - eigs[2];
+    if (std::abs(eigs[1]) < 1.e-12)
+        eigs[1] = 0.;
+    if (std::abs(eigs[2]) < 1.e-12)
+        eigs[2] = 0.;
+    if (std::abs(eigs[0]) < 1.e-16)
+        return;
+    double M1[9], M2[9], P[9];
+    for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++) {
+            M1[i + 3 * j] = C(i, j) - (i == j ? 1.0 : 0.0) * eigs[1];
+            M2[i + 3 * j] = C(i, j) - (i == j ? 1.0 : 0.0) * eigs[2];
+        }
+    for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++)
+                s += M1[i + 3 * k] * M2[k + 3 * j];
+            P[i + 3 * j] = s;
+        }
+    int ind        = 0;
+    double dirnorm = 0;
+    do {
+        dir[0]  = P[0 + 3 * ind];
+        dir[1]  = P[1 + 3 * ind];
+        dir[2]  = P[2 + 3 * ind];
+        dirnorm = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        ind++;
+    } while ((dirnorm < 1.e-15) && (ind < 3));
+    if (dirnorm < 1.e-15) {
+        dir[0] = 1, dir[1] = 0, dir[2] = 0;
+    } else {
+        dir[0] /= dirnorm, dir[1] /= dirnorm, dir[2] /= dirnorm;
+    }
+}
+
+void build_cluster(int n, const double *x, int mcs, std::vector<Node> &nodes, std::vector<int> &perm) {
+    nodes.assign(1, Node());
+    nodes[0].size = n;
+    perm.resize(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::vector<std::pair<int, std::vector<int>>> stack;
+    stack.emplace_back(0, perm);
+    while (!stack.empty()) {
+        int ci               = stack.back().first;
+        std::vector<int> num = std::move(stack.back().second);
+        stack.pop_back();
+        const int nb_sons = nodes[ci].depth == 0 ? 1 : 2; // pca.hpp:28 with one rank
+        const int nb_pt   = nodes[ci].size;
+        double G          = 0;
+        for (int j = 0; j < nb_pt; j++)
+            G += 1.0;
+        double xc[3] = {0, 0, 0};
+        for (int j = 0; j < nb_pt; j++)
+            for (int p = 0; p < 3; p++)
+                xc[p] += 1.0 * x[3 * num[j] + p];
+        for (int p = 0; p < 3; p++)
+            xc[p] = xc[p] * (1. / G); // pca.hpp:44 multiplies by 1/G
+        std::copy(xc, xc + 3, nodes[ci].ctr);
+        double cov[9] = {0};
+        double rad    = 0;
+        for (int j = 0; j < nb_pt; j++) {
+            double u[3];
+            for (int p = 0; p < 3; p++)
+                u[p] = x[3 * num[j] + p] - xc[p];
+            rad = std::max(rad, std::sqrt(std::abs(dot3(u, u))) + 0.0);
+            for (int p = 0; p < 3; p++)
+                for (int q = 0; q < 3; q++)
+                    cov[p + 3 * q] += 1.0 * u[p] * u[q];
+        }
+        nodes[ci].rad = rad;
+        double dir[3];
+        evp3(cov, dir);
+
+        // geometric_splitting (splitting.hpp:63-120)
+        std::vector<std::vector<int>> numbering(nb_sons);
+        if (nb_sons == 2) {
+            for (int j = 0; j < nb_pt; j++) {
+                double dx[3];
+                for (int p = 0; p < 3; p++)
+                    dx[p] = x[3 * num[j] + p] - xc[p];
+                if (dot3(dir, dx) > 0)
+                    numbering[0].push_back(num[j]);
+                else
+                    numbering[1].push_back(num[j]);
+            }
+        } else if (num.size() > 1) {
+            auto proj = [&](int a) { return dot3(x + 3 * a, dir); };
+            int imin = num[0], imax = num[0];
+            double pmin = proj(imin), pmax = pmin;
+            for (size_t j = 1; j < num.size(); j++) { // std::minmax_element: first min, last max
+                double pj = proj(num[j]);
+                if (pj < pmin)
+                    pmin = pj, imin = num[j];
+                if (!(pj < pmax))
+                    pmax = pj, imax = num[j];
+            }
+            double mn[3], mx[3], d[3];
+            for (int p = 0; p < 3; p++)
+                mn[p] = x[3 * imin + p], mx[p] = x[3 * imax + p], d[p] = mx[p] - mn[p];
+            double length = dot3(d, dir) / (double)nb_sons;
+            for (int j = 0; j < nb_pt; j++) {
+                double dx[3];
+                for (int p = 0; p < 3; p++)
+                    dx[p] = x[3 * num[j] + p] - mn[p];
+                int index = dot3(dx, dir) / length;
+                index     = (index == nb_sons) ? index - 1 : index;
+                numbering[index].push_back(num[j]);
+            }
+            // an empty son would fall back to regular splitting (splitting.hpp:106-116);
+            // with one son it cannot be empty.
+        }
+
+        bool test_mcs = true;
+        for (int p = 0; p < nb_sons; p++)
+            test_mcs = test_mcs && ((int)numbering[p].size() >= mcs);
+        if (test_mcs || nodes[ci].rank == -1) {
+            int count = 0;
+            for (int p = 0; p < nb_sons; p++) {
+                Node son;
+                son.depth  = nodes[ci].depth + 1;
+                son.offset = nodes[ci].offset + count;
+                son.size   = (int)numbering[p].size();
+                son.rank   = nodes[ci].depth == 0 ? p : nodes[ci].rank;
+                count += son.size;
+                nodes.push_back(son);
+                nodes[ci].sons.push_back((int)nodes.size() - 1);
+            }
+            for (int p = 0; p < nb_sons; p++)
+                stack.emplace_back(nodes[ci].sons[p], std::move(numbering[p]));
+        } else {
+            std::copy(num.begin(), num.end(), perm.begin() + nodes[ci].offset);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// partialACA / sympartialACA (H/lrmat/partialACA.hpp:39-171,
+// H/lrmat/sympartialACA.hpp:38-209). `swap` is sympartialACA's
+// t.offset < s.offset branch: the roles of rows and columns exchange.
+int aca(bool swap, double epsilon, int M, int N, const int *rows, const int *cols, const Kernel &A,
+        const double *ctr1, int reqrank, int cap, double *U, double *V) {
+    const int n1 = swap ? N : M, n2 = swap ? M : N;
+    const int *i1 = swap ? cols : rows;
+    const double *x1 = swap ? A.xs : A.xt;
+    auto g = [&](int a, int b) { return swap ? A(rows[b], cols[a]) : A(rows[a], cols[b]); };
+
+    double dist = 1e30;
+    int I1      = 0;
+    for (int i = 0; i < n1; i++) {
+        double s = 0;
+        for (int d = 0; d < 3; d++) {
+            double u = x1[3 * i1[i] + d] - ctr1[d];
+            s        = s + u * u;
+        }
+        double aux_dist = std::sqrt(s);
+        if (dist > aux_dist) {
+            dist = aux_dist;
+            I1   = i;
+        }
+    }
+    int I2 = 0, q = 0;
+    std::vector<std::vector<double>> uu, vv;
+    std::vector<bool> visited_1(n1, false), visited_2(n2, false);
+    double frob = 0, aux = 0;
+    std::vector<double> u1(n2), u2(n1);
+    while (((reqrank > 0) && (q < std::min(reqrank, std::min(M, N)))) || ((reqrank < 0) && (q == 0 || std::sqrt(aux / frob) > epsilon))) {
+        q += 1;
+        if ((long long)q * (M + N) > (long long)M * N) { // 64-bit: the int product overflows past 46341^2
+            q = -1;
+            break;
+        }
+        for (int b = 0; b < n2; b++)
+            u1[b] = g(I1, b);
+        for (size_t j = 0; j < uu.size(); j++) {
+            double coef = -uu[j][I1];
+            for (int b = 0; b < n2; b++)
+                u1[b] = std::fma(coef, vv[j][b], u1[b]);
+        }
+        double pivot = 0., tmp = 0;
+        for (int k = 0; k < n2; k++) {
+            if (visited_2[k])
+                continue;
+            tmp = std::abs(u1[k]);
+            if (tmp < pivot)
+                continue;
+            pivot = tmp;
+            I2    = k;
+        }
+        visited_1[I1] = true;
+        double gamma  = 1. / u1[I2];
+        if (std::abs(u1[I2]) > 1e-15) {
+            for (int a = 0; a < n1; a++)
+                u2[a] = g(a, I2);
+            for (size_t k = 0; k < uu.size(); k++) {
+                double coef = -vv[k][I2];
+                for (int a = 0; a < n1; a++)
+                    u2[a] = std::fma(coef, uu[k][a], u2[a]);
+            }
+            for (int a = 0; a < n1; a++)
+                u2[a] *= gamma;
+            pivot = 0., tmp = 0;
+            for (int k = 0; k < n1; k++) {
+                if (visited_1[k])
+                    continue;
+                tmp = std::abs(u2[k]);
+                if (tmp < pivot)
+                    continue;
+                pivot = tmp;
+                I1    = k;
+            }
+            visited_2[I2] = true;
+            if (reqrank < 0) {
+                auto dot = [](const std::vector<double> &a, const std::vector<double> &b) {
+                    double s = 0;
+                    for (size_t i = 0; i < a.size(); i++)
+                        s += a[i] * b[i];
+                    return s;
+                };
+                double frob_aux = 0.;
+                aux             = std::abs(dot(u2, u2)) * std::abs(dot(u1, u1));
+                for (size_t j = 0; j < uu.size(); j++)
+                    frob_aux += dot(u1, vv[j]) * dot(u2, uu[j]);
+                frob += aux + 2 * frob_aux;
+            }
+            uu.push_back(u2);
+            vv.push_back(u1);
+        } else {
+            q -= 1;
+            if (q == 0)
+                q = -1;
+            break;
+        }
+    }
+    if (q > cap)
+        return -2;
+    if (q > 0) {
+        for (int k = 0; k < q; k++) {
+            const std::vector<double> &ucol = swap ? vv[k] : uu[k]; // length M
+            const std::vector<double> &vrow = swap ? uu[k] : vv[k]; // length N
+            std::copy(ucol.begin(), ucol.end(), U + (size_t)k * M);
+            for (int j = 0; j < N; j++)
+                V[(size_t)q * j + k] = vrow[j];
+        }
+    }
+    return q;
+}
+
+// ----------------------------------------------------------------------------
+// Block tree (H/blocks/blocks.hpp:37-139, 237-278) and leaf assembly
+// (H/types/hmatrix.hpp:422-607, 700-735).
+struct Leaf {
+    int oi, m, oj, n, k; // k = -1 dense
+    std::vector<double> a, U, V;
+};
+
+struct Builder {
+    const std::vector<Node> &T, &S;
+    const std::vector<int> &perm_t, &perm_s;
+    const Kernel &A;
+    double eta, eps;
+    size_t maxblocksize;
+    int sym_aca, reqrank;
+
+    // RjasanowSteinbach (H/blocks/admissibility_conditions.hpp:16-19)
+    bool admissible(int t, int s) const {
+        const Node &a = T[t], &b = S[s];
+        double d[3];
+        for (int p = 0; p < 3; p++)
+            d[p] = a.ctr[p] - b.ctr[p];
+        double dist = std::sqrt(std::abs(dot3(d, d)));
+        return 2 * std::min(a.rad, b.rad) < eta * std::max((dist - a.rad - b.rad), 0.);
+    }
+
+    // build_block_tree: returns true when the block was not pushed (blocks.hpp:37-139).
+    bool block_tree(int t, int s, std::vector<std::pair<int, int>> &tasks) const {
+        const Node &a = T[t], &b = S[s];
+        size_t bsize  = size_t(a.size) * size_t(b.size);
+        if (admissible(t, s) && a.rank >= 0) {
+            tasks.emplace_back(t, s);
+            return false;
+        }
+        bool split_t;
+        if (b.sons.empty()) {
+            if (a.sons.empty())
+                return true;
+            split_t = true;
+        } else if (a.sons.empty()) {
+            split_t = false;
+        } else {
+            split_t = a.size > b.size;
+        }
+        const std::vector<int> &sons = split_t ? a.sons : b.sons;
+        std::vector<bool> not_pushed(sons.size());
+        for (size_t p = 0; p < sons.size(); p++)
+            not_pushed[p] = split_t ? block_tree(sons[p], s, tasks) : block_tree(t, sons[p], tasks);
+        if (bsize <= maxblocksize && std::all_of(not_pushed.begin(), not_pushed.end(), [](bool v) { return v; }) && a.rank >= 0)
+            return true;
+        for (size_t p = 0; p < sons.size(); p++)
+            if (not_pushed[p])
+                tasks.push_back(split_t ? std::make_pair(sons[p], s) : std::make_pair(t, sons[p]));
+        return false;
+    }
+
+    void dense(int t, int s, std::vector<Leaf> &out) const {
+        const Node &a = T[t], &b = S[s];
+        Leaf L{a.offset, a.size, b.offset, b.size, -1, {}, {}, {}};
+        L.a.resize(size_t(a.size) * b.size);
+        for (int j = 0; j < b.size; j++)
+            for (int i = 0; i < a.size; i++)
+                L.a[i + size_t(a.size) * j] = A(perm_t[a.offset + i], perm_s[b.offset + j]);
+        out.push_back(std::move(L));
+    }
+
+    // AddFarFieldMat + LowRankMatrix::build (hmatrix.hpp:719-735, lrmat.hpp:36-55)
+    bool lowrank(int t, int s, std::vector<Leaf> &out) const {
+        const Node &a = T[t], &b = S[s];
+        const int M = a.size, N = b.size;
+        const bool swap = sym_aca && (a.offset < b.offset);
+        std::vector<int> rows(perm_t.begin() + a.offset, perm_t.begin() + a.offset + M);
+        std::vector<int> cols(perm_s.begin() + b.offset, perm_s.begin() + b.offset + N);
+        int cap = std::min(M, N) + 1;
+        std::vector<double> U(size_t(M) * cap), V(size_t(N) * cap);
+        int k = aca(swap, eps, M, N, rows.data(), cols.data(), A, swap ? b.ctr : a.ctr, reqrank, cap, U.data(), V.data());
+        if (k == -1)
+            return false;
+        Leaf L{a.offset, M, b.offset, N, k, {}, {}, {}};
+        L.U.assign(U.begin(), U.begin() + size_t(M) * k);
+        L.V.assign(V.begin(), V.begin() + size_t(N) * k);
+        out.push_back(std::move(L));
+        return true;
+    }
+
+    // ComputeAdmissibleBlock (hmatrix.hpp:503-607): true = not pushed.
+    bool admissible_block(int t, int s, bool adm, std::vector<Leaf> &out, int &fp) const {
+        if (adm) {
+            if (lowrank(t, s, out))
+                return false;
+            fp += 1;
+        }
+        const Node &a = T[t], &b = S[s];
+        size_t bsize  = size_t(a.size) * size_t(b.size);
+        bool split_t;
+        if (b.sons.empty()) {
+            if (a.sons.empty())
+                return true;
+            split_t = true;
+        } else if (a.sons.empty()) {
+            split_t = false;
+        } else {
+            split_t = a.size > b.size;
+        }
+        const std::vector<int> &sons = split_t ? a.sons : b.sons;
+        std::vector<bool> not_pushed(sons.size());
+        for (size_t p = 0; p < sons.size(); p++) {
+            int ts = split_t ? sons[p] : t, ss = split_t ? s : sons[p];
+            not_pushed[p] = admissible_block(ts, ss, admissible(ts, ss), out, fp);
+        }
+        if (bsize <= maxblocksize && std::all_of(not_pushed.begin(), not_pushed.end(), [](bool v) { return v; }))
+            return true;
+        for (size_t p = 0; p < sons.size(); p++)
+            if (not_pushed[p])
+                dense(split_t ? sons[p] : t, split_t ? s : sons[p], out);
+        return false;
+    }
+};
+
+} // namespace
+
+struct orc_hmat {
+    int nr = 0, nc = 0, false_positive = 0;
+    std::vector<int> perm_t, perm_s;
+    std::vector<Leaf> leaves;
+};
+
+extern "C" {
+
+void orc_kernel_block(int kernel, double delta, double diag, int same_points, const double *xt, const double *xs,
+                      int M, int N, const int *rows, const int *cols, double *out) {
+    Kernel A{kernel, delta, diag, same_points != 0, xt, same_points ? xt : xs};
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < M; i++)
+            out[i + size_t(M) * j] = A(rows[i], cols[j]);
+}
+
+int orc_cluster_build(int n, const double *x, int mcs, int *perm, int *nodes_int, double *nodes_dbl) {
+    std::vector<Node> nodes;
+    std::vector<int> p;
+    build_cluster(n, x, mcs, nodes, p);
+    std::copy(p.begin(), p.end(), perm);
+    for (size_t i = 0; i < nodes.size(); i++) {
+        const Node &c      = nodes[i];
+        int *ni            = nodes_int + 6 * i;
+        ni[0]              = c.offset;
+        ni[1]              = c.size;
+        ni[2]              = c.depth;
+        ni[3]              = c.rank;
+        ni[4]              = (int)c.sons.size();
+        ni[5]              = c.sons.empty() ? -1 : c.sons[0];
+        double *nd         = nodes_dbl + 4 * i;
+        nd[0]              = c.ctr[0];
+        nd[1]              = c.ctr[1];
+        nd[2]              = c.ctr[2];
+        nd[3]              = c.rad;
+    }
+    return (int)nodes.size();
+}
+
+int orc_aca(int sym, double eps, int M, int N, const int *rows, const int *cols, int kernel, double delta,
+            double diag, int same_points, const double *xt, const double *xs, const double *ctr_t, int off_t,
+            const double *ctr_s, int off_s, int reqrank, int cap, double *U, double *V) {
+    Kernel A{kernel, delta, diag, same_points != 0, xt, same_points ? xt : xs};
+    bool swap = sym && off_t < off_s;
+    return aca(swap, eps, M, N, rows, cols, A, swap ? ctr_s : ctr_t, reqrank, cap, U, V);
+}
+
+orc_hmat *orc_hmat_build(int nt, const double *xt, int ns, const double *xs, int kernel, double delta, double diag,
+                         int mcs, double eta, double eps, int sym_aca, int reqrank, double maxblocksize) {
+    auto *h = new orc_hmat;
+    h->nr   = nt;
+    h->nc   = ns;
+    const bool same = (xs == nullptr);
+    std::vector<Node> T, S;
+    build_cluster(nt, xt, mcs, T, h->perm_t);
+    if (same) {
+        S         = T;
+        h->perm_s = h->perm_t;
+    } else {
+        build_cluster(ns, xs, mcs, S, h->perm_s);
+    }
+    Kernel A{kernel, delta, diag, same, xt, same ? xt : xs};
+    Builder B{T, S, h->perm_t, h->perm_s, A, eta, eps, (size_t)maxblocksize, sym_aca, reqrank};
+
+    // Block::build (blocks.hpp:237-278): root admissibility, tree, root task,
+    // local tasks of rank 0, sorted by (target offset, source offset).
+    std::vector<std::pair<int, int>> tasks;
+    bool not_pushed = B.block_tree(0, 0, tasks);
+    if (not_pushed)
+        tasks.emplace_back(0, 0);
+    std::vector<std::pair<int, int>> local;
+    for (auto &ts : tasks)
+        if (T[ts.first].rank == 0)
+            local.push_back(ts);
+    std::sort(local.begin(), local.end(), [&](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+        if (T[a.first].offset == T[b.first].offset)
+            return S[a.second].offset < S[b.second].offset;
+        return T[a.first].offset < T[b.first].offset;
+    });
+
+    // ComputeBlocks (hmatrix.hpp:422-501): one result list per task, merged in task order.
+    std::vector<std::vector<Leaf>> per_task(local.size());
+    std::vector<int> fp(local.size(), 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long p = 0; p < (long)local.size(); p++) {
+        int t = local[p].first, s = local[p].second;
+        bool adm = B.admissible(t, s);
+        if (!adm) {
+            B.dense(t, s, per_task[p]);
+        } else if (B.admissible_block(t, s, true, per_task[p], fp[p])) {
+            B.dense(t, s, per_task[p]);
+        }
+    }
+    for (size_t p = 0; p < local.size(); p++) {
+        h->false_positive += fp[p];
+        for (auto &L : per_task[p])
+            h->leaves.push_back(std::move(L));
+    }
+    std::stable_sort(h->leaves.begin(), h->leaves.end(), [](const Leaf &a, const Leaf &b) {
+        return a.oi == b.oi ? a.oj < b.oj : a.oi < b.oi;
+    });
+    return h;
+}
+
+int orc_hmat_nleaves(const orc_hmat *h) { return (int)h->leaves.size(); }
+
+void orc_hmat_leaves(const orc_hmat *h, int *out5) {
+    for (size_t l = 0; l < h->leaves.size(); l++) {
+        const Leaf &L = h->leaves[l];
+        int v[5]      = {L.oi, L.m, L.oj, L.n, L.k};
+        std::copy(v, v + 5, out5 + 5 * l);
+    }
+}
+
+long long orc_hmat_data_size(const orc_hmat *h) {
+    long long s = 0;
+    for (const Leaf &L : h->leaves)
+        s += L.k < 0 ? (long long)L.m * L.n : (long long)L.k * (L.m + L.n);
+    return s;
+}
+
+void orc_hmat_data(const orc_hmat *h, double *out) {
+    for (const Leaf &L : h->leaves) {
+        if (L.k < 0) {
+            out = std::copy(L.a.begin(), L.a.end(), out);
+        } else {
+            out = std::copy(L.U.begin(), L.U.end(), out);
+            out = std::copy(L.V.begin(), L.V.end(), out);
+        }
+    }
+}
+
+void orc_hmat_perms(const orc_hmat *h, int *perm_t, int *perm_s) {
+    std::copy(h->perm_t.begin(), h->perm_t.end(), perm_t);
+    std::copy(h->perm_s.begin(), h->perm_s.end(), perm_s);
+}
+
+int orc_hmat_false_positives(const orc_hmat *h) { return h->false_positive; }
+
+void orc_hmat_free(orc_hmat *h) { delete h; }
+
+// mvprod_global_to_global (hmatrix.hpp:1073-1182) with the leaf loop of
+// mymvprod_global_to_local (843-911) and the leaf products of
+// Matrix::add_mvprod_row_major (matrix.hpp:353-383) and
+// LowRankMatrix::add_mvprod_row_major (lrmat.hpp:89-100), one RHS column at a time.
+void orc_leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
+                     const double *data, const double *x, double *y, int mu) {
+    std::vector<const double *> ptr(nleaves);
+    const double *p = data;
+    for (int l = 0; l < nleaves; l++) {
+        ptr[l]     = p;
+        const int *L = leaves5 + 5 * l;
+        p += L[4] < 0 ? size_t(L[1]) * L[3] : size_t(L[4]) * (L[1] + L[3]);
+    }
+    std::vector<double> xin(nc), yout(nr), t;
+    for (int c = 0; c < mu; c++) {
+        for (int j = 0; j < nc; j++)
+            xin[j] = x[perm_s[j] + size_t(nc) * c]; // global_to_cluster (virtual_cluster.hpp:95-100)
+        std::fill(yout.begin(), yout.end(), 0.0);
+        for (int l = 0; l < nleaves; l++) {
+            const int *L = leaves5 + 5 * l;
+            const int oi = L[0], m = L[1], oj = L[2], n = L[3], k = L[4];
+            const double *d = ptr[l];
+            if (k < 0) {
+                for (int j = 0; j < n; j++) {
+                    double xj = xin[oj + j];
+                    for (int i = 0; i < m; i++)
+                        yout[oi + i] += d[i + size_t(m) * j] * xj;
+                }
+            } else {
+                const double *U = d, *V = d + size_t(m) * k;
+                t.assign(k, 0.0);
+                for (int j = 0; j < n; j++)
+                    for (int r = 0; r < k; r++)
+                        t[r] += V[r + size_t(k) * j] * xin[oj + j];
+                for (int r = 0; r < k; r++)
+                    for (int i = 0; i < m; i++)
+                        yout[oi + i] += U[i + size_t(m) * r] * t[r];
+            }
+        }
+        for (int i = 0; i < nr; i++)
+            y[perm_t[i] + size_t(nr) * c] = yout[i]; // cluster_to_global (virtual_cluster.hpp:89-94)
+    }
+}
+
+void orc_hmat_mvprod(const orc_hmat *h, const double *x, double *y, int mu) {
+    std::vector<int> leaves(5 * h->leaves.size());
+    orc_hmat_leaves(h, leaves.data());
+    std::vector<double> data(orc_hmat_data_size(h));
+    orc_hmat_data(h, data.data());
+    orc_leaf_mvprod(h->nr, h->nc, h->perm_t.data(), h->perm_s.data(), (int)h->leaves.size(), leaves.data(),
+                    data.data(), x, y, mu);
+}
+
+void orc_dense_rows(int kernel, double delta, double diag, int same_points, const double *xt, const double *xs,
+                    int ns, int nrows, const int *rows, const double *x, int mu, double *out) {
+    Kernel A{kernel, delta, diag, same_points != 0, xt, same_points ? xt : xs};
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < nrows; r++) {
+        for (int c = 0; c < mu; c++) {
+            double s = 0;
+            for (int j = 0; j < ns; j++)
+                s += A(rows[r], j) * x[j + size_t(ns) * c];
+            out[r + size_t(nrows) * c] = s;
+        }
+    }
+}
+
+} // extern "C"

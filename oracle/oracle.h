@@ -1,0 +1,73 @@
+/* CPU oracle: an FP64 restatement of the reference's H-matrix hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and
+ * bench.py's CPU-baseline leg may load this library, and only as the checker
+ * (or as the timed CPU restatement in the baseline). The product library
+ * (htool_amd/) never links, loads or calls it.
+ *
+ * Every function follows a reference file (paths relative to the reference
+ * root); oracle.cpp cites file:line for each step.
+ *
+ * Pinning (what checks each function beyond retyping it):
+ *   orc_kernel_block       closed forms; bitwise vs the reference driver's dense leaves
+ *   orc_cluster_build      permutation bitwise vs the reference driver; invariants
+ *                          (contiguous sons, every point once)
+ *   orc_aca                ranks/U/V vs the reference driver; cross interpolation
+ *                          property; space-saving bands of the reference's lrmat tests;
+ *                          Eckart-Young floor vs numpy SVD
+ *   orc_hmat_build         leaf list identical to the reference driver; block cover
+ *                          exactly once; admissibility recomputed per far leaf
+ *   orc_hmat_mvprod /      vs the reference driver's y on the same tree (1e-12);
+ *   orc_leaf_mvprod        vs numpy dense reconstruction; vs brute-force dense rows
+ *   orc_dense_rows         closed form on tiny inputs (numpy)
+ */
+#ifndef HTOOL_AMD_ORACLE_H
+#define HTOOL_AMD_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_hmat orc_hmat;
+
+/* One kernel entry family: 0 = 1/(4 pi r) (diag on i==j when same points),
+ * 1 = 1/(delta+r), 2 = 1/(delta+4 pi r), 3 = 1/(4 pi sqrt(delta+r^2)). */
+void orc_kernel_block(int kernel, double delta, double diag, int same_points, const double *xt,
+                      const double *xs, int M, int N, const int *rows, const int *cols, double *out);
+
+/* PCA bisection cluster tree (geometric splitting, one MPI rank).
+ * nodes_int: per node {offset, size, depth, rank, nb_sons, first_son} (6 ints),
+ * nodes_dbl: per node {ctr0, ctr1, ctr2, rad}; capacity 2n+2 nodes. Returns node count. */
+int orc_cluster_build(int n, const double *x, int minclustersize, int *perm, int *nodes_int, double *nodes_dbl);
+
+/* Leaf compression of one block (partialACA when sym == 0, sympartialACA when sym == 1).
+ * rows/cols are original indices. U is M x k column-major, V is k x N column-major
+ * (V(r, j) at r + k*j), both with room for `cap` terms. Returns the rank (-1 = incompressible,
+ * -2 = cap too small). */
+int orc_aca(int sym, double eps, int M, int N, const int *rows, const int *cols, int kernel, double delta,
+            double diag, int same_points, const double *xt, const double *xs, const double *ctr_t, int off_t,
+            const double *ctr_s, int off_s, int reqrank, int cap, double *U, double *V);
+
+orc_hmat *orc_hmat_build(int nt, const double *xt, int ns, const double *xs, int kernel, double delta, double diag,
+                         int minclustersize, double eta, double eps, int sym_aca, int reqrank, double maxblocksize);
+int orc_hmat_nleaves(const orc_hmat *h);
+void orc_hmat_leaves(const orc_hmat *h, int *out5); /* off_i, m, off_j, n, rank(-1 dense); sorted by (off_i, off_j) */
+long long orc_hmat_data_size(const orc_hmat *h);
+void orc_hmat_data(const orc_hmat *h, double *out); /* dense m*n col-major | U m*k col-major then V k*n col-major */
+void orc_hmat_perms(const orc_hmat *h, int *perm_t, int *perm_s);
+int orc_hmat_false_positives(const orc_hmat *h);
+void orc_hmat_mvprod(const orc_hmat *h, const double *x, double *y, int mu);
+void orc_hmat_free(orc_hmat *h);
+
+/* y = H x over an explicit leaf list with the data layout of orc_hmat_data.
+ * x: nc x mu column-major, y: nr x mu column-major, both in original ordering. */
+void orc_leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
+                     const double *data, const double *x, double *y, int mu);
+
+/* Brute-force dense rows: out[r + nrows*c] = sum_j A(rows[r], j) x[j + ns*c]. */
+void orc_dense_rows(int kernel, double delta, double diag, int same_points, const double *xt, const double *xs,
+                    int ns, int nrows, const int *rows, const double *x, int mu, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

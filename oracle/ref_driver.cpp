@@ -1,0 +1,224 @@
+// Driver that runs the reference Htool library (header-only, compiled from
+// the reference's own headers by oracle/Makefile) on one input case and dumps
+// what the tests and the CPU baseline need: permutations, the leaf list, the
+// leaf data (dense blocks and U, V factors), y = H x and timings.
+//
+// It is test infrastructure: only tests/ and bench.py's CPU-baseline leg run it.
+//
+// Reference API used (file:line, paths relative to the reference root):
+//   Cluster<PCAGeometricClustering>::build   include/htool/clustering/cluster.hpp:57-92
+//   HMatrix<T>(t, s, eps, eta)               include/htool/types/hmatrix.hpp:115-122
+//   HMatrix::set_compression                 include/htool/types/hmatrix.hpp:220
+//   HMatrix::build(A, xt, xs)                include/htool/types/hmatrix.hpp:285-341
+//   HMatrix::mvprod_global_to_global         include/htool/types/hmatrix.hpp:1073-1182
+//   LowRankMatrix::build (stand-alone leaf)  include/htool/lrmat/lrmat.hpp:36-55
+//
+// Input file layout (little endian), written by tests/refcase.py:
+//   int32  magic 0x48545246 ('HTRF'), version 1
+//   int32  mode (0 = H-matrix build + product, 1 = one leaf compression of the whole t x s block)
+//   int32  nt, ns, same_points, kernel, minclustersize, compressor (0 partialACA, 1 sympartialACA)
+//   int32  mu, nreps, dump_data, reqrank
+//   double eta, eps, delta, diag, maxblocksize
+//   double xt[3 nt]; double xs[3 ns] (absent when same_points); double x[ns mu] (column-major)
+#include <htool/clustering/pca.hpp>
+#include <htool/lrmat/partialACA.hpp>
+#include <htool/lrmat/sympartialACA.hpp>
+#include <htool/types/hmatrix.hpp>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <omp.h>
+#include <string>
+#include <vector>
+
+using namespace htool;
+
+namespace {
+
+// Kernel families shared with the HIP generator and the oracle. The arithmetic
+// order is fixed (squared distance summed x, y, z from zero, then sqrt) so that
+// all three produce identical doubles.
+struct KernelGen final : public VirtualGenerator<double> {
+    const double *xt, *xs;
+    int kernel;
+    bool same_points;
+    double delta, diag;
+    KernelGen(int nr, int nc, const double *xt0, const double *xs0, int kernel0, bool same0, double delta0, double diag0)
+        : VirtualGenerator<double>(nr, nc), xt(xt0), xs(xs0), kernel(kernel0), same_points(same0), delta(delta0), diag(diag0) {}
+
+    double coef(int i, int j) const {
+        const double *a = xt + 3 * i, *b = xs + 3 * j;
+        double r2 = 0;
+        for (int d = 0; d < 3; d++) {
+            double u = a[d] - b[d];
+            r2       = r2 + u * u;
+        }
+        switch (kernel) {
+        case 0:
+            if (same_points && i == j)
+                return diag;
+            return 1. / (4 * M_PI * std::sqrt(r2));
+        case 1:
+            return 1. / (delta + std::sqrt(r2));
+        case 2:
+            return 1. / (delta + 4 * M_PI * std::sqrt(r2));
+        default:
+            return 1. / (4 * M_PI * std::sqrt(delta + r2));
+        }
+    }
+    void copy_submatrix(int M, int N, const int *const rows, const int *const cols, double *ptr) const override {
+        for (int i = 0; i < M; i++)
+            for (int j = 0; j < N; j++)
+                ptr[i + M * j] = coef(rows[i], cols[j]);
+    }
+};
+
+template <typename V>
+void write_vec(const std::string &path, const std::vector<V> &v) {
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) {
+        std::fprintf(stderr, "cannot write %s\n", path.c_str());
+        std::exit(3);
+    }
+    if (!v.empty())
+        std::fwrite(v.data(), sizeof(V), v.size(), f);
+    std::fclose(f);
+}
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+} // namespace
+
+int main(int argc, char **argv) {
+    MPI_Init(&argc, &argv);
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s case.bin outdir\n", argv[0]);
+        return 2;
+    }
+    std::ifstream in(argv[1], std::ios::binary);
+    std::string out = argv[2];
+    int hdr[14];
+    double prm[5];
+    in.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
+    in.read(reinterpret_cast<char *>(prm), sizeof(prm));
+    if (!in || hdr[0] != 0x48545246 || hdr[1] != 1) {
+        std::fprintf(stderr, "bad case file\n");
+        return 2;
+    }
+    const int mode = hdr[2], nt = hdr[3], ns = hdr[4], same = hdr[5], kernel = hdr[6], mcs = hdr[7];
+    const int compressor = hdr[8], mu = hdr[9], nreps = hdr[10], dump = hdr[11], reqrank = hdr[12];
+    const double eta = prm[0], eps = prm[1], delta = prm[2], diag = prm[3], maxblocksize = prm[4];
+    std::vector<double> xt(3 * size_t(nt)), xs, x(size_t(ns) * mu);
+    in.read(reinterpret_cast<char *>(xt.data()), xt.size() * sizeof(double));
+    if (!same) {
+        xs.resize(3 * size_t(ns));
+        in.read(reinterpret_cast<char *>(xs.data()), xs.size() * sizeof(double));
+    }
+    in.read(reinterpret_cast<char *>(x.data()), x.size() * sizeof(double));
+    if (!in) {
+        std::fprintf(stderr, "truncated case file\n");
+        return 2;
+    }
+    const double *pxs = same ? xt.data() : xs.data();
+    KernelGen A(nt, ns, xt.data(), pxs, kernel, same != 0, delta, diag);
+
+    std::shared_ptr<Cluster<PCAGeometricClustering>> t = std::make_shared<Cluster<PCAGeometricClustering>>();
+    std::shared_ptr<Cluster<PCAGeometricClustering>> s;
+    t->set_minclustersize(mcs);
+    double t0 = now();
+    t->build(nt, xt.data());
+    if (same) {
+        s = t;
+    } else {
+        s = std::make_shared<Cluster<PCAGeometricClustering>>();
+        s->set_minclustersize(mcs);
+        s->build(ns, pxs);
+    }
+    double t_cluster = now() - t0;
+    write_vec(out + "/perm_t.i32", t->get_perm());
+    write_vec(out + "/perm_s.i32", s->get_perm());
+
+    std::shared_ptr<VirtualLowRankGenerator<double>> comp;
+    if (compressor == 0)
+        comp = std::make_shared<partialACA<double>>();
+    else
+        comp = std::make_shared<sympartialACA<double>>();
+
+    std::FILE *info = std::fopen((out + "/info.txt").c_str(), "w");
+    std::fprintf(info, "threads=%d\ncluster_s=%.9g\n", omp_get_max_threads(), t_cluster);
+
+    if (mode == 1) {
+        // One leaf compression of the whole t x s block (lrmat tests style).
+        LowRankMatrix<double> L(1, t->get_perm(), s->get_perm(), reqrank, eps);
+        L.build(A, *comp, *t, xt.data(), *s, pxs);
+        int k = L.rank_of();
+        std::vector<int> leaves = {0, nt, 0, ns, k};
+        write_vec(out + "/leaves.i32", leaves);
+        std::vector<double> data;
+        for (int r = 0; r < k; r++)
+            for (int i = 0; i < nt; i++)
+                data.push_back(L.get_U(i, r));
+        for (int j = 0; j < ns; j++)
+            for (int r = 0; r < k; r++)
+                data.push_back(L.get_V(r, j));
+        write_vec(out + "/data.f64", data);
+        std::fprintf(info, "rank=%d\n", k);
+        std::fclose(info);
+        MPI_Finalize();
+        return 0;
+    }
+
+    HMatrix<double> H(t, s, eps, eta, 'N', 'N', reqrank);
+    H.set_compression(comp);
+    H.set_maxblocksize(static_cast<unsigned int>(maxblocksize));
+    t0 = now();
+    H.build(A, xt.data(), pxs);
+    double t_build = now() - t0;
+
+    // Leaf list in save_plot order (near field, then far field).
+    std::vector<int> leaves;
+    std::vector<double> data;
+    for (auto *d : H.get_MyNearFieldMats()) {
+        leaves.insert(leaves.end(), {d->get_offset_i(), d->nb_rows(), d->get_offset_j(), d->nb_cols(), -1});
+        if (dump)
+            for (int j = 0; j < d->nb_cols(); j++)
+                for (int i = 0; i < d->nb_rows(); i++)
+                    data.push_back((*d)(i, j));
+    }
+    for (auto *l : H.get_MyFarFieldMats()) {
+        int k = l->rank_of();
+        leaves.insert(leaves.end(), {l->get_offset_i(), l->nb_rows(), l->get_offset_j(), l->nb_cols(), k});
+        if (dump) {
+            for (int r = 0; r < k; r++)
+                for (int i = 0; i < l->nb_rows(); i++)
+                    data.push_back(l->get_U(i, r));
+            for (int j = 0; j < l->nb_cols(); j++)
+                for (int r = 0; r < k; r++)
+                    data.push_back(l->get_V(r, j));
+        }
+    }
+    write_vec(out + "/leaves.i32", leaves);
+    if (dump)
+        write_vec(out + "/data.f64", data);
+
+    // Products: one untimed warm-up, then nreps timed ones.
+    std::vector<double> y(size_t(nt) * mu, 0.0);
+    H.mvprod_global_to_global(x.data(), y.data(), mu);
+    double tm = 0;
+    for (int r = 0; r < nreps; r++) {
+        t0 = now();
+        H.mvprod_global_to_global(x.data(), y.data(), mu);
+        tm += now() - t0;
+    }
+    write_vec(out + "/y.f64", y);
+    std::fprintf(info, "build_s=%.9g\nmatvec_s=%.9g\nreps=%d\nnlrmat=%d\nndmat=%d\nfalse_positive=%s\ncompression=%.9g\n",
+                 t_build, nreps > 0 ? tm / nreps : 0.0, nreps, H.get_nlrmat(), H.get_ndmat(),
+                 H.get_infos("Number_of_false_positive").c_str(), H.compression_ratio());
+    std::fclose(info);
+    MPI_Finalize();
+    return 0;
+}
