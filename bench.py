@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""H-matrix product benchmark on one MI355X.
+
+Workload (the headline configuration): N = 1e5 points uniform on the unit
+sphere (seeded), Laplace kernel 1/(4 pi r) with 1/(4 pi h) on the diagonal,
+h = sqrt(4 pi / N), eta = 10, epsilon = 1e-6, leaf size (minclustersize) 32,
+partial-pivot ACA. One step = one H-matvec (mu = 1) of the mixed-precision path
+(FP32 leaves, FP32 arithmetic inside a leaf tile, FP64 accumulation over leaves)
+on device-resident x and y. The plain FP64 path and the 32-RHS products are
+timed in the same run and reported beside it.
+
+value = algorithmic bytes of one product / time, in GB/s (every stored entry
+once, x read and y written once, the permutations).
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=100000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=10)
+    return ap.parse_args()
+
+
+def timed(fn, steps, warmup, stream, torch):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps
+
+
+def leaf_bytes(leaves):
+    """Entry counts read by each product kernel."""
+    import numpy as np
+    d = leaves[leaves[:, 4] < 0]
+    lr = leaves[leaves[:, 4] > 0]
+    dense = int((d[:, 1].astype(np.int64) * d[:, 3]).sum())
+    u = int((lr[:, 1].astype(np.int64) * lr[:, 4]).sum())
+    v = int((lr[:, 3].astype(np.int64) * lr[:, 4]).sum())
+    return dense, u, v
+
+
+def cpu_baseline(pts, diag, x, leaves_entries, args):
+    """The reference's own OpenMP CPU product (1 MPI rank) when its driver is
+    built, else the oracle's single-thread restatement, on the same workload."""
+    import numpy as np
+    from oracle import refcase
+    threads = os.cpu_count() or 1
+    threads = min(threads, 16)
+    if refcase.available():
+        try:
+            t0 = time.time()
+            r = refcase.run(pts, x=x, kernel=0, diag=diag, eta=10.0, eps=1e-6, minclustersize=32, compressor=0,
+                            nreps=args.cpu_reps, dump_data=False, threads=threads, timeout=900)
+            wall = time.time() - t0
+            L = r.leaves
+            dense = int((L[L[:, 4] < 0, 1].astype(np.int64) * L[L[:, 4] < 0, 3]).sum())
+            lr = L[L[:, 4] > 0]
+            lre = int((lr[:, 4].astype(np.int64) * (lr[:, 1] + lr[:, 3])).sum())
+            n = pts.shape[0]
+            b = 8 * (dense + lre) + 16 * n + 8 * n
+            t = float(r.info["matvec_s"])
+            return {"value": round(b / t / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "reference",
+                    "ms_per_matvec": round(t * 1e3, 3),
+                    "sample": f"full N={n} workload, {args.cpu_reps} timed FP64 matvecs of the reference HMatrix "
+                              f"(OpenMP, MKL sequential) after its own build ({float(r.info['build_s']):.1f}s); "
+                              f"driver wall {wall:.1f}s"}
+        except Exception as e:  # fall through to the restatement
+            print(f"reference baseline unavailable: {e}", file=sys.stderr)
+    from oracle import lib as orc
+    O = orc.hmat_build(pts, kernel=0, diag=diag, minclustersize=32, eta=10.0, eps=1e-6)
+    O.mvprod(x)
+    t0 = time.perf_counter()
+    reps = max(1, args.cpu_reps // 2)
+    for _ in range(reps):
+        O.mvprod(x)
+    t = (time.perf_counter() - t0) / reps
+    b = 8 * leaves_entries + 24 * pts.shape[0]
+    return {"value": round(b / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "ms_per_matvec": round(t * 1e3, 3),
+            "sample": f"oracle FP64 leaf loop, {reps} matvecs on the full workload, one thread"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only: the H-matrix product is measured on "
+                                   "one MI355X; a multi-GPU run is not part of this build", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from htool_amd import Generator, HMatrix, geometry
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n = args.n
+    pts = geometry.sphere(n, seed=0)
+    h = geometry.sphere_spacing(n)
+    diag = geometry.laplace_diag(h)
+    H = HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32)
+    H.set_compression("partialACA")
+    log(f"building H-matrix N={n}")
+    t0 = time.time()
+    H.build(Generator(geometry.KERNEL_LAPLACE, 0.0, diag), pts)
+    build_s = time.time() - t0
+    log(f"built in {build_s:.2f}s")
+    info = H.get_info()
+    leaves = H.get_leaves()
+    dense_e, u_e, v_e = leaf_bytes(leaves)
+
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(n)
+    X = rng.standard_normal((n, 32))
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    dx = torch.from_numpy(x).cuda()
+    dy = torch.zeros(n, dtype=torch.float64, device="cuda")
+    dX = torch.from_numpy(np.asfortranarray(X).reshape(-1, order="F").copy()).cuda()
+    dY = torch.zeros(n * 32, dtype=torch.float64, device="cuda")
+
+    res = {}
+    ys = {}
+    for storage in ("fp64", "fp32"):
+        H.set_storage(storage)
+        ms1 = timed(lambda: H.mvprod_device(dx.data_ptr(), dy.data_ptr(), 1, sp), args.steps, args.warmup, stream, torch)
+        ys[storage] = dy.cpu().numpy().copy()
+        ms32 = timed(lambda: H.mvprod_device(dX.data_ptr(), dY.data_ptr(), 32, sp), max(5, args.steps // 5),
+                     2, stream, torch)
+        prof = H.profile(dx.data_ptr(), dy.data_ptr(), 1, 20, sp)
+        log(f"{storage}: {ms1:.4f} ms/matvec, {ms32:.4f} ms/32-RHS product")
+        res[storage] = {"ms": ms1, "ms_mu32": ms32, "bytes": H.algorithmic_bytes(1, storage),
+                        "bytes_mu32": H.algorithmic_bytes(32, storage), "flops_mu32": H.flops(32), "prof": prof}
+
+    # accuracy: against dense FP64 rows and against the FP64 H product
+    from oracle import lib as orc
+    rows = np.random.default_rng(2).choice(n, 2000, replace=False).astype(np.int32)
+    yd = orc.dense_rows(pts, None, rows, x, kernel=0, diag=diag, same=True)
+    err = {s: float(np.linalg.norm(ys[s][rows] - yd) / np.linalg.norm(yd)) for s in ys}
+    err_mixed_vs_fp64 = float(np.linalg.norm(ys["fp32"] - ys["fp64"]) / np.linalg.norm(ys["fp64"]))
+
+    mixed = res["fp32"]
+    value = mixed["bytes"] / (mixed["ms"] * 1e-3) / 1e9
+    # dominant kernel of the headline step and its own achieved bandwidth
+    p = mixed["prof"]
+    s = 4
+    kern = {"row_apply": (p["row_apply_ms"], s * (dense_e + u_e) + 12 * n),
+            "lr_vx": (p["vx_ms"], s * v_e + 4 * n),
+            "gather_x": (p["gather_ms"], 8 * n + 4 * n + 4 * n)}
+    dom = max(kern, key=lambda k: kern[k][0])
+    achieved = kern[dom][1] / (kern[dom][0] * 1e-3) / 1e9
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.isfile(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline")
+        cpu = cpu_baseline(pts, diag, x, dense_e + u_e + v_e, args)
+
+    line = {
+        "metric": "hmatvec_hbm_gbps",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "hmatvec_laplace_sphere", "N": n, "eta": 10.0, "epsilon": 1e-6, "leaf_size": 32,
+                   "compressor": "partialACA", "mu": 1, "storage": "fp32 leaves, fp64 accumulation"},
+        "fp64": {"ms": round(res["fp64"]["ms"], 4),
+                 "gbps": round(res["fp64"]["bytes"] / (res["fp64"]["ms"] * 1e-3) / 1e9, 2),
+                 "ms_mu32": round(res["fp64"]["ms_mu32"], 4),
+                 "tflops_mu32": round(res["fp64"]["flops_mu32"] / (res["fp64"]["ms_mu32"] * 1e-3) / 1e12, 3),
+                 "rel_err_vs_dense": err["fp64"]},
+        "mixed": {"ms": round(mixed["ms"], 4), "gbps": round(value, 2), "ms_mu32": round(mixed["ms_mu32"], 4),
+                  "tflops_mu32": round(mixed["flops_mu32"] / (mixed["ms_mu32"] * 1e-3) / 1e12, 3),
+                  "rel_err_vs_dense": err["fp32"], "rel_err_vs_fp64_hmat": err_mixed_vs_fp64},
+        "speedup_mixed_vs_fp64": round(res["fp64"]["ms"] / mixed["ms"], 3),
+        "build_s": round(build_s, 3),
+        "leaves": {"dense": info.n_dense, "lowrank": info.n_lowrank, "false_positive": info.n_false_positive,
+                   "rank_mean": round(info.rank_mean, 3), "rank_max": info.rank_max,
+                   "dense_entries": dense_e, "lowrank_entries": u_e + v_e},
+        "kernels_ms": {k: round(v, 5) for k, v in p.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

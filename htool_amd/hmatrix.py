@@ -1,0 +1,203 @@
+"""Python mirror of the reference's HMatrix product surface over the C ABI.
+
+Names and argument meaning follow Htool 0.8.1 (reference files cited per
+method); the work is done by libhtool_amd.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import capi
+from . import geometry as _geo
+
+
+@dataclass
+class Generator:
+    """Device kernel descriptor standing in for a VirtualGenerator
+    (include/htool/types/virtual_generator.hpp:11-29) of a built-in kernel family."""
+    kernel: int = _geo.KERNEL_LAPLACE
+    delta: float = 0.0
+    diag: float = 0.0
+
+    def _c(self, same: bool) -> capi.htg_generator:
+        return capi.htg_generator(self.kernel, 1 if same else 0, float(self.delta), float(self.diag))
+
+
+_COMPRESSORS = {"partialACA": capi.PARTIAL_ACA, "sympartialACA": capi.SYM_PARTIAL_ACA}
+_STORAGE = {"fp64": capi.STORAGE_FP64, "fp32": capi.STORAGE_FP32}
+
+
+class HMatrix:
+    """H-matrix on the GPU.
+
+    HMatrix(epsilon, eta, reqrank) mirrors HMatrix<T>(t, s, epsilon, eta, 'N', 'N', reqrank)
+    (include/htool/types/hmatrix.hpp:115-122); the cluster trees are built inside
+    `build` from the points and `minclustersize` (cluster.hpp:57-92, 334).
+    """
+
+    def __init__(self, epsilon: float = 1e-6, eta: float = 10.0, reqrank: int = -1, minclustersize: int = 10):
+        self.epsilon = float(epsilon)
+        self.eta = float(eta)
+        self.reqrank = int(reqrank)
+        self.minclustersize = int(minclustersize)
+        self.maxblocksize = 1e6
+        self.compressor = capi.SYM_PARTIAL_ACA  # the reference's default (hmatrix.hpp:297-299)
+        self._h = None
+        self._storage = "fp64"
+
+    # setters of include/htool/types/hmatrix.hpp:213-220
+    def set_epsilon(self, e): self.epsilon = float(e)
+    def set_eta(self, e): self.eta = float(e)
+    def set_maxblocksize(self, m):
+        if m == 0:
+            raise ValueError("[Htool error] MaxBlockSize parameter cannot be zero")
+        self.maxblocksize = float(m)
+    def set_minclustersize(self, m):
+        if m == 0:
+            raise ValueError("[Htool error] MinClusterSize parameter cannot be zero")
+        self.minclustersize = int(m)
+
+    def set_compression(self, name: str):
+        self.compressor = _COMPRESSORS[name]
+
+    # -- assembly ---------------------------------------------------------------
+    def build(self, gen: Generator, xt: np.ndarray, xs: np.ndarray | None = None) -> "HMatrix":
+        """HMatrix::build(A, xt, xs) (include/htool/types/hmatrix.hpp:285-341)."""
+        L = capi.lib()
+        xt = np.ascontiguousarray(xt, dtype=np.float64)
+        same = xs is None
+        if not same:
+            xs = np.ascontiguousarray(xs, dtype=np.float64)
+        prm = capi.htg_params(self.epsilon, self.eta, self.maxblocksize, self.minclustersize, self.compressor,
+                              self.reqrank, 0)
+        g = gen._c(same)
+        h = C.c_void_p()
+        capi.check(L.htg_hmatrix_build(C.byref(prm), C.byref(g), xt.shape[0], capi.ptr(xt),
+                                       xt.shape[0] if same else xs.shape[0], capi.ptr(xs), C.byref(h)))
+        self._close()
+        self._h = h
+        self._storage = "fp64"
+        return self
+
+    @classmethod
+    def from_leaves(cls, nr: int, nc: int, perm_t, perm_s, leaves, data) -> "HMatrix":
+        """Product structures from externally built leaves (dense SubMatrix data and
+        LowRankMatrix U, V factors; layout of include/htool_amd.h)."""
+        L = capi.lib()
+        perm_t = np.ascontiguousarray(perm_t, dtype=np.int32)
+        perm_s = np.ascontiguousarray(perm_s, dtype=np.int32)
+        leaves = np.ascontiguousarray(leaves, dtype=np.int32).reshape(-1, 5)
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        h = C.c_void_p()
+        capi.check(L.htg_hmatrix_from_leaves(nr, nc, capi.ptr(perm_t), capi.ptr(perm_s), leaves.shape[0],
+                                             capi.ptr(leaves), capi.ptr(data), C.byref(h)))
+        obj = cls()
+        obj._h = h
+        return obj
+
+    def _close(self):
+        if self._h is not None and self._h.value:
+            capi.lib().htg_hmatrix_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self._close()
+        except Exception:
+            pass
+
+    def _handle(self):
+        if self._h is None or not self._h.value:
+            raise RuntimeError("HMatrix is not built")
+        return self._h
+
+    # -- storage ----------------------------------------------------------------
+    def set_storage(self, storage: str) -> None:
+        capi.check(capi.lib().htg_hmatrix_set_storage(self._handle(), _STORAGE[storage]))
+        self._storage = storage
+
+    @property
+    def storage(self) -> str:
+        return self._storage
+
+    # -- products ---------------------------------------------------------------
+    def mvprod_global_to_global(self, x: np.ndarray, mu: int = 1) -> np.ndarray:
+        """y = H x with x (nc, mu) and y (nr, mu) in original ordering
+        (include/htool/types/hmatrix.hpp:1073-1182). Host arrays, copied over PCIe."""
+        info = self.get_info()
+        x = np.asarray(x, dtype=np.float64)
+        vec = x.ndim == 1
+        xm = x.reshape(info.nc, -1)
+        mu = xm.shape[1]
+        xf = np.asfortranarray(xm)
+        y = np.zeros((info.nr, mu), order="F")
+        capi.check(capi.lib().htg_hmatrix_mvprod(self._handle(), capi.ptr(xf), capi.ptr(y), mu))
+        return y[:, 0].copy() if vec else np.ascontiguousarray(y)
+
+    def __matmul__(self, x):
+        """operator* (include/htool/types/hmatrix.hpp:1596-1602)."""
+        return self.mvprod_global_to_global(x)
+
+    def mvprod_device(self, d_x: int, d_y: int, mu: int, stream: int = 0) -> None:
+        """Same product on device pointers (e.g. torch tensors' data_ptr()), column-major
+        nc x mu / nr x mu, enqueued on `stream`."""
+        capi.check(capi.lib().htg_hmatrix_mvprod_device(self._handle(), C.c_void_p(d_x), C.c_void_p(d_y), mu,
+                                                        C.c_void_p(stream)))
+
+    def profile(self, d_x: int, d_y: int, mu: int, reps: int, stream: int = 0) -> dict:
+        ms = (C.c_double * 4)()
+        capi.check(capi.lib().htg_hmatrix_profile(self._handle(), C.c_void_p(d_x), C.c_void_p(d_y), mu, reps,
+                                                  C.c_void_p(stream), ms))
+        return {"gather_ms": ms[0], "vx_ms": ms[1], "row_apply_ms": ms[2], "total_ms": ms[3]}
+
+    # -- introspection ------------------------------------------------------------
+    def get_info(self) -> capi.htg_info:
+        info = capi.htg_info()
+        capi.check(capi.lib().htg_hmatrix_get_info(self._handle(), C.byref(info)))
+        return info
+
+    def get_infos(self) -> dict:
+        """Subset of the reference's infos map (include/htool/types/hmatrix.hpp:794-840)."""
+        i = self.get_info()
+        return {"Number_of_lrmat": i.n_lowrank, "Number_of_dmat": i.n_dense,
+                "Number_of_false_positive": i.n_false_positive, "Rank_min": i.rank_min, "Rank_max": i.rank_max,
+                "Rank_mean": i.rank_mean, "Compression_ratio": i.compression_ratio,
+                "Target_size": i.nr, "Source_size": i.nc, "Blocks_max": i.build_seconds}
+
+    def nb_rows(self) -> int:
+        return self.get_info().nr
+
+    def nb_cols(self) -> int:
+        return self.get_info().nc
+
+    def get_perms(self):
+        i = self.get_info()
+        pt = np.zeros(i.nr, np.int32)
+        ps = np.zeros(i.nc, np.int32)
+        capi.check(capi.lib().htg_hmatrix_get_perms(self._handle(), capi.ptr(pt), capi.ptr(ps)))
+        return pt, ps
+
+    def get_leaves(self, with_data: bool = False):
+        """Leaf list (save_plot layout, sorted by (off_i, off_j)) and, optionally,
+        the FP64 leaf data."""
+        n = C.c_int()
+        ds = C.c_int64()
+        capi.check(capi.lib().htg_hmatrix_nleaves(self._handle(), C.byref(n), C.byref(ds)))
+        leaves = np.zeros((n.value, 5), np.int32)
+        data = np.zeros(ds.value) if with_data else None
+        capi.check(capi.lib().htg_hmatrix_get_leaves(self._handle(), capi.ptr(leaves), capi.ptr(data)))
+        return (leaves, data) if with_data else leaves
+
+    def algorithmic_bytes(self, mu: int = 1, storage: str | None = None) -> int:
+        """Bytes one product must move: every stored entry once, x read and y
+        written once, the two permutations."""
+        i = self.get_info()
+        s = 4 if (storage or self._storage) == "fp32" else 8
+        return int(s * (i.dense_entries + i.lowrank_entries) + mu * (i.nc + i.nr) * 8 + 4 * (i.nr + i.nc))
+
+    def flops(self, mu: int = 1) -> int:
+        i = self.get_info()
+        return int(2 * mu * (i.dense_entries + i.lowrank_entries))

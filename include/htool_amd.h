@@ -1,0 +1,156 @@
+/* htool_amd: MI355X-native H-matrix product and partial-pivot ACA compression.
+ *
+ * C ABI of libhtool_amd.so. Plain pointers and sizes only; no exception ever
+ * crosses it. Every entry point returns an htg_status (HTG_OK == 0) and leaves
+ * a message for htg_last_error() on failure.
+ *
+ * Layout conventions follow the reference (Htool 0.8.1):
+ *  - points: interleaved x[3*i + d], like HMatrix::build(A, xt, xs)
+ *    (include/htool/types/hmatrix.hpp:285);
+ *  - products: x is nc x mu and y is nr x mu, column-major, original (not
+ *    cluster) ordering, like HMatrix::mvprod_global_to_global
+ *    (include/htool/types/hmatrix.hpp:1073-1182);
+ *  - leaves: (off_i, m, off_j, n, rank) in cluster ordering, rank -1 for a dense
+ *    leaf, like HMatrix::save_plot (include/htool/types/hmatrix.hpp:1695-1712);
+ *    leaf data: dense m x n column-major (SubMatrix, include/htool/types/matrix.hpp:572),
+ *    low-rank U (m x k column-major) then V (k x n column-major, V(r,j) at r + k*j)
+ *    (LowRankMatrix, include/htool/lrmat/lrmat.hpp:47-53).
+ */
+#ifndef HTOOL_AMD_H
+#define HTOOL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum htg_status {
+    HTG_OK               = 0,
+    HTG_ERR_INVALID      = 1, /* bad argument (the reference throws std::invalid_argument) */
+    HTG_ERR_HIP          = 2, /* HIP runtime error, or no usable GPU */
+    HTG_ERR_OOM          = 3, /* device or host allocation failed */
+    HTG_ERR_UNSUPPORTED  = 4, /* kernel family / mode not implemented */
+    HTG_ERR_RANK_CAP     = 5  /* a block needed more ACA terms than the library allows */
+} htg_status;
+
+/* Kernel families evaluated on the device; they stand in for a user
+ * VirtualGenerator::copy_submatrix (include/htool/types/virtual_generator.hpp:22). */
+typedef enum htg_kernel {
+    HTG_KERNEL_LAPLACE        = 0, /* 1/(4 pi r); `diag` on the diagonal when the point sets coincide */
+    HTG_KERNEL_INV_DELTA_R    = 1, /* 1/(delta + r) */
+    HTG_KERNEL_INV_DELTA_4PIR = 2, /* 1/(delta + 4 pi r): GeneratorTestDoubleSymmetric,
+                                      include/htool/testing/generator_test.hpp:135-145 */
+    HTG_KERNEL_SMOOTH         = 3  /* 1/(4 pi sqrt(delta + r^2)): examples/smallest_example.cpp:16-27 */
+} htg_kernel;
+
+typedef enum htg_compressor {
+    HTG_PARTIAL_ACA     = 0, /* include/htool/lrmat/partialACA.hpp */
+    HTG_SYM_PARTIAL_ACA = 1  /* include/htool/lrmat/sympartialACA.hpp (the reference default) */
+} htg_compressor;
+
+/* Storage/arithmetic of the leaf products. Accumulation over leaves and the
+ * output are always FP64. */
+typedef enum htg_storage {
+    HTG_STORAGE_FP64 = 0, /* FP64 leaves, FP64 arithmetic */
+    HTG_STORAGE_FP32 = 1  /* FP32 leaves, FP32 arithmetic inside a leaf tile */
+} htg_storage;
+
+typedef struct htg_generator {
+    int kernel;      /* htg_kernel */
+    int same_points; /* 1: source points are the target points (square matrix) */
+    double delta;
+    double diag;
+} htg_generator;
+
+/* Parameters of HMatrix<T>(t, s, epsilon, eta, 'N', 'N', reqrank) plus the
+ * setters the hot path reads (include/htool/types/hmatrix.hpp:115, 213-220)
+ * and Cluster::set_minclustersize (include/htool/clustering/cluster.hpp:334). */
+typedef struct htg_params {
+    double epsilon;     /* ACA tolerance (default 1e-6) */
+    double eta;         /* admissibility parameter (default 10) */
+    double maxblocksize;/* largest coalesced dense block, entries (default 1e6) */
+    int minclustersize; /* default 10 */
+    int compressor;     /* htg_compressor */
+    int reqrank;        /* -1: epsilon-driven ranks; >0: fixed rank */
+    int reserved;
+} htg_params;
+
+typedef struct htg_info {
+    int nr, nc;
+    int n_dense, n_lowrank, n_false_positive;
+    int rank_min, rank_max;
+    double rank_mean;
+    double compression_ratio; /* nr*nc / stored entries, include/htool/types/hmatrix.hpp:1604-1632 */
+    int64_t dense_entries;    /* sum of m*n over dense leaves */
+    int64_t lowrank_entries;  /* sum of k*(m+n) over low-rank leaves */
+    int n_row_blocks, n_pieces, n_vtiles;
+    double build_seconds;     /* block tree + assembly */
+} htg_info;
+
+typedef struct htg_hmatrix htg_hmatrix;
+
+const char *htg_last_error(void);
+const char *htg_status_string(int status);
+
+/* Replaces HMatrix<T>(t, s, eps, eta) + set_compression + build(A, xt, xs)
+ * (include/htool/types/hmatrix.hpp:115-122, 220, 285-341) for the built-in
+ * kernel families: cluster and block trees on the host, kernel entries and ACA
+ * on the current HIP device. xs == NULL (or gen->same_points) uses the target points. */
+int htg_hmatrix_build(const htg_params *params, const htg_generator *gen, int nt, const double *xt, int ns,
+                      const double *xs, htg_hmatrix **out);
+
+/* Builds the product structures from leaves produced elsewhere (dense
+ * SubMatrix blocks and LowRankMatrix U, V factors; the layout of the header
+ * comment), e.g. the leaves of a reference HMatrix: replaces the leaf list that
+ * HMatrix::ComputeBlocks leaves in MyComputedBlocks (include/htool/types/hmatrix.hpp:422-501). */
+int htg_hmatrix_from_leaves(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
+                            const double *data, htg_hmatrix **out);
+
+int htg_hmatrix_destroy(htg_hmatrix *h);
+
+/* Adds (or keeps) the storage variant used by the next products. */
+int htg_hmatrix_set_storage(htg_hmatrix *h, int storage);
+
+int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info);
+
+/* Leaf list sorted by (off_i, off_j) (MyComputedBlocks order, hmatrix.hpp:494-500),
+ * 5 ints per leaf; leaf data in FP64 in the layout of the header comment. */
+int htg_hmatrix_nleaves(const htg_hmatrix *h, int *nleaves, int64_t *data_size);
+int htg_hmatrix_get_leaves(const htg_hmatrix *h, int *leaves5, double *data /* may be NULL */);
+int htg_hmatrix_get_perms(const htg_hmatrix *h, int *perm_t, int *perm_s);
+
+/* y = H x, replaces HMatrix::mvprod_global_to_global(in, out, mu)
+ * (include/htool/types/hmatrix.hpp:1073-1182). Host pointers: copies over PCIe. */
+int htg_hmatrix_mvprod(htg_hmatrix *h, const double *x, double *y, int mu);
+
+/* Same product on device pointers, enqueued on `stream` (a hipStream_t, NULL
+ * for the default stream); returns without synchronising. */
+int htg_hmatrix_mvprod_device(htg_hmatrix *h, const double *d_x, double *d_y, int mu, void *stream);
+
+/* Times each product kernel with HIP events over `reps` products of device
+ * buffers on `stream`; ms[0..3] = average ms of (gather, V x, row apply, whole product). */
+int htg_hmatrix_profile(htg_hmatrix *h, const double *d_x, double *d_y, int mu, int reps, void *stream, double *ms);
+
+/* Batched stand-alone leaf compression, replaces calling
+ * VirtualLowRankGenerator::copy_low_rank_approximation once per block
+ * (include/htool/lrmat/virtual_lrmat_generator.hpp:15, partialACA.hpp:39-171,
+ * sympartialACA.hpp:38-209). Block b covers target points rows[row_off[b] ..
+ * row_off[b]+M[b]) and source points cols[col_off[b] .. +N[b]) (original
+ * indices into xt / xs); ctr_t/ctr_s (3 doubles per block) are the target and
+ * source cluster centres and off_t/off_s their cluster offsets (used for the
+ * first pivot and the sympartialACA orientation). Outputs: ranks[b] (-1 =
+ * incompressible), U at u_off[b] (M x k column-major) and V at v_off[b]
+ * (k x N column-major) into the caller's host arrays sized with `cap` terms per
+ * block (u_off[b+1]-u_off[b] >= M[b]*cap). */
+int htg_aca_batch(const htg_generator *gen, int compressor, double epsilon, int reqrank, int nt, const double *xt,
+                  int ns, const double *xs, int nblocks, const int *M, const int *N, const int *rows,
+                  const int *row_off, const int *cols, const int *col_off, const double *ctr_t, const double *ctr_s,
+                  const int *off_t, const int *off_s, int cap, int *ranks, double *U, const int64_t *u_off, double *V,
+                  const int64_t *v_off);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

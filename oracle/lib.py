@@ -1,0 +1,175 @@
+"""ctypes binding of oracle/liboracle.so (see oracle/oracle.h). Test infrastructure."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_VP, _I, _D = C.c_void_p, C.c_int, C.c_double
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built: run `make -C oracle oracle`")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "orc_kernel_block": (None, [_I, _D, _D, _I, _VP, _VP, _I, _I, _VP, _VP, _VP]),
+            "orc_cluster_build": (_I, [_I, _VP, _I, _VP, _VP, _VP]),
+            "orc_aca": (_I, [_I, _D, _I, _I, _VP, _VP, _I, _D, _D, _I, _VP, _VP, _VP, _I, _VP, _I, _I, _I, _VP, _VP]),
+            "orc_hmat_build": (_VP, [_I, _VP, _I, _VP, _I, _D, _D, _I, _D, _D, _I, _I, _D]),
+            "orc_hmat_nleaves": (_I, [_VP]),
+            "orc_hmat_leaves": (None, [_VP, _VP]),
+            "orc_hmat_data_size": (C.c_longlong, [_VP]),
+            "orc_hmat_data": (None, [_VP, _VP]),
+            "orc_hmat_perms": (None, [_VP, _VP, _VP]),
+            "orc_hmat_false_positives": (_I, [_VP]),
+            "orc_hmat_mvprod": (None, [_VP, _VP, _VP, _I]),
+            "orc_hmat_free": (None, [_VP]),
+            "orc_leaf_mvprod": (None, [_I, _I, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I]),
+            "orc_dense_rows": (None, [_I, _D, _D, _I, _VP, _VP, _I, _I, _VP, _VP, _I, _VP]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def kernel_block(kernel, delta, diag, xt, xs, rows, cols, same=False):
+    xt = np.ascontiguousarray(xt, np.float64)
+    xs = xt if same else np.ascontiguousarray(xs, np.float64)
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    out = np.zeros(len(rows) * len(cols))
+    lib().orc_kernel_block(kernel, delta, diag, int(same), _p(xt), _p(xs), len(rows), len(cols), _p(rows), _p(cols),
+                           _p(out))
+    return out.reshape(len(cols), len(rows)).T
+
+
+@dataclass
+class ClusterTree:
+    perm: np.ndarray
+    nodes: np.ndarray   # (n, 6): offset, size, depth, rank, nb_sons, first_son
+    geom: np.ndarray    # (n, 4): ctr0..2, rad
+
+
+def cluster_build(x, minclustersize=10) -> ClusterTree:
+    x = np.ascontiguousarray(x, np.float64)
+    n = x.shape[0]
+    perm = np.zeros(n, np.int32)
+    ni = np.zeros((2 * n + 2, 6), np.int32)
+    nd = np.zeros((2 * n + 2, 4))
+    cnt = lib().orc_cluster_build(n, _p(x), minclustersize, _p(perm), _p(ni), _p(nd))
+    return ClusterTree(perm, ni[:cnt].copy(), nd[:cnt].copy())
+
+
+def aca(xt, xs, rows, cols, *, kernel=0, delta=0.0, diag=0.0, same=False, eps=1e-6, sym=False, ctr_t=None,
+        ctr_s=None, off_t=0, off_s=0, reqrank=-1, cap=None):
+    xt = np.ascontiguousarray(xt, np.float64)
+    xs = xt if same else np.ascontiguousarray(xs, np.float64)
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    M, N = len(rows), len(cols)
+    cap = cap or (min(M, N) + 1)
+    U = np.zeros(M * cap)
+    V = np.zeros(N * cap)
+    ct = np.ascontiguousarray(ctr_t, np.float64)
+    cs = np.ascontiguousarray(ctr_s if ctr_s is not None else ctr_t, np.float64)
+    k = lib().orc_aca(int(sym), eps, M, N, _p(rows), _p(cols), kernel, delta, diag, int(same), _p(xt), _p(xs),
+                      _p(ct), off_t, _p(cs), off_s, reqrank, cap, _p(U), _p(V))
+    if k <= 0:
+        return k, None, None
+    return k, U[:M * k].reshape(k, M).T.copy(), V[:N * k].reshape(N, k).T.copy()
+
+
+@dataclass
+class OracleHMatrix:
+    nr: int
+    nc: int
+    perm_t: np.ndarray
+    perm_s: np.ndarray
+    leaves: np.ndarray
+    data: np.ndarray
+    false_positive: int
+
+    def mvprod(self, x):
+        return leaf_mvprod(self.nr, self.nc, self.perm_t, self.perm_s, self.leaves, self.data, x)
+
+
+def hmat_build(xt, xs=None, *, kernel=0, delta=0.0, diag=0.0, minclustersize=10, eta=10.0, eps=1e-6, sym=False,
+               reqrank=-1, maxblocksize=1e6) -> OracleHMatrix:
+    L = lib()
+    xt = np.ascontiguousarray(xt, np.float64)
+    xs_ = None if xs is None else np.ascontiguousarray(xs, np.float64)
+    nt = xt.shape[0]
+    ns = nt if xs is None else xs_.shape[0]
+    h = L.orc_hmat_build(nt, _p(xt), ns, _p(xs_), kernel, delta, diag, minclustersize, eta, eps, int(sym), reqrank,
+                         maxblocksize)
+    try:
+        n = L.orc_hmat_nleaves(h)
+        leaves = np.zeros((n, 5), np.int32)
+        L.orc_hmat_leaves(h, _p(leaves))
+        data = np.zeros(L.orc_hmat_data_size(h))
+        L.orc_hmat_data(h, _p(data))
+        pt = np.zeros(nt, np.int32)
+        ps = np.zeros(ns, np.int32)
+        L.orc_hmat_perms(h, _p(pt), _p(ps))
+        fp = L.orc_hmat_false_positives(h)
+    finally:
+        L.orc_hmat_free(h)
+    return OracleHMatrix(nt, ns, pt, ps, leaves, data, fp)
+
+
+def leaf_mvprod(nr, nc, perm_t, perm_s, leaves, data, x):
+    x = np.asarray(x, np.float64)
+    vec = x.ndim == 1
+    xm = np.asfortranarray(x.reshape(nc, -1))
+    mu = xm.shape[1]
+    y = np.zeros((nr, mu), order="F")
+    leaves = np.ascontiguousarray(leaves, np.int32)
+    data = np.ascontiguousarray(data, np.float64)
+    lib().orc_leaf_mvprod(nr, nc, _p(np.ascontiguousarray(perm_t, np.int32)), _p(np.ascontiguousarray(perm_s, np.int32)),
+                          leaves.shape[0], _p(leaves), _p(data), _p(xm), _p(y), mu)
+    return y[:, 0].copy() if vec else np.ascontiguousarray(y)
+
+
+def dense_rows(xt, xs, rows, x, *, kernel=0, delta=0.0, diag=0.0, same=False):
+    xt = np.ascontiguousarray(xt, np.float64)
+    xs = xt if same else np.ascontiguousarray(xs, np.float64)
+    rows = np.ascontiguousarray(rows, np.int32)
+    x = np.asarray(x, np.float64)
+    vec = x.ndim == 1
+    xm = np.asfortranarray(x.reshape(xs.shape[0], -1))
+    mu = xm.shape[1]
+    out = np.zeros((len(rows), mu), order="F")
+    lib().orc_dense_rows(kernel, delta, diag, int(same), _p(xt), _p(xs), xs.shape[0], len(rows), _p(rows), _p(xm),
+                         mu, _p(out))
+    return out[:, 0].copy() if vec else np.ascontiguousarray(out)
+
+
+def iter_leaf_data(leaves, data):
+    """Yields (off_i, m, off_j, n, k, A or (U, V)) in leaf order."""
+    pos = 0
+    for oi, m, oj, n, k in leaves:
+        if k < 0:
+            yield oi, m, oj, n, k, data[pos:pos + m * n].reshape(n, m).T
+            pos += m * n
+        else:
+            u = data[pos:pos + m * k].reshape(k, m).T
+            pos += m * k
+            v = data[pos:pos + k * n].reshape(n, k).T
+            pos += k * n
+            yield oi, m, oj, n, k, (u, v)

@@ -1,0 +1,181 @@
+"""GPU parity: libhtool_amd (through its C ABI) against the CPU oracle.
+
+Bars: FP64 products within 1e-12 relative l2 of the oracle on the same leaves;
+leaf lists (offsets, sizes, ranks) identical; dense leaf entries bitwise equal
+(same arithmetic on both sides); low-rank factors within 1e-10 relative (the
+ACA's dot products are reduced in a different order on the GPU); products
+within 10 epsilon of brute-force dense FP64 rows (the reference's product tests
+check rel l2 < epsilon at epsilon = 1e-3,
+tests/functional_tests/types/virtual_hmatrix/test_virtual_hmat_product.hpp:214-219; at
+epsilon = 1e-6 the ACA stopping estimate lets the two-disk case reach 2.7e-6,
+identically in the oracle).
+"""
+import numpy as np
+import pytest
+
+from htool_amd import geometry as geo
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+CASES = {
+    # name: (points_t, points_s or None, kernel, delta, diag, minclustersize, compressor)
+    "cylinder_reg": lambda: (geo.cylinder(4000, seed=1), None, geo.KERNEL_INV_DELTA_R, 1e-5, 0.0, 10, "partialACA"),
+    "cylinder_reg_sym": lambda: (geo.cylinder(4000, seed=1), None, geo.KERNEL_INV_DELTA_R, 1e-5, 0.0, 10,
+                                 "sympartialACA"),
+    "sphere_laplace": lambda: (geo.sphere(6000, seed=2), None, geo.KERNEL_LAPLACE, 0.0,
+                               geo.laplace_diag(geo.sphere_spacing(6000)), 32, "partialACA"),
+    "two_disks": lambda: (geo.disk(1500, 0.0, seed=3), geo.disk(1100, 1.5, seed=4), geo.KERNEL_LAPLACE, 0.0, 0.0, 10,
+                          "partialACA"),
+    "smooth_sym": lambda: (geo.ellipsoid(3000, (1.0, 0.7, 1.6), seed=5), None, geo.KERNEL_SMOOTH, 1e-5, 0.0, 10,
+                           "sympartialACA"),
+}
+
+
+def build_both(htg, orc, name, eps=1e-6, eta=10.0):
+    xt, xs, kernel, delta, diag, mcs, comp = CASES[name]()
+    H = htg.HMatrix(epsilon=eps, eta=eta, minclustersize=mcs)
+    H.set_compression(comp)
+    H.build(htg.Generator(kernel, delta, diag), xt, xs)
+    O = orc.hmat_build(xt, xs, kernel=kernel, delta=delta, diag=diag, minclustersize=mcs, eta=eta, eps=eps,
+                       sym=(comp == "sympartialACA"))
+    return H, O, (xt, xs, kernel, delta, diag)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_build_matches_oracle(htg, orc, name):
+    print(f"case {name}", flush=True)
+    H, O, (xt, xs, kernel, delta, diag) = build_both(htg, orc, name)
+    pt, ps = H.get_perms()
+    assert np.array_equal(pt, O.perm_t) and np.array_equal(ps, O.perm_s)
+    leaves, data = H.get_leaves(with_data=True)
+    assert np.array_equal(leaves, O.leaves), "leaf list or ranks differ"
+    assert H.get_info().n_false_positive == O.false_positive
+    worst = 0.0
+    for (oi, m, oj, n, k, a), (_, _, _, _, _, b) in zip(orc.iter_leaf_data(leaves, data),
+                                                        orc.iter_leaf_data(O.leaves, O.data)):
+        if k < 0:
+            assert np.array_equal(a, b), f"dense leaf ({oi},{oj}) entries differ"
+        else:
+            worst = max(worst, np.abs(a[0] - b[0]).max() / np.abs(b[0]).max(),
+                        np.abs(a[1] - b[1]).max() / np.abs(b[1]).max())
+    assert worst < 1e-10, worst
+    n = H.nb_cols()
+    x = np.random.default_rng(11).standard_normal(n)
+    y = H @ x
+    yo = O.mvprod(x)
+    assert rel(y, yo) < 1e-12
+    # against the true matrix on sampled rows
+    same = xs is None
+    rows = np.random.default_rng(12).choice(H.nb_rows(), min(400, H.nb_rows()), replace=False).astype(np.int32)
+    yd = orc.dense_rows(xt, xs, rows, x, kernel=kernel, delta=delta, diag=diag, same=same)
+    assert rel(y[rows], yd) < 1e-5
+
+
+@pytest.mark.parametrize("mu", [1, 3, 8, 32, 64, 70])
+def test_multi_rhs_matches_oracle(htg, orc, mu):
+    print(f"mu {mu}", flush=True)
+    H, O, _ = build_both(htg, orc, "sphere_laplace")
+    X = np.random.default_rng(mu).standard_normal((H.nb_cols(), mu))
+    Y = H.mvprod_global_to_global(X)
+    Yo = O.mvprod(X)
+    assert Y.shape == Yo.shape
+    for c in range(mu):
+        assert rel(Y[:, c], Yo[:, c]) < 1e-12
+
+
+def test_from_leaves_same_leaves(htg, orc):
+    """Same tree, same U and V: the GPU product against the oracle's product."""
+    xt = geo.cylinder(4000, seed=1)
+    O = orc.hmat_build(xt, kernel=geo.KERNEL_INV_DELTA_R, delta=1e-5, minclustersize=10, eta=10.0, eps=1e-6)
+    H = htg.HMatrix.from_leaves(O.nr, O.nc, O.perm_t, O.perm_s, O.leaves, O.data)
+    for mu in (1, 5):
+        x = np.random.default_rng(mu).uniform(-1, 1, (O.nc, mu))
+        assert rel(H.mvprod_global_to_global(x), O.mvprod(x)) < 1e-12
+    leaves, data = H.get_leaves(with_data=True)
+    assert np.array_equal(leaves, O.leaves) and np.array_equal(data, O.data)
+
+
+@pytest.mark.parametrize("name", ["sphere_laplace", "two_disks", "cylinder_reg"])
+def test_fp32_storage(htg, orc, name):
+    H, O, (xt, xs, kernel, delta, diag) = build_both(htg, orc, name)
+    x = np.random.default_rng(21).standard_normal(H.nb_cols())
+    y64 = H @ x
+    H.set_storage("fp32")
+    y32 = H @ x
+    assert rel(y32, y64) < 2e-6
+    X = np.random.default_rng(22).standard_normal((H.nb_cols(), 32))
+    Y32 = H.mvprod_global_to_global(X)
+    H.set_storage("fp64")
+    Y64 = H.mvprod_global_to_global(X)
+    assert rel(Y32, Y64) < 2e-6
+
+
+def test_zero_leaves_give_zero(htg):
+    """All-zero leaves: ||H x|| < 1e-15 (tests/functional_tests/types/virtual_hmatrix/hmatrix/test_hmat_zero.hpp:163-166)."""
+    n = 300
+    perm = np.arange(n, dtype=np.int32)
+    leaves = np.array([[0, 150, 0, 300, -1], [150, 150, 0, 150, 3], [150, 150, 150, 150, -1]], np.int32)
+    size = 150 * 300 + 3 * 300 + 150 * 150
+    H = htg.HMatrix.from_leaves(n, n, perm, perm, leaves, np.zeros(size))
+    y = H @ np.random.default_rng(0).standard_normal(n)
+    assert np.linalg.norm(y) < 1e-15
+
+
+def test_tiny_problem_single_dense_leaf(htg, orc):
+    """Fewer points than minclustersize: one dense block, product equals the dense product."""
+    xt = geo.sphere(20, seed=9)
+    diag = 0.5
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32)
+    H.build(htg.Generator(geo.KERNEL_LAPLACE, 0.0, diag), xt)
+    x = np.random.default_rng(1).standard_normal(20)
+    yd = orc.dense_rows(xt, None, np.arange(20, dtype=np.int32), x, kernel=geo.KERNEL_LAPLACE, diag=diag, same=True)
+    assert rel(H @ x, yd) < 1e-14
+
+
+def test_aca_batch_lrmat_disks(htg, orc):
+    """The reference's leaf-compression test geometry (500 x 100 disks at distance
+    15/20/30/40, eps 1e-4; tests/functional_tests/lrmat/test_lrmat_partialACA.cpp:23-67):
+    ranks and factors equal to the oracle's, fixed rank 10 gives space saving 0.88."""
+    blocks = []
+    ref = []
+    for i, dist in enumerate([15, 20, 30, 40]):
+        xt = geo.disk(500, 0.0, seed=100 + i)
+        xs = geo.disk(100, float(dist), seed=200 + i)
+        Tt = orc.cluster_build(xt, 10)
+        Ts = orc.cluster_build(xs, 10)
+        ct, cs = tuple(Tt.geom[0, :3]), tuple(Ts.geom[0, :3])
+        blocks.append((xt, xs, Tt.perm, Ts.perm, ct, cs))
+    for sym in (False, True):
+        for reqrank in (-1, 10):
+            for xt, xs, pt, ps, ct, cs in blocks:
+                b = htg.Block(pt, ps, ct, cs, 0, 0)
+                (k, U, V), = htg.aca_batch(htg.Generator(geo.KERNEL_LAPLACE), [b], xt, xs, epsilon=1e-4,
+                                          reqrank=reqrank, compressor="sympartialACA" if sym else "partialACA",
+                                          cap=64)
+                ko, Uo, Vo = orc.aca(xt, xs, pt, ps, kernel=geo.KERNEL_LAPLACE, eps=1e-4, sym=sym, ctr_t=ct,
+                                     ctr_s=cs, reqrank=reqrank)
+                assert k == ko
+                assert np.abs(U - Uo).max() <= 1e-10 * np.abs(Uo).max()
+                assert np.abs(V - Vo).max() <= 1e-10 * np.abs(Vo).max()
+                saving = 1 - k * (1 / 500 + 1 / 100)
+                if reqrank == 10:
+                    assert k == 10 and 0.87 < saving < 0.89
+                else:
+                    assert 0.93 < saving < 0.96
+
+
+def test_device_pointer_product(htg, orc):
+    torch = pytest.importorskip("torch")
+    H, O, _ = build_both(htg, orc, "two_disks")
+    x = np.random.default_rng(5).standard_normal((H.nb_cols(), 4))
+    dx = torch.from_numpy(np.asfortranarray(x).reshape(-1, order="F").copy()).cuda()
+    dy = torch.zeros(H.nb_rows() * 4, dtype=torch.float64, device="cuda")
+    H.mvprod_device(dx.data_ptr(), dy.data_ptr(), 4, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    y = dy.cpu().numpy().reshape(4, H.nb_rows()).T
+    assert rel(y, O.mvprod(x)) < 1e-12
