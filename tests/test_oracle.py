@@ -1,0 +1,221 @@
+"""CPU checks that pin the oracle (oracle/oracle.cpp) before it is trusted:
+
+* golden fixtures written by the reference Htool build (tests/golden/*.npz):
+  permutations, leaf lists, ranks and leaf data must be identical, products
+  within 1e-13;
+* the reference driver itself where it is built (same checks on fresh cases);
+* what the mathematics fixes: closed-form kernel entries, cluster-tree
+  invariants, every matrix entry covered by exactly one leaf, admissibility of
+  every low-rank leaf, the reference's lrmat bars (fixed rank 10 error < 1e-8,
+  space saving 0.88 / (0.93, 0.96)), the Eckart-Young floor of a truncated SVD,
+  the leaf product against a dense numpy reconstruction, brute-force dense rows.
+"""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from htool_amd import geometry as geo
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLDEN, name + ".npz")))
+
+
+HMAT_FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                       if not os.path.basename(p).startswith("lrmat"))
+LRMAT_FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "lrmat*.npz")))
+
+
+def oracle_from_fixture(orc, g):
+    xs = None if int(g["same"]) else g["xs"]
+    return orc.hmat_build(g["xt"], xs, kernel=int(g["kernel"]), delta=float(g["delta"]), diag=float(g["diag"]),
+                          minclustersize=int(g["mcs"]), eta=float(g["eta"]), eps=float(g["eps"]),
+                          sym=bool(int(g["sym"])))
+
+
+@pytest.mark.parametrize("name", HMAT_FIXTURES)
+def test_oracle_matches_reference_fixture(orc, name):
+    g = load(name)
+    O = oracle_from_fixture(orc, g)
+    assert np.array_equal(O.perm_t, g["perm_t"]) and np.array_equal(O.perm_s, g["perm_s"])
+    assert np.array_equal(O.leaves, g["leaves"])
+    assert O.false_positive == int(g["false_positive"])
+    ours = list(orc.iter_leaf_data(O.leaves, O.data))
+    pos = 0
+    for li in g["data_leaves"]:
+        oi, m, oj, n, k, arr = ours[li]
+        if k < 0:
+            ref = g["data"][pos:pos + m * n].reshape(n, m).T
+            pos += m * n
+            assert np.array_equal(arr, ref)
+        else:
+            u = g["data"][pos:pos + m * k].reshape(k, m).T
+            pos += m * k
+            v = g["data"][pos:pos + k * n].reshape(n, k).T
+            pos += k * n
+            assert np.array_equal(arr[0], u) and np.array_equal(arr[1], v)
+    y = O.mvprod(g["x"])
+    assert rel(y, g["y"]) < 1e-13
+
+
+@pytest.mark.parametrize("name", LRMAT_FIXTURES)
+def test_oracle_aca_matches_reference_fixture(orc, name):
+    g = load(name)
+    T = orc.cluster_build(g["xt"], 10)
+    S = orc.cluster_build(g["xs"], 10)
+    assert np.array_equal(T.perm, g["perm_t"]) and np.array_equal(S.perm, g["perm_s"])
+    k, U, V = orc.aca(g["xt"], g["xs"], T.perm, S.perm, kernel=geo.KERNEL_LAPLACE, eps=float(g["eps"]),
+                      sym=bool(int(g["sym"])), ctr_t=T.geom[0, :3], ctr_s=S.geom[0, :3], reqrank=int(g["reqrank"]))
+    assert k == int(g["rank"])
+    assert np.array_equal(U, g["U"]) and np.array_equal(V, g["V"])
+
+
+def test_oracle_matches_reference_driver(orc, ref):
+    """Fresh cases straight from the reference build (this container only)."""
+    cases = [
+        (geo.cylinder(4000, seed=1), None, geo.KERNEL_INV_DELTA_R, 1e-5, 0.0, 10, 0),
+        (geo.cylinder(4000, seed=1), None, geo.KERNEL_INV_DELTA_R, 1e-5, 0.0, 10, 1),
+        (geo.sphere(8000, seed=3), None, geo.KERNEL_LAPLACE, 0.0, geo.laplace_diag(geo.sphere_spacing(8000)), 32, 0),
+    ]
+    for xt, xs, kernel, delta, diag, mcs, sym in cases:
+        x = np.random.default_rng(5).standard_normal((xt.shape[0], 1))
+        r = ref.run(xt, xs, x=x, kernel=kernel, delta=delta, diag=diag, minclustersize=mcs, eta=10.0, eps=1e-6,
+                    compressor=sym)
+        O = orc.hmat_build(xt, xs, kernel=kernel, delta=delta, diag=diag, minclustersize=mcs, eta=10.0, eps=1e-6,
+                           sym=bool(sym))
+        assert np.array_equal(O.perm_t, r.perm_t)
+        order = np.lexsort((r.leaves[:, 2], r.leaves[:, 0]))
+        assert np.array_equal(O.leaves, r.leaves[order])
+        assert rel(O.mvprod(x), r.y) < 1e-13
+
+
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3])
+def test_kernel_entries_closed_form(orc, kernel):
+    xt = geo.sphere(37, seed=1)
+    xs = geo.disk(23, 2.0, seed=2)
+    rows = np.arange(37, dtype=np.int32)[::-1].copy()
+    cols = np.arange(23, dtype=np.int32)
+    A = orc.kernel_block(kernel, 1e-5, 0.0, xt, xs, rows, cols)
+    r = np.linalg.norm(xt[rows][:, None, :] - xs[cols][None, :, :], axis=2)
+    ref = {0: 1 / (4 * math.pi * r), 1: 1 / (1e-5 + r), 2: 1 / (1e-5 + 4 * math.pi * r),
+           3: 1 / (4 * math.pi * np.sqrt(1e-5 + r * r))}[kernel]
+    assert np.allclose(A, ref, rtol=1e-14, atol=0)
+    # diagonal rule of the singular Laplace kernel on a square same-point matrix
+    D = orc.kernel_block(0, 0.0, 7.5, xt, None, rows, rows, same=True)
+    assert np.all(np.diag(D) == 7.5)
+
+
+def test_cluster_tree_invariants(orc):
+    x = geo.ellipsoid(3000, (1.0, 0.5, 2.0), seed=4)
+    T = orc.cluster_build(x, 25)
+    assert np.array_equal(np.sort(T.perm), np.arange(3000))
+    nodes, geom = T.nodes, T.geom
+    assert nodes[0, 1] == 3000 and nodes[0, 4] == 1  # root: all points, one son (one process)
+    for c in range(len(nodes)):
+        off, size, depth, rank, nsons, first = nodes[c]
+        pts = x[T.perm[off:off + size]]
+        assert np.allclose(geom[c, :3], pts.mean(axis=0), atol=1e-12)
+        assert abs(geom[c, 3] - np.linalg.norm(pts - geom[c, :3], axis=1).max()) < 1e-12
+        if nsons:
+            sons = nodes[first:first + nsons]
+            assert sons[0, 0] == off and sons[:, 1].sum() == size
+            assert np.all(sons[1:, 0] == sons[:-1, 0] + sons[:-1, 1])  # contiguous
+            if depth > 0:
+                assert np.all(sons[:, 1] >= 25)
+        # PCA split: son 0 lies on the positive side of the principal axis
+        if nsons == 2:
+            s0 = x[T.perm[nodes[first, 0]:nodes[first, 0] + nodes[first, 1]]]
+            s1 = x[T.perm[nodes[first + 1, 0]:nodes[first + 1, 0] + nodes[first + 1, 1]]]
+            w, vecs = np.linalg.eigh(np.cov((pts - geom[c, :3]).T, bias=True))
+            d = vecs[:, -1]
+            p0 = (s0 - geom[c, :3]) @ d
+            p1 = (s1 - geom[c, :3]) @ d
+            assert (np.all(p0 > -1e-9) and np.all(p1 < 1e-9)) or (np.all(p0 < 1e-9) and np.all(p1 > -1e-9))
+
+
+def test_leaves_cover_matrix_once_and_far_leaves_admissible(orc):
+    n = 1500
+    x = geo.sphere(n, seed=6)
+    eta = 10.0
+    O = orc.hmat_build(x, kernel=0, diag=1.0, minclustersize=20, eta=eta, eps=1e-6)
+    cover = np.zeros((n, n), np.int8)
+    for oi, m, oj, nn, k in O.leaves:
+        cover[oi:oi + m, oj:oj + nn] += 1
+        if k >= 0:  # Rjasanow-Steinbach on the leaf's own clusters
+            pt = x[O.perm_t[oi:oi + m]]
+            ps = x[O.perm_s[oj:oj + nn]]
+            ct, cs = pt.mean(axis=0), ps.mean(axis=0)
+            rt = np.linalg.norm(pt - ct, axis=1).max()
+            rs = np.linalg.norm(ps - cs, axis=1).max()
+            assert 2 * min(rt, rs) < eta * max(np.linalg.norm(ct - cs) - rt - rs, 0.0) + 1e-12
+    assert np.all(cover == 1)
+
+
+def test_leaf_product_equals_dense_reconstruction(orc):
+    n = 900
+    x = geo.cylinder(n, seed=7)
+    O = orc.hmat_build(x, kernel=1, delta=1e-5, minclustersize=10, eta=10.0, eps=1e-6)
+    H = np.zeros((n, n))
+    for oi, m, oj, nn, k, a in orc.iter_leaf_data(O.leaves, O.data):
+        H[oi:oi + m, oj:oj + nn] = a if k < 0 else a[0] @ a[1]
+    Hg = np.zeros((n, n))
+    Hg[np.ix_(O.perm_t, O.perm_s)] = H  # back to original ordering
+    v = np.random.default_rng(1).standard_normal((n, 3))
+    assert rel(O.mvprod(v), Hg @ v) < 1e-13
+    # and the H-matrix is the kernel matrix up to epsilon
+    r = np.linalg.norm(x[:, None, :] - x[None, :, :], axis=2)
+    A = 1 / (1e-5 + r)
+    assert np.linalg.norm(Hg - A) / np.linalg.norm(A) < 1e-5
+
+
+def test_dense_rows_brute_force(orc):
+    xt = geo.sphere(300, seed=8)
+    xs = geo.disk(200, 3.0, seed=9)
+    x = np.random.default_rng(2).standard_normal((200, 2))
+    rows = np.array([0, 17, 299, 150], np.int32)
+    out = orc.dense_rows(xt, xs, rows, x, kernel=0)
+    A = 1 / (4 * math.pi * np.linalg.norm(xt[rows][:, None, :] - xs[None, :, :], axis=2))
+    assert rel(out, A @ x) < 1e-14
+
+
+@pytest.mark.parametrize("dist,seed", [(15, 100), (20, 101), (30, 102), (40, 200)])
+def test_aca_reference_bars_and_svd_floor(orc, dist, seed):
+    """tests/functional_tests/lrmat/test_lrmat.hpp:28-42, 83-88 with 500 x 100 disks. The
+    reference draws its disks with C rand(); these are PCG64 disks (the ones of the
+    lrmat fixtures for distances 15 and 40), on which its bars hold as well."""
+    xt = geo.disk(500, 0.0, seed=seed)
+    xs = geo.disk(100, float(dist), seed=seed + 1)
+    T = orc.cluster_build(xt, 10)
+    S = orc.cluster_build(xs, 10)
+    A = orc.kernel_block(0, 0.0, 0.0, xt, xs, T.perm, S.perm)
+    sv = np.linalg.svd(A, compute_uv=False)
+    for sym in (False, True):
+        k, U, V = orc.aca(xt, xs, T.perm, S.perm, kernel=0, eps=1e-4, sym=sym, ctr_t=T.geom[0, :3],
+                          ctr_s=S.geom[0, :3], reqrank=10)
+        assert k == 10
+        err10 = np.linalg.norm(A - U @ V)
+        assert err10 < 1e-8
+        assert err10 >= np.sqrt((sv[10:] ** 2).sum()) * (1 - 1e-9)  # Eckart-Young: no rank-10 beats the SVD
+        assert 0.87 < 1 - k * (1 / 500 + 1 / 100) < 0.89
+        k, U, V = orc.aca(xt, xs, T.perm, S.perm, kernel=0, eps=1e-4, sym=sym, ctr_t=T.geom[0, :3],
+                          ctr_s=S.geom[0, :3], reqrank=-1)
+        assert 0.93 < 1 - k * (1 / 500 + 1 / 100) < 0.96
+        assert np.linalg.norm(A - U @ V) / np.linalg.norm(A) < 1e-4
+
+
+def test_aca_rejects_incompressible_block(orc):
+    """A random dense block is not compressible: rank -1 (partialACA.hpp:74-77)."""
+    # points far apart relative to each other within an 8 x 8 block: q(m+n) > mn quickly
+    xt = geo.sphere(8, seed=3) * 5
+    xs = geo.sphere(8, seed=4) * 5 + 0.01
+    k, U, V = orc.aca(xt, xs, np.arange(8), np.arange(8), kernel=0, eps=1e-14, ctr_t=xt.mean(0), ctr_s=xs.mean(0))
+    assert k == -1 and U is None
