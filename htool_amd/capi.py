@@ -41,7 +41,7 @@ class htg_info(C.Structure):
     _fields_ = [("nr", C.c_int), ("nc", C.c_int), ("n_dense", C.c_int), ("n_lowrank", C.c_int),
                 ("n_false_positive", C.c_int), ("rank_min", C.c_int), ("rank_max", C.c_int),
                 ("rank_mean", C.c_double), ("compression_ratio", C.c_double), ("dense_entries", C.c_int64),
-                ("lowrank_entries", C.c_int64), ("n_row_blocks", C.c_int), ("n_pieces", C.c_int),
+                ("lowrank_entries", C.c_int64), ("n_row_blocks", C.c_int), ("n_columns", C.c_int),
                 ("n_vtiles", C.c_int), ("build_seconds", C.c_double)]
 
 

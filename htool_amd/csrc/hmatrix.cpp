@@ -72,17 +72,24 @@ struct DevArray {
 
 constexpr int RB_MAX     = 128;   // rows per row block
 constexpr int ALIGN      = 32;    // tile alignment in elements
-constexpr int VCHUNK_EL  = 32768; // target entries per V column chunk
+constexpr int VCHUNK_EL  = 8192;  // target entries per V column chunk
+constexpr int VGROUP_EL  = 8192;  // target entries per phase-1 wavefront
+constexpr int VGROUP_TILES = 32;  // most V chunks per phase-1 wavefront
 constexpr int CAP_FIRST  = 64;    // ACA terms reserved per block in the first attempt
 constexpr int CAP_MAX    = 512;   // must match the ACA kernel's CAPMAX
 constexpr size_t WS_MAX  = size_t(1) << 30; // doubles of ACA workspace per sub-batch (8 GiB)
 
 inline uint64_t align_up(uint64_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
+inline int pad4(int v) { return (v + 3) & ~3; }
 
 struct Leaf {
     int oi, m, oj, n, k; // k = -1: dense
     uint64_t src_u = 0, src_v = 0; // build path: offsets of U and V in the compressed store
 };
+
+// Stored row length of a low-rank leaf's V: the global columns
+// [oj & ~3, round4(oj + n)), so rows and the matching x entries are 16-byte aligned.
+inline int v_ld(const Leaf &L) { return pad4(L.oj + L.n) - (L.oj & ~3); }
 
 } // namespace
 } // namespace htg
@@ -96,24 +103,23 @@ struct htg_hmatrix {
     int false_positive = 0;
     double build_seconds = 0;
 
-    // product layout
-    std::vector<int> rb_row0, piece_off;
-    std::vector<Piece> pieces;
+    // product layout (see build_layout)
+    std::vector<int> rb_row0, rb_col0, rb_order, colidx;
+    std::vector<uint64_t> rb_aoff;
+    std::vector<int> rb_ld; // row stride of each row block's block (columns rounded up to 4)
+    std::vector<int> vg_off, multi_first; // multi_first: first chunk of each leaf with several chunks
     std::vector<VTile> vtiles;
-    std::vector<uint64_t> leaf_piece_first; // first piece of each leaf in `piece_of_leaf` order
-    std::vector<std::vector<int>> leaf_pieces;
+    std::vector<std::vector<std::pair<int, uint64_t>>> leaf_tiles; // per leaf: (row block, tile offset)
     std::vector<uint64_t> leaf_v_off;
     uint64_t arena_size = 0, varena_size = 0;
     int t_size = 0, part_size = 0, n_lr = 0;
 
-    DevArray<int> d_perm_t, d_perm_s, d_rb_row0, d_piece_off;
-    DevArray<Piece> d_pieces;
+    DevArray<int> d_perm_t, d_perm_s, d_rb_row0, d_rb_col0, d_rb_order, d_colidx, d_vg_off, d_multi_first;
+    DevArray<uint64_t> d_rb_aoff;
     DevArray<VTile> d_vtiles;
     DevArray<double> d_arena64, d_varena64;
     DevArray<float> d_arena32, d_varena32;
-    DevArray<double> d_t, d_part;
-    DevArray<unsigned> d_counters;
-    DevArray<double> d_xc; // RHS-interleaved x in cluster order (FP64 or FP32 view)
+    DevArray<double> d_z, d_part; // z: cluster-ordered x then t (FP64 or FP32 view)
     DevArray<double> d_hx, d_hy; // staging for host-pointer products
     int storage = HTG_STORAGE_FP64;
 };
@@ -121,8 +127,16 @@ struct htg_hmatrix {
 namespace htg {
 namespace {
 
-// Row blocks: the segments between consecutive leaf row boundaries (every
-// leaf covers a segment entirely or not at all), split to at most RB_MAX rows.
+// Product layout.
+//  * Row blocks: the segments between consecutive leaf row boundaries (every
+//    leaf covers a segment entirely or not at all), split to <= RB_MAX rows.
+//  * Each row block owns one contiguous R x C column-major block in the tile
+//    arena: the rows of its dense leaves (n columns each) and of its low-rank
+//    leaves' U (k columns each), in leaf order. colidx gives, per column, the
+//    z entry it multiplies: an x entry (dense) or a t entry (low rank).
+//  * Each low-rank leaf stores V column-major (k x n) in the V arena; its
+//    columns are cut into chunks of about VCHUNK_EL entries, and chunks are
+//    grouped into wavefront work lists of about VGROUP_EL entries.
 void build_layout(htg_hmatrix &h) {
     std::vector<int> bnd = {0, h.nr};
     for (const Leaf &L : h.leaves) {
@@ -141,94 +155,126 @@ void build_layout(htg_hmatrix &h) {
     h.rb_row0.push_back(h.nr);
     const int nrb = (int)h.rb_row0.size() - 1;
 
-    // (row block, leaf) incidences, grouped by row block, leaves in sorted order.
-    std::vector<int> count(nrb + 1, 0);
-    std::vector<std::pair<int, int>> inc; // (first rb, last rb+1) per leaf
-    inc.reserve(h.leaves.size());
-    for (const Leaf &L : h.leaves) {
-        const int r0 = (int)(std::upper_bound(h.rb_row0.begin(), h.rb_row0.end(), L.oi) - h.rb_row0.begin()) - 1;
-        int r1       = r0;
-        while (r1 < nrb && h.rb_row0[r1] < L.oi + L.m)
-            r1++;
-        inc.emplace_back(r0, r1);
-        if (L.k != 0)
-            for (int r = r0; r < r1; r++)
-                count[r + 1]++;
-    }
-    h.piece_off.assign(nrb + 1, 0);
-    for (int r = 0; r < nrb; r++)
-        h.piece_off[r + 1] = h.piece_off[r] + count[r + 1];
-    h.pieces.assign(h.piece_off[nrb], Piece{});
-    std::vector<int> fill(h.piece_off.begin(), h.piece_off.end() - 1);
-    h.leaf_pieces.assign(h.leaves.size(), {});
-    h.leaf_v_off.assign(h.leaves.size(), 0);
-
-    // t offsets and V arena offsets of the low-rank leaves.
-    h.t_size = 0;
+    // t offsets (low-rank terms) and V arena offsets.
+    h.t_size      = 0;
     h.varena_size = 0;
-    h.n_lr = 0;
     std::vector<int> t_off(h.leaves.size(), -1);
+    h.leaf_v_off.assign(h.leaves.size(), 0);
     for (size_t l = 0; l < h.leaves.size(); l++) {
         const Leaf &L = h.leaves[l];
         if (L.k > 0) {
             t_off[l] = h.t_size;
             h.t_size += L.k;
             h.leaf_v_off[l] = h.varena_size;
-            h.varena_size   = align_up(h.varena_size + (uint64_t)L.k * L.n);
+            h.varena_size   = align_up(h.varena_size + (uint64_t)L.k * v_ld(L));
         }
     }
+
+    // Leaves of each row block, in leaf order.
+    std::vector<int> cnt(nrb + 1, 0);
+    std::vector<std::pair<int, int>> inc(h.leaves.size());
     for (size_t l = 0; l < h.leaves.size(); l++) {
         const Leaf &L = h.leaves[l];
-        if (L.k == 0)
-            continue;
-        for (int r = inc[l].first; r < inc[l].second; r++) {
-            Piece &p = h.pieces[fill[r]];
-            p.kind    = L.k < 0 ? 0 : 1;
-            p.ncols   = L.k < 0 ? L.n : L.k;
-            p.vec_off = L.k < 0 ? L.oj : t_off[l];
-            h.leaf_pieces[l].push_back(fill[r]);
-            fill[r]++;
-        }
+        const int r0 = (int)(std::upper_bound(h.rb_row0.begin(), h.rb_row0.end(), L.oi) - h.rb_row0.begin()) - 1;
+        int r1       = r0;
+        while (r1 < nrb && h.rb_row0[r1] < L.oi + L.m)
+            r1++;
+        inc[l] = {r0, r1};
+        if (L.k != 0)
+            for (int r = r0; r < r1; r++)
+                cnt[r + 1]++;
     }
-    // Tile arena offsets in row-block order, so a row block reads one contiguous stretch.
+    for (int r = 0; r < nrb; r++)
+        cnt[r + 1] += cnt[r];
+    std::vector<int> rb_leaves(cnt[nrb]);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (size_t l = 0; l < h.leaves.size(); l++)
+        if (h.leaves[l].k != 0)
+            for (int r = inc[l].first; r < inc[l].second; r++)
+                rb_leaves[fill[r]++] = (int)l;
+
+    h.leaf_tiles.assign(h.leaves.size(), {});
+    h.rb_col0.assign(nrb + 1, 0);
+    h.rb_aoff.assign(nrb, 0);
+    h.rb_ld.assign(nrb, 0);
+    h.colidx.clear();
     h.arena_size = 0;
     for (int r = 0; r < nrb; r++) {
-        const int R = h.rb_row0[r + 1] - h.rb_row0[r];
-        for (int p = h.piece_off[r]; p < h.piece_off[r + 1]; p++) {
-            h.pieces[p].a_off = h.arena_size;
-            h.arena_size      = align_up(h.arena_size + (uint64_t)R * h.pieces[p].ncols);
+        const int R   = h.rb_row0[r + 1] - h.rb_row0[r];
+        h.rb_aoff[r]  = h.arena_size;
+        int col       = 0;
+        for (int q = cnt[r]; q < cnt[r + 1]; q++) {
+            const int l   = rb_leaves[q];
+            const Leaf &L = h.leaves[l];
+            h.leaf_tiles[l].emplace_back(r, h.rb_aoff[r] + (uint64_t)col);
+            if (L.k < 0) {
+                for (int j = 0; j < L.n; j++)
+                    h.colidx.push_back(L.oj + j);
+                col += L.n;
+            } else {
+                for (int j = 0; j < L.k; j++)
+                    h.colidx.push_back(pad4(h.nc) + t_off[l] + j);
+                col += L.k;
+            }
         }
+        h.rb_col0[r + 1] = h.rb_col0[r] + col;
+        h.rb_ld[r]       = pad4(col);
+        h.arena_size     = align_up(h.arena_size + (uint64_t)R * h.rb_ld[r]);
     }
-    // V column chunks.
+    // Heaviest row blocks are dispatched first.
+    h.rb_order.resize(nrb);
+    for (int r = 0; r < nrb; r++)
+        h.rb_order[r] = r;
+    std::stable_sort(h.rb_order.begin(), h.rb_order.end(), [&](int a, int b) {
+        const int64_t wa = (int64_t)(h.rb_row0[a + 1] - h.rb_row0[a]) * (h.rb_col0[a + 1] - h.rb_col0[a]);
+        const int64_t wb = (int64_t)(h.rb_row0[b + 1] - h.rb_row0[b]) * (h.rb_col0[b + 1] - h.rb_col0[b]);
+        return wa > wb;
+    });
+
+    // V column chunks, grouped per workgroup.
     h.vtiles.clear();
+    h.multi_first.clear();
+    h.vg_off.assign(1, 0);
     h.part_size = 0;
-    int lr_id   = 0;
+    int lr_id = 0;
+    int64_t group_el = 0;
     for (size_t l = 0; l < h.leaves.size(); l++) {
         const Leaf &L = h.leaves[l];
         if (L.k <= 0)
             continue;
-        int cpc = std::max(256, (VCHUNK_EL / L.k) / 64 * 64);
-        const int nparts  = (L.n + cpc - 1) / cpc;
-        const int pbase   = h.part_size;
+        const int ld     = v_ld(L);
+        const int cpc    = pad4(std::max(64, VCHUNK_EL / L.k));
+        const int nparts = (ld + cpc - 1) / cpc;
+        const int pbase  = h.part_size;
         if (nparts > 1)
             h.part_size += nparts * L.k;
         for (int c = 0; c < nparts; c++) {
             VTile v{};
             const int j0 = c * cpc;
-            v.v_off      = h.leaf_v_off[l] + j0;
-            v.k          = L.k;
-            v.ld         = L.n;
-            v.ncols      = std::min(cpc, L.n - j0);
-            v.x_off      = L.oj + j0;
-            v.part_off   = pbase + c * L.k;
-            v.leaf       = lr_id;
-            v.nparts     = nparts;
-            v.part_base  = pbase;
-            v.t_off      = t_off[l];
+            v.v_off     = h.leaf_v_off[l] + (uint64_t)j0;
+            v.ld        = ld;
+            v.k         = L.k;
+            v.ncols     = std::min(cpc, ld - j0);
+            v.x_off     = (L.oj & ~3) + j0;
+            v.t_index   = pad4(h.nc) + t_off[l];
+            v.part_off  = pbase + c * L.k;
+            v.part_base = pbase;
+            v.nparts    = nparts;
+            if (nparts > 1 && c == 0)
+                h.multi_first.push_back((int)h.vtiles.size());
+            const int64_t el = (int64_t)v.k * v.ncols;
+            const int in_group = (int)h.vtiles.size() - h.vg_off.back();
+            if (in_group > 0 && (group_el + el > VGROUP_EL || in_group >= VGROUP_TILES)) {
+                h.vg_off.push_back((int)h.vtiles.size());
+                group_el = 0;
+            }
             h.vtiles.push_back(v);
+            group_el += el;
         }
         lr_id++;
     }
+    if ((int)h.vtiles.size() > h.vg_off.back())
+        h.vg_off.push_back((int)h.vtiles.size());
     h.n_lr = lr_id;
 }
 
@@ -236,14 +282,18 @@ void upload_layout(htg_hmatrix &h) {
     h.d_perm_t.upload(h.perm_t);
     h.d_perm_s.upload(h.perm_s);
     h.d_rb_row0.upload(h.rb_row0);
-    h.d_piece_off.upload(h.piece_off);
-    h.d_pieces.upload(h.pieces);
+    h.d_rb_col0.upload(h.rb_col0);
+    h.d_rb_order.upload(h.rb_order);
+    h.d_rb_aoff.upload(h.rb_aoff);
+    h.d_colidx.upload(h.colidx);
+    h.d_vg_off.upload(h.vg_off);
     h.d_vtiles.upload(h.vtiles);
-    h.d_t.alloc((size_t)std::max(h.t_size, 1) * 64);
+    h.d_multi_first.upload(h.multi_first);
+    // z = [x (nc entries, zero padded to a multiple of 4) | t], per RHS; zeroed
+    // once so the padding entries are always finite zeros.
+    h.d_z.alloc((size_t)(pad4(h.nc) + h.t_size) * 64);
+    HTG_HIP(hipMemset(h.d_z.p, 0, h.d_z.n * sizeof(double)));
     h.d_part.alloc((size_t)std::max(h.part_size, 1) * 64);
-    h.d_counters.alloc((size_t)std::max(h.n_lr, 1));
-    HTG_HIP(hipMemset(h.d_counters.p, 0, h.d_counters.n * sizeof(unsigned)));
-    h.d_xc.alloc((size_t)h.nc * 64);
 }
 
 struct PointsOnDevice {
@@ -443,14 +493,16 @@ struct Assembler {
                         LrPack p{};
                         p.src_u = B.swap ? B.ws2 : B.ws1;
                         p.src_v = B.swap ? B.ws1 : B.ws2;
-                        p.M = B.M, p.N = B.N, p.k = k, p.oi = B.off_t;
+                        p.M = B.M, p.N = B.N, p.k = k;
+                        p.src_ldv = B.N;
+                        p.dst_ldv = B.N;
                         Leaf L{B.off_t, B.M, B.off_s, B.N, k};
                         L.src_u = store_used;
                         L.src_v = store_used + (uint64_t)B.M * k;
                         p.dst_v = L.src_v;
                         p.piece_first = (int)udst.size();
                         p.npieces     = 1;
-                        udst.push_back(UTileDst{L.src_u, 0, B.M});
+                        udst.push_back(UTileDst{L.src_u, 0, B.M, 1, B.M});
                         store_used += (uint64_t)k * (B.M + B.N);
                         packs.push_back(p);
                         lr_out.push_back(L);
@@ -507,26 +559,24 @@ int pick_mu_chunk(int remaining) {
 template <typename TS, typename TC>
 void product(htg_hmatrix &h, const TS *arena, const TS *varena, const double *d_x, double *d_y, int mu,
              hipStream_t st, hipEvent_t *ev) {
-    TC *xc = reinterpret_cast<TC *>(h.d_xc.p);
-    for (int base = 0; base < mu; base += 64) {
-        int mb = 0;
-        while (base + mb < std::min(mu, base + 64)) {
-            const int MU = pick_mu_chunk(std::min(mu, base + 64) - (base + mb));
-            if (ev)
-                HTG_HIP(hipEventRecord(ev[0], st));
-            launch_gather_x<TS, TC>(d_x, h.d_perm_s.p, h.nc, mu, base + mb, MU, xc, st);
-            if (ev)
-                HTG_HIP(hipEventRecord(ev[1], st));
-            launch_lr_vx<TS, TC>(h.d_vtiles.p, (int)h.vtiles.size(), varena, xc, h.d_part.p, h.d_t.p,
-                                 h.d_counters.p, MU, st);
-            if (ev)
-                HTG_HIP(hipEventRecord(ev[2], st));
-            launch_row_apply<TS, TC>(h.d_rb_row0.p, h.d_piece_off.p, h.d_pieces.p, (int)h.rb_row0.size() - 1, arena,
-                                     xc, h.d_t.p, h.d_perm_t.p, h.nr, base + mb, MU, d_y, st);
-            if (ev)
-                HTG_HIP(hipEventRecord(ev[3], st));
-            mb += MU;
-        }
+    TC *z = reinterpret_cast<TC *>(h.d_z.p);
+    int done = 0;
+    while (done < mu) {
+        const int MU = pick_mu_chunk(mu - done);
+        if (ev)
+            HTG_HIP(hipEventRecord(ev[0], st));
+        launch_gather_x<TC>(d_x, h.d_perm_s.p, h.nc, done, MU, z, st);
+        if (ev)
+            HTG_HIP(hipEventRecord(ev[1], st));
+        launch_vx<TS, TC>(h.d_vg_off.p, (int)h.vg_off.size() - 1, h.d_vtiles.p, h.d_multi_first.p,
+                          (int)h.multi_first.size(), varena, z, h.d_part.p, MU, st);
+        if (ev)
+            HTG_HIP(hipEventRecord(ev[2], st));
+        launch_rows<TS, TC>(h.d_rb_order.p, h.d_rb_row0.p, h.d_rb_col0.p, h.d_rb_aoff.p, (int)h.rb_row0.size() - 1,
+                            h.d_colidx.p, arena, z, h.d_perm_t.p, h.nr, done, MU, d_y, st);
+        if (ev)
+            HTG_HIP(hipEventRecord(ev[3], st));
+        done += MU;
     }
     HTG_HIP(hipGetLastError());
 }
@@ -663,31 +713,33 @@ int htg_hmatrix_build(const htg_params *prm, const htg_generator *gen, int nt, c
         // Dense tiles straight from the generator; low-rank tiles from the store.
         h->d_arena64.alloc(h->arena_size);
         h->d_varena64.alloc(h->varena_size);
+        HTG_HIP(hipMemset(h->d_arena64.p, 0, h->d_arena64.n * sizeof(double))); // row padding stays zero
+        HTG_HIP(hipMemset(h->d_varena64.p, 0, h->d_varena64.n * sizeof(double)));
         std::vector<FillTile> fills;
         std::vector<LrPack> packs;
         std::vector<UTileDst> udst;
         for (size_t l = 0; l < h->leaves.size(); l++) {
             const Leaf &L = h->leaves[l];
             if (L.k < 0) {
-                for (int p : h->leaf_pieces[l]) {
-                    const int rb = (int)(std::upper_bound(h->piece_off.begin(), h->piece_off.end(), p) -
-                                         h->piece_off.begin()) - 1;
+                for (const auto &rt : h->leaf_tiles[l]) {
+                    const int rb = rt.first;
                     const int R  = h->rb_row0[rb + 1] - h->rb_row0[rb];
-                    fills.push_back(FillTile{h->pieces[p].a_off, h->rb_row0[rb], R, L.oj, L.n});
+                    fills.push_back(FillTile{rt.second, h->rb_row0[rb], R, L.oj, L.n, h->rb_ld[rb]});
                 }
             } else if (L.k > 0) {
                 LrPack pk{};
                 pk.src_u = L.src_u;
                 pk.src_v = L.src_v;
-                pk.dst_v = h->leaf_v_off[l];
-                pk.M = L.m, pk.N = L.n, pk.k = L.k, pk.oi = L.oi;
+                pk.dst_v = h->leaf_v_off[l] + (L.oj & 3);
+                pk.M = L.m, pk.N = L.n, pk.k = L.k;
+                pk.src_ldv = L.n;
+                pk.dst_ldv = v_ld(L);
                 pk.piece_first = (int)udst.size();
-                pk.npieces     = (int)h->leaf_pieces[l].size();
-                for (int p : h->leaf_pieces[l]) {
-                    const int rb = (int)(std::upper_bound(h->piece_off.begin(), h->piece_off.end(), p) -
-                                         h->piece_off.begin()) - 1;
+                pk.npieces     = (int)h->leaf_tiles[l].size();
+                for (const auto &rt : h->leaf_tiles[l]) {
+                    const int rb = rt.first;
                     const int R  = h->rb_row0[rb + 1] - h->rb_row0[rb];
-                    udst.push_back(UTileDst{h->pieces[p].a_off, h->rb_row0[rb] - L.oi, R});
+                    udst.push_back(UTileDst{rt.second, h->rb_row0[rb] - L.oi, R, h->rb_ld[rb], 1});
                 }
                 packs.push_back(pk);
             }
@@ -754,23 +806,24 @@ int htg_hmatrix_from_leaves(int nr, int nc, const int *perm_t, const int *perm_s
         for (int l = 0; l < nleaves; l++) {
             const Leaf &L  = h->leaves[l];
             const double *d = ssrc[l];
-            for (int pidx : h->leaf_pieces[l]) {
-                const int rb = (int)(std::upper_bound(h->piece_off.begin(), h->piece_off.end(), pidx) -
-                                     h->piece_off.begin()) - 1;
+            for (const auto &rt : h->leaf_tiles[l]) {
+                const int rb = rt.first;
                 const int R  = h->rb_row0[rb + 1] - h->rb_row0[rb];
                 const int i0 = h->rb_row0[rb] - L.oi;
-                double *dst  = arena.data() + h->pieces[pidx].a_off;
+                double *dst  = arena.data() + rt.second;
+                const size_t ld = h->rb_ld[rb];
                 const int nc_tile = L.k < 0 ? L.n : L.k; // dense: A columns; low rank: U columns
                 for (int j = 0; j < nc_tile; j++)
                     for (int i = 0; i < R; i++)
-                        dst[i + (size_t)R * j] = d[(size_t)(i0 + i) + (size_t)L.m * j];
+                        dst[i * ld + j] = d[(size_t)(i0 + i) + (size_t)L.m * j];
             }
-            if (L.k > 0) {
-                const double *V = d + (size_t)L.m * L.k; // k x n column-major
-                double *dst     = varena.data() + h->leaf_v_off[l];
-                for (int r = 0; r < L.k; r++)
-                    for (int j = 0; j < L.n; j++)
-                        dst[(size_t)r * L.n + j] = V[r + (size_t)L.k * j];
+            if (L.k > 0) { // V: column-major k x n in, row-major (padded rows) on the device
+                const double *V = d + (size_t)L.m * L.k;
+                double *dst     = varena.data() + h->leaf_v_off[l] + (L.oj & 3);
+                const size_t np = v_ld(L);
+                for (int j = 0; j < L.n; j++)
+                    for (int r = 0; r < L.k; r++)
+                        dst[r * np + j] = V[r + (size_t)L.k * j];
             }
         }
         upload_layout(*h);
@@ -832,7 +885,7 @@ int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info) {
         const double stored  = double(r.dense_entries) + double(r.lowrank_entries);
         r.compression_ratio  = stored > 0 ? double(h->nr) * double(h->nc) / stored : 0.0;
         r.n_row_blocks       = (int)h->rb_row0.size() - 1;
-        r.n_pieces           = (int)h->pieces.size();
+        r.n_columns           = (int)h->colidx.size();
         r.n_vtiles           = (int)h->vtiles.size();
         r.build_seconds      = h->build_seconds;
         *info                = r;
@@ -873,22 +926,23 @@ int htg_hmatrix_get_leaves(const htg_hmatrix *h, int *leaves5, double *data) {
             const int ncol = L.k < 0 ? L.n : L.k;
             if (L.k == 0)
                 continue;
-            for (int pidx : h->leaf_pieces[l]) {
-                const int rb = (int)(std::upper_bound(h->piece_off.begin(), h->piece_off.end(), pidx) -
-                                     h->piece_off.begin()) - 1;
+            for (const auto &rt : h->leaf_tiles[l]) {
+                const int rb = rt.first;
                 const int R  = h->rb_row0[rb + 1] - h->rb_row0[rb];
                 const int i0 = h->rb_row0[rb] - L.oi;
-                const double *src = arena.data() + h->pieces[pidx].a_off;
+                const double *src = arena.data() + rt.second;
+                const size_t ld   = h->rb_ld[rb];
                 for (int j = 0; j < ncol; j++)
                     for (int i = 0; i < R; i++)
-                        out[(size_t)(i0 + i) + (size_t)L.m * j] = src[i + (size_t)R * j];
+                        out[(size_t)(i0 + i) + (size_t)L.m * j] = src[i * ld + j];
             }
             out += (size_t)L.m * ncol;
             if (L.k > 0) {
-                const double *src = varena.data() + h->leaf_v_off[l];
-                for (int r = 0; r < L.k; r++)
-                    for (int j = 0; j < L.n; j++)
-                        out[r + (size_t)L.k * j] = src[(size_t)r * L.n + j];
+                const double *src = varena.data() + h->leaf_v_off[l] + (L.oj & 3);
+                const size_t np   = v_ld(L);
+                for (int j = 0; j < L.n; j++)
+                    for (int r = 0; r < L.k; r++)
+                        out[r + (size_t)L.k * j] = src[r * np + j];
                 out += (size_t)L.k * L.n;
             }
         }

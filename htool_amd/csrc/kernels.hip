@@ -50,13 +50,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // Sum over the workgroup, same value returned to every thread, fixed order.
 __device__ double block_sum(double v, double *s_red) {
     v = wave_sum(v);
@@ -119,8 +112,8 @@ __global__ void __launch_bounds__(WG) k_dense_fill(KernelDesc k, DevPoints T, De
     const FillTile ft = tiles[blockIdx.x];
     const int total   = ft.R * ft.ncols;
     for (int e = threadIdx.x; e < total; e += WG) {
-        const int i = e % ft.R, j = e / ft.R;
-        arena[ft.off + e] = kernel_entry(k, T, ft.row0 + i, S, ft.col0 + j);
+        const int i = e / ft.ncols, j = e % ft.ncols;
+        arena[ft.off + (size_t)i * ft.ld + j] = kernel_entry(k, T, ft.row0 + i, S, ft.col0 + j);
     }
 }
 
@@ -303,21 +296,22 @@ __global__ void __launch_bounds__(WG) k_aca(KernelDesc kd, DevPoints T, DevPoint
 }
 
 // ---------------------------------------------------------------------------
-// Copies compressed leaves from the ACA workspace into the product layout:
-// V as a k x N row-major block, U split into one R x k column-major tile per
-// row block.
+// Copies compressed leaves (from the ACA workspace or the compressed store)
+// into another layout: V row by row, U into one tile per destination.
 __global__ void __launch_bounds__(WG) k_lr_pack(const LrPack *packs, const UTileDst *utiles, const double *ws,
                                                 double *arena, double *varena) {
-    const LrPack p = packs[blockIdx.x];
+    const LrPack p  = packs[blockIdx.x];
     const size_t nv = (size_t)p.k * p.N;
-    for (size_t e = threadIdx.x; e < nv; e += WG)
-        varena[p.dst_v + e] = ws[p.src_v + e];
+    for (size_t e = threadIdx.x; e < nv; e += WG) {
+        const size_t r = e / p.N, j = e % p.N;
+        varena[p.dst_v + r * p.dst_ldv + j] = ws[p.src_v + r * p.src_ldv + j];
+    }
     for (int t = 0; t < p.npieces; t++) {
         const UTileDst d = utiles[p.piece_first + t];
         const int tot    = d.R * p.k;
         for (int e = threadIdx.x; e < tot; e += WG) {
-            const int i = e % d.R, r = e / d.R;
-            arena[d.off + e] = ws[p.src_u + (size_t)r * p.M + d.row0 + i];
+            const int i = e / p.k, r = e % p.k;
+            arena[d.off + (size_t)i * d.si + (size_t)r * d.sr] = ws[p.src_u + (size_t)r * p.M + d.row0 + i];
         }
     }
 }
@@ -328,142 +322,228 @@ __global__ void k_convert_f32(const double *src, float *dst, uint64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// Product phase 0: x (nc x mu column-major, original order) -> xc
-// (cluster order, RHS-interleaved, compute type), for RHS [mu_base, mu_base+MU).
+// Product phase 0: x (nc x mu column-major, original order) -> z[0, nc*MU)
+// (cluster order, RHS-interleaved, compute type) for RHS [mu_base, mu_base+MU).
 template <typename TC, int MU>
-__global__ void k_gather_x(const double *x, const int *perm_s, int nc, int mu_base, TC *xc) {
+__global__ void k_gather_x(const double *x, const int *perm_s, int nc, int mu_base, TC *z) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nc * MU)
         return;
     const int j = e / MU, s = e % MU;
-    xc[e] = (TC)x[perm_s[j] + (size_t)nc * (mu_base + s)];
+    z[e] = (TC)x[perm_s[j] + (size_t)nc * (mu_base + s)];
 }
 
-// Product phase 1: partial sums of V x per column chunk of each low-rank leaf;
-// the last chunk to finish adds the partials in chunk order into t.
+// 16-byte vector loads of the stored entries.
+template <typename TS>
+struct Vec;
+template <>
+struct Vec<float> {
+    static constexpr int W = 4;
+    __device__ static void load(const float *p, float (&a)[4]) {
+        const float4 v = *reinterpret_cast<const float4 *>(p);
+        a[0] = v.x, a[1] = v.y, a[2] = v.z, a[3] = v.w;
+    }
+};
+template <>
+struct Vec<double> {
+    static constexpr int W = 2;
+    __device__ static void load(const double *p, double (&a)[2]) {
+        const double2 v = *reinterpret_cast<const double2 *>(p);
+        a[0] = v.x, a[1] = v.y;
+    }
+};
+
+// Lanes per row for a row of nvec 16-byte vectors: a power of two between 8
+// (one 128-byte line per load) and 64, aiming at about four loads per lane.
+__device__ __forceinline__ int lanes_per_row(int nvec) {
+    int lpr = 8;
+    while (lpr < 64 && lpr * 4 < nvec)
+        lpr <<= 1;
+    return lpr;
+}
+
+// Row-major block times an LDS-staged vector segment, shared by both product
+// phases. Rows [0, nrows) with row stride ld, nvec 16-byte vectors of columns
+// per row. Each row is taken by a group of lanes of one wave (lanes_per_row),
+// so short rows still fill the wave; the group sums its products in TC,
+// reduces in FP64 with shuffles, and its first lane adds the row total into
+// s_acc[row*MU + s].
 template <typename TS, typename TC, int MU>
-__global__ void __launch_bounds__(WG) k_lr_vx(const VTile *tiles, const TS *varena, const TC *xc, double *part,
-                                              double *t, unsigned *counters) {
-    __shared__ int s_last;
-    const VTile d = tiles[blockIdx.x];
+__device__ __forceinline__ void rows_gemv(const TS *base, size_t ld, int nrows, int nvec, const TC *s_z,
+                                          double *s_acc) {
+    constexpr int W = Vec<TS>::W;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // part_off, part_base and t_off count k-vectors; one k-vector holds MU RHS.
-    double *out = d.nparts > 1 ? part + (size_t)d.part_off * MU : t + (size_t)d.t_off * MU;
-    for (int r = w; r < d.k; r += NWAVE) {
-        const TS *row = varena + d.v_off + (size_t)r * d.ld;
-        TC acc[MU];
+    const int lpr  = lanes_per_row(nvec);
+    const int rpw  = 64 / lpr;
+    const int sl = lane & (lpr - 1), sub = lane / lpr;
+    for (int rbase = w * rpw; rbase < nrows; rbase += NWAVE * rpw) {
+        const int r = rbase + sub;
+        TC part[MU];
 #pragma unroll
         for (int s = 0; s < MU; s++)
-            acc[s] = 0;
+            part[s] = 0;
+        if (r < nrows) {
+            const TS *row = base + (size_t)r * ld;
 #pragma unroll 4
-        for (int j = lane; j < d.ncols; j += 64) {
-            const TC a   = (TC)row[j];
-            const TC *xv = xc + (size_t)(d.x_off + j) * MU;
+            for (int q = sl; q < nvec; q += lpr) {
+                TS a[W];
+                Vec<TS>::load(row + (size_t)q * W, a);
+#pragma unroll
+                for (int u = 0; u < W; u++) {
+                    const TC *zz = s_z + (q * W + u) * MU;
+#pragma unroll
+                    for (int s = 0; s < MU; s++)
+                        part[s] += (TC)a[u] * zz[s];
+                }
+            }
+        }
+        double v[MU];
+#pragma unroll
+        for (int s = 0; s < MU; s++)
+            v[s] = (double)part[s];
+        for (int o = lpr >> 1; o > 0; o >>= 1) {
 #pragma unroll
             for (int s = 0; s < MU; s++)
-                acc[s] += a * xv[s];
+                v[s] += __shfl_xor(v[s], o, 64);
         }
+        if (sl == 0 && r < nrows) {
 #pragma unroll
-        for (int s = 0; s < MU; s++) {
-            const TC v = wave_sum(acc[s]);
-            if (lane == 0)
-                out[r * MU + s] = (double)v;
-        }
-    }
-    if (d.nparts > 1) {
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const unsigned prev = atomicAdd(&counters[d.leaf], 1u);
-            s_last              = (prev == (unsigned)(d.nparts - 1));
-        }
-        __syncthreads();
-        if (s_last) {
-            __threadfence();
-            const int km       = d.k * MU;
-            const double *pb   = part + (size_t)d.part_base * MU;
-            double *tout       = t + (size_t)d.t_off * MU;
-            for (int e = threadIdx.x; e < km; e += WG) {
-                double s = 0;
-                for (int c = 0; c < d.nparts; c++)
-                    s += pb[(size_t)c * km + e];
-                tout[e] = s;
-            }
-            if (threadIdx.x == 0)
-                counters[d.leaf] = 0;
+            for (int s = 0; s < MU; s++)
+                s_acc[r * MU + s] += v[s];
         }
     }
 }
 
-// Product phase 2: one workgroup per row block (<= 128 rows). Each piece is an
-// R x ncols column-major tile times a vector segment; lanes run over rows,
-// thread groups over columns. Per-tile sums are in the compute type, the sum
-// over tiles in FP64; the result is scattered to original order.
+constexpr int ZCAP   = 2048; // vector entries staged in LDS per pass (a multiple of 4 * MU)
+constexpr int ROWMAX = 128;  // rows of a row block / V rows per pass
+
+// Product phase 1: t = V x for every low-rank leaf. Each wavefront walks its
+// own list of V chunks (no workgroup barrier anywhere). V rows are stored
+// aligned to the global column index, so a lane reads 16 bytes of V and the
+// matching 16 bytes of x from the L2-resident z with the same vector index.
+// Rows are taken by lane groups (lanes_per_row), reduced in FP64 with
+// shuffles; the group's first lane writes the row's t entry. A leaf cut into
+// several chunks writes one partial sum per chunk; k_vx_sum adds them in
+// chunk order.
 template <typename TS, typename TC, int MU>
-__global__ void __launch_bounds__(WG) k_row_apply(const int *rb_row0, const int *piece_off, const Piece *pieces,
-                                                  const TS *arena, const TC *xc, const double *t,
-                                                  const int *perm_t, int nr, int mu_base, double *y) {
-    constexpr int SC = MU < 8 ? MU : 8; // RHS per LDS reduction pass
-    __shared__ double s_red[WG * SC];
-    const int rb = blockIdx.x;
-    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
-    const int pshift = R <= 32 ? 5 : (R <= 64 ? 6 : 7);
-    const int P = 1 << pshift, G = WG >> pshift;
-    const int i = threadIdx.x & (P - 1), g = threadIdx.x >> pshift;
-    double acc[MU];
+__global__ void __launch_bounds__(WG) k_vx(const int *wv_off, int nwaves, const VTile *tiles, const TS *varena,
+                                           TC *z, double *part) {
+    constexpr int W = Vec<TS>::W;
+    const int wv    = blockIdx.x * NWAVE + (threadIdx.x >> 6);
+    if (wv >= nwaves)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int tend = wv_off[wv + 1];
+    for (int ti = wv_off[wv]; ti < tend; ti++) {
+        const VTile d  = tiles[ti];
+        const int nvec = d.ncols / W;
+        const int lpr  = lanes_per_row(nvec);
+        const int rpw  = 64 / lpr;
+        const int sl = lane & (lpr - 1), sub = lane / lpr;
+        const TC *xv = z + (size_t)d.x_off * MU;
+        for (int rbase = 0; rbase < d.k; rbase += rpw) {
+            const int r = rbase + sub;
+            TC acc[MU];
 #pragma unroll
-    for (int s = 0; s < MU; s++)
-        acc[s] = 0;
-    const int pend = piece_off[rb + 1];
-    for (int pi = piece_off[rb]; pi < pend; pi++) {
-        const Piece pc = pieces[pi];
-        TC pacc[MU];
-#pragma unroll
-        for (int s = 0; s < MU; s++)
-            pacc[s] = 0;
-        if (i < R) {
-            const TS *A = arena + pc.a_off + i;
-            if (pc.kind == 0) {
-                const TC *xv = xc + (size_t)pc.vec_off * MU;
+            for (int s = 0; s < MU; s++)
+                acc[s] = 0;
+            if (r < d.k) {
+                const TS *row = varena + d.v_off + (size_t)r * d.ld;
 #pragma unroll 4
-                for (int j = g; j < pc.ncols; j += G) {
-                    const TC a = (TC)A[(size_t)R * j];
+                for (int q = sl; q < nvec; q += lpr) {
+                    TS a[W];
+                    Vec<TS>::load(row + (size_t)q * W, a);
+                    if constexpr (MU == 1) {
+                        TC xq[W];
+                        Vec<TC>::load(xv + (size_t)q * W, xq);
 #pragma unroll
-                    for (int s = 0; s < MU; s++)
-                        pacc[s] += a * xv[j * MU + s];
+                        for (int u = 0; u < W; u++)
+                            acc[0] += (TC)a[u] * xq[u];
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < W; u++) {
+                            const TC *xj = xv + (size_t)(q * W + u) * MU;
+#pragma unroll
+                            for (int s = 0; s < MU; s++)
+                                acc[s] += (TC)a[u] * xj[s];
+                        }
+                    }
                 }
-            } else {
-                const double *tv = t + (size_t)pc.vec_off * MU;
-#pragma unroll 4
-                for (int j = g; j < pc.ncols; j += G) {
-                    const TC a = (TC)A[(size_t)R * j];
+            }
+            double v[MU];
+#pragma unroll
+            for (int s = 0; s < MU; s++)
+                v[s] = (double)acc[s];
+            for (int o = lpr >> 1; o > 0; o >>= 1) {
+#pragma unroll
+                for (int s = 0; s < MU; s++)
+                    v[s] += __shfl_xor(v[s], o, 64);
+            }
+            if (sl == 0 && r < d.k) {
+                if (d.nparts > 1) {
+                    double *o = part + (size_t)(d.part_off + r) * MU;
 #pragma unroll
                     for (int s = 0; s < MU; s++)
-                        pacc[s] += a * (TC)tv[j * MU + s];
+                        o[s] = v[s];
+                } else {
+                    TC *o = z + (size_t)(d.t_index + r) * MU;
+#pragma unroll
+                    for (int s = 0; s < MU; s++)
+                        o[s] = (TC)v[s];
                 }
             }
         }
-#pragma unroll
-        for (int s = 0; s < MU; s++)
-            acc[s] += (double)pacc[s];
     }
-#pragma unroll
-    for (int s0 = 0; s0 < MU; s0 += SC) {
-#pragma unroll
-        for (int s = 0; s < SC; s++)
-            s_red[s * WG + threadIdx.x] = acc[s0 + s];
-        __syncthreads();
-        if (g == 0 && i < R) {
-            const size_t row = (size_t)perm_t[r0 + i];
-#pragma unroll
-            for (int s = 0; s < SC; s++) {
-                double v = 0;
-                for (int gg = 0; gg < G; gg++)
-                    v += s_red[s * WG + gg * P + i];
-                y[row + (size_t)nr * (mu_base + s0 + s)] = v;
-            }
+}
+
+// Adds the chunk partial sums of each multi-chunk leaf into its t entries.
+template <typename TC, int MU>
+__global__ void __launch_bounds__(WG) k_vx_sum(const VTile *tiles, const int *first_chunk, const double *part, TC *z) {
+    const VTile d    = tiles[first_chunk[blockIdx.x]];
+    const int km     = d.k * MU;
+    const double *pb = part + (size_t)d.part_base * MU;
+    TC *o            = z + (size_t)d.t_index * MU;
+    for (int e = threadIdx.x; e < km; e += WG) {
+        double v = 0;
+        for (int c = 0; c < d.nparts; c++)
+            v += pb[(size_t)c * km + e];
+        o[e] = (TC)v;
+    }
+}
+
+// Product phase 2: one workgroup per row block (<= 128 rows). Its dense-leaf
+// rows and U rows form one row-major R x C block (row stride C rounded up to 4,
+// zero padded); the vector entry of each column (an x entry for a dense
+// column, a t entry for a U column) is staged through LDS, and the row sums
+// are scattered to original order.
+template <typename TS, typename TC, int MU>
+__global__ void __launch_bounds__(WG) k_rows(const int *rb_order, const int *rb_row0, const int *rb_col0,
+                                             const uint64_t *rb_aoff, const int *colidx, const TS *arena,
+                                             const TC *z, const int *perm_t, int nr, int mu_base, double *y) {
+    constexpr int ZCOLS = ZCAP / MU;
+    constexpr int W     = Vec<TS>::W;
+    __shared__ TC s_z[ZCAP];
+    __shared__ double s_acc[ROWMAX * MU];
+    const int rb = rb_order[blockIdx.x];
+    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
+    const int c0 = rb_col0[rb], C = rb_col0[rb + 1] - c0;
+    const int Cp = (C + 3) & ~3;
+    const TS *A  = arena + rb_aoff[rb];
+    for (int e = threadIdx.x; e < R * MU; e += WG)
+        s_acc[e] = 0;
+    for (int cc = 0; cc < Cp; cc += ZCOLS) {
+        const int nz = min(ZCOLS, Cp - cc);
+        for (int e = threadIdx.x; e < nz * MU; e += WG) {
+            const int c = cc + e / MU;
+            s_z[e]      = c < C ? z[(size_t)colidx[c0 + c] * MU + e % MU] : (TC)0;
         }
         __syncthreads();
+        rows_gemv<TS, TC, MU>(A + cc, (size_t)Cp, R, nz / W, s_z, s_acc);
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < R * MU; e += WG) {
+        const int r = e / MU, s = e % MU;
+        y[(size_t)perm_t[r0 + r] + (size_t)nr * (mu_base + s)] = s_acc[e];
     }
 }
 
@@ -510,51 +590,55 @@ void launch_convert_f32(const double *src, float *dst, uint64_t n, hipStream_t s
     default: CALL(64); break;    \
     }
 
-template <typename TS, typename TC>
-void launch_gather_x(const double *x, const int *perm_s, int nc, int /*mu_total*/, int mu_base, int MU, TC *xc,
-                     hipStream_t st) {
+template <typename TC>
+void launch_gather_x(const double *x, const int *perm_s, int nc, int mu_base, int MU, TC *z, hipStream_t st) {
     const int n = nc * MU;
     const dim3 grid((n + WG - 1) / WG);
-#define HTG_GATHER(M) hipLaunchKernelGGL((k_gather_x<TC, M>), grid, dim3(WG), 0, st, x, perm_s, nc, mu_base, xc)
+#define HTG_GATHER(M) hipLaunchKernelGGL((k_gather_x<TC, M>), grid, dim3(WG), 0, st, x, perm_s, nc, mu_base, z)
     HTG_MU_SWITCH(MU, HTG_GATHER)
 #undef HTG_GATHER
 }
 
 template <typename TS, typename TC>
-void launch_lr_vx(const VTile *tiles, int ntiles, const TS *varena, const TC *xc, double *part, double *t,
-                  unsigned *counters, int MU, hipStream_t st) {
-    if (ntiles <= 0)
+void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *multi_first, int nmulti,
+               const TS *varena, TC *z, double *part, int MU, hipStream_t st) {
+    if (ngroups <= 0)
         return;
-#define HTG_VX(M) \
-    hipLaunchKernelGGL((k_lr_vx<TS, TC, M>), dim3(ntiles), dim3(WG), 0, st, tiles, varena, xc, part, t, counters)
+#define HTG_VX(M)                                                                                         \
+    do {                                                                                                  \
+        hipLaunchKernelGGL((k_vx<TS, TC, M>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off,    \
+                           ngroups, tiles, varena, z, part);                                              \
+        if (nmulti > 0)                                                                                   \
+            hipLaunchKernelGGL((k_vx_sum<TC, M>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first, part, z); \
+    } while (0)
     HTG_MU_SWITCH(MU, HTG_VX)
 #undef HTG_VX
 }
 
 template <typename TS, typename TC>
-void launch_row_apply(const int *rb_row0, const int *piece_off, const Piece *pieces, int nrb, const TS *arena,
-                      const TC *xc, const double *t, const int *perm_t, int nr, int mu_base, int MU, double *y,
-                      hipStream_t st) {
+void launch_rows(const int *rb_order, const int *rb_row0, const int *rb_col0, const uint64_t *rb_aoff, int nrb,
+                 const int *colidx, const TS *arena, const TC *z, const int *perm_t, int nr, int mu_base, int MU,
+                 double *y, hipStream_t st) {
     if (nrb <= 0)
         return;
-#define HTG_RA(M)                                                                                                  \
-    hipLaunchKernelGGL((k_row_apply<TS, TC, M>), dim3(nrb), dim3(WG), 0, st, rb_row0, piece_off, pieces, arena, xc, \
-                       t, perm_t, nr, mu_base, y)
-    HTG_MU_SWITCH(MU, HTG_RA)
-#undef HTG_RA
+#define HTG_ROWS(M)                                                                                              \
+    hipLaunchKernelGGL((k_rows<TS, TC, M>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, rb_col0, rb_aoff, colidx, \
+                       arena, z, perm_t, nr, mu_base, y)
+    HTG_MU_SWITCH(MU, HTG_ROWS)
+#undef HTG_ROWS
 }
 
-template void launch_gather_x<double, double>(const double *, const int *, int, int, int, int, double *, hipStream_t);
-template void launch_gather_x<float, float>(const double *, const int *, int, int, int, int, float *, hipStream_t);
-template void launch_lr_vx<double, double>(const VTile *, int, const double *, const double *, double *, double *,
-                                           unsigned *, int, hipStream_t);
-template void launch_lr_vx<float, float>(const VTile *, int, const float *, const float *, double *, double *,
-                                         unsigned *, int, hipStream_t);
-template void launch_row_apply<double, double>(const int *, const int *, const Piece *, int, const double *,
-                                               const double *, const double *, const int *, int, int, int,
-                                               double *, hipStream_t);
-template void launch_row_apply<float, float>(const int *, const int *, const Piece *, int, const float *,
-                                             const float *, const double *, const int *, int, int, int, double *,
-                                             hipStream_t);
+template void launch_gather_x<double>(const double *, const int *, int, int, int, double *, hipStream_t);
+template void launch_gather_x<float>(const double *, const int *, int, int, int, float *, hipStream_t);
+template void launch_vx<double, double>(const int *, int, const VTile *, const int *, int, const double *, double *,
+                                        double *, int, hipStream_t);
+template void launch_vx<float, float>(const int *, int, const VTile *, const int *, int, const float *, float *,
+                                      double *, int, hipStream_t);
+template void launch_rows<double, double>(const int *, const int *, const int *, const uint64_t *, int, const int *,
+                                          const double *, const double *, const int *, int, int, int, double *,
+                                          hipStream_t);
+template void launch_rows<float, float>(const int *, const int *, const int *, const uint64_t *, int, const int *,
+                                        const float *, const float *, const int *, int, int, int, double *,
+                                        hipStream_t);
 
 } // namespace htg

@@ -24,10 +24,10 @@ struct DevPoints {
 };
 
 // One dense-generation tile: rows [row0, row0+R) x cols [col0, col0+ncols) of the
-// matrix in cluster order, written column-major with leading dimension R.
+// matrix in cluster order, written row-major with row stride ld.
 struct FillTile {
     uint64_t off;
-    int row0, R, col0, ncols;
+    int row0, R, col0, ncols, ld;
 };
 
 // One block handed to the batched ACA kernel.
@@ -39,41 +39,33 @@ struct AcaBlock {
     double ctr[3];          // centre of the cluster whose points choose the first pivot
 };
 
-// Row-block piece of the product: a column-major R x ncols tile (R = rows of the
-// row block) multiplied by a vector segment: x (dense leaf) or t (low-rank leaf).
-struct Piece {
-    uint64_t a_off;
-    int ncols;
-    int vec_off; // dense: first cluster column; low-rank: first entry of t
-    int kind;    // 0 dense, 1 low-rank
-    int pad;
-};
-
-// Column chunk of a low-rank leaf's V (k x n, row-major, leading dimension ld):
-// computes k partial sums of V x over columns [x_off, x_off+ncols).
+// Column chunk of a low-rank leaf's V (k x n, row-major, row stride ld). Each
+// stored row covers the global columns [oj & ~3, round4(oj + n)), zero outside
+// the leaf, so stored column c multiplies z entry (oj & ~3) + c and both are
+// 16-byte aligned. A chunk is k sums over ncols (a multiple of 4) stored
+// columns starting at z entry x_off; chunks are grouped per wavefront.
 struct VTile {
-    uint64_t v_off;
-    int k, ld, ncols, x_off;
-    int part_off; // partial sums of this tile: part[part_off + r*MU + s]
-    int leaf;     // low-rank leaf id (completion counter index)
-    int nparts;   // number of column chunks of this leaf
-    int part_base;// partial sums of chunk 0 of this leaf
-    int t_off;    // t[t_off + r*MU + s]
-    int pad;
+    uint64_t v_off; // first entry of the chunk in the V arena
+    int k, ncols, x_off, ld;
+    int t_index;   // z position (in RHS vectors) of the leaf's first t entry
+    int part_off;  // partial sums of this chunk (in RHS vectors), when nparts > 1
+    int part_base; // partial sums of chunk 0 of this leaf
+    int nparts;    // number of column chunks of this leaf
 };
 
+// Copy of one compressed leaf: U (M x k column-major at src_u) into U tiles, V
+// (k rows of N entries, row strides src_ldv / dst_ldv) from src_v to dst_v.
 struct LrPack {
-    uint64_t src_u, src_v; // workspace offsets of the U columns (len M) and V rows (len N)
-    uint64_t dst_v;        // V arena offset (k x N row-major)
-    int M, N, k, oi;       // oi: first cluster row of the leaf
-    int piece_first;       // first U tile in the U-tile list
+    uint64_t src_u, src_v, dst_v;
+    int M, N, k, src_ldv, dst_ldv;
+    int piece_first; // first U tile in the U-tile list
     int npieces;
 };
 
+// U rows [row0, row0+R) of a leaf: entry (i, r) goes to off + i*si + r*sr.
 struct UTileDst {
-    uint64_t off; // destination in the tile arena
-    int row0;     // first row of the tile relative to the leaf
-    int R;
+    uint64_t off;
+    int row0, R, si, sr;
 };
 
 // ---- launchers (kernels.hip) -------------------------------------------------
@@ -86,16 +78,17 @@ void launch_lr_pack(const LrPack *packs, int npacks, const UTileDst *utiles, con
                     double *varena, hipStream_t st);
 void launch_convert_f32(const double *src, float *dst, uint64_t n, hipStream_t st);
 
-// Product phases. TS = storage type, TC = compute type inside a leaf tile.
+// Product phases. TS = storage type, TC = compute type inside a tile. z holds
+// the cluster-ordered x (nc RHS vectors) followed by t (one RHS vector per
+// low-rank term), both in TC.
+template <typename TC>
+void launch_gather_x(const double *x, const int *perm_s, int nc, int mu_base, int MU, TC *z, hipStream_t st);
 template <typename TS, typename TC>
-void launch_gather_x(const double *x, const int *perm_s, int nc, int mu_total, int mu_base, int MU, TC *xc,
-                     hipStream_t st);
+void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *multi_first, int nmulti,
+               const TS *varena, TC *z, double *part, int MU, hipStream_t st);
 template <typename TS, typename TC>
-void launch_lr_vx(const VTile *tiles, int ntiles, const TS *varena, const TC *xc, double *part, double *t,
-                  unsigned *counters, int MU, hipStream_t st);
-template <typename TS, typename TC>
-void launch_row_apply(const int *rb_row0, const int *piece_off, const Piece *pieces, int nrb, const TS *arena,
-                      const TC *xc, const double *t, const int *perm_t, int nr, int mu_base, int MU, double *y,
-                      hipStream_t st);
+void launch_rows(const int *rb_order, const int *rb_row0, const int *rb_col0, const uint64_t *rb_aoff, int nrb,
+                 const int *colidx, const TS *arena, const TC *z, const int *perm_t, int nr, int mu_base, int MU,
+                 double *y, hipStream_t st);
 
 } // namespace htg

@@ -85,7 +85,7 @@ typedef struct htg_info {
     double compression_ratio; /* nr*nc / stored entries, include/htool/types/hmatrix.hpp:1604-1632 */
     int64_t dense_entries;    /* sum of m*n over dense leaves */
     int64_t lowrank_entries;  /* sum of k*(m+n) over low-rank leaves */
-    int n_row_blocks, n_pieces, n_vtiles;
+    int n_row_blocks, n_columns, n_vtiles;
     double build_seconds;     /* block tree + assembly */
 } htg_info;
 

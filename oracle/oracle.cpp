@@ -434,9 +434,15 @@ struct Builder {
         const bool swap = sym_aca && (a.offset < b.offset);
         std::vector<int> rows(perm_t.begin() + a.offset, perm_t.begin() + a.offset + M);
         std::vector<int> cols(perm_s.begin() + b.offset, perm_s.begin() + b.offset + N);
-        int cap = std::min(M, N) + 1;
+        int cap = std::min(std::min(M, N) + 1, 64); // grown below when the rank needs more room
         std::vector<double> U(size_t(M) * cap), V(size_t(N) * cap);
         int k = aca(swap, eps, M, N, rows.data(), cols.data(), A, swap ? b.ctr : a.ctr, reqrank, cap, U.data(), V.data());
+        if (k == -2) {
+            cap = std::min(M, N) + 1;
+            U.assign(size_t(M) * cap, 0.0);
+            V.assign(size_t(N) * cap, 0.0);
+            k = aca(swap, eps, M, N, rows.data(), cols.data(), A, swap ? b.ctr : a.ctr, reqrank, cap, U.data(), V.data());
+        }
         if (k == -1)
             return false;
         Leaf L{a.offset, M, b.offset, N, k, {}, {}, {}};
