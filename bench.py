@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=10)
+    ap.add_argument("--counters-run", action="store_true",
+                    help="short run for rocprofv3 --pmc: a 256 MiB device copy of known size, then a few "
+                         "mixed-precision and FP64 products; prints nothing else")
     return ap.parse_args()
 
 
@@ -138,6 +141,20 @@ def main():
 
     rng = np.random.default_rng(1)
     x = rng.standard_normal(n)
+    if args.counters_run:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        dx = torch.from_numpy(x).cuda()
+        dy = torch.zeros(n, dtype=torch.float64, device="cuda")
+        for storage in ("fp32", "fp64"):
+            H.set_storage(storage)
+            for _ in range(3):
+                H.mvprod_device(dx.data_ptr(), dy.data_ptr(), 1, torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        log("counters run done")
+        return
     X = rng.standard_normal((n, 32))
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream

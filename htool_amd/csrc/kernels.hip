@@ -376,6 +376,45 @@ __device__ __forceinline__ void rows_gemv(const TS *base, size_t ld, int nrows, 
     const int lpr  = lanes_per_row(nvec);
     const int rpw  = 64 / lpr;
     const int sl = lane & (lpr - 1), sub = lane / lpr;
+    if constexpr (MU == 1) {
+        // two rows per lane group: one staged vector read serves both
+        for (int rbase = w * 2 * rpw; rbase < nrows; rbase += NWAVE * 2 * rpw) {
+            const int r0 = rbase + sub, r1 = r0 + rpw;
+            const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
+            const TS *row0 = base + (size_t)(ok0 ? r0 : 0) * ld;
+            const TS *row1 = base + (size_t)(ok1 ? r1 : 0) * ld;
+            TC acc0 = 0, acc1 = 0;
+            if (ok0) {
+#pragma unroll 2
+                for (int q = sl; q < nvec; q += lpr) {
+                    TS a[W], b[W];
+                    Vec<TS>::load(row0 + (size_t)q * W, a);
+                    if (ok1)
+                        Vec<TS>::load(row1 + (size_t)q * W, b);
+                    else
+#pragma unroll
+                        for (int u = 0; u < W; u++)
+                            b[u] = 0;
+#pragma unroll
+                    for (int u = 0; u < W; u++) {
+                        const TC zz = s_z[q * W + u];
+                        acc0 += (TC)a[u] * zz;
+                        acc1 += (TC)b[u] * zz;
+                    }
+                }
+            }
+            double v0 = (double)acc0, v1 = (double)acc1;
+            for (int o = lpr >> 1; o > 0; o >>= 1) {
+                v0 += __shfl_xor(v0, o, 64);
+                v1 += __shfl_xor(v1, o, 64);
+            }
+            if (sl == 0 && ok0)
+                s_acc[r0] += v0;
+            if (sl == 0 && ok1)
+                s_acc[r1] += v1;
+        }
+        return;
+    }
     for (int rbase = w * rpw; rbase < nrows; rbase += NWAVE * rpw) {
         const int r = rbase + sub;
         TC part[MU];
@@ -426,6 +465,21 @@ constexpr int ROWMAX = 128;  // rows of a row block / V rows per pass
 // several chunks writes one partial sum per chunk; k_vx_sum adds them in
 // chunk order.
 template <typename TS, typename TC, int MU>
+__device__ __forceinline__ void vx_store(const VTile &d, int r, const double (&v)[MU], TC *z, double *part) {
+    if (d.nparts > 1) {
+        double *o = part + (size_t)(d.part_off + r) * MU;
+#pragma unroll
+        for (int s = 0; s < MU; s++)
+            o[s] = v[s];
+    } else {
+        TC *o = z + (size_t)(d.t_index + r) * MU;
+#pragma unroll
+        for (int s = 0; s < MU; s++)
+            o[s] = (TC)v[s];
+    }
+}
+
+template <typename TS, typename TC, int MU>
 __global__ void __launch_bounds__(WG) k_vx(const int *wv_off, int nwaves, const VTile *tiles, const TS *varena,
                                            TC *z, double *part) {
     constexpr int W = Vec<TS>::W;
@@ -434,32 +488,66 @@ __global__ void __launch_bounds__(WG) k_vx(const int *wv_off, int nwaves, const 
         return;
     const int lane = threadIdx.x & 63;
     const int tend = wv_off[wv + 1];
-    for (int ti = wv_off[wv]; ti < tend; ti++) {
-        const VTile d  = tiles[ti];
-        const int nvec = d.ncols / W;
-        const int lpr  = lanes_per_row(nvec);
-        const int rpw  = 64 / lpr;
+    int ti         = wv_off[wv];
+    VTile d        = tiles[ti];
+    for (; ti < tend; ti++) {
+        const VTile next = tiles[ti + 1 < tend ? ti + 1 : ti]; // descriptor of the next chunk, loaded early
+        const int nvec   = d.ncols / W;
+        const int lpr    = lanes_per_row(nvec);
+        const int rpw    = 64 / lpr;
         const int sl = lane & (lpr - 1), sub = lane / lpr;
         const TC *xv = z + (size_t)d.x_off * MU;
-        for (int rbase = 0; rbase < d.k; rbase += rpw) {
-            const int r = rbase + sub;
-            TC acc[MU];
-#pragma unroll
-            for (int s = 0; s < MU; s++)
-                acc[s] = 0;
-            if (r < d.k) {
-                const TS *row = varena + d.v_off + (size_t)r * d.ld;
-#pragma unroll 4
-                for (int q = sl; q < nvec; q += lpr) {
-                    TS a[W];
-                    Vec<TS>::load(row + (size_t)q * W, a);
-                    if constexpr (MU == 1) {
+        if constexpr (MU == 1) {
+            // two rows per lane group: one x load serves both
+            for (int rbase = 0; rbase < d.k; rbase += 2 * rpw) {
+                const int r0 = rbase + sub, r1 = r0 + rpw;
+                const bool ok0 = r0 < d.k, ok1 = r1 < d.k;
+                const TS *row0 = varena + d.v_off + (size_t)(ok0 ? r0 : 0) * d.ld;
+                const TS *row1 = varena + d.v_off + (size_t)(ok1 ? r1 : 0) * d.ld;
+                TC acc0 = 0, acc1 = 0;
+                if (ok0) {
+#pragma unroll 2
+                    for (int q = sl; q < nvec; q += lpr) {
+                        TS a[W], b[W];
                         TC xq[W];
                         Vec<TC>::load(xv + (size_t)q * W, xq);
+                        Vec<TS>::load(row0 + (size_t)q * W, a);
+                        if (ok1)
+                            Vec<TS>::load(row1 + (size_t)q * W, b);
+                        else
 #pragma unroll
-                        for (int u = 0; u < W; u++)
-                            acc[0] += (TC)a[u] * xq[u];
-                    } else {
+                            for (int u = 0; u < W; u++)
+                                b[u] = 0;
+#pragma unroll
+                        for (int u = 0; u < W; u++) {
+                            acc0 += (TC)a[u] * xq[u];
+                            acc1 += (TC)b[u] * xq[u];
+                        }
+                    }
+                }
+                double v0[1] = {(double)acc0}, v1[1] = {(double)acc1};
+                for (int o = lpr >> 1; o > 0; o >>= 1) {
+                    v0[0] += __shfl_xor(v0[0], o, 64);
+                    v1[0] += __shfl_xor(v1[0], o, 64);
+                }
+                if (sl == 0 && ok0)
+                    vx_store<TS, TC, 1>(d, r0, v0, z, part);
+                if (sl == 0 && ok1)
+                    vx_store<TS, TC, 1>(d, r1, v1, z, part);
+            }
+        } else {
+            for (int rbase = 0; rbase < d.k; rbase += rpw) {
+                const int r = rbase + sub;
+                TC acc[MU];
+#pragma unroll
+                for (int s = 0; s < MU; s++)
+                    acc[s] = 0;
+                if (r < d.k) {
+                    const TS *row = varena + d.v_off + (size_t)r * d.ld;
+#pragma unroll 2
+                    for (int q = sl; q < nvec; q += lpr) {
+                        TS a[W];
+                        Vec<TS>::load(row + (size_t)q * W, a);
 #pragma unroll
                         for (int u = 0; u < W; u++) {
                             const TC *xj = xv + (size_t)(q * W + u) * MU;
@@ -469,30 +557,20 @@ __global__ void __launch_bounds__(WG) k_vx(const int *wv_off, int nwaves, const 
                         }
                     }
                 }
-            }
-            double v[MU];
-#pragma unroll
-            for (int s = 0; s < MU; s++)
-                v[s] = (double)acc[s];
-            for (int o = lpr >> 1; o > 0; o >>= 1) {
+                double v[MU];
 #pragma unroll
                 for (int s = 0; s < MU; s++)
-                    v[s] += __shfl_xor(v[s], o, 64);
-            }
-            if (sl == 0 && r < d.k) {
-                if (d.nparts > 1) {
-                    double *o = part + (size_t)(d.part_off + r) * MU;
+                    v[s] = (double)acc[s];
+                for (int o = lpr >> 1; o > 0; o >>= 1) {
 #pragma unroll
                     for (int s = 0; s < MU; s++)
-                        o[s] = v[s];
-                } else {
-                    TC *o = z + (size_t)(d.t_index + r) * MU;
-#pragma unroll
-                    for (int s = 0; s < MU; s++)
-                        o[s] = (TC)v[s];
+                        v[s] += __shfl_xor(v[s], o, 64);
                 }
+                if (sl == 0 && r < d.k)
+                    vx_store<TS, TC, MU>(d, r, v, z, part);
             }
         }
+        d = next;
     }
 }
 
