@@ -27,6 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+RECOMPRESS_TOL = 2e-6  # per-leaf relative Frobenius tail dropped by the FP64 recompression
+LOWPREC_TOL = 5e-7     # per-leaf relative error budget of the FP16-stored trailing terms
 
 
 def log(msg):
@@ -135,6 +137,16 @@ def main():
     H.build(Generator(geometry.KERNEL_LAPLACE, 0.0, diag), pts)
     build_s = time.time() - t0
     log(f"built in {build_s:.2f}s")
+    # the mixed-precision H-matrix: same tree and ACA, then FP64 recompression
+    # and FP16 storage of the trailing terms
+    Hm = HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32)
+    Hm.set_compression("partialACA")
+    Hm.set_recompression(RECOMPRESS_TOL)
+    Hm.set_lowprec(LOWPREC_TOL)
+    t0 = time.time()
+    Hm.build(Generator(geometry.KERNEL_LAPLACE, 0.0, diag), pts)
+    build_m_s = time.time() - t0
+    log(f"mixed H-matrix built in {build_m_s:.2f}s")
     info = H.get_info()
     leaves = H.get_leaves()
     dense_e, u_e, v_e = leaf_bytes(leaves)
@@ -148,10 +160,10 @@ def main():
         torch.cuda.synchronize()
         dx = torch.from_numpy(x).cuda()
         dy = torch.zeros(n, dtype=torch.float64, device="cuda")
-        for storage in ("fp32", "fp64"):
-            H.set_storage(storage)
+        for M, storage in ((Hm, "fp32_fp16"), (H, "fp64")):
+            M.set_storage(storage)
             for _ in range(3):
-                H.mvprod_device(dx.data_ptr(), dy.data_ptr(), 1, torch.cuda.current_stream().cuda_stream)
+                M.mvprod_device(dx.data_ptr(), dy.data_ptr(), 1, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         log("counters run done")
         return
@@ -165,31 +177,33 @@ def main():
 
     res = {}
     ys = {}
-    for storage in ("fp64", "fp32"):
-        H.set_storage(storage)
-        ms1 = timed(lambda: H.mvprod_device(dx.data_ptr(), dy.data_ptr(), 1, sp), args.steps, args.warmup, stream, torch)
-        ys[storage] = dy.cpu().numpy().copy()
-        ms32 = timed(lambda: H.mvprod_device(dX.data_ptr(), dY.data_ptr(), 32, sp), max(5, args.steps // 5),
+    for name, M, storage in (("fp64", H, "fp64"), ("fp32", H, "fp32"), ("mixed", Hm, "fp32_fp16")):
+        M.set_storage(storage)
+        ms1 = timed(lambda: M.mvprod_device(dx.data_ptr(), dy.data_ptr(), 1, sp), args.steps, args.warmup, stream, torch)
+        ys[name] = dy.cpu().numpy().copy()
+        ms32 = timed(lambda: M.mvprod_device(dX.data_ptr(), dY.data_ptr(), 32, sp), max(5, args.steps // 5),
                      2, stream, torch)
-        prof = H.profile(dx.data_ptr(), dy.data_ptr(), 1, 20, sp)
-        log(f"{storage}: {ms1:.4f} ms/matvec, {ms32:.4f} ms/32-RHS product")
-        res[storage] = {"ms": ms1, "ms_mu32": ms32, "bytes": H.algorithmic_bytes(1, storage),
-                        "bytes_mu32": H.algorithmic_bytes(32, storage), "flops_mu32": H.flops(32), "prof": prof}
+        prof = M.profile(dx.data_ptr(), dy.data_ptr(), 1, 20, sp)
+        log(f"{name}: {ms1:.4f} ms/matvec, {ms32:.4f} ms/32-RHS product")
+        res[name] = {"ms": ms1, "ms_mu32": ms32, "bytes": M.algorithmic_bytes(1, storage),
+                     "bytes_mu32": M.algorithmic_bytes(32, storage), "flops_mu32": M.flops(32), "prof": prof}
 
     # accuracy: against dense FP64 rows and against the FP64 H product
     from oracle import lib as orc
     rows = np.random.default_rng(2).choice(n, 2000, replace=False).astype(np.int32)
     yd = orc.dense_rows(pts, None, rows, x, kernel=0, diag=diag, same=True)
     err = {s: float(np.linalg.norm(ys[s][rows] - yd) / np.linalg.norm(yd)) for s in ys}
-    err_mixed_vs_fp64 = float(np.linalg.norm(ys["fp32"] - ys["fp64"]) / np.linalg.norm(ys["fp64"]))
+    err_vs_fp64 = {s: float(np.linalg.norm(ys[s] - ys["fp64"]) / np.linalg.norm(ys["fp64"])) for s in ys}
 
-    mixed = res["fp32"]
+    mixed = res["mixed"]
     value = mixed["bytes"] / (mixed["ms"] * 1e-3) / 1e9
     # dominant kernel of the headline step and its own achieved bandwidth
     p = mixed["prof"]
-    s = 4
-    kern = {"row_apply": (p["row_apply_ms"], s * (dense_e + u_e) + 12 * n),
-            "lr_vx": (p["vx_ms"], s * v_e + 4 * n),
+    mi = Hm.get_info()
+    u_hi = mi.lowrank_u_entries - mi.lowprec_u_entries
+    v_hi = (mi.lowrank_entries - mi.lowrank_u_entries) - (mi.lowprec_entries - mi.lowprec_u_entries)
+    kern = {"row_apply": (p["row_apply_ms"], 4 * (mi.dense_entries + u_hi) + 2 * mi.lowprec_u_entries + 12 * n),
+            "lr_vx": (p["vx_ms"], 4 * v_hi + 2 * (mi.lowprec_entries - mi.lowprec_u_entries) + 4 * n),
             "gather_x": (p["gather_ms"], 8 * n + 4 * n + 4 * n)}
     dom = max(kern, key=lambda k: kern[k][0])
     achieved = kern[dom][1] / (kern[dom][0] * 1e-3) / 1e9
@@ -205,6 +219,13 @@ def main():
         log("CPU baseline")
         cpu = cpu_baseline(pts, diag, x, dense_e + u_e + v_e, args)
 
+    def entry(name):
+        r = res[name]
+        return {"ms": round(r["ms"], 4), "gbps": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 2),
+                "bytes": r["bytes"], "ms_mu32": round(r["ms_mu32"], 4),
+                "tflops_mu32": round(r["flops_mu32"] / (r["ms_mu32"] * 1e-3) / 1e12, 3),
+                "rel_err_vs_dense": err[name], "rel_err_vs_fp64_hmat": err_vs_fp64[name]}
+
     line = {
         "metric": "hmatvec_hbm_gbps",
         "value": round(value, 2),
@@ -219,20 +240,20 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "hmatvec_laplace_sphere", "N": n, "eta": 10.0, "epsilon": 1e-6, "leaf_size": 32,
-                   "compressor": "partialACA", "mu": 1, "storage": "fp32 leaves, fp64 accumulation"},
-        "fp64": {"ms": round(res["fp64"]["ms"], 4),
-                 "gbps": round(res["fp64"]["bytes"] / (res["fp64"]["ms"] * 1e-3) / 1e9, 2),
-                 "ms_mu32": round(res["fp64"]["ms_mu32"], 4),
-                 "tflops_mu32": round(res["fp64"]["flops_mu32"] / (res["fp64"]["ms_mu32"] * 1e-3) / 1e12, 3),
-                 "rel_err_vs_dense": err["fp64"]},
-        "mixed": {"ms": round(mixed["ms"], 4), "gbps": round(value, 2), "ms_mu32": round(mixed["ms_mu32"], 4),
-                  "tflops_mu32": round(mixed["flops_mu32"] / (mixed["ms_mu32"] * 1e-3) / 1e12, 3),
-                  "rel_err_vs_dense": err["fp32"], "rel_err_vs_fp64_hmat": err_mixed_vs_fp64},
+                   "compressor": "partialACA", "mu": 1,
+                   "storage": f"mixed: FP64 recompression (tol {RECOMPRESS_TOL}), FP32 leaves and leading terms, "
+                              f"FP16 trailing terms (budget {LOWPREC_TOL}), FP32 tile arithmetic, FP64 accumulation"},
+        "fp64": entry("fp64"),
+        "fp32": entry("fp32"),
+        "mixed": entry("mixed"),
         "speedup_mixed_vs_fp64": round(res["fp64"]["ms"] / mixed["ms"], 3),
         "build_s": round(build_s, 3),
+        "build_mixed_s": round(build_m_s, 3),
         "leaves": {"dense": info.n_dense, "lowrank": info.n_lowrank, "false_positive": info.n_false_positive,
                    "rank_mean": round(info.rank_mean, 3), "rank_max": info.rank_max,
-                   "dense_entries": dense_e, "lowrank_entries": u_e + v_e},
+                   "dense_entries": dense_e, "lowrank_entries": u_e + v_e,
+                   "mixed_rank_mean": round(mi.rank_mean, 3), "mixed_lowrank_entries": mi.lowrank_entries,
+                   "mixed_fp16_entries": mi.lowprec_entries},
         "kernels_ms": {k: round(v, 5) for k, v in p.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

@@ -23,6 +23,8 @@ HTG_ERR_RANK_CAP = 5
 
 STORAGE_FP64 = 0
 STORAGE_FP32 = 1
+STORAGE_FP32_FP16 = 2
+STORAGE_FP32_BF16 = 3
 
 PARTIAL_ACA = 0
 SYM_PARTIAL_ACA = 1
@@ -34,14 +36,17 @@ class htg_generator(C.Structure):
 
 class htg_params(C.Structure):
     _fields_ = [("epsilon", C.c_double), ("eta", C.c_double), ("maxblocksize", C.c_double),
-                ("minclustersize", C.c_int), ("compressor", C.c_int), ("reqrank", C.c_int), ("reserved", C.c_int)]
+                ("minclustersize", C.c_int), ("compressor", C.c_int), ("reqrank", C.c_int),
+                ("lowprec_all", C.c_int), ("recompress_tol", C.c_double), ("lowprec_tol", C.c_double)]
 
 
 class htg_info(C.Structure):
     _fields_ = [("nr", C.c_int), ("nc", C.c_int), ("n_dense", C.c_int), ("n_lowrank", C.c_int),
                 ("n_false_positive", C.c_int), ("rank_min", C.c_int), ("rank_max", C.c_int),
                 ("rank_mean", C.c_double), ("compression_ratio", C.c_double), ("dense_entries", C.c_int64),
-                ("lowrank_entries", C.c_int64), ("n_row_blocks", C.c_int), ("n_columns", C.c_int),
+                ("lowrank_entries", C.c_int64), ("lowprec_entries", C.c_int64), ("lowrank_u_entries", C.c_int64),
+                ("lowprec_u_entries", C.c_int64), ("n_row_blocks", C.c_int),
+                ("n_columns", C.c_int),
                 ("n_vtiles", C.c_int), ("build_seconds", C.c_double)]
 
 

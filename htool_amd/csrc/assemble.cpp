@@ -1,0 +1,394 @@
+// H-matrix assembly on the GPU (see assemble.hpp).
+//
+// Follows HMatrix::build / ComputeBlocks / ComputeAdmissibleBlock (reference:
+// include/htool/types/hmatrix.hpp:285-341, 422-607): the block tasks come from
+// the host block tree; admissible tasks are compressed on the GPU in batches
+// ("waves"), and a block whose ACA reports rank -1 is split into its sons
+// exactly as the reference's recursion does, the sons going into the next wave.
+// The merge of all-dense sons into one dense block is decided afterwards,
+// bottom up, with the reference's rule (hmatrix.hpp:536-546).
+#include "assemble.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace htg {
+
+constexpr int CAP_FIRST = 64;                       // ACA terms reserved per block in the first attempt
+constexpr int CAP_MAX   = 512;                      // must match the ACA kernel's CAPMAX
+constexpr size_t WS_MAX = size_t(1) << 30;          // doubles of ACA workspace per sub-batch (8 GiB)
+
+void PointsOnDevice::upload(const double *pts, const std::vector<int> &perm_h) {
+    const size_t n = perm_h.size();
+    std::vector<double> a(n), b(n), c(n);
+    for (size_t i = 0; i < n; i++) {
+        a[i] = pts[3 * (size_t)perm_h[i] + 0];
+        b[i] = pts[3 * (size_t)perm_h[i] + 1];
+        c[i] = pts[3 * (size_t)perm_h[i] + 2];
+    }
+    x.upload(a);
+    y.upload(b);
+    z.upload(c);
+    perm.upload(perm_h);
+}
+
+KernelDesc kernel_desc(const htg_generator &g, bool same) {
+    if (g.kernel < 0 || g.kernel > 3)
+        throw Error(HTG_ERR_UNSUPPORTED, "unknown kernel family " + std::to_string(g.kernel));
+    return KernelDesc{g.kernel, same ? 1 : 0, g.delta, g.diag};
+}
+
+namespace {
+
+// Nodes of the false-positive recursion (hmatrix.hpp:503-607).
+struct FpNode {
+    int t, s;
+    int state; // 0 pending ACA, 1 low rank, 2 split into kids, 3 pair of leaf clusters
+    int lr = -1;
+    int kid0 = -1, nkids = 0;
+};
+
+struct Assembler {
+    const ClusterTree &T, &S;
+    const htg_params &prm;
+    const KernelDesc kd;
+    DevPoints dT, dS;
+    std::vector<FpNode> nodes;
+    std::vector<int> pending;
+    std::vector<Leaf> out;
+    int false_positive = 0;
+    // compressed store: U (M x k col-major) then V (k x N row-major) per low-rank leaf
+    std::vector<DevArray<double> *> store_chunks;
+    DevArray<double> store;
+    uint64_t store_used = 0;
+
+    void expand(int id) {
+        const ClusterNode &a = T.nodes[nodes[id].t], &b = S.nodes[nodes[id].s];
+        if (a.leaf() && b.leaf()) {
+            nodes[id].state = 3;
+            return;
+        }
+        const bool split_t       = b.leaf() ? true : (a.leaf() ? false : a.size > b.size);
+        const ClusterNode &owner = split_t ? a : b;
+        const int kid0           = (int)nodes.size();
+        for (int p = 0; p < owner.nb_sons; p++) {
+            FpNode c{};
+            c.t = split_t ? owner.first_son + p : nodes[id].t;
+            c.s = split_t ? nodes[id].s : owner.first_son + p;
+            nodes.push_back(c);
+        }
+        nodes[id].state = 2;
+        nodes[id].kid0  = kid0;
+        nodes[id].nkids = owner.nb_sons;
+        for (int p = 0; p < owner.nb_sons; p++) {
+            const int c = kid0 + p;
+            if (admissible(T.nodes[nodes[c].t], S.nodes[nodes[c].s], prm.eta)) {
+                nodes[c].state = 0;
+                pending.push_back(c);
+            } else {
+                expand(c);
+            }
+        }
+    }
+
+    void dense(int t, int s) {
+        const ClusterNode &a = T.nodes[t], &b = S.nodes[s];
+        out.push_back(Leaf{a.offset, a.size, b.offset, b.size, -1});
+    }
+
+    // true = not pushed (may merge into its parent's dense block)
+    bool resolve(int id) {
+        const FpNode nd = nodes[id];
+        if (nd.state == 1)
+            return false;
+        if (nd.state == 3)
+            return true;
+        bool all = true;
+        std::vector<char> np(nd.nkids);
+        for (int p = 0; p < nd.nkids; p++) {
+            np[p] = resolve(nd.kid0 + p);
+            all   = all && np[p];
+        }
+        const ClusterNode &a = T.nodes[nd.t], &b = S.nodes[nd.s];
+        const double bsize   = double(a.size) * double(b.size);
+        if (bsize <= prm.maxblocksize && all)
+            return true;
+        for (int p = 0; p < nd.nkids; p++)
+            if (np[p])
+                dense(nodes[nd.kid0 + p].t, nodes[nd.kid0 + p].s);
+        return false;
+    }
+
+    // Runs ACA on every pending node; returns the low-rank leaves found.
+    void run_wave(std::vector<int> wave, std::vector<Leaf> &lr_out, std::vector<int> &lr_node) {
+        std::vector<int> caps(wave.size());
+        for (size_t w = 0; w < wave.size(); w++) {
+            const ClusterNode &a = T.nodes[nodes[wave[w]].t], &b = S.nodes[nodes[wave[w]].s];
+            const long long M = a.size, N = b.size;
+            long long kmax    = (M * N) / (M + N);
+            if (prm.reqrank > 0)
+                kmax = std::min<long long>(kmax, std::min<long long>(prm.reqrank, std::min(M, N)));
+            caps[w] = (int)std::max<long long>(1, std::min<long long>(kmax, CAP_FIRST));
+        }
+        std::vector<int> todo(wave.size());
+        for (size_t w = 0; w < wave.size(); w++)
+            todo[w] = (int)w;
+        while (!todo.empty()) {
+            // sub-batch bounded by workspace size
+            size_t b0 = 0;
+            std::vector<int> retry;
+            while (b0 < todo.size()) {
+                std::vector<AcaBlock> blocks;
+                std::vector<uint64_t> vis_off;
+                uint64_t ws_used = 0, vis_used = 0;
+                size_t b1        = b0;
+                while (b1 < todo.size()) {
+                    const int w          = todo[b1];
+                    const FpNode &nd     = nodes[wave[w]];
+                    const ClusterNode &a = T.nodes[nd.t], &b = S.nodes[nd.s];
+                    const uint64_t need  = (uint64_t)(a.size + b.size) * caps[w];
+                    if (!blocks.empty() && ws_used + need > WS_MAX)
+                        break;
+                    AcaBlock B{};
+                    B.off_t = a.offset;
+                    B.M     = a.size;
+                    B.off_s = b.offset;
+                    B.N     = b.size;
+                    B.swap  = (prm.compressor == HTG_SYM_PARTIAL_ACA && a.offset < b.offset) ? 1 : 0;
+                    B.cap   = caps[w];
+                    const int n1 = B.swap ? B.N : B.M;
+                    B.ws1 = ws_used;
+                    B.ws2 = ws_used + (uint64_t)n1 * B.cap;
+                    ws_used += need;
+                    const double *c = B.swap ? b.ctr : a.ctr;
+                    B.ctr[0] = c[0], B.ctr[1] = c[1], B.ctr[2] = c[2];
+                    vis_off.push_back(vis_used);
+                    vis_used += (uint64_t)(a.size + b.size);
+                    blocks.push_back(B);
+                    b1++;
+                }
+                DevArray<AcaBlock> d_blocks;
+                DevArray<uint64_t> d_vis_off;
+                DevArray<double> d_ws;
+                DevArray<unsigned char> d_vis;
+                DevArray<int> d_ranks;
+                d_blocks.upload(blocks);
+                d_vis_off.upload(vis_off);
+                d_ws.alloc(ws_used);
+                d_vis.alloc(vis_used);
+                d_ranks.alloc(blocks.size());
+                launch_aca(kd, dT, dS, d_blocks.p, (int)blocks.size(), prm.epsilon, prm.reqrank, d_ws.p, d_vis.p,
+                           d_vis_off.p, d_ranks.p, nullptr);
+                HTG_HIP(hipGetLastError());
+                std::vector<int> ranks(blocks.size());
+                HTG_HIP(hipMemcpy(ranks.data(), d_ranks.p, ranks.size() * sizeof(int), hipMemcpyDeviceToHost));
+
+                // Move the accepted factors into the compressed store.
+                std::vector<LrPack> packs;
+                std::vector<UTileDst> udst;
+                uint64_t add = 0;
+                for (size_t i = 0; i < blocks.size(); i++)
+                    if (ranks[i] > 0)
+                        add += (uint64_t)ranks[i] * (blocks[i].M + blocks[i].N);
+                grow_store(store_used + add);
+                for (size_t i = 0; i < blocks.size(); i++) {
+                    const int w      = todo[b0 + i];
+                    const AcaBlock &B = blocks[i];
+                    const int k      = ranks[i];
+                    if (k == -3) {
+                        if (caps[w] >= CAP_MAX)
+                            throw Error(HTG_ERR_RANK_CAP, "ACA needed more than " + std::to_string(CAP_MAX) +
+                                                              " terms in a " + std::to_string(B.M) + "x" +
+                                                              std::to_string(B.N) + " block");
+                        const long long kmax = ((long long)B.M * B.N) / (B.M + B.N);
+                        caps[w]              = (int)std::min<long long>(kmax, CAP_MAX);
+                        retry.push_back(w);
+                        continue;
+                    }
+                    if (k == 0) { // fixed rank 0: a zero block with no factors (lrmat.hpp:37-44)
+                        lr_out.push_back(Leaf{B.off_t, B.M, B.off_s, B.N, 0});
+                        lr_node.push_back(wave[w]);
+                    } else if (k > 0) {
+                        LrPack p{};
+                        p.src_u = B.swap ? B.ws2 : B.ws1;
+                        p.src_v = B.swap ? B.ws1 : B.ws2;
+                        p.M = B.M, p.N = B.N, p.k = k;
+                        p.src_ldv = B.N;
+                        p.dst_ldv = B.N;
+                        Leaf L{B.off_t, B.M, B.off_s, B.N, k};
+                        L.src_u = store_used;
+                        L.src_v = store_used + (uint64_t)B.M * k;
+                        p.dst_v = L.src_v;
+                        p.piece_first = (int)udst.size();
+                        p.npieces     = 1;
+                        udst.push_back(UTileDst{L.src_u, 0, B.M, 1, B.M});
+                        store_used += (uint64_t)k * (B.M + B.N);
+                        packs.push_back(p);
+                        lr_out.push_back(L);
+                        lr_node.push_back(wave[w]);
+                    } else {
+                        lr_node.push_back(-1 - wave[w]); // false positive marker
+                        lr_out.push_back(Leaf{B.off_t, B.M, B.off_s, B.N, k});
+                    }
+                }
+                if (!packs.empty()) {
+                    DevArray<LrPack> d_packs;
+                    DevArray<UTileDst> d_udst;
+                    d_packs.upload(packs);
+                    d_udst.upload(udst);
+                    launch_lr_pack(d_packs.p, (int)packs.size(), d_udst.p, d_ws.p, store.p, store.p, nullptr);
+                    HTG_HIP(hipGetLastError());
+                    HTG_HIP(hipDeviceSynchronize());
+                }
+                b0 = b1;
+            }
+            todo.swap(retry);
+        }
+    }
+
+    void grow_store(uint64_t need) {
+        if (need <= store.n)
+            return;
+        uint64_t cap = std::max<uint64_t>(need, store.n + store.n / 2 + (1u << 20));
+        double *np   = nullptr;
+        HTG_HIP(hipMalloc(&np, cap * sizeof(double)));
+        if (store.p && store_used)
+            HTG_HIP(hipMemcpy(np, store.p, store_used * sizeof(double), hipMemcpyDeviceToDevice));
+        store.reset();
+        store.p = np;
+        store.n = cap;
+    }
+};
+
+} // namespace
+
+void assemble(const htg_params &prm, const htg_generator &gen, int nt, const double *xt, int ns, const double *xs,
+              Assembly &res) {
+    const bool same = gen.same_points || xs == nullptr;
+    ClusterTree T, Sown;
+    build_cluster_tree(nt, xt, prm.minclustersize, T);
+    if (!same)
+        build_cluster_tree(ns, xs, prm.minclustersize, Sown);
+    const ClusterTree &S = same ? T : Sown;
+    res.nr     = nt;
+    res.nc     = same ? nt : ns;
+    res.perm_t = T.perm;
+    res.perm_s = S.perm;
+    res.same   = same;
+    res.kd     = kernel_desc(gen, same);
+    res.pT.upload(xt, T.perm);
+    if (!same)
+        res.pS.upload(xs, S.perm);
+
+    Assembler as{T, S, prm, res.kd, res.dT(), res.dS(), {}, {}, {}, 0, {}, {}, 0};
+    const std::vector<BlockTask> tasks = build_block_tasks(T, S, prm.eta, (size_t)prm.maxblocksize);
+    std::vector<int> roots(tasks.size(), -1);
+    for (size_t p = 0; p < tasks.size(); p++) {
+        if (!tasks[p].admissible) {
+            as.dense(tasks[p].t, tasks[p].s);
+        } else {
+            FpNode nd{};
+            nd.t     = tasks[p].t;
+            nd.s     = tasks[p].s;
+            nd.state = 0;
+            roots[p] = (int)as.nodes.size();
+            as.nodes.push_back(nd);
+            as.pending.push_back(roots[p]);
+        }
+    }
+    std::vector<Leaf> lr_leaves;
+    while (!as.pending.empty()) {
+        std::vector<int> wave;
+        wave.swap(as.pending);
+        std::vector<Leaf> got;
+        std::vector<int> got_node;
+        as.run_wave(wave, got, got_node);
+        for (size_t i = 0; i < got.size(); i++) {
+            if (got_node[i] >= 0) {
+                as.nodes[got_node[i]].state = 1;
+                lr_leaves.push_back(got[i]);
+            } else {
+                as.false_positive++;
+                as.expand(-1 - got_node[i]);
+            }
+        }
+    }
+    for (size_t p = 0; p < tasks.size(); p++)
+        if (roots[p] >= 0 && as.resolve(roots[p]))
+            as.dense(tasks[p].t, tasks[p].s);
+
+    for (Leaf &L : lr_leaves)
+        L.k_hi = L.k;
+    if (prm.recompress_tol > 0 || prm.lowprec_all)
+        recompress_and_split(prm, lr_leaves, as.store);
+    res.leaves = as.out;
+    res.leaves.insert(res.leaves.end(), lr_leaves.begin(), lr_leaves.end());
+    res.false_positive = as.false_positive;
+    res.store.take(as.store);
+}
+
+void recompress_and_split(const htg_params &prm, std::vector<Leaf> &leaves, DevArray<double> &store) {
+    if (prm.recompress_tol > 0) {
+        std::vector<RcLeaf> rc;
+        std::vector<int> which;
+        for (size_t l = 0; l < leaves.size(); l++) {
+            const Leaf &L = leaves[l];
+            if (L.k >= 2 && L.k <= RC_KMAX) {
+                rc.push_back(RcLeaf{L.src_u, L.src_v, L.m, L.n, L.k, 0});
+                which.push_back((int)l);
+            }
+        }
+        if (!rc.empty()) {
+            DevArray<RcLeaf> d_rc;
+            DevArray<int> d_rank, d_ok;
+            DevArray<double> d_sig;
+            d_rc.upload(rc);
+            d_rank.alloc(rc.size());
+            d_ok.alloc(rc.size());
+            d_sig.alloc(rc.size() * RC_KMAX);
+            launch_recompress(d_rc.p, (int)rc.size(), store.p, prm.recompress_tol, d_rank.p, d_ok.p, d_sig.p, nullptr);
+            HTG_HIP(hipGetLastError());
+            std::vector<int> rank(rc.size()), ok(rc.size());
+            std::vector<double> sig(rc.size() * RC_KMAX);
+            HTG_HIP(hipMemcpy(rank.data(), d_rank.p, rank.size() * sizeof(int), hipMemcpyDeviceToHost));
+            HTG_HIP(hipMemcpy(ok.data(), d_ok.p, ok.size() * sizeof(int), hipMemcpyDeviceToHost));
+            HTG_HIP(hipMemcpy(sig.data(), d_sig.p, sig.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < rc.size(); i++) {
+                if (!ok[i])
+                    continue;
+                Leaf &L = leaves[which[i]];
+                // V now holds r rows of N entries at src_v with row stride N: unchanged layout.
+                L.k = rank[i];
+                L.sig.assign(sig.begin() + i * RC_KMAX, sig.begin() + i * RC_KMAX + rank[i]);
+            }
+        }
+    }
+    // Precision classes. Terms are ordered by decreasing scale; the trailing
+    // terms go to class 1 (16-bit storage) while the error they add,
+    // 2 * 2^-11 * sigma_j summed, stays within lowprec_tol * ||sigma||_2.
+    const double u16 = std::ldexp(1.0, -11);
+    for (Leaf &L : leaves) {
+        if (L.k <= 0)
+            continue;
+        if (prm.lowprec_all) {
+            L.k_hi = 0;
+            continue;
+        }
+        L.k_hi = L.k;
+        if (prm.lowprec_tol <= 0 || L.sig.empty())
+            continue;
+        double tot = 0;
+        for (double s : L.sig)
+            tot += s * s;
+        const double budget = prm.lowprec_tol * std::sqrt(tot);
+        double acc          = 0;
+        int hi              = L.k;
+        while (hi > 0 && acc + 2 * u16 * L.sig[hi - 1] <= budget) {
+            acc += 2 * u16 * L.sig[hi - 1];
+            hi--;
+        }
+        L.k_hi = hi;
+    }
+}
+
+} // namespace htg

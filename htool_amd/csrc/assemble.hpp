@@ -1,0 +1,44 @@
+// H-matrix assembly on the GPU: block tasks from the host trees, dense leaves
+// left for generation in the product layout, low-rank leaves compressed by the
+// batched ACA into a compressed store, optionally recompressed (FP64 QR + SVD)
+// and split into precision classes.
+#pragma once
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "tree.hpp"
+
+namespace htg {
+
+struct PointsOnDevice {
+    DevArray<double> x, y, z;
+    DevArray<int> perm;
+    // Cluster-ordered copy of interleaved points: entry i is point perm[i].
+    void upload(const double *pts, const std::vector<int> &perm_h);
+    DevPoints view() const { return DevPoints{x.p, y.p, z.p, perm.p}; }
+};
+
+struct Assembly {
+    int nr = 0, nc = 0;
+    std::vector<int> perm_t, perm_s;
+    std::vector<Leaf> leaves; // in no particular order
+    int false_positive = 0;
+    DevArray<double> store;   // U and V of the low-rank leaves (Leaf::src_u, src_v)
+    KernelDesc kd{};
+    bool same = true;
+    PointsOnDevice pT, pS;
+    DevPoints dT() const { return pT.view(); }
+    DevPoints dS() const { return same ? pT.view() : pS.view(); }
+};
+
+KernelDesc kernel_desc(const htg_generator &g, bool same);
+
+// HMatrix::build (reference: include/htool/types/hmatrix.hpp:285-341, 422-607).
+void assemble(const htg_params &prm, const htg_generator &gen, int nt, const double *xt, int ns, const double *xs,
+              Assembly &out);
+
+// Recompresses the low-rank leaves in the store (FP64) and sets each leaf's
+// per-term scales; then assigns precision classes (Leaf::k_hi).
+void recompress_and_split(const htg_params &prm, std::vector<Leaf> &leaves, DevArray<double> &store);
+
+} // namespace htg

@@ -316,10 +316,305 @@ __global__ void __launch_bounds__(WG) k_lr_pack(const LrPack *packs, const UTile
     }
 }
 
-__global__ void k_convert_f32(const double *src, float *dst, uint64_t n) {
-    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x)
-        dst[e] = (float)src[e];
+// ---------------------------------------------------------------------------
+// FP64 recompression of one low-rank leaf per workgroup (no counterpart in the
+// reference; it follows the QR + SVD recompression of low-rank blocks,
+// e.g. Bebendorf, "Hierarchical Matrices", 2008, sec. 1.1.5). With U = Qu Ru
+// and V^T = Qv Rv (Ru, Rv from Cholesky factors of the Gram matrices, columns
+// scaled to unit norm first), C = Ru Rv^T = P S Q^T by one-sided Jacobi, the
+// leaf becomes U' S_r V' with U' = U Ru^-1 P_r and V' = Q_r^T Rv^-T V: unit
+// columns / rows and the kept singular values S_r, r the smallest rank whose
+// dropped singular values have a Frobenius norm <= tol * ||S||.
+namespace rc {
+constexpr int K = RC_KMAX;
+constexpr int CH = 32; // rows (or columns) staged per pass
+
+// Gram of the k vectors of a leaf factor. col_major: vector a is X[a*ld + i]
+// (entries i < len); otherwise entries are strided rows of a row-major k x len block.
+__device__ void gram(const double *X, size_t ld, int len, int k, double *G, double *sT) {
+    double acc[(K * K + WG - 1) / WG];
+    const int nent = k * k;
+#pragma unroll
+    for (int q = 0; q < (K * K + WG - 1) / WG; q++)
+        acc[q] = 0;
+    for (int i0 = 0; i0 < len; i0 += CH) {
+        const int nb = min(CH, len - i0);
+        for (int e = threadIdx.x; e < CH * k; e += WG) {
+            const int ii = e % CH, a = e / CH;
+            sT[ii * K + a] = ii < nb ? X[(size_t)a * ld + i0 + ii] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < (K * K + WG - 1) / WG; q++) {
+            const int e = threadIdx.x + q * WG;
+            if (e < nent) {
+                const int a = e / k, b = e % k;
+                double s = 0;
+                for (int ii = 0; ii < CH; ii++)
+                    s += sT[ii * K + a] * sT[ii * K + b];
+                acc[q] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < (K * K + WG - 1) / WG; q++) {
+        const int e = threadIdx.x + q * WG;
+        if (e < nent)
+            G[(e / k) * K + e % k] = acc[q];
+    }
+    __syncthreads();
 }
+
+// In place: G (k x k, row stride K) -> R upper triangular with R^T R = D^-1 G D^-1,
+// then R <- R D (D = sqrt(diag G)). Returns false when not positive definite.
+__device__ bool cholesky_scaled(double *G, int k, double *sd, int *s_ok) {
+    if (threadIdx.x < k)
+        sd[threadIdx.x] = sqrt(G[threadIdx.x * K + threadIdx.x]);
+    if (threadIdx.x == 0)
+        *s_ok = 1;
+    __syncthreads();
+    for (int e = threadIdx.x; e < k * k; e += WG) {
+        const int a = e / k, b = e % k;
+        G[a * K + b] = G[a * K + b] / (sd[a] * sd[b]);
+    }
+    __syncthreads();
+    for (int j = 0; j < k; j++) {
+        if (threadIdx.x == 0) {
+            double d = G[j * K + j];
+            for (int i = 0; i < j; i++)
+                d -= G[i * K + j] * G[i * K + j];
+            if (!(d > 1e-13))
+                *s_ok = 0;
+            G[j * K + j] = sqrt(fmax(d, 1e-300));
+        }
+        __syncthreads();
+        const int c = j + 1 + threadIdx.x;
+        if (c < k) {
+            double v = G[j * K + c];
+            for (int i = 0; i < j; i++)
+                v -= G[i * K + j] * G[i * K + c];
+            G[j * K + c] = v / G[j * K + j];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < k * k; e += WG) {
+        const int a = e / k, b = e % k;
+        G[a * K + b] = b >= a ? G[a * K + b] * sd[b] : 0.0;
+    }
+    const bool ok = *s_ok != 0;
+    __syncthreads(); // every thread has read s_ok before the next call resets it
+    return ok;
+}
+
+// Solves R X = B (R upper triangular k x k, B k x r, both row stride K) in place in B.
+__device__ void upper_solve(const double *R, double *B, int k, int r) {
+    const int c = threadIdx.x;
+    if (c < r) {
+        for (int i = k - 1; i >= 0; i--) {
+            double v = B[i * K + c];
+            for (int l = i + 1; l < k; l++)
+                v -= R[i * K + l] * B[l * K + c];
+            B[i * K + c] = v / R[i * K + i];
+        }
+    }
+    __syncthreads();
+}
+} // namespace rc
+
+__global__ void __launch_bounds__(WG) k_recompress(const RcLeaf *leaves, double *store, double tol, int *rank_out,
+                                                   int *ok_out, double *sigma_out) {
+    using namespace rc;
+    __shared__ double sRu[K * K], sRv[K * K], sW[K * K], sQ[K * K], sT[CH * K];
+    __shared__ double s_sig[K], s_d[K];
+    __shared__ int s_idx[K], s_ok, s_rot, s_r;
+    const RcLeaf L = leaves[blockIdx.x];
+    const int k    = L.k;
+    double *U      = store + L.u_off;
+    double *V      = store + L.v_off;
+    if (k < 2 || k > K) {
+        if (threadIdx.x == 0) {
+            rank_out[blockIdx.x] = k;
+            ok_out[blockIdx.x]   = 0;
+        }
+        return;
+    }
+    // Gram matrices: U^T U (vector a = column a of U) and V V^T (vector a = row a of V).
+    gram(U, (size_t)L.M, L.M, k, sRu, sT);
+    gram(V, (size_t)L.N, L.N, k, sRv, sT);
+    const bool oku = cholesky_scaled(sRu, k, s_d, &s_ok);
+    const bool okv = cholesky_scaled(sRv, k, s_d, &s_ok);
+    if (!oku || !okv) {
+        if (threadIdx.x == 0) {
+            rank_out[blockIdx.x] = k;
+            ok_out[blockIdx.x]   = 0;
+        }
+        return;
+    }
+    // C = Ru Rv^T into sW (row stride K); sQ = identity.
+    for (int e = threadIdx.x; e < k * k; e += WG) {
+        const int a = e / k, b = e % k;
+        double v = 0;
+        for (int i = max(a, b); i < k; i++)
+            v += sRu[a * K + i] * sRv[b * K + i];
+        sW[a * K + b] = v;
+        sQ[a * K + b] = a == b ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // One-sided Jacobi on the columns of C (round-robin pairs, kk even).
+    const int kk = k + (k & 1);
+    if (k & 1) {
+        for (int a = threadIdx.x; a < k; a += WG) {
+            sW[a * K + k] = 0.0;
+            sQ[a * K + k] = 0.0;
+        }
+        if (threadIdx.x == 0)
+            sQ[k * K + k] = 1.0;
+        for (int b = threadIdx.x; b < k; b += WG)
+            sQ[k * K + b] = 0.0;
+        __syncthreads();
+    }
+    const int rows = k; // rows of W; Q has kk rows when padded
+    for (int sweep = 0; sweep < 40; sweep++) {
+        if (threadIdx.x == 0)
+            s_rot = 0;
+        __syncthreads();
+        for (int round = 0; round < kk - 1; round++) {
+            const int t = threadIdx.x;
+            if (t < kk / 2) {
+                // round-robin: position 0 fixed, the others rotate
+                auto at = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + round) % (kk - 1); };
+                int p = at(t), q = at(kk - 1 - t);
+                if (p > q) {
+                    const int tmp = p;
+                    p = q, q = tmp;
+                }
+                double al = 0, be = 0, ga = 0;
+                for (int a = 0; a < rows; a++) {
+                    const double wp = sW[a * K + p], wq = sW[a * K + q];
+                    al += wp * wp;
+                    be += wq * wq;
+                    ga += wp * wq;
+                }
+                if (fabs(ga) > 1e-15 * sqrt(al * be) && ga != 0.0) {
+                    const double zeta = (be - al) / (2 * ga);
+                    const double tt   = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+                    const double c = 1 / sqrt(1 + tt * tt), sn = c * tt;
+                    for (int a = 0; a < rows; a++) {
+                        const double wp = sW[a * K + p], wq = sW[a * K + q];
+                        sW[a * K + p] = c * wp - sn * wq;
+                        sW[a * K + q] = sn * wp + c * wq;
+                    }
+                    for (int a = 0; a < kk; a++) {
+                        const double vp = sQ[a * K + p], vq = sQ[a * K + q];
+                        sQ[a * K + p] = c * vp - sn * vq;
+                        sQ[a * K + q] = sn * vp + c * vq;
+                    }
+                    s_rot = 1;
+                }
+            }
+            __syncthreads();
+        }
+        if (!s_rot)
+            break;
+        __syncthreads();
+    }
+    // Singular values (column norms of W), sorted descending; rank r by the tail test.
+    if (threadIdx.x < kk) {
+        double v = 0;
+        for (int a = 0; a < rows; a++)
+            v += sW[a * K + threadIdx.x] * sW[a * K + threadIdx.x];
+        s_d[threadIdx.x] = sqrt(v);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 0; j < kk; j++)
+            s_idx[j] = j;
+        for (int j = 1; j < kk; j++) { // insertion sort, descending
+            const int id = s_idx[j];
+            int l        = j - 1;
+            while (l >= 0 && s_d[s_idx[l]] < s_d[id]) {
+                s_idx[l + 1] = s_idx[l];
+                l--;
+            }
+            s_idx[l + 1] = id;
+        }
+        double tot = 0;
+        for (int j = 0; j < k; j++)
+            tot += s_d[s_idx[j]] * s_d[s_idx[j]];
+        int r      = k;
+        double tail = 0;
+        for (int j = k - 1; j >= 1; j--) {
+            tail += s_d[s_idx[j]] * s_d[s_idx[j]];
+            if (tail <= tol * tol * tot)
+                r = j;
+            else
+                break;
+        }
+        for (int j = 0; j < k; j++)
+            s_sig[j] = s_d[s_idx[j]];
+        s_r = r;
+    }
+    __syncthreads();
+    const int r = s_r;
+    // A = Ru^-1 P_r (P_r columns = W columns / sigma) into sT-free scratch: reuse sW's
+    // right part is unsafe, so build P_r in sQ's transpose order first.
+    // Bt = Rv^-1 Q_r.
+    __shared__ double sA[K * K], sB[K * K];
+    for (int e = threadIdx.x; e < k * r; e += WG) {
+        const int a = e / r, c = e % r;
+        const int id = s_idx[c];
+        sA[a * K + c] = sW[a * K + id] / s_sig[c];
+        sB[a * K + c] = sQ[a * K + id];
+    }
+    __syncthreads();
+    upper_solve(sRu, sA, k, r);
+    upper_solve(sRv, sB, k, r);
+    // U <- U A (rows staged through sT), V <- B^T V (columns staged through sT).
+    for (int i0 = 0; i0 < L.M; i0 += CH) {
+        const int nb = min(CH, L.M - i0);
+        for (int e = threadIdx.x; e < CH * k; e += WG) {
+            const int ii = e % CH, a = e / CH;
+            sT[ii * K + a] = ii < nb ? U[(size_t)a * L.M + i0 + ii] : 0.0;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < CH * r; e += WG) {
+            const int ii = e % CH, c = e / CH;
+            if (ii < nb) {
+                double v = 0;
+                for (int a = 0; a < k; a++)
+                    v += sT[ii * K + a] * sA[a * K + c];
+                U[(size_t)c * L.M + i0 + ii] = v;
+            }
+        }
+        __syncthreads();
+    }
+    for (int j0 = 0; j0 < L.N; j0 += CH) {
+        const int nb = min(CH, L.N - j0);
+        for (int e = threadIdx.x; e < CH * k; e += WG) {
+            const int jj = e % CH, a = e / CH;
+            sT[jj * K + a] = jj < nb ? V[(size_t)a * L.N + j0 + jj] : 0.0;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < CH * r; e += WG) {
+            const int jj = e % CH, c = e / CH;
+            if (jj < nb) {
+                double v = 0;
+                for (int a = 0; a < k; a++)
+                    v += sB[a * K + c] * sT[jj * K + a];
+                V[(size_t)c * L.N + j0 + jj] = v;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        rank_out[blockIdx.x] = r;
+        ok_out[blockIdx.x]   = 1;
+    }
+    for (int j = threadIdx.x; j < r; j += WG)
+        sigma_out[(size_t)blockIdx.x * K + j] = s_sig[j];
+}
+
 
 // ---------------------------------------------------------------------------
 // Product phase 0: x (nc x mu column-major, original order) -> z[0, nc*MU)
@@ -352,6 +647,57 @@ struct Vec<double> {
         a[0] = v.x, a[1] = v.y;
     }
 };
+template <>
+struct Vec<__half> {
+    static constexpr int W = 8;
+    __device__ static void load(const __half *p, __half (&a)[8]) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(p);
+        const __half *h = reinterpret_cast<const __half *>(&v);
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            a[u] = h[u];
+    }
+};
+template <>
+struct Vec<__hip_bfloat16> {
+    static constexpr int W = 8;
+    __device__ static void load(const __hip_bfloat16 *p, __hip_bfloat16 (&a)[8]) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(p);
+        const __hip_bfloat16 *h = reinterpret_cast<const __hip_bfloat16 *>(&v);
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            a[u] = h[u];
+    }
+};
+
+// W consecutive entries of the compute type (16-byte aligned), in 16-byte loads.
+template <typename TC, int W>
+__device__ __forceinline__ void load_n(const TC *p, TC (&a)[W]) {
+    constexpr int VW = Vec<TC>::W;
+    static_assert(W % VW == 0, "vector width");
+#pragma unroll
+    for (int h = 0; h < W / VW; h++) {
+        TC t[VW];
+        Vec<TC>::load(p + h * VW, t);
+#pragma unroll
+        for (int u = 0; u < VW; u++)
+            a[h * VW + u] = t[u];
+    }
+}
+
+// Stored entry -> compute type.
+template <typename TC, typename TS>
+__device__ __forceinline__ TC cvt(TS v) {
+    return (TC)v;
+}
+template <>
+__device__ __forceinline__ float cvt<float, __half>(__half v) {
+    return __half2float(v);
+}
+template <>
+__device__ __forceinline__ float cvt<float, __hip_bfloat16>(__hip_bfloat16 v) {
+    return __bfloat162float(v);
+}
 
 // Lanes per row for a row of nvec 16-byte vectors: a power of two between 8
 // (one 128-byte line per load) and 64, aiming at about four loads per lane.
@@ -389,17 +735,12 @@ __device__ __forceinline__ void rows_gemv(const TS *base, size_t ld, int nrows, 
                 for (int q = sl; q < nvec; q += lpr) {
                     TS a[W], b[W];
                     Vec<TS>::load(row0 + (size_t)q * W, a);
-                    if (ok1)
-                        Vec<TS>::load(row1 + (size_t)q * W, b);
-                    else
-#pragma unroll
-                        for (int u = 0; u < W; u++)
-                            b[u] = 0;
+                    Vec<TS>::load(row1 + (size_t)q * W, b); // row 0 again when r1 is past the end
 #pragma unroll
                     for (int u = 0; u < W; u++) {
                         const TC zz = s_z[q * W + u];
-                        acc0 += (TC)a[u] * zz;
-                        acc1 += (TC)b[u] * zz;
+                        acc0 += cvt<TC>(a[u]) * zz;
+                        acc1 += cvt<TC>(b[u]) * zz;
                     }
                 }
             }
@@ -432,7 +773,7 @@ __device__ __forceinline__ void rows_gemv(const TS *base, size_t ld, int nrows, 
                     const TC *zz = s_z + (q * W + u) * MU;
 #pragma unroll
                     for (int s = 0; s < MU; s++)
-                        part[s] += (TC)a[u] * zz[s];
+                        part[s] += cvt<TC>(a[u]) * zz[s];
                 }
             }
         }
@@ -465,7 +806,13 @@ constexpr int ROWMAX = 128;  // rows of a row block / V rows per pass
 // several chunks writes one partial sum per chunk; k_vx_sum adds them in
 // chunk order.
 template <typename TS, typename TC, int MU>
-__device__ __forceinline__ void vx_store(const VTile &d, int r, const double (&v)[MU], TC *z, double *part) {
+__device__ __forceinline__ void vx_store(const VTile &d, int r, const double (&v0)[MU], TC *z, double *part,
+                                         const double *tscale) {
+    double v[MU];
+    const double sc = (tscale && d.nparts == 1) ? tscale[d.t_index + r] : 1.0;
+#pragma unroll
+    for (int s = 0; s < MU; s++)
+        v[s] = v0[s] * sc;
     if (d.nparts > 1) {
         double *o = part + (size_t)(d.part_off + r) * MU;
 #pragma unroll
@@ -479,11 +826,96 @@ __device__ __forceinline__ void vx_store(const VTile &d, int r, const double (&v
     }
 }
 
+// One V chunk: rows taken by lane groups (lanes_per_row), FP64 shuffle reduction.
 template <typename TS, typename TC, int MU>
-__global__ void __launch_bounds__(WG) k_vx(const int *wv_off, int nwaves, const VTile *tiles, const TS *varena,
-                                           TC *z, double *part) {
+__device__ __forceinline__ void vx_chunk(const VTile &d, const TS *varena, TC *z, double *part, const double *tscale,
+                                         int lane) {
     constexpr int W = Vec<TS>::W;
-    const int wv    = blockIdx.x * NWAVE + (threadIdx.x >> 6);
+    const int nvec  = d.ncols / W;
+    const int lpr   = lanes_per_row(nvec);
+    const int rpw   = 64 / lpr;
+    const int sl = lane & (lpr - 1), sub = lane / lpr;
+    const TC *xv = z + (size_t)d.x_off * MU;
+    if constexpr (MU == 1) {
+        // two rows per lane group: one x load serves both
+        for (int rbase = 0; rbase < d.k; rbase += 2 * rpw) {
+            const int r0 = rbase + sub, r1 = r0 + rpw;
+            const bool ok0 = r0 < d.k, ok1 = r1 < d.k;
+            const TS *row0 = varena + d.v_off + (size_t)(ok0 ? r0 : 0) * d.ld;
+            const TS *row1 = varena + d.v_off + (size_t)(ok1 ? r1 : 0) * d.ld;
+            TC acc0 = 0, acc1 = 0;
+            if (ok0) {
+#pragma unroll 2
+                for (int q = sl; q < nvec; q += lpr) {
+                    TS a[W], b[W];
+                    TC xq[W];
+                    load_n<TC, W>(xv + (size_t)q * W, xq);
+                    Vec<TS>::load(row0 + (size_t)q * W, a);
+                    Vec<TS>::load(row1 + (size_t)q * W, b); // row 0 again when r1 is past the end
+#pragma unroll
+                    for (int u = 0; u < W; u++) {
+                        acc0 += cvt<TC>(a[u]) * xq[u];
+                        acc1 += cvt<TC>(b[u]) * xq[u];
+                    }
+                }
+            }
+            double v0[1] = {(double)acc0}, v1[1] = {(double)acc1};
+            for (int o = lpr >> 1; o > 0; o >>= 1) {
+                v0[0] += __shfl_xor(v0[0], o, 64);
+                v1[0] += __shfl_xor(v1[0], o, 64);
+            }
+            if (sl == 0 && ok0)
+                vx_store<TS, TC, 1>(d, r0, v0, z, part, tscale);
+            if (sl == 0 && ok1)
+                vx_store<TS, TC, 1>(d, r1, v1, z, part, tscale);
+        }
+    } else {
+        for (int rbase = 0; rbase < d.k; rbase += rpw) {
+            const int r = rbase + sub;
+            TC acc[MU];
+#pragma unroll
+            for (int s = 0; s < MU; s++)
+                acc[s] = 0;
+            if (r < d.k) {
+                const TS *row = varena + d.v_off + (size_t)r * d.ld;
+#pragma unroll 2
+                for (int q = sl; q < nvec; q += lpr) {
+                    TS a[W];
+                    Vec<TS>::load(row + (size_t)q * W, a);
+#pragma unroll
+                    for (int u = 0; u < W; u++) {
+                        const TC *xj = xv + (size_t)(q * W + u) * MU;
+#pragma unroll
+                        for (int s = 0; s < MU; s++)
+                            acc[s] += cvt<TC>(a[u]) * xj[s];
+                    }
+                }
+            }
+            double v[MU];
+#pragma unroll
+            for (int s = 0; s < MU; s++)
+                v[s] = (double)acc[s];
+            for (int o = lpr >> 1; o > 0; o >>= 1) {
+#pragma unroll
+                for (int s = 0; s < MU; s++)
+                    v[s] += __shfl_xor(v[s], o, 64);
+            }
+            if (sl == 0 && r < d.k)
+                vx_store<TS, TC, MU>(d, r, v, z, part, tscale);
+        }
+    }
+}
+
+// Product phase 1: t = V x for every low-rank leaf. Each wavefront walks its
+// own list of V chunks (no workgroup barrier anywhere). V rows are stored
+// aligned to the global column index, so a lane reads 16 bytes of V and the
+// matching x entries from the L2-resident z with the same vector index. A
+// leaf cut into several chunks writes one partial sum per chunk; k_vx_sum adds
+// them in chunk order.
+template <typename TS0, typename TS1, typename TC, int MU>
+__global__ void __launch_bounds__(WG) k_vx(const int *wv_off, int nwaves, const VTile *tiles, const TS0 *varena0,
+                                           const TS1 *varena1, TC *z, double *part, const double *tscale) {
+    const int wv = blockIdx.x * NWAVE + (threadIdx.x >> 6);
     if (wv >= nwaves)
         return;
     const int lane = threadIdx.x & 63;
@@ -492,91 +924,18 @@ __global__ void __launch_bounds__(WG) k_vx(const int *wv_off, int nwaves, const 
     VTile d        = tiles[ti];
     for (; ti < tend; ti++) {
         const VTile next = tiles[ti + 1 < tend ? ti + 1 : ti]; // descriptor of the next chunk, loaded early
-        const int nvec   = d.ncols / W;
-        const int lpr    = lanes_per_row(nvec);
-        const int rpw    = 64 / lpr;
-        const int sl = lane & (lpr - 1), sub = lane / lpr;
-        const TC *xv = z + (size_t)d.x_off * MU;
-        if constexpr (MU == 1) {
-            // two rows per lane group: one x load serves both
-            for (int rbase = 0; rbase < d.k; rbase += 2 * rpw) {
-                const int r0 = rbase + sub, r1 = r0 + rpw;
-                const bool ok0 = r0 < d.k, ok1 = r1 < d.k;
-                const TS *row0 = varena + d.v_off + (size_t)(ok0 ? r0 : 0) * d.ld;
-                const TS *row1 = varena + d.v_off + (size_t)(ok1 ? r1 : 0) * d.ld;
-                TC acc0 = 0, acc1 = 0;
-                if (ok0) {
-#pragma unroll 2
-                    for (int q = sl; q < nvec; q += lpr) {
-                        TS a[W], b[W];
-                        TC xq[W];
-                        Vec<TC>::load(xv + (size_t)q * W, xq);
-                        Vec<TS>::load(row0 + (size_t)q * W, a);
-                        if (ok1)
-                            Vec<TS>::load(row1 + (size_t)q * W, b);
-                        else
-#pragma unroll
-                            for (int u = 0; u < W; u++)
-                                b[u] = 0;
-#pragma unroll
-                        for (int u = 0; u < W; u++) {
-                            acc0 += (TC)a[u] * xq[u];
-                            acc1 += (TC)b[u] * xq[u];
-                        }
-                    }
-                }
-                double v0[1] = {(double)acc0}, v1[1] = {(double)acc1};
-                for (int o = lpr >> 1; o > 0; o >>= 1) {
-                    v0[0] += __shfl_xor(v0[0], o, 64);
-                    v1[0] += __shfl_xor(v1[0], o, 64);
-                }
-                if (sl == 0 && ok0)
-                    vx_store<TS, TC, 1>(d, r0, v0, z, part);
-                if (sl == 0 && ok1)
-                    vx_store<TS, TC, 1>(d, r1, v1, z, part);
-            }
-        } else {
-            for (int rbase = 0; rbase < d.k; rbase += rpw) {
-                const int r = rbase + sub;
-                TC acc[MU];
-#pragma unroll
-                for (int s = 0; s < MU; s++)
-                    acc[s] = 0;
-                if (r < d.k) {
-                    const TS *row = varena + d.v_off + (size_t)r * d.ld;
-#pragma unroll 2
-                    for (int q = sl; q < nvec; q += lpr) {
-                        TS a[W];
-                        Vec<TS>::load(row + (size_t)q * W, a);
-#pragma unroll
-                        for (int u = 0; u < W; u++) {
-                            const TC *xj = xv + (size_t)(q * W + u) * MU;
-#pragma unroll
-                            for (int s = 0; s < MU; s++)
-                                acc[s] += (TC)a[u] * xj[s];
-                        }
-                    }
-                }
-                double v[MU];
-#pragma unroll
-                for (int s = 0; s < MU; s++)
-                    v[s] = (double)acc[s];
-                for (int o = lpr >> 1; o > 0; o >>= 1) {
-#pragma unroll
-                    for (int s = 0; s < MU; s++)
-                        v[s] += __shfl_xor(v[s], o, 64);
-                }
-                if (sl == 0 && r < d.k)
-                    vx_store<TS, TC, MU>(d, r, v, z, part);
-            }
-        }
+        if (d.cls == 0)
+            vx_chunk<TS0, TC, MU>(d, varena0, z, part, tscale, lane);
+        else
+            vx_chunk<TS1, TC, MU>(d, varena1, z, part, tscale, lane);
         d = next;
     }
 }
 
 // Adds the chunk partial sums of each multi-chunk leaf into its t entries.
 template <typename TC, int MU>
-__global__ void __launch_bounds__(WG) k_vx_sum(const VTile *tiles, const int *first_chunk, const double *part, TC *z) {
+__global__ void __launch_bounds__(WG) k_vx_sum(const VTile *tiles, const int *first_chunk, const double *part, TC *z,
+                                               const double *tscale) {
     const VTile d    = tiles[first_chunk[blockIdx.x]];
     const int km     = d.k * MU;
     const double *pb = part + (size_t)d.part_base * MU;
@@ -585,30 +944,26 @@ __global__ void __launch_bounds__(WG) k_vx_sum(const VTile *tiles, const int *fi
         double v = 0;
         for (int c = 0; c < d.nparts; c++)
             v += pb[(size_t)c * km + e];
-        o[e] = (TC)v;
+        o[e] = (TC)(tscale ? v * tscale[d.t_index + e / MU] : v);
     }
 }
 
-// Product phase 2: one workgroup per row block (<= 128 rows). Its dense-leaf
-// rows and U rows form one row-major R x C block (row stride C rounded up to 4,
-// zero padded); the vector entry of each column (an x entry for a dense
-// column, a t entry for a U column) is staged through LDS, and the row sums
-// are scattered to original order.
+// Product phase 2: one workgroup per row block (<= 128 rows). In each
+// precision class the row block's dense-leaf rows and U rows form one
+// row-major R x C block (row stride C rounded up to 4, zero padded); the vector
+// entry of each column (an x entry for a dense column, a t entry for a U
+// column) is staged through LDS. Both classes add into the same FP64 row
+// sums, which are scattered to original order.
 template <typename TS, typename TC, int MU>
-__global__ void __launch_bounds__(WG) k_rows(const int *rb_order, const int *rb_row0, const int *rb_col0,
-                                             const uint64_t *rb_aoff, const int *colidx, const TS *arena,
-                                             const TC *z, const int *perm_t, int nr, int mu_base, double *y) {
+__device__ __forceinline__ void rows_class(const TS *arena, const RowClass &rc, int rb, int R, const TC *z, TC *s_z,
+                                           double *s_acc) {
     constexpr int ZCOLS = ZCAP / MU;
     constexpr int W     = Vec<TS>::W;
-    __shared__ TC s_z[ZCAP];
-    __shared__ double s_acc[ROWMAX * MU];
-    const int rb = rb_order[blockIdx.x];
-    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
-    const int c0 = rb_col0[rb], C = rb_col0[rb + 1] - c0;
-    const int Cp = (C + 3) & ~3;
+    const int *colidx   = rc.colidx;
+    const int c0 = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - c0;
+    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+    const uint64_t *rb_aoff = rc.rb_aoff;
     const TS *A  = arena + rb_aoff[rb];
-    for (int e = threadIdx.x; e < R * MU; e += WG)
-        s_acc[e] = 0;
     for (int cc = 0; cc < Cp; cc += ZCOLS) {
         const int nz = min(ZCOLS, Cp - cc);
         for (int e = threadIdx.x; e < nz * MU; e += WG) {
@@ -619,10 +974,32 @@ __global__ void __launch_bounds__(WG) k_rows(const int *rb_order, const int *rb_
         rows_gemv<TS, TC, MU>(A + cc, (size_t)Cp, R, nz / W, s_z, s_acc);
         __syncthreads();
     }
+}
+
+template <typename TS0, typename TS1, typename TC, int MU>
+__global__ void __launch_bounds__(WG) k_rows(const int *rb_order, const int *rb_row0, RowClass c0, const TS0 *arena0,
+                                             RowClass c1, const TS1 *arena1, const TC *z, const int *perm_t, int nr,
+                                             int mu_base, double *y) {
+    __shared__ TC s_z[ZCAP];
+    __shared__ double s_acc[ROWMAX * MU];
+    const int rb = rb_order[blockIdx.x];
+    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
+    for (int e = threadIdx.x; e < R * MU; e += WG)
+        s_acc[e] = 0;
+    rows_class<TS0, TC, MU>(arena0, c0, rb, R, z, s_z, s_acc);
+    if (arena1)
+        rows_class<TS1, TC, MU>(arena1, c1, rb, R, z, s_z, s_acc);
+    __syncthreads();
     for (int e = threadIdx.x; e < R * MU; e += WG) {
         const int r = e / MU, s = e % MU;
         y[(size_t)perm_t[r0 + r] + (size_t)nr * (mu_base + s)] = s_acc[e];
     }
+}
+
+template <typename TS>
+__global__ void k_convert(const double *src, TS *dst, uint64_t n) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x)
+        dst[e] = (TS)src[e];
 }
 
 } // namespace
@@ -648,14 +1025,24 @@ void launch_lr_pack(const LrPack *packs, int npacks, const UTileDst *utiles, con
         hipLaunchKernelGGL(k_lr_pack, dim3(npacks), dim3(WG), 0, st, packs, utiles, ws, arena, varena);
 }
 
-void launch_convert_f32(const double *src, float *dst, uint64_t n, hipStream_t st) {
+void launch_recompress(const RcLeaf *leaves, int nleaves, double *store, double tol, int *rank, int *ok,
+                       double *sigma, hipStream_t st) {
+    if (nleaves > 0)
+        hipLaunchKernelGGL(k_recompress, dim3(nleaves), dim3(WG), 0, st, leaves, store, tol, rank, ok, sigma);
+}
+
+template <typename TS>
+void launch_convert(const double *src, TS *dst, uint64_t n, hipStream_t st) {
     if (n == 0)
         return;
     uint64_t blocks = (n + WG - 1) / WG;
     if (blocks > 65536)
         blocks = 65536;
-    hipLaunchKernelGGL(k_convert_f32, dim3((unsigned)blocks), dim3(WG), 0, st, src, dst, n);
+    hipLaunchKernelGGL(k_convert<TS>, dim3((unsigned)blocks), dim3(WG), 0, st, src, dst, n);
 }
+template void launch_convert<float>(const double *, float *, uint64_t, hipStream_t);
+template void launch_convert<__half>(const double *, __half *, uint64_t, hipStream_t);
+template void launch_convert<__hip_bfloat16>(const double *, __hip_bfloat16 *, uint64_t, hipStream_t);
 
 #define HTG_MU_SWITCH(MUV, CALL) \
     switch (MUV) {               \
@@ -677,46 +1064,59 @@ void launch_gather_x(const double *x, const int *perm_s, int nc, int mu_base, in
 #undef HTG_GATHER
 }
 
-template <typename TS, typename TC>
+template <typename TS0, typename TS1, typename TC>
 void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *multi_first, int nmulti,
-               const TS *varena, TC *z, double *part, int MU, hipStream_t st) {
+               const TS0 *varena0, const TS1 *varena1, TC *z, double *part, const double *tscale, int MU,
+               hipStream_t st) {
     if (ngroups <= 0)
         return;
-#define HTG_VX(M)                                                                                         \
-    do {                                                                                                  \
-        hipLaunchKernelGGL((k_vx<TS, TC, M>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off,    \
-                           ngroups, tiles, varena, z, part);                                              \
-        if (nmulti > 0)                                                                                   \
-            hipLaunchKernelGGL((k_vx_sum<TC, M>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first, part, z); \
+#define HTG_VX(M)                                                                                          \
+    do {                                                                                                   \
+        hipLaunchKernelGGL((k_vx<TS0, TS1, TC, M>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, \
+                           ngroups, tiles, varena0, varena1, z, part, tscale);                             \
+        if (nmulti > 0)                                                                                    \
+            hipLaunchKernelGGL((k_vx_sum<TC, M>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first, part, z,  \
+                               tscale);                                                                    \
     } while (0)
     HTG_MU_SWITCH(MU, HTG_VX)
 #undef HTG_VX
 }
 
-template <typename TS, typename TC>
-void launch_rows(const int *rb_order, const int *rb_row0, const int *rb_col0, const uint64_t *rb_aoff, int nrb,
-                 const int *colidx, const TS *arena, const TC *z, const int *perm_t, int nr, int mu_base, int MU,
-                 double *y, hipStream_t st) {
+template <typename TS0, typename TS1, typename TC>
+void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, const TS0 *arena0, RowClass c1,
+                 const TS1 *arena1, const TC *z, const int *perm_t, int nr, int mu_base, int MU, double *y,
+                 hipStream_t st) {
     if (nrb <= 0)
         return;
-#define HTG_ROWS(M)                                                                                              \
-    hipLaunchKernelGGL((k_rows<TS, TC, M>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, rb_col0, rb_aoff, colidx, \
-                       arena, z, perm_t, nr, mu_base, y)
+#define HTG_ROWS(M)                                                                                          \
+    hipLaunchKernelGGL((k_rows<TS0, TS1, TC, M>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0, c1, \
+                       arena1, z, perm_t, nr, mu_base, y)
     HTG_MU_SWITCH(MU, HTG_ROWS)
 #undef HTG_ROWS
 }
 
 template void launch_gather_x<double>(const double *, const int *, int, int, int, double *, hipStream_t);
 template void launch_gather_x<float>(const double *, const int *, int, int, int, float *, hipStream_t);
-template void launch_vx<double, double>(const int *, int, const VTile *, const int *, int, const double *, double *,
-                                        double *, int, hipStream_t);
-template void launch_vx<float, float>(const int *, int, const VTile *, const int *, int, const float *, float *,
-                                      double *, int, hipStream_t);
-template void launch_rows<double, double>(const int *, const int *, const int *, const uint64_t *, int, const int *,
-                                          const double *, const double *, const int *, int, int, int, double *,
-                                          hipStream_t);
-template void launch_rows<float, float>(const int *, const int *, const int *, const uint64_t *, int, const int *,
-                                        const float *, const float *, const int *, int, int, int, double *,
-                                        hipStream_t);
+template void launch_vx<double, double, double>(const int *, int, const VTile *, const int *, int, const double *,
+                                                const double *, double *, double *, const double *, int, hipStream_t);
+template void launch_vx<float, float, float>(const int *, int, const VTile *, const int *, int, const float *,
+                                             const float *, float *, double *, const double *, int, hipStream_t);
+template void launch_vx<float, __half, float>(const int *, int, const VTile *, const int *, int, const float *,
+                                              const __half *, float *, double *, const double *, int, hipStream_t);
+template void launch_vx<float, __hip_bfloat16, float>(const int *, int, const VTile *, const int *, int,
+                                                      const float *, const __hip_bfloat16 *, float *, double *,
+                                                      const double *, int, hipStream_t);
+template void launch_rows<double, double, double>(const int *, const int *, int, RowClass, const double *, RowClass,
+                                                  const double *, const double *, const int *, int, int, int,
+                                                  double *, hipStream_t);
+template void launch_rows<float, float, float>(const int *, const int *, int, RowClass, const float *, RowClass,
+                                               const float *, const float *, const int *, int, int, int, double *,
+                                               hipStream_t);
+template void launch_rows<float, __half, float>(const int *, const int *, int, RowClass, const float *, RowClass,
+                                                const __half *, const float *, const int *, int, int, int, double *,
+                                                hipStream_t);
+template void launch_rows<float, __hip_bfloat16, float>(const int *, const int *, int, RowClass, const float *,
+                                                        RowClass, const __hip_bfloat16 *, const float *,
+                                                        const int *, int, int, int, double *, hipStream_t);
 
 } // namespace htg

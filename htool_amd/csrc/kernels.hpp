@@ -1,6 +1,8 @@
 // Device data structures and kernel launchers of the H-matrix hot path.
 #pragma once
 
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -40,10 +42,11 @@ struct AcaBlock {
 };
 
 // Column chunk of a low-rank leaf's V (k x n, row-major, row stride ld). Each
-// stored row covers the global columns [oj & ~3, round4(oj + n)), zero outside
-// the leaf, so stored column c multiplies z entry (oj & ~3) + c and both are
-// 16-byte aligned. A chunk is k sums over ncols (a multiple of 4) stored
-// columns starting at z entry x_off; chunks are grouped per wavefront.
+// stored row covers the global columns [oj & ~(a-1), round_a(oj + n)), zero
+// outside the leaf (a = 4 for class 0, 8 for class 1), so stored column c
+// multiplies z entry (oj & ~(a-1)) + c and both are 16-byte aligned. A chunk is
+// k sums over ncols (a multiple of a) stored columns starting at z entry x_off;
+// chunks are grouped per wavefront.
 struct VTile {
     uint64_t v_off; // first entry of the chunk in the V arena
     int k, ncols, x_off, ld;
@@ -51,7 +54,17 @@ struct VTile {
     int part_off;  // partial sums of this chunk (in RHS vectors), when nparts > 1
     int part_base; // partial sums of chunk 0 of this leaf
     int nparts;    // number of column chunks of this leaf
+    int cls;       // precision class of the rows (which V arena)
+    int pad;
 };
+
+// One low-rank leaf handed to the recompression kernel: U (M x k column-major
+// at u_off) and V (k x N row-major at v_off) in the compressed store.
+struct RcLeaf {
+    uint64_t u_off, v_off;
+    int M, N, k, pad;
+};
+constexpr int RC_KMAX = 48; // largest rank the recompression kernel handles
 
 // Copy of one compressed leaf: U (M x k column-major at src_u) into U tiles, V
 // (k rows of N entries, row strides src_ldv / dst_ldv) from src_v to dst_v.
@@ -76,19 +89,38 @@ void launch_aca(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBlock *b
                 hipStream_t st);
 void launch_lr_pack(const LrPack *packs, int npacks, const UTileDst *utiles, const double *ws, double *arena,
                     double *varena, hipStream_t st);
-void launch_convert_f32(const double *src, float *dst, uint64_t n, hipStream_t st);
+// In place: U <- U A, V <- B V with rank r <= k; rank[b] = r, ok[b] = 1 when
+// recompressed (0: kept as is), sigma[b*RC_KMAX + j] = the kept singular values.
+void launch_recompress(const RcLeaf *leaves, int nleaves, double *store, double tol, int *rank, int *ok,
+                       double *sigma, hipStream_t st);
 
 // Product phases. TS = storage type, TC = compute type inside a tile. z holds
 // the cluster-ordered x (nc RHS vectors) followed by t (one RHS vector per
 // low-rank term), both in TC.
 template <typename TC>
 void launch_gather_x(const double *x, const int *perm_s, int nc, int mu_base, int MU, TC *z, hipStream_t st);
-template <typename TS, typename TC>
+// Both precision classes in one launch: chunk d reads varena0 or varena1 by d.cls.
+template <typename TS0, typename TS1, typename TC>
 void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *multi_first, int nmulti,
-               const TS *varena, TC *z, double *part, int MU, hipStream_t st);
-template <typename TS, typename TC>
-void launch_rows(const int *rb_order, const int *rb_row0, const int *rb_col0, const uint64_t *rb_aoff, int nrb,
-                 const int *colidx, const TS *arena, const TC *z, const int *perm_t, int nr, int mu_base, int MU,
-                 double *y, hipStream_t st);
+               const TS0 *varena0, const TS1 *varena1, TC *z, double *part, const double *tscale, int MU,
+               hipStream_t st);
+
+// One precision class of the row blocks: per row block its first column in
+// colidx and the offset of its row-major block (row stride = columns rounded
+// up to 4) in the class's arena.
+struct RowClass {
+    const int *rb_col0;
+    const uint64_t *rb_aoff;
+    const int *colidx;
+    int align; // row stride = columns rounded up to a multiple of align (4, or 8 for class 1)
+};
+template <typename TS0, typename TS1, typename TC>
+void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, const TS0 *arena0, RowClass c1,
+                 const TS1 *arena1, const TC *z, const int *perm_t, int nr, int mu_base, int MU, double *y,
+                 hipStream_t st);
+
+// FP64 master copy -> storage type (round to nearest).
+template <typename TS>
+void launch_convert(const double *src, TS *dst, uint64_t n, hipStream_t st);
 
 } // namespace htg

@@ -27,7 +27,10 @@ class Generator:
 
 
 _COMPRESSORS = {"partialACA": capi.PARTIAL_ACA, "sympartialACA": capi.SYM_PARTIAL_ACA}
-_STORAGE = {"fp64": capi.STORAGE_FP64, "fp32": capi.STORAGE_FP32}
+_STORAGE = {"fp64": capi.STORAGE_FP64, "fp32": capi.STORAGE_FP32, "fp32_fp16": capi.STORAGE_FP32_FP16,
+            "fp32_bf16": capi.STORAGE_FP32_BF16}
+# bytes per stored entry of (class 0, class 1)
+_BYTES = {"fp64": (8, 8), "fp32": (4, 4), "fp32_fp16": (4, 2), "fp32_bf16": (4, 2)}
 
 
 class HMatrix:
@@ -45,6 +48,9 @@ class HMatrix:
         self.minclustersize = int(minclustersize)
         self.maxblocksize = 1e6
         self.compressor = capi.SYM_PARTIAL_ACA  # the reference's default (hmatrix.hpp:297-299)
+        self.recompress_tol = 0.0
+        self.lowprec_tol = 0.0
+        self.lowprec_all = False
         self._h = None
         self._storage = "fp64"
 
@@ -63,6 +69,17 @@ class HMatrix:
     def set_compression(self, name: str):
         self.compressor = _COMPRESSORS[name]
 
+    # build-time options of the mixed-precision product (no reference counterpart)
+    def set_recompression(self, tol: float):
+        """FP64 QR+SVD recompression of every low-rank leaf, dropping a Frobenius tail <= tol (relative)."""
+        self.recompress_tol = float(tol)
+
+    def set_lowprec(self, tol: float = 0.0, all_terms: bool = False):
+        """Assign trailing low-rank terms to 16-bit storage within an error budget of tol (relative,
+        per leaf; needs recompression), or every low-rank term when all_terms."""
+        self.lowprec_tol = float(tol)
+        self.lowprec_all = bool(all_terms)
+
     # -- assembly ---------------------------------------------------------------
     def build(self, gen: Generator, xt: np.ndarray, xs: np.ndarray | None = None) -> "HMatrix":
         """HMatrix::build(A, xt, xs) (include/htool/types/hmatrix.hpp:285-341)."""
@@ -72,7 +89,7 @@ class HMatrix:
         if not same:
             xs = np.ascontiguousarray(xs, dtype=np.float64)
         prm = capi.htg_params(self.epsilon, self.eta, self.maxblocksize, self.minclustersize, self.compressor,
-                              self.reqrank, 0)
+                              self.reqrank, int(self.lowprec_all), self.recompress_tol, self.lowprec_tol)
         g = gen._c(same)
         h = C.c_void_p()
         capi.check(L.htg_hmatrix_build(C.byref(prm), C.byref(g), xt.shape[0], capi.ptr(xt),
@@ -192,11 +209,12 @@ class HMatrix:
         return (leaves, data) if with_data else leaves
 
     def algorithmic_bytes(self, mu: int = 1, storage: str | None = None) -> int:
-        """Bytes one product must move: every stored entry once, x read and y
-        written once, the two permutations."""
+        """Bytes one product must move: every stored entry once (at its storage
+        width), x read and y written once, the two permutations."""
         i = self.get_info()
-        s = 4 if (storage or self._storage) == "fp32" else 8
-        return int(s * (i.dense_entries + i.lowrank_entries) + mu * (i.nc + i.nr) * 8 + 4 * (i.nr + i.nc))
+        s0, s1 = _BYTES[storage or self._storage]
+        stored = s0 * (i.dense_entries + i.lowrank_entries - i.lowprec_entries) + s1 * i.lowprec_entries
+        return int(stored + mu * (i.nc + i.nr) * 8 + 4 * (i.nr + i.nc))
 
     def flops(self, mu: int = 1) -> int:
         i = self.get_info()

@@ -50,11 +50,16 @@ typedef enum htg_compressor {
     HTG_SYM_PARTIAL_ACA = 1  /* include/htool/lrmat/sympartialACA.hpp (the reference default) */
 } htg_compressor;
 
-/* Storage/arithmetic of the leaf products. Accumulation over leaves and the
- * output are always FP64. */
+/* Storage/arithmetic of the leaf products. Leaf entries are in two precision
+ * classes: class 0 (dense leaves and the leading low-rank terms) and class 1
+ * (the low-rank terms assigned to 16-bit storage at build time, see
+ * htg_params.lowprec_tol / lowprec_all; empty otherwise). Arithmetic inside a
+ * tile is FP32 except in FP64 mode; the sum over tiles and the output are FP64. */
 typedef enum htg_storage {
-    HTG_STORAGE_FP64 = 0, /* FP64 leaves, FP64 arithmetic */
-    HTG_STORAGE_FP32 = 1  /* FP32 leaves, FP32 arithmetic inside a leaf tile */
+    HTG_STORAGE_FP64      = 0, /* class 0 and 1 in FP64 */
+    HTG_STORAGE_FP32      = 1, /* class 0 and 1 in FP32 */
+    HTG_STORAGE_FP32_FP16 = 2, /* class 0 FP32, class 1 FP16 (per-term scaled) */
+    HTG_STORAGE_FP32_BF16 = 3  /* class 0 FP32, class 1 BF16 (per-term scaled) */
 } htg_storage;
 
 typedef struct htg_generator {
@@ -74,7 +79,12 @@ typedef struct htg_params {
     int minclustersize; /* default 10 */
     int compressor;     /* htg_compressor */
     int reqrank;        /* -1: epsilon-driven ranks; >0: fixed rank */
-    int reserved;
+    int lowprec_all;    /* 1: every low-rank term in class 1 (16-bit storage) */
+    double recompress_tol; /* > 0: FP64 QR+SVD recompression of each low-rank leaf, dropping
+                              singular values whose Frobenius tail is <= tol relative (no
+                              counterpart in the reference); 0: keep the ACA factors */
+    double lowprec_tol;    /* > 0 (with recompression): trailing terms go to class 1 while
+                              2 * 2^-11 * sum(sigma_j) <= lowprec_tol * ||sigma||_2 */
 } htg_params;
 
 typedef struct htg_info {
@@ -85,6 +95,9 @@ typedef struct htg_info {
     double compression_ratio; /* nr*nc / stored entries, include/htool/types/hmatrix.hpp:1604-1632 */
     int64_t dense_entries;    /* sum of m*n over dense leaves */
     int64_t lowrank_entries;  /* sum of k*(m+n) over low-rank leaves */
+    int64_t lowprec_entries;  /* the part of lowrank_entries stored in class 1 */
+    int64_t lowrank_u_entries;/* sum of k*m (the U part of lowrank_entries) */
+    int64_t lowprec_u_entries;/* the U part of lowprec_entries */
     int n_row_blocks, n_columns, n_vtiles;
     double build_seconds;     /* block tree + assembly */
 } htg_info;

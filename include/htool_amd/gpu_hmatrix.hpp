@@ -45,7 +45,9 @@ class GpuHMatrix {
         prm_.minclustersize = 10;
         prm_.compressor     = HTG_SYM_PARTIAL_ACA; // the reference's default (hmatrix.hpp:297-299)
         prm_.reqrank        = reqrank;
-        prm_.reserved       = 0;
+        prm_.lowprec_all    = 0;
+        prm_.recompress_tol = 0.0;
+        prm_.lowprec_tol    = 0.0;
     }
     GpuHMatrix(const GpuHMatrix &) = delete;
     GpuHMatrix &operator=(const GpuHMatrix &) = delete;
@@ -65,6 +67,12 @@ class GpuHMatrix {
         prm_.minclustersize = (int)m;
     }
     void set_compression(htg_compressor c) { prm_.compressor = c; }
+    // Build-time options of the mixed-precision product (no reference counterpart).
+    void set_recompression(double tol) { prm_.recompress_tol = tol; }
+    void set_lowprec(double tol, bool all_terms = false) {
+        prm_.lowprec_tol = tol;
+        prm_.lowprec_all = all_terms ? 1 : 0;
+    }
     void set_storage(htg_storage s) { check(htg_hmatrix_set_storage(handle(), s)); }
 
     // HMatrix::build(A, xt, xs), hmatrix.hpp:285-341 (points interleaved xyz).

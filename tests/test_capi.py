@@ -27,7 +27,7 @@ def test_status_strings(htg):
 def test_invalid_arguments_rejected_without_device(htg):
     from htool_amd import capi
     L = capi.lib()
-    prm = capi.htg_params(1e-6, 10.0, 1e6, 0, 0, -1, 0)  # minclustersize 0
+    prm = capi.htg_params(1e-6, 10.0, 1e6, 0, 0, -1, 0, 0.0, 0.0)  # minclustersize 0
     gen = capi.htg_generator(0, 1, 0.0, 0.0)
     pts = np.zeros((4, 3))
     h = C.c_void_p()

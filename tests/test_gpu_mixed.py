@@ -1,0 +1,109 @@
+"""GPU: FP64 recompression and the mixed-precision storage classes.
+
+Recompression (QR + SVD of each low-rank leaf, no counterpart in the
+reference) is checked three ways: the product of the recompressed H-matrix
+equals the oracle's product of the same exported leaves (1e-12, FP64); the kept
+ranks match a numpy SVD truncation of the ACA leaves at the same tolerance
+(within one term, where a singular value sits on the threshold); the
+H-matrix stays within 10 epsilon of dense FP64. Mixed storage (FP32 terms plus
+FP16 trailing terms) is checked against the FP64 product of the same leaves
+within the error budget it was built with.
+"""
+import numpy as np
+import pytest
+
+from htool_amd import geometry as geo
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def sphere_case(n=8000, seed=2):
+    x = geo.sphere(n, seed=seed)
+    return x, geo.laplace_diag(geo.sphere_spacing(n))
+
+
+def build(htg, x, diag, rc=0.0, lp=0.0, lp_all=False):
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32)
+    H.set_compression("partialACA")
+    H.set_recompression(rc)
+    H.set_lowprec(lp, lp_all)
+    H.build(htg.Generator(geo.KERNEL_LAPLACE, 0.0, diag), x)
+    return H
+
+
+def test_recompressed_product_matches_oracle_on_same_leaves(htg, orc):
+    x, diag = sphere_case()
+    H = build(htg, x, diag, rc=2e-6)
+    leaves, data = H.get_leaves(with_data=True)
+    pt, ps = H.get_perms()
+    v = np.random.default_rng(1).standard_normal((x.shape[0], 3))
+    assert rel(H.mvprod_global_to_global(v), orc.leaf_mvprod(len(pt), len(ps), pt, ps, leaves, data, v)) < 1e-12
+    rows = np.random.default_rng(2).choice(x.shape[0], 500, replace=False).astype(np.int32)
+    yd = orc.dense_rows(x, None, rows, v[:, 0], kernel=geo.KERNEL_LAPLACE, diag=diag, same=True)
+    assert rel(H.mvprod_global_to_global(v[:, 0])[rows], yd) < 1e-5
+
+
+def test_recompression_ranks_match_numpy_svd(htg, orc):
+    x, diag = sphere_case(6000, 3)
+    tol = 2e-6
+    H0 = build(htg, x, diag)
+    H1 = build(htg, x, diag, rc=tol)
+    L0, D0 = H0.get_leaves(with_data=True)
+    L1 = H1.get_leaves()
+    assert np.array_equal(L0[:, :4], L1[:, :4])  # same tree, same leaves
+    off = 0
+    checked = 0
+    for (oi, m, oj, n, k, a), k1 in zip(orc.iter_leaf_data(L0, D0), L1[:, 4]):
+        if k < 0:
+            continue
+        if 2 <= k <= 48:
+            s = np.linalg.svd(a[0] @ a[1], compute_uv=False)[:k]
+            tail = np.sqrt(np.cumsum((s ** 2)[::-1]))[::-1]
+            r = k
+            for j in range(1, k):
+                if tail[j] <= tol * np.sqrt((s ** 2).sum()):
+                    r = j
+                    break
+            assert abs(int(k1) - r) <= 1, (oi, oj, k, k1, r)
+            checked += 1
+        else:
+            assert k1 == k
+    assert checked > 100
+    assert H1.get_info().lowrank_entries < H0.get_info().lowrank_entries
+
+
+@pytest.mark.parametrize("storage,bound", [("fp32_fp16", 1e-6), ("fp32_bf16", 8e-6)])
+def test_mixed_storage_within_budget(htg, orc, storage, bound):
+    x, diag = sphere_case()
+    H = build(htg, x, diag, rc=2e-6, lp=5e-7)
+    info = H.get_info()
+    assert 0 < info.lowprec_entries < info.lowrank_entries
+    v = np.random.default_rng(4).standard_normal(x.shape[0])
+    y64 = H @ v
+    H.set_storage(storage)
+    y = H @ v
+    assert rel(y, y64) < bound
+    H.set_storage("fp32")
+    assert rel(H @ v, y64) < 1e-6
+
+
+def test_all_lowrank_terms_16bit(htg, orc):
+    """Every low-rank term in BF16/FP16 (per-term scaled): error stays at the 16-bit level."""
+    x, diag = sphere_case(6000, 5)
+    H = build(htg, x, diag, rc=1e-6, lp_all=True)
+    info = H.get_info()
+    assert info.lowprec_entries == info.lowrank_entries
+    v = np.random.default_rng(6).standard_normal(x.shape[0])
+    y64 = H @ v
+    for storage, bound in (("fp32_fp16", 2e-3), ("fp32_bf16", 2e-2)):
+        H.set_storage(storage)
+        assert rel(H @ v, y64) < bound
+    H.set_storage("fp64")
+    X = np.random.default_rng(7).standard_normal((x.shape[0], 8))
+    Y64 = H.mvprod_global_to_global(X)
+    H.set_storage("fp32_fp16")
+    assert rel(H.mvprod_global_to_global(X), Y64) < 2e-3
