@@ -184,9 +184,12 @@ def main():
         ms32 = timed(lambda: M.mvprod_device(dX.data_ptr(), dY.data_ptr(), 32, sp), max(5, args.steps // 5),
                      2, stream, torch)
         prof = M.profile(dx.data_ptr(), dy.data_ptr(), 1, 20, sp)
-        log(f"{name}: {ms1:.4f} ms/matvec, {ms32:.4f} ms/32-RHS product")
+        prof32 = M.profile(dX.data_ptr(), dY.data_ptr(), 32, 5, sp)
+        log(f"{name}: {ms1:.4f} ms/matvec, {ms32:.4f} ms/32-RHS product "
+            f"(V.x {prof32['vx_ms']:.4f} ms, rows {prof32['row_apply_ms']:.4f} ms)")
         res[name] = {"ms": ms1, "ms_mu32": ms32, "bytes": M.algorithmic_bytes(1, storage),
-                     "bytes_mu32": M.algorithmic_bytes(32, storage), "flops_mu32": M.flops(32), "prof": prof}
+                     "bytes_mu32": M.algorithmic_bytes(32, storage), "flops_mu32": M.flops(32), "prof": prof,
+                     "prof_mu32": prof32}
 
     # accuracy: against dense FP64 rows and against the FP64 H product
     from oracle import lib as orc
@@ -224,6 +227,7 @@ def main():
         return {"ms": round(r["ms"], 4), "gbps": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 2),
                 "bytes": r["bytes"], "ms_mu32": round(r["ms_mu32"], 4),
                 "tflops_mu32": round(r["flops_mu32"] / (r["ms_mu32"] * 1e-3) / 1e12, 3),
+                "kernels_ms_mu32": {k: round(v, 5) for k, v in r["prof_mu32"].items()},
                 "rel_err_vs_dense": err[name], "rel_err_vs_fp64_hmat": err_vs_fp64[name]}
 
     line = {

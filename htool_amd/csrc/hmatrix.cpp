@@ -6,8 +6,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace htg {
@@ -20,6 +22,7 @@ constexpr int ALIGN        = 32;   // block alignment in elements
 constexpr int VCHUNK_EL    = 8192; // target entries per V column chunk
 constexpr int VGROUP_EL    = 2048; // target entries per phase-1 wavefront
 constexpr int VGROUP_TILES = 8;    // most V chunks per phase-1 wavefront
+constexpr int VROWS        = 32;   // stacked V rows per chunk (one matrix-core tile)
 constexpr int CLASS_ALIGN[2] = {4, 8}; // entries per aligned group: class 1 may hold 16-bit values
 constexpr int CAP_MAX        = 512;    // must match the ACA kernel's CAPMAX
 
@@ -63,7 +66,8 @@ struct htg_hmatrix {
     std::vector<int> rb_row0, rb_order;
     // per leaf, per covered row block: (row block, class-0 tile offset, class-1 tile offset)
     std::vector<std::vector<std::tuple<int, uint64_t, uint64_t>>> leaf_tiles;
-    std::vector<int> t_off;
+    std::vector<std::vector<int>> panels; // low-rank leaves per source cluster
+    std::vector<int> t_first[2];          // per class, per leaf: first t entry (-1 if none)
     std::vector<double> tscale; // per z entry (x part unused)
     int t_size = 0, part_size = 0;
     ClassLayout cls[2];
@@ -77,11 +81,19 @@ struct htg_hmatrix {
     DevArray<double> d_tscale;
     DevArray<double> d_z, d_part; // z: cluster-ordered x (tpad entries) then t, per RHS
     DevArray<double> d_hx, d_hy;  // staging for host-pointer products
+    DevArray<int> d_iperm_s;      // inverse column permutation (32-RHS gather)
+    DevArray<float> d_xstage;     // 32-RHS gather staging, nc x 32
     int storage = HTG_STORAGE_FP64;
 };
 
 namespace htg {
 namespace {
+
+// z entry of term j of low-rank leaf l.
+int t_entry(const htg_hmatrix &h, int l, int j) {
+    const int kh = h.leaves[l].k_hi;
+    return j < kh ? h.tpad + h.t_first[0][l] + j : h.tpad + h.t_first[1][l] + (j - kh);
+}
 
 // Product layout.
 //  * Row blocks: the segments between consecutive leaf row boundaries (every
@@ -92,8 +104,10 @@ namespace {
 //    the low-rank leaves' leading k_hi U columns; the class-1 block holds the
 //    remaining U columns. colidx gives, per column, the z entry it multiplies.
 //  * Each low-rank leaf stores its V rows (class 0: the first k_hi, class 1 the
-//    rest) row-major and aligned to the global column index; columns are cut
-//    into chunks of about VCHUNK_EL entries, grouped into wavefront work lists.
+//    rest) row-major and aligned to the global column index, stacked with the
+//    other leaves of its source cluster (a panel); panel rows are cut into
+//    groups of VROWS and columns into chunks of about VCHUNK_EL entries,
+//    grouped into wavefront work lists.
 void build_layout(htg_hmatrix &h) {
     std::vector<int> bnd = {0, h.nr};
     for (const Leaf &L : h.leaves) {
@@ -113,22 +127,54 @@ void build_layout(htg_hmatrix &h) {
     const int nrb = (int)h.rb_row0.size() - 1;
     h.tpad        = round_up(h.nc, 8);
 
-    // t entries (one per low-rank term) and their scales.
+    // Low-rank leaves sharing a source cluster (same column range) form one
+    // panel per class: their V rows are stacked with a common stride, so a
+    // phase-1 chunk covers rows of several leaves and reads its x slice once.
+    std::map<std::pair<int, int>, int> panel_of;
+    h.panels.clear();
+    for (size_t l = 0; l < h.leaves.size(); l++) {
+        const Leaf &L = h.leaves[l];
+        if (L.k <= 0)
+            continue;
+        auto it = panel_of.emplace(std::make_pair(L.oj, L.n), (int)panel_of.size()).first;
+        if (it->second == (int)h.panels.size())
+            h.panels.emplace_back();
+        h.panels[it->second].push_back((int)l);
+    }
+    // panels in column order
+    std::vector<int> porder;
+    for (const auto &kv : panel_of)
+        porder.push_back(kv.second);
+    {
+        std::vector<std::vector<int>> sorted;
+        for (int p : porder)
+            sorted.push_back(std::move(h.panels[p]));
+        h.panels.swap(sorted);
+    }
+
+    // t entries (one per low-rank term): class by class, panel by panel, leaf
+    // by leaf, so the class-c terms of a panel are contiguous. Scales per entry.
     h.t_size = 0;
-    h.t_off.assign(h.leaves.size(), -1);
     h.scaled = false;
-    for (size_t l = 0; l < h.leaves.size(); l++)
-        if (h.leaves[l].k > 0) {
-            h.t_off[l] = h.t_size;
-            h.t_size += h.leaves[l].k;
-            h.scaled = h.scaled || !h.leaves[l].sig.empty();
-        }
+    for (int c = 0; c < 2; c++) {
+        h.t_first[c].assign(h.leaves.size(), -1);
+        for (const auto &P : h.panels)
+            for (int l : P) {
+                const int kc = terms(h.leaves[l], c);
+                if (kc > 0) {
+                    h.t_first[c][l] = h.t_size;
+                    h.t_size += kc;
+                }
+            }
+    }
+    for (const Leaf &L : h.leaves)
+        h.scaled = h.scaled || !L.sig.empty();
     h.tscale.clear();
     if (h.scaled) {
         h.tscale.assign(h.tpad + h.t_size, 1.0);
         for (size_t l = 0; l < h.leaves.size(); l++)
-            for (size_t j = 0; j < h.leaves[l].sig.size(); j++)
-                h.tscale[h.tpad + h.t_off[l] + j] = h.leaves[l].sig[j];
+            for (int j = 0; j < (int)h.leaves[l].sig.size(); j++)
+                h.tscale[t_entry(h, (int)l, j)] = h.leaves[l].sig[j];
     }
 
     // Leaves of each row block, in leaf order.
@@ -179,9 +225,9 @@ void build_layout(htg_hmatrix &h) {
                 col[0] += L.n;
             } else {
                 for (int j = 0; j < L.k_hi; j++)
-                    h.cls[0].colidx.push_back(h.tpad + h.t_off[l] + j);
+                    h.cls[0].colidx.push_back(t_entry(h, l, j));
                 for (int j = L.k_hi; j < L.k; j++)
-                    h.cls[1].colidx.push_back(h.tpad + h.t_off[l] + j);
+                    h.cls[1].colidx.push_back(t_entry(h, l, j));
                 col[0] += L.k_hi;
                 col[1] += L.k - L.k_hi;
             }
@@ -204,7 +250,9 @@ void build_layout(htg_hmatrix &h) {
     };
     std::stable_sort(h.rb_order.begin(), h.rb_order.end(), [&](int a, int b) { return work(a) > work(b); });
 
-    // V rows and their column chunks, class 0 then class 1, grouped per wavefront.
+    // V rows and their chunks, class 0 then class 1, grouped per wavefront. A
+    // panel's stacked rows are cut into groups of VROWS rows, each group's
+    // columns into chunks of about VCHUNK_EL entries.
     h.part_size = 0;
     h.vtiles.clear();
     h.multi_first.clear();
@@ -215,43 +263,61 @@ void build_layout(htg_hmatrix &h) {
         C.varena_size  = 0;
         C.nvtiles      = 0;
         C.leaf_v_off.assign(h.leaves.size(), 0);
-        for (size_t l = 0; l < h.leaves.size(); l++) {
-            const Leaf &L = h.leaves[l];
-            const int kc  = terms(L, c);
-            if (kc <= 0)
-                continue;
-            const int ld     = v_stride(L, c);
-            C.leaf_v_off[l]  = C.varena_size;
-            C.varena_size    = align_up(C.varena_size + (uint64_t)kc * ld);
-            const int cpc    = round_up(std::max(64, VCHUNK_EL / kc), CLASS_ALIGN[c]);
-            const int nparts = (ld + cpc - 1) / cpc;
-            const int pbase  = h.part_size;
-            if (nparts > 1)
-                h.part_size += nparts * kc;
-            for (int q = 0; q < nparts; q++) {
-                VTile v{};
-                const int j0 = q * cpc;
-                v.v_off      = C.leaf_v_off[l] + (uint64_t)j0;
-                v.ld         = ld;
-                v.k          = kc;
-                v.ncols      = std::min(cpc, ld - j0);
-                v.x_off      = v_first(L, c) + j0;
-                v.t_index    = h.tpad + h.t_off[l] + (c == 0 ? 0 : L.k_hi);
-                v.part_off   = pbase + q * kc;
-                v.part_base  = pbase;
-                v.nparts     = nparts;
-                v.cls        = c;
-                if (nparts > 1 && q == 0)
-                    h.multi_first.push_back((int)h.vtiles.size());
-                const int64_t el   = (int64_t)v.k * v.ncols;
-                const int in_group = (int)h.vtiles.size() - h.vg_off.back();
-                if (in_group > 0 && (group_el + el > VGROUP_EL || in_group >= VGROUP_TILES)) {
-                    h.vg_off.push_back((int)h.vtiles.size());
-                    group_el = 0;
+        for (const auto &P : h.panels) {
+            int K = 0, first = -1;
+            for (int l : P)
+                if (terms(h.leaves[l], c) > 0) {
+                    if (first < 0)
+                        first = l;
+                    K += terms(h.leaves[l], c);
                 }
-                h.vtiles.push_back(v);
-                C.nvtiles++;
-                group_el += el;
+            if (K == 0)
+                continue;
+            const Leaf &L0       = h.leaves[first];
+            const int ld         = v_stride(L0, c);
+            const uint64_t base  = C.varena_size;
+            int row              = 0;
+            for (int l : P) {
+                const int kc = terms(h.leaves[l], c);
+                if (kc > 0) {
+                    C.leaf_v_off[l] = base + (uint64_t)row * ld;
+                    row += kc;
+                }
+            }
+            C.varena_size    = align_up(base + (uint64_t)K * ld);
+            const int tfirst = t_entry(h, first, c == 0 ? 0 : L0.k_hi);
+            for (int g0 = 0; g0 < K; g0 += VROWS) {
+                const int kr     = std::min(VROWS, K - g0);
+                const int cpc    = round_up(std::max(64, VCHUNK_EL / kr), CLASS_ALIGN[c]);
+                const int nparts = (ld + cpc - 1) / cpc;
+                const int pbase  = h.part_size;
+                if (nparts > 1)
+                    h.part_size += nparts * kr;
+                for (int q = 0; q < nparts; q++) {
+                    VTile v{};
+                    const int j0 = q * cpc;
+                    v.v_off      = base + (uint64_t)g0 * ld + (uint64_t)j0;
+                    v.ld         = ld;
+                    v.k          = kr;
+                    v.ncols      = std::min(cpc, ld - j0);
+                    v.x_off      = v_first(L0, c) + j0;
+                    v.t_index    = tfirst + g0;
+                    v.part_off   = pbase + q * kr;
+                    v.part_base  = pbase;
+                    v.nparts     = nparts;
+                    v.cls        = c;
+                    if (nparts > 1 && q == 0)
+                        h.multi_first.push_back((int)h.vtiles.size());
+                    const int64_t el   = (int64_t)v.k * v.ncols;
+                    const int in_group = (int)h.vtiles.size() - h.vg_off.back();
+                    if (in_group > 0 && (group_el + el > VGROUP_EL || in_group >= VGROUP_TILES)) {
+                        h.vg_off.push_back((int)h.vtiles.size());
+                        group_el = 0;
+                    }
+                    h.vtiles.push_back(v);
+                    C.nvtiles++;
+                    group_el += el;
+                }
             }
         }
     }
@@ -296,8 +362,10 @@ int row_stride(const htg_hmatrix &h, int c, int rb) {
     return round_up(h.cls[c].rb_col0[rb + 1] - h.cls[c].rb_col0[rb], CLASS_ALIGN[c]);
 }
 
-int pick_mu_chunk(int remaining) {
-    int c = 64;
+// Right-hand sides per pass: up to 64, or 32 for FP32 arithmetic, whose
+// 32-wide passes run on the matrix cores.
+int pick_mu_chunk(int remaining, int cap) {
+    int c = cap;
     while (c > remaining)
         c >>= 1;
     return c;
@@ -323,10 +391,21 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
     const RowClass rc1{h.cls[1].d_rb_col0.p, h.cls[1].d_rb_aoff.p, h.cls[1].d_colidx.p, CLASS_ALIGN[1]};
     int done = 0;
     while (done < mu) {
-        const int MU = pick_mu_chunk(mu - done);
+        const int MU = pick_mu_chunk(mu - done, std::is_same<TC, float>::value ? 32 : 64);
         if (ev)
             HTG_HIP(hipEventRecord(ev[0], st));
-        launch_gather_x<TC>(d_x, h.d_perm_s.p, h.nc, done, MU, z, st);
+        if (std::is_same<TC, float>::value && MU == 32) {
+            if (!h.d_iperm_s.p) {
+                std::vector<int> ip(h.nc);
+                for (int j = 0; j < h.nc; j++)
+                    ip[h.perm_s[j]] = j;
+                h.d_iperm_s.upload(ip);
+                h.d_xstage.alloc((size_t)h.nc * 32);
+            }
+            launch_gather_x32(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, reinterpret_cast<float *>(z), st);
+        } else {
+            launch_gather_x<TC>(d_x, h.d_perm_s.p, h.nc, h.tpad, done, MU, z, st);
+        }
         if (ev)
             HTG_HIP(hipEventRecord(ev[1], st));
         const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];

@@ -6,6 +6,7 @@
 #include "kernels.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace htg {
 namespace {
@@ -617,15 +618,61 @@ __global__ void __launch_bounds__(WG) k_recompress(const RcLeaf *leaves, double 
 
 
 // ---------------------------------------------------------------------------
-// Product phase 0: x (nc x mu column-major, original order) -> z[0, nc*MU)
-// (cluster order, RHS-interleaved, compute type) for RHS [mu_base, mu_base+MU).
-template <typename TC, int MU>
-__global__ void k_gather_x(const double *x, const int *perm_s, int nc, int mu_base, TC *z) {
+// Product phase 0: x (nc x mu column-major, original order) -> x part of z
+// (cluster order, compute type) for RHS [mu_base, mu_base+MU). Entries
+// [nc, tpad) are written as zeros on every call: they meet the zero padding of
+// V rows, and must hold finite values whatever an earlier product of another
+// width or type left in the buffer.
+// Layouts of z: RHS-interleaved, entry j of RHS s at j*MU + s; or, for the
+// 32-wide matrix-core passes, blocked by 4 entries: zblk(j, s).
+__device__ __forceinline__ size_t zblk(size_t j, int s) { return (j >> 2) * 128 + (size_t)s * 4 + (j & 3); }
+
+template <typename TC, int MU, bool BLK>
+__global__ void k_gather_x(const double *x, const int *perm_s, int nc, int tpad, int mu_base, TC *z) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nc * MU)
+    if (e >= tpad * MU)
         return;
-    const int j = e / MU, s = e % MU;
-    z[e] = (TC)x[perm_s[j] + (size_t)nc * (mu_base + s)];
+    int j, s;
+    if constexpr (BLK) {
+        j = (e >> 7) * 4 + (e & 3);
+        s = (e >> 2) & 31;
+    } else {
+        j = e / MU;
+        s = e % MU;
+    }
+    z[e] = j < nc ? (TC)x[perm_s[j] + (size_t)nc * (mu_base + s)] : (TC)0;
+}
+
+// The same for a 32-wide blocked pass, in two coalesced steps: (1) RHS s of
+// original entry i goes to row iperm[i] of an RHS-interleaved staging copy
+// (consecutive i read consecutive x; each thread writes 32 contiguous bytes);
+// (2) the staging copy is reordered into the blocked layout, zeros from nc on.
+__global__ void k_scatter_x32(const double *x, const int *iperm_s, int nc, int mu_base, float *stage) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    const int s0 = 8 * blockIdx.y;
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+        v[u] = (float)x[i + (size_t)nc * (mu_base + s0 + u)];
+    float4 *o = reinterpret_cast<float4 *>(stage + (size_t)iperm_s[i] * 32 + s0);
+    o[0]      = make_float4(v[0], v[1], v[2], v[3]);
+    o[1]      = make_float4(v[4], v[5], v[6], v[7]);
+}
+
+__global__ void k_block_x32(const float *stage, int nc, int tpad, float *z) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x; // (group, RHS)
+    if (e >= tpad * 8)
+        return;
+    const int g = e >> 5, s = e & 31;
+    float v[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int j = 4 * g + c;
+        v[c]        = j < nc ? stage[(size_t)j * 32 + s] : 0.0f;
+    }
+    reinterpret_cast<float4 *>(z)[e] = make_float4(v[0], v[1], v[2], v[3]);
 }
 
 // 16-byte vector loads of the stored entries.
@@ -933,18 +980,19 @@ __global__ void __launch_bounds__(WG) k_vx(const int *wv_off, int nwaves, const 
 }
 
 // Adds the chunk partial sums of each multi-chunk leaf into its t entries.
-template <typename TC, int MU>
+template <typename TC, int MU, bool BLK = false>
 __global__ void __launch_bounds__(WG) k_vx_sum(const VTile *tiles, const int *first_chunk, const double *part, TC *z,
                                                const double *tscale) {
     const VTile d    = tiles[first_chunk[blockIdx.x]];
     const int km     = d.k * MU;
     const double *pb = part + (size_t)d.part_base * MU;
-    TC *o            = z + (size_t)d.t_index * MU;
     for (int e = threadIdx.x; e < km; e += WG) {
         double v = 0;
         for (int c = 0; c < d.nparts; c++)
             v += pb[(size_t)c * km + e];
-        o[e] = (TC)(tscale ? v * tscale[d.t_index + e / MU] : v);
+        const int r = e / MU, s = e % MU;
+        z[BLK ? zblk(d.t_index + r, s) : (size_t)(d.t_index + r) * MU + s] =
+            (TC)(tscale ? v * tscale[d.t_index + r] : v);
     }
 }
 
@@ -993,6 +1041,377 @@ __global__ void __launch_bounds__(WG) k_rows(const int *rb_order, const int *rb_
     for (int e = threadIdx.x; e < R * MU; e += WG) {
         const int r = e / MU, s = e % MU;
         y[(size_t)perm_t[r0 + r] + (size_t)nr * (mu_base + s)] = s_acc[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 32-RHS products on the matrix cores (FP32 MFMA, wave64). The reduction index
+// of an MFMA is a column of the stored block: lane l feeds one row and one
+// column of the A side and the same column for one right-hand side on the B
+// side. Each lane reads VW consecutive columns of its row with one 16-byte
+// load; every load of a pass is issued before the pass's first MFMA, so a wave
+// waits for memory once per pass. Out-of-range rows and columns read a valid
+// address and are masked to zero (columns) or never written (rows). Products
+// accumulate in FP32 within a pass, then in FP64.
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int MFMA_MU    = 32;  // right-hand sides of one matrix-core pass
+constexpr int RPASS_COLS = 128; // row-block columns staged in LDS per pass
+constexpr int VPASS16 = 64; // V columns per pass, 16-row tiles
+constexpr int VPASS32 = 32; // V columns per pass, 32-row tiles
+
+// Output row of accumulator i of lane l in a 32x32 tile (v_mfma_f32_32x32x2f32).
+__device__ __forceinline__ int mfma32_row(int i, int lane) { return (i & 3) + 4 * (lane >> 5) + 8 * (i >> 2); }
+
+// Phase 2, one precision class of a row block. The workgroup's waves split as
+// T row tiles of 32 rows times S column slices (T S <= 4): wave (t, q) owns
+// rows 32t..32t+31 and columns [q W, (q+1) W) of every 128-column pass, W =
+// 128 / S. Lane l takes row l % 32 and, in MFMA m of step it, column
+// q W + 2 VW it + VW (l / 32) + m. The vector entries of a pass are staged in
+// LDS (blocked by 4 columns); the next pass's entries and matrix rows are
+// loaded while the current pass runs on the matrix cores.
+template <typename TS, int S>
+__device__ __forceinline__ void rows_class_mfma(const TS *arena, const RowClass &rc, int rb, int R, const float *z,
+                                                float *s_z, int t, int q, bool active, double (&accd)[16]) {
+    constexpr int MU   = MFMA_MU;
+    constexpr int VW   = Vec<TS>::W;
+    constexpr int WCOL = RPASS_COLS / S;
+    constexpr int NIT  = WCOL / (2 * VW);
+    constexpr int ZPT  = RPASS_COLS * MU / WG; // staged entries per thread
+    constexpr bool PREFETCH_A = S > 1;      // four row tiles: registers too few for a second copy
+    const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
+    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+    if (Cp == 0)
+        return;
+    const TS *A    = arena + rc.rb_aoff[rb];
+    const int lane = threadIdx.x & 63, half = lane >> 5;
+    const int row  = 32 * t + (lane & 31);
+    const TS *rowp = A + (size_t)(row < R ? row : 0) * Cp + q * WCOL;
+    const int zs = threadIdx.x & (MU - 1), zc = threadIdx.x / MU; // staging: RHS, first column
+    float v[ZPT];
+    TS a[NIT][VW], an[PREFETCH_A ? NIT : 1][VW];
+    auto load_z = [&](int cc) {
+        int idx[ZPT];
+#pragma unroll
+        for (int i = 0; i < ZPT; i++) {
+            const int c = cc + zc + (WG / MU) * i;
+            idx[i]      = rc.colidx[cb + (c < C ? c : 0)];
+        }
+#pragma unroll
+        for (int i = 0; i < ZPT; i++)
+            v[i] = z[zblk(idx[i], zs)];
+    };
+    auto load_a = [&](int cc, auto &dst) {
+        const int nz = min(RPASS_COLS, Cp - cc);
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int kk = q * WCOL + 2 * VW * it + VW * half;
+            Vec<TS>::load(rowp + cc + (kk < nz ? kk - q * WCOL : -q * WCOL), dst[it]);
+        }
+    };
+    load_z(0);
+    if (active)
+        load_a(0, a);
+    for (int cc = 0; cc < Cp; cc += RPASS_COLS) {
+        const int nz = min(RPASS_COLS, Cp - cc);
+#pragma unroll
+        for (int i = 0; i < ZPT; i++) {
+            const int c      = zc + (WG / MU) * i;
+            s_z[zblk(c, zs)] = cc + c < C ? v[i] : 0.0f;
+        }
+        __syncthreads();
+        const bool more = cc + RPASS_COLS < Cp;
+        if (more) {
+            load_z(cc + RPASS_COLS);
+            if (PREFETCH_A && active)
+                load_a(cc + RPASS_COLS, an);
+        }
+        if (active) {
+            floatx16 acc = {0};
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                const int k0 = q * WCOL + 2 * VW * it;
+                if (k0 < nz) {
+                    const int kk = k0 + VW * half;
+#pragma unroll
+                    for (int m4 = 0; m4 < VW; m4 += 4) {
+                        const float4 b = *reinterpret_cast<const float4 *>(s_z + zblk(kk + m4, lane & 31));
+                        const float bb[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+                        for (int m = 0; m < 4; m++)
+                            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cvt<float>(a[it][m4 + m]), bb[m], acc, 0, 0,
+                                                                       0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                accd[i] += (double)acc[i];
+        }
+        __syncthreads();
+        if (more && active) {
+            if constexpr (PREFETCH_A) {
+#pragma unroll
+                for (int it = 0; it < NIT; it++)
+#pragma unroll
+                    for (int u = 0; u < VW; u++)
+                        a[it][u] = an[it][u];
+            } else {
+                load_a(cc + RPASS_COLS, a);
+            }
+        }
+    }
+}
+
+template <typename TS0, typename TS1, int S>
+__device__ __forceinline__ void rows_mfma_split(const int rb, const int R, const RowClass &c0, const TS0 *arena0,
+                                                const RowClass &c1, const TS1 *arena1, const float *z, float *s_z,
+                                                double (&accd)[16], int &q, bool &active) {
+    const int w = threadIdx.x >> 6, T = (R + 31) / 32;
+    const int t = w / S;
+    q           = w % S;
+    active      = t < T;
+    rows_class_mfma<TS0, S>(arena0, c0, rb, R, z, s_z, t, q, active, accd);
+    if (arena1)
+        rows_class_mfma<TS1, S>(arena1, c1, rb, R, z, s_z, t, q, active, accd);
+}
+
+template <typename TS0, typename TS1>
+__global__ void __launch_bounds__(WG) k_rows_mfma(const int *rb_order, const int *rb_row0, RowClass c0,
+                                                  const TS0 *arena0, RowClass c1, const TS1 *arena1, const float *z,
+                                                  const int *perm_t, int nr, int mu_base, double *y) {
+    // LDS: the staged pass (16 KB), then the column slices' partial row sums (24 KB)
+    __shared__ __align__(16) double smem[3 * 16 * 64];
+    float *s_z   = reinterpret_cast<float *>(smem);
+    const int rb = rb_order[blockIdx.x];
+    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int T = (R + 31) / 32;
+    const int S = T == 1 ? 4 : (T == 2 ? 2 : 1);
+    double accd[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        accd[i] = 0;
+    int q       = 0;
+    bool active = false;
+    if (S == 4)
+        rows_mfma_split<TS0, TS1, 4>(rb, R, c0, arena0, c1, arena1, z, s_z, accd, q, active);
+    else if (S == 2)
+        rows_mfma_split<TS0, TS1, 2>(rb, R, c0, arena0, c1, arena1, z, s_z, accd, q, active);
+    else
+        rows_mfma_split<TS0, TS1, 1>(rb, R, c0, arena0, c1, arena1, z, s_z, accd, q, active);
+    const int t = w / S;
+    if (S > 1) {
+        // every wave has passed the last barrier of the pass loop: the LDS is
+        // free. Slice q > 0 of tile t leaves its sums in slot t (S - 1) + q - 1.
+        if (active && q > 0)
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                smem[((t * (S - 1) + q - 1) * 16 + i) * 64 + lane] = accd[i];
+        __syncthreads();
+        if (active && q == 0)
+            for (int qq = 1; qq < S; qq++)
+#pragma unroll
+                for (int i = 0; i < 16; i++)
+                    accd[i] += smem[((t * (S - 1) + qq - 1) * 16 + i) * 64 + lane];
+    }
+    if (active && q == 0) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int orow = 32 * t + mfma32_row(i, lane);
+            if (orow < R)
+                y[(size_t)perm_t[r0 + orow] + (size_t)nr * (mu_base + (lane & 31))] = accd[i];
+        }
+    }
+}
+
+// Stores row r of a chunk's V x for right-hand side s (scaled when the leaf is
+// one chunk; a partial sum otherwise).
+__device__ __forceinline__ void vx_store_mfma(const VTile &d, int r, int s, double v, float *z, double *part,
+                                              const double *tscale) {
+    if (d.nparts > 1)
+        part[(size_t)(d.part_off + r) * MFMA_MU + s] = v;
+    else
+        z[zblk(d.t_index + r, s)] = (float)(tscale ? v * tscale[d.t_index + r] : v);
+}
+
+// Phase 1 for a chunk of at most 16 rows: v_mfma_f32_16x16x4f32 on a 16-row
+// tile, two MFMAs (right-hand sides 0-15 and 16-31) per A value; lane l takes
+// row l % 16 and, in MFMA m of step it, column 4 VW it + VW (l / 16) + m. The
+// next pass's V and x entries are loaded while the current pass runs.
+template <typename TS>
+__device__ __forceinline__ void vx_chunk_mfma16(const VTile &d, const TS *varena, float *z, double *part,
+                                                const double *tscale, int lane) {
+    constexpr int VW  = Vec<TS>::W;
+    constexpr int NIT = VPASS16 / (4 * VW);
+    const int g = lane >> 4, r = lane & 15;
+    const TS *rowp = varena + d.v_off + (size_t)(r < d.k ? r : 0) * d.ld;
+    TS a[NIT][VW], an[NIT][VW];
+    float4 b[NIT][VW / 4][2], bn[NIT][VW / 4][2];
+    auto load = [&](int c0, TS (&da)[NIT][VW], float4 (&db)[NIT][VW / 4][2]) {
+        const int nc = min(VPASS16, d.ncols - c0);
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int kk = 4 * VW * it + VW * g;
+            const int kc = kk < nc ? kk : 0;
+            Vec<TS>::load(rowp + c0 + kc, da[it]);
+#pragma unroll
+            for (int m4 = 0; m4 < VW / 4; m4++) {
+                const size_t j = (size_t)d.x_off + c0 + kc + 4 * m4;
+                db[it][m4][0]  = *reinterpret_cast<const float4 *>(z + zblk(j, r));
+                db[it][m4][1]  = *reinterpret_cast<const float4 *>(z + zblk(j, r + 16));
+            }
+        }
+    };
+    double accd[2][4] = {};
+    load(0, a, b);
+    for (int c0 = 0; c0 < d.ncols; c0 += VPASS16) {
+        const int nc    = min(VPASS16, d.ncols - c0);
+        const bool more = c0 + VPASS16 < d.ncols;
+        if (more)
+            load(c0 + VPASS16, an, bn);
+        floatx4 acc0 = {0}, acc1 = {0};
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            if (4 * VW * it < nc) {
+                const bool in = 4 * VW * it + VW * g < nc;
+#pragma unroll
+                for (int m4 = 0; m4 < VW / 4; m4++) {
+                    const float b0[4] = {b[it][m4][0].x, b[it][m4][0].y, b[it][m4][0].z, b[it][m4][0].w};
+                    const float b1[4] = {b[it][m4][1].x, b[it][m4][1].y, b[it][m4][1].z, b[it][m4][1].w};
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        const float av = in ? cvt<float>(a[it][4 * m4 + m]) : 0.0f;
+                        acc0           = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b0[m], acc0, 0, 0, 0);
+                        acc1           = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b1[m], acc1, 0, 0, 0);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            accd[0][i] += (double)acc0[i];
+            accd[1][i] += (double)acc1[i];
+        }
+        if (more) {
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+#pragma unroll
+                for (int u = 0; u < VW; u++)
+                    a[it][u] = an[it][u];
+#pragma unroll
+                for (int m4 = 0; m4 < VW / 4; m4++)
+                    b[it][m4][0] = bn[it][m4][0], b[it][m4][1] = bn[it][m4][1];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int row = 4 * g + i;
+        if (row < d.k) {
+            vx_store_mfma(d, row, r, accd[0][i], z, part, tscale);
+            vx_store_mfma(d, row, r + 16, accd[1][i], z, part, tscale);
+        }
+    }
+}
+
+// Phase 1 for a chunk of more than 16 rows: v_mfma_f32_32x32x2f32 over 32-row
+// tiles, next pass prefetched as above.
+template <typename TS>
+__device__ __forceinline__ void vx_chunk_mfma32(const VTile &d, const TS *varena, float *z, double *part,
+                                                const double *tscale, int lane) {
+    constexpr int VW  = Vec<TS>::W;
+    constexpr int NIT = VPASS32 / (2 * VW);
+    const int half = lane >> 5, s = lane & 31;
+    for (int rt = 0; rt < d.k; rt += 32) {
+        const int row  = rt + s;
+        const TS *rowp = varena + d.v_off + (size_t)(row < d.k ? row : 0) * d.ld;
+        TS a[NIT][VW], an[NIT][VW];
+        float4 b[NIT][VW / 4], bn[NIT][VW / 4];
+        auto load = [&](int c0, TS (&da)[NIT][VW], float4 (&db)[NIT][VW / 4]) {
+            const int nc = min(VPASS32, d.ncols - c0);
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                const int kk = 2 * VW * it + VW * half;
+                const int kc = kk < nc ? kk : 0;
+                Vec<TS>::load(rowp + c0 + kc, da[it]);
+#pragma unroll
+                for (int m4 = 0; m4 < VW / 4; m4++)
+                    db[it][m4] = *reinterpret_cast<const float4 *>(z + zblk((size_t)d.x_off + c0 + kc + 4 * m4, s));
+            }
+        };
+        double accd[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            accd[i] = 0;
+        load(0, a, b);
+        for (int c0 = 0; c0 < d.ncols; c0 += VPASS32) {
+            const int nc    = min(VPASS32, d.ncols - c0);
+            const bool more = c0 + VPASS32 < d.ncols;
+            if (more)
+                load(c0 + VPASS32, an, bn);
+            floatx16 acc = {0};
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                if (2 * VW * it < nc) {
+                    const bool in = 2 * VW * it + VW * half < nc;
+#pragma unroll
+                    for (int m4 = 0; m4 < VW / 4; m4++) {
+                        const float bb[4] = {b[it][m4].x, b[it][m4].y, b[it][m4].z, b[it][m4].w};
+#pragma unroll
+                        for (int m = 0; m < 4; m++)
+                            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(in ? cvt<float>(a[it][4 * m4 + m]) : 0.0f,
+                                                                       bb[m], acc, 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                accd[i] += (double)acc[i];
+            if (more) {
+#pragma unroll
+                for (int it = 0; it < NIT; it++) {
+#pragma unroll
+                    for (int u = 0; u < VW; u++)
+                        a[it][u] = an[it][u];
+#pragma unroll
+                    for (int m4 = 0; m4 < VW / 4; m4++)
+                        b[it][m4] = bn[it][m4];
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int r = rt + mfma32_row(i, lane);
+            if (r < d.k)
+                vx_store_mfma(d, r, s, accd[i], z, part, tscale);
+        }
+    }
+}
+
+template <typename TS>
+__device__ __forceinline__ void vx_chunk_mfma(const VTile &d, const TS *varena, float *z, double *part,
+                                              const double *tscale, int lane) {
+    if (d.k <= 16)
+        vx_chunk_mfma16<TS>(d, varena, z, part, tscale, lane);
+    else
+        vx_chunk_mfma32<TS>(d, varena, z, part, tscale, lane);
+}
+
+template <typename TS0, typename TS1>
+__global__ void __launch_bounds__(WG) k_vx_mfma(const int *wv_off, int nwaves, const VTile *tiles, const TS0 *varena0,
+                                                const TS1 *varena1, float *z, double *part, const double *tscale) {
+    const int wv = blockIdx.x * NWAVE + (threadIdx.x >> 6);
+    if (wv >= nwaves)
+        return;
+    const int lane = threadIdx.x & 63;
+    for (int ti = wv_off[wv]; ti < wv_off[wv + 1]; ti++) {
+        const VTile d = tiles[ti];
+        if (d.cls == 0)
+            vx_chunk_mfma<TS0>(d, varena0, z, part, tscale, lane);
+        else
+            vx_chunk_mfma<TS1>(d, varena1, z, part, tscale, lane);
     }
 }
 
@@ -1056,12 +1475,28 @@ template void launch_convert<__hip_bfloat16>(const double *, __hip_bfloat16 *, u
     }
 
 template <typename TC>
-void launch_gather_x(const double *x, const int *perm_s, int nc, int mu_base, int MU, TC *z, hipStream_t st) {
-    const int n = nc * MU;
+void launch_gather_x(const double *x, const int *perm_s, int nc, int tpad, int mu_base, int MU, TC *z,
+                     hipStream_t st) {
+    const int n = tpad * MU;
     const dim3 grid((n + WG - 1) / WG);
-#define HTG_GATHER(M) hipLaunchKernelGGL((k_gather_x<TC, M>), grid, dim3(WG), 0, st, x, perm_s, nc, mu_base, z)
+    if constexpr (std::is_same<TC, float>::value) {
+        if (MU == MFMA_MU) {
+            hipLaunchKernelGGL((k_gather_x<TC, MFMA_MU, true>), grid, dim3(WG), 0, st, x, perm_s, nc, tpad, mu_base,
+                               z);
+            return;
+        }
+    }
+#define HTG_GATHER(M) \
+    hipLaunchKernelGGL((k_gather_x<TC, M, false>), grid, dim3(WG), 0, st, x, perm_s, nc, tpad, mu_base, z)
     HTG_MU_SWITCH(MU, HTG_GATHER)
 #undef HTG_GATHER
+}
+
+void launch_gather_x32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *stage, float *z,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_scatter_x32, dim3((nc + WG - 1) / WG, MFMA_MU / 8), dim3(WG), 0, st, x, iperm_s, nc, mu_base,
+                       stage);
+    hipLaunchKernelGGL(k_block_x32, dim3((tpad * 8 + WG - 1) / WG), dim3(WG), 0, st, stage, nc, tpad, z);
 }
 
 template <typename TS0, typename TS1, typename TC>
@@ -1070,6 +1505,16 @@ void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *mu
                hipStream_t st) {
     if (ngroups <= 0)
         return;
+    if constexpr (std::is_same<TC, float>::value) {
+        if (MU == MFMA_MU) {
+            hipLaunchKernelGGL((k_vx_mfma<TS0, TS1>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off,
+                               ngroups, tiles, varena0, varena1, z, part, tscale);
+            if (nmulti > 0)
+                hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU, true>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first,
+                                   part, z, tscale);
+            return;
+        }
+    }
 #define HTG_VX(M)                                                                                          \
     do {                                                                                                   \
         hipLaunchKernelGGL((k_vx<TS0, TS1, TC, M>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, \
@@ -1088,6 +1533,13 @@ void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, 
                  hipStream_t st) {
     if (nrb <= 0)
         return;
+    if constexpr (std::is_same<TC, float>::value) {
+        if (MU == MFMA_MU) {
+            hipLaunchKernelGGL((k_rows_mfma<TS0, TS1>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0,
+                               c1, arena1, z, perm_t, nr, mu_base, y);
+            return;
+        }
+    }
 #define HTG_ROWS(M)                                                                                          \
     hipLaunchKernelGGL((k_rows<TS0, TS1, TC, M>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0, c1, \
                        arena1, z, perm_t, nr, mu_base, y)
@@ -1095,8 +1547,8 @@ void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, 
 #undef HTG_ROWS
 }
 
-template void launch_gather_x<double>(const double *, const int *, int, int, int, double *, hipStream_t);
-template void launch_gather_x<float>(const double *, const int *, int, int, int, float *, hipStream_t);
+template void launch_gather_x<double>(const double *, const int *, int, int, int, int, double *, hipStream_t);
+template void launch_gather_x<float>(const double *, const int *, int, int, int, int, float *, hipStream_t);
 template void launch_vx<double, double, double>(const int *, int, const VTile *, const int *, int, const double *,
                                                 const double *, double *, double *, const double *, int, hipStream_t);
 template void launch_vx<float, float, float>(const int *, int, const VTile *, const int *, int, const float *,

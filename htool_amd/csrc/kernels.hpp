@@ -98,7 +98,12 @@ void launch_recompress(const RcLeaf *leaves, int nleaves, double *store, double 
 // the cluster-ordered x (nc RHS vectors) followed by t (one RHS vector per
 // low-rank term), both in TC.
 template <typename TC>
-void launch_gather_x(const double *x, const int *perm_s, int nc, int mu_base, int MU, TC *z, hipStream_t st);
+void launch_gather_x(const double *x, const int *perm_s, int nc, int tpad, int mu_base, int MU, TC *z,
+                     hipStream_t st);
+// 32-RHS gather into the blocked layout of the matrix-core passes; iperm_s is
+// the inverse of perm_s, stage holds nc x 32 floats.
+void launch_gather_x32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *stage, float *z,
+                       hipStream_t st);
 // Both precision classes in one launch: chunk d reads varena0 or varena1 by d.cls.
 template <typename TS0, typename TS1, typename TC>
 void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *multi_first, int nmulti,

@@ -107,3 +107,21 @@ def test_all_lowrank_terms_16bit(htg, orc):
     Y64 = H.mvprod_global_to_global(X)
     H.set_storage("fp32_fp16")
     assert rel(H.mvprod_global_to_global(X), Y64) < 2e-3
+
+
+@pytest.mark.parametrize("storage,bound", [("fp32", 1e-6), ("fp32_fp16", 1e-6), ("fp32_bf16", 8e-6)])
+def test_multi_rhs_matrix_core_path(htg, storage, bound):
+    """40 right-hand sides in FP32 arithmetic run as one 32-wide matrix-core pass
+    plus an 8-wide pass; every column agrees with the single-vector product of the
+    same storage (both FP32 arithmetic, different summation order) and with FP64
+    within the storage's budget."""
+    x, diag = sphere_case(7000, 9)
+    H = build(htg, x, diag, rc=2e-6, lp=5e-7)
+    X = np.random.default_rng(11).standard_normal((x.shape[0], 40))
+    Y64 = H.mvprod_global_to_global(X)
+    H.set_storage(storage)
+    Y = H.mvprod_global_to_global(X)
+    for c in (0, 7, 31, 32, 39):
+        assert rel(Y[:, c], H @ X[:, c]) < 1e-6, c
+    for c in range(40):
+        assert rel(Y[:, c], Y64[:, c]) < bound, c
