@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-assembly-5e5", action="store_true",
+                    help="skip the N=5e5 assembly with and without the learned rank predictor")
     ap.add_argument("--cpu-reps", type=int, default=10)
     ap.add_argument("--counters-run", action="store_true",
                     help="short run for rocprofv3 --pmc: a 256 MiB device copy of known size, then a few "
@@ -113,6 +115,51 @@ def cpu_baseline(pts, diag, x, leaves_entries, args):
     return {"value": round(b / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
             "ms_per_matvec": round(t * 1e3, 3),
             "sample": f"oracle FP64 leaf loop, {reps} matvecs on the full workload, one thread"}
+
+
+def assembly_5e5(torch, log):
+    """N=5e5 unit sphere, Laplace, FP64 ACA at epsilon 1e-6: build without and
+    with the trained rank predictor (trained on other sizes and geometries), the
+    workspace each reserved, reruns, the misprediction rate, and the product."""
+    import numpy as np
+    from htool_amd import Generator, HMatrix, RankPredictor, geometry
+    from htool_amd.predictor import misprediction
+    n = 500000
+    pts = geometry.sphere(n, seed=3)
+    diag = geometry.laplace_diag(geometry.sphere_spacing(n))
+    P = RankPredictor.load(os.path.join(ROOT, "htool_amd", "data", "rank_predictor.npz"))
+    out = {"N": n, "epsilon": 1e-6, "eta": 10.0, "leaf_size": 32}
+    x = torch.randn(n, dtype=torch.float64, device="cuda")
+    y = torch.zeros_like(x)
+    sp = torch.cuda.current_stream().cuda_stream
+    for tag, pred in (("fixed_first_workspace", None), ("predictor", P)):
+        H = HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32)
+        H.set_compression("partialACA")
+        H.set_predictor(pred)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        H.build(Generator(geometry.KERNEL_LAPLACE, 0.0, diag), pts)
+        torch.cuda.synchronize()
+        build_s = time.time() - t0
+        i = H.get_info()
+        r = {"build_s": round(build_s, 3), "aca_s": round(i.aca_seconds, 3), "aca_runs": i.n_aca,
+             "reruns": i.n_aca_retry, "workspace_entries": int(i.aca_ws_entries),
+             "needed_entries": int(i.aca_need_entries)}
+        if pred is not None:
+            f, t, cap = H.aca_records()
+            m = misprediction(P.predict_terms(f), t)
+            r["misprediction_rate"] = round(float(np.mean(t > cap)), 5)
+            r["mean_abs_log_ratio"] = round(m["mean_abs_log_ratio"], 4)
+            ms = timed(lambda: H.mvprod_device(x.data_ptr(), y.data_ptr(), 1, sp), 20, 3,
+                       torch.cuda.current_stream(), torch)
+            r["matvec_fp64_ms"] = round(ms, 4)
+            H.set_storage("fp32")
+            r["matvec_fp32_ms"] = round(timed(lambda: H.mvprod_device(x.data_ptr(), y.data_ptr(), 1, sp), 20, 3,
+                                              torch.cuda.current_stream(), torch), 4)
+        out[tag] = r
+        log(f"N=5e5 {tag}: {r}")
+        del H
+    return out
 
 
 def main():
@@ -217,6 +264,8 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get(dom)
 
+    asm = None if args.no_assembly_5e5 else assembly_5e5(torch, log)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline")
@@ -262,6 +311,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "assembly_5e5": asm,
     }
     print(json.dumps(line), flush=True)
 

@@ -8,5 +8,6 @@ from . import geometry
 from .capi import HtgError
 from .hmatrix import Generator, HMatrix
 from .lrmat import Block, aca_batch
+from .predictor import RankPredictor
 
-__all__ = ["geometry", "Generator", "HMatrix", "HtgError", "Block", "aca_batch"]
+__all__ = ["geometry", "Generator", "HMatrix", "HtgError", "Block", "aca_batch", "RankPredictor"]

@@ -37,7 +37,8 @@ class htg_generator(C.Structure):
 class htg_params(C.Structure):
     _fields_ = [("epsilon", C.c_double), ("eta", C.c_double), ("maxblocksize", C.c_double),
                 ("minclustersize", C.c_int), ("compressor", C.c_int), ("reqrank", C.c_int),
-                ("lowprec_all", C.c_int), ("recompress_tol", C.c_double), ("lowprec_tol", C.c_double)]
+                ("lowprec_all", C.c_int), ("recompress_tol", C.c_double), ("lowprec_tol", C.c_double),
+                ("predictor", C.c_void_p), ("predictor_margin", C.c_double)]
 
 
 class htg_info(C.Structure):
@@ -47,7 +48,9 @@ class htg_info(C.Structure):
                 ("lowrank_entries", C.c_int64), ("lowprec_entries", C.c_int64), ("lowrank_u_entries", C.c_int64),
                 ("lowprec_u_entries", C.c_int64), ("n_row_blocks", C.c_int),
                 ("n_columns", C.c_int),
-                ("n_vtiles", C.c_int), ("build_seconds", C.c_double)]
+                ("n_vtiles", C.c_int), ("build_seconds", C.c_double), ("n_aca", C.c_int),
+                ("n_aca_retry", C.c_int), ("aca_ws_entries", C.c_int64), ("aca_need_entries", C.c_int64),
+                ("aca_seconds", C.c_double)]
 
 
 class HtgError(RuntimeError):
@@ -76,7 +79,18 @@ _SIGNATURES = {
     "htg_hmatrix_profile": (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP]),
     "htg_aca_batch": (_I, [C.POINTER(htg_generator), _I, _D, _I, _I, _VP, _I, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP,
                            _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    "htg_hmatrix_aca_records": (_I, [_VP, C.POINTER(_I), _VP, _VP, _VP]),
+    "htg_predictor_create": (_I, [C.c_ulonglong, C.POINTER(_VP)]),
+    "htg_predictor_destroy": (_I, [_VP]),
+    "htg_predictor_set_params": (_I, [_VP, _VP, _VP, _VP]),
+    "htg_predictor_get_params": (_I, [_VP, _VP, _VP, _VP]),
+    "htg_predictor_loss_grad": (_I, [_VP, _VP, _VP, _I, C.POINTER(_D), _VP]),
+    "htg_predictor_train": (_I, [_VP, _VP, _VP, _I, _I, _I, C.c_float, C.c_ulonglong, _I, _VP]),
+    "htg_predictor_predict": (_I, [_VP, _VP, _I, _VP]),
 }
+
+MLP_NFEAT = 8
+MLP_NPARAMS = 4801
 
 _lib = None
 

@@ -8,6 +8,9 @@
 // The merge of all-dense sons into one dense block is decided afterwards,
 // bottom up, with the reference's rule (hmatrix.hpp:536-546).
 #include "assemble.hpp"
+#include "predictor.hpp"
+
+#include <chrono>
 
 #include <algorithm>
 #include <cmath>
@@ -61,6 +64,7 @@ struct Assembler {
     std::vector<DevArray<double> *> store_chunks;
     DevArray<double> store;
     uint64_t store_used = 0;
+    Assembly *rec = nullptr; // ACA records and counters
 
     void expand(int id) {
         const ClusterNode &a = T.nodes[nodes[id].t], &b = S.nodes[nodes[id].s];
@@ -121,18 +125,46 @@ struct Assembler {
 
     // Runs ACA on every pending node; returns the low-rank leaves found.
     void run_wave(std::vector<int> wave, std::vector<Leaf> &lr_out, std::vector<int> &lr_node) {
-        std::vector<int> caps(wave.size());
-        for (size_t w = 0; w < wave.size(); w++) {
+        const size_t nw = wave.size();
+        std::vector<int> caps(nw), kmaxs(nw);
+        std::vector<float> feat(nw * 8);
+        for (size_t w = 0; w < nw; w++) {
             const ClusterNode &a = T.nodes[nodes[wave[w]].t], &b = S.nodes[nodes[wave[w]].s];
             const long long M = a.size, N = b.size;
             long long kmax    = (M * N) / (M + N);
             if (prm.reqrank > 0)
                 kmax = std::min<long long>(kmax, std::min<long long>(prm.reqrank, std::min(M, N)));
-            caps[w] = (int)std::max<long long>(1, std::min<long long>(kmax, CAP_FIRST));
+            kmaxs[w] = (int)std::max<long long>(1, std::min<long long>(kmax, CAP_MAX));
+            caps[w]  = (int)std::max<long long>(1, std::min<long long>(kmax, CAP_FIRST));
+            block_features(a, b, prm.eta, prm.epsilon, &feat[w * 8]);
         }
-        std::vector<int> todo(wave.size());
-        for (size_t w = 0; w < wave.size(); w++)
+        std::vector<int> todo(nw);
+        for (size_t w = 0; w < nw; w++)
             todo[w] = (int)w;
+        if (prm.predictor && nw > 0) {
+            // presize each block's workspace from the predicted terms, heaviest blocks first
+            std::vector<float> t(nw);
+            predictor_predict(*prm.predictor, feat.data(), (int)nw, t.data());
+            const double f = 1.0 + std::max(prm.predictor_margin, 0.0);
+            for (size_t w = 0; w < nw; w++) {
+                const double c = std::ceil(f * t[w]) + 2;
+                caps[w]        = (int)std::max(1.0, std::min<double>(c, kmaxs[w]));
+            }
+            auto work = [&](int w) {
+                const FpNode &nd = nodes[wave[w]];
+                return (int64_t)(T.nodes[nd.t].size + S.nodes[nd.s].size) * caps[w];
+            };
+            std::stable_sort(todo.begin(), todo.end(), [&](int x, int y) { return work(x) > work(y); });
+        }
+        std::vector<int> first_cap(caps);
+        for (size_t w = 0; w < nw; w++) {
+            const FpNode &nd = nodes[wave[w]];
+            rec->aca_ws_entries += (int64_t)(T.nodes[nd.t].size + S.nodes[nd.s].size) * caps[w];
+        }
+        const size_t rec0 = rec->aca_terms.size();
+        rec->aca_feat.insert(rec->aca_feat.end(), feat.begin(), feat.end());
+        rec->aca_terms.resize(rec0 + nw, 0);
+        rec->aca_cap.insert(rec->aca_cap.end(), first_cap.begin(), first_cap.end());
         while (!todo.empty()) {
             // sub-batch bounded by workspace size
             size_t b0 = 0;
@@ -196,15 +228,18 @@ struct Assembler {
                     const AcaBlock &B = blocks[i];
                     const int k      = ranks[i];
                     if (k == -3) {
-                        if (caps[w] >= CAP_MAX)
+                        if (caps[w] >= kmaxs[w])
                             throw Error(HTG_ERR_RANK_CAP, "ACA needed more than " + std::to_string(CAP_MAX) +
                                                               " terms in a " + std::to_string(B.M) + "x" +
                                                               std::to_string(B.N) + " block");
-                        const long long kmax = ((long long)B.M * B.N) / (B.M + B.N);
-                        caps[w]              = (int)std::min<long long>(kmax, CAP_MAX);
+                        caps[w] = kmaxs[w];
+                        rec->aca_retry++;
                         retry.push_back(w);
                         continue;
                     }
+                    const int need               = aca_terms_needed(B.M, B.N, k);
+                    rec->aca_terms[rec0 + w]     = need;
+                    rec->aca_need_entries += (int64_t)(B.M + B.N) * need;
                     if (k == 0) { // fixed rank 0: a zero block with no factors (lrmat.hpp:37-44)
                         lr_out.push_back(Leaf{B.off_t, B.M, B.off_s, B.N, 0});
                         lr_node.push_back(wave[w]);
@@ -280,7 +315,7 @@ void assemble(const htg_params &prm, const htg_generator &gen, int nt, const dou
     if (!same)
         res.pS.upload(xs, S.perm);
 
-    Assembler as{T, S, prm, res.kd, res.dT(), res.dS(), {}, {}, {}, 0, {}, {}, 0};
+    Assembler as{T, S, prm, res.kd, res.dT(), res.dS(), {}, {}, {}, 0, {}, {}, 0, &res};
     const std::vector<BlockTask> tasks = build_block_tasks(T, S, prm.eta, (size_t)prm.maxblocksize);
     std::vector<int> roots(tasks.size(), -1);
     for (size_t p = 0; p < tasks.size(); p++) {
@@ -297,6 +332,7 @@ void assemble(const htg_params &prm, const htg_generator &gen, int nt, const dou
         }
     }
     std::vector<Leaf> lr_leaves;
+    const auto aca_t0 = std::chrono::steady_clock::now();
     while (!as.pending.empty()) {
         std::vector<int> wave;
         wave.swap(as.pending);
@@ -313,6 +349,7 @@ void assemble(const htg_params &prm, const htg_generator &gen, int nt, const dou
             }
         }
     }
+    res.aca_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - aca_t0).count();
     for (size_t p = 0; p < tasks.size(); p++)
         if (roots[p] >= 0 && as.resolve(roots[p]))
             as.dense(tasks[p].t, tasks[p].s);

@@ -27,6 +27,13 @@ struct Assembly {
     KernelDesc kd{};
     bool same = true;
     PointsOnDevice pT, pS;
+    // ACA runs of admissible blocks (first attempts): features, terms needed,
+    // first workspace terms; retries and workspace totals.
+    std::vector<float> aca_feat;
+    std::vector<int> aca_terms, aca_cap;
+    int aca_retry = 0;
+    int64_t aca_ws_entries = 0, aca_need_entries = 0;
+    double aca_seconds = 0;
     DevPoints dT() const { return pT.view(); }
     DevPoints dS() const { return same ? pT.view() : pS.view(); }
 };

@@ -69,6 +69,30 @@ struct DevArray {
 
 inline int pad4(int v) { return (v + 3) & ~3; }
 
+// Message of the last failed entry point on this thread (htg_last_error).
+std::string &last_error();
+
+// Runs an entry point's body; exceptions become a status and a message.
+template <typename F>
+int guard(F &&f) {
+    try {
+        f();
+        return HTG_OK;
+    } catch (const Error &e) {
+        last_error() = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        last_error() = "host allocation failed";
+        return HTG_ERR_OOM;
+    } catch (const std::invalid_argument &e) {
+        last_error() = std::string("[Htool error] ") + e.what();
+        return HTG_ERR_INVALID;
+    } catch (const std::exception &e) {
+        last_error() = e.what();
+        return HTG_ERR_HIP;
+    }
+}
+
 // One leaf of the H-matrix in cluster numbering.
 struct Leaf {
     int oi, m, oj, n, k;           // k = -1: dense

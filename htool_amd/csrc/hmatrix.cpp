@@ -15,7 +15,6 @@
 namespace htg {
 namespace {
 
-thread_local std::string g_last_error;
 
 constexpr int RB_MAX       = 128;  // rows per row block
 constexpr int ALIGN        = 32;   // block alignment in elements
@@ -61,6 +60,12 @@ struct htg_hmatrix {
     std::vector<Leaf> leaves; // sorted by (oi, oj)
     int false_positive = 0;
     double build_seconds = 0;
+    // ACA runs of the build (first attempts): features, terms needed, first workspace terms
+    std::vector<float> aca_feat;
+    std::vector<int> aca_terms, aca_cap;
+    int aca_retry = 0;
+    int64_t aca_ws_entries = 0, aca_need_entries = 0;
+    double aca_seconds = 0;
     bool scaled = false; // some low-rank terms carry scales (recompressed)
 
     std::vector<int> rb_row0, rb_order;
@@ -442,26 +447,6 @@ void run_product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStre
     }
 }
 
-template <typename F>
-int guard(F &&f) {
-    try {
-        f();
-        return HTG_OK;
-    } catch (const Error &e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        g_last_error = "host allocation failed";
-        return HTG_ERR_OOM;
-    } catch (const std::invalid_argument &e) {
-        g_last_error = std::string("[Htool error] ") + e.what();
-        return HTG_ERR_INVALID;
-    } catch (const std::exception &e) {
-        g_last_error = e.what();
-        return HTG_ERR_HIP;
-    }
-}
-
 void require_device() {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
@@ -551,7 +536,7 @@ void ensure_copy(DevArray<TS> &dst, const DevArray<double> &src, uint64_t n) {
 
 extern "C" {
 
-const char *htg_last_error(void) { return htg::g_last_error.c_str(); }
+const char *htg_last_error(void) { return htg::last_error().c_str(); }
 
 const char *htg_status_string(int status) {
     switch (status) {
@@ -594,6 +579,13 @@ int htg_hmatrix_build(const htg_params *prm, const htg_generator *gen, int nt, c
         h->perm_s         = A.perm_s;
         h->false_positive = A.false_positive;
         h->leaves         = A.leaves;
+        h->aca_feat.swap(A.aca_feat);
+        h->aca_terms.swap(A.aca_terms);
+        h->aca_cap.swap(A.aca_cap);
+        h->aca_retry        = A.aca_retry;
+        h->aca_ws_entries   = A.aca_ws_entries;
+        h->aca_need_entries = A.aca_need_entries;
+        h->aca_seconds      = A.aca_seconds;
         sort_leaves(h->leaves);
         build_layout(*h);
         upload_layout(*h);
@@ -734,7 +726,30 @@ int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info) {
         r.n_columns         = (int)(h->cls[0].colidx.size() + h->cls[1].colidx.size());
         r.n_vtiles          = (int)h->vtiles.size();
         r.build_seconds     = h->build_seconds;
+        r.n_aca             = (int)h->aca_terms.size();
+        r.n_aca_retry       = h->aca_retry;
+        r.aca_ws_entries    = h->aca_ws_entries;
+        r.aca_need_entries  = h->aca_need_entries;
+        r.aca_seconds       = h->aca_seconds;
         *info               = r;
+    });
+}
+
+int htg_hmatrix_aca_records(const htg_hmatrix *h, int *n, float *feat, int *terms, int *cap_first) {
+    return guard([&] {
+        if (!h || !n)
+            throw Error(HTG_ERR_INVALID, "null argument");
+        const int cnt = (int)h->aca_terms.size();
+        if (feat) {
+            if (*n < cnt)
+                throw Error(HTG_ERR_INVALID, "record buffers hold fewer than the build's ACA runs");
+            std::memcpy(feat, h->aca_feat.data(), h->aca_feat.size() * sizeof(float));
+            if (terms)
+                std::memcpy(terms, h->aca_terms.data(), (size_t)cnt * sizeof(int));
+            if (cap_first)
+                std::memcpy(cap_first, h->aca_cap.data(), (size_t)cnt * sizeof(int));
+        }
+        *n = cnt;
     });
 }
 

@@ -60,6 +60,13 @@ def ellipsoid(n: int, axes=(1.0, 1.0, 4.0), seed: int = 0) -> np.ndarray:
     return np.ascontiguousarray(np.concatenate(out, axis=0)[:n])
 
 
+def prolate_spheroid_area(a: float, c: float) -> float:
+    """Surface area of the spheroid with semi-axes (a, a, c), c > a:
+    2 pi a^2 (1 + c / (a e) asin e), e = sqrt(1 - a^2 / c^2)."""
+    e = np.sqrt(1.0 - (a / c) ** 2)
+    return float(2 * np.pi * a * a * (1 + c / (a * e) * np.arcsin(e)))
+
+
 def disk(n: int, z: float, seed: int = 0) -> np.ndarray:
     """Uniform points on the unit disk in the plane ``z`` (the reference's test geometry,
     include/htool/testing/geometry.hpp:10-22, drawn with PCG64 instead of C rand())."""

@@ -51,6 +51,8 @@ class HMatrix:
         self.recompress_tol = 0.0
         self.lowprec_tol = 0.0
         self.lowprec_all = False
+        self.predictor = None
+        self.predictor_margin = 0.25
         self._h = None
         self._storage = "fp64"
 
@@ -80,6 +82,12 @@ class HMatrix:
         self.lowprec_tol = float(tol)
         self.lowprec_all = bool(all_terms)
 
+    def set_predictor(self, predictor, margin: float = 0.25):
+        """Presize ACA workspaces from a trained RankPredictor (None: fixed first
+        workspace of 64 terms). Ranks are the same either way."""
+        self.predictor = predictor
+        self.predictor_margin = float(margin)
+
     # -- assembly ---------------------------------------------------------------
     def build(self, gen: Generator, xt: np.ndarray, xs: np.ndarray | None = None) -> "HMatrix":
         """HMatrix::build(A, xt, xs) (include/htool/types/hmatrix.hpp:285-341)."""
@@ -89,7 +97,9 @@ class HMatrix:
         if not same:
             xs = np.ascontiguousarray(xs, dtype=np.float64)
         prm = capi.htg_params(self.epsilon, self.eta, self.maxblocksize, self.minclustersize, self.compressor,
-                              self.reqrank, int(self.lowprec_all), self.recompress_tol, self.lowprec_tol)
+                              self.reqrank, int(self.lowprec_all), self.recompress_tol, self.lowprec_tol,
+                              self.predictor.handle if self.predictor is not None else None,
+                              self.predictor_margin)
         g = gen._c(same)
         h = C.c_void_p()
         capi.check(L.htg_hmatrix_build(C.byref(prm), C.byref(g), xt.shape[0], capi.ptr(xt),
@@ -189,6 +199,18 @@ class HMatrix:
 
     def nb_cols(self) -> int:
         return self.get_info().nc
+
+    def aca_records(self):
+        """(features n x 8, terms needed, first workspace terms) of the build's ACA runs."""
+        n = C.c_int(0)
+        L = capi.lib()
+        capi.check(L.htg_hmatrix_aca_records(self._handle(), C.byref(n), None, None, None))
+        feat = np.zeros((n.value, 8), np.float32)
+        terms = np.zeros(n.value, np.int32)
+        cap = np.zeros(n.value, np.int32)
+        capi.check(L.htg_hmatrix_aca_records(self._handle(), C.byref(n), capi.ptr(feat), capi.ptr(terms),
+                                             capi.ptr(cap)))
+        return feat, terms, cap
 
     def get_perms(self):
         i = self.get_info()

@@ -72,6 +72,8 @@ typedef struct htg_generator {
 /* Parameters of HMatrix<T>(t, s, epsilon, eta, 'N', 'N', reqrank) plus the
  * setters the hot path reads (include/htool/types/hmatrix.hpp:115, 213-220)
  * and Cluster::set_minclustersize (include/htool/clustering/cluster.hpp:334). */
+typedef struct htg_predictor htg_predictor;
+
 typedef struct htg_params {
     double epsilon;     /* ACA tolerance (default 1e-6) */
     double eta;         /* admissibility parameter (default 10) */
@@ -85,6 +87,10 @@ typedef struct htg_params {
                               counterpart in the reference); 0: keep the ACA factors */
     double lowprec_tol;    /* > 0 (with recompression): trailing terms go to class 1 while
                               2 * 2^-11 * sum(sigma_j) <= lowprec_tol * ||sigma||_2 */
+    const htg_predictor *predictor; /* NULL, or a trained rank predictor: presizes each block's
+                                       ACA workspace and orders the blocks by expected work
+                                       (no counterpart in the reference); the ranks do not change */
+    double predictor_margin;        /* room above the prediction: cap = ceil((1 + margin) t) + 2 */
 } htg_params;
 
 typedef struct htg_info {
@@ -100,6 +106,11 @@ typedef struct htg_info {
     int64_t lowprec_u_entries;/* the U part of lowprec_entries */
     int n_row_blocks, n_columns, n_vtiles;
     double build_seconds;     /* block tree + assembly */
+    int n_aca;                /* ACA runs of admissible blocks (false positives included) */
+    int n_aca_retry;          /* runs that outgrew their first workspace and ran again */
+    int64_t aca_ws_entries;   /* sum over first runs of (m + n) x workspace terms */
+    int64_t aca_need_entries; /* sum over runs of (m + n) x terms actually needed */
+    double aca_seconds;       /* batched ACA (kernels and host scheduling) */
 } htg_info;
 
 typedef struct htg_hmatrix htg_hmatrix;
@@ -162,6 +173,38 @@ int htg_aca_batch(const htg_generator *gen, int compressor, double epsilon, int 
                   const int *row_off, const int *cols, const int *col_off, const double *ctr_t, const double *ctr_s,
                   const int *off_t, const int *off_s, int cap, int *ranks, double *U, const int64_t *u_off, double *V,
                   const int64_t *v_off);
+
+/* ACA records of the last build, one per ACA run of an admissible block (first
+ * attempt): the 8 block features (ln m, ln n, ln d, ln diam_t, ln diam_s, eta,
+ * ln epsilon, ln gap; d the centre distance, gap = max(d - rad_t - rad_s,
+ * 1e-6 (rad_t + rad_s))), the cross terms the run needed (its rank, or
+ * floor(mn/(m+n)) when the block was not compressible) and the workspace terms
+ * of its first attempt. Call with feat == NULL to get the count in *n. */
+int htg_hmatrix_aca_records(const htg_hmatrix *h, int *n, float *feat, int *terms, int *cap_first);
+
+/* Learned ACA rank predictor (no counterpart in the reference): an
+ * 8 -> 64 -> 64 -> 1 perceptron with GELU activations in FP32, mapping the
+ * normalised features (f - mean) / std of an admissible block to ln(1 + t), t
+ * the cross terms its ACA run needs. Parameters: HTG_MLP_NPARAMS floats, W1
+ * [64][8], b1 [64], W2 [64][64], b2 [64], w3 [64], b3. Trained on the GPU with
+ * the library's own forward and backward kernels and Adam. */
+#define HTG_MLP_NFEAT 8
+#define HTG_MLP_NPARAMS 4801
+/* Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights and biases from `seed`. */
+int htg_predictor_create(unsigned long long seed, htg_predictor **out);
+int htg_predictor_destroy(htg_predictor *p);
+int htg_predictor_set_params(htg_predictor *p, const float *params, const float *mean, const float *stdv);
+int htg_predictor_get_params(const htg_predictor *p, float *params, float *mean, float *stdv);
+/* Loss (1/n) sum 0.5 (out_i - target_i)^2 and its gradient with respect to
+ * the parameters at the current weights, over all n samples (host arrays). */
+int htg_predictor_loss_grad(const htg_predictor *p, const float *feat, const float *target, int n, double *loss,
+                            float *grad);
+/* Adam on mini-batches of `batch` samples, shuffled from `seed`; fit_norm = 1
+ * first sets mean and std from the data. epoch_loss[e] = mean loss of epoch e. */
+int htg_predictor_train(htg_predictor *p, const float *feat, const float *target, int n, int epochs, int batch,
+                        float lr, unsigned long long seed, int fit_norm, double *epoch_loss);
+/* Network outputs (ln(1 + t) estimates) for n feature rows. */
+int htg_predictor_predict(const htg_predictor *p, const float *feat, int n, float *out);
 
 #ifdef __cplusplus
 }

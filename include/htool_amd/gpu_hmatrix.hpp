@@ -48,6 +48,8 @@ class GpuHMatrix {
         prm_.lowprec_all    = 0;
         prm_.recompress_tol = 0.0;
         prm_.lowprec_tol    = 0.0;
+        prm_.predictor        = nullptr;
+        prm_.predictor_margin = 0.25;
     }
     GpuHMatrix(const GpuHMatrix &) = delete;
     GpuHMatrix &operator=(const GpuHMatrix &) = delete;
@@ -74,6 +76,11 @@ class GpuHMatrix {
         prm_.lowprec_all = all_terms ? 1 : 0;
     }
     void set_storage(htg_storage s) { check(htg_hmatrix_set_storage(handle(), s)); }
+    // Presizes ACA workspaces from a trained rank predictor (not owned; nullptr: none).
+    void set_predictor(const htg_predictor *p, double margin = 0.25) {
+        prm_.predictor        = p;
+        prm_.predictor_margin = margin;
+    }
 
     // HMatrix::build(A, xt, xs), hmatrix.hpp:285-341 (points interleaved xyz).
     void build(const Generator &g, int nt, const double *xt, int ns, const double *xs) {

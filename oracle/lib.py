@@ -35,6 +35,8 @@ def lib():
             "orc_hmat_free": (None, [_VP]),
             "orc_leaf_mvprod": (None, [_I, _I, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I]),
             "orc_dense_rows": (None, [_I, _D, _D, _I, _VP, _VP, _I, _I, _VP, _VP, _I, _VP]),
+            "orc_mlp_loss_grad": (None, [_VP, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
+            "orc_mlp_predict": (None, [_VP, _VP, _VP, _VP, _I, _VP]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -141,8 +143,9 @@ def leaf_mvprod(nr, nc, perm_t, perm_s, leaves, data, x):
     y = np.zeros((nr, mu), order="F")
     leaves = np.ascontiguousarray(leaves, np.int32)
     data = np.ascontiguousarray(data, np.float64)
-    lib().orc_leaf_mvprod(nr, nc, _p(np.ascontiguousarray(perm_t, np.int32)), _p(np.ascontiguousarray(perm_s, np.int32)),
-                          leaves.shape[0], _p(leaves), _p(data), _p(xm), _p(y), mu)
+    perm_t = np.ascontiguousarray(perm_t, np.int32)  # kept alive across the call
+    perm_s = np.ascontiguousarray(perm_s, np.int32)
+    lib().orc_leaf_mvprod(nr, nc, _p(perm_t), _p(perm_s), leaves.shape[0], _p(leaves), _p(data), _p(xm), _p(y), mu)
     return y[:, 0].copy() if vec else np.ascontiguousarray(y)
 
 
@@ -173,3 +176,30 @@ def iter_leaf_data(leaves, data):
             v = data[pos:pos + k * n].reshape(n, k).T
             pos += k * n
             yield oi, m, oj, n, k, (u, v)
+
+
+MLP_NPARAMS = 4801
+
+
+def mlp_loss_grad(params, mean, std, feat, target):
+    """FP64 loss and gradient of the rank predictor network (oracle/oracle.h)."""
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    mean = np.ascontiguousarray(mean, dtype=np.float64)
+    std = np.ascontiguousarray(std, dtype=np.float64)
+    feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, 8)
+    target = np.ascontiguousarray(target, dtype=np.float32)
+    loss = C.c_double()
+    grad = np.zeros(MLP_NPARAMS)
+    lib().orc_mlp_loss_grad(_p(params), _p(mean), _p(std), _p(feat), _p(target), feat.shape[0], C.byref(loss),
+                            _p(grad))
+    return loss.value, grad
+
+
+def mlp_predict(params, mean, std, feat):
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    mean = np.ascontiguousarray(mean, dtype=np.float64)  # kept alive across the call
+    std = np.ascontiguousarray(std, dtype=np.float64)
+    feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, 8)
+    out = np.zeros(feat.shape[0])
+    lib().orc_mlp_predict(_p(params), _p(mean), _p(std), _p(feat), feat.shape[0], _p(out))
+    return out

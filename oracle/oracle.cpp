@@ -697,3 +697,84 @@ void orc_dense_rows(int kernel, double delta, double diag, int same_points, cons
 }
 
 } // extern "C"
+
+/* ------------------------------------------------------------------------- */
+/* Rank predictor network (test infrastructure for the library's own MLP     */
+/* kernels; the network has no counterpart in the reference).                */
+namespace {
+
+constexpr int NI = 8, NH = 64;
+constexpr int OW1 = 0, OB1 = OW1 + NH * NI, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH, OB3 = OW3 + NH;
+
+double gelu(double x) { return 0.5 * x * (1.0 + std::erf(x / std::sqrt(2.0))); }
+double gelu_grad(double x) {
+    return 0.5 * (1.0 + std::erf(x / std::sqrt(2.0))) + x * std::exp(-0.5 * x * x) / std::sqrt(2.0 * M_PI);
+}
+
+struct Act {
+    double x[NI], z1[NH], h1[NH], z2[NH], h2[NH], out;
+};
+
+void mlp_forward(const double *P, const double *mean, const double *stdv, const float *f, Act &a) {
+    for (int i = 0; i < NI; i++)
+        a.x[i] = ((double)f[i] - mean[i]) / stdv[i];
+    for (int j = 0; j < NH; j++) {
+        double s = P[OB1 + j];
+        for (int i = 0; i < NI; i++)
+            s += P[OW1 + j * NI + i] * a.x[i];
+        a.z1[j] = s;
+        a.h1[j] = gelu(s);
+    }
+    a.out = P[OB3];
+    for (int j = 0; j < NH; j++) {
+        double s = P[OB2 + j];
+        for (int i = 0; i < NH; i++)
+            s += P[OW2 + j * NH + i] * a.h1[i];
+        a.z2[j] = s;
+        a.h2[j] = gelu(s);
+        a.out += P[OW3 + j] * a.h2[j];
+    }
+}
+
+} // namespace
+
+extern "C" void orc_mlp_predict(const double *params, const double *mean, const double *stdv, const float *feat,
+                                int n, double *out) {
+    for (int b = 0; b < n; b++) {
+        Act a;
+        mlp_forward(params, mean, stdv, feat + (size_t)b * NI, a);
+        out[b] = a.out;
+    }
+}
+
+extern "C" void orc_mlp_loss_grad(const double *params, const double *mean, const double *stdv, const float *feat,
+                                  const float *target, int n, double *loss, double *grad) {
+    std::fill(grad, grad + OB3 + 1, 0.0);
+    double L = 0;
+    for (int b = 0; b < n; b++) {
+        Act a;
+        mlp_forward(params, mean, stdv, feat + (size_t)b * NI, a);
+        const double r = a.out - (double)target[b];
+        L += 0.5 * r * r;
+        const double g = r / n;
+        grad[OB3] += g;
+        double dz2[NH];
+        for (int j = 0; j < NH; j++) {
+            grad[OW3 + j] += g * a.h2[j];
+            dz2[j] = g * params[OW3 + j] * gelu_grad(a.z2[j]);
+            grad[OB2 + j] += dz2[j];
+            for (int i = 0; i < NH; i++)
+                grad[OW2 + j * NH + i] += dz2[j] * a.h1[i];
+        }
+        for (int i = 0; i < NH; i++) {
+            double dh = 0;
+            for (int j = 0; j < NH; j++)
+                dh += dz2[j] * params[OW2 + j * NH + i];
+            const double dz1 = dh * gelu_grad(a.z1[i]);
+            grad[OB1 + i] += dz1;
+            for (int k = 0; k < NI; k++)
+                grad[OW1 + i * NI + k] += dz1 * a.x[k];
+        }
+    }
+    *loss = L / n;
+}

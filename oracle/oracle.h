@@ -67,6 +67,16 @@ void orc_leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int n
 void orc_dense_rows(int kernel, double delta, double diag, int same_points, const double *xt, const double *xs,
                     int ns, int nrows, const int *rows, const double *x, int mu, double *out);
 
+/* Rank predictor network of the library (8 -> 64 -> 64 -> 1 perceptron, GELU
+ * x Phi(x), no counterpart in the reference), restated in FP64. params: W1
+ * [64][8], b1 [64], W2 [64][64], b2 [64], w3 [64], b3 (4801 values); inputs
+ * (feat - mean) / stdv. orc_mlp_loss_grad: loss (1/n) sum 0.5 (out - target)^2
+ * and its gradient (4801 values); orc_mlp_predict: the outputs. */
+void orc_mlp_loss_grad(const double *params, const double *mean, const double *stdv, const float *feat,
+                       const float *target, int n, double *loss, double *grad);
+void orc_mlp_predict(const double *params, const double *mean, const double *stdv, const float *feat, int n,
+                     double *out);
+
 #ifdef __cplusplus
 }
 #endif

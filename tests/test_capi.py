@@ -27,7 +27,7 @@ def test_status_strings(htg):
 def test_invalid_arguments_rejected_without_device(htg):
     from htool_amd import capi
     L = capi.lib()
-    prm = capi.htg_params(1e-6, 10.0, 1e6, 0, 0, -1, 0, 0.0, 0.0)  # minclustersize 0
+    prm = capi.htg_params(1e-6, 10.0, 1e6, 0, 0, -1, 0, 0.0, 0.0, None, 0.25)  # minclustersize 0
     gen = capi.htg_generator(0, 1, 0.0, 0.0)
     pts = np.zeros((4, 3))
     h = C.c_void_p()
@@ -41,8 +41,9 @@ def test_invalid_arguments_rejected_without_device(htg):
     # from_leaves: leaf out of range
     perm = np.arange(10, dtype=np.int32)
     leaves = np.array([[5, 10, 0, 10, -1]], np.int32)
-    st = L.htg_hmatrix_from_leaves(10, 10, capi.ptr(perm), capi.ptr(perm), 1, capi.ptr(leaves),
-                                   capi.ptr(np.zeros(100)), C.byref(h))
+    data = np.zeros(100)
+    st = L.htg_hmatrix_from_leaves(10, 10, capi.ptr(perm), capi.ptr(perm), 1, capi.ptr(leaves), capi.ptr(data),
+                                   C.byref(h))
     assert st == capi.HTG_ERR_INVALID
     assert L.htg_hmatrix_destroy(None) == capi.HTG_OK
 
