@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched product")
     ap.add_argument("--no-assembly-5e5", action="store_true",
                     help="skip the N=5e5 assembly with and without the learned rank predictor")
     ap.add_argument("--cpu-reps", type=int, default=10)
@@ -115,6 +116,74 @@ def cpu_baseline(pts, diag, x, leaves_entries, args):
     return {"value": round(b / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
             "ms_per_matvec": round(t * 1e3, 3),
             "sample": f"oracle FP64 leaf loop, {reps} matvecs on the full workload, one thread"}
+
+
+def ellipsoid_area(a, b, c):
+    """Knud Thomsen's approximation of an ellipsoid's area (relative error < 1.1%)."""
+    p = 1.6075
+    return 4 * math.pi * (((a * b) ** p + (a * c) ** p + (b * c) ** p) / 3) ** (1 / p)
+
+
+def batch_64(torch, log):
+    """64 independent Laplace H-matrices, N log-uniform in [2e3, 5e4], each on a
+    seeded ellipsoid with semi-axes U(0.5, 2); eta 10, epsilon 1e-6, FP32
+    storage. One block-diagonal H-matrix: every instance's leaves in the same
+    launches, replayed from a HIP graph, layout picked by the autotuner; against
+    the 64 instances' products launched one after another."""
+    import numpy as np
+    from htool_amd import Generator, HMatrix, geometry
+    rng = np.random.default_rng(2024)
+    pts, gens = [], []
+    for i in range(64):
+        n = int(round(math.exp(rng.uniform(math.log(2e3), math.log(5e4)))))
+        ax = rng.uniform(0.5, 2.0, 3)
+        pts.append(geometry.ellipsoid(n, tuple(ax), seed=1000 + i))
+        gens.append(Generator(geometry.KERNEL_LAPLACE, 0.0,
+                              geometry.laplace_diag(math.sqrt(ellipsoid_area(*ax) / n))))
+    ntot = sum(p.shape[0] for p in pts)
+
+    def make():
+        H = HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32)
+        H.set_compression("partialACA")
+        return H
+
+    torch.cuda.synchronize()
+    t0 = time.time()
+    B = make().build_batch(gens, pts)
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    B.set_storage("fp32")
+    s = torch.cuda.Stream()
+    sp = s.cuda_stream
+    x = torch.randn(ntot, dtype=torch.float64, device="cuda")
+    y = torch.zeros_like(x)
+    out = {"instances": 64, "N_total": ntot, "N_min": min(p.shape[0] for p in pts),
+           "N_max": max(p.shape[0] for p in pts), "storage": "fp32", "build_s": round(build_s, 3)}
+    with torch.cuda.stream(s):
+        out["eager_default_layout_ms"] = round(timed(lambda: B.mvprod_device(x.data_ptr(), y.data_ptr(), 1, sp),
+                                                     50, 5, s, torch), 4)
+        lay, tuned_ms, tried = B.autotune(mu=1, reps=20)
+        out["autotune"] = {"layout": lay, "ms": round(tuned_ms, 4), "candidates": tried}
+        g = B.capture(x.data_ptr(), y.data_ptr(), 1, sp)
+        ms = timed(lambda: g.launch(sp), 50, 5, s, torch)
+        out["graph_ms"] = round(ms, 4)
+        out["graph_gbps"] = round(B.algorithmic_bytes(1, "fp32") / (ms * 1e-3) / 1e9, 1)
+        # the same 64 H-matrices, one product launch sequence each
+        singles = []
+        for p, gsp in zip(pts, gens):
+            H = make().build(gsp, p)
+            H.set_storage("fp32")
+            singles.append(H)
+        xs = [torch.randn(p.shape[0], dtype=torch.float64, device="cuda") for p in pts]
+        ys = [torch.zeros_like(v) for v in xs]
+
+        def all_singles():
+            for H, a, b in zip(singles, xs, ys):
+                H.mvprod_device(a.data_ptr(), b.data_ptr(), 1, sp)
+        out["separate_launches_ms"] = round(timed(all_singles, 20, 3, s, torch), 4)
+    out["speedup_vs_separate"] = round(out["separate_launches_ms"] / out["graph_ms"], 2)
+    log(f"batch of 64: {out}")
+    return out
 
 
 def assembly_5e5(torch, log):
@@ -265,6 +334,7 @@ def main():
             traffic = json.load(f).get(dom)
 
     asm = None if args.no_assembly_5e5 else assembly_5e5(torch, log)
+    batch = None if args.no_batch else batch_64(torch, log)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -312,6 +382,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "assembly_5e5": asm,
+        "batch_64": batch,
     }
     print(json.dumps(line), flush=True)
 

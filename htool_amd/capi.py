@@ -41,6 +41,11 @@ class htg_params(C.Structure):
                 ("predictor", C.c_void_p), ("predictor_margin", C.c_double)]
 
 
+class htg_layout(C.Structure):
+    _fields_ = [("rb_max", C.c_int), ("vchunk_el", C.c_int), ("vgroup_el", C.c_int), ("vgroup_tiles", C.c_int),
+                ("vrows", C.c_int)]
+
+
 class htg_info(C.Structure):
     _fields_ = [("nr", C.c_int), ("nc", C.c_int), ("n_dense", C.c_int), ("n_lowrank", C.c_int),
                 ("n_false_positive", C.c_int), ("rank_min", C.c_int), ("rank_max", C.c_int),
@@ -80,6 +85,13 @@ _SIGNATURES = {
     "htg_aca_batch": (_I, [C.POINTER(htg_generator), _I, _D, _I, _I, _VP, _I, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP,
                            _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "htg_hmatrix_aca_records": (_I, [_VP, C.POINTER(_I), _VP, _VP, _VP]),
+    "htg_hmatrix_build_batch": (_I, [C.POINTER(htg_params), _I, _VP, _VP, _VP, C.POINTER(_VP)]),
+    "htg_hmatrix_get_layout": (_I, [_VP, C.POINTER(htg_layout)]),
+    "htg_hmatrix_set_layout": (_I, [_VP, C.POINTER(htg_layout)]),
+    "htg_hmatrix_autotune": (_I, [_VP, _I, _I, C.POINTER(htg_layout), C.POINTER(_D), C.POINTER(_I)]),
+    "htg_graph_capture": (_I, [_VP, _VP, _VP, _I, _VP, C.POINTER(_VP)]),
+    "htg_graph_launch": (_I, [_VP, _VP]),
+    "htg_graph_destroy": (_I, [_VP]),
     "htg_predictor_create": (_I, [C.c_ulonglong, C.POINTER(_VP)]),
     "htg_predictor_destroy": (_I, [_VP]),
     "htg_predictor_set_params": (_I, [_VP, _VP, _VP, _VP]),

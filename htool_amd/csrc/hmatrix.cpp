@@ -16,12 +16,11 @@ namespace htg {
 namespace {
 
 
-constexpr int RB_MAX       = 128;  // rows per row block
 constexpr int ALIGN        = 32;   // block alignment in elements
-constexpr int VCHUNK_EL    = 8192; // target entries per V column chunk
-constexpr int VGROUP_EL    = 2048; // target entries per phase-1 wavefront
-constexpr int VGROUP_TILES = 8;    // most V chunks per phase-1 wavefront
-constexpr int VROWS        = 32;   // stacked V rows per chunk (one matrix-core tile)
+// Default work decomposition (htg_layout): rows per row block, target entries
+// per V chunk and per phase-1 wavefront, most chunks per wavefront, stacked V
+// rows per chunk (one matrix-core tile).
+constexpr htg_layout DEFAULT_LAYOUT = {128, 8192, 2048, 8, 32};
 constexpr int CLASS_ALIGN[2] = {4, 8}; // entries per aligned group: class 1 may hold 16-bit values
 constexpr int CAP_MAX        = 512;    // must match the ACA kernel's CAPMAX
 
@@ -54,8 +53,18 @@ struct ClassLayout {
 
 using namespace htg;
 
+// One assembled instance inside an H-matrix: its leaves are [leaf_begin,
+// leaf_end) of the (sorted) leaf list, its rows and columns start at `shift`.
+struct Part {
+    std::unique_ptr<htg::Assembly> A;
+    size_t leaf_begin = 0, leaf_end = 0;
+    int shift = 0;
+};
+
 struct htg_hmatrix {
     int nr = 0, nc = 0, tpad = 0;
+    htg_layout lay = htg::DEFAULT_LAYOUT;
+    std::vector<Part> parts; // assemblies kept for re-layout (empty for leaves from outside)
     std::vector<int> perm_t, perm_s;
     std::vector<Leaf> leaves; // sorted by (oi, oj)
     int false_positive = 0;
@@ -102,7 +111,7 @@ int t_entry(const htg_hmatrix &h, int l, int j) {
 
 // Product layout.
 //  * Row blocks: the segments between consecutive leaf row boundaries (every
-//    leaf covers a segment entirely or not at all), split to <= RB_MAX rows.
+//    leaf covers a segment entirely or not at all), split to <= rb_max rows.
 //  * Per precision class, each row block owns one contiguous row-major R x C
 //    block (row stride C rounded up to the class alignment, zero padded): the
 //    class-0 block holds the rows of the dense leaves (n columns each) and of
@@ -111,8 +120,8 @@ int t_entry(const htg_hmatrix &h, int l, int j) {
 //  * Each low-rank leaf stores its V rows (class 0: the first k_hi, class 1 the
 //    rest) row-major and aligned to the global column index, stacked with the
 //    other leaves of its source cluster (a panel); panel rows are cut into
-//    groups of VROWS and columns into chunks of about VCHUNK_EL entries,
-//    grouped into wavefront work lists.
+//    groups of vrows and columns into chunks of about vchunk_el entries,
+//    grouped into wavefront work lists (htg_layout).
 void build_layout(htg_hmatrix &h) {
     std::vector<int> bnd = {0, h.nr};
     for (const Leaf &L : h.leaves) {
@@ -124,7 +133,7 @@ void build_layout(htg_hmatrix &h) {
     h.rb_row0.clear();
     for (size_t b = 0; b + 1 < bnd.size(); b++) {
         const int a = bnd[b], len = bnd[b + 1] - a;
-        const int parts = (len + RB_MAX - 1) / RB_MAX;
+        const int parts = (len + h.lay.rb_max - 1) / h.lay.rb_max;
         for (int p = 0; p < parts; p++)
             h.rb_row0.push_back(a + (int)((long long)len * p / parts));
     }
@@ -256,8 +265,8 @@ void build_layout(htg_hmatrix &h) {
     std::stable_sort(h.rb_order.begin(), h.rb_order.end(), [&](int a, int b) { return work(a) > work(b); });
 
     // V rows and their chunks, class 0 then class 1, grouped per wavefront. A
-    // panel's stacked rows are cut into groups of VROWS rows, each group's
-    // columns into chunks of about VCHUNK_EL entries.
+    // panel's stacked rows are cut into groups of vrows rows, each group's
+    // columns into chunks of about vchunk_el entries.
     h.part_size = 0;
     h.vtiles.clear();
     h.multi_first.clear();
@@ -291,9 +300,9 @@ void build_layout(htg_hmatrix &h) {
             }
             C.varena_size    = align_up(base + (uint64_t)K * ld);
             const int tfirst = t_entry(h, first, c == 0 ? 0 : L0.k_hi);
-            for (int g0 = 0; g0 < K; g0 += VROWS) {
-                const int kr     = std::min(VROWS, K - g0);
-                const int cpc    = round_up(std::max(64, VCHUNK_EL / kr), CLASS_ALIGN[c]);
+            for (int g0 = 0; g0 < K; g0 += h.lay.vrows) {
+                const int kr     = std::min(h.lay.vrows, K - g0);
+                const int cpc    = round_up(std::max(64, h.lay.vchunk_el / kr), CLASS_ALIGN[c]);
                 const int nparts = (ld + cpc - 1) / cpc;
                 const int pbase  = h.part_size;
                 if (nparts > 1)
@@ -315,7 +324,7 @@ void build_layout(htg_hmatrix &h) {
                         h.multi_first.push_back((int)h.vtiles.size());
                     const int64_t el   = (int64_t)v.k * v.ncols;
                     const int in_group = (int)h.vtiles.size() - h.vg_off.back();
-                    if (in_group > 0 && (group_el + el > VGROUP_EL || in_group >= VGROUP_TILES)) {
+                    if (in_group > 0 && (group_el + el > h.lay.vgroup_el || in_group >= h.lay.vgroup_tiles)) {
                         h.vg_off.push_back((int)h.vtiles.size());
                         group_el = 0;
                     }
@@ -465,17 +474,19 @@ namespace {
 
 // Dense tiles from the generator and low-rank tiles from the compressed store,
 // into the FP64 master arenas.
-void fill_from_assembly(htg_hmatrix &h, const Assembly &A) {
+void fill_from_assembly(htg_hmatrix &h, const Part &part) {
+    const Assembly &A = *part.A;
     std::vector<FillTile> fills;
     std::vector<LrPack> packs[2];
     std::vector<UTileDst> udst[2];
-    for (size_t l = 0; l < h.leaves.size(); l++) {
+    for (size_t l = part.leaf_begin; l < part.leaf_end; l++) {
         const Leaf &L = h.leaves[l];
         if (L.k < 0) {
+            // the generator sees the instance's own point numbering
             for (const auto &rt : h.leaf_tiles[l]) {
                 const int rb = std::get<0>(rt);
-                fills.push_back(FillTile{std::get<1>(rt), h.rb_row0[rb], row_block_rows(h, rb), L.oj, L.n,
-                                         row_stride(h, 0, rb)});
+                fills.push_back(FillTile{std::get<1>(rt), h.rb_row0[rb] - part.shift, row_block_rows(h, rb),
+                                         L.oj - part.shift, L.n, row_stride(h, 0, rb)});
             }
             continue;
         }
@@ -531,8 +542,107 @@ void ensure_copy(DevArray<TS> &dst, const DevArray<double> &src, uint64_t n) {
     HTG_HIP(hipGetLastError());
 }
 
+// Adds the storage variant's copies of both classes (converted from FP64).
+void apply_storage(htg_hmatrix &h, int storage) {
+    for (int c = 0; c < 2; c++) {
+        ClassLayout &C = h.cls[c];
+        const bool low = c == 1 && (storage == HTG_STORAGE_FP32_FP16 || storage == HTG_STORAGE_FP32_BF16);
+        if (storage == HTG_STORAGE_FP64)
+            continue;
+        if (storage != HTG_STORAGE_FP32 && storage != HTG_STORAGE_FP32_FP16 && storage != HTG_STORAGE_FP32_BF16)
+            throw Error(HTG_ERR_UNSUPPORTED, "unknown storage mode " + std::to_string(storage));
+        if (!low) {
+            ensure_copy(C.a32, C.a64, C.a64.n);
+            ensure_copy(C.v32, C.v64, C.v64.n);
+        } else if (storage == HTG_STORAGE_FP32_FP16) {
+            ensure_copy(C.a16, C.a64, C.a64.n);
+            ensure_copy(C.v16, C.v64, C.v64.n);
+        } else {
+            ensure_copy(C.ab16, C.a64, C.a64.n);
+            ensure_copy(C.vb16, C.v64, C.v64.n);
+        }
+    }
+    HTG_HIP(hipDeviceSynchronize());
+    h.storage = storage;
+}
+
+// Lays out the product for h.lay and fills it from the kept assemblies.
+void relayout(htg_hmatrix &h) {
+    if (h.parts.empty())
+        throw Error(HTG_ERR_UNSUPPORTED,
+                    "a layout change needs an H-matrix assembled by htg_hmatrix_build or htg_hmatrix_build_batch");
+    for (ClassLayout &C : h.cls) {
+        C.a64.reset(), C.v64.reset(), C.a32.reset(), C.v32.reset();
+        C.a16.reset(), C.v16.reset(), C.ab16.reset(), C.vb16.reset();
+    }
+    build_layout(h);
+    upload_layout(h);
+    alloc_master(h);
+    for (const Part &part : h.parts)
+        fill_from_assembly(h, part);
+    apply_storage(h, h.storage);
+}
+
+void check_layout(const htg_layout &L) {
+    if (L.rb_max < 16 || L.rb_max > 128 || L.vchunk_el < 256 || L.vchunk_el > (1 << 20) || L.vgroup_el < 64 ||
+        L.vgroup_el > (1 << 20) || L.vgroup_tiles < 1 || L.vgroup_tiles > 1024 || L.vrows < 1 || L.vrows > 512)
+        throw Error(HTG_ERR_INVALID, "layout out of range (rb_max 16..128, vchunk_el 256..2^20, vgroup_el "
+                                     "64..2^20, vgroup_tiles 1..1024, vrows 1..512)");
+}
+
+// Leaves of part p: the contiguous range of the sorted list whose rows lie in
+// [shift, shift + n).
+void locate_part(htg_hmatrix &h, Part &p, int n) {
+    auto lo = std::lower_bound(h.leaves.begin(), h.leaves.end(), p.shift,
+                               [](const Leaf &L, int v) { return L.oi < v; });
+    auto hi = std::lower_bound(h.leaves.begin(), h.leaves.end(), p.shift + n,
+                               [](const Leaf &L, int v) { return L.oi < v; });
+    p.leaf_begin = (size_t)(lo - h.leaves.begin());
+    p.leaf_end   = (size_t)(hi - h.leaves.begin());
+}
+
+void take_aca_stats(htg_hmatrix &h, Assembly &A) {
+    h.false_positive += A.false_positive;
+    h.aca_feat.insert(h.aca_feat.end(), A.aca_feat.begin(), A.aca_feat.end());
+    h.aca_terms.insert(h.aca_terms.end(), A.aca_terms.begin(), A.aca_terms.end());
+    h.aca_cap.insert(h.aca_cap.end(), A.aca_cap.begin(), A.aca_cap.end());
+    h.aca_retry += A.aca_retry;
+    h.aca_ws_entries += A.aca_ws_entries;
+    h.aca_need_entries += A.aca_need_entries;
+    h.aca_seconds += A.aca_seconds;
+    std::vector<float>().swap(A.aca_feat);
+    std::vector<int>().swap(A.aca_terms);
+    std::vector<int>().swap(A.aca_cap);
+}
+
 } // namespace
 } // namespace htg
+
+namespace htg {
+namespace {
+void validate_params(const htg_params &prm) {
+    if (prm.minclustersize <= 0)
+        throw Error(HTG_ERR_INVALID, "[Htool error] MinClusterSize parameter cannot be zero");
+    if (prm.maxblocksize < 1)
+        throw Error(HTG_ERR_INVALID, "[Htool error] MaxBlockSize parameter cannot be zero");
+    if (prm.compressor != HTG_PARTIAL_ACA && prm.compressor != HTG_SYM_PARTIAL_ACA)
+        throw Error(HTG_ERR_UNSUPPORTED, "unknown compressor");
+    if (prm.recompress_tol < 0 || prm.lowprec_tol < 0)
+        throw Error(HTG_ERR_INVALID, "[Htool error] negative tolerance");
+}
+} // namespace
+} // namespace htg
+
+struct htg_graph {
+    hipGraph_t graph    = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~htg_graph() {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+    }
+};
 
 extern "C" {
 
@@ -555,14 +665,7 @@ int htg_hmatrix_build(const htg_params *prm, const htg_generator *gen, int nt, c
     return guard([&] {
         if (!prm || !gen || !xt || !out || nt <= 0)
             throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments to build HMatrix");
-        if (prm->minclustersize <= 0)
-            throw Error(HTG_ERR_INVALID, "[Htool error] MinClusterSize parameter cannot be zero");
-        if (prm->maxblocksize < 1)
-            throw Error(HTG_ERR_INVALID, "[Htool error] MaxBlockSize parameter cannot be zero");
-        if (prm->compressor != HTG_PARTIAL_ACA && prm->compressor != HTG_SYM_PARTIAL_ACA)
-            throw Error(HTG_ERR_UNSUPPORTED, "unknown compressor");
-        if (prm->recompress_tol < 0 || prm->lowprec_tol < 0)
-            throw Error(HTG_ERR_INVALID, "[Htool error] negative tolerance");
+        validate_params(*prm);
         const bool same = gen->same_points || xs == nullptr;
         if (same && ns != nt && ns > 0)
             throw Error(HTG_ERR_INVALID, "[Htool error] Invalid size in arguments for building HMatrix");
@@ -571,26 +674,24 @@ int htg_hmatrix_build(const htg_params *prm, const htg_generator *gen, int nt, c
         require_device();
         const auto t0 = std::chrono::steady_clock::now();
         std::unique_ptr<htg_hmatrix> h(new htg_hmatrix);
-        Assembly A;
+        Part part;
+        part.A.reset(new Assembly);
+        Assembly &A = *part.A;
         assemble(*prm, *gen, nt, xt, ns, xs, A);
-        h->nr             = A.nr;
-        h->nc             = A.nc;
-        h->perm_t         = A.perm_t;
-        h->perm_s         = A.perm_s;
-        h->false_positive = A.false_positive;
-        h->leaves         = A.leaves;
-        h->aca_feat.swap(A.aca_feat);
-        h->aca_terms.swap(A.aca_terms);
-        h->aca_cap.swap(A.aca_cap);
-        h->aca_retry        = A.aca_retry;
-        h->aca_ws_entries   = A.aca_ws_entries;
-        h->aca_need_entries = A.aca_need_entries;
-        h->aca_seconds      = A.aca_seconds;
+        h->nr     = A.nr;
+        h->nc     = A.nc;
+        h->perm_t = A.perm_t;
+        h->perm_s = A.perm_s;
+        h->leaves = A.leaves;
+        take_aca_stats(*h, A);
         sort_leaves(h->leaves);
+        part.leaf_begin = 0;
+        part.leaf_end   = h->leaves.size();
+        h->parts.push_back(std::move(part));
         build_layout(*h);
         upload_layout(*h);
         alloc_master(*h);
-        fill_from_assembly(*h, A);
+        fill_from_assembly(*h, h->parts[0]);
         h->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         *out             = h.release();
     });
@@ -671,27 +772,179 @@ int htg_hmatrix_set_storage(htg_hmatrix *h, int storage) {
     return guard([&] {
         if (!h)
             throw Error(HTG_ERR_INVALID, "null handle");
-        for (int c = 0; c < 2; c++) {
-            ClassLayout &C = h->cls[c];
-            const bool low = c == 1 && (storage == HTG_STORAGE_FP32_FP16 || storage == HTG_STORAGE_FP32_BF16);
-            if (storage == HTG_STORAGE_FP64)
-                continue;
-            if (storage != HTG_STORAGE_FP32 && storage != HTG_STORAGE_FP32_FP16 && storage != HTG_STORAGE_FP32_BF16)
-                throw Error(HTG_ERR_UNSUPPORTED, "unknown storage mode " + std::to_string(storage));
-            if (!low) {
-                ensure_copy(C.a32, C.a64, C.a64.n);
-                ensure_copy(C.v32, C.v64, C.v64.n);
-            } else if (storage == HTG_STORAGE_FP32_FP16) {
-                ensure_copy(C.a16, C.a64, C.a64.n);
-                ensure_copy(C.v16, C.v64, C.v64.n);
-            } else {
-                ensure_copy(C.ab16, C.a64, C.a64.n);
-                ensure_copy(C.vb16, C.v64, C.v64.n);
-            }
-        }
-        HTG_HIP(hipDeviceSynchronize());
-        h->storage = storage;
+        apply_storage(*h, storage);
     });
+}
+
+int htg_hmatrix_build_batch(const htg_params *prm, int count, const htg_generator *gens, const int *n,
+                            const double *const *pts, htg_hmatrix **out) {
+    return guard([&] {
+        if (!prm || count <= 0 || !gens || !n || !pts || !out)
+            throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments for building a batch of HMatrix");
+        long long total = 0;
+        for (int i = 0; i < count; i++) {
+            if (n[i] <= 0 || !pts[i])
+                throw Error(HTG_ERR_INVALID, "[Htool error] instance " + std::to_string(i) + " has no points");
+            total += n[i];
+        }
+        if (total > (1LL << 30))
+            throw Error(HTG_ERR_INVALID, "batch too large");
+        validate_params(*prm);
+        require_device();
+        const auto t0 = std::chrono::steady_clock::now();
+        std::unique_ptr<htg_hmatrix> h(new htg_hmatrix);
+        int shift = 0;
+        for (int i = 0; i < count; i++) {
+            Part part;
+            part.A.reset(new Assembly);
+            htg_generator g = gens[i];
+            g.same_points   = 1;
+            assemble(*prm, g, n[i], pts[i], n[i], nullptr, *part.A);
+            Assembly &A = *part.A;
+            for (int v : A.perm_t)
+                h->perm_t.push_back(v + shift);
+            for (int v : A.perm_s)
+                h->perm_s.push_back(v + shift);
+            for (Leaf L : A.leaves) {
+                L.oi += shift;
+                L.oj += shift;
+                h->leaves.push_back(L);
+            }
+            std::vector<Leaf>().swap(A.leaves);
+            take_aca_stats(*h, A);
+            part.shift = shift;
+            h->parts.push_back(std::move(part));
+            shift += n[i];
+        }
+        h->nr = h->nc = shift;
+        sort_leaves(h->leaves);
+        for (int i = 0; i < count; i++)
+            locate_part(*h, h->parts[i], n[i]);
+        build_layout(*h);
+        upload_layout(*h);
+        alloc_master(*h);
+        for (const Part &part : h->parts)
+            fill_from_assembly(*h, part);
+        h->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *out             = h.release();
+    });
+}
+
+int htg_hmatrix_get_layout(const htg_hmatrix *h, htg_layout *layout) {
+    return guard([&] {
+        if (!h || !layout)
+            throw Error(HTG_ERR_INVALID, "null argument");
+        *layout = h->lay;
+    });
+}
+
+int htg_hmatrix_set_layout(htg_hmatrix *h, const htg_layout *layout) {
+    return guard([&] {
+        if (!h || !layout)
+            throw Error(HTG_ERR_INVALID, "null argument");
+        check_layout(*layout);
+        const htg_layout old = h->lay;
+        h->lay               = *layout;
+        try {
+            relayout(*h);
+        } catch (...) {
+            h->lay = old;
+            throw;
+        }
+    });
+}
+
+int htg_hmatrix_autotune(htg_hmatrix *h, int mu, int reps, htg_layout *best, double *best_ms, int *ncandidates) {
+    return guard([&] {
+        if (!h || mu <= 0 || reps <= 0)
+            throw Error(HTG_ERR_INVALID, "invalid autotune arguments");
+        if (h->parts.empty())
+            throw Error(HTG_ERR_UNSUPPORTED, "autotuning needs an H-matrix assembled by htg_hmatrix_build or "
+                                             "htg_hmatrix_build_batch");
+        DevArray<double> dx, dy;
+        dx.alloc((size_t)h->nc * mu);
+        dy.alloc((size_t)h->nr * mu);
+        dx.zero();
+        hipEvent_t e0, e1;
+        HTG_HIP(hipEventCreate(&e0));
+        HTG_HIP(hipEventCreate(&e1));
+        struct EvGuard {
+            hipEvent_t a, b;
+            ~EvGuard() {
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+            }
+        } eg{e0, e1};
+        const htg_layout start = h->lay;
+        htg_layout top         = start;
+        double top_ms          = 1e300;
+        int tried              = 0;
+        for (int rb : {64, 128})
+            for (int chunk : {4096, 8192, 16384})
+                for (int vrows : {16, 32}) {
+                    htg_layout L{rb, chunk, chunk / 4, start.vgroup_tiles, vrows};
+                    h->lay = L;
+                    relayout(*h);
+                    run_product(*h, dx.p, dy.p, mu, nullptr); // warm-up
+                    HTG_HIP(hipEventRecord(e0, nullptr));
+                    for (int r = 0; r < reps; r++)
+                        run_product(*h, dx.p, dy.p, mu, nullptr);
+                    HTG_HIP(hipEventRecord(e1, nullptr));
+                    HTG_HIP(hipEventSynchronize(e1));
+                    float ms = 0;
+                    HTG_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    tried++;
+                    if (ms / reps < top_ms) {
+                        top_ms = ms / reps;
+                        top    = L;
+                    }
+                }
+        h->lay = top;
+        relayout(*h);
+        if (best)
+            *best = top;
+        if (best_ms)
+            *best_ms = top_ms;
+        if (ncandidates)
+            *ncandidates = tried;
+    });
+}
+
+int htg_graph_capture(htg_hmatrix *h, const double *d_x, double *d_y, int mu, void *stream, htg_graph **out) {
+    return guard([&] {
+        if (!h || !d_x || !d_y || mu <= 0 || !stream || !out)
+            throw Error(HTG_ERR_INVALID, "invalid graph capture arguments (a stream is required)");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        // one eager product first: every lazily allocated buffer exists before capture
+        run_product(*h, d_x, d_y, mu, st);
+        HTG_HIP(hipStreamSynchronize(st));
+        std::unique_ptr<htg_graph> g(new htg_graph);
+        HTG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            run_product(*h, d_x, d_y, mu, st);
+        } catch (...) {
+            hipGraph_t dummy = nullptr;
+            (void)hipStreamEndCapture(st, &dummy);
+            if (dummy)
+                (void)hipGraphDestroy(dummy);
+            throw;
+        }
+        HTG_HIP(hipStreamEndCapture(st, &g->graph));
+        HTG_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        *out = g.release();
+    });
+}
+
+int htg_graph_launch(htg_graph *g, void *stream) {
+    return guard([&] {
+        if (!g || !g->exec)
+            throw Error(HTG_ERR_INVALID, "null graph");
+        HTG_HIP(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+    });
+}
+
+int htg_graph_destroy(htg_graph *g) {
+    return guard([&] { delete g; });
 }
 
 int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info) {

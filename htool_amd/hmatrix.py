@@ -96,10 +96,7 @@ class HMatrix:
         same = xs is None
         if not same:
             xs = np.ascontiguousarray(xs, dtype=np.float64)
-        prm = capi.htg_params(self.epsilon, self.eta, self.maxblocksize, self.minclustersize, self.compressor,
-                              self.reqrank, int(self.lowprec_all), self.recompress_tol, self.lowprec_tol,
-                              self.predictor.handle if self.predictor is not None else None,
-                              self.predictor_margin)
+        prm = self._params()
         g = gen._c(same)
         h = C.c_void_p()
         capi.check(L.htg_hmatrix_build(C.byref(prm), C.byref(g), xt.shape[0], capi.ptr(xt),
@@ -108,6 +105,57 @@ class HMatrix:
         self._h = h
         self._storage = "fp64"
         return self
+
+    def _params(self) -> capi.htg_params:
+        return capi.htg_params(self.epsilon, self.eta, self.maxblocksize, self.minclustersize, self.compressor,
+                               self.reqrank, int(self.lowprec_all), self.recompress_tol, self.lowprec_tol,
+                               self.predictor.handle if self.predictor is not None else None,
+                               self.predictor_margin)
+
+    def build_batch(self, gens, point_sets) -> "HMatrix":
+        """Many independent square H-matrices as one block-diagonal H-matrix
+        diag(H_0, ..., H_{K-1}); x and y stack the instances' vectors."""
+        L = capi.lib()
+        pts = [np.ascontiguousarray(p, dtype=np.float64) for p in point_sets]
+        if len(gens) != len(pts) or not pts:
+            raise ValueError("one generator per point set")
+        n = np.array([p.shape[0] for p in pts], np.int32)
+        garr = (capi.htg_generator * len(pts))(*[g._c(True) for g in gens])
+        parr = (C.c_void_p * len(pts))(*[p.ctypes.data for p in pts])
+        prm = self._params()
+        h = C.c_void_p()
+        capi.check(L.htg_hmatrix_build_batch(C.byref(prm), len(pts), garr, capi.ptr(n), parr, C.byref(h)))
+        self._close()
+        self._h = h
+        self._storage = "fp64"
+        self.instance_sizes = n
+        return self
+
+    # -- work decomposition -------------------------------------------------------
+    def get_layout(self) -> dict:
+        lay = capi.htg_layout()
+        capi.check(capi.lib().htg_hmatrix_get_layout(self._handle(), C.byref(lay)))
+        return {f: getattr(lay, f) for f, _ in capi.htg_layout._fields_}
+
+    def set_layout(self, **kw) -> None:
+        cur = self.get_layout()
+        cur.update(kw)
+        lay = capi.htg_layout(**cur)
+        capi.check(capi.lib().htg_hmatrix_set_layout(self._handle(), C.byref(lay)))
+
+    def autotune(self, mu: int = 1, reps: int = 10):
+        """Times each candidate layout in the current storage, keeps the fastest;
+        returns (layout dict, ms per product, candidates tried)."""
+        lay = capi.htg_layout()
+        ms = C.c_double()
+        cnt = C.c_int()
+        capi.check(capi.lib().htg_hmatrix_autotune(self._handle(), mu, reps, C.byref(lay), C.byref(ms),
+                                                   C.byref(cnt)))
+        return {f: getattr(lay, f) for f, _ in capi.htg_layout._fields_}, ms.value, cnt.value
+
+    def capture(self, d_x: int, d_y: int, mu: int, stream: int) -> "ProductGraph":
+        """One product of device buffers captured in a HIP graph on `stream`."""
+        return ProductGraph(self, d_x, d_y, mu, stream)
 
     @classmethod
     def from_leaves(cls, nr: int, nc: int, perm_t, perm_s, leaves, data) -> "HMatrix":
@@ -241,3 +289,26 @@ class HMatrix:
     def flops(self, mu: int = 1) -> int:
         i = self.get_info()
         return int(2 * mu * (i.dense_entries + i.lowrank_entries))
+
+
+class ProductGraph:
+    """A captured product (htg_graph_capture); `launch(stream)` replays it. Keeps
+    the H-matrix alive; capture again after a storage or layout change."""
+
+    def __init__(self, H: HMatrix, d_x: int, d_y: int, mu: int, stream: int):
+        self._H = H
+        g = C.c_void_p()
+        capi.check(capi.lib().htg_graph_capture(H._handle(), C.c_void_p(d_x), C.c_void_p(d_y), mu,
+                                                C.c_void_p(stream), C.byref(g)))
+        self._g = g
+
+    def launch(self, stream: int) -> None:
+        capi.check(capi.lib().htg_graph_launch(self._g, C.c_void_p(stream)))
+
+    def __del__(self):
+        try:
+            if self._g is not None and self._g.value:
+                capi.lib().htg_graph_destroy(self._g)
+        except Exception:
+            pass
+        self._g = None

@@ -113,7 +113,20 @@ typedef struct htg_info {
     double aca_seconds;       /* batched ACA (kernels and host scheduling) */
 } htg_info;
 
+/* Work decomposition of the product (no counterpart in the reference): rows
+ * per row block of the near-field/U phase (16..128), target entries per V
+ * chunk and per phase-1 wavefront, most chunks per wavefront, and stacked V
+ * rows per chunk (the rank bucket of a source-cluster panel). */
+typedef struct htg_layout {
+    int rb_max;
+    int vchunk_el;
+    int vgroup_el;
+    int vgroup_tiles;
+    int vrows;
+} htg_layout;
+
 typedef struct htg_hmatrix htg_hmatrix;
+typedef struct htg_graph htg_graph;
 
 const char *htg_last_error(void);
 const char *htg_status_string(int status);
@@ -132,7 +145,36 @@ int htg_hmatrix_build(const htg_params *params, const htg_generator *gen, int nt
 int htg_hmatrix_from_leaves(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
                             const double *data, htg_hmatrix **out);
 
+/* Many independent square H-matrices assembled as one: instance i has n[i]
+ * points pts[i] (interleaved xyz) and generator gens[i]; the result is the
+ * block-diagonal H-matrix diag(H_0, ..., H_{count-1}) of size sum(n), whose
+ * product runs every instance's leaves in the same launches. x and y stack the
+ * instances' vectors in instance order (original ordering within each). Each
+ * instance follows HMatrix::build (include/htool/types/hmatrix.hpp:285-341). */
+int htg_hmatrix_build_batch(const htg_params *params, int count, const htg_generator *gens, const int *n,
+                            const double *const *pts, htg_hmatrix **out);
+
 int htg_hmatrix_destroy(htg_hmatrix *h);
+
+/* Product work decomposition: read, or set and rebuild the product layout
+ * (matrices from htg_hmatrix_build / _build_batch only). */
+int htg_hmatrix_get_layout(const htg_hmatrix *h, htg_layout *layout);
+int htg_hmatrix_set_layout(htg_hmatrix *h, const htg_layout *layout);
+
+/* Autotuner: times `reps` products with `mu` right-hand sides in the current
+ * storage for each candidate layout (rows per row block x V chunk size x
+ * stacked V rows), keeps the fastest and returns it with its time (ms per
+ * product) and the number of candidates tried. */
+int htg_hmatrix_autotune(htg_hmatrix *h, int mu, int reps, htg_layout *best, double *best_ms, int *ncandidates);
+
+/* Captures one product of device buffers d_x -> d_y (mu right-hand sides, the
+ * current storage) into a HIP graph on `stream` (a hipStream_t, not NULL);
+ * htg_graph_launch replays it on a stream. The graph reads the H-matrix's
+ * device data: destroy it before the H-matrix, and capture again after
+ * htg_hmatrix_set_storage / _set_layout. */
+int htg_graph_capture(htg_hmatrix *h, const double *d_x, double *d_y, int mu, void *stream, htg_graph **out);
+int htg_graph_launch(htg_graph *g, void *stream);
+int htg_graph_destroy(htg_graph *g);
 
 /* Adds (or keeps) the storage variant used by the next products. */
 int htg_hmatrix_set_storage(htg_hmatrix *h, int storage);
