@@ -276,7 +276,7 @@ def main():
         torch.cuda.synchronize()
         dx = torch.from_numpy(x).cuda()
         dy = torch.zeros(n, dtype=torch.float64, device="cuda")
-        for M, storage in ((Hm, "fp32_fp16"), (H, "fp64")):
+        for M, storage in ((Hm, "fp32_fp16"), (H, "fp32"), (H, "fp64")):
             M.set_storage(storage)
             for _ in range(3):
                 M.mvprod_device(dx.data_ptr(), dy.data_ptr(), 1, torch.cuda.current_stream().cuda_stream)
@@ -378,6 +378,7 @@ def main():
                    "mixed_rank_mean": round(mi.rank_mean, 3), "mixed_lowrank_entries": mi.lowrank_entries,
                    "mixed_fp16_entries": mi.lowprec_entries},
         "kernels_ms": {k: round(v, 5) for k, v in p.items()},
+        "kernel_algorithmic_bytes": {k: int(v[1]) for k, v in kern.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
