@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched product")
+    ap.add_argument("--no-helmholtz", action="store_true", help="skip the complex Helmholtz N=2e5 workload")
     ap.add_argument("--no-assembly-5e5", action="store_true",
                     help="skip the N=5e5 assembly with and without the learned rank predictor")
     ap.add_argument("--cpu-reps", type=int, default=10)
@@ -183,6 +184,61 @@ def batch_64(torch, log):
         out["separate_launches_ms"] = round(timed(all_singles, 20, 3, s, torch), 4)
     out["speedup_vs_separate"] = round(out["separate_launches_ms"] / out["graph_ms"], 2)
     log(f"batch of 64: {out}")
+    return out
+
+
+def helmholtz_2e5(torch, log):
+    """N=2e5 points on a 4:1 prolate spheroid (semi-axes 1, 1, 4, area-uniform),
+    e^{ikr}/(4 pi r) with k = 2 pi / (10 h) (10 points per wavelength),
+    h = sqrt(area / N), 1/(4 pi h) on the diagonal, epsilon 1e-5, eta 2, leaf
+    size 32. Every low-rank term per-term scaled in 16-bit storage; dense leaves
+    FP32 (mixed) against the FP64 complex product; 1 and 64 right-hand sides
+    CN(0,1)."""
+    import numpy as np
+    from htool_amd import Generator, HMatrix, geometry
+    from oracle import lib as orc
+    n = 200000
+    pts = geometry.ellipsoid(n, (1.0, 1.0, 4.0), seed=5)
+    h = math.sqrt(geometry.prolate_spheroid_area(1.0, 4.0) / n)
+    k = 2 * math.pi / (10 * h)
+    diag = geometry.laplace_diag(h)
+    H = HMatrix(epsilon=1e-5, eta=2.0, minclustersize=32)
+    H.set_compression("partialACA")
+    H.set_lowprec(0.0, all_terms=True)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    H.build(Generator(geometry.KERNEL_HELMHOLTZ, 0.0, diag, k), pts)
+    torch.cuda.synchronize()
+    i = H.get_info()
+    out = {"N": n, "wavenumber": round(k, 4), "h": round(h, 6), "epsilon": 1e-5, "eta": 2.0, "leaf_size": 32,
+           "build_s": round(time.time() - t0, 3), "aca_s": round(i.aca_seconds, 3), "rank_mean": round(i.rank_mean, 2),
+           "rank_max": i.rank_max, "dense_entries": int(i.dense_entries), "lowrank_entries": int(i.lowrank_entries)}
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((n, 64)) + 1j * rng.standard_normal((n, 64))
+    dX = torch.from_numpy(np.asfortranarray(X).reshape(-1, order="F").copy()).cuda()
+    dY = torch.zeros_like(dX)
+    sp = torch.cuda.current_stream().cuda_stream
+    ys = {}
+    for storage in ("fp64", "fp32", "fp32_fp16", "fp32_bf16"):
+        H.set_storage(storage)
+        r = {}
+        for mu in (1, 64):
+            ms = timed(lambda: H.mvprod_device(dX.data_ptr(), dY.data_ptr(), mu, sp), 5 if mu == 1 else 2, 1,
+                       torch.cuda.current_stream(), torch)
+            r[f"ms_mu{mu}"] = round(ms, 4)
+            r[f"gbps_mu{mu}"] = round(H.algorithmic_bytes(mu, storage) / (ms * 1e-3) / 1e9, 1)
+        H.mvprod_device(dX.data_ptr(), dY.data_ptr(), 64, sp)
+        torch.cuda.synchronize()
+        ys[storage] = dY.cpu().numpy().view(np.complex128).reshape(64, n).T.copy()
+        out[storage] = r
+    rows = rng.choice(n, 200, replace=False).astype(np.int32)
+    yd = orc.dense_rows_c(pts, None, rows, X[:, 0], kernel=4, wavenumber=k, diag=diag, same=True)
+    for storage in ys:
+        out[storage]["rel_err_vs_fp64_hmat"] = float(np.linalg.norm(ys[storage] - ys["fp64"]) /
+                                                     np.linalg.norm(ys["fp64"]))
+        out[storage]["rel_err_vs_dense_200_rows"] = float(np.linalg.norm(ys[storage][rows, 0] - yd) /
+                                                          np.linalg.norm(yd))
+    log(f"Helmholtz N=2e5: {out}")
     return out
 
 
@@ -335,6 +391,7 @@ def main():
 
     asm = None if args.no_assembly_5e5 else assembly_5e5(torch, log)
     batch = None if args.no_batch else batch_64(torch, log)
+    helm = None if args.no_helmholtz else helmholtz_2e5(torch, log)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -384,6 +441,7 @@ def main():
         "cpu_baseline": cpu,
         "assembly_5e5": asm,
         "batch_64": batch,
+        "helmholtz_2e5": helm,
     }
     print(json.dumps(line), flush=True)
 

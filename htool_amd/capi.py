@@ -31,7 +31,8 @@ SYM_PARTIAL_ACA = 1
 
 
 class htg_generator(C.Structure):
-    _fields_ = [("kernel", C.c_int), ("same_points", C.c_int), ("delta", C.c_double), ("diag", C.c_double)]
+    _fields_ = [("kernel", C.c_int), ("same_points", C.c_int), ("delta", C.c_double), ("diag", C.c_double),
+                ("wavenumber", C.c_double)]
 
 
 class htg_params(C.Structure):
@@ -55,7 +56,7 @@ class htg_info(C.Structure):
                 ("n_columns", C.c_int),
                 ("n_vtiles", C.c_int), ("build_seconds", C.c_double), ("n_aca", C.c_int),
                 ("n_aca_retry", C.c_int), ("aca_ws_entries", C.c_int64), ("aca_need_entries", C.c_int64),
-                ("aca_seconds", C.c_double)]
+                ("aca_seconds", C.c_double), ("is_complex", C.c_int)]
 
 
 class HtgError(RuntimeError):

@@ -36,9 +36,9 @@ void PointsOnDevice::upload(const double *pts, const std::vector<int> &perm_h) {
 }
 
 KernelDesc kernel_desc(const htg_generator &g, bool same) {
-    if (g.kernel < 0 || g.kernel > 3)
+    if (g.kernel < 0 || g.kernel > HTG_KERNEL_COMPLEX_TEST)
         throw Error(HTG_ERR_UNSUPPORTED, "unknown kernel family " + std::to_string(g.kernel));
-    return KernelDesc{g.kernel, same ? 1 : 0, g.delta, g.diag};
+    return KernelDesc{g.kernel, same ? 1 : 0, g.delta, g.diag, g.wavenumber};
 }
 
 namespace {
@@ -63,8 +63,9 @@ struct Assembler {
     // compressed store: U (M x k col-major) then V (k x N row-major) per low-rank leaf
     std::vector<DevArray<double> *> store_chunks;
     DevArray<double> store;
-    uint64_t store_used = 0;
+    uint64_t store_used = 0;     // entries (complex entries for the complex families)
     Assembly *rec = nullptr; // ACA records and counters
+    int esz       = 1;       // doubles per entry
 
     void expand(int id) {
         const ClusterNode &a = T.nodes[nodes[id].t], &b = S.nodes[nodes[id].s];
@@ -206,11 +207,15 @@ struct Assembler {
                 DevArray<int> d_ranks;
                 d_blocks.upload(blocks);
                 d_vis_off.upload(vis_off);
-                d_ws.alloc(ws_used);
+                d_ws.alloc(ws_used * esz);
                 d_vis.alloc(vis_used);
                 d_ranks.alloc(blocks.size());
-                launch_aca(kd, dT, dS, d_blocks.p, (int)blocks.size(), prm.epsilon, prm.reqrank, d_ws.p, d_vis.p,
-                           d_vis_off.p, d_ranks.p, nullptr);
+                if (esz == 2)
+                    launch_aca_c(kd, dT, dS, d_blocks.p, (int)blocks.size(), prm.epsilon, prm.reqrank,
+                                 reinterpret_cast<cdbl *>(d_ws.p), d_vis.p, d_vis_off.p, d_ranks.p, nullptr);
+                else
+                    launch_aca(kd, dT, dS, d_blocks.p, (int)blocks.size(), prm.epsilon, prm.reqrank, d_ws.p, d_vis.p,
+                               d_vis_off.p, d_ranks.p, nullptr);
                 HTG_HIP(hipGetLastError());
                 std::vector<int> ranks(blocks.size());
                 HTG_HIP(hipMemcpy(ranks.data(), d_ranks.p, ranks.size() * sizeof(int), hipMemcpyDeviceToHost));
@@ -271,7 +276,12 @@ struct Assembler {
                     DevArray<UTileDst> d_udst;
                     d_packs.upload(packs);
                     d_udst.upload(udst);
-                    launch_lr_pack(d_packs.p, (int)packs.size(), d_udst.p, d_ws.p, store.p, store.p, nullptr);
+                    if (esz == 2)
+                        launch_lr_pack_c(d_packs.p, (int)packs.size(), d_udst.p, reinterpret_cast<cdbl *>(d_ws.p),
+                                         reinterpret_cast<cdbl *>(store.p), reinterpret_cast<cdbl *>(store.p),
+                                         nullptr);
+                    else
+                        launch_lr_pack(d_packs.p, (int)packs.size(), d_udst.p, d_ws.p, store.p, store.p, nullptr);
                     HTG_HIP(hipGetLastError());
                     HTG_HIP(hipDeviceSynchronize());
                 }
@@ -281,14 +291,14 @@ struct Assembler {
         }
     }
 
-    void grow_store(uint64_t need) {
-        if (need <= store.n)
+    void grow_store(uint64_t need) { // need: entries
+        if (need * esz <= store.n)
             return;
-        uint64_t cap = std::max<uint64_t>(need, store.n + store.n / 2 + (1u << 20));
+        uint64_t cap = std::max<uint64_t>(need * esz, store.n + store.n / 2 + (1u << 20));
         double *np   = nullptr;
         HTG_HIP(hipMalloc(&np, cap * sizeof(double)));
         if (store.p && store_used)
-            HTG_HIP(hipMemcpy(np, store.p, store_used * sizeof(double), hipMemcpyDeviceToDevice));
+            HTG_HIP(hipMemcpy(np, store.p, store_used * esz * sizeof(double), hipMemcpyDeviceToDevice));
         store.reset();
         store.p = np;
         store.n = cap;
@@ -315,7 +325,10 @@ void assemble(const htg_params &prm, const htg_generator &gen, int nt, const dou
     if (!same)
         res.pS.upload(xs, S.perm);
 
-    Assembler as{T, S, prm, res.kd, res.dT(), res.dS(), {}, {}, {}, 0, {}, {}, 0, &res};
+    res.cplx = kernel_is_complex(res.kd.kind);
+    if (res.cplx && prm.recompress_tol > 0)
+        throw Error(HTG_ERR_UNSUPPORTED, "QR+SVD recompression is implemented for the real kernel families only");
+    Assembler as{T, S, prm, res.kd, res.dT(), res.dS(), {}, {}, {}, 0, {}, {}, 0, &res, res.cplx ? 2 : 1};
     const std::vector<BlockTask> tasks = build_block_tasks(T, S, prm.eta, (size_t)prm.maxblocksize);
     std::vector<int> roots(tasks.size(), -1);
     for (size_t p = 0; p < tasks.size(); p++) {
@@ -356,16 +369,47 @@ void assemble(const htg_params &prm, const htg_generator &gen, int nt, const dou
 
     for (Leaf &L : lr_leaves)
         L.k_hi = L.k;
-    if (prm.recompress_tol > 0 || prm.lowprec_all)
-        recompress_and_split(prm, lr_leaves, as.store);
+    if (prm.recompress_tol > 0 || prm.lowprec_all || prm.lowprec_tol > 0)
+        recompress_and_split(prm, lr_leaves, as.store, res.cplx);
     res.leaves = as.out;
     res.leaves.insert(res.leaves.end(), lr_leaves.begin(), lr_leaves.end());
     res.false_positive = as.false_positive;
     res.store.take(as.store);
 }
 
-void recompress_and_split(const htg_params &prm, std::vector<Leaf> &leaves, DevArray<double> &store) {
-    if (prm.recompress_tol > 0) {
+void recompress_and_split(const htg_params &prm, std::vector<Leaf> &leaves, DevArray<double> &store, bool cplx) {
+    if (prm.recompress_tol <= 0) {
+        // No recompression: each ACA term gets unit factors and a scale, so a
+        // 16-bit class holds every term near unit magnitude.
+        std::vector<ScaleLeaf> sl;
+        std::vector<int> which;
+        int off = 0;
+        for (size_t l = 0; l < leaves.size(); l++) {
+            const Leaf &L = leaves[l];
+            if (L.k > 0) {
+                sl.push_back(ScaleLeaf{L.src_u, L.src_v, L.m, L.n, L.k, off});
+                which.push_back((int)l);
+                off += L.k;
+            }
+        }
+        if (!sl.empty()) {
+            DevArray<ScaleLeaf> d_sl;
+            DevArray<double> d_sig;
+            d_sl.upload(sl);
+            d_sig.alloc(off);
+            if (cplx)
+                launch_term_scale_c(d_sl.p, (int)sl.size(), reinterpret_cast<cdbl *>(store.p), d_sig.p, nullptr);
+            else
+                launch_term_scale(d_sl.p, (int)sl.size(), store.p, d_sig.p, nullptr);
+            HTG_HIP(hipGetLastError());
+            std::vector<double> sig(off);
+            HTG_HIP(hipMemcpy(sig.data(), d_sig.p, sig.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < sl.size(); i++) {
+                Leaf &L = leaves[which[i]];
+                L.sig.assign(sig.begin() + sl[i].sig_off, sig.begin() + sl[i].sig_off + L.k);
+            }
+        }
+    } else {
         std::vector<RcLeaf> rc;
         std::vector<int> which;
         for (size_t l = 0; l < leaves.size(); l++) {
@@ -400,9 +444,9 @@ void recompress_and_split(const htg_params &prm, std::vector<Leaf> &leaves, DevA
             }
         }
     }
-    // Precision classes. Terms are ordered by decreasing scale; the trailing
-    // terms go to class 1 (16-bit storage) while the error they add,
-    // 2 * 2^-11 * sigma_j summed, stays within lowprec_tol * ||sigma||_2.
+    // Precision classes. The trailing terms (after recompression, the
+    // smallest singular values) go to class 1 (16-bit storage) while the error
+    // they add, 2 * 2^-11 * sigma_j summed, stays within lowprec_tol * ||sigma||_2.
     const double u16 = std::ldexp(1.0, -11);
     for (Leaf &L : leaves) {
         if (L.k <= 0)

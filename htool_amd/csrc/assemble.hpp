@@ -23,9 +23,10 @@ struct Assembly {
     std::vector<int> perm_t, perm_s;
     std::vector<Leaf> leaves; // in no particular order
     int false_positive = 0;
-    DevArray<double> store;   // U and V of the low-rank leaves (Leaf::src_u, src_v)
+    DevArray<double> store;   // U and V of the low-rank leaves (Leaf::src_u, src_v; complex: 2 doubles per entry)
     KernelDesc kd{};
     bool same = true;
+    bool cplx = false; // complex kernel family: the store holds complex<double> entries
     PointsOnDevice pT, pS;
     // ACA runs of admissible blocks (first attempts): features, terms needed,
     // first workspace terms; retries and workspace totals.
@@ -46,6 +47,7 @@ void assemble(const htg_params &prm, const htg_generator &gen, int nt, const dou
 
 // Recompresses the low-rank leaves in the store (FP64) and sets each leaf's
 // per-term scales; then assigns precision classes (Leaf::k_hi).
-void recompress_and_split(const htg_params &prm, std::vector<Leaf> &leaves, DevArray<double> &store);
+// Without recompression (tol 0) each term is scaled to unit factors instead.
+void recompress_and_split(const htg_params &prm, std::vector<Leaf> &leaves, DevArray<double> &store, bool cplx);
 
 } // namespace htg

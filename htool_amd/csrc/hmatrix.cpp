@@ -98,6 +98,8 @@ struct htg_hmatrix {
     DevArray<int> d_iperm_s;      // inverse column permutation (32-RHS gather)
     DevArray<float> d_xstage;     // 32-RHS gather staging, nc x 32
     int storage = HTG_STORAGE_FP64;
+    bool cplx   = false; // complex<double> entries, x and y
+    int esz() const { return cplx ? 2 : 1; }
 };
 
 namespace htg {
@@ -364,8 +366,8 @@ void upload_layout(htg_hmatrix &h) {
 // FP64 master arenas of both classes, zero filled (row padding stays zero).
 void alloc_master(htg_hmatrix &h) {
     for (ClassLayout &C : h.cls) {
-        C.a64.alloc(C.arena_size);
-        C.v64.alloc(C.varena_size);
+        C.a64.alloc(C.arena_size * h.esz());
+        C.v64.alloc(C.varena_size * h.esz());
         C.a64.zero();
         C.v64.zero();
     }
@@ -438,9 +440,57 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
     HTG_HIP(hipGetLastError());
 }
 
+// Complex product: the same layout with (re, im) pairs; at most 16 RHS per pass.
+template <typename TS0, typename TS1, typename R>
+void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t st, hipEvent_t *ev) {
+    R *z                 = reinterpret_cast<R *>(h.d_z.p);
+    const double *tscale = h.scaled ? h.d_tscale.p : nullptr;
+    const bool has1      = !h.cls[1].empty();
+    const RowClass rc0{h.cls[0].d_rb_col0.p, h.cls[0].d_rb_aoff.p, h.cls[0].d_colidx.p, CLASS_ALIGN[0]};
+    const RowClass rc1{h.cls[1].d_rb_col0.p, h.cls[1].d_rb_aoff.p, h.cls[1].d_colidx.p, CLASS_ALIGN[1]};
+    int done = 0;
+    while (done < mu) {
+        const int MU = pick_mu_chunk(mu - done, 16);
+        if (ev)
+            HTG_HIP(hipEventRecord(ev[0], st));
+        launch_gather_xc<R>(d_x, h.d_perm_s.p, h.nc, h.tpad, done, MU, z, st);
+        if (ev)
+            HTG_HIP(hipEventRecord(ev[1], st));
+        const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
+        launch_vx_c<TS0, TS1, R>(h.d_vg_off.p, (int)h.vg_off.size() - 1, h.d_vtiles.p, h.d_multi_first.p,
+                                 (int)h.multi_first.size(), arena_of<TS0>(C0, true),
+                                 has1 ? arena_of<TS1>(C1, true) : nullptr, z, h.d_part.p, tscale, MU, st);
+        if (ev)
+            HTG_HIP(hipEventRecord(ev[2], st));
+        launch_rows_c<TS0, TS1, R>(h.d_rb_order.p, h.d_rb_row0.p, (int)h.rb_row0.size() - 1, rc0,
+                                   arena_of<TS0>(C0, false), rc1, has1 ? arena_of<TS1>(C1, false) : nullptr, z,
+                                   h.d_perm_t.p, h.nr, done, MU, d_y, st);
+        if (ev)
+            HTG_HIP(hipEventRecord(ev[3], st));
+        done += MU;
+    }
+    HTG_HIP(hipGetLastError());
+}
+
 void run_product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t st, hipEvent_t *ev = nullptr) {
     if (mu <= 0)
         throw Error(HTG_ERR_INVALID, "mu must be positive");
+    if (h.cplx) {
+        switch (h.storage) {
+        case HTG_STORAGE_FP32:
+            product_c<float, float, float>(h, d_x, d_y, mu, st, ev);
+            break;
+        case HTG_STORAGE_FP32_FP16:
+            product_c<float, __half, float>(h, d_x, d_y, mu, st, ev);
+            break;
+        case HTG_STORAGE_FP32_BF16:
+            product_c<float, __hip_bfloat16, float>(h, d_x, d_y, mu, st, ev);
+            break;
+        default:
+            product_c<double, double, double>(h, d_x, d_y, mu, st, ev);
+        }
+        return;
+    }
     switch (h.storage) {
     case HTG_STORAGE_FP32:
         product<float, float, float>(h, d_x, d_y, mu, st, ev);
@@ -516,7 +566,11 @@ void fill_from_assembly(htg_hmatrix &h, const Part &part) {
     }
     DevArray<FillTile> d_fills;
     d_fills.upload(fills);
-    launch_dense_fill(A.kd, A.dT(), A.dS(), d_fills.p, (int)fills.size(), h.cls[0].a64.p, nullptr);
+    if (h.cplx)
+        launch_dense_fill_c(A.kd, A.dT(), A.dS(), d_fills.p, (int)fills.size(),
+                            reinterpret_cast<cdbl *>(h.cls[0].a64.p), nullptr);
+    else
+        launch_dense_fill(A.kd, A.dT(), A.dS(), d_fills.p, (int)fills.size(), h.cls[0].a64.p, nullptr);
     HTG_HIP(hipGetLastError());
     for (int c = 0; c < 2; c++) {
         if (packs[c].empty())
@@ -525,7 +579,12 @@ void fill_from_assembly(htg_hmatrix &h, const Part &part) {
         DevArray<UTileDst> d_udst;
         d_packs.upload(packs[c]);
         d_udst.upload(udst[c]);
-        launch_lr_pack(d_packs.p, (int)packs[c].size(), d_udst.p, A.store.p, h.cls[c].a64.p, h.cls[c].v64.p,
+        if (h.cplx)
+            launch_lr_pack_c(d_packs.p, (int)packs[c].size(), d_udst.p, reinterpret_cast<const cdbl *>(A.store.p),
+                             reinterpret_cast<cdbl *>(h.cls[c].a64.p), reinterpret_cast<cdbl *>(h.cls[c].v64.p),
+                             nullptr);
+        else
+            launch_lr_pack(d_packs.p, (int)packs[c].size(), d_udst.p, A.store.p, h.cls[c].a64.p, h.cls[c].v64.p,
                        nullptr);
         HTG_HIP(hipGetLastError());
         HTG_HIP(hipDeviceSynchronize());
@@ -680,6 +739,7 @@ int htg_hmatrix_build(const htg_params *prm, const htg_generator *gen, int nt, c
         assemble(*prm, *gen, nt, xt, ns, xs, A);
         h->nr     = A.nr;
         h->nc     = A.nc;
+        h->cplx   = A.cplx;
         h->perm_t = A.perm_t;
         h->perm_s = A.perm_s;
         h->leaves = A.leaves;
@@ -801,6 +861,10 @@ int htg_hmatrix_build_batch(const htg_params *prm, int count, const htg_generato
             g.same_points   = 1;
             assemble(*prm, g, n[i], pts[i], n[i], nullptr, *part.A);
             Assembly &A = *part.A;
+            if (i == 0)
+                h->cplx = A.cplx;
+            else if (A.cplx != h->cplx)
+                throw Error(HTG_ERR_INVALID, "a batch mixes real and complex kernel families");
             for (int v : A.perm_t)
                 h->perm_t.push_back(v + shift);
             for (int v : A.perm_s)
@@ -862,8 +926,8 @@ int htg_hmatrix_autotune(htg_hmatrix *h, int mu, int reps, htg_layout *best, dou
             throw Error(HTG_ERR_UNSUPPORTED, "autotuning needs an H-matrix assembled by htg_hmatrix_build or "
                                              "htg_hmatrix_build_batch");
         DevArray<double> dx, dy;
-        dx.alloc((size_t)h->nc * mu);
-        dy.alloc((size_t)h->nr * mu);
+        dx.alloc((size_t)h->nc * mu * h->esz());
+        dy.alloc((size_t)h->nr * mu * h->esz());
         dx.zero();
         hipEvent_t e0, e1;
         HTG_HIP(hipEventCreate(&e0));
@@ -984,6 +1048,7 @@ int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info) {
         r.aca_ws_entries    = h->aca_ws_entries;
         r.aca_need_entries  = h->aca_need_entries;
         r.aca_seconds       = h->aca_seconds;
+        r.is_complex        = h->cplx ? 1 : 0;
         *info               = r;
     });
 }
@@ -1040,7 +1105,12 @@ int htg_hmatrix_get_leaves(const htg_hmatrix *h, int *leaves5, double *data) {
             HTG_HIP(hipMemcpy(varena[c].data(), h->cls[c].v64.p, varena[c].size() * sizeof(double),
                               hipMemcpyDeviceToHost));
         }
+        const int e = h->esz(); // doubles per entry
         double *out = data;
+        auto put    = [&](double *dst, const double *src, double sc) {
+            for (int u = 0; u < e; u++)
+                dst[u] = src[u] * sc;
+        };
         for (size_t l = 0; l < h->leaves.size(); l++) {
             const Leaf &L = h->leaves[l];
             if (L.k == 0)
@@ -1052,27 +1122,27 @@ int htg_hmatrix_get_leaves(const htg_hmatrix *h, int *leaves5, double *data) {
                 const int R  = row_block_rows(*h, rb);
                 const int i0 = h->rb_row0[rb] - L.oi;
                 for (int j = 0; j < ncol; j++) {
-                    const int c      = (L.k < 0 || j < L.k_hi) ? 0 : 1;
-                    const int jc     = c == 0 ? j : j - L.k_hi;
-                    const size_t ld  = row_stride(*h, c, rb);
-                    const double *src = arena[c].data() + (c == 0 ? std::get<1>(rt) : std::get<2>(rt));
-                    const double sc  = (L.k > 0 && !L.sig.empty()) ? L.sig[j] : 1.0;
+                    const int c       = (L.k < 0 || j < L.k_hi) ? 0 : 1;
+                    const int jc      = c == 0 ? j : j - L.k_hi;
+                    const size_t ld   = row_stride(*h, c, rb);
+                    const double *src = arena[c].data() + e * (c == 0 ? std::get<1>(rt) : std::get<2>(rt));
+                    const double sc   = (L.k > 0 && !L.sig.empty()) ? L.sig[j] : 1.0;
                     for (int i = 0; i < R; i++)
-                        out[(size_t)(i0 + i) + (size_t)L.m * j] = src[i * ld + jc] * sc;
+                        put(out + e * ((size_t)(i0 + i) + (size_t)L.m * j), src + e * (i * ld + jc), sc);
                 }
             }
-            out += (size_t)L.m * ncol;
+            out += e * (size_t)L.m * ncol;
             if (L.k > 0) {
                 for (int r = 0; r < L.k; r++) {
                     const int c       = r < L.k_hi ? 0 : 1;
                     const int rc      = c == 0 ? r : r - L.k_hi;
                     const size_t np   = v_stride(L, c);
-                    const double *src = varena[c].data() + h->cls[c].leaf_v_off[l] + (L.oj - v_first(L, c)) +
-                                        (size_t)rc * np;
+                    const double *src = varena[c].data() +
+                                        e * (h->cls[c].leaf_v_off[l] + (L.oj - v_first(L, c)) + (size_t)rc * np);
                     for (int j = 0; j < L.n; j++)
-                        out[r + (size_t)L.k * j] = src[j];
+                        put(out + e * (r + (size_t)L.k * j), src + e * j, 1.0);
                 }
-                out += (size_t)L.k * L.n;
+                out += e * (size_t)L.k * L.n;
             }
         }
     });
@@ -1101,7 +1171,7 @@ int htg_hmatrix_mvprod(htg_hmatrix *h, const double *x, double *y, int mu) {
     return guard([&] {
         if (!h || !x || !y || mu <= 0)
             throw Error(HTG_ERR_INVALID, "null argument");
-        const size_t nx = (size_t)h->nc * mu, ny = (size_t)h->nr * mu;
+        const size_t nx = (size_t)h->nc * mu * h->esz(), ny = (size_t)h->nr * mu * h->esz();
         if (h->d_hx.n < nx)
             h->d_hx.alloc(nx);
         if (h->d_hy.n < ny)

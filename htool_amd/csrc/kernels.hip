@@ -44,10 +44,92 @@ __device__ __forceinline__ double kernel_entry(const KernelDesc &k, const DevPoi
     }
 }
 
+// Complex kernel families (entries are std::complex<double>: re then im).
+// Helmholtz e^{ikr}/(4 pi r) with `diag` on the diagonal of a square matrix;
+// GeneratorTestComplex (1 + i)/(4 pi r) (include/htool/testing/generator_test.hpp:110-120).
+// The distance is summed as in kernel_entry.
+__device__ __forceinline__ cdbl kernel_entry_c(const KernelDesc &k, const DevPoints &T, int i, const DevPoints &S,
+                                               int j) {
+#pragma clang fp contract(off)
+    double r2 = 0;
+    double u  = T.x[i] - S.x[j];
+    r2        = r2 + u * u;
+    u         = T.y[i] - S.y[j];
+    r2        = r2 + u * u;
+    u         = T.z[i] - S.z[j];
+    r2        = r2 + u * u;
+    const double r = sqrt(r2);
+    if (k.kind == HTG_KERNEL_HELMHOLTZ) {
+        if (k.same && T.perm[i] == S.perm[j])
+            return cdbl{k.diag, 0.0};
+        double sn, cs;
+        sincos(k.wavenumber * r, &sn, &cs);
+        const double d = 4 * M_PI * r;
+        return cdbl{cs / d, sn / d};
+    }
+    const double d = 4 * M_PI * r;
+    return cdbl{1. / d, 1. / d};
+}
+
+// Scalar algebra of the ACA, real and complex. Complex products and sums are
+// written out (no contraction) except the residual update, an FMA pair per
+// component; the oracle restates the same operations.
+__device__ __forceinline__ double axpy_s(double a, double x, double y) { return fma(a, x, y); }
+__device__ __forceinline__ cdbl axpy_s(cdbl a, cdbl x, cdbl y) {
+    return cdbl{fma(a.re, x.re, fma(-a.im, x.im, y.re)), fma(a.re, x.im, fma(a.im, x.re, y.im))};
+}
+__device__ __forceinline__ double mag(double x) { return fabs(x); }
+__device__ __forceinline__ double mag(cdbl x) { return hypot(x.re, x.im); }
+__device__ __forceinline__ double neg(double x) { return -x; }
+__device__ __forceinline__ cdbl neg(cdbl x) { return cdbl{-x.re, -x.im}; }
+__device__ __forceinline__ double recip(double x) { return 1. / x; }
+// 1 / (a + ib) by Smith's method (Smith, CACM 5(8), 1962).
+__device__ __forceinline__ cdbl recip(cdbl x) {
+#pragma clang fp contract(off)
+    if (fabs(x.re) >= fabs(x.im)) {
+        const double r = x.im / x.re, d = x.re + x.im * r;
+        return cdbl{1. / d, -r / d};
+    }
+    const double r = x.re / x.im, d = x.re * r + x.im;
+    return cdbl{r / d, -1. / d};
+}
+__device__ __forceinline__ double mul(double a, double b) { return a * b; }
+__device__ __forceinline__ cdbl mul(cdbl a, cdbl b) {
+#pragma clang fp contract(off)
+    return cdbl{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// acc + conj(a) b (conj is the identity for reals), as Blas<T>::dot
+// (include/htool/wrappers/wrapper_blas.hpp:111-117) accumulates.
+__device__ __forceinline__ double cdot_acc(double acc, double a, double b) { return acc + a * b; }
+__device__ __forceinline__ cdbl cdot_acc(cdbl acc, cdbl a, cdbl b) {
+#pragma clang fp contract(off)
+    return cdbl{acc.re + (a.re * b.re + a.im * b.im), acc.im + (a.re * b.im - a.im * b.re)};
+}
+__device__ __forceinline__ double add(double a, double b) { return a + b; }
+__device__ __forceinline__ cdbl add(cdbl a, cdbl b) { return cdbl{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ double real_part(double a) { return a; }
+__device__ __forceinline__ double real_part(cdbl a) { return a.re; }
+__device__ __forceinline__ double shfl_xor_s(double v, int o) { return __shfl_xor(v, o, 64); }
+__device__ __forceinline__ cdbl shfl_xor_s(cdbl v, int o) {
+    return cdbl{__shfl_xor(v.re, o, 64), __shfl_xor(v.im, o, 64)};
+}
+template <typename T>
+__device__ __forceinline__ T zero_s() {
+    return T{};
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
         v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum_s(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = add(v, shfl_xor_s(v, o));
     return v;
 }
 
@@ -68,13 +150,14 @@ __device__ double block_sum(double v, double *s_red) {
 // Index of the largest |vec[b]| over unvisited b; ties go to the largest index,
 // as the reference's `if (tmp < pivot) continue;` scan does
 // (include/htool/lrmat/partialACA.hpp:89-100). -1 when every entry is visited.
-__device__ int block_argmax_last(const double *vec, const unsigned char *visited, int n, double *s_v, int *s_i) {
+template <typename T>
+__device__ int block_argmax_last(const T *vec, const unsigned char *visited, int n, double *s_v, int *s_i) {
     double best = -1.0;
     int bi      = -1;
     for (int b = threadIdx.x; b < n; b += WG) {
         if (visited[b])
             continue;
-        const double a = fabs(vec[b]);
+        const double a = mag(vec[b]);
         if (a >= best) {
             best = a;
             bi   = b;
@@ -118,6 +201,29 @@ __global__ void __launch_bounds__(WG) k_dense_fill(KernelDesc k, DevPoints T, De
     }
 }
 
+__global__ void __launch_bounds__(WG) k_dense_fill_c(KernelDesc k, DevPoints T, DevPoints S, const FillTile *tiles,
+                                                     cdbl *arena) {
+    const FillTile ft = tiles[blockIdx.x];
+    const int total   = ft.R * ft.ncols;
+    for (int e = threadIdx.x; e < total; e += WG) {
+        const int i = e / ft.ncols, j = e % ft.ncols;
+        arena[ft.off + (size_t)i * ft.ld + j] = kernel_entry_c(k, T, ft.row0 + i, S, ft.col0 + j);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T entry_s(const KernelDesc &k, const DevPoints &T_, int i, const DevPoints &S, int j);
+template <>
+__device__ __forceinline__ double entry_s<double>(const KernelDesc &k, const DevPoints &T_, int i, const DevPoints &S,
+                                                  int j) {
+    return kernel_entry(k, T_, i, S, j);
+}
+template <>
+__device__ __forceinline__ cdbl entry_s<cdbl>(const KernelDesc &k, const DevPoints &T_, int i, const DevPoints &S,
+                                              int j) {
+    return kernel_entry_c(k, T_, i, S, j);
+}
+
 // ---------------------------------------------------------------------------
 // Batched partial-pivot ACA, one workgroup per block
 // (include/htool/lrmat/partialACA.hpp:39-171; with `swap`, sympartialACA's
@@ -126,11 +232,15 @@ __global__ void __launch_bounds__(WG) k_dense_fill(KernelDesc k, DevPoints T, De
 // (length n1), side 2 the other one (length n2). Cross term q is kept as
 // ws1[q*n1 + a] (scaled column) and ws2[q*n2 + b] (residual row). FP64
 // throughout: pivots, residuals and the Frobenius estimate.
+// Scalar type E: double, or cdbl for the complex kernel families (magnitudes
+// |.| for the pivots, conjugated dot products in the Frobenius estimate,
+// frob += aux + 2 Re(frob_aux), partialACA.hpp:127-141).
+template <typename E>
 __global__ void __launch_bounds__(WG) k_aca(KernelDesc kd, DevPoints T, DevPoints S, const AcaBlock *blocks,
-                                            double eps, int reqrank, double *ws, unsigned char *vis_ws,
+                                            double eps, int reqrank, E *ws, unsigned char *vis_ws,
                                             const uint64_t *vis_off, int *ranks) {
-    __shared__ double s_coef[CAPMAX];
-    __shared__ double s_prod[CAPMAX];
+    __shared__ E s_coef[CAPMAX];
+    __shared__ E s_prod[CAPMAX];
     __shared__ double s_v[NWAVE];
     __shared__ int s_i[NWAVE];
     __shared__ double s_red[NWAVE];
@@ -139,8 +249,8 @@ __global__ void __launch_bounds__(WG) k_aca(KernelDesc kd, DevPoints T, DevPoint
     const int M = B.M, N = B.N;
     const bool swap = B.swap != 0;
     const int n1 = swap ? N : M, n2 = swap ? M : N;
-    double *U1          = ws + B.ws1;
-    double *V2          = ws + B.ws2;
+    E *U1               = ws + B.ws1;
+    E *V2               = ws + B.ws2;
     unsigned char *vis1 = vis_ws + vis_off[blockIdx.x];
     unsigned char *vis2 = vis1 + n1;
     const int tid       = threadIdx.x;
@@ -200,8 +310,8 @@ __global__ void __launch_bounds__(WG) k_aca(KernelDesc kd, DevPoints T, DevPoint
     int I1 = s_i[0] < n1 ? s_i[0] : 0;
     __syncthreads();
 
-    auto entry = [&](int a, int b) -> double {
-        return swap ? kernel_entry(kd, T, B.off_t + b, S, B.off_s + a) : kernel_entry(kd, T, B.off_t + a, S, B.off_s + b);
+    auto entry = [&](int a, int b) -> E {
+        return swap ? entry_s<E>(kd, T, B.off_t + b, S, B.off_s + a) : entry_s<E>(kd, T, B.off_t + a, S, B.off_s + b);
     };
 
     int I2 = 0, q = 0;
@@ -219,17 +329,17 @@ __global__ void __launch_bounds__(WG) k_aca(KernelDesc kd, DevPoints T, DevPoint
             break;
         }
         const int p = q - 1; // number of previous cross terms
-        double *u1  = V2 + (size_t)p * n2;
-        double *u2  = U1 + (size_t)p * n1;
+        E *u1       = V2 + (size_t)p * n2;
+        E *u2       = U1 + (size_t)p * n1;
 
         // Residual row: generator row, minus the previous crosses in order (partialACA.hpp:83-87).
         for (int j = tid; j < p; j += WG)
-            s_coef[j] = -U1[(size_t)j * n1 + I1];
+            s_coef[j] = neg(U1[(size_t)j * n1 + I1]);
         __syncthreads();
         for (int b = tid; b < n2; b += WG) {
-            double val = entry(I1, b);
+            E val = entry(I1, b);
             for (int j = 0; j < p; j++)
-                val = fma(s_coef[j], V2[(size_t)j * n2 + b], val);
+                val = axpy_s(s_coef[j], V2[(size_t)j * n2 + b], val);
             u1[b] = val;
         }
         __syncthreads();
@@ -238,18 +348,18 @@ __global__ void __launch_bounds__(WG) k_aca(KernelDesc kd, DevPoints T, DevPoint
             I2 = J;
         if (tid == 0)
             vis1[I1] = 1;
-        const double piv   = u1[I2];
-        const double gamma = 1. / piv;
-        if (fabs(piv) > 1e-15) {
+        const E piv   = u1[I2];
+        const E gamma = recip(piv);
+        if (mag(piv) > 1e-15) {
             // Residual column, scaled by 1/pivot (partialACA.hpp:108-113).
             for (int j = tid; j < p; j += WG)
-                s_coef[j] = -V2[(size_t)j * n2 + I2];
+                s_coef[j] = neg(V2[(size_t)j * n2 + I2]);
             __syncthreads();
             for (int a = tid; a < n1; a += WG) {
-                double val = entry(a, I2);
+                E val = entry(a, I2);
                 for (int j = 0; j < p; j++)
-                    val = fma(s_coef[j], U1[(size_t)j * n1 + a], val);
-                u2[a] = val * gamma;
+                    val = axpy_s(s_coef[j], U1[(size_t)j * n1 + a], val);
+                u2[a] = mul(val, gamma);
             }
             __syncthreads();
             const int I = block_argmax_last(u2, vis1, n1, s_v, s_i);
@@ -258,31 +368,32 @@ __global__ void __launch_bounds__(WG) k_aca(KernelDesc kd, DevPoints T, DevPoint
             if (tid == 0)
                 vis2[I2] = 1;
             if (reqrank < 0) {
-                // Frobenius estimate of the approximant (partialACA.hpp:127-141).
+                // Frobenius estimate of the approximant (partialACA.hpp:127-141);
+                // conj(u) u is real, so only real parts enter aux.
                 double d22 = 0, d11 = 0;
                 for (int a = tid; a < n1; a += WG)
-                    d22 += u2[a] * u2[a];
+                    d22 = real_part(cdot_acc(E{d22}, u2[a], u2[a]));
                 for (int b = tid; b < n2; b += WG)
-                    d11 += u1[b] * u1[b];
+                    d11 = real_part(cdot_acc(E{d11}, u1[b], u1[b]));
                 d22 = block_sum(d22, s_red);
                 d11 = block_sum(d11, s_red);
                 aux = fabs(d22) * fabs(d11);
                 for (int j = w; j < p; j += NWAVE) {
-                    double x1 = 0, x2 = 0;
+                    E x1 = zero_s<E>(), x2 = zero_s<E>();
                     for (int b = lane; b < n2; b += 64)
-                        x1 += u1[b] * V2[(size_t)j * n2 + b];
+                        x1 = cdot_acc(x1, V2[(size_t)j * n2 + b], u1[b]);
                     for (int a = lane; a < n1; a += 64)
-                        x2 += u2[a] * U1[(size_t)j * n1 + a];
-                    x1 = wave_sum(x1);
-                    x2 = wave_sum(x2);
+                        x2 = cdot_acc(x2, U1[(size_t)j * n1 + a], u2[a]);
+                    x1 = wave_sum_s(x1);
+                    x2 = wave_sum_s(x2);
                     if (lane == 0)
-                        s_prod[j] = x1 * x2;
+                        s_prod[j] = mul(x1, x2);
                 }
                 __syncthreads();
-                double frob_aux = 0;
+                E frob_aux = zero_s<E>();
                 for (int j = 0; j < p; j++)
-                    frob_aux += s_prod[j];
-                frob += aux + 2 * frob_aux;
+                    frob_aux = add(frob_aux, s_prod[j]);
+                frob += aux + 2 * real_part(frob_aux);
             }
             __syncthreads();
         } else {
@@ -299,8 +410,9 @@ __global__ void __launch_bounds__(WG) k_aca(KernelDesc kd, DevPoints T, DevPoint
 // ---------------------------------------------------------------------------
 // Copies compressed leaves (from the ACA workspace or the compressed store)
 // into another layout: V row by row, U into one tile per destination.
-__global__ void __launch_bounds__(WG) k_lr_pack(const LrPack *packs, const UTileDst *utiles, const double *ws,
-                                                double *arena, double *varena) {
+template <typename E>
+__global__ void __launch_bounds__(WG) k_lr_pack(const LrPack *packs, const UTileDst *utiles, const E *ws, E *arena,
+                                                E *varena) {
     const LrPack p  = packs[blockIdx.x];
     const size_t nv = (size_t)p.k * p.N;
     for (size_t e = threadIdx.x; e < nv; e += WG) {
@@ -1421,6 +1533,251 @@ __global__ void k_convert(const double *src, TS *dst, uint64_t n) {
         dst[e] = (TS)src[e];
 }
 
+// ---------------------------------------------------------------------------
+// Per-term scaling of low-rank leaves (real or complex), one workgroup per
+// leaf: unit U columns and V rows, the product of the norms kept as the term's
+// scale. Lets 16-bit storage hold every term near unit magnitude.
+__device__ __forceinline__ double sq_mag(double v) { return v * v; }
+__device__ __forceinline__ double sq_mag(cdbl v) { return v.re * v.re + v.im * v.im; }
+__device__ __forceinline__ double scale_s(double v, double f) { return v * f; }
+__device__ __forceinline__ cdbl scale_s(cdbl v, double f) { return cdbl{v.re * f, v.im * f}; }
+
+template <typename E>
+__global__ void __launch_bounds__(WG) k_term_scale(const ScaleLeaf *leaves, E *store, double *sig) {
+    __shared__ double s_red[NWAVE];
+    const ScaleLeaf L = leaves[blockIdx.x];
+    for (int j = 0; j < L.k; j++) {
+        E *u = store + L.src_u + (size_t)j * L.M;
+        E *v = store + L.src_v + (size_t)j * L.N;
+        double su = 0, sv = 0;
+        for (int a = threadIdx.x; a < L.M; a += WG)
+            su += sq_mag(u[a]);
+        for (int b = threadIdx.x; b < L.N; b += WG)
+            sv += sq_mag(v[b]);
+        const double nu = sqrt(block_sum(su, s_red)), nv = sqrt(block_sum(sv, s_red));
+        if (nu > 0 && nv > 0) {
+            const double fu = 1.0 / nu, fv = 1.0 / nv;
+            for (int a = threadIdx.x; a < L.M; a += WG)
+                u[a] = scale_s(u[a], fu);
+            for (int b = threadIdx.x; b < L.N; b += WG)
+                v[b] = scale_s(v[b], fv);
+        }
+        if (threadIdx.x == 0)
+            sig[L.sig_off + j] = nu * nv;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Complex products. Each stored entry is a (re, im) pair of the storage type
+// in the real product's layout (the same offsets, counted in complex entries);
+// z holds (re, im) pairs of R, RHS-interleaved; arithmetic in R (float, or
+// double in FP64 mode); lane-group sums, chunk partial sums and the output in
+// FP64.
+template <typename R>
+struct cpair {
+    R re, im;
+};
+
+template <typename R, int MU>
+__global__ void k_gather_xc(const double *x, const int *perm_s, int nc, int tpad, int mu_base, R *z) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= tpad * MU)
+        return;
+    const int j = e / MU, s = e % MU;
+    R re = 0, im = 0;
+    if (j < nc) {
+        const double *src = x + 2 * (perm_s[j] + (size_t)nc * (mu_base + s));
+        re                = (R)src[0];
+        im                = (R)src[1];
+    }
+    z[2 * (size_t)e]     = re;
+    z[2 * (size_t)e + 1] = im;
+}
+
+// W stored values of a row (W/2 complex entries) times z entries; adds into acc.
+template <typename TS, typename R, int MU>
+__device__ __forceinline__ void cmac_vec(const TS *rowp, const R *zc, cpair<R> (&acc)[MU]) {
+    constexpr int W = Vec<TS>::W;
+    TS a[W];
+    Vec<TS>::load(rowp, a);
+#pragma unroll
+    for (int u = 0; u < W / 2; u++) {
+        const R ar = cvt<R>(a[2 * u]), ai = cvt<R>(a[2 * u + 1]);
+        const R *zz = zc + (size_t)u * MU * 2;
+#pragma unroll
+        for (int s = 0; s < MU; s++) {
+            const R zr = zz[2 * s], zi = zz[2 * s + 1];
+            acc[s].re += ar * zr - ai * zi;
+            acc[s].im += ar * zi + ai * zr;
+        }
+    }
+}
+
+template <int MU, typename R>
+__device__ __forceinline__ void group_reduce(const cpair<R> (&acc)[MU], int lpr, cpair<double> (&v)[MU]) {
+#pragma unroll
+    for (int s = 0; s < MU; s++)
+        v[s] = cpair<double>{(double)acc[s].re, (double)acc[s].im};
+    for (int o = lpr >> 1; o > 0; o >>= 1) {
+#pragma unroll
+        for (int s = 0; s < MU; s++) {
+            v[s].re += __shfl_xor(v[s].re, o, 64);
+            v[s].im += __shfl_xor(v[s].im, o, 64);
+        }
+    }
+}
+
+template <typename TS, typename R, int MU>
+__device__ __forceinline__ void vx_chunk_c(const VTile &d, const TS *varena, R *z, double *part, const double *tscale,
+                                           int lane) {
+    constexpr int W = Vec<TS>::W; // stored values per 16-byte vector: W/2 complex entries
+    const int nvec  = d.ncols * 2 / W;
+    const int lpr   = lanes_per_row(nvec);
+    const int rpw   = 64 / lpr;
+    const int sl = lane & (lpr - 1), sub = lane / lpr;
+    const R *xv = z + (size_t)d.x_off * MU * 2;
+    for (int rbase = 0; rbase < d.k; rbase += rpw) {
+        const int r = rbase + sub;
+        cpair<R> acc[MU];
+#pragma unroll
+        for (int s = 0; s < MU; s++)
+            acc[s] = cpair<R>{0, 0};
+        if (r < d.k) {
+            const TS *row = varena + 2 * (d.v_off + (size_t)r * d.ld);
+            for (int q = sl; q < nvec; q += lpr)
+                cmac_vec<TS, R, MU>(row + (size_t)q * W, xv + (size_t)q * (W / 2) * MU * 2, acc);
+        }
+        cpair<double> v[MU];
+        group_reduce<MU>(acc, lpr, v);
+        if (sl == 0 && r < d.k) {
+            if (d.nparts > 1) {
+                double *o = part + 2 * (size_t)(d.part_off + r) * MU;
+#pragma unroll
+                for (int s = 0; s < MU; s++)
+                    o[2 * s] = v[s].re, o[2 * s + 1] = v[s].im;
+            } else {
+                const double sc = tscale ? tscale[d.t_index + r] : 1.0;
+                R *o            = z + 2 * (size_t)(d.t_index + r) * MU;
+#pragma unroll
+                for (int s = 0; s < MU; s++)
+                    o[2 * s] = (R)(v[s].re * sc), o[2 * s + 1] = (R)(v[s].im * sc);
+            }
+        }
+    }
+}
+
+template <typename TS0, typename TS1, typename R, int MU>
+__global__ void __launch_bounds__(WG) k_vx_c(const int *wv_off, int nwaves, const VTile *tiles, const TS0 *varena0,
+                                             const TS1 *varena1, R *z, double *part, const double *tscale) {
+    const int wv = blockIdx.x * NWAVE + (threadIdx.x >> 6);
+    if (wv >= nwaves)
+        return;
+    const int lane = threadIdx.x & 63;
+    for (int ti = wv_off[wv]; ti < wv_off[wv + 1]; ti++) {
+        const VTile d = tiles[ti];
+        if (d.cls == 0)
+            vx_chunk_c<TS0, R, MU>(d, varena0, z, part, tscale, lane);
+        else
+            vx_chunk_c<TS1, R, MU>(d, varena1, z, part, tscale, lane);
+    }
+}
+
+template <typename R, int MU>
+__global__ void __launch_bounds__(WG) k_vx_sum_c(const VTile *tiles, const int *first_chunk, const double *part, R *z,
+                                                 const double *tscale) {
+    const VTile d    = tiles[first_chunk[blockIdx.x]];
+    const int km     = d.k * MU;
+    const double *pb = part + 2 * (size_t)d.part_base * MU;
+    for (int e = threadIdx.x; e < km; e += WG) {
+        double re = 0, im = 0;
+        for (int c = 0; c < d.nparts; c++) {
+            re += pb[2 * ((size_t)c * km + e)];
+            im += pb[2 * ((size_t)c * km + e) + 1];
+        }
+        const double sc = tscale ? tscale[d.t_index + e / MU] : 1.0;
+        R *o            = z + 2 * ((size_t)d.t_index * MU + e);
+        o[0]            = (R)(re * sc);
+        o[1]            = (R)(im * sc);
+    }
+}
+
+constexpr int ZCAP_C = 1024; // complex vector entries staged in LDS per pass
+
+template <typename TS, typename R, int MU>
+__device__ __forceinline__ void rows_class_c(const TS *arena, const RowClass &rc, int rb, int R_, const R *z, R *s_z,
+                                             cpair<double> *s_acc) {
+    constexpr int W     = Vec<TS>::W;
+    constexpr int ZCOLS = ZCAP_C / MU;
+    const int c0 = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - c0;
+    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+    const TS *A  = arena + 2 * rc.rb_aoff[rb];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int cc = 0; cc < Cp; cc += ZCOLS) {
+        const int nz = min(ZCOLS, Cp - cc);
+        for (int e = threadIdx.x; e < nz * MU; e += WG) {
+            const int c = cc + e / MU;
+            R re = 0, im = 0;
+            if (c < C) {
+                const R *src = z + 2 * ((size_t)rc.colidx[c0 + c] * MU + e % MU);
+                re           = src[0];
+                im           = src[1];
+            }
+            s_z[2 * e]     = re;
+            s_z[2 * e + 1] = im;
+        }
+        __syncthreads();
+        const int nvec = nz * 2 / W;
+        const int lpr  = lanes_per_row(nvec);
+        const int rpw  = 64 / lpr;
+        const int sl = lane & (lpr - 1), sub = lane / lpr;
+        for (int rbase = w * rpw; rbase < R_; rbase += NWAVE * rpw) {
+            const int r = rbase + sub;
+            cpair<R> acc[MU];
+#pragma unroll
+            for (int s = 0; s < MU; s++)
+                acc[s] = cpair<R>{0, 0};
+            if (r < R_) {
+                const TS *row = A + 2 * ((size_t)r * Cp + cc);
+                for (int q = sl; q < nvec; q += lpr)
+                    cmac_vec<TS, R, MU>(row + (size_t)q * W, s_z + (size_t)q * (W / 2) * MU * 2, acc);
+            }
+            cpair<double> v[MU];
+            group_reduce<MU>(acc, lpr, v);
+            if (sl == 0 && r < R_)
+#pragma unroll
+                for (int s = 0; s < MU; s++) {
+                    s_acc[r * MU + s].re += v[s].re;
+                    s_acc[r * MU + s].im += v[s].im;
+                }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename TS0, typename TS1, typename R, int MU>
+__global__ void __launch_bounds__(WG) k_rows_c(const int *rb_order, const int *rb_row0, RowClass c0, const TS0 *arena0,
+                                               RowClass c1, const TS1 *arena1, const R *z, const int *perm_t, int nr,
+                                               int mu_base, double *y) {
+    __shared__ R s_z[2 * ZCAP_C];
+    __shared__ cpair<double> s_acc[ROWMAX * MU];
+    const int rb = rb_order[blockIdx.x];
+    const int r0 = rb_row0[rb], R_ = rb_row0[rb + 1] - r0;
+    for (int e = threadIdx.x; e < R_ * MU; e += WG)
+        s_acc[e] = cpair<double>{0, 0};
+    __syncthreads();
+    rows_class_c<TS0, R, MU>(arena0, c0, rb, R_, z, s_z, s_acc);
+    if (arena1)
+        rows_class_c<TS1, R, MU>(arena1, c1, rb, R_, z, s_z, s_acc);
+    __syncthreads();
+    for (int e = threadIdx.x; e < R_ * MU; e += WG) {
+        const int r = e / MU, s = e % MU;
+        double *o   = y + 2 * ((size_t)perm_t[r0 + r] + (size_t)nr * (mu_base + s));
+        o[0]        = s_acc[e].re;
+        o[1]        = s_acc[e].im;
+    }
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -1434,14 +1791,43 @@ void launch_aca(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBlock *b
                 int reqrank, double *ws, unsigned char *vis_ws, const uint64_t *vis_off, int *ranks,
                 hipStream_t st) {
     if (nblocks > 0)
-        hipLaunchKernelGGL(k_aca, dim3(nblocks), dim3(WG), 0, st, k, T, S, blocks, eps, reqrank, ws, vis_ws, vis_off,
-                           ranks);
+        hipLaunchKernelGGL(k_aca<double>, dim3(nblocks), dim3(WG), 0, st, k, T, S, blocks, eps, reqrank, ws, vis_ws,
+                           vis_off, ranks);
+}
+
+void launch_aca_c(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBlock *blocks, int nblocks, double eps,
+                  int reqrank, cdbl *ws, unsigned char *vis_ws, const uint64_t *vis_off, int *ranks, hipStream_t st) {
+    if (nblocks > 0)
+        hipLaunchKernelGGL(k_aca<cdbl>, dim3(nblocks), dim3(WG), 0, st, k, T, S, blocks, eps, reqrank, ws, vis_ws,
+                           vis_off, ranks);
 }
 
 void launch_lr_pack(const LrPack *packs, int npacks, const UTileDst *utiles, const double *ws, double *arena,
                     double *varena, hipStream_t st) {
     if (npacks > 0)
-        hipLaunchKernelGGL(k_lr_pack, dim3(npacks), dim3(WG), 0, st, packs, utiles, ws, arena, varena);
+        hipLaunchKernelGGL(k_lr_pack<double>, dim3(npacks), dim3(WG), 0, st, packs, utiles, ws, arena, varena);
+}
+
+void launch_lr_pack_c(const LrPack *packs, int npacks, const UTileDst *utiles, const cdbl *ws, cdbl *arena,
+                      cdbl *varena, hipStream_t st) {
+    if (npacks > 0)
+        hipLaunchKernelGGL(k_lr_pack<cdbl>, dim3(npacks), dim3(WG), 0, st, packs, utiles, ws, arena, varena);
+}
+
+void launch_dense_fill_c(const KernelDesc &k, DevPoints T, DevPoints S, const FillTile *tiles, int ntiles, cdbl *arena,
+                         hipStream_t st) {
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_dense_fill_c, dim3(ntiles), dim3(WG), 0, st, k, T, S, tiles, arena);
+}
+
+void launch_term_scale(const ScaleLeaf *leaves, int nleaves, double *store, double *sig, hipStream_t st) {
+    if (nleaves > 0)
+        hipLaunchKernelGGL(k_term_scale<double>, dim3(nleaves), dim3(WG), 0, st, leaves, store, sig);
+}
+
+void launch_term_scale_c(const ScaleLeaf *leaves, int nleaves, cdbl *store, double *sig, hipStream_t st) {
+    if (nleaves > 0)
+        hipLaunchKernelGGL(k_term_scale<cdbl>, dim3(nleaves), dim3(WG), 0, st, leaves, store, sig);
 }
 
 void launch_recompress(const RcLeaf *leaves, int nleaves, double *store, double tol, int *rank, int *ok,
@@ -1546,6 +1932,68 @@ void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, 
     HTG_MU_SWITCH(MU, HTG_ROWS)
 #undef HTG_ROWS
 }
+
+#define HTG_MU_SWITCH_C(MUV, CALL) \
+    switch (MUV) {                 \
+    case 1: CALL(1); break;        \
+    case 2: CALL(2); break;        \
+    case 4: CALL(4); break;        \
+    case 8: CALL(8); break;        \
+    default: CALL(16); break;      \
+    }
+
+template <typename R>
+void launch_gather_xc(const double *x, const int *perm_s, int nc, int tpad, int mu_base, int MU, R *z,
+                      hipStream_t st) {
+    const dim3 grid((tpad * MU + WG - 1) / WG);
+#define HTG_GATHER_C(M) hipLaunchKernelGGL((k_gather_xc<R, M>), grid, dim3(WG), 0, st, x, perm_s, nc, tpad, mu_base, z)
+    HTG_MU_SWITCH_C(MU, HTG_GATHER_C)
+#undef HTG_GATHER_C
+}
+
+template <typename TS0, typename TS1, typename R>
+void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const int *multi_first, int nmulti,
+                 const TS0 *varena0, const TS1 *varena1, R *z, double *part, const double *tscale, int MU,
+                 hipStream_t st) {
+    if (ngroups <= 0)
+        return;
+#define HTG_VXC(M)                                                                                                  \
+    do {                                                                                                            \
+        hipLaunchKernelGGL((k_vx_c<TS0, TS1, R, M>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, \
+                           ngroups, tiles, varena0, varena1, z, part, tscale);                                      \
+        if (nmulti > 0)                                                                                             \
+            hipLaunchKernelGGL((k_vx_sum_c<R, M>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first, part, z,      \
+                               tscale);                                                                             \
+    } while (0)
+    HTG_MU_SWITCH_C(MU, HTG_VXC)
+#undef HTG_VXC
+}
+
+template <typename TS0, typename TS1, typename R>
+void launch_rows_c(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, const TS0 *arena0, RowClass c1,
+                   const TS1 *arena1, const R *z, const int *perm_t, int nr, int mu_base, int MU, double *y,
+                   hipStream_t st) {
+    if (nrb <= 0)
+        return;
+#define HTG_ROWSC(M)                                                                                          \
+    hipLaunchKernelGGL((k_rows_c<TS0, TS1, R, M>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0, c1, \
+                       arena1, z, perm_t, nr, mu_base, y)
+    HTG_MU_SWITCH_C(MU, HTG_ROWSC)
+#undef HTG_ROWSC
+}
+
+template void launch_gather_xc<double>(const double *, const int *, int, int, int, int, double *, hipStream_t);
+template void launch_gather_xc<float>(const double *, const int *, int, int, int, int, float *, hipStream_t);
+#define HTG_INST_C(TS0, TS1, R)                                                                                       \
+    template void launch_vx_c<TS0, TS1, R>(const int *, int, const VTile *, const int *, int, const TS0 *,          \
+                                           const TS1 *, R *, double *, const double *, int, hipStream_t);           \
+    template void launch_rows_c<TS0, TS1, R>(const int *, const int *, int, RowClass, const TS0 *, RowClass,        \
+                                             const TS1 *, const R *, const int *, int, int, int, double *, hipStream_t);
+HTG_INST_C(double, double, double)
+HTG_INST_C(float, float, float)
+HTG_INST_C(float, __half, float)
+HTG_INST_C(float, __hip_bfloat16, float)
+#undef HTG_INST_C
 
 template void launch_gather_x<double>(const double *, const int *, int, int, int, int, double *, hipStream_t);
 template void launch_gather_x<float>(const double *, const int *, int, int, int, int, float *, hipStream_t);

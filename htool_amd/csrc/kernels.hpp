@@ -1,6 +1,8 @@
 // Device data structures and kernel launchers of the H-matrix hot path.
 #pragma once
 
+#include "../../include/htool_amd.h"
+
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -13,10 +15,18 @@ namespace htg {
 // (replaces VirtualGenerator::copy_submatrix for the built-in kernels,
 // reference: include/htool/types/virtual_generator.hpp:22).
 struct KernelDesc {
-    int kind;     // HTG_KERNEL_*
-    int same;     // target and source point sets are the same set
-    double delta; // regularisation
-    double diag;  // value of the singular Laplace kernel on its diagonal
+    int kind;          // HTG_KERNEL_*
+    int same;          // target and source point sets are the same set
+    double delta;      // regularisation
+    double diag;       // value of the singular kernels on their diagonal
+    double wavenumber; // Helmholtz
+};
+
+inline bool kernel_is_complex(int kind) { return kind == HTG_KERNEL_HELMHOLTZ || kind == HTG_KERNEL_COMPLEX_TEST; }
+
+// complex<double> in device memory and registers: re then im.
+struct cdbl {
+    double re, im;
 };
 
 // Cluster-ordered points (structure of arrays) and the cluster -> original map.
@@ -127,5 +137,38 @@ void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, 
 // FP64 master copy -> storage type (round to nearest).
 template <typename TS>
 void launch_convert(const double *src, TS *dst, uint64_t n, hipStream_t st);
+
+// Complex kernel families.
+void launch_dense_fill_c(const KernelDesc &k, DevPoints T, DevPoints S, const FillTile *tiles, int ntiles, cdbl *arena,
+                         hipStream_t st);
+void launch_aca_c(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBlock *blocks, int nblocks, double eps,
+                  int reqrank, cdbl *ws, unsigned char *vis_ws, const uint64_t *vis_off, int *ranks, hipStream_t st);
+void launch_lr_pack_c(const LrPack *packs, int npacks, const UTileDst *utiles, const cdbl *ws, cdbl *arena,
+                      cdbl *varena, hipStream_t st);
+
+// Per-term scaling of low-rank leaves in a store (real: E = double, complex:
+// E = cdbl): term j of leaf l becomes (U[:,j] / |U[:,j]|) sig_j (V[j,:] / |V[j,:]|),
+// sig_j = |U[:,j]| |V[j,:]| written at sig[sig_off[l] + j].
+struct ScaleLeaf {
+    uint64_t src_u, src_v; // U (M x k col-major) and V (k x N row-major) in the store
+    int M, N, k;
+    int sig_off;
+};
+void launch_term_scale(const ScaleLeaf *leaves, int nleaves, double *store, double *sig, hipStream_t st);
+void launch_term_scale_c(const ScaleLeaf *leaves, int nleaves, cdbl *store, double *sig, hipStream_t st);
+
+// Complex products: entries stored as (re, im) pairs of TS in the real
+// product's layout; R = float or double arithmetic; z holds R pairs.
+template <typename R>
+void launch_gather_xc(const double *x, const int *perm_s, int nc, int tpad, int mu_base, int MU, R *z,
+                      hipStream_t st);
+template <typename TS0, typename TS1, typename R>
+void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const int *multi_first, int nmulti,
+                 const TS0 *varena0, const TS1 *varena1, R *z, double *part, const double *tscale, int MU,
+                 hipStream_t st);
+template <typename TS0, typename TS1, typename R>
+void launch_rows_c(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, const TS0 *arena0, RowClass c1,
+                   const TS1 *arena1, const R *z, const int *perm_t, int nr, int mu_base, int MU, double *y,
+                   hipStream_t st);
 
 } // namespace htg

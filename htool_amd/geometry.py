@@ -17,6 +17,8 @@ KERNEL_LAPLACE = 0          # 1/(4 pi r), value `diag` on the diagonal of a squa
 KERNEL_INV_DELTA_R = 1      # 1/(delta + r)
 KERNEL_INV_DELTA_4PIR = 2   # 1/(delta + 4 pi r)   (GeneratorTestDoubleSymmetric)
 KERNEL_SMOOTH = 3           # 1/(4 pi sqrt(delta + r^2))   (examples/smallest_example.cpp)
+KERNEL_HELMHOLTZ = 4        # e^{ikr}/(4 pi r), complex; `diag` on the diagonal of a square same-point matrix
+KERNEL_COMPLEX_TEST = 5     # (1 + i)/(4 pi r), complex   (GeneratorTestComplex)
 
 
 def cylinder(n: int, seed: int = 0, radius: float = 1.0, height: float = 1.0) -> np.ndarray:

@@ -21,9 +21,11 @@ class Generator:
     kernel: int = _geo.KERNEL_LAPLACE
     delta: float = 0.0
     diag: float = 0.0
+    wavenumber: float = 0.0  # KERNEL_HELMHOLTZ
 
     def _c(self, same: bool) -> capi.htg_generator:
-        return capi.htg_generator(self.kernel, 1 if same else 0, float(self.delta), float(self.diag))
+        return capi.htg_generator(self.kernel, 1 if same else 0, float(self.delta), float(self.diag),
+                                  float(self.wavenumber))
 
 
 _COMPRESSORS = {"partialACA": capi.PARTIAL_ACA, "sympartialACA": capi.SYM_PARTIAL_ACA}
@@ -199,16 +201,22 @@ class HMatrix:
         return self._storage
 
     # -- products ---------------------------------------------------------------
+    @property
+    def is_complex(self) -> bool:
+        return bool(self.get_info().is_complex)
+
     def mvprod_global_to_global(self, x: np.ndarray, mu: int = 1) -> np.ndarray:
         """y = H x with x (nc, mu) and y (nr, mu) in original ordering
-        (include/htool/types/hmatrix.hpp:1073-1182). Host arrays, copied over PCIe."""
+        (include/htool/types/hmatrix.hpp:1073-1182). Host arrays, copied over PCIe;
+        complex128 for the complex kernel families."""
         info = self.get_info()
-        x = np.asarray(x, dtype=np.float64)
+        dt = np.complex128 if info.is_complex else np.float64
+        x = np.asarray(x, dtype=dt)
         vec = x.ndim == 1
         xm = x.reshape(info.nc, -1)
         mu = xm.shape[1]
         xf = np.asfortranarray(xm)
-        y = np.zeros((info.nr, mu), order="F")
+        y = np.zeros((info.nr, mu), dtype=dt, order="F")
         capi.check(capi.lib().htg_hmatrix_mvprod(self._handle(), capi.ptr(xf), capi.ptr(y), mu))
         return y[:, 0].copy() if vec else np.ascontiguousarray(y)
 
@@ -274,8 +282,11 @@ class HMatrix:
         ds = C.c_int64()
         capi.check(capi.lib().htg_hmatrix_nleaves(self._handle(), C.byref(n), C.byref(ds)))
         leaves = np.zeros((n.value, 5), np.int32)
-        data = np.zeros(ds.value) if with_data else None
+        cplx = self.is_complex
+        data = np.zeros(ds.value * (2 if cplx else 1)) if with_data else None
         capi.check(capi.lib().htg_hmatrix_get_leaves(self._handle(), capi.ptr(leaves), capi.ptr(data)))
+        if with_data and cplx:
+            data = data.view(np.complex128)
         return (leaves, data) if with_data else leaves
 
     def algorithmic_bytes(self, mu: int = 1, storage: str | None = None) -> int:
@@ -283,12 +294,13 @@ class HMatrix:
         width), x read and y written once, the two permutations."""
         i = self.get_info()
         s0, s1 = _BYTES[storage or self._storage]
-        stored = s0 * (i.dense_entries + i.lowrank_entries - i.lowprec_entries) + s1 * i.lowprec_entries
-        return int(stored + mu * (i.nc + i.nr) * 8 + 4 * (i.nr + i.nc))
+        e = 2 if i.is_complex else 1  # a complex entry is two values
+        stored = e * (s0 * (i.dense_entries + i.lowrank_entries - i.lowprec_entries) + s1 * i.lowprec_entries)
+        return int(stored + e * mu * (i.nc + i.nr) * 8 + 4 * (i.nr + i.nc))
 
     def flops(self, mu: int = 1) -> int:
         i = self.get_info()
-        return int(2 * mu * (i.dense_entries + i.lowrank_entries))
+        return int((8 if i.is_complex else 2) * mu * (i.dense_entries + i.lowrank_entries))
 
 
 class ProductGraph:

@@ -4,7 +4,9 @@
  * crosses it. Every entry point returns an htg_status (HTG_OK == 0) and leaves
  * a message for htg_last_error() on failure.
  *
- * Layout conventions follow the reference (Htool 0.8.1):
+ * Layout conventions follow the reference (Htool 0.8.1); for the complex kernel
+ * families every vector and leaf entry is a complex<double> (two doubles, re
+ * then im) and the sizes below count complex entries:
  *  - points: interleaved x[3*i + d], like HMatrix::build(A, xt, xs)
  *    (include/htool/types/hmatrix.hpp:285);
  *  - products: x is nc x mu and y is nr x mu, column-major, original (not
@@ -42,7 +44,12 @@ typedef enum htg_kernel {
     HTG_KERNEL_INV_DELTA_R    = 1, /* 1/(delta + r) */
     HTG_KERNEL_INV_DELTA_4PIR = 2, /* 1/(delta + 4 pi r): GeneratorTestDoubleSymmetric,
                                       include/htool/testing/generator_test.hpp:135-145 */
-    HTG_KERNEL_SMOOTH         = 3  /* 1/(4 pi sqrt(delta + r^2)): examples/smallest_example.cpp:16-27 */
+    HTG_KERNEL_SMOOTH         = 3, /* 1/(4 pi sqrt(delta + r^2)): examples/smallest_example.cpp:16-27 */
+    /* Complex families: the H-matrix, x and y are complex<double> (interleaved re, im). */
+    HTG_KERNEL_HELMHOLTZ      = 4, /* e^{i k r}/(4 pi r), k = `wavenumber`; `diag` on the diagonal when the
+                                      point sets coincide */
+    HTG_KERNEL_COMPLEX_TEST   = 5  /* (1 + i)/(4 pi r): GeneratorTestComplex,
+                                      include/htool/testing/generator_test.hpp:110-120 */
 } htg_kernel;
 
 typedef enum htg_compressor {
@@ -67,6 +74,7 @@ typedef struct htg_generator {
     int same_points; /* 1: source points are the target points (square matrix) */
     double delta;
     double diag;
+    double wavenumber; /* HTG_KERNEL_HELMHOLTZ */
 } htg_generator;
 
 /* Parameters of HMatrix<T>(t, s, epsilon, eta, 'N', 'N', reqrank) plus the
@@ -111,6 +119,7 @@ typedef struct htg_info {
     int64_t aca_ws_entries;   /* sum over first runs of (m + n) x workspace terms */
     int64_t aca_need_entries; /* sum over runs of (m + n) x terms actually needed */
     double aca_seconds;       /* batched ACA (kernels and host scheduling) */
+    int is_complex;           /* entries, x and y are complex<double> */
 } htg_info;
 
 /* Work decomposition of the product (no counterpart in the reference): rows

@@ -33,6 +33,7 @@ struct Generator {
     htg_kernel kernel = HTG_KERNEL_LAPLACE;
     double delta      = 0.0;
     double diag       = 0.0;
+    double wavenumber = 0.0; // HTG_KERNEL_HELMHOLTZ
 };
 
 class GpuHMatrix {
@@ -84,7 +85,7 @@ class GpuHMatrix {
 
     // HMatrix::build(A, xt, xs), hmatrix.hpp:285-341 (points interleaved xyz).
     void build(const Generator &g, int nt, const double *xt, int ns, const double *xs) {
-        htg_generator gen{g.kernel, xs == nullptr ? 1 : 0, g.delta, g.diag};
+        htg_generator gen{g.kernel, xs == nullptr ? 1 : 0, g.delta, g.diag, g.wavenumber};
         reset();
         check(htg_hmatrix_build(&prm_, &gen, nt, xt, ns, xs, &h_));
     }

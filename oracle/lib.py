@@ -36,6 +36,9 @@ def lib():
             "orc_leaf_mvprod": (None, [_I, _I, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I]),
             "orc_dense_rows": (None, [_I, _D, _D, _I, _VP, _VP, _I, _I, _VP, _VP, _I, _VP]),
             "orc_mlp_loss_grad": (None, [_VP, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
+            "orc_hmat_is_complex": (_I, [_VP]),
+            "orc_leaf_mvprod_c": (None, [_I, _I, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I]),
+            "orc_dense_rows_c": (None, [_I, _D, _D, _I, _VP, _VP, _I, _I, _VP, _VP, _I, _VP]),
             "orc_mlp_predict": (None, [_VP, _VP, _VP, _VP, _I, _VP]),
         }
         for name, (res, args) in sig.items():
@@ -112,7 +115,11 @@ class OracleHMatrix:
 
 
 def hmat_build(xt, xs=None, *, kernel=0, delta=0.0, diag=0.0, minclustersize=10, eta=10.0, eps=1e-6, sym=False,
-               reqrank=-1, maxblocksize=1e6) -> OracleHMatrix:
+               reqrank=-1, maxblocksize=1e6, wavenumber=0.0) -> OracleHMatrix:
+    """Complex kernel families (kernel 4, 5): data is complex128 and mvprod takes
+    complex vectors; the Helmholtz wavenumber travels in `delta`."""
+    if kernel == 4 and wavenumber:
+        delta = wavenumber
     L = lib()
     xt = np.ascontiguousarray(xt, np.float64)
     xs_ = None if xs is None else np.ascontiguousarray(xs, np.float64)
@@ -124,8 +131,11 @@ def hmat_build(xt, xs=None, *, kernel=0, delta=0.0, diag=0.0, minclustersize=10,
         n = L.orc_hmat_nleaves(h)
         leaves = np.zeros((n, 5), np.int32)
         L.orc_hmat_leaves(h, _p(leaves))
-        data = np.zeros(L.orc_hmat_data_size(h))
+        cplx = L.orc_hmat_is_complex(h) != 0
+        data = np.zeros(L.orc_hmat_data_size(h) * (2 if cplx else 1))
         L.orc_hmat_data(h, _p(data))
+        if cplx:
+            data = data.view(np.complex128)
         pt = np.zeros(nt, np.int32)
         ps = np.zeros(ns, np.int32)
         L.orc_hmat_perms(h, _p(pt), _p(ps))
@@ -136,17 +146,35 @@ def hmat_build(xt, xs=None, *, kernel=0, delta=0.0, diag=0.0, minclustersize=10,
 
 
 def leaf_mvprod(nr, nc, perm_t, perm_s, leaves, data, x):
-    x = np.asarray(x, np.float64)
+    cplx = np.iscomplexobj(data)
+    dt = np.complex128 if cplx else np.float64
+    x = np.asarray(x, dt)
     vec = x.ndim == 1
     xm = np.asfortranarray(x.reshape(nc, -1))
     mu = xm.shape[1]
-    y = np.zeros((nr, mu), order="F")
+    y = np.zeros((nr, mu), dtype=dt, order="F")
     leaves = np.ascontiguousarray(leaves, np.int32)
-    data = np.ascontiguousarray(data, np.float64)
+    data = np.ascontiguousarray(data, dt)
     perm_t = np.ascontiguousarray(perm_t, np.int32)  # kept alive across the call
     perm_s = np.ascontiguousarray(perm_s, np.int32)
-    lib().orc_leaf_mvprod(nr, nc, _p(perm_t), _p(perm_s), leaves.shape[0], _p(leaves), _p(data), _p(xm), _p(y), mu)
+    fn = lib().orc_leaf_mvprod_c if cplx else lib().orc_leaf_mvprod
+    fn(nr, nc, _p(perm_t), _p(perm_s), leaves.shape[0], _p(leaves), _p(data), _p(xm), _p(y), mu)
     return y[:, 0].copy() if vec else np.ascontiguousarray(y)
+
+
+def dense_rows_c(xt, xs, rows, x, *, kernel=4, wavenumber=0.0, diag=0.0, same=False):
+    """Brute-force complex dense rows (kernel 4 or 5)."""
+    xt = np.ascontiguousarray(xt, np.float64)
+    xs = xt if same else np.ascontiguousarray(xs, np.float64)
+    rows = np.ascontiguousarray(rows, np.int32)
+    x = np.asarray(x, np.complex128)
+    vec = x.ndim == 1
+    xm = np.asfortranarray(x.reshape(xs.shape[0], -1))
+    mu = xm.shape[1]
+    out = np.zeros((len(rows), mu), dtype=np.complex128, order="F")
+    lib().orc_dense_rows_c(kernel, wavenumber, diag, int(same), _p(xt), _p(xs), xs.shape[0], len(rows), _p(rows),
+                           _p(xm), mu, _p(out))
+    return out[:, 0].copy() if vec else np.ascontiguousarray(out)
 
 
 def dense_rows(xt, xs, rows, x, *, kernel=0, delta=0.0, diag=0.0, same=False):

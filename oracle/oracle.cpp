@@ -15,6 +15,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
+#include <type_traits>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -51,7 +53,67 @@ struct Kernel {
             return 1. / (4 * M_PI * std::sqrt(delta + r2));
         }
     }
+    // Complex families: 4 = e^{ikr}/(4 pi r) (k = `delta`; `diag` on the
+    // diagonal of a square matrix), 5 = (1 + i)/(4 pi r)
+    // (GeneratorTestComplex, H/testing/generator_test.hpp:110-120).
+    std::complex<double> c(int i, int j) const {
+        const double *a = xt + 3 * i, *b = xs + 3 * j;
+        double r2 = 0;
+        for (int d = 0; d < 3; d++) {
+            double u = a[d] - b[d];
+            r2       = r2 + u * u;
+        }
+        const double r = std::sqrt(r2), den = 4 * M_PI * r;
+        if (kind == 4) {
+            if (same && i == j)
+                return {diag, 0.0};
+            return {std::cos(delta * r) / den, std::sin(delta * r) / den};
+        }
+        return {1. / den, 1. / den};
+    }
+    template <typename T>
+    T value(int i, int j) const;
 };
+template <>
+double Kernel::value<double>(int i, int j) const { return (*this)(i, j); }
+template <>
+std::complex<double> Kernel::value<std::complex<double>>(int i, int j) const { return c(i, j); }
+
+using cd = std::complex<double>;
+
+// Scalar algebra of the ACA restatement, real and complex: the residual
+// update is an FMA (a pair per complex component), complex products and
+// conjugated dot products are written out, 1/z uses Smith's method (Smith,
+// CACM 5(8), 1962) and |z| is hypot.
+inline double axpy_s(double a, double x, double y) { return std::fma(a, x, y); }
+inline cd axpy_s(cd a, cd x, cd y) {
+    return {std::fma(a.real(), x.real(), std::fma(-a.imag(), x.imag(), y.real())),
+            std::fma(a.real(), x.imag(), std::fma(a.imag(), x.real(), y.imag()))};
+}
+inline double mag(double x) { return std::abs(x); }
+inline double mag(cd x) { return std::hypot(x.real(), x.imag()); }
+inline double recip(double x) { return 1. / x; }
+inline cd recip(cd x) {
+    const double a = x.real(), b = x.imag();
+    if (std::abs(a) >= std::abs(b)) {
+        const double r = b / a, d = a + b * r;
+        return {1. / d, -r / d};
+    }
+    const double r = a / b, d = a * r + b;
+    return {r / d, -1. / d};
+}
+inline double mul(double a, double b) { return a * b; }
+inline cd mul(cd a, cd b) {
+    return {a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real()};
+}
+// acc + conj(a) b, the accumulation of Blas<T>::dot (H/wrappers/wrapper_blas.hpp:111-117)
+inline double cdot_acc(double acc, double a, double b) { return acc + a * b; }
+inline cd cdot_acc(cd acc, cd a, cd b) {
+    return {acc.real() + (a.real() * b.real() + a.imag() * b.imag()),
+            acc.imag() + (a.real() * b.imag() - a.imag() * b.real())};
+}
+inline double re(double a) { return a; }
+inline double re(cd a) { return a.real(); }
 
 // ----------------------------------------------------------------------------
 // Cluster tree: PCA with geometric splitting (H/clustering/pca.hpp:15-128,
@@ -253,12 +315,13 @@ void build_cluster(int n, const double *x, int mcs, std::vector<Node> &nodes, st
 // partialACA / sympartialACA (H/lrmat/partialACA.hpp:39-171,
 // H/lrmat/sympartialACA.hpp:38-209). `swap` is sympartialACA's
 // t.offset < s.offset branch: the roles of rows and columns exchange.
+template <typename T>
 int aca(bool swap, double epsilon, int M, int N, const int *rows, const int *cols, const Kernel &A,
-        const double *ctr1, int reqrank, int cap, double *U, double *V) {
+        const double *ctr1, int reqrank, int cap, T *U, T *V) {
     const int n1 = swap ? N : M, n2 = swap ? M : N;
     const int *i1 = swap ? cols : rows;
     const double *x1 = swap ? A.xs : A.xt;
-    auto g = [&](int a, int b) { return swap ? A(rows[b], cols[a]) : A(rows[a], cols[b]); };
+    auto g = [&](int a, int b) { return swap ? A.value<T>(rows[b], cols[a]) : A.value<T>(rows[a], cols[b]); };
 
     double dist = 1e30;
     int I1      = 0;
@@ -275,10 +338,10 @@ int aca(bool swap, double epsilon, int M, int N, const int *rows, const int *col
         }
     }
     int I2 = 0, q = 0;
-    std::vector<std::vector<double>> uu, vv;
+    std::vector<std::vector<T>> uu, vv;
     std::vector<bool> visited_1(n1, false), visited_2(n2, false);
     double frob = 0, aux = 0;
-    std::vector<double> u1(n2), u2(n1);
+    std::vector<T> u1(n2), u2(n1);
     while (((reqrank > 0) && (q < std::min(reqrank, std::min(M, N)))) || ((reqrank < 0) && (q == 0 || std::sqrt(aux / frob) > epsilon))) {
         q += 1;
         if ((long long)q * (M + N) > (long long)M * N) { // 64-bit: the int product overflows past 46341^2
@@ -288,37 +351,37 @@ int aca(bool swap, double epsilon, int M, int N, const int *rows, const int *col
         for (int b = 0; b < n2; b++)
             u1[b] = g(I1, b);
         for (size_t j = 0; j < uu.size(); j++) {
-            double coef = -uu[j][I1];
+            T coef = -uu[j][I1];
             for (int b = 0; b < n2; b++)
-                u1[b] = std::fma(coef, vv[j][b], u1[b]);
+                u1[b] = axpy_s(coef, vv[j][b], u1[b]);
         }
         double pivot = 0., tmp = 0;
         for (int k = 0; k < n2; k++) {
             if (visited_2[k])
                 continue;
-            tmp = std::abs(u1[k]);
+            tmp = mag(u1[k]);
             if (tmp < pivot)
                 continue;
             pivot = tmp;
             I2    = k;
         }
         visited_1[I1] = true;
-        double gamma  = 1. / u1[I2];
-        if (std::abs(u1[I2]) > 1e-15) {
+        T gamma       = recip(u1[I2]);
+        if (mag(u1[I2]) > 1e-15) {
             for (int a = 0; a < n1; a++)
                 u2[a] = g(a, I2);
             for (size_t k = 0; k < uu.size(); k++) {
-                double coef = -vv[k][I2];
+                T coef = -vv[k][I2];
                 for (int a = 0; a < n1; a++)
-                    u2[a] = std::fma(coef, uu[k][a], u2[a]);
+                    u2[a] = axpy_s(coef, uu[k][a], u2[a]);
             }
             for (int a = 0; a < n1; a++)
-                u2[a] *= gamma;
+                u2[a] = mul(u2[a], gamma);
             pivot = 0., tmp = 0;
             for (int k = 0; k < n1; k++) {
                 if (visited_1[k])
                     continue;
-                tmp = std::abs(u2[k]);
+                tmp = mag(u2[k]);
                 if (tmp < pivot)
                     continue;
                 pivot = tmp;
@@ -326,17 +389,18 @@ int aca(bool swap, double epsilon, int M, int N, const int *rows, const int *col
             }
             visited_2[I2] = true;
             if (reqrank < 0) {
-                auto dot = [](const std::vector<double> &a, const std::vector<double> &b) {
-                    double s = 0;
+                // dot(a, b) = sum conj(a) b; conj(u) u is real
+                auto dot = [](const std::vector<T> &a, const std::vector<T> &b) {
+                    T s = T(0);
                     for (size_t i = 0; i < a.size(); i++)
-                        s += a[i] * b[i];
+                        s = cdot_acc(s, a[i], b[i]);
                     return s;
                 };
-                double frob_aux = 0.;
-                aux             = std::abs(dot(u2, u2)) * std::abs(dot(u1, u1));
+                T frob_aux = T(0);
+                aux        = std::abs(re(dot(u2, u2))) * std::abs(re(dot(u1, u1)));
                 for (size_t j = 0; j < uu.size(); j++)
-                    frob_aux += dot(u1, vv[j]) * dot(u2, uu[j]);
-                frob += aux + 2 * frob_aux;
+                    frob_aux += mul(dot(vv[j], u1), dot(uu[j], u2));
+                frob += aux + 2 * re(frob_aux);
             }
             uu.push_back(u2);
             vv.push_back(u1);
@@ -351,8 +415,8 @@ int aca(bool swap, double epsilon, int M, int N, const int *rows, const int *col
         return -2;
     if (q > 0) {
         for (int k = 0; k < q; k++) {
-            const std::vector<double> &ucol = swap ? vv[k] : uu[k]; // length M
-            const std::vector<double> &vrow = swap ? uu[k] : vv[k]; // length N
+            const std::vector<T> &ucol = swap ? vv[k] : uu[k]; // length M
+            const std::vector<T> &vrow = swap ? uu[k] : vv[k]; // length N
             std::copy(ucol.begin(), ucol.end(), U + (size_t)k * M);
             for (int j = 0; j < N; j++)
                 V[(size_t)q * j + k] = vrow[j];
@@ -364,13 +428,15 @@ int aca(bool swap, double epsilon, int M, int N, const int *rows, const int *col
 // ----------------------------------------------------------------------------
 // Block tree (H/blocks/blocks.hpp:37-139, 237-278) and leaf assembly
 // (H/types/hmatrix.hpp:422-607, 700-735).
+template <typename T>
 struct Leaf {
     int oi, m, oj, n, k; // k = -1 dense
-    std::vector<double> a, U, V;
+    std::vector<T> a, U, V;
 };
 
+template <typename T>
 struct Builder {
-    const std::vector<Node> &T, &S;
+    const std::vector<Node> &Tn, &Sn;
     const std::vector<int> &perm_t, &perm_s;
     const Kernel &A;
     double eta, eps;
@@ -379,7 +445,7 @@ struct Builder {
 
     // RjasanowSteinbach (H/blocks/admissibility_conditions.hpp:16-19)
     bool admissible(int t, int s) const {
-        const Node &a = T[t], &b = S[s];
+        const Node &a = Tn[t], &b = Sn[s];
         double d[3];
         for (int p = 0; p < 3; p++)
             d[p] = a.ctr[p] - b.ctr[p];
@@ -389,7 +455,7 @@ struct Builder {
 
     // build_block_tree: returns true when the block was not pushed (blocks.hpp:37-139).
     bool block_tree(int t, int s, std::vector<std::pair<int, int>> &tasks) const {
-        const Node &a = T[t], &b = S[s];
+        const Node &a = Tn[t], &b = Sn[s];
         size_t bsize  = size_t(a.size) * size_t(b.size);
         if (admissible(t, s) && a.rank >= 0) {
             tasks.emplace_back(t, s);
@@ -417,35 +483,37 @@ struct Builder {
         return false;
     }
 
-    void dense(int t, int s, std::vector<Leaf> &out) const {
-        const Node &a = T[t], &b = S[s];
-        Leaf L{a.offset, a.size, b.offset, b.size, -1, {}, {}, {}};
+    void dense(int t, int s, std::vector<Leaf<T>> &out) const {
+        const Node &a = Tn[t], &b = Sn[s];
+        Leaf<T> L{a.offset, a.size, b.offset, b.size, -1, {}, {}, {}};
         L.a.resize(size_t(a.size) * b.size);
         for (int j = 0; j < b.size; j++)
             for (int i = 0; i < a.size; i++)
-                L.a[i + size_t(a.size) * j] = A(perm_t[a.offset + i], perm_s[b.offset + j]);
+                L.a[i + size_t(a.size) * j] = A.value<T>(perm_t[a.offset + i], perm_s[b.offset + j]);
         out.push_back(std::move(L));
     }
 
     // AddFarFieldMat + LowRankMatrix::build (hmatrix.hpp:719-735, lrmat.hpp:36-55)
-    bool lowrank(int t, int s, std::vector<Leaf> &out) const {
-        const Node &a = T[t], &b = S[s];
+    bool lowrank(int t, int s, std::vector<Leaf<T>> &out) const {
+        const Node &a = Tn[t], &b = Sn[s];
         const int M = a.size, N = b.size;
         const bool swap = sym_aca && (a.offset < b.offset);
         std::vector<int> rows(perm_t.begin() + a.offset, perm_t.begin() + a.offset + M);
         std::vector<int> cols(perm_s.begin() + b.offset, perm_s.begin() + b.offset + N);
         int cap = std::min(std::min(M, N) + 1, 64); // grown below when the rank needs more room
-        std::vector<double> U(size_t(M) * cap), V(size_t(N) * cap);
-        int k = aca(swap, eps, M, N, rows.data(), cols.data(), A, swap ? b.ctr : a.ctr, reqrank, cap, U.data(), V.data());
+        std::vector<T> U(size_t(M) * cap), V(size_t(N) * cap);
+        int k = aca<T>(swap, eps, M, N, rows.data(), cols.data(), A, swap ? b.ctr : a.ctr, reqrank, cap, U.data(),
+                       V.data());
         if (k == -2) {
             cap = std::min(M, N) + 1;
-            U.assign(size_t(M) * cap, 0.0);
-            V.assign(size_t(N) * cap, 0.0);
-            k = aca(swap, eps, M, N, rows.data(), cols.data(), A, swap ? b.ctr : a.ctr, reqrank, cap, U.data(), V.data());
+            U.assign(size_t(M) * cap, T(0));
+            V.assign(size_t(N) * cap, T(0));
+            k = aca<T>(swap, eps, M, N, rows.data(), cols.data(), A, swap ? b.ctr : a.ctr, reqrank, cap, U.data(),
+                       V.data());
         }
         if (k == -1)
             return false;
-        Leaf L{a.offset, M, b.offset, N, k, {}, {}, {}};
+        Leaf<T> L{a.offset, M, b.offset, N, k, {}, {}, {}};
         L.U.assign(U.begin(), U.begin() + size_t(M) * k);
         L.V.assign(V.begin(), V.begin() + size_t(N) * k);
         out.push_back(std::move(L));
@@ -453,13 +521,13 @@ struct Builder {
     }
 
     // ComputeAdmissibleBlock (hmatrix.hpp:503-607): true = not pushed.
-    bool admissible_block(int t, int s, bool adm, std::vector<Leaf> &out, int &fp) const {
+    bool admissible_block(int t, int s, bool adm, std::vector<Leaf<T>> &out, int &fp) const {
         if (adm) {
             if (lowrank(t, s, out))
                 return false;
             fp += 1;
         }
-        const Node &a = T[t], &b = S[s];
+        const Node &a = Tn[t], &b = Sn[s];
         size_t bsize  = size_t(a.size) * size_t(b.size);
         bool split_t;
         if (b.sons.empty()) {
@@ -490,9 +558,129 @@ struct Builder {
 
 struct orc_hmat {
     int nr = 0, nc = 0, false_positive = 0;
+    bool cplx = false;
     std::vector<int> perm_t, perm_s;
-    std::vector<Leaf> leaves;
+    std::vector<Leaf<double>> leaves;
+    std::vector<Leaf<cd>> cleaves;
 };
+
+namespace {
+
+// ComputeBlocks (hmatrix.hpp:422-501) for the local tasks, one result list per
+// task, merged in task order and sorted by (off_i, off_j).
+template <typename T>
+void compute_blocks(const Builder<T> &B, const std::vector<std::pair<int, int>> &local, std::vector<Leaf<T>> &leaves,
+                    int &false_positive) {
+    std::vector<std::vector<Leaf<T>>> per_task(local.size());
+    std::vector<int> fp(local.size(), 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long p = 0; p < (long)local.size(); p++) {
+        int t = local[p].first, s = local[p].second;
+        bool adm = B.admissible(t, s);
+        if (!adm) {
+            B.dense(t, s, per_task[p]);
+        } else if (B.admissible_block(t, s, true, per_task[p], fp[p])) {
+            B.dense(t, s, per_task[p]);
+        }
+    }
+    for (size_t p = 0; p < local.size(); p++) {
+        false_positive += fp[p];
+        for (auto &L : per_task[p])
+            leaves.push_back(std::move(L));
+    }
+    std::stable_sort(leaves.begin(), leaves.end(), [](const Leaf<T> &a, const Leaf<T> &b) {
+        return a.oi == b.oi ? a.oj < b.oj : a.oi < b.oi;
+    });
+}
+
+template <typename T>
+void put(double *&out, const T &v);
+template <>
+void put<double>(double *&out, const double &v) {
+    *out++ = v;
+}
+template <>
+void put<cd>(double *&out, const cd &v) {
+    *out++ = v.real();
+    *out++ = v.imag();
+}
+
+template <typename T>
+void leaves_data(const std::vector<Leaf<T>> &leaves, double *out) {
+    for (const Leaf<T> &L : leaves) {
+        if (L.k < 0) {
+            for (const T &v : L.a)
+                put(out, v);
+        } else {
+            for (const T &v : L.U)
+                put(out, v);
+            for (const T &v : L.V)
+                put(out, v);
+        }
+    }
+}
+
+template <typename T>
+T load(const double *p, size_t i);
+template <>
+double load<double>(const double *p, size_t i) {
+    return p[i];
+}
+template <>
+cd load<cd>(const double *p, size_t i) {
+    return {p[2 * i], p[2 * i + 1]};
+}
+
+// mvprod_global_to_global (hmatrix.hpp:1073-1182) with the leaf loop of
+// mymvprod_global_to_local (843-911) and the leaf products of
+// Matrix::add_mvprod_row_major (matrix.hpp:353-383) and
+// LowRankMatrix::add_mvprod_row_major (lrmat.hpp:89-100), one RHS column at a
+// time. T = cd: data, x and y hold (re, im) pairs.
+template <typename T>
+void leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
+                 const double *data, const double *x, double *y, int mu) {
+    std::vector<size_t> off(nleaves);
+    size_t p = 0;
+    for (int l = 0; l < nleaves; l++) {
+        off[l]       = p;
+        const int *L = leaves5 + 5 * l;
+        p += L[4] < 0 ? size_t(L[1]) * L[3] : size_t(L[4]) * (L[1] + L[3]);
+    }
+    std::vector<T> xin(nc), yout(nr), t;
+    for (int c = 0; c < mu; c++) {
+        for (int j = 0; j < nc; j++)
+            xin[j] = load<T>(x, perm_s[j] + size_t(nc) * c); // global_to_cluster (virtual_cluster.hpp:95-100)
+        std::fill(yout.begin(), yout.end(), T(0));
+        for (int l = 0; l < nleaves; l++) {
+            const int *L = leaves5 + 5 * l;
+            const int oi = L[0], m = L[1], oj = L[2], n = L[3], k = L[4];
+            const size_t d = off[l];
+            if (k < 0) {
+                for (int j = 0; j < n; j++) {
+                    T xj = xin[oj + j];
+                    for (int i = 0; i < m; i++)
+                        yout[oi + i] += load<T>(data, d + i + size_t(m) * j) * xj;
+                }
+            } else {
+                const size_t U = d, V = d + size_t(m) * k;
+                t.assign(k, T(0));
+                for (int j = 0; j < n; j++)
+                    for (int r = 0; r < k; r++)
+                        t[r] += load<T>(data, V + r + size_t(k) * j) * xin[oj + j];
+                for (int r = 0; r < k; r++)
+                    for (int i = 0; i < m; i++)
+                        yout[oi + i] += load<T>(data, U + i + size_t(m) * r) * t[r];
+            }
+        }
+        double *yy = y;
+        for (int i = 0; i < nr; i++) { // cluster_to_global (virtual_cluster.hpp:89-94)
+            double *o = yy + (std::is_same<T, cd>::value ? 2 : 1) * (perm_t[i] + size_t(nr) * c);
+            put(o, yout[i]);
+        }
+    }
+}
+
+} // namespace
 
 extern "C" {
 
@@ -532,7 +720,7 @@ int orc_aca(int sym, double eps, int M, int N, const int *rows, const int *cols,
             const double *ctr_s, int off_s, int reqrank, int cap, double *U, double *V) {
     Kernel A{kernel, delta, diag, same_points != 0, xt, same_points ? xt : xs};
     bool swap = sym && off_t < off_s;
-    return aca(swap, eps, M, N, rows, cols, A, swap ? ctr_s : ctr_t, reqrank, cap, U, V);
+    return aca<double>(swap, eps, M, N, rows, cols, A, swap ? ctr_s : ctr_t, reqrank, cap, U, V);
 }
 
 orc_hmat *orc_hmat_build(int nt, const double *xt, int ns, const double *xs, int kernel, double delta, double diag,
@@ -550,7 +738,7 @@ orc_hmat *orc_hmat_build(int nt, const double *xt, int ns, const double *xs, int
         build_cluster(ns, xs, mcs, S, h->perm_s);
     }
     Kernel A{kernel, delta, diag, same, xt, same ? xt : xs};
-    Builder B{T, S, h->perm_t, h->perm_s, A, eta, eps, (size_t)maxblocksize, sym_aca, reqrank};
+    Builder<double> B{T, S, h->perm_t, h->perm_s, A, eta, eps, (size_t)maxblocksize, sym_aca, reqrank};
 
     // Block::build (blocks.hpp:237-278): root admissibility, tree, root task,
     // local tasks of rank 0, sorted by (target offset, source offset).
@@ -567,57 +755,52 @@ orc_hmat *orc_hmat_build(int nt, const double *xt, int ns, const double *xs, int
             return S[a.second].offset < S[b.second].offset;
         return T[a.first].offset < T[b.first].offset;
     });
-
-    // ComputeBlocks (hmatrix.hpp:422-501): one result list per task, merged in task order.
-    std::vector<std::vector<Leaf>> per_task(local.size());
-    std::vector<int> fp(local.size(), 0);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (long p = 0; p < (long)local.size(); p++) {
-        int t = local[p].first, s = local[p].second;
-        bool adm = B.admissible(t, s);
-        if (!adm) {
-            B.dense(t, s, per_task[p]);
-        } else if (B.admissible_block(t, s, true, per_task[p], fp[p])) {
-            B.dense(t, s, per_task[p]);
-        }
+    h->cplx = kernel >= 4;
+    if (h->cplx) {
+        Builder<cd> BC{T, S, h->perm_t, h->perm_s, A, eta, eps, (size_t)maxblocksize, sym_aca, reqrank};
+        compute_blocks(BC, local, h->cleaves, h->false_positive);
+    } else {
+        compute_blocks(B, local, h->leaves, h->false_positive);
     }
-    for (size_t p = 0; p < local.size(); p++) {
-        h->false_positive += fp[p];
-        for (auto &L : per_task[p])
-            h->leaves.push_back(std::move(L));
-    }
-    std::stable_sort(h->leaves.begin(), h->leaves.end(), [](const Leaf &a, const Leaf &b) {
-        return a.oi == b.oi ? a.oj < b.oj : a.oi < b.oi;
-    });
     return h;
 }
 
-int orc_hmat_nleaves(const orc_hmat *h) { return (int)h->leaves.size(); }
+int orc_hmat_nleaves(const orc_hmat *h) { return (int)(h->cplx ? h->cleaves.size() : h->leaves.size()); }
+
+int orc_hmat_is_complex(const orc_hmat *h) { return h->cplx ? 1 : 0; }
 
 void orc_hmat_leaves(const orc_hmat *h, int *out5) {
-    for (size_t l = 0; l < h->leaves.size(); l++) {
-        const Leaf &L = h->leaves[l];
-        int v[5]      = {L.oi, L.m, L.oj, L.n, L.k};
-        std::copy(v, v + 5, out5 + 5 * l);
-    }
+    auto emit = [&](const auto &leaves) {
+        for (size_t l = 0; l < leaves.size(); l++) {
+            const auto &L = leaves[l];
+            int v[5]      = {L.oi, L.m, L.oj, L.n, L.k};
+            std::copy(v, v + 5, out5 + 5 * l);
+        }
+    };
+    if (h->cplx)
+        emit(h->cleaves);
+    else
+        emit(h->leaves);
 }
 
 long long orc_hmat_data_size(const orc_hmat *h) {
     long long s = 0;
-    for (const Leaf &L : h->leaves)
-        s += L.k < 0 ? (long long)L.m * L.n : (long long)L.k * (L.m + L.n);
+    auto count  = [&](const auto &leaves) {
+        for (const auto &L : leaves)
+            s += L.k < 0 ? (long long)L.m * L.n : (long long)L.k * (L.m + L.n);
+    };
+    if (h->cplx)
+        count(h->cleaves);
+    else
+        count(h->leaves);
     return s;
 }
 
 void orc_hmat_data(const orc_hmat *h, double *out) {
-    for (const Leaf &L : h->leaves) {
-        if (L.k < 0) {
-            out = std::copy(L.a.begin(), L.a.end(), out);
-        } else {
-            out = std::copy(L.U.begin(), L.U.end(), out);
-            out = std::copy(L.V.begin(), L.V.end(), out);
-        }
-    }
+    if (h->cplx)
+        leaves_data(h->cleaves, out);
+    else
+        leaves_data(h->leaves, out);
 }
 
 void orc_hmat_perms(const orc_hmat *h, int *perm_t, int *perm_s) {
@@ -629,57 +812,27 @@ int orc_hmat_false_positives(const orc_hmat *h) { return h->false_positive; }
 
 void orc_hmat_free(orc_hmat *h) { delete h; }
 
-// mvprod_global_to_global (hmatrix.hpp:1073-1182) with the leaf loop of
-// mymvprod_global_to_local (843-911) and the leaf products of
-// Matrix::add_mvprod_row_major (matrix.hpp:353-383) and
-// LowRankMatrix::add_mvprod_row_major (lrmat.hpp:89-100), one RHS column at a time.
 void orc_leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
                      const double *data, const double *x, double *y, int mu) {
-    std::vector<const double *> ptr(nleaves);
-    const double *p = data;
-    for (int l = 0; l < nleaves; l++) {
-        ptr[l]     = p;
-        const int *L = leaves5 + 5 * l;
-        p += L[4] < 0 ? size_t(L[1]) * L[3] : size_t(L[4]) * (L[1] + L[3]);
-    }
-    std::vector<double> xin(nc), yout(nr), t;
-    for (int c = 0; c < mu; c++) {
-        for (int j = 0; j < nc; j++)
-            xin[j] = x[perm_s[j] + size_t(nc) * c]; // global_to_cluster (virtual_cluster.hpp:95-100)
-        std::fill(yout.begin(), yout.end(), 0.0);
-        for (int l = 0; l < nleaves; l++) {
-            const int *L = leaves5 + 5 * l;
-            const int oi = L[0], m = L[1], oj = L[2], n = L[3], k = L[4];
-            const double *d = ptr[l];
-            if (k < 0) {
-                for (int j = 0; j < n; j++) {
-                    double xj = xin[oj + j];
-                    for (int i = 0; i < m; i++)
-                        yout[oi + i] += d[i + size_t(m) * j] * xj;
-                }
-            } else {
-                const double *U = d, *V = d + size_t(m) * k;
-                t.assign(k, 0.0);
-                for (int j = 0; j < n; j++)
-                    for (int r = 0; r < k; r++)
-                        t[r] += V[r + size_t(k) * j] * xin[oj + j];
-                for (int r = 0; r < k; r++)
-                    for (int i = 0; i < m; i++)
-                        yout[oi + i] += U[i + size_t(m) * r] * t[r];
-            }
-        }
-        for (int i = 0; i < nr; i++)
-            y[perm_t[i] + size_t(nr) * c] = yout[i]; // cluster_to_global (virtual_cluster.hpp:89-94)
-    }
+    leaf_mvprod<double>(nr, nc, perm_t, perm_s, nleaves, leaves5, data, x, y, mu);
+}
+
+void orc_leaf_mvprod_c(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
+                       const double *data, const double *x, double *y, int mu) {
+    leaf_mvprod<cd>(nr, nc, perm_t, perm_s, nleaves, leaves5, data, x, y, mu);
 }
 
 void orc_hmat_mvprod(const orc_hmat *h, const double *x, double *y, int mu) {
-    std::vector<int> leaves(5 * h->leaves.size());
+    const int nl = orc_hmat_nleaves(h);
+    std::vector<int> leaves(5 * nl);
     orc_hmat_leaves(h, leaves.data());
-    std::vector<double> data(orc_hmat_data_size(h));
+    std::vector<double> data(orc_hmat_data_size(h) * (h->cplx ? 2 : 1));
     orc_hmat_data(h, data.data());
-    orc_leaf_mvprod(h->nr, h->nc, h->perm_t.data(), h->perm_s.data(), (int)h->leaves.size(), leaves.data(),
-                    data.data(), x, y, mu);
+    if (h->cplx)
+        leaf_mvprod<cd>(h->nr, h->nc, h->perm_t.data(), h->perm_s.data(), nl, leaves.data(), data.data(), x, y, mu);
+    else
+        leaf_mvprod<double>(h->nr, h->nc, h->perm_t.data(), h->perm_s.data(), nl, leaves.data(), data.data(), x, y,
+                            mu);
 }
 
 void orc_dense_rows(int kernel, double delta, double diag, int same_points, const double *xt, const double *xs,
@@ -692,6 +845,22 @@ void orc_dense_rows(int kernel, double delta, double diag, int same_points, cons
             for (int j = 0; j < ns; j++)
                 s += A(rows[r], j) * x[j + size_t(ns) * c];
             out[r + size_t(nrows) * c] = s;
+        }
+    }
+}
+
+// Complex kernels: x and out hold (re, im) pairs.
+void orc_dense_rows_c(int kernel, double wavenumber, double diag, int same_points, const double *xt, const double *xs,
+                      int ns, int nrows, const int *rows, const double *x, int mu, double *out) {
+    Kernel A{kernel, wavenumber, diag, same_points != 0, xt, same_points ? xt : xs};
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < nrows; r++) {
+        for (int c = 0; c < mu; c++) {
+            cd s = 0;
+            for (int j = 0; j < ns; j++)
+                s += A.c(rows[r], j) * cd(x[2 * (j + size_t(ns) * c)], x[2 * (j + size_t(ns) * c) + 1]);
+            out[2 * (r + size_t(nrows) * c)]     = s.real();
+            out[2 * (r + size_t(nrows) * c) + 1] = s.imag();
         }
     }
 }

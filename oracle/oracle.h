@@ -57,11 +57,24 @@ void orc_hmat_perms(const orc_hmat *h, int *perm_t, int *perm_s);
 int orc_hmat_false_positives(const orc_hmat *h);
 void orc_hmat_mvprod(const orc_hmat *h, const double *x, double *y, int mu);
 void orc_hmat_free(orc_hmat *h);
+/* Complex kernel families (kernel 4: e^{ikr}/(4 pi r) with k passed as `delta`;
+ * kernel 5: (1 + i)/(4 pi r)) build complex H-matrices: orc_hmat_data writes
+ * (re, im) pairs (2 x orc_hmat_data_size doubles) and orc_hmat_mvprod takes and
+ * returns complex vectors as (re, im) pairs. */
+int orc_hmat_is_complex(const orc_hmat *h);
 
 /* y = H x over an explicit leaf list with the data layout of orc_hmat_data.
  * x: nc x mu column-major, y: nr x mu column-major, both in original ordering. */
 void orc_leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
                      const double *data, const double *x, double *y, int mu);
+
+/* The same over complex leaf data and vectors, (re, im) pairs. */
+void orc_leaf_mvprod_c(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
+                       const double *data, const double *x, double *y, int mu);
+
+/* Brute-force dense rows, complex kernels, (re, im) pairs. */
+void orc_dense_rows_c(int kernel, double wavenumber, double diag, int same_points, const double *xt, const double *xs,
+                      int ns, int nrows, const int *rows, const double *x, int mu, double *out);
 
 /* Brute-force dense rows: out[r + nrows*c] = sum_j A(rows[r], j) x[j + ns*c]. */
 void orc_dense_rows(int kernel, double delta, double diag, int same_points, const double *xt, const double *xs,

@@ -20,12 +20,16 @@
 //   int32  mu, nreps, dump_data, reqrank
 //   double eta, eps, delta, diag, maxblocksize
 //   double xt[3 nt]; double xs[3 ns] (absent when same_points); double x[ns mu] (column-major)
+// Complex kernel families (kernel 4: e^{ikr}/(4 pi r) with k = `delta`, `diag`
+// on the diagonal; kernel 5: (1 + i)/(4 pi r)) run HMatrix<std::complex<double>>:
+// x, y and the leaf data are then complex<double> (re, im interleaved), mode 0 only.
 #include <htool/clustering/pca.hpp>
 #include <htool/lrmat/partialACA.hpp>
 #include <htool/lrmat/sympartialACA.hpp>
 #include <htool/types/hmatrix.hpp>
 
 #include <chrono>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -75,6 +79,48 @@ struct KernelGen final : public VirtualGenerator<double> {
     }
 };
 
+// Complex families, with the distance summed as above. The Helmholtz kernel
+// is the config's e^{ikr}/(4 pi r) (no generator of the reference computes it);
+// kernel 5 is GeneratorTestComplex's (1 + i)/(4 pi r)
+// (include/htool/testing/generator_test.hpp:110-120).
+struct KernelGenC final : public VirtualGenerator<std::complex<double>> {
+    const double *xt, *xs;
+    int kernel;
+    bool same_points;
+    double wavenumber, diag;
+    KernelGenC(int nr, int nc, const double *xt0, const double *xs0, int kernel0, bool same0, double k0, double diag0)
+        : VirtualGenerator<std::complex<double>>(nr, nc), xt(xt0), xs(xs0), kernel(kernel0), same_points(same0),
+          wavenumber(k0), diag(diag0) {}
+
+    std::complex<double> coef(int i, int j) const {
+        const double *a = xt + 3 * i, *b = xs + 3 * j;
+        double r2 = 0;
+        for (int d = 0; d < 3; d++) {
+            double u = a[d] - b[d];
+            r2       = r2 + u * u;
+        }
+        const double r = std::sqrt(r2), den = 4 * M_PI * r;
+        if (kernel == 4) {
+            if (same_points && i == j)
+                return {diag, 0.0};
+            return {std::cos(wavenumber * r) / den, std::sin(wavenumber * r) / den};
+        }
+        return {1. / den, 1. / den};
+    }
+    void copy_submatrix(int M, int N, const int *const rows, const int *const cols,
+                        std::complex<double> *ptr) const override {
+        for (int i = 0; i < M; i++)
+            for (int j = 0; j < N; j++)
+                ptr[i + M * j] = coef(rows[i], cols[j]);
+    }
+};
+
+void push_entry(std::vector<double> &d, double v) { d.push_back(v); }
+void push_entry(std::vector<double> &d, std::complex<double> v) {
+    d.push_back(v.real());
+    d.push_back(v.imag());
+}
+
 template <typename V>
 void write_vec(const std::string &path, const std::vector<V> &v) {
     FILE *f = std::fopen(path.c_str(), "wb");
@@ -89,6 +135,63 @@ void write_vec(const std::string &path, const std::vector<V> &v) {
 
 double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T>
+void run_hmatrix(std::shared_ptr<Cluster<PCAGeometricClustering>> t, std::shared_ptr<Cluster<PCAGeometricClustering>> s,
+                 VirtualGenerator<T> &A, std::shared_ptr<VirtualLowRankGenerator<T>> comp, const double *xt,
+                 const double *pxs, std::vector<T> &x, int nt, int mu, int nreps, int dump, int reqrank, double eps,
+                 double eta, double maxblocksize, const std::string &out, std::FILE *info) {
+    HMatrix<T> H(t, s, eps, eta, 'N', 'N', reqrank);
+    H.set_compression(comp);
+    H.set_maxblocksize(static_cast<unsigned int>(maxblocksize));
+    double t0 = now();
+    H.build(A, xt, pxs);
+    double t_build = now() - t0;
+
+    // Leaf list in save_plot order (near field, then far field).
+    std::vector<int> leaves;
+    std::vector<double> data;
+    for (auto *d : H.get_MyNearFieldMats()) {
+        leaves.insert(leaves.end(), {d->get_offset_i(), d->nb_rows(), d->get_offset_j(), d->nb_cols(), -1});
+        if (dump)
+            for (int j = 0; j < d->nb_cols(); j++)
+                for (int i = 0; i < d->nb_rows(); i++)
+                    push_entry(data, (*d)(i, j));
+    }
+    for (auto *l : H.get_MyFarFieldMats()) {
+        int k = l->rank_of();
+        leaves.insert(leaves.end(), {l->get_offset_i(), l->nb_rows(), l->get_offset_j(), l->nb_cols(), k});
+        if (dump) {
+            for (int r = 0; r < k; r++)
+                for (int i = 0; i < l->nb_rows(); i++)
+                    push_entry(data, l->get_U(i, r));
+            for (int j = 0; j < l->nb_cols(); j++)
+                for (int r = 0; r < k; r++)
+                    push_entry(data, l->get_V(r, j));
+        }
+    }
+    write_vec(out + "/leaves.i32", leaves);
+    if (dump)
+        write_vec(out + "/data.f64", data);
+
+    // Products: one untimed warm-up, then nreps timed ones.
+    std::vector<T> y(size_t(nt) * mu, T(0));
+    H.mvprod_global_to_global(x.data(), y.data(), mu);
+    double tm = 0;
+    for (int r = 0; r < nreps; r++) {
+        t0 = now();
+        H.mvprod_global_to_global(x.data(), y.data(), mu);
+        tm += now() - t0;
+    }
+    std::vector<double> yd;
+    for (const T &v : y)
+        push_entry(yd, v);
+    write_vec(out + "/y.f64", yd);
+    std::fprintf(info, "build_s=%.9g\nmatvec_s=%.9g\nreps=%d\nnlrmat=%d\nndmat=%d\nfalse_positive=%s\ncompression=%.9g\n",
+                 t_build, nreps > 0 ? tm / nreps : 0.0, nreps, H.get_nlrmat(), H.get_ndmat(),
+                 H.get_infos("Number_of_false_positive").c_str(), H.compression_ratio());
+    std::fclose(info);
 }
 
 } // namespace
@@ -112,7 +215,7 @@ int main(int argc, char **argv) {
     const int mode = hdr[2], nt = hdr[3], ns = hdr[4], same = hdr[5], kernel = hdr[6], mcs = hdr[7];
     const int compressor = hdr[8], mu = hdr[9], nreps = hdr[10], dump = hdr[11], reqrank = hdr[12];
     const double eta = prm[0], eps = prm[1], delta = prm[2], diag = prm[3], maxblocksize = prm[4];
-    std::vector<double> xt(3 * size_t(nt)), xs, x(size_t(ns) * mu);
+    std::vector<double> xt(3 * size_t(nt)), xs, x(size_t(ns) * mu * (kernel >= 4 ? 2 : 1));
     in.read(reinterpret_cast<char *>(xt.data()), xt.size() * sizeof(double));
     if (!same) {
         xs.resize(3 * size_t(ns));
@@ -172,53 +275,21 @@ int main(int argc, char **argv) {
         return 0;
     }
 
-    HMatrix<double> H(t, s, eps, eta, 'N', 'N', reqrank);
-    H.set_compression(comp);
-    H.set_maxblocksize(static_cast<unsigned int>(maxblocksize));
-    t0 = now();
-    H.build(A, xt.data(), pxs);
-    double t_build = now() - t0;
-
-    // Leaf list in save_plot order (near field, then far field).
-    std::vector<int> leaves;
-    std::vector<double> data;
-    for (auto *d : H.get_MyNearFieldMats()) {
-        leaves.insert(leaves.end(), {d->get_offset_i(), d->nb_rows(), d->get_offset_j(), d->nb_cols(), -1});
-        if (dump)
-            for (int j = 0; j < d->nb_cols(); j++)
-                for (int i = 0; i < d->nb_rows(); i++)
-                    data.push_back((*d)(i, j));
+    if (kernel >= 4) {
+        KernelGenC Ac(nt, ns, xt.data(), pxs, kernel, same != 0, delta, diag);
+        std::vector<std::complex<double>> xc(size_t(ns) * mu);
+        for (size_t e = 0; e < xc.size(); e++)
+            xc[e] = {x[2 * e], x[2 * e + 1]};
+        auto compc = compressor == 0
+                         ? std::shared_ptr<VirtualLowRankGenerator<std::complex<double>>>(
+                               std::make_shared<partialACA<std::complex<double>>>())
+                         : std::make_shared<sympartialACA<std::complex<double>>>();
+        run_hmatrix<std::complex<double>>(t, s, Ac, compc, xt.data(), pxs, xc, nt, mu, nreps, dump, reqrank, eps, eta,
+                                          maxblocksize, out, info);
+    } else {
+        run_hmatrix<double>(t, s, A, comp, xt.data(), pxs, x, nt, mu, nreps, dump, reqrank, eps, eta, maxblocksize,
+                            out, info);
     }
-    for (auto *l : H.get_MyFarFieldMats()) {
-        int k = l->rank_of();
-        leaves.insert(leaves.end(), {l->get_offset_i(), l->nb_rows(), l->get_offset_j(), l->nb_cols(), k});
-        if (dump) {
-            for (int r = 0; r < k; r++)
-                for (int i = 0; i < l->nb_rows(); i++)
-                    data.push_back(l->get_U(i, r));
-            for (int j = 0; j < l->nb_cols(); j++)
-                for (int r = 0; r < k; r++)
-                    data.push_back(l->get_V(r, j));
-        }
-    }
-    write_vec(out + "/leaves.i32", leaves);
-    if (dump)
-        write_vec(out + "/data.f64", data);
-
-    // Products: one untimed warm-up, then nreps timed ones.
-    std::vector<double> y(size_t(nt) * mu, 0.0);
-    H.mvprod_global_to_global(x.data(), y.data(), mu);
-    double tm = 0;
-    for (int r = 0; r < nreps; r++) {
-        t0 = now();
-        H.mvprod_global_to_global(x.data(), y.data(), mu);
-        tm += now() - t0;
-    }
-    write_vec(out + "/y.f64", y);
-    std::fprintf(info, "build_s=%.9g\nmatvec_s=%.9g\nreps=%d\nnlrmat=%d\nndmat=%d\nfalse_positive=%s\ncompression=%.9g\n",
-                 t_build, nreps > 0 ? tm / nreps : 0.0, nreps, H.get_nlrmat(), H.get_ndmat(),
-                 H.get_infos("Number_of_false_positive").c_str(), H.compression_ratio());
-    std::fclose(info);
     MPI_Finalize();
     return 0;
 }

@@ -50,16 +50,21 @@ class RefResult:
 
 def run(xt, xs=None, x=None, *, kernel=0, eta=10.0, eps=1e-6, minclustersize=10, compressor=0,
         delta=0.0, diag=0.0, maxblocksize=1e6, mu=1, nreps=0, dump_data=True, reqrank=-1, mode=0,
-        threads=None, timeout=600) -> RefResult:
+        threads=None, timeout=600, wavenumber=0.0) -> RefResult:
+    """kernel 4 (Helmholtz, wavenumber travels in the case file's delta) and 5
+    run the reference's HMatrix<std::complex<double>>."""
+    if kernel == 4 and wavenumber:
+        delta = wavenumber
     if not available():
         raise RuntimeError("reference driver not built: run `make -C oracle`")
     xt = np.ascontiguousarray(xt, dtype=np.float64)
     same = xs is None
     nt = xt.shape[0]
     ns = nt if same else xs.shape[0]
+    cplx = kernel >= 4  # complex families: x, y and the leaf data are complex128
     if x is None:
         x = np.zeros((ns, mu))
-    x = np.asarray(x, dtype=np.float64).reshape(ns, mu)
+    x = np.asarray(x, dtype=np.complex128 if cplx else np.float64).reshape(ns, mu)
     hdr = struct.pack("<14i", 0x48545246, 1, mode, nt, ns, int(same), kernel, minclustersize, compressor,
                       mu, nreps, int(dump_data), reqrank, 0)
     prm = struct.pack("<5d", eta, eps, delta, diag, maxblocksize)
@@ -87,7 +92,8 @@ def run(xt, xs=None, x=None, *, kernel=0, eta=10.0, eps=1e-6, minclustersize=10,
         perm_s = np.fromfile(os.path.join(td, "perm_s.i32"), dtype=np.int32)
         leaves = np.fromfile(os.path.join(td, "leaves.i32"), dtype=np.int32).reshape(-1, 5)
         dpath = os.path.join(td, "data.f64")
-        data = np.fromfile(dpath, dtype=np.float64) if os.path.exists(dpath) else None
+        dt = np.complex128 if cplx else np.float64
+        data = np.fromfile(dpath, dtype=dt) if os.path.exists(dpath) else None
         ypath = os.path.join(td, "y.f64")
-        y = np.fromfile(ypath, dtype=np.float64).reshape(mu, nt).T if os.path.exists(ypath) else None
+        y = np.fromfile(ypath, dtype=dt).reshape(mu, nt).T if os.path.exists(ypath) else None
     return RefResult(perm_t, perm_s, leaves, data, y, info)

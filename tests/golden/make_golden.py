@@ -49,7 +49,7 @@ def save(name, *, xt, xs, kernel, delta, diag, mcs, eta, eps, sym, x, sample=Non
         de = np.flatnonzero(leaves[:, 4] < 0)
         keep = np.sort(np.concatenate([rng.choice(lr, min(sample, len(lr)), replace=False),
                                        rng.choice(de, min(sample // 2, len(de)), replace=False)]))
-    data = np.concatenate([chunks[i] for i in keep]) if len(keep) else np.zeros(0)
+    data = np.concatenate([chunks[i] for i in keep]) if len(keep) else np.zeros(0, dtype=r.data.dtype)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), xt=xt, xs=(np.zeros((0, 3)) if xs is None else xs),
                         same=np.int32(xs is None), kernel=np.int32(kernel), delta=delta, diag=diag,
                         mcs=np.int32(mcs), eta=eta, eps=eps, sym=np.int32(sym), perm_t=r.perm_t, perm_s=r.perm_s,
@@ -94,6 +94,16 @@ def main():
     e = geo.ellipsoid(1500, (1.0, 0.7, 1.6), seed=15)
     save("ellipsoid1500_smooth_sym", xt=e, xs=None, kernel=geo.KERNEL_SMOOTH, delta=1e-5, diag=0.0, mcs=10,
          eta=10.0, eps=1e-6, sym=True, x=rng.standard_normal((1500, 1)), sample=20)
+    # complex families: Helmholtz e^{ikr}/(4 pi r) (wavenumber in `delta`) and (1 + i)/(4 pi r)
+    n = 1000
+    s = geo.sphere(n, seed=21)
+    save("sphere1000_helmholtz", xt=s, xs=None, kernel=4, delta=6.0, diag=geo.laplace_diag(geo.sphere_spacing(n)),
+         mcs=10, eta=10.0, eps=1e-6, sym=False, x=rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2)),
+         sample=20)
+    t = geo.disk(600, 0.0, seed=22)
+    u = geo.disk(400, 1.2, seed=23)
+    save("disks600x400_complex_sym", xt=t, xs=u, kernel=5, delta=0.0, diag=0.0, mcs=10, eta=10.0, eps=1e-6,
+         sym=True, x=rng.standard_normal((400, 1)) + 1j * rng.standard_normal((400, 1)), sample=20)
     for dist, seed in ((15, 100), (40, 200)):
         for reqrank in (-1, 10):
             for sym in (False, True):

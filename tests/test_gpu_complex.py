@@ -1,0 +1,105 @@
+"""GPU: complex kernel families (Helmholtz e^{ikr}/(4 pi r) and the reference's
+GeneratorTestComplex (1 + i)/(4 pi r)).
+
+The complex build must give the oracle's leaves (the oracle is pinned against
+HMatrix<std::complex<double>> of the reference build); the FP64 product must
+equal the oracle's product of the same leaves within 1e-12 and the dense
+product within the ACA tolerance; FP32 and 16-bit storage stay within their
+rounding budgets.
+"""
+import numpy as np
+import pytest
+
+from htool_amd import geometry as geo
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def cvec(n, mu, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((n, mu)) + 1j * rng.standard_normal((n, mu))
+
+
+def helmholtz_case(n, seed, ppw=10.0):
+    x = geo.sphere(n, seed=seed)
+    h = geo.sphere_spacing(n)
+    return x, 2 * np.pi / (ppw * h), geo.laplace_diag(h)
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_helmholtz_build_and_product_match_oracle(htg, orc, sym):
+    print(f"helmholtz sym={sym}", flush=True)
+    x, k, diag = helmholtz_case(3000, 31)
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=10)
+    H.set_compression("sympartialACA" if sym else "partialACA")
+    H.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, diag, k), x)
+    assert H.is_complex
+    O = orc.hmat_build(x, kernel=4, wavenumber=k, diag=diag, minclustersize=10, eta=10.0, eps=1e-6, sym=sym)
+    leaves, data = H.get_leaves(with_data=True)
+    assert np.array_equal(leaves, O.leaves)
+    assert np.iscomplexobj(data)
+    # dense leaves (generator) to rounding of sin/cos, factors to the ACA's rounding
+    for (oi, m, oj, n, kk, a), (_, _, _, _, _, b) in zip(orc.iter_leaf_data(leaves, data),
+                                                          orc.iter_leaf_data(O.leaves, O.data)):
+        if kk < 0:
+            assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b))
+        elif kk > 0:
+            assert np.max(np.abs(a[0] @ a[1] - b[0] @ b[1])) <= 1e-9 * np.max(np.abs(b[0] @ b[1]))
+    v = cvec(3000, 3, 1)
+    y = H.mvprod_global_to_global(v)
+    assert rel(y, O.mvprod(v)) < 1e-12
+    pt, ps = H.get_perms()
+    assert rel(y, orc.leaf_mvprod(3000, 3000, pt, ps, leaves, data, v)) < 1e-12
+    rows = np.random.default_rng(2).choice(3000, 300, replace=False).astype(np.int32)
+    yd = orc.dense_rows_c(x, None, rows, v[:, 0], kernel=4, wavenumber=k, diag=diag, same=True)
+    assert rel(y[rows, 0], yd) < 1e-5
+
+
+def test_complex_test_kernel_on_two_disks(htg, orc):
+    t = geo.disk(1500, 0.0, seed=41)
+    s = geo.disk(1200, 1.0, seed=42)
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=10)
+    H.set_compression("partialACA")
+    H.build(htg.Generator(geo.KERNEL_COMPLEX_TEST), t, s)
+    O = orc.hmat_build(t, s, kernel=5, minclustersize=10, eta=10.0, eps=1e-6)
+    assert np.array_equal(H.get_leaves(), O.leaves)
+    v = cvec(1200, 2, 3)
+    assert rel(H.mvprod_global_to_global(v), O.mvprod(v)) < 1e-12
+
+
+@pytest.mark.parametrize("mu", [1, 5, 16, 40])
+def test_complex_multi_rhs_columns(htg, mu):
+    print(f"complex mu={mu}", flush=True)
+    x, k, diag = helmholtz_case(6000, 51)
+    H = htg.HMatrix(epsilon=1e-5, eta=10.0, minclustersize=32)
+    H.set_compression("partialACA")
+    H.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, diag, k), x)
+    V = cvec(6000, mu, mu)
+    Y = H.mvprod_global_to_global(V)
+    for c in sorted({0, mu // 2, mu - 1}):
+        assert rel(Y[:, c], H @ V[:, c]) < 1e-12
+
+
+def test_complex_storage_modes(htg, orc):
+    x, k, diag = helmholtz_case(8000, 61)
+    H = htg.HMatrix(epsilon=1e-5, eta=10.0, minclustersize=32)
+    H.set_compression("partialACA")
+    H.set_lowprec(0.0, all_terms=True)  # every low-rank term 16-bit, per-term scaled
+    H.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, diag, k), x)
+    info = H.get_info()
+    assert info.is_complex and info.lowprec_entries == info.lowrank_entries > 0
+    V = cvec(8000, 4, 7)
+    Y64 = H.mvprod_global_to_global(V)
+    rows = np.random.default_rng(8).choice(8000, 300, replace=False).astype(np.int32)
+    yd = orc.dense_rows_c(x, None, rows, V[:, 0], kernel=4, wavenumber=k, diag=diag, same=True)
+    assert rel(Y64[rows, 0], yd) < 1e-4
+    errs = {}
+    for storage, bound in (("fp32", 2e-6), ("fp32_fp16", 2e-3), ("fp32_bf16", 2e-2)):
+        H.set_storage(storage)
+        errs[storage] = rel(H.mvprod_global_to_global(V), Y64)
+        assert errs[storage] < bound, (storage, errs[storage])
+    print("complex storage errors", errs, flush=True)

