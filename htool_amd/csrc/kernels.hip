@@ -1527,6 +1527,348 @@ __global__ void __launch_bounds__(WG) k_vx_mfma(const int *wv_off, int nwaves, c
     }
 }
 
+// ---------------------------------------------------------------------------
+// Complex 16-RHS products on the matrix cores. The 32 B columns of a tile are
+// the 16 right-hand sides' real parts (slots 0-15) then imaginary parts (slots
+// 16-31), in the blocked z layout (zblk(j, slot)). Each stored complex entry
+// (ar, ai) feeds two MFMAs with the same B row: one accumulator for Re(A), one
+// for Im(A). Slot c of the result is Re y_s = (Re A) x_re - (Im A) x_im for
+// c = s < 16 and Im y_s = (Re A) x_im + (Im A) x_re for c = s + 16; the Im(A)
+// accumulator's slots are exchanged by one lane shuffle (c xor 16). Entries
+// per 16-byte vector: VE = 2 (FP32 pairs) or 4 (16-bit pairs).
+constexpr int MFMA_MU_C   = 16; // complex right-hand sides per matrix-core pass
+constexpr int RPASS_C     = 64; // row-block entries staged per pass
+
+template <int VE>
+__device__ __forceinline__ void load_b_slots(const float *p, float (&b)[VE]) {
+    if constexpr (VE == 2) {
+        const float2 v = *reinterpret_cast<const float2 *>(p);
+        b[0] = v.x, b[1] = v.y;
+    } else {
+        const float4 v = *reinterpret_cast<const float4 *>(p);
+        b[0] = v.x, b[1] = v.y, b[2] = v.z, b[3] = v.w;
+    }
+}
+
+// Combines the Re(A) and Im(A) accumulators of a 32x32 tile into slot values.
+__device__ __forceinline__ void combine32(const floatx16 &ar, const floatx16 &ai, int lane, double (&accd)[16]) {
+    const bool im_slot = (lane & 31) >= 16;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const float t = __shfl_xor(ai[i], 16, 64);
+        accd[i] += (double)(im_slot ? ar[i] + t : ar[i] - t);
+    }
+}
+
+template <typename TS, int S>
+__device__ __forceinline__ void rows_class_mfma_c(const TS *arena, const RowClass &rc, int rb, int R, const float *z,
+                                                  float *s_z, int t, int q, bool active, double (&accd)[16]) {
+    constexpr int SL   = 32; // B slots
+    constexpr int VW   = Vec<TS>::W;
+    constexpr int VE   = VW / 2;
+    constexpr int WCOL = RPASS_C / S; // entries per wave per pass
+    constexpr int NIT  = WCOL / (2 * VE);
+    constexpr int ZPT  = RPASS_C * SL / WG;
+    constexpr bool PREFETCH_A = S == 4;
+    const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
+    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+    if (Cp == 0)
+        return;
+    const TS *A    = arena + 2 * rc.rb_aoff[rb];
+    const int lane = threadIdx.x & 63, half = lane >> 5;
+    const int row  = 32 * t + (lane & 31);
+    const TS *rowp = A + 2 * ((size_t)(row < R ? row : 0) * Cp + q * WCOL);
+    const int zs = threadIdx.x & (SL - 1), zc = threadIdx.x / SL;
+    float v[ZPT];
+    TS a[NIT][VW], an[PREFETCH_A ? NIT : 1][VW];
+    auto load_z = [&](int cc) {
+        int idx[ZPT];
+#pragma unroll
+        for (int i = 0; i < ZPT; i++) {
+            const int c = cc + zc + (WG / SL) * i;
+            idx[i]      = rc.colidx[cb + (c < C ? c : 0)];
+        }
+#pragma unroll
+        for (int i = 0; i < ZPT; i++)
+            v[i] = z[zblk(idx[i], zs)];
+    };
+    auto load_a = [&](int cc, auto &dst) {
+        const int nz = min(RPASS_C, Cp - cc);
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int kk = q * WCOL + 2 * VE * it + VE * half;
+            Vec<TS>::load(rowp + 2 * (cc + (kk < nz ? kk - q * WCOL : -q * WCOL)), dst[it]);
+        }
+    };
+    load_z(0);
+    if (active)
+        load_a(0, a);
+    for (int cc = 0; cc < Cp; cc += RPASS_C) {
+        const int nz = min(RPASS_C, Cp - cc);
+#pragma unroll
+        for (int i = 0; i < ZPT; i++) {
+            const int c      = zc + (WG / SL) * i;
+            s_z[zblk(c, zs)] = cc + c < C ? v[i] : 0.0f;
+        }
+        __syncthreads();
+        const bool more = cc + RPASS_C < Cp;
+        if (more) {
+            load_z(cc + RPASS_C);
+            if (PREFETCH_A && active)
+                load_a(cc + RPASS_C, an);
+        }
+        if (active) {
+            floatx16 ar = {0}, ai = {0};
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                const int k0 = q * WCOL + 2 * VE * it;
+                if (k0 < nz) {
+                    const int kk = k0 + VE * half;
+                    float b[VE];
+                    load_b_slots<VE>(s_z + zblk(kk, lane & 31), b);
+#pragma unroll
+                    for (int e = 0; e < VE; e++) {
+                        ar = __builtin_amdgcn_mfma_f32_32x32x2f32(cvt<float>(a[it][2 * e]), b[e], ar, 0, 0, 0);
+                        ai = __builtin_amdgcn_mfma_f32_32x32x2f32(cvt<float>(a[it][2 * e + 1]), b[e], ai, 0, 0, 0);
+                    }
+                }
+            }
+            combine32(ar, ai, lane, accd);
+        }
+        __syncthreads();
+        if (more && active) {
+            if constexpr (PREFETCH_A) {
+#pragma unroll
+                for (int it = 0; it < NIT; it++)
+#pragma unroll
+                    for (int u = 0; u < VW; u++)
+                        a[it][u] = an[it][u];
+            } else {
+                load_a(cc + RPASS_C, a);
+            }
+        }
+    }
+}
+
+template <typename TS0, typename TS1, int S>
+__device__ __forceinline__ void rows_mfma_split_c(const int rb, const int R, const RowClass &c0, const TS0 *arena0,
+                                                  const RowClass &c1, const TS1 *arena1, const float *z, float *s_z,
+                                                  double (&accd)[16], int &q, bool &active) {
+    const int w = threadIdx.x >> 6, T = (R + 31) / 32;
+    const int t = w / S;
+    q           = w % S;
+    active      = t < T;
+    rows_class_mfma_c<TS0, S>(arena0, c0, rb, R, z, s_z, t, q, active, accd);
+    if (arena1)
+        rows_class_mfma_c<TS1, S>(arena1, c1, rb, R, z, s_z, t, q, active, accd);
+}
+
+template <typename TS0, typename TS1>
+__global__ void __launch_bounds__(WG) k_rows_mfma_c(const int *rb_order, const int *rb_row0, RowClass c0,
+                                                    const TS0 *arena0, RowClass c1, const TS1 *arena1,
+                                                    const float *z, const int *perm_t, int nr, int mu_base,
+                                                    double *y) {
+    __shared__ __align__(16) double smem[3 * 16 * 64];
+    float *s_z   = reinterpret_cast<float *>(smem);
+    const int rb = rb_order[blockIdx.x];
+    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int T = (R + 31) / 32;
+    const int S = T == 1 ? 4 : (T == 2 ? 2 : 1);
+    double accd[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        accd[i] = 0;
+    int q       = 0;
+    bool active = false;
+    if (S == 4)
+        rows_mfma_split_c<TS0, TS1, 4>(rb, R, c0, arena0, c1, arena1, z, s_z, accd, q, active);
+    else if (S == 2)
+        rows_mfma_split_c<TS0, TS1, 2>(rb, R, c0, arena0, c1, arena1, z, s_z, accd, q, active);
+    else
+        rows_mfma_split_c<TS0, TS1, 1>(rb, R, c0, arena0, c1, arena1, z, s_z, accd, q, active);
+    const int t = w / S;
+    if (S > 1) {
+        if (active && q > 0)
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                smem[((t * (S - 1) + q - 1) * 16 + i) * 64 + lane] = accd[i];
+        __syncthreads();
+        if (active && q == 0)
+            for (int qq = 1; qq < S; qq++)
+#pragma unroll
+                for (int i = 0; i < 16; i++)
+                    accd[i] += smem[((t * (S - 1) + qq - 1) * 16 + i) * 64 + lane];
+    }
+    if (active && q == 0) {
+        const int c = lane & 31, sr = c & 15, part = c >> 4;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int orow = 32 * t + mfma32_row(i, lane);
+            if (orow < R)
+                y[2 * ((size_t)perm_t[r0 + orow] + (size_t)nr * (mu_base + sr)) + part] = accd[i];
+        }
+    }
+}
+
+// Stores slot c of row r of a chunk's V x (scaled when the leaf is one chunk).
+__device__ __forceinline__ void vx_store_slot(const VTile &d, int r, int c, double v, float *z, double *part,
+                                              const double *tscale) {
+    if (d.nparts > 1)
+        part[(size_t)(d.part_off + r) * 32 + c] = v;
+    else
+        z[zblk(d.t_index + r, c)] = (float)(tscale ? v * tscale[d.t_index + r] : v);
+}
+
+// Chunks of at most 16 rows: v_mfma_f32_16x16x4f32, lane (g, r) = (l / 16, l % 16)
+// takes row r and entry 4 VE it + VE g + e; four accumulators (Re/Im A x slots
+// 0-15 / 16-31), so lane r holds both parts of right-hand side r.
+template <typename TS>
+__device__ __forceinline__ void vx_chunk_mfma16_c(const VTile &d, const TS *varena, float *z, double *part,
+                                                  const double *tscale, int lane) {
+    constexpr int VW  = Vec<TS>::W;
+    constexpr int VE  = VW / 2;
+    constexpr int NIT = VPASS16 / (4 * VE);
+    const int g = lane >> 4, r = lane & 15;
+    const TS *rowp = varena + 2 * (d.v_off + (size_t)(r < d.k ? r : 0) * d.ld);
+    double re[4] = {}, im[4] = {};
+    for (int c0 = 0; c0 < d.ncols; c0 += VPASS16) {
+        const int nc = min(VPASS16, d.ncols - c0);
+        TS a[NIT][VW];
+        float b0[NIT][VE], b1[NIT][VE];
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int kk = 4 * VE * it + VE * g;
+            const int kc = kk < nc ? kk : 0;
+            Vec<TS>::load(rowp + 2 * (c0 + kc), a[it]);
+            const size_t j = (size_t)d.x_off + c0 + kc;
+            load_b_slots<VE>(z + zblk(j, r), b0[it]);
+            load_b_slots<VE>(z + zblk(j, r + 16), b1[it]);
+        }
+        floatx4 rr = {0}, ri = {0}, ir = {0}, ii = {0}; // (Re|Im A) x (Re|Im x)
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            if (4 * VE * it < nc) {
+                const bool in = 4 * VE * it + VE * g < nc;
+#pragma unroll
+                for (int e = 0; e < VE; e++) {
+                    const float xr = cvt<float>(a[it][2 * e]), xi = cvt<float>(a[it][2 * e + 1]);
+                    const float arv = in ? xr : 0.0f, aiv = in ? xi : 0.0f;
+                    rr = __builtin_amdgcn_mfma_f32_16x16x4f32(arv, b0[it][e], rr, 0, 0, 0);
+                    ri = __builtin_amdgcn_mfma_f32_16x16x4f32(arv, b1[it][e], ri, 0, 0, 0);
+                    ir = __builtin_amdgcn_mfma_f32_16x16x4f32(aiv, b0[it][e], ir, 0, 0, 0);
+                    ii = __builtin_amdgcn_mfma_f32_16x16x4f32(aiv, b1[it][e], ii, 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            re[i] += (double)(rr[i] - ii[i]);
+            im[i] += (double)(ri[i] + ir[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int row = 4 * g + i;
+        if (row < d.k) {
+            vx_store_slot(d, row, r, re[i], z, part, tscale);
+            vx_store_slot(d, row, r + 16, im[i], z, part, tscale);
+        }
+    }
+}
+
+// Chunks of more than 16 rows: v_mfma_f32_32x32x2f32 over 32-row tiles.
+template <typename TS>
+__device__ __forceinline__ void vx_chunk_mfma32_c(const VTile &d, const TS *varena, float *z, double *part,
+                                                  const double *tscale, int lane) {
+    constexpr int VW  = Vec<TS>::W;
+    constexpr int VE  = VW / 2;
+    constexpr int NIT = VPASS32 / (2 * VE);
+    const int half = lane >> 5, c = lane & 31;
+    for (int rt = 0; rt < d.k; rt += 32) {
+        const int row  = rt + c;
+        const TS *rowp = varena + 2 * (d.v_off + (size_t)(row < d.k ? row : 0) * d.ld);
+        double accd[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            accd[i] = 0;
+        for (int c0 = 0; c0 < d.ncols; c0 += VPASS32) {
+            const int nc = min(VPASS32, d.ncols - c0);
+            TS a[NIT][VW];
+            float b[NIT][VE];
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                const int kk = 2 * VE * it + VE * half;
+                const int kc = kk < nc ? kk : 0;
+                Vec<TS>::load(rowp + 2 * (c0 + kc), a[it]);
+                load_b_slots<VE>(z + zblk((size_t)d.x_off + c0 + kc, c), b[it]);
+            }
+            floatx16 ar = {0}, ai = {0};
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                if (2 * VE * it < nc) {
+                    const bool in = 2 * VE * it + VE * half < nc;
+#pragma unroll
+                    for (int e = 0; e < VE; e++) {
+                        ar = __builtin_amdgcn_mfma_f32_32x32x2f32(in ? cvt<float>(a[it][2 * e]) : 0.0f, b[it][e], ar,
+                                                                  0, 0, 0);
+                        ai = __builtin_amdgcn_mfma_f32_32x32x2f32(in ? cvt<float>(a[it][2 * e + 1]) : 0.0f,
+                                                                  b[it][e], ai, 0, 0, 0);
+                    }
+                }
+            }
+            combine32(ar, ai, lane, accd);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int r = rt + mfma32_row(i, lane);
+            if (r < d.k)
+                vx_store_slot(d, r, c, accd[i], z, part, tscale);
+        }
+    }
+}
+
+template <typename TS0, typename TS1>
+__global__ void __launch_bounds__(WG) k_vx_mfma_c(const int *wv_off, int nwaves, const VTile *tiles,
+                                                  const TS0 *varena0, const TS1 *varena1, float *z, double *part,
+                                                  const double *tscale) {
+    const int wv = blockIdx.x * NWAVE + (threadIdx.x >> 6);
+    if (wv >= nwaves)
+        return;
+    const int lane = threadIdx.x & 63;
+    for (int ti = wv_off[wv]; ti < wv_off[wv + 1]; ti++) {
+        const VTile d = tiles[ti];
+        if (d.cls == 0) {
+            if (d.k <= 16)
+                vx_chunk_mfma16_c<TS0>(d, varena0, z, part, tscale, lane);
+            else
+                vx_chunk_mfma32_c<TS0>(d, varena0, z, part, tscale, lane);
+        } else {
+            if (d.k <= 16)
+                vx_chunk_mfma16_c<TS1>(d, varena1, z, part, tscale, lane);
+            else
+                vx_chunk_mfma32_c<TS1>(d, varena1, z, part, tscale, lane);
+        }
+    }
+}
+
+// Complex x (nc x 16, original order) into the blocked slots: Re in slot s,
+// Im in slot s + 16; zeros from nc on.
+__global__ void k_gather_xc_blk(const double *x, const int *perm_s, int nc, int tpad, int mu_base, float *z) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= tpad * MFMA_MU_C)
+        return;
+    const int j = e / MFMA_MU_C, s = e % MFMA_MU_C;
+    float re = 0, im = 0;
+    if (j < nc) {
+        const double *src = x + 2 * (perm_s[j] + (size_t)nc * (mu_base + s));
+        re                = (float)src[0];
+        im                = (float)src[1];
+    }
+    z[zblk(j, s)]      = re;
+    z[zblk(j, s + 16)] = im;
+}
+
 template <typename TS>
 __global__ void k_convert(const double *src, TS *dst, uint64_t n) {
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x)
@@ -1946,6 +2288,12 @@ template <typename R>
 void launch_gather_xc(const double *x, const int *perm_s, int nc, int tpad, int mu_base, int MU, R *z,
                       hipStream_t st) {
     const dim3 grid((tpad * MU + WG - 1) / WG);
+    if constexpr (std::is_same<R, float>::value) {
+        if (MU == MFMA_MU_C) {
+            hipLaunchKernelGGL(k_gather_xc_blk, grid, dim3(WG), 0, st, x, perm_s, nc, tpad, mu_base, z);
+            return;
+        }
+    }
 #define HTG_GATHER_C(M) hipLaunchKernelGGL((k_gather_xc<R, M>), grid, dim3(WG), 0, st, x, perm_s, nc, tpad, mu_base, z)
     HTG_MU_SWITCH_C(MU, HTG_GATHER_C)
 #undef HTG_GATHER_C
@@ -1957,6 +2305,16 @@ void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const int *
                  hipStream_t st) {
     if (ngroups <= 0)
         return;
+    if constexpr (std::is_same<R, float>::value) {
+        if (MU == MFMA_MU_C) {
+            hipLaunchKernelGGL((k_vx_mfma_c<TS0, TS1>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off,
+                               ngroups, tiles, varena0, varena1, z, part, tscale);
+            if (nmulti > 0) // slots sum independently: the real 32-slot reduction applies
+                hipLaunchKernelGGL((k_vx_sum<float, 32, true>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first,
+                                   part, z, tscale);
+            return;
+        }
+    }
 #define HTG_VXC(M)                                                                                                  \
     do {                                                                                                            \
         hipLaunchKernelGGL((k_vx_c<TS0, TS1, R, M>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, \
@@ -1975,6 +2333,13 @@ void launch_rows_c(const int *rb_order, const int *rb_row0, int nrb, RowClass c0
                    hipStream_t st) {
     if (nrb <= 0)
         return;
+    if constexpr (std::is_same<R, float>::value) {
+        if (MU == MFMA_MU_C) {
+            hipLaunchKernelGGL((k_rows_mfma_c<TS0, TS1>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0,
+                               c1, arena1, z, perm_t, nr, mu_base, y);
+            return;
+        }
+    }
 #define HTG_ROWSC(M)                                                                                          \
     hipLaunchKernelGGL((k_rows_c<TS0, TS1, R, M>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0, c1, \
                        arena1, z, perm_t, nr, mu_base, y)

@@ -103,3 +103,23 @@ def test_complex_storage_modes(htg, orc):
         errs[storage] = rel(H.mvprod_global_to_global(V), Y64)
         assert errs[storage] < bound, (storage, errs[storage])
     print("complex storage errors", errs, flush=True)
+
+
+@pytest.mark.parametrize("storage,bound", [("fp32", 1e-6), ("fp32_fp16", 1e-6), ("fp32_bf16", 1e-6)])
+def test_complex_matrix_core_passes_match_single_rhs(htg, storage, bound):
+    """40 complex right-hand sides in FP32 arithmetic: two 16-wide matrix-core
+    passes and an 8-wide pass; every column agrees with the single-vector
+    product of the same storage (FP32 arithmetic both, different order)."""
+    print(f"complex mfma {storage}", flush=True)
+    x, k, diag = helmholtz_case(7000, 71)
+    H = htg.HMatrix(epsilon=1e-5, eta=10.0, minclustersize=32)
+    H.set_compression("partialACA")
+    H.set_lowprec(2e-5)  # trailing terms in the 16-bit class
+    H.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, diag, k), x)
+    info = H.get_info()
+    assert 0 < info.lowprec_entries < info.lowrank_entries
+    H.set_storage(storage)
+    V = cvec(7000, 40, 12)
+    Y = H.mvprod_global_to_global(V)
+    for c in (0, 5, 15, 16, 31, 32, 39):
+        assert rel(Y[:, c], H @ V[:, c]) < bound, c
