@@ -1091,14 +1091,26 @@ __global__ void __launch_bounds__(WG) k_vx(const int *wv_off, int nwaves, const 
     }
 }
 
-// Adds the chunk partial sums of each multi-chunk leaf into its t entries.
+// Adds the chunk partial sums of each multi-chunk row group into its t
+// entries, in chunk order: one wavefront per group for one RHS (at most a few
+// dozen sums), one workgroup per group otherwise.
+template <int MU>
+constexpr int vx_sum_groups_per_block() {
+    return MU == 1 ? NWAVE : 1;
+}
+
 template <typename TC, int MU, bool BLK = false>
-__global__ void __launch_bounds__(WG) k_vx_sum(const VTile *tiles, const int *first_chunk, const double *part, TC *z,
-                                               const double *tscale) {
-    const VTile d    = tiles[first_chunk[blockIdx.x]];
+__global__ void __launch_bounds__(WG) k_vx_sum(const VTile *tiles, const int *first_chunk, int nmulti,
+                                               const double *part, TC *z, const double *tscale) {
+    constexpr int GPB = vx_sum_groups_per_block<MU>();
+    constexpr int TPG = WG / GPB; // threads per group
+    const int gi      = blockIdx.x * GPB + (int)threadIdx.x / TPG;
+    if (gi >= nmulti)
+        return;
+    const VTile d    = tiles[first_chunk[gi]];
     const int km     = d.k * MU;
     const double *pb = part + (size_t)d.part_base * MU;
-    for (int e = threadIdx.x; e < km; e += WG) {
+    for (int e = threadIdx.x % TPG; e < km; e += TPG) {
         double v = 0;
         for (int c = 0; c < d.nparts; c++)
             v += pb[(size_t)c * km + e];
@@ -2026,12 +2038,17 @@ __global__ void __launch_bounds__(WG) k_vx_c(const int *wv_off, int nwaves, cons
 }
 
 template <typename R, int MU>
-__global__ void __launch_bounds__(WG) k_vx_sum_c(const VTile *tiles, const int *first_chunk, const double *part, R *z,
-                                                 const double *tscale) {
-    const VTile d    = tiles[first_chunk[blockIdx.x]];
+__global__ void __launch_bounds__(WG) k_vx_sum_c(const VTile *tiles, const int *first_chunk, int nmulti,
+                                                 const double *part, R *z, const double *tscale) {
+    constexpr int GPB = vx_sum_groups_per_block<MU>();
+    constexpr int TPG = WG / GPB;
+    const int gi      = blockIdx.x * GPB + (int)threadIdx.x / TPG;
+    if (gi >= nmulti)
+        return;
+    const VTile d    = tiles[first_chunk[gi]];
     const int km     = d.k * MU;
     const double *pb = part + 2 * (size_t)d.part_base * MU;
-    for (int e = threadIdx.x; e < km; e += WG) {
+    for (int e = threadIdx.x % TPG; e < km; e += TPG) {
         double re = 0, im = 0;
         for (int c = 0; c < d.nparts; c++) {
             re += pb[2 * ((size_t)c * km + e)];
@@ -2239,7 +2256,7 @@ void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *mu
                                ngroups, tiles, varena0, varena1, z, part, tscale);
             if (nmulti > 0)
                 hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU, true>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first,
-                                   part, z, tscale);
+                                   nmulti, part, z, tscale);
             return;
         }
     }
@@ -2248,8 +2265,9 @@ void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *mu
         hipLaunchKernelGGL((k_vx<TS0, TS1, TC, M>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, \
                            ngroups, tiles, varena0, varena1, z, part, tscale);                             \
         if (nmulti > 0)                                                                                    \
-            hipLaunchKernelGGL((k_vx_sum<TC, M>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first, part, z,  \
-                               tscale);                                                                    \
+            hipLaunchKernelGGL((k_vx_sum<TC, M>), dim3((nmulti + vx_sum_groups_per_block<M>() - 1) /         \
+                                                     vx_sum_groups_per_block<M>()),                         \
+                               dim3(WG), 0, st, tiles, multi_first, nmulti, part, z, tscale);              \
     } while (0)
     HTG_MU_SWITCH(MU, HTG_VX)
 #undef HTG_VX
@@ -2311,7 +2329,7 @@ void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const int *
                                ngroups, tiles, varena0, varena1, z, part, tscale);
             if (nmulti > 0) // slots sum independently: the real 32-slot reduction applies
                 hipLaunchKernelGGL((k_vx_sum<float, 32, true>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first,
-                                   part, z, tscale);
+                                   nmulti, part, z, tscale);
             return;
         }
     }
@@ -2320,8 +2338,9 @@ void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const int *
         hipLaunchKernelGGL((k_vx_c<TS0, TS1, R, M>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, \
                            ngroups, tiles, varena0, varena1, z, part, tscale);                                      \
         if (nmulti > 0)                                                                                             \
-            hipLaunchKernelGGL((k_vx_sum_c<R, M>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first, part, z,      \
-                               tscale);                                                                             \
+            hipLaunchKernelGGL((k_vx_sum_c<R, M>), dim3((nmulti + vx_sum_groups_per_block<M>() - 1) /            \
+                                                       vx_sum_groups_per_block<M>()),                            \
+                               dim3(WG), 0, st, tiles, multi_first, nmulti, part, z, tscale);                       \
     } while (0)
     HTG_MU_SWITCH_C(MU, HTG_VXC)
 #undef HTG_VXC
