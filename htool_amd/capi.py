@@ -87,6 +87,7 @@ _SIGNATURES = {
                            _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "htg_hmatrix_aca_records": (_I, [_VP, C.POINTER(_I), _VP, _VP, _VP]),
     "htg_hmatrix_build_batch": (_I, [C.POINTER(htg_params), _I, _VP, _VP, _VP, C.POINTER(_VP)]),
+    "htg_hmatrix_transpose": (_I, [_VP, C.POINTER(_VP)]),
     "htg_hmatrix_get_layout": (_I, [_VP, C.POINTER(htg_layout)]),
     "htg_hmatrix_set_layout": (_I, [_VP, C.POINTER(htg_layout)]),
     "htg_hmatrix_autotune": (_I, [_VP, _I, _I, C.POINTER(htg_layout), C.POINTER(_D), C.POINTER(_I)]),

@@ -55,10 +55,14 @@ using namespace htg;
 
 // One assembled instance inside an H-matrix: its leaves are [leaf_begin,
 // leaf_end) of the (sorted) leaf list, its rows and columns start at `shift`.
+// A transposed part (the H-matrix of A^T) shares the assembly of A: its
+// leaves have rows and columns exchanged, U and V exchanged in the store, and
+// dense leaves come from the generator with the point sets exchanged.
 struct Part {
-    std::unique_ptr<htg::Assembly> A;
+    std::shared_ptr<htg::Assembly> A;
     size_t leaf_begin = 0, leaf_end = 0;
-    int shift = 0;
+    int shift       = 0;
+    bool transposed = false;
 };
 
 struct htg_hmatrix {
@@ -526,6 +530,7 @@ namespace {
 // into the FP64 master arenas.
 void fill_from_assembly(htg_hmatrix &h, const Part &part) {
     const Assembly &A = *part.A;
+    const DevPoints rowsP = part.transposed ? A.dS() : A.dT(), colsP = part.transposed ? A.dT() : A.dS();
     std::vector<FillTile> fills;
     std::vector<LrPack> packs[2];
     std::vector<UTileDst> udst[2];
@@ -567,10 +572,10 @@ void fill_from_assembly(htg_hmatrix &h, const Part &part) {
     DevArray<FillTile> d_fills;
     d_fills.upload(fills);
     if (h.cplx)
-        launch_dense_fill_c(A.kd, A.dT(), A.dS(), d_fills.p, (int)fills.size(),
+        launch_dense_fill_c(A.kd, rowsP, colsP, d_fills.p, (int)fills.size(),
                             reinterpret_cast<cdbl *>(h.cls[0].a64.p), nullptr);
     else
-        launch_dense_fill(A.kd, A.dT(), A.dS(), d_fills.p, (int)fills.size(), h.cls[0].a64.p, nullptr);
+        launch_dense_fill(A.kd, rowsP, colsP, d_fills.p, (int)fills.size(), h.cls[0].a64.p, nullptr);
     HTG_HIP(hipGetLastError());
     for (int c = 0; c < 2; c++) {
         if (packs[c].empty())
@@ -891,6 +896,46 @@ int htg_hmatrix_build_batch(const htg_params *prm, int count, const htg_generato
             fill_from_assembly(*h, part);
         h->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         *out             = h.release();
+    });
+}
+
+int htg_hmatrix_transpose(const htg_hmatrix *h, htg_hmatrix **out) {
+    return guard([&] {
+        if (!h || !out)
+            throw Error(HTG_ERR_INVALID, "null argument");
+        if (h->parts.empty())
+            throw Error(HTG_ERR_UNSUPPORTED, "the transpose needs an H-matrix assembled by htg_hmatrix_build or "
+                                             "htg_hmatrix_build_batch");
+        std::unique_ptr<htg_hmatrix> t(new htg_hmatrix);
+        t->nr             = h->nc;
+        t->nc             = h->nr;
+        t->cplx           = h->cplx;
+        t->lay            = h->lay;
+        t->perm_t         = h->perm_s;
+        t->perm_s         = h->perm_t;
+        t->false_positive = h->false_positive;
+        t->leaves.reserve(h->leaves.size());
+        for (Leaf L : h->leaves) {
+            std::swap(L.oi, L.oj);
+            std::swap(L.m, L.n);
+            std::swap(L.src_u, L.src_v); // V (k x n row-major) is U^T's column-major block, and back
+            t->leaves.push_back(std::move(L));
+        }
+        sort_leaves(t->leaves);
+        for (const Part &p : h->parts) {
+            Part q;
+            q.A          = p.A;
+            q.shift      = p.shift;
+            q.transposed = !p.transposed;
+            locate_part(*t, q, q.transposed ? p.A->nc : p.A->nr);
+            t->parts.push_back(std::move(q));
+        }
+        build_layout(*t);
+        upload_layout(*t);
+        alloc_master(*t);
+        for (const Part &part : t->parts)
+            fill_from_assembly(*t, part);
+        *out = t.release();
     });
 }
 

@@ -176,6 +176,7 @@ class HMatrix:
         return obj
 
     def _close(self):
+        self._transposed = None
         if self._h is not None and self._h.value:
             capi.lib().htg_hmatrix_destroy(self._h)
         self._h = None
@@ -219,6 +220,25 @@ class HMatrix:
         y = np.zeros((info.nr, mu), dtype=dt, order="F")
         capi.check(capi.lib().htg_hmatrix_mvprod(self._handle(), capi.ptr(xf), capi.ptr(y), mu))
         return y[:, 0].copy() if vec else np.ascontiguousarray(y)
+
+    def transpose(self) -> "HMatrix":
+        """The H-matrix of A^T (htg_hmatrix_transpose), in FP64 storage."""
+        h = C.c_void_p()
+        capi.check(capi.lib().htg_hmatrix_transpose(self._handle(), C.byref(h)))
+        T = HMatrix(self.epsilon, self.eta, self.reqrank, self.minclustersize)
+        T._h = h
+        return T
+
+    def mvprod_transp_global_to_global(self, x: np.ndarray) -> np.ndarray:
+        """y = A^T x (include/htool/types/virtual_hmatrix.hpp:58), through the
+        transposed H-matrix, built on first use and kept in this storage mode."""
+        T = getattr(self, "_transposed", None)
+        if T is None or T.storage != self._storage:
+            T = self.transpose()
+            if self._storage != "fp64":
+                T.set_storage(self._storage)
+            self._transposed = T
+        return T.mvprod_global_to_global(x)
 
     def __matmul__(self, x):
         """operator* (include/htool/types/hmatrix.hpp:1596-1602)."""

@@ -165,6 +165,15 @@ int htg_hmatrix_build_batch(const htg_params *params, int count, const htg_gener
 
 int htg_hmatrix_destroy(htg_hmatrix *h);
 
+/* The H-matrix of the transposed matrix (no conjugation), sharing the
+ * assembly: its products are HMatrix::mvprod_transp_global_to_global
+ * (include/htool/types/virtual_hmatrix.hpp:58; hmatrix.hpp:1276-1455). The
+ * leaves of A^T are the leaves of A with rows and columns exchanged (dense
+ * blocks transposed, U and V exchanged). Matrices from htg_hmatrix_build /
+ * _build_batch only; the result starts in FP64 storage and is destroyed with
+ * htg_hmatrix_destroy, independently of h. */
+int htg_hmatrix_transpose(const htg_hmatrix *h, htg_hmatrix **out);
+
 /* Product work decomposition: read, or set and rebuild the product layout
  * (matrices from htg_hmatrix_build / _build_batch only). */
 int htg_hmatrix_get_layout(const htg_hmatrix *h, htg_layout *layout);

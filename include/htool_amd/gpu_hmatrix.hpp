@@ -54,7 +54,10 @@ class GpuHMatrix {
     }
     GpuHMatrix(const GpuHMatrix &) = delete;
     GpuHMatrix &operator=(const GpuHMatrix &) = delete;
-    ~GpuHMatrix() { htg_hmatrix_destroy(h_); }
+    ~GpuHMatrix() {
+        htg_hmatrix_destroy(ht_);
+        htg_hmatrix_destroy(h_);
+    }
 
     // setters of hmatrix.hpp:213-220 and Cluster::set_minclustersize (cluster.hpp:334)
     void set_epsilon(double e) { prm_.epsilon = e; }
@@ -76,7 +79,12 @@ class GpuHMatrix {
         prm_.lowprec_tol = tol;
         prm_.lowprec_all = all_terms ? 1 : 0;
     }
-    void set_storage(htg_storage s) { check(htg_hmatrix_set_storage(handle(), s)); }
+    void set_storage(htg_storage s) {
+        check(htg_hmatrix_set_storage(handle(), s));
+        storage_ = s;
+        if (ht_)
+            check(htg_hmatrix_set_storage(ht_, s));
+    }
     // Presizes ACA workspaces from a trained rank predictor (not owned; nullptr: none).
     void set_predictor(const htg_predictor *p, double margin = 0.25) {
         prm_.predictor        = p;
@@ -107,6 +115,16 @@ class GpuHMatrix {
     // Device-resident variant (HIP device pointers, a hipStream_t or nullptr).
     void mvprod_device(const double *d_in, double *d_out, int mu, void *stream = nullptr) const {
         check(htg_hmatrix_mvprod_device(handle(), d_in, d_out, mu, stream));
+    }
+    // mvprod_transp_global_to_global(in, out, mu), virtual_hmatrix.hpp:58: the
+    // transposed H-matrix is built on first use (same storage mode).
+    void mvprod_transp_global_to_global(const double *in, double *out, const int &mu = 1) const {
+        if (!ht_) {
+            check(htg_hmatrix_transpose(handle(), &ht_));
+            if (storage_ != HTG_STORAGE_FP64)
+                check(htg_hmatrix_set_storage(ht_, storage_));
+        }
+        check(htg_hmatrix_mvprod(ht_, in, out, mu));
     }
     // operator*, hmatrix.hpp:1596-1602.
     std::vector<double> operator*(const std::vector<double> &x) const {
@@ -149,11 +167,16 @@ class GpuHMatrix {
         return h_;
     }
     void reset() {
+        htg_hmatrix_destroy(ht_);
         htg_hmatrix_destroy(h_);
-        h_ = nullptr;
+        h_       = nullptr;
+        ht_      = nullptr;
+        storage_ = HTG_STORAGE_FP64;
     }
     htg_params prm_{};
-    htg_hmatrix *h_ = nullptr;
+    htg_hmatrix *h_          = nullptr;
+    mutable htg_hmatrix *ht_ = nullptr; // transpose, built on first use
+    htg_storage storage_     = HTG_STORAGE_FP64;
 };
 
 } // namespace htool_amd

@@ -1,0 +1,116 @@
+"""GPU: the transposed product y = A^T x (HMatrix::mvprod_transp_global_to_global,
+include/htool/types/virtual_hmatrix.hpp:58).
+
+The transposed H-matrix must equal the oracle's product over the transposed
+leaves (dense blocks transposed, U and V exchanged; 1e-12 in FP64), and the
+brute-force product with the transposed kernel matrix within the ACA tolerance.
+"""
+import numpy as np
+import pytest
+
+from htool_amd import geometry as geo
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def transposed_leaves(orc, leaves, data):
+    lt, dt = [], []
+    for oi, m, oj, n, k, arr in orc.iter_leaf_data(leaves, data):
+        lt.append((oj, n, oi, m, k))
+        if k < 0:
+            dt.append(np.asarray(arr).T.ravel(order="F"))
+        elif k > 0:
+            u, v = arr
+            dt.append(np.asarray(v).T.ravel(order="F"))  # U' = V^T, n x k
+            dt.append(np.asarray(u).T.ravel(order="F"))  # V' = U^T, k x m
+    lt = np.array(lt, np.int32)
+    order = np.lexsort((lt[:, 2], lt[:, 0]))
+    # keep the data in the order of the sorted leaf list
+    chunks, pos = [], 0
+    sizes = [(n * m if k < 0 else k * (m + n)) for (_, m, _, n, k) in lt]
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    flat = np.concatenate(dt) if dt else np.zeros(0)
+    for i in order:
+        chunks.append(flat[starts[i]:starts[i + 1]])
+    return lt[order], np.concatenate(chunks)
+
+
+def test_transpose_two_disks_matches_oracle(htg, orc):
+    t = geo.disk(1800, 0.0, seed=81)
+    s = geo.disk(1100, 1.0, seed=82)
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=10)
+    H.set_compression("partialACA")
+    H.build(htg.Generator(geo.KERNEL_LAPLACE), t, s)
+    T = H.transpose()
+    assert T.nb_rows() == 1100 and T.nb_cols() == 1800
+    leaves, data = H.get_leaves(with_data=True)
+    pt, ps = H.get_perms()
+    lt, dt = transposed_leaves(orc, leaves, data)
+    tl, td = T.get_leaves(with_data=True)
+    assert np.array_equal(tl, lt) and np.array_equal(td, dt)
+    x = np.random.default_rng(1).standard_normal((1800, 3))
+    y = T.mvprod_global_to_global(x)
+    assert rel(y, orc.leaf_mvprod(1100, 1800, ps, pt, lt, dt, x)) < 1e-12
+    rows = np.arange(0, 1100, 7, dtype=np.int32)
+    yd = orc.dense_rows(s, t, rows, x[:, 0], kernel=geo.KERNEL_LAPLACE)
+    assert rel(y[rows, 0], yd) < 1e-5
+    assert rel(H.mvprod_transp_global_to_global(x), y) == 0.0
+
+
+def test_transpose_mixed_storage_and_square(htg, orc):
+    n = 6000
+    x = geo.sphere(n, seed=83)
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32)
+    H.set_compression("sympartialACA")
+    H.set_recompression(2e-6)
+    H.set_lowprec(5e-7)
+    H.build(htg.Generator(geo.KERNEL_LAPLACE, 0.0, geo.laplace_diag(geo.sphere_spacing(n))), x)
+    v = np.random.default_rng(2).standard_normal((n, 2))
+    y64 = H.mvprod_transp_global_to_global(v)
+    rows = np.random.default_rng(3).choice(n, 300, replace=False).astype(np.int32)
+    yd = orc.dense_rows(x, None, rows, v[:, 0], kernel=geo.KERNEL_LAPLACE,
+                        diag=geo.laplace_diag(geo.sphere_spacing(n)), same=True)
+    assert rel(y64[rows, 0], yd) < 1e-5
+    H.set_storage("fp32_fp16")
+    assert rel(H.mvprod_transp_global_to_global(v), y64) < 1e-6
+
+
+def test_transpose_helmholtz_is_not_conjugated(htg, orc):
+    n = 3000
+    x = geo.sphere(n, seed=84)
+    h = geo.sphere_spacing(n)
+    k = 2 * np.pi / (10 * h)
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=10)
+    H.set_compression("partialACA")
+    H.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, geo.laplace_diag(h), k), x)
+    leaves, data = H.get_leaves(with_data=True)
+    pt, ps = H.get_perms()
+    lt, dt = transposed_leaves(orc, leaves, data)
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    y = H.mvprod_transp_global_to_global(v)
+    assert rel(y, orc.leaf_mvprod(n, n, ps, pt, lt, dt, v)) < 1e-12
+    # the kernel is symmetric (not Hermitian): A^T x = A x up to the ACA tolerance
+    assert rel(y, H.mvprod_global_to_global(v)) < 1e-5
+
+
+def test_transpose_of_a_batch(htg):
+    pts = [geo.ellipsoid(n, (1.0, 0.8, 1.5), seed=90 + n) for n in (2500, 4000, 3100)]
+    gens = [htg.Generator(geo.KERNEL_INV_DELTA_R, 1e-5) for _ in pts]
+    B = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=16)
+    B.set_compression("partialACA")
+    B.build_batch(gens, pts)
+    x = np.random.default_rng(5).standard_normal(B.nb_cols())
+    yT = B.mvprod_transp_global_to_global(x)
+    off = 0
+    for p, g in zip(pts, gens):
+        H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=16)
+        H.set_compression("partialACA")
+        H.build(g, p)
+        n = p.shape[0]
+        assert rel(yT[off:off + n], H.mvprod_transp_global_to_global(x[off:off + n])) < 1e-13
+        off += n
