@@ -382,8 +382,8 @@ int row_stride(const htg_hmatrix &h, int c, int rb) {
     return round_up(h.cls[c].rb_col0[rb + 1] - h.cls[c].rb_col0[rb], CLASS_ALIGN[c]);
 }
 
-// Right-hand sides per pass: up to 64, or 32 for FP32 arithmetic, whose
-// 32-wide passes run on the matrix cores.
+// Right-hand sides per pass: at most 32, the width of the matrix-core passes
+// (FP32 and FP64 arithmetic).
 int pick_mu_chunk(int remaining, int cap) {
     int c = cap;
     while (c > remaining)
@@ -411,7 +411,7 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
     const RowClass rc1{h.cls[1].d_rb_col0.p, h.cls[1].d_rb_aoff.p, h.cls[1].d_colidx.p, CLASS_ALIGN[1]};
     int done = 0;
     while (done < mu) {
-        const int MU = pick_mu_chunk(mu - done, std::is_same<TC, float>::value ? 32 : 64);
+        const int MU = pick_mu_chunk(mu - done, 32);
         if (ev)
             HTG_HIP(hipEventRecord(ev[0], st));
         if (std::is_same<TC, float>::value && MU == 32) {

@@ -1540,6 +1540,165 @@ __global__ void __launch_bounds__(WG) k_vx_mfma(const int *wv_off, int nwaves, c
 }
 
 // ---------------------------------------------------------------------------
+// FP64 32-RHS products on the matrix cores (v_mfma_f64_16x16x4f64): 16-row
+// tiles; lane l = (g, r) = (l / 16, l % 16) takes row r of the tile and, in
+// MFMA m of step it, column 8 it + 2 g + m (its 16-byte load holds both m);
+// two MFMAs per A value, right-hand sides 0-15 and 16-31. Result register i
+// of lane (g, r) holds row 4 i + g, right-hand side r (the FP32 16x16x4 form
+// holds row 4 g + i). z stays RHS-interleaved (z[j * 32 + s]); sums are FP64
+// throughout.
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+constexpr int RPASS64 = 64; // row-block columns staged per pass
+
+__global__ void __launch_bounds__(WG) k_rows_mfma64(const int *rb_order, const int *rb_row0, RowClass c0,
+                                                    const double *arena0, RowClass c1, const double *arena1,
+                                                    const double *z, const int *perm_t, int nr, int mu_base,
+                                                    double *y) {
+    constexpr int MU  = 32;
+    constexpr int NIT = RPASS64 / 8;
+    constexpr int ZPT = RPASS64 * MU / WG;
+    __shared__ double s_z[RPASS64 * MU];
+    const int rb = rb_order[blockIdx.x];
+    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int T16 = (R + 15) / 16; // <= 8 tiles; wave w takes tiles w and w + 4
+    doublex4 acc[2][2] = {};
+    const int zs = threadIdx.x & (MU - 1), zc = threadIdx.x / MU;
+    for (int cls = 0; cls < 2; cls++) {
+        const RowClass &rc = cls == 0 ? c0 : c1;
+        const double *arena = cls == 0 ? arena0 : arena1;
+        if (!arena)
+            continue;
+        const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
+        const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+        const double *A = arena + rc.rb_aoff[rb];
+        for (int cc = 0; cc < Cp; cc += RPASS64) {
+            const int nz = min(RPASS64, Cp - cc);
+            {
+                int idx[ZPT];
+#pragma unroll
+                for (int i = 0; i < ZPT; i++) {
+                    const int c = cc + zc + (WG / MU) * i;
+                    idx[i]      = rc.colidx[cb + (c < C ? c : 0)];
+                }
+#pragma unroll
+                for (int i = 0; i < ZPT; i++) {
+                    const int c           = zc + (WG / MU) * i;
+                    const double v        = z[(size_t)idx[i] * MU + zs];
+                    s_z[c * MU + zs]      = cc + c < C ? v : 0.0;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++) {
+                const int t = w + 4 * tt;
+                if (t < T16) {
+                    const int row   = 16 * t + r16;
+                    const double *rp = A + (size_t)(row < R ? row : 0) * Cp + cc;
+                    double a[NIT][2];
+#pragma unroll
+                    for (int it = 0; it < NIT; it++) {
+                        const int kk = 8 * it + 2 * g;
+                        Vec<double>::load(rp + (kk < nz ? kk : 0), a[it]);
+                    }
+#pragma unroll
+                    for (int it = 0; it < NIT; it++) {
+                        if (8 * it < nz) {
+                            const int kk = 8 * it + 2 * g;
+#pragma unroll
+                            for (int m = 0; m < 2; m++) {
+                                const double av = kk < nz ? a[it][m] : 0.0;
+                                const double *bz = s_z + (kk + m) * MU;
+                                acc[tt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bz[r16], acc[tt][0], 0, 0, 0);
+                                acc[tt][1] =
+                                    __builtin_amdgcn_mfma_f64_16x16x4f64(av, bz[r16 + 16], acc[tt][1], 0, 0, 0);
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int tt = 0; tt < 2; tt++) {
+        const int t = w + 4 * tt;
+        if (t < T16) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int orow = 16 * t + 4 * i + g;
+                if (orow < R) {
+                    double *o = y + (size_t)perm_t[r0 + orow] + (size_t)nr * mu_base;
+                    o[(size_t)nr * r16]        = acc[tt][0][i];
+                    o[(size_t)nr * (r16 + 16)] = acc[tt][1][i];
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void vx_store64(const VTile &d, int r, int s, double v, double *z, double *part,
+                                           const double *tscale) {
+    if (d.nparts > 1)
+        part[(size_t)(d.part_off + r) * 32 + s] = v;
+    else
+        z[(size_t)(d.t_index + r) * 32 + s] = tscale ? v * tscale[d.t_index + r] : v;
+}
+
+__global__ void __launch_bounds__(WG) k_vx_mfma64(const int *wv_off, int nwaves, const VTile *tiles,
+                                                  const double *varena0, const double *varena1, double *z,
+                                                  double *part, const double *tscale) {
+    constexpr int MU  = 32;
+    constexpr int NIT = 8; // 64 columns per pass
+    const int wv      = blockIdx.x * NWAVE + (threadIdx.x >> 6);
+    if (wv >= nwaves)
+        return;
+    const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
+    for (int ti = wv_off[wv]; ti < wv_off[wv + 1]; ti++) {
+        const VTile d       = tiles[ti];
+        const double *vare = d.cls == 0 ? varena0 : varena1;
+        const double *xv    = z + (size_t)d.x_off * MU;
+        for (int rt = 0; rt < d.k; rt += 16) {
+            const int row    = rt + r16;
+            const double *rp = vare + d.v_off + (size_t)(row < d.k ? row : 0) * d.ld;
+            doublex4 acc0 = {0}, acc1 = {0};
+            for (int c0 = 0; c0 < d.ncols; c0 += 8 * NIT) {
+                const int nc = min(8 * NIT, d.ncols - c0);
+                double a[NIT][2];
+#pragma unroll
+                for (int it = 0; it < NIT; it++) {
+                    const int kk = 8 * it + 2 * g;
+                    Vec<double>::load(rp + c0 + (kk < nc ? kk : 0), a[it]);
+                }
+#pragma unroll
+                for (int it = 0; it < NIT; it++) {
+                    if (8 * it < nc) {
+                        const int kk = 8 * it + 2 * g;
+#pragma unroll
+                        for (int m = 0; m < 2; m++) {
+                            const bool in    = kk < nc;
+                            const double *bz = xv + (size_t)(c0 + (in ? kk : 0) + m) * MU;
+                            const double av  = in ? a[it][m] : 0.0;
+                            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bz[r16], acc0, 0, 0, 0);
+                            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bz[r16 + 16], acc1, 0, 0, 0);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = rt + 4 * i + g;
+                if (r < d.k) {
+                    vx_store64(d, r, r16, acc0[i], z, part, tscale);
+                    vx_store64(d, r, r16 + 16, acc1[i], z, part, tscale);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Complex 16-RHS products on the matrix cores. The 32 B columns of a tile are
 // the 16 right-hand sides' real parts (slots 0-15) then imaginary parts (slots
 // 16-31), in the blocked z layout (zblk(j, slot)). Each stored complex entry
@@ -2250,6 +2409,17 @@ void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *mu
                hipStream_t st) {
     if (ngroups <= 0)
         return;
+    if constexpr (std::is_same<TC, double>::value && std::is_same<TS0, double>::value &&
+                  std::is_same<TS1, double>::value) {
+        if (MU == MFMA_MU) {
+            hipLaunchKernelGGL(k_vx_mfma64, dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, ngroups,
+                               tiles, varena0, varena1, z, part, tscale);
+            if (nmulti > 0)
+                hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first,
+                                   nmulti, part, z, tscale);
+            return;
+        }
+    }
     if constexpr (std::is_same<TC, float>::value) {
         if (MU == MFMA_MU) {
             hipLaunchKernelGGL((k_vx_mfma<TS0, TS1>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off,
@@ -2279,6 +2449,14 @@ void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, 
                  hipStream_t st) {
     if (nrb <= 0)
         return;
+    if constexpr (std::is_same<TC, double>::value && std::is_same<TS0, double>::value &&
+                  std::is_same<TS1, double>::value) {
+        if (MU == MFMA_MU) {
+            hipLaunchKernelGGL(k_rows_mfma64, dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0, c1, arena1,
+                               z, perm_t, nr, mu_base, y);
+            return;
+        }
+    }
     if constexpr (std::is_same<TC, float>::value) {
         if (MU == MFMA_MU) {
             hipLaunchKernelGGL((k_rows_mfma<TS0, TS1>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0,

@@ -93,7 +93,7 @@ def test_from_leaves_same_leaves(htg, orc):
     xt = geo.cylinder(4000, seed=1)
     O = orc.hmat_build(xt, kernel=geo.KERNEL_INV_DELTA_R, delta=1e-5, minclustersize=10, eta=10.0, eps=1e-6)
     H = htg.HMatrix.from_leaves(O.nr, O.nc, O.perm_t, O.perm_s, O.leaves, O.data)
-    for mu in (1, 5):
+    for mu in (1, 5, 32):
         x = np.random.default_rng(mu).uniform(-1, 1, (O.nc, mu))
         assert rel(H.mvprod_global_to_global(x), O.mvprod(x)) < 1e-12
     leaves, data = H.get_leaves(with_data=True)
