@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 RECOMPRESS_TOL = 2e-6  # per-leaf relative Frobenius tail dropped by the FP64 recompression
-LOWPREC_TOL = 5e-7     # per-leaf relative error budget of the FP16-stored trailing terms
+LOWPREC_TOL = 4e-6     # per-leaf relative error budget of the FP16-stored trailing terms
 
 
 def log(msg):
@@ -334,6 +334,7 @@ def main():
         dy = torch.zeros(n, dtype=torch.float64, device="cuda")
         for M, storage in ((Hm, "fp32_fp16"), (H, "fp32"), (H, "fp64")):
             M.set_storage(storage)
+            M.autotune(mu=1, reps=5)
             for _ in range(3):
                 M.mvprod_device(dx.data_ptr(), dy.data_ptr(), 1, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
@@ -351,6 +352,8 @@ def main():
     ys = {}
     for name, M, storage in (("fp64", H, "fp64"), ("fp32", H, "fp32"), ("mixed", Hm, "fp32_fp16")):
         M.set_storage(storage)
+        # work decomposition picked by the library's autotuner for this storage
+        layout, _, _ = M.autotune(mu=1, reps=20)
         ms1 = timed(lambda: M.mvprod_device(dx.data_ptr(), dy.data_ptr(), 1, sp), args.steps, args.warmup, stream, torch)
         ys[name] = dy.cpu().numpy().copy()
         ms32 = timed(lambda: M.mvprod_device(dX.data_ptr(), dY.data_ptr(), 32, sp), max(5, args.steps // 5),
@@ -359,7 +362,7 @@ def main():
         prof32 = M.profile(dX.data_ptr(), dY.data_ptr(), 32, 5, sp)
         log(f"{name}: {ms1:.4f} ms/matvec, {ms32:.4f} ms/32-RHS product "
             f"(V.x {prof32['vx_ms']:.4f} ms, rows {prof32['row_apply_ms']:.4f} ms)")
-        res[name] = {"ms": ms1, "ms_mu32": ms32, "bytes": M.algorithmic_bytes(1, storage),
+        res[name] = {"layout": layout, "ms": ms1, "ms_mu32": ms32, "bytes": M.algorithmic_bytes(1, storage),
                      "bytes_mu32": M.algorithmic_bytes(32, storage), "flops_mu32": M.flops(32), "prof": prof,
                      "prof_mu32": prof32}
 
@@ -400,7 +403,8 @@ def main():
 
     def entry(name):
         r = res[name]
-        return {"ms": round(r["ms"], 4), "gbps": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 2),
+        return {"layout": r["layout"], "ms": round(r["ms"], 4),
+                "gbps": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 2),
                 "bytes": r["bytes"], "ms_mu32": round(r["ms_mu32"], 4),
                 "tflops_mu32": round(r["flops_mu32"] / (r["ms_mu32"] * 1e-3) / 1e12, 3),
                 "kernels_ms_mu32": {k: round(v, 5) for k, v in r["prof_mu32"].items()},

@@ -20,7 +20,7 @@ constexpr int ALIGN        = 32;   // block alignment in elements
 // Default work decomposition (htg_layout): rows per row block, target entries
 // per V chunk and per phase-1 wavefront, most chunks per wavefront, stacked V
 // rows per chunk (one matrix-core tile).
-constexpr htg_layout DEFAULT_LAYOUT = {128, 8192, 2048, 8, 32};
+constexpr htg_layout DEFAULT_LAYOUT = {128, 8192, 2048, 4, 16};
 constexpr int CLASS_ALIGN[2] = {4, 8}; // entries per aligned group: class 1 may hold 16-bit values
 constexpr int CAP_MAX        = 512;    // must match the ACA kernel's CAPMAX
 
@@ -91,9 +91,11 @@ struct htg_hmatrix {
     ClassLayout cls[2];
     // V chunks of both classes, grouped per wavefront
     std::vector<VTile> vtiles;
-    std::vector<int> vg_off, multi_first;
+    std::vector<int> vg_off;
+    std::vector<VTile> multi_tiles; // first chunk of every row group cut into several chunks
     DevArray<VTile> d_vtiles;
-    DevArray<int> d_vg_off, d_multi_first;
+    DevArray<int> d_vg_off;
+    DevArray<VTile> d_multi_tiles;
 
     DevArray<int> d_perm_t, d_perm_s, d_rb_row0, d_rb_order;
     DevArray<double> d_tscale;
@@ -275,7 +277,7 @@ void build_layout(htg_hmatrix &h) {
     // columns into chunks of about vchunk_el entries.
     h.part_size = 0;
     h.vtiles.clear();
-    h.multi_first.clear();
+    h.multi_tiles.clear();
     h.vg_off.assign(1, 0);
     int64_t group_el = 0;
     for (int c = 0; c < 2; c++) {
@@ -327,7 +329,7 @@ void build_layout(htg_hmatrix &h) {
                     v.nparts     = nparts;
                     v.cls        = c;
                     if (nparts > 1 && q == 0)
-                        h.multi_first.push_back((int)h.vtiles.size());
+                        h.multi_tiles.push_back(v);
                     const int64_t el   = (int64_t)v.k * v.ncols;
                     const int in_group = (int)h.vtiles.size() - h.vg_off.back();
                     if (in_group > 0 && (group_el + el > h.lay.vgroup_el || in_group >= h.lay.vgroup_tiles)) {
@@ -357,7 +359,7 @@ void upload_layout(htg_hmatrix &h) {
     }
     h.d_vtiles.upload(h.vtiles);
     h.d_vg_off.upload(h.vg_off);
-    h.d_multi_first.upload(h.multi_first);
+    h.d_multi_tiles.upload(h.multi_tiles);
     if (h.scaled)
         h.d_tscale.upload(h.tscale);
     // z = [x (nc entries, zero padded to tpad) | t], per RHS; zeroed once so the
@@ -429,8 +431,8 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
         if (ev)
             HTG_HIP(hipEventRecord(ev[1], st));
         const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
-        launch_vx<TS0, TS1, TC>(h.d_vg_off.p, (int)h.vg_off.size() - 1, h.d_vtiles.p, h.d_multi_first.p,
-                                (int)h.multi_first.size(), arena_of<TS0>(C0, true),
+        launch_vx<TS0, TS1, TC>(h.d_vg_off.p, (int)h.vg_off.size() - 1, h.d_vtiles.p, h.d_multi_tiles.p,
+                                (int)h.multi_tiles.size(), arena_of<TS0>(C0, true),
                                 has1 ? arena_of<TS1>(C1, true) : nullptr, z, h.d_part.p, tscale, MU, st);
         if (ev)
             HTG_HIP(hipEventRecord(ev[2], st));
@@ -461,8 +463,8 @@ void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream
         if (ev)
             HTG_HIP(hipEventRecord(ev[1], st));
         const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
-        launch_vx_c<TS0, TS1, R>(h.d_vg_off.p, (int)h.vg_off.size() - 1, h.d_vtiles.p, h.d_multi_first.p,
-                                 (int)h.multi_first.size(), arena_of<TS0>(C0, true),
+        launch_vx_c<TS0, TS1, R>(h.d_vg_off.p, (int)h.vg_off.size() - 1, h.d_vtiles.p, h.d_multi_tiles.p,
+                                 (int)h.multi_tiles.size(), arena_of<TS0>(C0, true),
                                  has1 ? arena_of<TS1>(C1, true) : nullptr, z, h.d_part.p, tscale, MU, st);
         if (ev)
             HTG_HIP(hipEventRecord(ev[2], st));
@@ -988,10 +990,14 @@ int htg_hmatrix_autotune(htg_hmatrix *h, int mu, int reps, htg_layout *best, dou
         htg_layout top         = start;
         double top_ms          = 1e300;
         int tried              = 0;
+        // the current layout, then rows per row block x (V chunk, wavefront
+        // share) x stacked V rows
+        std::vector<htg_layout> cand{start};
         for (int rb : {64, 128})
-            for (int chunk : {4096, 8192, 16384})
-                for (int vrows : {16, 32}) {
-                    htg_layout L{rb, chunk, chunk / 4, start.vgroup_tiles, vrows};
+            for (auto cg : {std::pair<int, int>{8192, 2048}, {8192, 4096}, {16384, 4096}, {16384, 16384}})
+                for (int vrows : {8, 16, 32})
+                    cand.push_back(htg_layout{rb, cg.first, cg.second, 4, vrows});
+        for (const htg_layout &L : cand) {
                     h->lay = L;
                     relayout(*h);
                     run_product(*h, dx.p, dy.p, mu, nullptr); // warm-up

@@ -985,6 +985,82 @@ __device__ __forceinline__ void vx_store(const VTile &d, int r, const double (&v
     }
 }
 
+// One V chunk, one right-hand side. Rows go in blocks of RB (8 to 32): lane
+// group g of LPR lanes (LPR = 64 / RPW) takes rows g, g + RPW, ... (NI = RB /
+// RPW <= 8 rows, which bounds the registers) and lane l of the group the vectors l,
+// l + LPR, ... of every row, so the lane loads its x entries once per block
+// and reuses them for all NI rows. The NI partial sums per lane are reduced by
+// recursive halving: at each step a lane keeps half of its rows and adds the
+// partner lane's copy of that half (NI - 1 shuffles), then SH = LPR / NI lanes
+// hold partial sums of the same row and finish with log2(SH) shuffles; lane l
+// ends with row g + RPW (l / SH).
+template <typename TS, typename TC, int LPR>
+__device__ __forceinline__ void vx_chunk_rows(const VTile &d, const TS *varena, TC *z, double *part,
+                                              const double *tscale, int lane) {
+    constexpr int W   = Vec<TS>::W;
+    constexpr int RPW = 64 / LPR;
+    constexpr int NI  = LPR == 64 ? 8 : (RPW >= 16 ? 1 : 16 / RPW); // at most 8 rows in flight per lane
+    constexpr int RB  = NI * RPW;
+    constexpr int SH  = LPR / NI;
+    const int sl = lane & (LPR - 1), sub = lane / LPR;
+    const int nvec = d.ncols / W;
+    const TC *xv   = z + d.x_off;
+    const TS *vb   = varena + d.v_off;
+#pragma unroll 1
+    for (int rb0 = 0; rb0 < d.k; rb0 += RB) {
+        const int rfin  = rb0 + sub + RPW * (sl / SH);
+        const bool wr   = sl % SH == 0 && rfin < d.k;
+        const double sc = (tscale && d.nparts == 1 && wr) ? tscale[d.t_index + rfin] : 1.0;
+        TC acc[NI];
+#pragma unroll
+        for (int i = 0; i < NI; i++)
+            acc[i] = 0;
+#pragma unroll 1
+        for (int q0 = 0; q0 < nvec; q0 += LPR) {
+            const int q    = q0 + sl;
+            const bool okq = q < nvec;
+            TC xq[W];
+            load_n<TC, W>(xv + (size_t)(okq ? q : 0) * W, xq);
+#pragma unroll
+            for (int u = 0; u < W; u++)
+                xq[u] = okq ? xq[u] : (TC)0;
+            TS a[NI][W];
+#pragma unroll
+            for (int i = 0; i < NI; i++) {
+                const int r = rb0 + sub + RPW * i;
+                Vec<TS>::load(vb + (size_t)(r < d.k ? r : 0) * d.ld + (size_t)(okq ? q : 0) * W, a[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < NI; i++) {
+                const bool ok = rb0 + sub + RPW * i < d.k;
+#pragma unroll
+                for (int u = 0; u < W; u++)
+                    acc[i] += ok ? cvt<TC>(a[i][u]) * xq[u] : (TC)0;
+            }
+        }
+#pragma unroll
+        for (int h = NI / 2, o = LPR / 2; h >= 1; h >>= 1, o >>= 1) {
+            const bool up = (sl & o) != 0;
+#pragma unroll
+            for (int j = 0; j < h; j++) {
+                const TC send = up ? acc[j] : acc[j + h];
+                const TC keep = up ? acc[j + h] : acc[j];
+                acc[j]        = keep + __shfl_xor(send, o, 64);
+            }
+        }
+#pragma unroll
+        for (int o = SH / 2; o >= 1; o >>= 1)
+            acc[0] += __shfl_xor(acc[0], o, 64);
+        if (wr) {
+            const double v = (double)acc[0] * sc;
+            if (d.nparts > 1)
+                part[(size_t)(d.part_off + rfin)] = v;
+            else
+                z[(size_t)(d.t_index + rfin)] = (TC)v;
+        }
+    }
+}
+
 // One V chunk: rows taken by lane groups (lanes_per_row), FP64 shuffle reduction.
 template <typename TS, typename TC, int MU>
 __device__ __forceinline__ void vx_chunk(const VTile &d, const TS *varena, TC *z, double *part, const double *tscale,
@@ -995,7 +1071,22 @@ __device__ __forceinline__ void vx_chunk(const VTile &d, const TS *varena, TC *z
     const int rpw   = 64 / lpr;
     const int sl = lane & (lpr - 1), sub = lane / lpr;
     const TC *xv = z + (size_t)d.x_off * MU;
-    if constexpr (MU == 1) {
+    if constexpr (MU == 1 && std::is_same<TS, double>::value) {
+        // FP64 rows: x held in registers across up to 8 rows (faster here;
+        // for 32- and 16-bit rows the two-row form below is faster)
+        if (nvec > 32)
+            vx_chunk_rows<TS, TC, 64>(d, varena, z, part, tscale, lane);
+        else if (nvec > 16)
+            vx_chunk_rows<TS, TC, 32>(d, varena, z, part, tscale, lane);
+        else if (nvec > 8)
+            vx_chunk_rows<TS, TC, 16>(d, varena, z, part, tscale, lane);
+        else if (nvec > 4)
+            vx_chunk_rows<TS, TC, 8>(d, varena, z, part, tscale, lane);
+        else if (nvec > 2)
+            vx_chunk_rows<TS, TC, 4>(d, varena, z, part, tscale, lane);
+        else
+            vx_chunk_rows<TS, TC, 2>(d, varena, z, part, tscale, lane);
+    } else if constexpr (MU == 1) {
         // two rows per lane group: one x load serves both
         for (int rbase = 0; rbase < d.k; rbase += 2 * rpw) {
             const int r0 = rbase + sub, r1 = r0 + rpw;
@@ -1100,14 +1191,14 @@ constexpr int vx_sum_groups_per_block() {
 }
 
 template <typename TC, int MU, bool BLK = false>
-__global__ void __launch_bounds__(WG) k_vx_sum(const VTile *tiles, const int *first_chunk, int nmulti,
+__global__ void __launch_bounds__(WG) k_vx_sum(const VTile *first, int nmulti,
                                                const double *part, TC *z, const double *tscale) {
     constexpr int GPB = vx_sum_groups_per_block<MU>();
     constexpr int TPG = WG / GPB; // threads per group
     const int gi      = blockIdx.x * GPB + (int)threadIdx.x / TPG;
     if (gi >= nmulti)
         return;
-    const VTile d    = tiles[first_chunk[gi]];
+    const VTile d    = first[gi];
     const int km     = d.k * MU;
     const double *pb = part + (size_t)d.part_base * MU;
     for (int e = threadIdx.x % TPG; e < km; e += TPG) {
@@ -2197,14 +2288,14 @@ __global__ void __launch_bounds__(WG) k_vx_c(const int *wv_off, int nwaves, cons
 }
 
 template <typename R, int MU>
-__global__ void __launch_bounds__(WG) k_vx_sum_c(const VTile *tiles, const int *first_chunk, int nmulti,
+__global__ void __launch_bounds__(WG) k_vx_sum_c(const VTile *first, int nmulti,
                                                  const double *part, R *z, const double *tscale) {
     constexpr int GPB = vx_sum_groups_per_block<MU>();
     constexpr int TPG = WG / GPB;
     const int gi      = blockIdx.x * GPB + (int)threadIdx.x / TPG;
     if (gi >= nmulti)
         return;
-    const VTile d    = tiles[first_chunk[gi]];
+    const VTile d    = first[gi];
     const int km     = d.k * MU;
     const double *pb = part + 2 * (size_t)d.part_base * MU;
     for (int e = threadIdx.x % TPG; e < km; e += TPG) {
@@ -2404,7 +2495,7 @@ void launch_gather_x32(const double *x, const int *iperm_s, int nc, int tpad, in
 }
 
 template <typename TS0, typename TS1, typename TC>
-void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *multi_first, int nmulti,
+void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const VTile *multi_tiles, int nmulti,
                const TS0 *varena0, const TS1 *varena1, TC *z, double *part, const double *tscale, int MU,
                hipStream_t st) {
     if (ngroups <= 0)
@@ -2415,7 +2506,7 @@ void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *mu
             hipLaunchKernelGGL(k_vx_mfma64, dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, ngroups,
                                tiles, varena0, varena1, z, part, tscale);
             if (nmulti > 0)
-                hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first,
+                hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU>), dim3(nmulti), dim3(WG), 0, st, multi_tiles,
                                    nmulti, part, z, tscale);
             return;
         }
@@ -2425,7 +2516,7 @@ void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *mu
             hipLaunchKernelGGL((k_vx_mfma<TS0, TS1>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off,
                                ngroups, tiles, varena0, varena1, z, part, tscale);
             if (nmulti > 0)
-                hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU, true>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first,
+                hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU, true>), dim3(nmulti), dim3(WG), 0, st, multi_tiles,
                                    nmulti, part, z, tscale);
             return;
         }
@@ -2437,7 +2528,7 @@ void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *mu
         if (nmulti > 0)                                                                                    \
             hipLaunchKernelGGL((k_vx_sum<TC, M>), dim3((nmulti + vx_sum_groups_per_block<M>() - 1) /         \
                                                      vx_sum_groups_per_block<M>()),                         \
-                               dim3(WG), 0, st, tiles, multi_first, nmulti, part, z, tscale);              \
+                               dim3(WG), 0, st, multi_tiles, nmulti, part, z, tscale);              \
     } while (0)
     HTG_MU_SWITCH(MU, HTG_VX)
 #undef HTG_VX
@@ -2496,7 +2587,7 @@ void launch_gather_xc(const double *x, const int *perm_s, int nc, int tpad, int 
 }
 
 template <typename TS0, typename TS1, typename R>
-void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const int *multi_first, int nmulti,
+void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const VTile *multi_tiles, int nmulti,
                  const TS0 *varena0, const TS1 *varena1, R *z, double *part, const double *tscale, int MU,
                  hipStream_t st) {
     if (ngroups <= 0)
@@ -2506,7 +2597,7 @@ void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const int *
             hipLaunchKernelGGL((k_vx_mfma_c<TS0, TS1>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off,
                                ngroups, tiles, varena0, varena1, z, part, tscale);
             if (nmulti > 0) // slots sum independently: the real 32-slot reduction applies
-                hipLaunchKernelGGL((k_vx_sum<float, 32, true>), dim3(nmulti), dim3(WG), 0, st, tiles, multi_first,
+                hipLaunchKernelGGL((k_vx_sum<float, 32, true>), dim3(nmulti), dim3(WG), 0, st, multi_tiles,
                                    nmulti, part, z, tscale);
             return;
         }
@@ -2518,7 +2609,7 @@ void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const int *
         if (nmulti > 0)                                                                                             \
             hipLaunchKernelGGL((k_vx_sum_c<R, M>), dim3((nmulti + vx_sum_groups_per_block<M>() - 1) /            \
                                                        vx_sum_groups_per_block<M>()),                            \
-                               dim3(WG), 0, st, tiles, multi_first, nmulti, part, z, tscale);                       \
+                               dim3(WG), 0, st, multi_tiles, nmulti, part, z, tscale);                       \
     } while (0)
     HTG_MU_SWITCH_C(MU, HTG_VXC)
 #undef HTG_VXC
@@ -2547,7 +2638,7 @@ void launch_rows_c(const int *rb_order, const int *rb_row0, int nrb, RowClass c0
 template void launch_gather_xc<double>(const double *, const int *, int, int, int, int, double *, hipStream_t);
 template void launch_gather_xc<float>(const double *, const int *, int, int, int, int, float *, hipStream_t);
 #define HTG_INST_C(TS0, TS1, R)                                                                                       \
-    template void launch_vx_c<TS0, TS1, R>(const int *, int, const VTile *, const int *, int, const TS0 *,          \
+    template void launch_vx_c<TS0, TS1, R>(const int *, int, const VTile *, const VTile *, int, const TS0 *,          \
                                            const TS1 *, R *, double *, const double *, int, hipStream_t);           \
     template void launch_rows_c<TS0, TS1, R>(const int *, const int *, int, RowClass, const TS0 *, RowClass,        \
                                              const TS1 *, const R *, const int *, int, int, int, double *, hipStream_t);
@@ -2559,13 +2650,13 @@ HTG_INST_C(float, __hip_bfloat16, float)
 
 template void launch_gather_x<double>(const double *, const int *, int, int, int, int, double *, hipStream_t);
 template void launch_gather_x<float>(const double *, const int *, int, int, int, int, float *, hipStream_t);
-template void launch_vx<double, double, double>(const int *, int, const VTile *, const int *, int, const double *,
+template void launch_vx<double, double, double>(const int *, int, const VTile *, const VTile *, int, const double *,
                                                 const double *, double *, double *, const double *, int, hipStream_t);
-template void launch_vx<float, float, float>(const int *, int, const VTile *, const int *, int, const float *,
+template void launch_vx<float, float, float>(const int *, int, const VTile *, const VTile *, int, const float *,
                                              const float *, float *, double *, const double *, int, hipStream_t);
-template void launch_vx<float, __half, float>(const int *, int, const VTile *, const int *, int, const float *,
+template void launch_vx<float, __half, float>(const int *, int, const VTile *, const VTile *, int, const float *,
                                               const __half *, float *, double *, const double *, int, hipStream_t);
-template void launch_vx<float, __hip_bfloat16, float>(const int *, int, const VTile *, const int *, int,
+template void launch_vx<float, __hip_bfloat16, float>(const int *, int, const VTile *, const VTile *, int,
                                                       const float *, const __hip_bfloat16 *, float *, double *,
                                                       const double *, int, hipStream_t);
 template void launch_rows<double, double, double>(const int *, const int *, int, RowClass, const double *, RowClass,

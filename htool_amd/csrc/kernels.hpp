@@ -116,7 +116,7 @@ void launch_gather_x32(const double *x, const int *iperm_s, int nc, int tpad, in
                        hipStream_t st);
 // Both precision classes in one launch: chunk d reads varena0 or varena1 by d.cls.
 template <typename TS0, typename TS1, typename TC>
-void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const int *multi_first, int nmulti,
+void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const VTile *multi_tiles, int nmulti,
                const TS0 *varena0, const TS1 *varena1, TC *z, double *part, const double *tscale, int MU,
                hipStream_t st);
 
@@ -163,7 +163,7 @@ template <typename R>
 void launch_gather_xc(const double *x, const int *perm_s, int nc, int tpad, int mu_base, int MU, R *z,
                       hipStream_t st);
 template <typename TS0, typename TS1, typename R>
-void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const int *multi_first, int nmulti,
+void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const VTile *multi_tiles, int nmulti,
                  const TS0 *varena0, const TS1 *varena1, R *z, double *part, const double *tscale, int MU,
                  hipStream_t st);
 template <typename TS0, typename TS1, typename R>

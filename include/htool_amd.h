@@ -180,9 +180,9 @@ int htg_hmatrix_get_layout(const htg_hmatrix *h, htg_layout *layout);
 int htg_hmatrix_set_layout(htg_hmatrix *h, const htg_layout *layout);
 
 /* Autotuner: times `reps` products with `mu` right-hand sides in the current
- * storage for each candidate layout (rows per row block x V chunk size x
- * stacked V rows), keeps the fastest and returns it with its time (ms per
- * product) and the number of candidates tried. */
+ * storage for the current layout and 24 candidates (rows per row block x
+ * (V chunk size, wavefront share) x stacked V rows), keeps the fastest and
+ * returns it with its time (ms per product) and the number of layouts tried. */
 int htg_hmatrix_autotune(htg_hmatrix *h, int mu, int reps, htg_layout *best, double *best_ms, int *ncandidates);
 
 /* Captures one product of device buffers d_x -> d_y (mu right-hand sides, the

@@ -89,9 +89,11 @@ def test_autotune_picks_a_candidate_and_keeps_the_product(htg):
     y = B @ x
     B.set_storage("fp32")
     y32 = B @ x
+    start = B.get_layout()
     lay, ms, tried = B.autotune(mu=1, reps=3)
-    assert tried == 12 and ms > 0
-    assert lay == B.get_layout() and lay["rb_max"] in (64, 128) and lay["vrows"] in (16, 32)
+    assert tried == 25 and ms > 0
+    assert lay == B.get_layout()
+    assert lay == start or (lay["rb_max"] in (64, 128) and lay["vrows"] in (8, 16, 32) and lay["vgroup_tiles"] == 4)
     assert B.storage == "fp32"
     assert rel(B @ x, y32) < 1e-6
     B.set_storage("fp64")
