@@ -1227,11 +1227,34 @@ __device__ __forceinline__ void rows_class(const TS *arena, const RowClass &rc, 
     const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
     const uint64_t *rb_aoff = rc.rb_aoff;
     const TS *A  = arena + rb_aoff[rb];
+    constexpr int ZPT = ZCAP / WG; // staged entries per thread
     for (int cc = 0; cc < Cp; cc += ZCOLS) {
         const int nz = min(ZCOLS, Cp - cc);
-        for (int e = threadIdx.x; e < nz * MU; e += WG) {
-            const int c = cc + e / MU;
-            s_z[e]      = c < C ? z[(size_t)colidx[c0 + c] * MU + e % MU] : (TC)0;
+        if constexpr (MU == 1 && sizeof(TC) == 4) {
+            // all column indices of the pass, then all vector entries, in
+            // flight together: two memory latencies per pass (FP32 vectors;
+            // for FP64 the extra registers cost more occupancy than this saves)
+            int idx[ZPT];
+#pragma unroll
+            for (int i = 0; i < ZPT; i++) {
+                const int c = threadIdx.x + i * WG;
+                idx[i]      = c < nz && cc + c < C ? colidx[c0 + cc + c] : -1;
+            }
+            TC v[ZPT];
+#pragma unroll
+            for (int i = 0; i < ZPT; i++)
+                v[i] = idx[i] >= 0 ? z[idx[i]] : (TC)0;
+#pragma unroll
+            for (int i = 0; i < ZPT; i++) {
+                const int c = threadIdx.x + i * WG;
+                if (c < nz)
+                    s_z[c] = v[i];
+            }
+        } else {
+            for (int e = threadIdx.x; e < nz * MU; e += WG) {
+                const int c = cc + e / MU;
+                s_z[e]      = c < C ? z[(size_t)colidx[c0 + c] * MU + e % MU] : (TC)0;
+            }
         }
         __syncthreads();
         rows_gemv<TS, TC, MU>(A + cc, (size_t)Cp, R, nz / W, s_z, s_acc);
