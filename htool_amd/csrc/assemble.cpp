@@ -66,6 +66,10 @@ struct Assembler {
     uint64_t store_used = 0;     // entries (complex entries for the complex families)
     Assembly *rec = nullptr; // ACA records and counters
     int esz       = 1;       // doubles per entry
+    // ACA workspace and visited flags of a sub-batch: grow-only, so the
+    // multi-GiB workspace is allocated once per build, not per sub-batch
+    DevArray<double> d_ws;
+    DevArray<unsigned char> d_vis;
 
     void expand(int id) {
         const ClusterNode &a = T.nodes[nodes[id].t], &b = S.nodes[nodes[id].s];
@@ -202,13 +206,11 @@ struct Assembler {
                 }
                 DevArray<AcaBlock> d_blocks;
                 DevArray<uint64_t> d_vis_off;
-                DevArray<double> d_ws;
-                DevArray<unsigned char> d_vis;
                 DevArray<int> d_ranks;
                 d_blocks.upload(blocks);
                 d_vis_off.upload(vis_off);
-                d_ws.alloc(ws_used * esz);
-                d_vis.alloc(vis_used);
+                d_ws.reserve(ws_used * esz); // kept across sub-batches and waves
+                d_vis.reserve(vis_used);
                 d_ranks.alloc(blocks.size());
                 if (esz == 2)
                     launch_aca_c(kd, dT, dS, d_blocks.p, (int)blocks.size(), prm.epsilon, prm.reqrank,

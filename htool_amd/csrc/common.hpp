@@ -48,6 +48,11 @@ struct DevArray {
         HTG_HIP(hipMalloc(&p, count * sizeof(T)));
         n = count;
     }
+    // at least `count` elements, contents not kept; grows only
+    void reserve(size_t count) {
+        if (count > n || !p)
+            alloc(count);
+    }
     void zero() {
         if (p)
             HTG_HIP(hipMemset(p, 0, n * sizeof(T)));
