@@ -2387,6 +2387,168 @@ __device__ __forceinline__ void rows_class_c(const TS *arena, const RowClass &rc
     }
 }
 
+// ---------------------------------------------------------------------------
+// Complex FP64 16-RHS products on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64). A complex entry a = ar + i ai is one 16-byte
+// load; lane (g, r) = (l / 16, l % 16) takes row r of a 16-row tile and, in
+// step it, complex column 4 it + g. Four real MFMAs per step:
+// re += ar br - ai bi, im += ar bi + ai br, where (br, bi) is the vector entry
+// of that column for right-hand side r. Result register i of lane (g, r) is
+// row 4 i + g, right-hand side r. z, the partial sums and y keep the layout of
+// the other complex paths (re, im interleaved, RHS-interleaved).
+constexpr int MU_C64   = 16;
+constexpr int RPASS_C64 = 32;                // complex row-block columns staged per pass
+constexpr int ZST_C64   = 2 * MU_C64 + 2;    // doubles per staged column (padded against bank conflicts)
+
+__device__ __forceinline__ void cmfma64(double ar, double ai, double br, double bi, doublex4 &re, doublex4 &im) {
+    re = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, re, 0, 0, 0);
+    re = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, re, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, im, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, im, 0, 0, 0);
+}
+
+__global__ void __launch_bounds__(WG) k_rows_mfma64c(const int *rb_order, const int *rb_row0, RowClass c0,
+                                                     const double *arena0, RowClass c1, const double *arena1,
+                                                     const double *z, const int *perm_t, int nr, int mu_base,
+                                                     double *y) {
+    constexpr int MU  = MU_C64;
+    constexpr int NIT = RPASS_C64 / 4;
+    constexpr int ZPT = RPASS_C64 * MU / WG; // staged complex entries per thread
+    __shared__ double s_z[RPASS_C64 * ZST_C64];
+    const int rb = rb_order[blockIdx.x];
+    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int T16 = (R + 15) / 16; // <= 8 tiles; wave w takes tiles w and w + 4
+    doublex4 acc_re[2] = {}, acc_im[2] = {};
+    const int zs = threadIdx.x & (MU - 1), zc = threadIdx.x / MU;
+    for (int cls = 0; cls < 2; cls++) {
+        const RowClass &rc   = cls == 0 ? c0 : c1;
+        const double *arena = cls == 0 ? arena0 : arena1;
+        if (!arena)
+            continue;
+        const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
+        const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+        const double *A = arena + 2 * rc.rb_aoff[rb];
+        for (int cc = 0; cc < Cp; cc += RPASS_C64) {
+            const int nz = min(RPASS_C64, Cp - cc);
+            {
+                int idx[ZPT];
+#pragma unroll
+                for (int i = 0; i < ZPT; i++) {
+                    const int c = cc + zc + (WG / MU) * i;
+                    idx[i]      = rc.colidx[cb + (c < C ? c : 0)];
+                }
+#pragma unroll
+                for (int i = 0; i < ZPT; i++) {
+                    const int c = zc + (WG / MU) * i;
+                    double v[2];
+                    Vec<double>::load(z + 2 * ((size_t)idx[i] * MU + zs), v);
+                    const bool ok             = cc + c < C;
+                    s_z[c * ZST_C64 + 2 * zs]     = ok ? v[0] : 0.0;
+                    s_z[c * ZST_C64 + 2 * zs + 1] = ok ? v[1] : 0.0;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++) {
+                const int t = w + 4 * tt;
+                if (t < T16) {
+                    const int row    = 16 * t + r16;
+                    const double *rp = A + 2 * ((size_t)(row < R ? row : 0) * Cp + cc);
+                    double a[NIT][2];
+#pragma unroll
+                    for (int it = 0; it < NIT; it++) {
+                        const int kk = 4 * it + g;
+                        Vec<double>::load(rp + 2 * (kk < nz ? kk : 0), a[it]);
+                    }
+#pragma unroll
+                    for (int it = 0; it < NIT; it++) {
+                        if (4 * it < nz) {
+                            const int kk     = 4 * it + g;
+                            const bool in    = kk < nz;
+                            const double *bz = s_z + (in ? kk : 0) * ZST_C64 + 2 * r16;
+                            cmfma64(in ? a[it][0] : 0.0, in ? a[it][1] : 0.0, bz[0], bz[1], acc_re[tt], acc_im[tt]);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int tt = 0; tt < 2; tt++) {
+        const int t = w + 4 * tt;
+        if (t < T16) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int orow = 16 * t + 4 * i + g;
+                if (orow < R) {
+                    double *o = y + 2 * ((size_t)perm_t[r0 + orow] + (size_t)nr * (mu_base + r16));
+                    o[0]      = acc_re[tt][i];
+                    o[1]      = acc_im[tt][i];
+                }
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_vx_mfma64c(const int *wv_off, int nwaves, const VTile *tiles,
+                                                   const double *varena0, const double *varena1, double *z,
+                                                   double *part, const double *tscale) {
+    constexpr int MU  = MU_C64;
+    constexpr int NIT = 8; // 32 complex columns per pass
+    const int wv      = blockIdx.x * NWAVE + (threadIdx.x >> 6);
+    if (wv >= nwaves)
+        return;
+    const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
+    for (int ti = wv_off[wv]; ti < wv_off[wv + 1]; ti++) {
+        const VTile d       = tiles[ti];
+        const double *vare  = d.cls == 0 ? varena0 : varena1;
+        const double *xv    = z + 2 * (size_t)d.x_off * MU;
+        for (int rt = 0; rt < d.k; rt += 16) {
+            const int row    = rt + r16;
+            const double *rp = vare + 2 * (d.v_off + (size_t)(row < d.k ? row : 0) * d.ld);
+            doublex4 acc_re = {0}, acc_im = {0};
+            for (int c0 = 0; c0 < d.ncols; c0 += 4 * NIT) {
+                const int nc = min(4 * NIT, d.ncols - c0);
+                double a[NIT][2];
+#pragma unroll
+                for (int it = 0; it < NIT; it++) {
+                    const int kk = 4 * it + g;
+                    Vec<double>::load(rp + 2 * (c0 + (kk < nc ? kk : 0)), a[it]);
+                }
+#pragma unroll
+                for (int it = 0; it < NIT; it++) {
+                    if (4 * it < nc) {
+                        const int kk  = 4 * it + g;
+                        const bool in = kk < nc;
+                        double b[2];
+                        Vec<double>::load(xv + 2 * ((size_t)(c0 + (in ? kk : 0)) * MU + r16), b);
+                        cmfma64(in ? a[it][0] : 0.0, in ? a[it][1] : 0.0, b[0], b[1], acc_re, acc_im);
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = rt + 4 * i + g;
+                if (r < d.k) {
+                    if (d.nparts > 1) {
+                        double *o = part + 2 * ((size_t)(d.part_off + r) * MU + r16);
+                        o[0]      = acc_re[i];
+                        o[1]      = acc_im[i];
+                    } else {
+                        const double sc = tscale ? tscale[d.t_index + r] : 1.0;
+                        double *o       = z + 2 * ((size_t)(d.t_index + r) * MU + r16);
+                        o[0]            = acc_re[i] * sc;
+                        o[1]            = acc_im[i] * sc;
+                    }
+                }
+            }
+        }
+    }
+}
+
 template <typename TS0, typename TS1, typename R, int MU>
 __global__ void __launch_bounds__(WG) k_rows_c(const int *rb_order, const int *rb_row0, RowClass c0, const TS0 *arena0,
                                                RowClass c1, const TS1 *arena1, const R *z, const int *perm_t, int nr,
@@ -2615,6 +2777,17 @@ void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const VTile
                  hipStream_t st) {
     if (ngroups <= 0)
         return;
+    if constexpr (std::is_same<R, double>::value && std::is_same<TS0, double>::value &&
+                  std::is_same<TS1, double>::value) {
+        if (MU == MU_C64) {
+            hipLaunchKernelGGL(k_vx_mfma64c, dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, ngroups,
+                               tiles, varena0, varena1, z, part, tscale);
+            if (nmulti > 0)
+                hipLaunchKernelGGL((k_vx_sum_c<R, MU_C64>), dim3(nmulti), dim3(WG), 0, st, multi_tiles, nmulti, part,
+                                   z, tscale);
+            return;
+        }
+    }
     if constexpr (std::is_same<R, float>::value) {
         if (MU == MFMA_MU_C) {
             hipLaunchKernelGGL((k_vx_mfma_c<TS0, TS1>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off,
@@ -2644,6 +2817,14 @@ void launch_rows_c(const int *rb_order, const int *rb_row0, int nrb, RowClass c0
                    hipStream_t st) {
     if (nrb <= 0)
         return;
+    if constexpr (std::is_same<R, double>::value && std::is_same<TS0, double>::value &&
+                  std::is_same<TS1, double>::value) {
+        if (MU == MU_C64) {
+            hipLaunchKernelGGL(k_rows_mfma64c, dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0, c1, arena1,
+                               z, perm_t, nr, mu_base, y);
+            return;
+        }
+    }
     if constexpr (std::is_same<R, float>::value) {
         if (MU == MFMA_MU_C) {
             hipLaunchKernelGGL((k_rows_mfma_c<TS0, TS1>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0,
