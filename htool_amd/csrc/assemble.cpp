@@ -20,6 +20,7 @@ namespace htg {
 constexpr int CAP_FIRST = 64;                       // ACA terms reserved per block in the first attempt
 constexpr int CAP_MAX   = 512;                      // must match the ACA kernel's CAPMAX
 constexpr size_t WS_MAX = size_t(1) << 30;          // doubles of ACA workspace per sub-batch (8 GiB)
+constexpr int ACAW_MIN_DEFAULT = 16384;             // M + N from which a block runs the multi-workgroup ACA
 
 void PointsOnDevice::upload(const double *pts, const std::vector<int> &perm_h) {
     const size_t n = perm_h.size();
@@ -204,23 +205,9 @@ struct Assembler {
                     blocks.push_back(B);
                     b1++;
                 }
-                DevArray<AcaBlock> d_blocks;
-                DevArray<uint64_t> d_vis_off;
-                DevArray<int> d_ranks;
-                d_blocks.upload(blocks);
-                d_vis_off.upload(vis_off);
                 d_ws.reserve(ws_used * esz); // kept across sub-batches and waves
                 d_vis.reserve(vis_used);
-                d_ranks.alloc(blocks.size());
-                if (esz == 2)
-                    launch_aca_c(kd, dT, dS, d_blocks.p, (int)blocks.size(), prm.epsilon, prm.reqrank,
-                                 reinterpret_cast<cdbl *>(d_ws.p), d_vis.p, d_vis_off.p, d_ranks.p, nullptr);
-                else
-                    launch_aca(kd, dT, dS, d_blocks.p, (int)blocks.size(), prm.epsilon, prm.reqrank, d_ws.p, d_vis.p,
-                               d_vis_off.p, d_ranks.p, nullptr);
-                HTG_HIP(hipGetLastError());
-                std::vector<int> ranks(blocks.size());
-                HTG_HIP(hipMemcpy(ranks.data(), d_ranks.p, ranks.size() * sizeof(int), hipMemcpyDeviceToHost));
+                const std::vector<int> ranks = run_aca(blocks, vis_off);
 
                 // Move the accepted factors into the compressed store.
                 std::vector<LrPack> packs;
@@ -291,6 +278,105 @@ struct Assembler {
             }
             todo.swap(retry);
         }
+    }
+
+    // ACA of one sub-batch (workspace and visited flags in d_ws, d_vis): blocks
+    // with M + N >= aca_wide_min on the multi-workgroup path, the others one
+    // workgroup each. Ranks in block order.
+    std::vector<int> run_aca(const std::vector<AcaBlock> &blocks, const std::vector<uint64_t> &vis_off) {
+        const int wmin = prm.aca_wide_min == 0 ? ACAW_MIN_DEFAULT : prm.aca_wide_min;
+        std::vector<int> narrow, wide;
+        for (size_t i = 0; i < blocks.size(); i++)
+            ((wmin > 0 && blocks[i].M + blocks[i].N >= wmin) ? wide : narrow).push_back((int)i);
+        std::vector<int> ranks(blocks.size());
+        if (!narrow.empty()) {
+            std::vector<AcaBlock> nb;
+            std::vector<uint64_t> nv;
+            for (int i : narrow) {
+                nb.push_back(blocks[i]);
+                nv.push_back(vis_off[i]);
+            }
+            DevArray<AcaBlock> d_b;
+            DevArray<uint64_t> d_v;
+            DevArray<int> d_r;
+            d_b.upload(nb);
+            d_v.upload(nv);
+            d_r.alloc(nb.size());
+            if (esz == 2)
+                launch_aca_c(kd, dT, dS, d_b.p, (int)nb.size(), prm.epsilon, prm.reqrank,
+                             reinterpret_cast<cdbl *>(d_ws.p), d_vis.p, d_v.p, d_r.p, nullptr);
+            else
+                launch_aca(kd, dT, dS, d_b.p, (int)nb.size(), prm.epsilon, prm.reqrank, d_ws.p, d_vis.p, d_v.p,
+                           d_r.p, nullptr);
+            HTG_HIP(hipGetLastError());
+            std::vector<int> r(nb.size());
+            HTG_HIP(hipMemcpy(r.data(), d_r.p, r.size() * sizeof(int), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < narrow.size(); k++)
+                ranks[narrow[k]] = r[k];
+        }
+        if (wide.empty())
+            return ranks;
+        rec->aca_wide += (int)wide.size();
+        std::vector<AcaBlock> wb;
+        std::vector<uint64_t> wv;
+        std::vector<AcaWide> st;
+        std::vector<AcawTask> t1, t2;
+        for (size_t k = 0; k < wide.size(); k++) {
+            const AcaBlock &B = blocks[wide[k]];
+            const int n1 = B.swap ? B.N : B.M, n2 = B.swap ? B.M : B.N;
+            AcaWide w{};
+            w.nch1  = (n1 + ACAW_CHUNK - 1) / ACAW_CHUNK;
+            w.nch2  = (n2 + ACAW_CHUNK - 1) / ACAW_CHUNK;
+            w.task1 = (int)t1.size();
+            w.task2 = (int)t2.size();
+            for (int c = 0; c < w.nch1; c++)
+                t1.push_back(AcawTask{(int)k, c});
+            for (int c = 0; c < w.nch2; c++)
+                t2.push_back(AcawTask{(int)k, c});
+            wb.push_back(B);
+            wv.push_back(vis_off[wide[k]]);
+            st.push_back(w);
+        }
+        for (AcaWide &w : st)
+            w.task1 += (int)t2.size(); // partial results: side-2 chunks, then side-1 chunks
+        const size_t nslots = t1.size() + t2.size();
+        DevArray<AcaBlock> d_b;
+        DevArray<uint64_t> d_v;
+        DevArray<AcaWide> d_st;
+        DevArray<AcawTask> d_t1, d_t2;
+        DevArray<double> d_pv, d_dd, d_xs;
+        DevArray<int> d_pi;
+        d_b.upload(wb);
+        d_v.upload(wv);
+        d_st.upload(st);
+        d_t1.upload(t1);
+        d_t2.upload(t2);
+        d_pv.alloc(nslots);
+        d_pi.alloc(nslots);
+        d_dd.alloc(nslots);
+        d_xs.alloc(nslots * CAP_MAX * esz);
+        const int nw = (int)wide.size();
+        launch_acaw_init(kd, dT, dS, d_b.p, d_st.p, nw, prm.reqrank, prm.epsilon, d_vis.p, d_v.p, nullptr);
+        HTG_HIP(hipGetLastError());
+        // one cross term per iteration; every block stops by its cap (<= CAP_MAX terms)
+        for (int it = 0; it <= CAP_MAX; it++) {
+            launch_acaw_term(kd, dT, dS, d_b.p, d_st.p, nw, d_t1.p, (int)t1.size(), d_t2.p, (int)t2.size(),
+                             prm.reqrank, prm.epsilon, d_ws.p, esz == 2, d_vis.p, d_v.p, d_pv.p, d_pi.p, d_dd.p,
+                             d_xs.p, nullptr);
+            HTG_HIP(hipGetLastError());
+            if (it % 8 == 7 || it == CAP_MAX) {
+                HTG_HIP(hipMemcpy(st.data(), d_st.p, st.size() * sizeof(AcaWide), hipMemcpyDeviceToHost));
+                if (std::none_of(st.begin(), st.end(), [](const AcaWide &w) { return w.active != 0; }))
+                    break;
+            }
+        }
+        HTG_HIP(hipMemcpy(st.data(), d_st.p, st.size() * sizeof(AcaWide), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < wide.size(); k++) {
+            if (st[k].active)
+                throw Error(HTG_ERR_HIP, "multi-workgroup ACA did not finish");
+            ranks[wide[k]] = st[k].rank;
+        }
+        return ranks;
     }
 
     void grow_store(uint64_t need) { // need: entries

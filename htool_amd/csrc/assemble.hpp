@@ -32,7 +32,7 @@ struct Assembly {
     // first workspace terms; retries and workspace totals.
     std::vector<float> aca_feat;
     std::vector<int> aca_terms, aca_cap;
-    int aca_retry = 0;
+    int aca_retry = 0, aca_wide = 0;
     int64_t aca_ws_entries = 0, aca_need_entries = 0;
     double aca_seconds = 0;
     DevPoints dT() const { return pT.view(); }

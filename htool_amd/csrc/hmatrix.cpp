@@ -76,7 +76,7 @@ struct htg_hmatrix {
     // ACA runs of the build (first attempts): features, terms needed, first workspace terms
     std::vector<float> aca_feat;
     std::vector<int> aca_terms, aca_cap;
-    int aca_retry = 0;
+    int aca_retry = 0, aca_wide = 0;
     int64_t aca_ws_entries = 0, aca_need_entries = 0;
     double aca_seconds = 0;
     bool scaled = false; // some low-rank terms carry scales (recompressed)
@@ -673,6 +673,7 @@ void take_aca_stats(htg_hmatrix &h, Assembly &A) {
     h.aca_terms.insert(h.aca_terms.end(), A.aca_terms.begin(), A.aca_terms.end());
     h.aca_cap.insert(h.aca_cap.end(), A.aca_cap.begin(), A.aca_cap.end());
     h.aca_retry += A.aca_retry;
+    h.aca_wide += A.aca_wide;
     h.aca_ws_entries += A.aca_ws_entries;
     h.aca_need_entries += A.aca_need_entries;
     h.aca_seconds += A.aca_seconds;
@@ -1100,6 +1101,7 @@ int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info) {
         r.aca_need_entries  = h->aca_need_entries;
         r.aca_seconds       = h->aca_seconds;
         r.is_complex        = h->cplx ? 1 : 0;
+        r.n_aca_wide        = h->aca_wide;
         *info               = r;
     });
 }

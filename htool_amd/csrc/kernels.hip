@@ -408,6 +408,305 @@ __global__ void __launch_bounds__(WG) k_aca(KernelDesc kd, DevPoints T, DevPoint
 }
 
 // ---------------------------------------------------------------------------
+// Multi-workgroup partial ACA for very large blocks: the steps of k_aca
+// (partialACA.hpp:39-171, sympartialACA's exchanged roles with `swap`), with
+// each residual row and column cut into chunks of ACAW_CHUNK entries, one
+// workgroup per chunk. The host drives the cross terms; per term:
+//   k_acaw_row  residual row of pivot I1 (side 2), per chunk its argmax over
+//               unvisited entries, |row|^2 and the products with earlier rows;
+//   k_acaw_piv  pivot column I2 (ties to the last index), reciprocal pivot or
+//               the zero-pivot exit;
+//   k_acaw_col  scaled residual column (side 1), the same per-chunk partials;
+//   k_acaw_fin  next pivot row, Frobenius estimate (partials summed in chunk
+//               order), stopping test for the next term.
+// Residual entries are the same sums in the same order as in k_aca, so the
+// factors and ranks are those of the one-workgroup kernel; only the Frobenius
+// estimate's sums are taken in another order.
+
+// Loop entry of k_aca for the next cross term: the `while` test, then the
+// q (M + N) > MN exit and the workspace cap.
+__device__ void acaw_enter(AcaWide &st, const AcaBlock &B, int reqrank, double eps) {
+    const int p   = st.q;
+    const bool go = (reqrank > 0 && p < min(reqrank, min(B.M, B.N))) ||
+                    (reqrank < 0 && (p == 0 || sqrt(st.aux / st.frob) > eps));
+    st.active = 0;
+    if (!go) {
+        st.rank = p;
+    } else if ((long long)(p + 1) * (B.M + B.N) > (long long)B.M * B.N) {
+        st.rank = -1;
+    } else if (p + 1 > B.cap || p + 1 > CAPMAX) {
+        st.rank = -3;
+    } else {
+        st.active = 1;
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_acaw_init(const AcaBlock *blocks, AcaWide *states, int reqrank, double eps,
+                                                  DevPoints T, DevPoints S, unsigned char *vis_ws,
+                                                  const uint64_t *vis_off) {
+    __shared__ double s_v[NWAVE];
+    __shared__ int s_i[NWAVE];
+    const AcaBlock B = blocks[blockIdx.x];
+    const bool swap  = B.swap != 0;
+    const int n1 = swap ? B.N : B.M, n2 = swap ? B.M : B.N;
+    unsigned char *vis1 = vis_ws + vis_off[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int a = tid; a < n1 + n2; a += WG)
+        vis1[a] = 0;
+    // first pivot: the side-1 point nearest to the cluster centre, first on ties (partialACA.hpp:42-50)
+    const DevPoints &X = swap ? S : T;
+    const int off      = swap ? B.off_s : B.off_t;
+    double best        = 1e300;
+    int bi             = n1;
+    for (int a = tid; a < n1; a += WG) {
+#pragma clang fp contract(off)
+        double d2 = 0, u;
+        u         = X.x[off + a] - B.ctr[0];
+        d2        = d2 + u * u;
+        u         = X.y[off + a] - B.ctr[1];
+        d2        = d2 + u * u;
+        u         = X.z[off + a] - B.ctr[2];
+        d2        = d2 + u * u;
+        const double d = sqrt(d2);
+        if (d < best) {
+            best = d;
+            bi   = a;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const int oi    = __shfl_xor(bi, o, 64);
+        if (ob < best || (ob == best && oi < bi)) {
+            best = ob;
+            bi   = oi;
+        }
+    }
+    if (lane == 0) {
+        s_v[w] = best;
+        s_i[w] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double rb = s_v[0];
+        int ri    = s_i[0];
+        for (int q = 1; q < NWAVE; q++)
+            if (s_v[q] < rb || (s_v[q] == rb && s_i[q] < ri)) {
+                rb = s_v[q];
+                ri = s_i[q];
+            }
+        AcaWide st = states[blockIdx.x];
+        st.I1      = ri < n1 ? ri : 0;
+        st.I2      = 0;
+        st.q       = 0;
+        st.frob    = 0;
+        st.aux     = 0;
+        acaw_enter(st, B, reqrank, eps);
+        states[blockIdx.x] = st;
+    }
+}
+
+// Residual row (side 2, kRow) or scaled residual column (side 1) of the current
+// cross term over one chunk, with the chunk's partial results at `slot`.
+template <typename E, bool kRow>
+__global__ void __launch_bounds__(WG) k_acaw_vec(KernelDesc kd, DevPoints T, DevPoints S, const AcaBlock *blocks,
+                                                 const AcaWide *states, const AcawTask *tasks, int slot0, E *ws,
+                                                 const unsigned char *vis_ws, const uint64_t *vis_off, double *pv,
+                                                 int *pi, double *dd, E *xs) {
+    __shared__ E s_coef[CAPMAX];
+    __shared__ E s_u[ACAW_CHUNK];
+    __shared__ double s_v[NWAVE];
+    __shared__ int s_i[NWAVE];
+    __shared__ double s_red[NWAVE];
+    const AcawTask tk = tasks[blockIdx.x];
+    const AcaWide st  = states[tk.block];
+    if (!st.active)
+        return;
+    const AcaBlock B = blocks[tk.block];
+    const bool swap  = B.swap != 0;
+    const int n1 = swap ? B.N : B.M, n2 = swap ? B.M : B.N;
+    E *U1 = ws + B.ws1, *V2 = ws + B.ws2;
+    const unsigned char *vis = vis_ws + vis_off[tk.block] + (kRow ? n1 : 0);
+    const int p   = st.q;
+    const int len = kRow ? n2 : n1;
+    const int e0 = tk.chunk * ACAW_CHUNK, e1 = min(len, e0 + ACAW_CHUNK);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // row: coefficients -u_j(I1) of the previous columns; column: -v_j(I2)
+    for (int j = tid; j < p; j += WG)
+        s_coef[j] = kRow ? neg(U1[(size_t)j * n1 + st.I1]) : neg(V2[(size_t)j * n2 + st.I2]);
+    __syncthreads();
+    const E *prev = kRow ? V2 : U1; // earlier vectors of this side
+    E *out        = (kRow ? V2 : U1) + (size_t)p * len;
+    E g;                            // reciprocal pivot (column only)
+    if constexpr (std::is_same<E, double>::value)
+        g = st.gre;
+    else
+        g = E{st.gre, st.gim};
+    double best = -1.0, dsq = 0;
+    int bi = -1;
+    for (int e = e0 + tid; e < e1; e += WG) {
+        E val = kRow ? (swap ? entry_s<E>(kd, T, B.off_t + e, S, B.off_s + st.I1)
+                             : entry_s<E>(kd, T, B.off_t + st.I1, S, B.off_s + e))
+                     : (swap ? entry_s<E>(kd, T, B.off_t + st.I2, S, B.off_s + e)
+                             : entry_s<E>(kd, T, B.off_t + e, S, B.off_s + st.I2));
+        for (int j = 0; j < p; j++)
+            val = axpy_s(s_coef[j], prev[(size_t)j * len + e], val);
+        if (!kRow)
+            val = mul(val, g);
+        out[e]      = val;
+        s_u[e - e0] = val;
+        if (!vis[e]) {
+            const double a = mag(val);
+            if (a >= best) {
+                best = a;
+                bi   = e;
+            }
+        }
+        dsq = real_part(cdot_acc(E{dsq}, val, val));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const int oi    = __shfl_xor(bi, o, 64);
+        if (ob > best || (ob == best && oi > bi)) {
+            best = ob;
+            bi   = oi;
+        }
+    }
+    if (lane == 0) {
+        s_v[w] = best;
+        s_i[w] = bi;
+    }
+    dsq = block_sum(dsq, s_red); // synchronises: s_v, s_i and s_u complete
+    const int slot = slot0 + blockIdx.x;
+    if (tid == 0) {
+        double rb = s_v[0];
+        int ri    = s_i[0];
+        for (int q = 1; q < NWAVE; q++)
+            if (s_v[q] > rb || (s_v[q] == rb && s_i[q] > ri)) {
+                rb = s_v[q];
+                ri = s_i[q];
+            }
+        pv[slot] = rb;
+        pi[slot] = ri;
+        dd[slot] = dsq;
+    }
+    // products with the earlier vectors of this side, for the Frobenius estimate
+    for (int j = w; j < p; j += NWAVE) {
+        E x = zero_s<E>();
+        for (int e = e0 + lane; e < e1; e += 64)
+            x = cdot_acc(x, prev[(size_t)j * len + e], s_u[e - e0]);
+        x = wave_sum_s(x);
+        if (lane == 0)
+            xs[(size_t)slot * CAPMAX + j] = x;
+    }
+}
+
+// Pivot column from the row chunks' partials, one thread per block.
+template <typename E>
+__global__ void __launch_bounds__(WG) k_acaw_piv(const AcaBlock *blocks, AcaWide *states, int nblocks, const E *ws,
+                                                 unsigned char *vis_ws, const uint64_t *vis_off, const double *pv,
+                                                 const int *pi) {
+    const int bk = blockIdx.x * WG + threadIdx.x;
+    if (bk >= nblocks)
+        return;
+    AcaWide st = states[bk];
+    if (!st.active)
+        return;
+    const AcaBlock B = blocks[bk];
+    const int n2     = B.swap ? B.M : B.N;
+    double best      = -1.0;
+    int bi           = -1;
+    for (int c = 0; c < st.nch2; c++) {
+        const double v = pv[st.task2 + c];
+        const int i    = pi[st.task2 + c];
+        if (i >= 0 && (v > best || (v == best && i > bi))) {
+            best = v;
+            bi   = i;
+        }
+    }
+    if (bi >= 0)
+        st.I2 = bi;
+    vis_ws[vis_off[bk] + st.I1] = 1;
+    const E piv = ws[B.ws2 + (size_t)st.q * n2 + st.I2];
+    if (mag(piv) > 1e-15) {
+        const E g = recip(piv);
+        if constexpr (std::is_same<E, double>::value) {
+            st.gre = g;
+            st.gim = 0;
+        } else {
+            st.gre = g.re;
+            st.gim = g.im;
+        }
+    } else { // k_aca: q -= 1, and q == 0 means not compressible
+        st.active = 0;
+        st.rank   = st.q == 0 ? -1 : st.q;
+    }
+    states[bk] = st;
+}
+
+// Next pivot row, Frobenius estimate and stopping test, one workgroup per block.
+template <typename E>
+__global__ void __launch_bounds__(WG) k_acaw_fin(const AcaBlock *blocks, AcaWide *states, int reqrank, double eps,
+                                                 unsigned char *vis_ws, const uint64_t *vis_off, const double *pv,
+                                                 const int *pi, const double *dd, const E *xs) {
+    __shared__ E s_prod[CAPMAX];
+    __shared__ int s_act;
+    const int bk = blockIdx.x;
+    AcaWide st   = states[bk];
+    if (threadIdx.x == 0)
+        s_act = st.active;
+    __syncthreads();
+    if (!s_act)
+        return;
+    const AcaBlock B = blocks[bk];
+    const bool swap  = B.swap != 0;
+    const int n1     = swap ? B.N : B.M;
+    const int p      = st.q;
+    if (reqrank < 0) {
+        for (int j = threadIdx.x; j < p; j += WG) {
+            E x1 = zero_s<E>(), x2 = zero_s<E>();
+            for (int c = 0; c < st.nch2; c++)
+                x1 = add(x1, xs[(size_t)(st.task2 + c) * CAPMAX + j]);
+            for (int c = 0; c < st.nch1; c++)
+                x2 = add(x2, xs[(size_t)(st.task1 + c) * CAPMAX + j]);
+            s_prod[j] = mul(x1, x2);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double best = -1.0;
+        int bi      = -1;
+        for (int c = 0; c < st.nch1; c++) {
+            const double v = pv[st.task1 + c];
+            const int i    = pi[st.task1 + c];
+            if (i >= 0 && (v > best || (v == best && i > bi))) {
+                best = v;
+                bi   = i;
+            }
+        }
+        if (bi >= 0)
+            st.I1 = bi;
+        vis_ws[vis_off[bk] + n1 + st.I2] = 1;
+        if (reqrank < 0) {
+            double d22 = 0, d11 = 0;
+            for (int c = 0; c < st.nch1; c++)
+                d22 += dd[st.task1 + c];
+            for (int c = 0; c < st.nch2; c++)
+                d11 += dd[st.task2 + c];
+            st.aux     = fabs(d22) * fabs(d11);
+            E frob_aux = zero_s<E>();
+            for (int j = 0; j < p; j++)
+                frob_aux = add(frob_aux, s_prod[j]);
+            st.frob += st.aux + 2 * real_part(frob_aux);
+        }
+        st.q = p + 1;
+        acaw_enter(st, B, reqrank, eps);
+        states[bk] = st;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Copies compressed leaves (from the ACA workspace or the compressed store)
 // into another layout: V row by row, U into one tile per destination.
 template <typename E>
@@ -2594,6 +2893,44 @@ void launch_aca_c(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBlock 
     if (nblocks > 0)
         hipLaunchKernelGGL(k_aca<cdbl>, dim3(nblocks), dim3(WG), 0, st, k, T, S, blocks, eps, reqrank, ws, vis_ws,
                            vis_off, ranks);
+}
+
+void launch_acaw_init(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBlock *blocks, AcaWide *st,
+                      int nblocks, int reqrank, double eps, unsigned char *vis_ws, const uint64_t *vis_off,
+                      hipStream_t s) {
+    (void)k;
+    if (nblocks > 0)
+        hipLaunchKernelGGL(k_acaw_init, dim3(nblocks), dim3(WG), 0, s, blocks, st, reqrank, eps, T, S, vis_ws,
+                           vis_off);
+}
+
+template <typename E>
+static void acaw_term(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBlock *blocks, AcaWide *st,
+                      int nblocks, const AcawTask *tasks1, int ntasks1, const AcawTask *tasks2, int ntasks2,
+                      int reqrank, double eps, E *ws, unsigned char *vis_ws, const uint64_t *vis_off, double *pv,
+                      int *pi, double *dd, E *xs, hipStream_t s) {
+    hipLaunchKernelGGL((k_acaw_vec<E, true>), dim3(ntasks2), dim3(WG), 0, s, k, T, S, blocks, st, tasks2, 0, ws,
+                       vis_ws, vis_off, pv, pi, dd, xs);
+    hipLaunchKernelGGL(k_acaw_piv<E>, dim3((nblocks + WG - 1) / WG), dim3(WG), 0, s, blocks, st, nblocks, ws, vis_ws,
+                       vis_off, pv, pi);
+    hipLaunchKernelGGL((k_acaw_vec<E, false>), dim3(ntasks1), dim3(WG), 0, s, k, T, S, blocks, st, tasks1, ntasks2, ws,
+                       vis_ws, vis_off, pv, pi, dd, xs);
+    hipLaunchKernelGGL(k_acaw_fin<E>, dim3(nblocks), dim3(WG), 0, s, blocks, st, reqrank, eps, vis_ws, vis_off, pv, pi,
+                       dd, xs);
+}
+
+void launch_acaw_term(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBlock *blocks, AcaWide *st,
+                      int nblocks, const AcawTask *tasks1, int ntasks1, const AcawTask *tasks2, int ntasks2,
+                      int reqrank, double eps, void *ws, bool cplx, unsigned char *vis_ws, const uint64_t *vis_off,
+                      double *pv, int *pi, double *dd, void *xs, hipStream_t s) {
+    if (nblocks <= 0)
+        return;
+    if (cplx)
+        acaw_term<cdbl>(k, T, S, blocks, st, nblocks, tasks1, ntasks1, tasks2, ntasks2, reqrank, eps,
+                        static_cast<cdbl *>(ws), vis_ws, vis_off, pv, pi, dd, static_cast<cdbl *>(xs), s);
+    else
+        acaw_term<double>(k, T, S, blocks, st, nblocks, tasks1, ntasks1, tasks2, ntasks2, reqrank, eps,
+                          static_cast<double *>(ws), vis_ws, vis_off, pv, pi, dd, static_cast<double *>(xs), s);
 }
 
 void launch_lr_pack(const LrPack *packs, int npacks, const UTileDst *utiles, const double *ws, double *arena,

@@ -138,6 +138,33 @@ void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, 
 template <typename TS>
 void launch_convert(const double *src, TS *dst, uint64_t n, hipStream_t st);
 
+// Multi-workgroup partial ACA of very large blocks: per-block state between
+// the launches of one cross term (cluster-order row/column indices local to the
+// block, side 1 = the side that picks pivot rows, as in AcaBlock).
+constexpr int ACAW_CHUNK = 2048; // residual entries per workgroup
+struct AcaWide {
+    int I1, I2;       // current pivot row (side 1) and column (side 2)
+    int q;            // cross terms accepted so far
+    int active;       // 1 while the block needs another cross term
+    int rank;         // result once inactive: rank, -1 (not compressible) or -3 (workspace full)
+    int nch1, nch2;   // chunks of side 1 / side 2
+    int task1, task2; // first chunk task of each side in the partial-result arrays
+    int pad;
+    double frob, aux; // Frobenius estimate, |u|^2 |v|^2 of the last cross term
+    double gre, gim;  // reciprocal of the last pivot
+};
+struct AcawTask {
+    int block, chunk;
+};
+// One launch sequence per stage; `E` selects real (double) or complex (cdbl).
+void launch_acaw_init(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBlock *blocks, AcaWide *st,
+                      int nblocks, int reqrank, double eps, unsigned char *vis_ws, const uint64_t *vis_off,
+                      hipStream_t s);
+void launch_acaw_term(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBlock *blocks, AcaWide *st,
+                      int nblocks, const AcawTask *tasks1, int ntasks1, const AcawTask *tasks2, int ntasks2,
+                      int reqrank, double eps, void *ws, bool cplx, unsigned char *vis_ws, const uint64_t *vis_off,
+                      double *pv, int *pi, double *dd, void *xs, hipStream_t s);
+
 // Complex kernel families.
 void launch_dense_fill_c(const KernelDesc &k, DevPoints T, DevPoints S, const FillTile *tiles, int ntiles, cdbl *arena,
                          hipStream_t st);

@@ -55,6 +55,7 @@ class HMatrix:
         self.lowprec_all = False
         self.predictor = None
         self.predictor_margin = 0.25
+        self.aca_wide_min = 0
         self._h = None
         self._storage = "fp64"
 
@@ -90,6 +91,11 @@ class HMatrix:
         self.predictor = predictor
         self.predictor_margin = float(margin)
 
+    def set_aca_wide_min(self, m: int):
+        """Blocks with M + N >= m run the multi-workgroup ACA (0: the library
+        default, 16384; negative: never). Factors and ranks do not change."""
+        self.aca_wide_min = int(m)
+
     # -- assembly ---------------------------------------------------------------
     def build(self, gen: Generator, xt: np.ndarray, xs: np.ndarray | None = None) -> "HMatrix":
         """HMatrix::build(A, xt, xs) (include/htool/types/hmatrix.hpp:285-341)."""
@@ -112,7 +118,7 @@ class HMatrix:
         return capi.htg_params(self.epsilon, self.eta, self.maxblocksize, self.minclustersize, self.compressor,
                                self.reqrank, int(self.lowprec_all), self.recompress_tol, self.lowprec_tol,
                                self.predictor.handle if self.predictor is not None else None,
-                               self.predictor_margin)
+                               self.predictor_margin, self.aca_wide_min)
 
     def build_batch(self, gens, point_sets) -> "HMatrix":
         """Many independent square H-matrices as one block-diagonal H-matrix

@@ -99,6 +99,8 @@ typedef struct htg_params {
                                        ACA workspace and orders the blocks by expected work
                                        (no counterpart in the reference); the ranks do not change */
     double predictor_margin;        /* room above the prediction: cap = ceil((1 + margin) t) + 2 */
+    int aca_wide_min;               /* blocks with M + N >= this run the multi-workgroup ACA (same
+                                       factors and ranks); 0: 16384, < 0: never */
 } htg_params;
 
 typedef struct htg_info {
@@ -120,6 +122,7 @@ typedef struct htg_info {
     int64_t aca_need_entries; /* sum over runs of (m + n) x terms actually needed */
     double aca_seconds;       /* batched ACA (kernels and host scheduling) */
     int is_complex;           /* entries, x and y are complex<double> */
+    int n_aca_wide;           /* ACA runs on the multi-workgroup path (htg_params.aca_wide_min) */
 } htg_info;
 
 /* Work decomposition of the product (no counterpart in the reference): rows

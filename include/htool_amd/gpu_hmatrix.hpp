@@ -51,6 +51,7 @@ class GpuHMatrix {
         prm_.lowprec_tol    = 0.0;
         prm_.predictor        = nullptr;
         prm_.predictor_margin = 0.25;
+        prm_.aca_wide_min     = 0;
     }
     GpuHMatrix(const GpuHMatrix &) = delete;
     GpuHMatrix &operator=(const GpuHMatrix &) = delete;
