@@ -71,26 +71,42 @@ struct Assembler {
     // multi-GiB workspace is allocated once per build, not per sub-batch
     DevArray<double> d_ws;
     DevArray<unsigned char> d_vis;
+    bool sym = false; // symmetric storage (htg_params.symmetry 'S')
 
+    // Sons of a false positive (hmatrix.hpp:543-606): the larger side splits;
+    // in a symmetric build both sides split into every pair of sons
+    // (ComputeAdmissibleBlocksSym, hmatrix.hpp:610-690).
     void expand(int id) {
-        const ClusterNode &a = T.nodes[nodes[id].t], &b = S.nodes[nodes[id].s];
+        const int t = nodes[id].t, s = nodes[id].s;
+        const ClusterNode &a = T.nodes[t], &b = S.nodes[s];
         if (a.leaf() && b.leaf()) {
             nodes[id].state = 3;
             return;
         }
-        const bool split_t       = b.leaf() ? true : (a.leaf() ? false : a.size > b.size);
-        const ClusterNode &owner = split_t ? a : b;
-        const int kid0           = (int)nodes.size();
-        for (int p = 0; p < owner.nb_sons; p++) {
+        std::vector<std::pair<int, int>> kids;
+        const bool split_t = b.leaf() ? true : (a.leaf() ? false : a.size > b.size);
+        if (sym && !a.leaf() && !b.leaf()) {
+            for (int l = 0; l < b.nb_sons; l++)
+                for (int p = 0; p < a.nb_sons; p++)
+                    kids.push_back({a.first_son + p, b.first_son + l});
+        } else if (split_t) {
+            for (int p = 0; p < a.nb_sons; p++)
+                kids.push_back({a.first_son + p, s});
+        } else {
+            for (int p = 0; p < b.nb_sons; p++)
+                kids.push_back({t, b.first_son + p});
+        }
+        const int kid0 = (int)nodes.size();
+        for (const auto &k : kids) {
             FpNode c{};
-            c.t = split_t ? owner.first_son + p : nodes[id].t;
-            c.s = split_t ? nodes[id].s : owner.first_son + p;
+            c.t = k.first;
+            c.s = k.second;
             nodes.push_back(c);
         }
         nodes[id].state = 2;
         nodes[id].kid0  = kid0;
-        nodes[id].nkids = owner.nb_sons;
-        for (int p = 0; p < owner.nb_sons; p++) {
+        nodes[id].nkids = (int)kids.size();
+        for (int p = 0; p < (int)kids.size(); p++) {
             const int c = kid0 + p;
             if (admissible(T.nodes[nodes[c].t], S.nodes[nodes[c].s], prm.eta)) {
                 nodes[c].state = 0;
@@ -417,7 +433,9 @@ void assemble(const htg_params &prm, const htg_generator &gen, int nt, const dou
     if (res.cplx && prm.recompress_tol > 0)
         throw Error(HTG_ERR_UNSUPPORTED, "QR+SVD recompression is implemented for the real kernel families only");
     Assembler as{T, S, prm, res.kd, res.dT(), res.dS(), {}, {}, {}, 0, {}, {}, 0, &res, res.cplx ? 2 : 1};
-    const std::vector<BlockTask> tasks = build_block_tasks(T, S, prm.eta, (size_t)prm.maxblocksize);
+    const char uplo = sym_uplo(prm);
+    as.sym          = uplo != 'N';
+    const std::vector<BlockTask> tasks = build_block_tasks(T, S, prm.eta, (size_t)prm.maxblocksize, uplo);
     std::vector<int> roots(tasks.size(), -1);
     for (size_t p = 0; p < tasks.size(); p++) {
         if (!tasks[p].admissible) {

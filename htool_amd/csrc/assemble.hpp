@@ -41,6 +41,12 @@ struct Assembly {
 
 KernelDesc kernel_desc(const htg_generator &g, bool same);
 
+// Kept triangle of a symmetric build ('L' or 'U'), 'N' for a general one.
+inline char sym_uplo(const htg_params &p) {
+    const int s = p.symmetry == 0 ? 'N' : p.symmetry;
+    return s == 'N' ? 'N' : (char)p.uplo;
+}
+
 // HMatrix::build (reference: include/htool/types/hmatrix.hpp:285-341, 422-607).
 void assemble(const htg_params &prm, const htg_generator &gen, int nt, const double *xt, int ns, const double *xs,
               Assembly &out);

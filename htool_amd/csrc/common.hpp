@@ -104,6 +104,7 @@ struct Leaf {
     uint64_t src_u = 0, src_v = 0; // build path: U (m x k col-major) and V (k x n row-major) in the store
     int k_hi = 0;                  // low-rank: terms stored in class 0 (the rest in class 1)
     std::vector<double> sig;       // low-rank: per-term scale (empty: all 1)
+    bool mirror = false;           // symmetric storage: transposed copy of a kept off-diagonal leaf
 };
 
 // Stored row length of a low-rank leaf's V: the global columns

@@ -79,6 +79,7 @@ struct htg_hmatrix {
     int aca_retry = 0, aca_wide = 0;
     int64_t aca_ws_entries = 0, aca_need_entries = 0;
     double aca_seconds = 0;
+    char symmetry = 'N', uplo = 'N'; // symmetric storage keeps one triangle; `leaves` adds mirrored copies
     bool scaled = false; // some low-rank terms carry scales (recompressed)
 
     std::vector<int> rb_row0, rb_order;
@@ -667,6 +668,25 @@ void locate_part(htg_hmatrix &h, Part &p, int n) {
     p.leaf_end   = (size_t)(hi - h.leaves.begin());
 }
 
+// Symmetric storage: the product also applies the transpose of every kept
+// off-diagonal leaf (hmatrix.hpp:876-904). Each gets a mirrored copy that
+// shares its data: dense entries are regenerated from the symmetric kernel,
+// low-rank copies exchange U and V (V, k x n row-major, is the n x k
+// column-major U of the transpose, and back).
+void add_mirrors(std::vector<Leaf> &leaves, int first) {
+    const size_t n = leaves.size();
+    for (size_t l = (size_t)first; l < n; l++) {
+        if (leaves[l].oi == leaves[l].oj)
+            continue;
+        Leaf M = leaves[l];
+        std::swap(M.oi, M.oj);
+        std::swap(M.m, M.n);
+        std::swap(M.src_u, M.src_v);
+        M.mirror = true;
+        leaves.push_back(std::move(M));
+    }
+}
+
 void take_aca_stats(htg_hmatrix &h, Assembly &A) {
     h.false_positive += A.false_positive;
     h.aca_feat.insert(h.aca_feat.end(), A.aca_feat.begin(), A.aca_feat.end());
@@ -688,6 +708,13 @@ void take_aca_stats(htg_hmatrix &h, Assembly &A) {
 namespace htg {
 namespace {
 void validate_params(const htg_params &prm) {
+    // symmetry and storage triangle as HMatrix's constructor checks them (hmatrix.hpp:116-121)
+    const int sy = prm.symmetry == 0 ? 'N' : prm.symmetry, up = prm.uplo == 0 ? 'N' : prm.uplo;
+    if (!((sy == 'N' || sy == 'S' || sy == 'H') && (up == 'N' || up == 'L' || up == 'U') &&
+          ((sy == 'N' && up == 'N') || (sy != 'N' && up != 'N'))))
+        throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments to create HMatrix");
+    if (sy == 'H')
+        throw Error(HTG_ERR_UNSUPPORTED, "Hermitian storage: none of the kernel families is Hermitian");
     if (prm.minclustersize <= 0)
         throw Error(HTG_ERR_INVALID, "[Htool error] MinClusterSize parameter cannot be zero");
     if (prm.maxblocksize < 1)
@@ -734,6 +761,8 @@ int htg_hmatrix_build(const htg_params *prm, const htg_generator *gen, int nt, c
             throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments to build HMatrix");
         validate_params(*prm);
         const bool same = gen->same_points || xs == nullptr;
+        if (sym_uplo(*prm) != 'N' && !same)
+            throw Error(HTG_ERR_INVALID, "[Htool error] symmetric storage needs a single point set");
         if (same && ns != nt && ns > 0)
             throw Error(HTG_ERR_INVALID, "[Htool error] Invalid size in arguments for building HMatrix");
         if (!same && ns <= 0)
@@ -751,6 +780,10 @@ int htg_hmatrix_build(const htg_params *prm, const htg_generator *gen, int nt, c
         h->perm_t = A.perm_t;
         h->perm_s = A.perm_s;
         h->leaves = A.leaves;
+        h->uplo     = sym_uplo(*prm);
+        h->symmetry = h->uplo == 'N' ? 'N' : 'S';
+        if (h->uplo != 'N')
+            add_mirrors(h->leaves, 0);
         take_aca_stats(*h, A);
         sort_leaves(h->leaves);
         part.leaf_begin = 0;
@@ -877,11 +910,14 @@ int htg_hmatrix_build_batch(const htg_params *prm, int count, const htg_generato
                 h->perm_t.push_back(v + shift);
             for (int v : A.perm_s)
                 h->perm_s.push_back(v + shift);
+            const size_t first = h->leaves.size();
             for (Leaf L : A.leaves) {
                 L.oi += shift;
                 L.oj += shift;
                 h->leaves.push_back(L);
             }
+            if (sym_uplo(*prm) != 'N')
+                add_mirrors(h->leaves, (int)first);
             std::vector<Leaf>().swap(A.leaves);
             take_aca_stats(*h, A);
             part.shift = shift;
@@ -889,6 +925,8 @@ int htg_hmatrix_build_batch(const htg_params *prm, int count, const htg_generato
             shift += n[i];
         }
         h->nr = h->nc = shift;
+        h->uplo     = sym_uplo(*prm);
+        h->symmetry = h->uplo == 'N' ? 'N' : 'S';
         sort_leaves(h->leaves);
         for (int i = 0; i < count; i++)
             locate_part(*h, h->parts[i], n[i]);
@@ -917,6 +955,8 @@ int htg_hmatrix_transpose(const htg_hmatrix *h, htg_hmatrix **out) {
         t->perm_t         = h->perm_s;
         t->perm_s         = h->perm_t;
         t->false_positive = h->false_positive;
+        t->symmetry       = h->symmetry;
+        t->uplo           = h->uplo == 'L' ? 'U' : (h->uplo == 'U' ? 'L' : 'N');
         t->leaves.reserve(h->leaves.size());
         for (Leaf L : h->leaves) {
             std::swap(L.oi, L.oj);
@@ -1073,6 +1113,12 @@ int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info) {
         long long ksum = 0;
         bool first     = true;
         for (const Leaf &L : h->leaves) {
+            if (L.mirror) {
+                r.mirrored_entries += L.k < 0 ? (int64_t)L.m * L.n : (int64_t)L.k * (L.m + L.n);
+                if (L.k > 0)
+                    r.mirrored_lowprec_entries += (int64_t)(L.k - L.k_hi) * (L.m + L.n);
+                continue;
+            }
             if (L.k < 0) {
                 r.n_dense++;
                 r.dense_entries += (int64_t)L.m * L.n;
@@ -1102,6 +1148,8 @@ int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info) {
         r.aca_seconds       = h->aca_seconds;
         r.is_complex        = h->cplx ? 1 : 0;
         r.n_aca_wide        = h->aca_wide;
+        r.symmetry          = h->symmetry;
+        r.uplo              = h->uplo;
         *info               = r;
     });
 }
@@ -1129,10 +1177,15 @@ int htg_hmatrix_nleaves(const htg_hmatrix *h, int *nleaves, int64_t *data_size) 
         if (!h)
             throw Error(HTG_ERR_INVALID, "null handle");
         int64_t s = 0;
-        for (const Leaf &L : h->leaves)
+        int cnt   = 0;
+        for (const Leaf &L : h->leaves) {
+            if (L.mirror)
+                continue;
             s += L.k < 0 ? (int64_t)L.m * L.n : (int64_t)L.k * (L.m + L.n);
+            cnt++;
+        }
         if (nleaves)
-            *nleaves = (int)h->leaves.size();
+            *nleaves = cnt;
         if (data_size)
             *data_size = s;
     });
@@ -1142,10 +1195,12 @@ int htg_hmatrix_get_leaves(const htg_hmatrix *h, int *leaves5, double *data) {
     return guard([&] {
         if (!h || !leaves5)
             throw Error(HTG_ERR_INVALID, "null argument");
-        for (size_t l = 0; l < h->leaves.size(); l++) {
-            const Leaf &L = h->leaves[l];
-            int v[5]      = {L.oi, L.m, L.oj, L.n, L.k};
-            std::memcpy(leaves5 + 5 * l, v, sizeof(v));
+        size_t o = 0;
+        for (const Leaf &L : h->leaves) {
+            if (L.mirror)
+                continue;
+            int v[5] = {L.oi, L.m, L.oj, L.n, L.k};
+            std::memcpy(leaves5 + 5 * o++, v, sizeof(v));
         }
         if (!data)
             return;
@@ -1166,7 +1221,7 @@ int htg_hmatrix_get_leaves(const htg_hmatrix *h, int *leaves5, double *data) {
         };
         for (size_t l = 0; l < h->leaves.size(); l++) {
             const Leaf &L = h->leaves[l];
-            if (L.k == 0)
+            if (L.k == 0 || L.mirror)
                 continue;
             const int ncol = L.k < 0 ? L.n : L.k;
             // U (or the dense block): column j lives in class 0 (j < k_hi, or dense) or class 1.

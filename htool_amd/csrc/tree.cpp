@@ -240,20 +240,88 @@ struct TaskBuilder {
     }
 };
 
+// The symmetric traversal (blocks.hpp:140-223): tasks and merges only on
+// coinciding or disjoint clusters; when neither side is a leaf every pair of
+// sons is visited (son index p + l * nt for target son p, source son l).
+struct SymTaskBuilder {
+    const ClusterTree &T, &S;
+    double eta;
+    std::size_t maxblocksize;
+    std::vector<BlockTask> tasks;
+
+    bool visit(int t, int s) {
+        const ClusterNode &a = T.nodes[t], &b = S.nodes[s];
+        const bool ok        = sym_compatible(a, b);
+        if (admissible(a, b, eta) && a.rank >= 0 && ok) {
+            tasks.push_back({t, s, true});
+            return false;
+        }
+        if (a.leaf() && b.leaf())
+            return true;
+        std::vector<std::pair<int, int>> kids;
+        if (b.leaf()) {
+            for (int p = 0; p < a.nb_sons; p++)
+                kids.push_back({a.first_son + p, s});
+        } else if (a.leaf()) {
+            for (int p = 0; p < b.nb_sons; p++)
+                kids.push_back({t, b.first_son + p});
+        } else {
+            for (int l = 0; l < b.nb_sons; l++)
+                for (int p = 0; p < a.nb_sons; p++)
+                    kids.push_back({a.first_son + p, b.first_son + l});
+        }
+        std::vector<char> np(kids.size());
+        bool all_not_pushed = true;
+        for (size_t k = 0; k < kids.size(); k++) {
+            np[k]          = visit(kids[k].first, kids[k].second);
+            all_not_pushed = all_not_pushed && np[k];
+        }
+        const std::size_t bsize = std::size_t(a.size) * std::size_t(b.size);
+        if (bsize <= maxblocksize && all_not_pushed && a.rank >= 0 && ok)
+            return true;
+        for (size_t k = 0; k < kids.size(); k++)
+            if (np[k])
+                tasks.push_back({kids[k].first, kids[k].second,
+                                 admissible(T.nodes[kids[k].first], S.nodes[kids[k].second], eta)});
+        return false;
+    }
+};
+
 } // namespace
 
+bool sym_compatible(const ClusterNode &t, const ClusterNode &s) {
+    if (t.offset == s.offset)
+        return t.size == s.size;
+    return t.offset < s.offset ? s.offset - t.offset >= t.size : t.offset - s.offset >= s.size;
+}
+
 std::vector<BlockTask> build_block_tasks(const ClusterTree &T, const ClusterTree &S, double eta,
-                                         std::size_t maxblocksize) {
+                                         std::size_t maxblocksize, char uplo) {
     if (maxblocksize == 0)
         throw std::invalid_argument("maxblocksize cannot be zero");
-    TaskBuilder tb{T, S, eta, maxblocksize, {}};
-    if (tb.visit(0, 0))
-        tb.tasks.push_back({0, 0, admissible(T.nodes[0], S.nodes[0], eta)});
-    // Only blocks whose target lies in this process's depth-1 cluster (rank 0) are local.
+    std::vector<BlockTask> all;
+    if (uplo == 'L' || uplo == 'U') {
+        SymTaskBuilder tb{T, S, eta, maxblocksize, {}};
+        if (tb.visit(0, 0))
+            tb.tasks.push_back({0, 0, admissible(T.nodes[0], S.nodes[0], eta)});
+        all.swap(tb.tasks);
+    } else {
+        TaskBuilder tb{T, S, eta, maxblocksize, {}};
+        if (tb.visit(0, 0))
+            tb.tasks.push_back({0, 0, admissible(T.nodes[0], S.nodes[0], eta)});
+        all.swap(tb.tasks);
+    }
+    // Only blocks whose target lies in this process's depth-1 cluster (rank 0)
+    // are local (blocks.hpp:258-270); symmetric storage keeps one triangle.
     std::vector<BlockTask> local;
-    for (const BlockTask &bt : tb.tasks)
-        if (T.nodes[bt.t].rank == 0)
-            local.push_back(bt);
+    for (const BlockTask &bt : all) {
+        if (T.nodes[bt.t].rank != 0)
+            continue;
+        const int to = T.nodes[bt.t].offset, so = S.nodes[bt.s].offset;
+        if ((uplo == 'L' && so > to) || (uplo == 'U' && so < to))
+            continue;
+        local.push_back(bt);
+    }
     std::sort(local.begin(), local.end(), [&](const BlockTask &u, const BlockTask &v) {
         const int tu = T.nodes[u.t].offset, tv = T.nodes[v.t].offset;
         return tu == tv ? S.nodes[u.s].offset < S.nodes[v.s].offset : tu < tv;

@@ -39,8 +39,17 @@ struct BlockTask {
 };
 
 // Block tree traversal producing the sorted list of tasks of the (single)
-// process (include/htool/blocks/blocks.hpp:37-139, 237-278).
+// process (include/htool/blocks/blocks.hpp:37-139, 237-278). uplo 'L' or 'U'
+// selects the symmetric traversal (blocks.hpp:140-223: a block is a task or a
+// merged dense block only when its clusters coincide or their index ranges do
+// not overlap, and both sides split at once) and keeps the tasks of one
+// triangle (s.offset <= t.offset for 'L', >= for 'U'); any other value the
+// ordinary traversal.
 std::vector<BlockTask> build_block_tasks(const ClusterTree &T, const ClusterTree &S, double eta,
-                                         std::size_t maxblocksize);
+                                         std::size_t maxblocksize, char uplo = 'N');
+
+// Both clusters are the same, or their index ranges are disjoint: the blocks
+// a symmetric block tree may keep (blocks.hpp:152, 170).
+bool sym_compatible(const ClusterNode &t, const ClusterNode &s);
 
 } // namespace htg

@@ -38,12 +38,20 @@ _BYTES = {"fp64": (8, 8), "fp32": (4, 4), "fp32_fp16": (4, 2), "fp32_bf16": (4, 
 class HMatrix:
     """H-matrix on the GPU.
 
-    HMatrix(epsilon, eta, reqrank) mirrors HMatrix<T>(t, s, epsilon, eta, 'N', 'N', reqrank)
-    (include/htool/types/hmatrix.hpp:115-122); the cluster trees are built inside
+    HMatrix(epsilon, eta, reqrank, symmetry=..., UPLO=...) mirrors HMatrix<T>(t, s, epsilon, eta,
+    symmetry, UPLO, reqrank) (include/htool/types/hmatrix.hpp:115-122): symmetry 'S' with UPLO
+    'L' or 'U' keeps one triangle of blocks (single point set; the product applies each kept
+    off-diagonal block and its transpose); the cluster trees are built inside
     `build` from the points and `minclustersize` (cluster.hpp:57-92, 334).
     """
 
-    def __init__(self, epsilon: float = 1e-6, eta: float = 10.0, reqrank: int = -1, minclustersize: int = 10):
+    def __init__(self, epsilon: float = 1e-6, eta: float = 10.0, reqrank: int = -1, minclustersize: int = 10,
+                 symmetry: str = "N", UPLO: str = "N"):
+        if not (symmetry in "NSH" and UPLO in "NLU" and len(symmetry) == 1 and len(UPLO) == 1
+                and ((symmetry == "N") == (UPLO == "N"))):
+            raise ValueError("[Htool error] Invalid arguments to create HMatrix")
+        self.symmetry = symmetry
+        self.UPLO = UPLO
         self.epsilon = float(epsilon)
         self.eta = float(eta)
         self.reqrank = int(reqrank)
@@ -118,7 +126,7 @@ class HMatrix:
         return capi.htg_params(self.epsilon, self.eta, self.maxblocksize, self.minclustersize, self.compressor,
                                self.reqrank, int(self.lowprec_all), self.recompress_tol, self.lowprec_tol,
                                self.predictor.handle if self.predictor is not None else None,
-                               self.predictor_margin, self.aca_wide_min)
+                               self.predictor_margin, self.aca_wide_min, ord(self.symmetry), ord(self.UPLO))
 
     def build_batch(self, gens, point_sets) -> "HMatrix":
         """Many independent square H-matrices as one block-diagonal H-matrix
@@ -321,12 +329,14 @@ class HMatrix:
         i = self.get_info()
         s0, s1 = _BYTES[storage or self._storage]
         e = 2 if i.is_complex else 1  # a complex entry is two values
-        stored = e * (s0 * (i.dense_entries + i.lowrank_entries - i.lowprec_entries) + s1 * i.lowprec_entries)
+        # symmetric storage: the product also streams the mirrored copies
+        hi = i.dense_entries + i.lowrank_entries - i.lowprec_entries + i.mirrored_entries - i.mirrored_lowprec_entries
+        stored = e * (s0 * hi + s1 * (i.lowprec_entries + i.mirrored_lowprec_entries))
         return int(stored + e * mu * (i.nc + i.nr) * 8 + 4 * (i.nr + i.nc))
 
     def flops(self, mu: int = 1) -> int:
         i = self.get_info()
-        return int((8 if i.is_complex else 2) * mu * (i.dense_entries + i.lowrank_entries))
+        return int((8 if i.is_complex else 2) * mu * (i.dense_entries + i.lowrank_entries + i.mirrored_entries))
 
 
 class ProductGraph:

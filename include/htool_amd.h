@@ -101,6 +101,10 @@ typedef struct htg_params {
     double predictor_margin;        /* room above the prediction: cap = ceil((1 + margin) t) + 2 */
     int aca_wide_min;               /* blocks with M + N >= this run the multi-workgroup ACA (same
                                        factors and ranks); 0: 16384, < 0: never */
+    int symmetry;                   /* 'N' (or 0): general; 'S': symmetric storage, one triangle of
+                                       blocks kept (HMatrix(..., 'S', UPLO), hmatrix.hpp:115-122);
+                                       'H' is rejected (no Hermitian kernel family) */
+    int uplo;                       /* 'L' or 'U' with 'S' (the kept triangle); 'N' (or 0) otherwise */
 } htg_params;
 
 typedef struct htg_info {
@@ -123,6 +127,11 @@ typedef struct htg_info {
     double aca_seconds;       /* batched ACA (kernels and host scheduling) */
     int is_complex;           /* entries, x and y are complex<double> */
     int n_aca_wide;           /* ACA runs on the multi-workgroup path (htg_params.aca_wide_min) */
+    int symmetry, uplo;       /* as built ('N', or 'S' with the kept triangle 'L' / 'U') */
+    int64_t mirrored_entries; /* symmetric storage: entries of the transposed copies of the kept
+                                 off-diagonal leaves that the product also streams (the counts
+                                 above are of the kept leaves, as the reference's) */
+    int64_t mirrored_lowprec_entries; /* the part of mirrored_entries in class 1 */
 } htg_info;
 
 /* Work decomposition of the product (no counterpart in the reference): rows

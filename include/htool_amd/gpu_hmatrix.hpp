@@ -38,8 +38,11 @@ struct Generator {
 
 class GpuHMatrix {
   public:
-    // HMatrix<T>(t, s, epsilon, eta, 'N', 'N', reqrank), hmatrix.hpp:115-122.
-    explicit GpuHMatrix(double epsilon = 1e-6, double eta = 10, int reqrank = -1) {
+    // HMatrix<T>(t, s, epsilon, eta, symmetry, UPLO, reqrank), hmatrix.hpp:115-122;
+    // symmetry 'S' with UPLO 'L' / 'U' keeps one triangle of blocks (the
+    // arguments are checked by the build, as the reference's constructor does).
+    explicit GpuHMatrix(double epsilon = 1e-6, double eta = 10, char symmetry = 'N', char UPLO = 'N',
+                        int reqrank = -1) {
         prm_.epsilon        = epsilon;
         prm_.eta            = eta;
         prm_.maxblocksize   = 1e6;
@@ -52,6 +55,8 @@ class GpuHMatrix {
         prm_.predictor        = nullptr;
         prm_.predictor_margin = 0.25;
         prm_.aca_wide_min     = 0;
+        prm_.symmetry         = symmetry;
+        prm_.uplo             = UPLO;
     }
     GpuHMatrix(const GpuHMatrix &) = delete;
     GpuHMatrix &operator=(const GpuHMatrix &) = delete;

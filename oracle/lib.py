@@ -24,7 +24,7 @@ def lib():
             "orc_kernel_block": (None, [_I, _D, _D, _I, _VP, _VP, _I, _I, _VP, _VP, _VP]),
             "orc_cluster_build": (_I, [_I, _VP, _I, _VP, _VP, _VP]),
             "orc_aca": (_I, [_I, _D, _I, _I, _VP, _VP, _I, _D, _D, _I, _VP, _VP, _VP, _I, _VP, _I, _I, _I, _VP, _VP]),
-            "orc_hmat_build": (_VP, [_I, _VP, _I, _VP, _I, _D, _D, _I, _D, _D, _I, _I, _D]),
+            "orc_hmat_build": (_VP, [_I, _VP, _I, _VP, _I, _D, _D, _I, _D, _D, _I, _I, _D, _I]),
             "orc_hmat_nleaves": (_I, [_VP]),
             "orc_hmat_leaves": (None, [_VP, _VP]),
             "orc_hmat_data_size": (C.c_longlong, [_VP]),
@@ -38,6 +38,7 @@ def lib():
             "orc_mlp_loss_grad": (None, [_VP, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
             "orc_hmat_is_complex": (_I, [_VP]),
             "orc_leaf_mvprod_c": (None, [_I, _I, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I]),
+            "orc_leaf_mvprod_sym": (None, [_I, _VP, _I, _VP, _VP, _VP, _VP, _I, _I, _I]),
             "orc_dense_rows_c": (None, [_I, _D, _D, _I, _VP, _VP, _I, _I, _VP, _VP, _I, _VP]),
             "orc_mlp_predict": (None, [_VP, _VP, _VP, _VP, _I, _VP]),
         }
@@ -109,15 +110,22 @@ class OracleHMatrix:
     leaves: np.ndarray
     data: np.ndarray
     false_positive: int
+    uplo: str = "N"
 
     def mvprod(self, x):
+        if self.uplo != "N":
+            return leaf_mvprod_sym(self.nr, self.perm_t, self.leaves, self.data, x, self.uplo)
         return leaf_mvprod(self.nr, self.nc, self.perm_t, self.perm_s, self.leaves, self.data, x)
 
 
 def hmat_build(xt, xs=None, *, kernel=0, delta=0.0, diag=0.0, minclustersize=10, eta=10.0, eps=1e-6, sym=False,
-               reqrank=-1, maxblocksize=1e6, wavenumber=0.0) -> OracleHMatrix:
+               reqrank=-1, maxblocksize=1e6, wavenumber=0.0, uplo="N") -> OracleHMatrix:
     """Complex kernel families (kernel 4, 5): data is complex128 and mvprod takes
-    complex vectors; the Helmholtz wavenumber travels in `delta`."""
+    complex vectors; the Helmholtz wavenumber travels in `delta`. uplo 'L' / 'U':
+    symmetric storage (HMatrix(..., 'S', uplo), one point set): the leaves of one
+    triangle, and a product that also applies their transposes."""
+    if uplo != "N" and xs is not None:
+        raise ValueError("symmetric storage needs one point set")
     if kernel == 4 and wavenumber:
         delta = wavenumber
     L = lib()
@@ -126,7 +134,7 @@ def hmat_build(xt, xs=None, *, kernel=0, delta=0.0, diag=0.0, minclustersize=10,
     nt = xt.shape[0]
     ns = nt if xs is None else xs_.shape[0]
     h = L.orc_hmat_build(nt, _p(xt), ns, _p(xs_), kernel, delta, diag, minclustersize, eta, eps, int(sym), reqrank,
-                         maxblocksize)
+                         maxblocksize, ord(uplo))
     try:
         n = L.orc_hmat_nleaves(h)
         leaves = np.zeros((n, 5), np.int32)
@@ -142,7 +150,7 @@ def hmat_build(xt, xs=None, *, kernel=0, delta=0.0, diag=0.0, minclustersize=10,
         fp = L.orc_hmat_false_positives(h)
     finally:
         L.orc_hmat_free(h)
-    return OracleHMatrix(nt, ns, pt, ps, leaves, data, fp)
+    return OracleHMatrix(nt, ns, pt, ps, leaves, data, fp, uplo)
 
 
 def leaf_mvprod(nr, nc, perm_t, perm_s, leaves, data, x):
@@ -159,6 +167,24 @@ def leaf_mvprod(nr, nc, perm_t, perm_s, leaves, data, x):
     perm_s = np.ascontiguousarray(perm_s, np.int32)
     fn = lib().orc_leaf_mvprod_c if cplx else lib().orc_leaf_mvprod
     fn(nr, nc, _p(perm_t), _p(perm_s), leaves.shape[0], _p(leaves), _p(data), _p(xm), _p(y), mu)
+    return y[:, 0].copy() if vec else np.ascontiguousarray(y)
+
+
+def leaf_mvprod_sym(n, perm, leaves, data, x, uplo):
+    """Product of a symmetric-storage leaf list (one triangle): kept leaves, the
+    transposes of the off-diagonal ones, symv on the diagonal leaves."""
+    cplx = np.iscomplexobj(data)
+    dt = np.complex128 if cplx else np.float64
+    x = np.asarray(x, dt)
+    vec = x.ndim == 1
+    xm = np.asfortranarray(x.reshape(n, -1))
+    mu = xm.shape[1]
+    y = np.zeros((n, mu), dtype=dt, order="F")
+    leaves = np.ascontiguousarray(leaves, np.int32)
+    data = np.ascontiguousarray(data, dt)
+    perm = np.ascontiguousarray(perm, np.int32)
+    lib().orc_leaf_mvprod_sym(n, _p(perm), leaves.shape[0], _p(leaves), _p(data), _p(xm), _p(y), mu, ord(uplo),
+                              int(cplx))
     return y[:, 0].copy() if vec else np.ascontiguousarray(y)
 
 

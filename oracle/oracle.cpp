@@ -442,6 +442,7 @@ struct Builder {
     double eta, eps;
     size_t maxblocksize;
     int sym_aca, reqrank;
+    char uplo = 'N'; // 'L' / 'U': symmetric block tree and false-positive recursion
 
     // RjasanowSteinbach (H/blocks/admissibility_conditions.hpp:16-19)
     bool admissible(int t, int s) const {
@@ -480,6 +481,57 @@ struct Builder {
         for (size_t p = 0; p < sons.size(); p++)
             if (not_pushed[p])
                 tasks.push_back(split_t ? std::make_pair(sons[p], s) : std::make_pair(t, sons[p]));
+        return false;
+    }
+
+    // Coinciding or non-overlapping clusters (blocks.hpp:152, 170).
+    bool sym_ok(int t, int s) const {
+        const Node &a = Tn[t], &b = Sn[s];
+        if (a.offset == b.offset)
+            return a.size == b.size;
+        return a.offset < b.offset ? b.offset - a.offset >= a.size : a.offset - b.offset >= b.size;
+    }
+
+    // Sons of (t, s) in the symmetric recursions: one side when the other is a
+    // leaf, else every pair, target son fastest (blocks.hpp:190-198,
+    // hmatrix.hpp:672-677).
+    std::vector<std::pair<int, int>> sym_sons(int t, int s) const {
+        const Node &a = Tn[t], &b = Sn[s];
+        std::vector<std::pair<int, int>> out;
+        if (b.sons.empty()) {
+            for (int ts : a.sons)
+                out.emplace_back(ts, s);
+        } else if (a.sons.empty()) {
+            for (int ss : b.sons)
+                out.emplace_back(t, ss);
+        } else {
+            for (int ss : b.sons)
+                for (int ts : a.sons)
+                    out.emplace_back(ts, ss);
+        }
+        return out;
+    }
+
+    // build_sym_block_tree: returns true when the block was not pushed (blocks.hpp:140-223).
+    bool sym_block_tree(int t, int s, std::vector<std::pair<int, int>> &tasks) const {
+        const Node &a = Tn[t], &b = Sn[s];
+        size_t bsize  = size_t(a.size) * size_t(b.size);
+        if (admissible(t, s) && a.rank >= 0 && sym_ok(t, s)) {
+            tasks.emplace_back(t, s);
+            return false;
+        }
+        if (a.sons.empty() && b.sons.empty())
+            return true;
+        const auto kids = sym_sons(t, s);
+        std::vector<bool> not_pushed(kids.size());
+        for (size_t p = 0; p < kids.size(); p++)
+            not_pushed[p] = sym_block_tree(kids[p].first, kids[p].second, tasks);
+        if (bsize <= maxblocksize && std::all_of(not_pushed.begin(), not_pushed.end(), [](bool v) { return v; }) &&
+            a.rank >= 0 && sym_ok(t, s))
+            return true;
+        for (size_t p = 0; p < kids.size(); p++)
+            if (not_pushed[p])
+                tasks.push_back(kids[p]);
         return false;
     }
 
@@ -529,6 +581,21 @@ struct Builder {
         }
         const Node &a = Tn[t], &b = Sn[s];
         size_t bsize  = size_t(a.size) * size_t(b.size);
+        if (uplo != 'N') { // ComputeAdmissibleBlocksSym (hmatrix.hpp:610-690)
+            if (a.sons.empty() && b.sons.empty())
+                return true;
+            const auto kids = sym_sons(t, s);
+            std::vector<bool> np(kids.size());
+            for (size_t p = 0; p < kids.size(); p++)
+                np[p] = admissible_block(kids[p].first, kids[p].second,
+                                         admissible(kids[p].first, kids[p].second), out, fp);
+            if (bsize <= maxblocksize && std::all_of(np.begin(), np.end(), [](bool v) { return v; }))
+                return true;
+            for (size_t p = 0; p < kids.size(); p++)
+                if (np[p])
+                    dense(kids[p].first, kids[p].second, out);
+            return false;
+        }
         bool split_t;
         if (b.sons.empty()) {
             if (a.sons.empty())
@@ -559,6 +626,7 @@ struct Builder {
 struct orc_hmat {
     int nr = 0, nc = 0, false_positive = 0;
     bool cplx = false;
+    char uplo = 'N'; // symmetric storage: the kept triangle
     std::vector<int> perm_t, perm_s;
     std::vector<Leaf<double>> leaves;
     std::vector<Leaf<cd>> cleaves;
@@ -635,10 +703,13 @@ cd load<cd>(const double *p, size_t i) {
 // mymvprod_global_to_local (843-911) and the leaf products of
 // Matrix::add_mvprod_row_major (matrix.hpp:353-383) and
 // LowRankMatrix::add_mvprod_row_major (lrmat.hpp:89-100), one RHS column at a
-// time. T = cd: data, x and y hold (re, im) pairs.
+// time. T = cd: data, x and y hold (re, im) pairs. uplo 'L' / 'U' (symmetric
+// storage, hmatrix.hpp:864-904): every off-diagonal leaf also adds its
+// transpose (no conjugation, 'S'), a diagonal dense leaf is applied as symv
+// on its `uplo` triangle (matrix.hpp:386-447).
 template <typename T>
 void leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
-                 const double *data, const double *x, double *y, int mu) {
+                 const double *data, const double *x, double *y, int mu, char uplo = 'N') {
     std::vector<size_t> off(nleaves);
     size_t p = 0;
     for (int l = 0; l < nleaves; l++) {
@@ -655,6 +726,31 @@ void leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int nleav
             const int *L = leaves5 + 5 * l;
             const int oi = L[0], m = L[1], oj = L[2], n = L[3], k = L[4];
             const size_t d = off[l];
+            if (uplo != 'N' && oi == oj && k < 0) {
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++) {
+                        const bool keep = uplo == 'L' ? i >= j : i <= j; // entry (i, j) of the kept triangle
+                        const T a = keep ? load<T>(data, d + i + size_t(m) * j) : load<T>(data, d + j + size_t(m) * i);
+                        yout[oi + i] += a * xin[oj + j];
+                    }
+                continue;
+            }
+            if (uplo != 'N') { // the transpose of an off-diagonal leaf: y_J += A^T x_I
+                if (k < 0) {
+                    for (int j = 0; j < n; j++)
+                        for (int i = 0; i < m; i++)
+                            yout[oj + j] += load<T>(data, d + i + size_t(m) * j) * xin[oi + i];
+                } else {
+                    const size_t U = d, V = d + size_t(m) * k;
+                    t.assign(k, T(0));
+                    for (int r = 0; r < k; r++)
+                        for (int i = 0; i < m; i++)
+                            t[r] += load<T>(data, U + i + size_t(m) * r) * xin[oi + i];
+                    for (int j = 0; j < n; j++)
+                        for (int r = 0; r < k; r++)
+                            yout[oj + j] += load<T>(data, V + r + size_t(k) * j) * t[r];
+                }
+            }
             if (k < 0) {
                 for (int j = 0; j < n; j++) {
                     T xj = xin[oj + j];
@@ -724,7 +820,7 @@ int orc_aca(int sym, double eps, int M, int N, const int *rows, const int *cols,
 }
 
 orc_hmat *orc_hmat_build(int nt, const double *xt, int ns, const double *xs, int kernel, double delta, double diag,
-                         int mcs, double eta, double eps, int sym_aca, int reqrank, double maxblocksize) {
+                         int mcs, double eta, double eps, int sym_aca, int reqrank, double maxblocksize, int uplo) {
     auto *h = new orc_hmat;
     h->nr   = nt;
     h->nc   = ns;
@@ -738,18 +834,25 @@ orc_hmat *orc_hmat_build(int nt, const double *xt, int ns, const double *xs, int
         build_cluster(ns, xs, mcs, S, h->perm_s);
     }
     Kernel A{kernel, delta, diag, same, xt, same ? xt : xs};
-    Builder<double> B{T, S, h->perm_t, h->perm_s, A, eta, eps, (size_t)maxblocksize, sym_aca, reqrank};
+    h->uplo = (uplo == 'L' || uplo == 'U') ? (char)uplo : 'N';
+    Builder<double> B{T, S, h->perm_t, h->perm_s, A, eta, eps, (size_t)maxblocksize, sym_aca, reqrank, h->uplo};
 
     // Block::build (blocks.hpp:237-278): root admissibility, tree, root task,
-    // local tasks of rank 0, sorted by (target offset, source offset).
+    // local tasks of rank 0 (of the kept triangle with uplo), sorted by
+    // (target offset, source offset).
     std::vector<std::pair<int, int>> tasks;
-    bool not_pushed = B.block_tree(0, 0, tasks);
+    bool not_pushed = h->uplo != 'N' ? B.sym_block_tree(0, 0, tasks) : B.block_tree(0, 0, tasks);
     if (not_pushed)
         tasks.emplace_back(0, 0);
     std::vector<std::pair<int, int>> local;
-    for (auto &ts : tasks)
-        if (T[ts.first].rank == 0)
-            local.push_back(ts);
+    for (auto &ts : tasks) {
+        if (T[ts.first].rank != 0)
+            continue;
+        const int to = T[ts.first].offset, so = S[ts.second].offset;
+        if ((h->uplo == 'L' && so > to) || (h->uplo == 'U' && so < to))
+            continue;
+        local.push_back(ts);
+    }
     std::sort(local.begin(), local.end(), [&](const std::pair<int, int> &a, const std::pair<int, int> &b) {
         if (T[a.first].offset == T[b.first].offset)
             return S[a.second].offset < S[b.second].offset;
@@ -757,7 +860,7 @@ orc_hmat *orc_hmat_build(int nt, const double *xt, int ns, const double *xs, int
     });
     h->cplx = kernel >= 4;
     if (h->cplx) {
-        Builder<cd> BC{T, S, h->perm_t, h->perm_s, A, eta, eps, (size_t)maxblocksize, sym_aca, reqrank};
+        Builder<cd> BC{T, S, h->perm_t, h->perm_s, A, eta, eps, (size_t)maxblocksize, sym_aca, reqrank, h->uplo};
         compute_blocks(BC, local, h->cleaves, h->false_positive);
     } else {
         compute_blocks(B, local, h->leaves, h->false_positive);
@@ -822,6 +925,14 @@ void orc_leaf_mvprod_c(int nr, int nc, const int *perm_t, const int *perm_s, int
     leaf_mvprod<cd>(nr, nc, perm_t, perm_s, nleaves, leaves5, data, x, y, mu);
 }
 
+void orc_leaf_mvprod_sym(int n, const int *perm, int nleaves, const int *leaves5, const double *data,
+                         const double *x, double *y, int mu, int uplo, int cplx) {
+    if (cplx)
+        leaf_mvprod<cd>(n, n, perm, perm, nleaves, leaves5, data, x, y, mu, (char)uplo);
+    else
+        leaf_mvprod<double>(n, n, perm, perm, nleaves, leaves5, data, x, y, mu, (char)uplo);
+}
+
 void orc_hmat_mvprod(const orc_hmat *h, const double *x, double *y, int mu) {
     const int nl = orc_hmat_nleaves(h);
     std::vector<int> leaves(5 * nl);
@@ -829,10 +940,11 @@ void orc_hmat_mvprod(const orc_hmat *h, const double *x, double *y, int mu) {
     std::vector<double> data(orc_hmat_data_size(h) * (h->cplx ? 2 : 1));
     orc_hmat_data(h, data.data());
     if (h->cplx)
-        leaf_mvprod<cd>(h->nr, h->nc, h->perm_t.data(), h->perm_s.data(), nl, leaves.data(), data.data(), x, y, mu);
+        leaf_mvprod<cd>(h->nr, h->nc, h->perm_t.data(), h->perm_s.data(), nl, leaves.data(), data.data(), x, y, mu,
+                        h->uplo);
     else
         leaf_mvprod<double>(h->nr, h->nc, h->perm_t.data(), h->perm_s.data(), nl, leaves.data(), data.data(), x, y,
-                            mu);
+                            mu, h->uplo);
 }
 
 void orc_dense_rows(int kernel, double delta, double diag, int same_points, const double *xt, const double *xs,

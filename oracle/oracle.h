@@ -48,7 +48,8 @@ int orc_aca(int sym, double eps, int M, int N, const int *rows, const int *cols,
             const double *ctr_s, int off_s, int reqrank, int cap, double *U, double *V);
 
 orc_hmat *orc_hmat_build(int nt, const double *xt, int ns, const double *xs, int kernel, double delta, double diag,
-                         int minclustersize, double eta, double eps, int sym_aca, int reqrank, double maxblocksize);
+                         int minclustersize, double eta, double eps, int sym_aca, int reqrank, double maxblocksize,
+                         int uplo); /* 'L' / 'U': symmetric storage (one triangle, xs = NULL); else 'N' */
 int orc_hmat_nleaves(const orc_hmat *h);
 void orc_hmat_leaves(const orc_hmat *h, int *out5); /* off_i, m, off_j, n, rank(-1 dense); sorted by (off_i, off_j) */
 long long orc_hmat_data_size(const orc_hmat *h);
@@ -71,6 +72,11 @@ void orc_leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int n
 /* The same over complex leaf data and vectors, (re, im) pairs. */
 void orc_leaf_mvprod_c(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
                        const double *data, const double *x, double *y, int mu);
+/* Symmetric storage ('L' / 'U', square, one permutation): the kept leaves, the
+ * transposes of the off-diagonal ones (no conjugation) and symv on the kept
+ * triangle of the diagonal leaves; cplx: complex data and vectors. */
+void orc_leaf_mvprod_sym(int n, const int *perm, int nleaves, const int *leaves5, const double *data,
+                         const double *x, double *y, int mu, int uplo, int cplx);
 
 /* Brute-force dense rows, complex kernels, (re, im) pairs. */
 void orc_dense_rows_c(int kernel, double wavenumber, double diag, int same_points, const double *xt, const double *xs,

@@ -17,7 +17,8 @@
 //   int32  magic 0x48545246 ('HTRF'), version 1
 //   int32  mode (0 = H-matrix build + product, 1 = one leaf compression of the whole t x s block)
 //   int32  nt, ns, same_points, kernel, minclustersize, compressor (0 partialACA, 1 sympartialACA)
-//   int32  mu, nreps, dump_data, reqrank
+//   int32  mu, nreps, dump_data, reqrank, uplo (0: general; 'L' / 'U': HMatrix(..., 'S', uplo)
+//          built with the single-point-set build(A, xt), hmatrix.hpp:343-400)
 //   double eta, eps, delta, diag, maxblocksize
 //   double xt[3 nt]; double xs[3 ns] (absent when same_points); double x[ns mu] (column-major)
 // Complex kernel families (kernel 4: e^{ikr}/(4 pi r) with k = `delta`, `diag`
@@ -141,12 +142,15 @@ template <typename T>
 void run_hmatrix(std::shared_ptr<Cluster<PCAGeometricClustering>> t, std::shared_ptr<Cluster<PCAGeometricClustering>> s,
                  VirtualGenerator<T> &A, std::shared_ptr<VirtualLowRankGenerator<T>> comp, const double *xt,
                  const double *pxs, std::vector<T> &x, int nt, int mu, int nreps, int dump, int reqrank, double eps,
-                 double eta, double maxblocksize, const std::string &out, std::FILE *info) {
-    HMatrix<T> H(t, s, eps, eta, 'N', 'N', reqrank);
+                 double eta, double maxblocksize, const std::string &out, std::FILE *info, char uplo) {
+    HMatrix<T> H(t, s, eps, eta, uplo == 'N' ? 'N' : 'S', uplo, reqrank);
     H.set_compression(comp);
     H.set_maxblocksize(static_cast<unsigned int>(maxblocksize));
     double t0 = now();
-    H.build(A, xt, pxs);
+    if (uplo == 'N')
+        H.build(A, xt, pxs);
+    else
+        H.build(A, xt);
     double t_build = now() - t0;
 
     // Leaf list in save_plot order (near field, then far field).
@@ -214,6 +218,7 @@ int main(int argc, char **argv) {
     }
     const int mode = hdr[2], nt = hdr[3], ns = hdr[4], same = hdr[5], kernel = hdr[6], mcs = hdr[7];
     const int compressor = hdr[8], mu = hdr[9], nreps = hdr[10], dump = hdr[11], reqrank = hdr[12];
+    const char uplo = (hdr[13] == 'L' || hdr[13] == 'U') ? (char)hdr[13] : 'N';
     const double eta = prm[0], eps = prm[1], delta = prm[2], diag = prm[3], maxblocksize = prm[4];
     std::vector<double> xt(3 * size_t(nt)), xs, x(size_t(ns) * mu * (kernel >= 4 ? 2 : 1));
     in.read(reinterpret_cast<char *>(xt.data()), xt.size() * sizeof(double));
@@ -285,10 +290,10 @@ int main(int argc, char **argv) {
                                std::make_shared<partialACA<std::complex<double>>>())
                          : std::make_shared<sympartialACA<std::complex<double>>>();
         run_hmatrix<std::complex<double>>(t, s, Ac, compc, xt.data(), pxs, xc, nt, mu, nreps, dump, reqrank, eps, eta,
-                                          maxblocksize, out, info);
+                                          maxblocksize, out, info, uplo);
     } else {
         run_hmatrix<double>(t, s, A, comp, xt.data(), pxs, x, nt, mu, nreps, dump, reqrank, eps, eta, maxblocksize,
-                            out, info);
+                            out, info, uplo);
     }
     MPI_Finalize();
     return 0;

@@ -50,9 +50,10 @@ class RefResult:
 
 def run(xt, xs=None, x=None, *, kernel=0, eta=10.0, eps=1e-6, minclustersize=10, compressor=0,
         delta=0.0, diag=0.0, maxblocksize=1e6, mu=1, nreps=0, dump_data=True, reqrank=-1, mode=0,
-        threads=None, timeout=600, wavenumber=0.0) -> RefResult:
+        threads=None, timeout=600, wavenumber=0.0, uplo="N") -> RefResult:
     """kernel 4 (Helmholtz, wavenumber travels in the case file's delta) and 5
-    run the reference's HMatrix<std::complex<double>>."""
+    run the reference's HMatrix<std::complex<double>>. uplo 'L' / 'U': the
+    reference's symmetric storage HMatrix(..., 'S', uplo) (one point set)."""
     if kernel == 4 and wavenumber:
         delta = wavenumber
     if not available():
@@ -65,8 +66,10 @@ def run(xt, xs=None, x=None, *, kernel=0, eta=10.0, eps=1e-6, minclustersize=10,
     if x is None:
         x = np.zeros((ns, mu))
     x = np.asarray(x, dtype=np.complex128 if cplx else np.float64).reshape(ns, mu)
+    if uplo != "N" and not same:
+        raise ValueError("symmetric storage needs one point set")
     hdr = struct.pack("<14i", 0x48545246, 1, mode, nt, ns, int(same), kernel, minclustersize, compressor,
-                      mu, nreps, int(dump_data), reqrank, 0)
+                      mu, nreps, int(dump_data), reqrank, 0 if uplo == "N" else ord(uplo))
     prm = struct.pack("<5d", eta, eps, delta, diag, maxblocksize)
     with tempfile.TemporaryDirectory() as td:
         case = os.path.join(td, "case.bin")

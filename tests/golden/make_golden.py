@@ -7,7 +7,8 @@ data (all of it for the small cases, a sample of leaves for the larger ones) and
 y = H x. The fixtures are data only; tests/test_oracle.py checks the oracle
 against them on any machine.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py NAME ...   # only these
 """
 from __future__ import annotations
 
@@ -34,9 +35,14 @@ def leaf_slices(leaves):
     return out
 
 
-def save(name, *, xt, xs, kernel, delta, diag, mcs, eta, eps, sym, x, sample=None, **kw):
+ONLY = set(sys.argv[1:])
+
+
+def save(name, *, xt, xs, kernel, delta, diag, mcs, eta, eps, sym, x, sample=None, uplo="N", **kw):
+    if ONLY and name not in ONLY:
+        return
     r = refcase.run(xt, xs, x=x, kernel=kernel, eta=eta, eps=eps, minclustersize=mcs, compressor=int(sym),
-                    delta=delta, diag=diag, mu=x.shape[1], nreps=0, dump_data=True)
+                    delta=delta, diag=diag, mu=x.shape[1], nreps=0, dump_data=True, uplo=uplo)
     order = np.lexsort((r.leaves[:, 2], r.leaves[:, 0]))
     sl = leaf_slices(r.leaves)
     leaves = r.leaves[order]
@@ -54,13 +60,16 @@ def save(name, *, xt, xs, kernel, delta, diag, mcs, eta, eps, sym, x, sample=Non
                         same=np.int32(xs is None), kernel=np.int32(kernel), delta=delta, diag=diag,
                         mcs=np.int32(mcs), eta=eta, eps=eps, sym=np.int32(sym), perm_t=r.perm_t, perm_s=r.perm_s,
                         leaves=leaves, data_leaves=keep.astype(np.int32), data=data, x=x, y=r.y,
-                        false_positive=np.int32(r.info.get("false_positive", 0)), **kw)
+                        false_positive=np.int32(r.info.get("false_positive", 0)),
+                        uplo=np.int32(0 if uplo == "N" else ord(uplo)), **kw)
     print(name, "leaves", len(leaves), "stored leaf data", len(keep), "values", data.size)
 
 
 def save_lrmat(name, *, dist, seed, reqrank, sym):
     """One leaf compression of a 500 x 100 block of two disks (the reference's
     lrmat tests, tests/functional_tests/lrmat/test_lrmat_partialACA.cpp:23-67)."""
+    if ONLY and name not in ONLY:
+        return
     xt = geo.disk(500, 0.0, seed=seed)
     xs = geo.disk(100, float(dist), seed=seed + 1)
     r = refcase.run(xt, xs, kernel=geo.KERNEL_LAPLACE, eps=1e-4, minclustersize=10, compressor=int(sym), mode=1,
@@ -104,6 +113,20 @@ def main():
     u = geo.disk(400, 1.2, seed=23)
     save("disks600x400_complex_sym", xt=t, xs=u, kernel=5, delta=0.0, diag=0.0, mcs=10, eta=10.0, eps=1e-6,
          sym=True, x=rng.standard_normal((400, 1)) + 1j * rng.standard_normal((400, 1)), sample=20)
+    # symmetric storage HMatrix(..., 'S', UPLO): one triangle of blocks (hmatrix.hpp:343-400)
+    n = 1200
+    s = geo.sphere(n, seed=31)
+    save("sphere1200_laplace_symL", xt=s, xs=None, kernel=geo.KERNEL_LAPLACE, delta=0.0,
+         diag=geo.laplace_diag(geo.sphere_spacing(n)), mcs=10, eta=10.0, eps=1e-6, sym=True,
+         x=rng.standard_normal((n, 2)), sample=20, uplo="L")
+    c = geo.cylinder(900, seed=32)
+    save("cyl900_partial_symU", xt=c, xs=None, kernel=geo.KERNEL_INV_DELTA_R, delta=1e-5, diag=0.0, mcs=10,
+         eta=10.0, eps=1e-6, sym=False, x=rng.uniform(-1, 1, (900, 1)), sample=20, uplo="U")
+    n = 800
+    s = geo.sphere(n, seed=33)
+    save("sphere800_helmholtz_symL", xt=s, xs=None, kernel=4, delta=5.0, diag=geo.laplace_diag(geo.sphere_spacing(n)),
+         mcs=10, eta=10.0, eps=1e-6, sym=True, x=rng.standard_normal((n, 1)) + 1j * rng.standard_normal((n, 1)),
+         sample=20, uplo="L")
     for dist, seed in ((15, 100), (40, 200)):
         for reqrank in (-1, 10):
             for sym in (False, True):

@@ -38,6 +38,18 @@ def test_invalid_arguments_rejected_without_device(htg):
     prm.compressor = 7
     assert L.htg_hmatrix_build(C.byref(prm), C.byref(gen), 4, capi.ptr(pts), 4, None, C.byref(h)) == \
         capi.HTG_ERR_UNSUPPORTED
+    # symmetry / UPLO as HMatrix's constructor checks them (hmatrix.hpp:116-121)
+    prm.compressor = 0
+    for sy, up, status in ((ord("S"), ord("N"), capi.HTG_ERR_INVALID), (ord("N"), ord("L"), capi.HTG_ERR_INVALID),
+                           (ord("X"), ord("L"), capi.HTG_ERR_INVALID), (ord("H"), ord("L"), capi.HTG_ERR_UNSUPPORTED)):
+        prm.symmetry, prm.uplo = sy, up
+        assert L.htg_hmatrix_build(C.byref(prm), C.byref(gen), 4, capi.ptr(pts), 4, None, C.byref(h)) == status
+    prm.symmetry, prm.uplo = ord("S"), ord("L")
+    gen2 = capi.htg_generator(0, 0, 0.0, 0.0)
+    assert L.htg_hmatrix_build(C.byref(prm), C.byref(gen2), 4, capi.ptr(pts), 4, capi.ptr(pts), C.byref(h)) == \
+        capi.HTG_ERR_INVALID  # symmetric storage needs one point set
+    assert b"single point set" in L.htg_last_error()
+    prm.symmetry, prm.uplo = 0, 0
     # from_leaves: leaf out of range
     perm = np.arange(10, dtype=np.int32)
     leaves = np.array([[5, 10, 0, 10, -1]], np.int32)

@@ -1,0 +1,132 @@
+"""GPU: symmetric storage HMatrix(..., 'S', UPLO) (hmatrix.hpp:115-122, 343-400).
+
+The build keeps one triangle of blocks (blocks.hpp:140-223, the symmetric
+false-positive recursion hmatrix.hpp:610-690); the product applies every kept
+off-diagonal leaf and its transpose, and the diagonal leaves
+(hmatrix.hpp:864-904). Checked against the reference's own symmetric builds
+(golden fixtures: leaf list, false positives, y = H x) and against the oracle's
+restatement of the same steps, for both triangles, both compressors, real and
+complex kernels, one and many right-hand sides and the mixed storage."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from htool_amd import geometry as geo
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SYM_FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*_sym[LU].npz")))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.mark.parametrize("name", SYM_FIXTURES)
+def test_symmetric_build_matches_reference_fixture(htg, name):
+    print(f"fixture {name}", flush=True)
+    g = dict(np.load(os.path.join(GOLDEN, name + ".npz")))
+    uplo = chr(int(g["uplo"]))
+    H = htg.HMatrix(epsilon=float(g["eps"]), eta=float(g["eta"]), minclustersize=int(g["mcs"]), symmetry="S",
+                    UPLO=uplo)
+    H.set_compression("sympartialACA" if int(g["sym"]) else "partialACA")
+    kernel = int(g["kernel"])
+    gen = htg.Generator(kernel, float(g["delta"]) if kernel != 4 else 0.0, float(g["diag"]),
+                        float(g["delta"]) if kernel == 4 else 0.0)
+    H.build(gen, g["xt"])
+    pt, ps = H.get_perms()
+    assert np.array_equal(pt, g["perm_t"]) and np.array_equal(ps, g["perm_s"])
+    assert np.array_equal(H.get_leaves(), g["leaves"]), "leaf list or ranks differ"
+    i = H.get_info()
+    assert i.n_false_positive == int(g["false_positive"])
+    assert chr(i.symmetry) == "S" and chr(i.uplo) == uplo and i.mirrored_entries > 0
+    assert rel(H.mvprod_global_to_global(g["x"]), g["y"]) < 1e-12
+
+
+@pytest.mark.parametrize("uplo,comp", [("L", "sympartialACA"), ("U", "partialACA"), ("U", "sympartialACA")])
+def test_symmetric_matches_oracle(htg, orc, uplo, comp):
+    print(f"symmetric {uplo} {comp}", flush=True)
+    n = 6000
+    xt = geo.sphere(n, seed=41)
+    diag = geo.laplace_diag(geo.sphere_spacing(n))
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32, symmetry="S", UPLO=uplo)
+    H.set_compression(comp)
+    H.build(htg.Generator(geo.KERNEL_LAPLACE, 0.0, diag), xt)
+    O = orc.hmat_build(xt, kernel=geo.KERNEL_LAPLACE, diag=diag, minclustersize=32, eta=10.0, eps=1e-6,
+                       sym=comp == "sympartialACA", uplo=uplo)
+    leaves, data = H.get_leaves(with_data=True)
+    assert np.array_equal(leaves, O.leaves)
+    assert H.get_info().n_false_positive == O.false_positive
+    X = np.random.default_rng(3).standard_normal((n, 40))  # 32-wide matrix-core pass + an 8-wide pass
+    Y = H.mvprod_global_to_global(X)
+    Yo = O.mvprod(X)
+    for c in range(40):
+        assert rel(Y[:, c], Yo[:, c]) < 1e-12, c
+    # the same leaf data through the oracle's symmetric product
+    assert rel(Y[:, 0], orc.leaf_mvprod_sym(n, O.perm_t, leaves, data, X[:, 0], uplo)) < 1e-12
+    rows = np.random.default_rng(4).choice(n, 300, replace=False).astype(np.int32)
+    yd = orc.dense_rows(xt, None, rows, X[:, 0], kernel=geo.KERNEL_LAPLACE, diag=diag, same=True)
+    assert rel(Y[rows, 0], yd) < 1e-5
+
+
+def test_symmetric_half_the_compressed_entries_and_same_matrix(htg):
+    """The kept triangle holds about half of the general build's entries, and the
+    two matrices agree to the ACA tolerance."""
+    n = 8000
+    xt = geo.sphere(n, seed=42)
+    gen = htg.Generator(geo.KERNEL_LAPLACE, 0.0, geo.laplace_diag(geo.sphere_spacing(n)))
+    Hs = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32, symmetry="S", UPLO="L")
+    Hs.set_compression("sympartialACA")
+    Hs.build(gen, xt)
+    Hf = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32)
+    Hf.set_compression("sympartialACA")
+    Hf.build(gen, xt)
+    s, f = Hs.get_info(), Hf.get_info()
+    kept = s.dense_entries + s.lowrank_entries
+    assert kept < 0.65 * (f.dense_entries + f.lowrank_entries)
+    assert kept + s.mirrored_entries > f.dense_entries + f.lowrank_entries * 0.8
+    x = np.random.default_rng(5).standard_normal(n)
+    assert rel(Hs @ x, Hf @ x) < 1e-5
+
+
+def test_symmetric_mixed_storage_and_transpose(htg, orc):
+    n = 7000
+    xt = geo.sphere(n, seed=43)
+    diag = geo.laplace_diag(geo.sphere_spacing(n))
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32, symmetry="S", UPLO="L")
+    H.set_compression("sympartialACA")
+    H.set_recompression(2e-6)
+    H.set_lowprec(4e-6)
+    H.build(htg.Generator(geo.KERNEL_LAPLACE, 0.0, diag), xt)
+    X = np.random.default_rng(6).standard_normal((n, 3))
+    Y64 = H.mvprod_global_to_global(X)
+    leaves, data = H.get_leaves(with_data=True)
+    pt, _ = H.get_perms()
+    assert rel(Y64[:, 0], orc.leaf_mvprod_sym(n, pt, leaves, data, X[:, 0], "L")) < 1e-12
+    H.set_storage("fp32_fp16")
+    assert rel(H.mvprod_global_to_global(X), Y64) < 2e-6
+    H.set_storage("fp64")
+    # A symmetric matrix is its own transpose
+    assert rel(H.mvprod_transp_global_to_global(X), Y64) < 1e-12
+
+
+def test_symmetric_helmholtz(htg, orc):
+    n = 3000
+    xt = geo.sphere(n, seed=44)
+    h = geo.sphere_spacing(n)
+    k = 2 * np.pi / (10 * h)
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=10, symmetry="S", UPLO="U")
+    H.set_compression("sympartialACA")
+    H.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, geo.laplace_diag(h), k), xt)
+    O = orc.hmat_build(xt, kernel=4, wavenumber=k, diag=geo.laplace_diag(h), minclustersize=10, eta=10.0,
+                       eps=1e-6, sym=True, uplo="U")
+    assert np.array_equal(H.get_leaves(), O.leaves)
+    rng = np.random.default_rng(7)
+    V = rng.standard_normal((n, 16)) + 1j * rng.standard_normal((n, 16))  # one complex matrix-core pass
+    Y = H.mvprod_global_to_global(V)
+    Yo = O.mvprod(V)
+    for c in range(16):
+        assert rel(Y[:, c], Yo[:, c]) < 1e-12, c

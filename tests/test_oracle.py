@@ -37,9 +37,10 @@ LRMAT_FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join
 
 def oracle_from_fixture(orc, g):
     xs = None if int(g["same"]) else g["xs"]
+    uplo = chr(int(g["uplo"])) if "uplo" in g and int(g["uplo"]) else "N"
     return orc.hmat_build(g["xt"], xs, kernel=int(g["kernel"]), delta=float(g["delta"]), diag=float(g["diag"]),
                           minclustersize=int(g["mcs"]), eta=float(g["eta"]), eps=float(g["eps"]),
-                          sym=bool(int(g["sym"])))
+                          sym=bool(int(g["sym"])), uplo=uplo)
 
 
 @pytest.mark.parametrize("name", HMAT_FIXTURES)
@@ -104,6 +105,54 @@ def test_oracle_matches_reference_driver(orc, ref):
         order = np.lexsort((r.leaves[:, 2], r.leaves[:, 0]))
         assert np.array_equal(O.leaves, r.leaves[order])
         assert rel(O.mvprod(x), r.y) < 1e-13
+
+
+def test_symmetric_oracle_matches_reference_driver(orc, ref):
+    """HMatrix(..., 'S', UPLO) from the reference build: the kept triangle's leaf
+    list and false positives, and y of its symmetric product (hmatrix.hpp:864-904)."""
+    cases = [
+        (geo.sphere(3000, seed=5), geo.KERNEL_LAPLACE, 0.0, geo.laplace_diag(geo.sphere_spacing(3000)), 0.0, 1, "L"),
+        (geo.sphere(3000, seed=5), geo.KERNEL_LAPLACE, 0.0, geo.laplace_diag(geo.sphere_spacing(3000)), 0.0, 0, "U"),
+        (geo.cylinder(2500, seed=6), geo.KERNEL_INV_DELTA_R, 1e-5, 0.0, 0.0, 1, "U"),
+        (geo.sphere(2000, seed=7), 4, 0.0, 2.0, 9.0, 1, "L"),
+    ]
+    for xt, kernel, delta, diag, k, comp, uplo in cases:
+        n = xt.shape[0]
+        rng = np.random.default_rng(1)
+        x = rng.standard_normal((n, 2)) + (1j * rng.standard_normal((n, 2)) if kernel >= 4 else 0)
+        r = ref.run(xt, None, x=x, kernel=kernel, delta=delta, diag=diag, wavenumber=k, minclustersize=10, eta=10.0,
+                    eps=1e-6, compressor=comp, mu=2, uplo=uplo)
+        O = orc.hmat_build(xt, kernel=kernel, delta=delta, diag=diag, wavenumber=k, minclustersize=10, eta=10.0,
+                           eps=1e-6, sym=bool(comp), uplo=uplo)
+        order = np.lexsort((r.leaves[:, 2], r.leaves[:, 0]))
+        assert np.array_equal(O.leaves, r.leaves[order])
+        assert O.false_positive == int(r.info["false_positive"])
+        assert (O.leaves[:, 2] <= O.leaves[:, 0]).all() if uplo == "L" else (O.leaves[:, 2] >= O.leaves[:, 0]).all()
+        assert rel(O.mvprod(x), r.y) < 1e-13
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_symmetric_storage_is_the_same_matrix(orc, uplo):
+    """One triangle of blocks applied with its transposes is the whole matrix:
+    the symmetric product agrees with the general H-matrix and with dense rows to
+    the ACA tolerance, and the two triangles agree with each other; a symmetric
+    leaf list covers each (i, j) with i >= j ('L') exactly once."""
+    n = 1500
+    xt = geo.sphere(n, seed=8)
+    diag = geo.laplace_diag(geo.sphere_spacing(n))
+    x = np.random.default_rng(2).standard_normal(n)
+    Os = orc.hmat_build(xt, kernel=geo.KERNEL_LAPLACE, diag=diag, minclustersize=10, eps=1e-8, sym=True, uplo=uplo)
+    Of = orc.hmat_build(xt, kernel=geo.KERNEL_LAPLACE, diag=diag, minclustersize=10, eps=1e-8, sym=True)
+    ys = Os.mvprod(x)
+    assert rel(ys, Of.mvprod(x)) < 1e-7
+    rows = np.arange(0, n, 7, dtype=np.int32)
+    assert rel(ys[rows], orc.dense_rows(xt, None, rows, x, kernel=geo.KERNEL_LAPLACE, diag=diag, same=True)) < 1e-7
+    cover = np.zeros((n, n), np.int8)
+    for oi, m, oj, nn, k in Os.leaves:
+        cover[oi:oi + m, oj:oj + nn] += 1
+        if oi != oj:
+            cover[oj:oj + nn, oi:oi + m] += 1
+    assert (cover == 1).all()
 
 
 @pytest.mark.parametrize("kernel", [0, 1, 2, 3])
