@@ -83,6 +83,7 @@ _SIGNATURES = {
     "htg_hmatrix_nleaves": (_I, [_VP, C.POINTER(_I), C.POINTER(C.c_int64)]),
     "htg_hmatrix_get_leaves": (_I, [_VP, _VP, _VP]),
     "htg_hmatrix_get_perms": (_I, [_VP, _VP, _VP]),
+    "htg_cluster_build": (_I, [_I, _VP, _I, _VP, _VP, _VP, _VP]),
     "htg_hmatrix_mvprod": (_I, [_VP, _VP, _VP, _I]),
     "htg_hmatrix_mvprod_device": (_I, [_VP, _VP, _VP, _I, _VP]),
     "htg_hmatrix_profile": (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP]),

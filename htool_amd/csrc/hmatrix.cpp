@@ -1256,6 +1256,34 @@ int htg_hmatrix_get_leaves(const htg_hmatrix *h, int *leaves5, double *data) {
     });
 }
 
+int htg_cluster_build(int n, const double *x, int minclustersize, int *perm, int *nnodes, int *nodes_int,
+                      double *nodes_dbl) {
+    return guard([&] {
+        if (n <= 0 || !x || !nnodes)
+            throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments to build a cluster tree");
+        if (minclustersize <= 0)
+            throw Error(HTG_ERR_INVALID, "[Htool error] MinClusterSize parameter cannot be zero");
+        ClusterTree T;
+        build_cluster_tree(n, x, minclustersize, T);
+        if (perm)
+            std::copy(T.perm.begin(), T.perm.end(), perm);
+        if (nodes_int) {
+            if (*nnodes < (int)T.nodes.size())
+                throw Error(HTG_ERR_INVALID, "node arrays too small");
+            for (size_t i = 0; i < T.nodes.size(); i++) {
+                const ClusterNode &c = T.nodes[i];
+                const int v[6]       = {c.offset, c.size, c.depth, c.rank, c.nb_sons, c.first_son};
+                std::copy(v, v + 6, nodes_int + 6 * i);
+                if (nodes_dbl) {
+                    const double d[4] = {c.ctr[0], c.ctr[1], c.ctr[2], c.rad};
+                    std::copy(d, d + 4, nodes_dbl + 4 * i);
+                }
+            }
+        }
+        *nnodes = (int)T.nodes.size();
+    });
+}
+
 int htg_hmatrix_get_perms(const htg_hmatrix *h, int *perm_t, int *perm_s) {
     return guard([&] {
         if (!h)

@@ -99,6 +99,11 @@ class HMatrix:
         self.predictor = predictor
         self.predictor_margin = float(margin)
 
+    def save_plot(self, outputname: str) -> str:
+        """HMatrix::save_plot (hmatrix.hpp:1695-1712): the leaf list CSV `<outputname>_0.csv`."""
+        from .io import save_plot
+        return save_plot(self, outputname)
+
     def set_aca_wide_min(self, m: int):
         """Blocks with M + N >= m run the multi-workgroup ACA (0: the library
         default, 16384; negative: never). Factors and ranks do not change."""

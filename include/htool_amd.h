@@ -217,6 +217,17 @@ int htg_hmatrix_nleaves(const htg_hmatrix *h, int *nleaves, int64_t *data_size);
 int htg_hmatrix_get_leaves(const htg_hmatrix *h, int *leaves5, double *data /* may be NULL */);
 int htg_hmatrix_get_perms(const htg_hmatrix *h, int *perm_t, int *perm_s);
 
+/* The cluster tree the builds use for a point set (host only, no device
+ * needed): Cluster<PCAGeometricClustering>::build, cluster.hpp:57-92, with
+ * the tree that Cluster::save_cluster writes (cluster.hpp:433-484). Call once
+ * with nodes_int == NULL for *nnodes, then with arrays of *nnodes entries:
+ * nodes_int 6 per node (offset, size, depth, rank, nb_sons, first_son index or
+ * -1), nodes_dbl 4 per node (ctr x, y, z, radius); node 0 is the root, the
+ * sons of a node are consecutive. perm (n entries, may be NULL): cluster
+ * position -> original point index. */
+int htg_cluster_build(int n, const double *x, int minclustersize, int *perm, int *nnodes, int *nodes_int,
+                      double *nodes_dbl);
+
 /* y = H x, replaces HMatrix::mvprod_global_to_global(in, out, mu)
  * (include/htool/types/hmatrix.hpp:1073-1182). Host pointers: copies over PCIe. */
 int htg_hmatrix_mvprod(htg_hmatrix *h, const double *x, double *y, int mu);
