@@ -40,7 +40,7 @@ class htg_params(C.Structure):
                 ("minclustersize", C.c_int), ("compressor", C.c_int), ("reqrank", C.c_int),
                 ("lowprec_all", C.c_int), ("recompress_tol", C.c_double), ("lowprec_tol", C.c_double),
                 ("predictor", C.c_void_p), ("predictor_margin", C.c_double), ("aca_wide_min", C.c_int),
-                ("symmetry", C.c_int), ("uplo", C.c_int)]
+                ("symmetry", C.c_int), ("uplo", C.c_int), ("delay_dense", C.c_int)]
 
 
 class htg_layout(C.Structure):
@@ -84,6 +84,7 @@ _SIGNATURES = {
     "htg_hmatrix_get_leaves": (_I, [_VP, _VP, _VP]),
     "htg_hmatrix_get_perms": (_I, [_VP, _VP, _VP]),
     "htg_cluster_build": (_I, [_I, _VP, _I, _VP, _VP, _VP, _VP]),
+    "htg_hmatrix_set_dense_blocks": (_I, [_VP, _VP]),
     "htg_hmatrix_mvprod": (_I, [_VP, _VP, _VP, _I]),
     "htg_hmatrix_mvprod_device": (_I, [_VP, _VP, _VP, _I, _VP]),
     "htg_hmatrix_profile": (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP]),

@@ -429,7 +429,8 @@ void assemble(const htg_params &prm, const htg_generator &gen, int nt, const dou
     if (!same)
         res.pS.upload(xs, S.perm);
 
-    res.cplx = kernel_is_complex(res.kd.kind);
+    res.cplx        = kernel_is_complex(res.kd.kind);
+    res.delay_dense = prm.delay_dense != 0;
     if (res.cplx && prm.recompress_tol > 0)
         throw Error(HTG_ERR_UNSUPPORTED, "QR+SVD recompression is implemented for the real kernel families only");
     Assembler as{T, S, prm, res.kd, res.dT(), res.dS(), {}, {}, {}, 0, {}, {}, 0, &res, res.cplx ? 2 : 1};

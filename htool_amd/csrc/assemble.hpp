@@ -27,6 +27,7 @@ struct Assembly {
     KernelDesc kd{};
     bool same = true;
     bool cplx = false; // complex kernel family: the store holds complex<double> entries
+    bool delay_dense = false; // dense leaves not generated (filled by the caller later)
     PointsOnDevice pT, pS;
     // ACA runs of admissible blocks (first attempts): features, terms needed,
     // first workspace terms; retries and workspace totals.

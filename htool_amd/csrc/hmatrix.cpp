@@ -80,6 +80,7 @@ struct htg_hmatrix {
     int64_t aca_ws_entries = 0, aca_need_entries = 0;
     double aca_seconds = 0;
     char symmetry = 'N', uplo = 'N'; // symmetric storage keeps one triangle; `leaves` adds mirrored copies
+    std::vector<double> user_dense;  // dense leaves supplied by the caller (set_dense_blocks), get_leaves order
     bool scaled = false; // some low-rank terms carry scales (recompressed)
 
     std::vector<int> rb_row0, rb_order;
@@ -574,7 +575,10 @@ void fill_from_assembly(htg_hmatrix &h, const Part &part) {
     }
     DevArray<FillTile> d_fills;
     d_fills.upload(fills);
-    if (h.cplx)
+    if (A.delay_dense) // left zero (alloc_master) until the caller supplies the blocks
+        fills.clear();
+    if (fills.empty()) {
+    } else if (h.cplx)
         launch_dense_fill_c(A.kd, rowsP, colsP, d_fills.p, (int)fills.size(),
                             reinterpret_cast<cdbl *>(h.cls[0].a64.p), nullptr);
     else
@@ -633,6 +637,48 @@ void apply_storage(htg_hmatrix &h, int storage) {
     h.storage = storage;
 }
 
+// Writes the caller's dense leaves (h.user_dense) into the FP64 master arena:
+// each kept dense leaf into its row-block tiles, and its transpose into the
+// tiles of its mirrored copy (symmetric storage).
+void write_user_dense(htg_hmatrix &h) {
+    if (h.user_dense.empty())
+        return;
+    const int e = h.esz();
+    ClassLayout &C = h.cls[0];
+    std::vector<double> arena(C.a64.n);
+    HTG_HIP(hipMemcpy(arena.data(), C.a64.p, arena.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::map<std::pair<int, int>, size_t> where; // (off_i, off_j) of a kept dense leaf -> data offset
+    size_t pos = 0;
+    for (const Leaf &L : h.leaves) {
+        if (L.mirror || L.k >= 0)
+            continue;
+        where[{L.oi, L.oj}] = pos;
+        pos += (size_t)L.m * L.n * e;
+    }
+    for (size_t l = 0; l < h.leaves.size(); l++) {
+        const Leaf &L = h.leaves[l];
+        if (L.k >= 0)
+            continue;
+        const auto it = where.find(L.mirror ? std::make_pair(L.oj, L.oi) : std::make_pair(L.oi, L.oj));
+        if (it == where.end())
+            throw Error(HTG_ERR_INVALID, "dense leaf without supplied data");
+        const double *d = h.user_dense.data() + it->second;
+        for (const auto &rt : h.leaf_tiles[l]) {
+            const int rb = std::get<0>(rt), R = row_block_rows(h, rb), i0 = h.rb_row0[rb] - L.oi;
+            const size_t ld = row_stride(h, 0, rb);
+            double *dst     = arena.data() + e * std::get<1>(rt);
+            for (int i = 0; i < R; i++)
+                for (int j = 0; j < L.n; j++) {
+                    // kept leaf: column-major m x n; mirror: the kept n x m leaf's (j, i)
+                    const size_t s = L.mirror ? (size_t)j + (size_t)L.n * (i0 + i) : (size_t)(i0 + i) + (size_t)L.m * j;
+                    for (int u = 0; u < e; u++)
+                        dst[e * (i * ld + j) + u] = d[e * s + u];
+                }
+        }
+    }
+    HTG_HIP(hipMemcpy(C.a64.p, arena.data(), arena.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
 // Lays out the product for h.lay and fills it from the kept assemblies.
 void relayout(htg_hmatrix &h) {
     if (h.parts.empty())
@@ -647,6 +693,7 @@ void relayout(htg_hmatrix &h) {
     alloc_master(h);
     for (const Part &part : h.parts)
         fill_from_assembly(h, part);
+    write_user_dense(h);
     apply_storage(h, h.storage);
 }
 
@@ -1253,6 +1300,22 @@ int htg_hmatrix_get_leaves(const htg_hmatrix *h, int *leaves5, double *data) {
                 out += e * (size_t)L.k * L.n;
             }
         }
+    });
+}
+
+int htg_hmatrix_set_dense_blocks(htg_hmatrix *h, const double *data) {
+    return guard([&] {
+        if (!h || !data)
+            throw Error(HTG_ERR_INVALID, "null argument");
+        size_t n = 0;
+        for (const Leaf &L : h->leaves)
+            if (!L.mirror && L.k < 0)
+                n += (size_t)L.m * L.n * h->esz();
+        h->user_dense.assign(data, data + n);
+        write_user_dense(*h);
+        for (ClassLayout &C : h->cls) // storage copies are converted again from the new FP64 entries
+            C.a32.reset(), C.a16.reset(), C.ab16.reset();
+        apply_storage(*h, h->storage);
     });
 }
 

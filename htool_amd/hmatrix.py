@@ -64,6 +64,7 @@ class HMatrix:
         self.predictor = None
         self.predictor_margin = 0.25
         self.aca_wide_min = 0
+        self.delay_dense = False
         self._h = None
         self._storage = "fp64"
 
@@ -99,6 +100,31 @@ class HMatrix:
         self.predictor = predictor
         self.predictor_margin = float(margin)
 
+    def set_delay_dense_computation(self, delay: bool = True):
+        """set_delay_dense_computation (hmatrix.hpp:213-220): the build leaves the
+        dense leaves zero; `build_dense_blocks` fills them afterwards."""
+        self.delay_dense = bool(delay)
+
+    def build_dense_blocks(self, generator) -> None:
+        """HMatrix::build_dense_blocks (hmatrix.hpp:404-418): the generator's
+        copy_dense_blocks(M, N, rows, cols, ptr) fills ptr[b], an M[b] x N[b]
+        array, with the entries of dense leaf b for original row indices
+        rows[b] and column indices cols[b] (VirtualDenseBlocksGenerator,
+        virtual_dense_blocks_generator.hpp:11-17)."""
+        leaves = self.get_leaves()
+        pt, ps = self.get_perms()
+        dense = leaves[leaves[:, 4] < 0]
+        dt = np.complex128 if self.is_complex else np.float64
+        M = [int(m) for m in dense[:, 1]]
+        N = [int(n) for n in dense[:, 3]]
+        rows = [pt[oi:oi + m] for oi, m in zip(dense[:, 0], M)]
+        cols = [ps[oj:oj + n] for oj, n in zip(dense[:, 2], N)]
+        ptr = [np.zeros((m, n), dtype=dt, order="F") for m, n in zip(M, N)]
+        generator.copy_dense_blocks(M, N, rows, cols, ptr)
+        data = np.concatenate([np.asarray(p, dtype=dt).ravel(order="F") for p in ptr]) if ptr else np.zeros(0, dt)
+        data = np.ascontiguousarray(data)
+        capi.check(capi.lib().htg_hmatrix_set_dense_blocks(self._handle(), capi.ptr(data.view(np.float64))))
+
     def save_plot(self, outputname: str) -> str:
         """HMatrix::save_plot (hmatrix.hpp:1695-1712): the leaf list CSV `<outputname>_0.csv`."""
         from .io import save_plot
@@ -131,7 +157,8 @@ class HMatrix:
         return capi.htg_params(self.epsilon, self.eta, self.maxblocksize, self.minclustersize, self.compressor,
                                self.reqrank, int(self.lowprec_all), self.recompress_tol, self.lowprec_tol,
                                self.predictor.handle if self.predictor is not None else None,
-                               self.predictor_margin, self.aca_wide_min, ord(self.symmetry), ord(self.UPLO))
+                               self.predictor_margin, self.aca_wide_min, ord(self.symmetry), ord(self.UPLO),
+                               int(self.delay_dense))
 
     def build_batch(self, gens, point_sets) -> "HMatrix":
         """Many independent square H-matrices as one block-diagonal H-matrix

@@ -105,6 +105,8 @@ typedef struct htg_params {
                                        blocks kept (HMatrix(..., 'S', UPLO), hmatrix.hpp:115-122);
                                        'H' is rejected (no Hermitian kernel family) */
     int uplo;                       /* 'L' or 'U' with 'S' (the kept triangle); 'N' (or 0) otherwise */
+    int delay_dense;                /* 1: dense leaves left zero at build, filled later by
+                                       htg_hmatrix_set_dense_blocks (set_delay_dense_computation) */
 } htg_params;
 
 typedef struct htg_info {
@@ -216,6 +218,15 @@ int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info);
 int htg_hmatrix_nleaves(const htg_hmatrix *h, int *nleaves, int64_t *data_size);
 int htg_hmatrix_get_leaves(const htg_hmatrix *h, int *leaves5, double *data /* may be NULL */);
 int htg_hmatrix_get_perms(const htg_hmatrix *h, int *perm_t, int *perm_s);
+
+/* HMatrix::build_dense_blocks(VirtualDenseBlocksGenerator) (hmatrix.hpp:404-418,
+ * virtual_dense_blocks_generator.hpp:11-17): the caller supplies every kept
+ * dense leaf, in htg_hmatrix_get_leaves order, each m x n column-major with
+ * entry (i, j) = A(perm_t[off_i + i], perm_s[off_j + j]) (the dense part of the
+ * get_leaves data layout; complex: re, im pairs). They replace the dense
+ * leaves (zero after a build with delay_dense), are kept through layout
+ * changes and reach every storage mode. */
+int htg_hmatrix_set_dense_blocks(htg_hmatrix *h, const double *data);
 
 /* The cluster tree the builds use for a point set (host only, no device
  * needed): Cluster<PCAGeometricClustering>::build, cluster.hpp:57-92, with

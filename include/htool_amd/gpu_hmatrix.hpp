@@ -57,6 +57,7 @@ class GpuHMatrix {
         prm_.aca_wide_min     = 0;
         prm_.symmetry         = symmetry;
         prm_.uplo             = UPLO;
+        prm_.delay_dense      = 0;
     }
     GpuHMatrix(const GpuHMatrix &) = delete;
     GpuHMatrix &operator=(const GpuHMatrix &) = delete;
@@ -79,6 +80,10 @@ class GpuHMatrix {
         prm_.minclustersize = (int)m;
     }
     void set_compression(htg_compressor c) { prm_.compressor = c; }
+    void set_delay_dense_computation(bool d) { prm_.delay_dense = d ? 1 : 0; }
+    // build_dense_blocks (hmatrix.hpp:404-418): every kept dense leaf, in
+    // get_leaves order, m x n column-major in cluster ordering
+    void build_dense_blocks(const double *data) { check(htg_hmatrix_set_dense_blocks(handle(), data)); }
     // Build-time options of the mixed-precision product (no reference counterpart).
     void set_recompression(double tol) { prm_.recompress_tol = tol; }
     void set_lowprec(double tol, bool all_terms = false) {

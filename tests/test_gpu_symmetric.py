@@ -130,3 +130,47 @@ def test_symmetric_helmholtz(htg, orc):
     Yo = O.mvprod(V)
     for c in range(16):
         assert rel(Y[:, c], Yo[:, c]) < 1e-12, c
+
+
+class LaplaceBlocks:
+    """VirtualDenseBlocksGenerator of 1/(4 pi r) with `diag` on the diagonal, from
+    numpy (virtual_dense_blocks_generator.hpp:11-17)."""
+
+    def __init__(self, x, diag):
+        self.x, self.diag = x, diag
+
+    def copy_dense_blocks(self, M, N, rows, cols, ptr):
+        for m, n, r, c, out in zip(M, N, rows, cols, ptr):
+            d = np.linalg.norm(self.x[r][:, None, :] - self.x[c][None, :, :], axis=2)
+            same = r[:, None] == c[None, :]
+            with np.errstate(divide="ignore"):
+                out[:] = np.where(same, self.diag, 1.0 / (4 * np.pi * np.where(same, 1.0, d)))
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_delayed_dense_blocks(htg, symmetric):
+    """set_delay_dense_computation + build_dense_blocks: zero dense leaves after the
+    build, the supplied ones afterwards, kept through a layout change and
+    converted for the FP32 storage."""
+    print(f"delayed dense symmetric={symmetric}", flush=True)
+    n = 5000
+    xt = geo.sphere(n, seed=45)
+    diag = geo.laplace_diag(geo.sphere_spacing(n))
+    kw = dict(symmetry="S", UPLO="L") if symmetric else {}
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32, **kw)
+    H.set_compression("partialACA")
+    H.build(htg.Generator(geo.KERNEL_LAPLACE, 0.0, diag), xt)
+    D = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32, **kw)
+    D.set_compression("partialACA")
+    D.set_delay_dense_computation(True)
+    D.build(htg.Generator(geo.KERNEL_LAPLACE, 0.0, diag), xt)
+    x = np.random.default_rng(8).standard_normal(n)
+    y = H @ x
+    assert rel(D @ x, y) > 1e-2  # no near field yet
+    D.build_dense_blocks(LaplaceBlocks(xt, diag))
+    assert rel(D @ x, y) < 1e-13
+    D.set_layout(rb_max=64, vrows=8)
+    assert rel(D @ x, y) < 1e-13
+    D.set_storage("fp32")
+    H.set_storage("fp32")
+    assert rel(D @ x, H @ x) < 1e-6
