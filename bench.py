@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--no-assembly-5e5", action="store_true",
                     help="skip the N=5e5 assembly with and without the learned rank predictor")
     ap.add_argument("--cpu-reps", type=int, default=10)
+    ap.add_argument("--matmat-counters-run", action="store_true",
+                    help="short run for rocprofv3 --pmc on the 32-RHS products: three products per storage "
+                         "(FP64, FP32, FP32+FP16) of the headline H-matrices; prints nothing else")
     ap.add_argument("--counters-run", action="store_true",
                     help="short run for rocprofv3 --pmc: a 256 MiB device copy of known size, then a few "
                          "mixed-precision and FP64 products; prints nothing else")
@@ -340,6 +343,16 @@ def main():
                 M.mvprod_device(dx.data_ptr(), dy.data_ptr(), 1, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         log("counters run done")
+        return
+    if args.matmat_counters_run:
+        dX = torch.from_numpy(rng.standard_normal(n * 32)).cuda()
+        dY = torch.zeros(n * 32, dtype=torch.float64, device="cuda")
+        for M, storage in ((H, "fp64"), (H, "fp32"), (Hm, "fp32_fp16")):
+            M.set_storage(storage)
+            for _ in range(3):
+                M.mvprod_device(dX.data_ptr(), dY.data_ptr(), 32, torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        log("matmat counters run done")
         return
     X = rng.standard_normal((n, 32))
     stream = torch.cuda.current_stream()
