@@ -31,6 +31,15 @@ inline int v_first(const Leaf &L, int c) { return L.oj & ~(CLASS_ALIGN[c] - 1); 
 inline int v_stride(const Leaf &L, int c) { return round_up(L.oj + L.n, CLASS_ALIGN[c]) - v_first(L, c); }
 inline int terms(const Leaf &L, int c) { return L.k <= 0 ? 0 : (c == 0 ? L.k_hi : L.k - L.k_hi); }
 
+// V chunks grouped per wavefront, and the first chunk of every row group cut
+// into several chunks (its partial sums are added by a second pass).
+struct VTileList {
+    std::vector<VTile> tiles, multi;
+    std::vector<int> vg_off;
+    DevArray<VTile> d_tiles, d_multi;
+    DevArray<int> d_vg_off;
+};
+
 // Per precision class: the row blocks' column lists and blocks, the V arena and its chunks.
 struct ClassLayout {
     std::vector<int> rb_col0, colidx;
@@ -91,13 +100,12 @@ struct htg_hmatrix {
     std::vector<double> tscale; // per z entry (x part unused)
     int t_size = 0, part_size = 0;
     ClassLayout cls[2];
-    // V chunks of both classes, grouped per wavefront
-    std::vector<VTile> vtiles;
-    std::vector<int> vg_off;
-    std::vector<VTile> multi_tiles; // first chunk of every row group cut into several chunks
-    DevArray<VTile> d_vtiles;
-    DevArray<int> d_vg_off;
-    DevArray<VTile> d_multi_tiles;
+    // V chunks of both classes, grouped per wavefront, in two cuts of the same
+    // V arena: vt[0] cuts every panel into row groups of vrows rows and wide
+    // groups into column chunks (matrix-core tiles, FP64 rows); vt[1] keeps
+    // every panel no wider than vchunk_el whole and shrinks its row groups
+    // instead (single right-hand side in FP32 arithmetic).
+    VTileList vt[2];
 
     DevArray<int> d_perm_t, d_perm_s, d_rb_row0, d_rb_order;
     DevArray<double> d_tscale;
@@ -130,8 +138,9 @@ int t_entry(const htg_hmatrix &h, int l, int j) {
 //  * Each low-rank leaf stores its V rows (class 0: the first k_hi, class 1 the
 //    rest) row-major and aligned to the global column index, stacked with the
 //    other leaves of its source cluster (a panel); panel rows are cut into
-//    groups of vrows and columns into chunks of about vchunk_el entries,
-//    grouped into wavefront work lists (htg_layout).
+//    groups of at most vrows rows and about vchunk_el entries (columns are
+//    cut only in panels wider than vchunk_el), grouped into wavefront work
+//    lists (htg_layout).
 void build_layout(htg_hmatrix &h) {
     std::vector<int> bnd = {0, h.nr};
     for (const Leaf &L : h.leaves) {
@@ -274,79 +283,100 @@ void build_layout(htg_hmatrix &h) {
     };
     std::stable_sort(h.rb_order.begin(), h.rb_order.end(), [&](int a, int b) { return work(a) > work(b); });
 
-    // V rows and their chunks, class 0 then class 1, grouped per wavefront. A
-    // panel's stacked rows are cut into groups of vrows rows, each group's
-    // columns into chunks of about vchunk_el entries.
-    h.part_size = 0;
-    h.vtiles.clear();
-    h.multi_tiles.clear();
-    h.vg_off.assign(1, 0);
-    int64_t group_el = 0;
+    // V arena: per class, panel by panel, the stacked V rows of the panel's
+    // leaves with a common stride.
     for (int c = 0; c < 2; c++) {
         ClassLayout &C = h.cls[c];
         C.varena_size  = 0;
-        C.nvtiles      = 0;
         C.leaf_v_off.assign(h.leaves.size(), 0);
         for (const auto &P : h.panels) {
-            int K = 0, first = -1;
-            for (int l : P)
-                if (terms(h.leaves[l], c) > 0) {
-                    if (first < 0)
-                        first = l;
-                    K += terms(h.leaves[l], c);
-                }
-            if (K == 0)
-                continue;
-            const Leaf &L0       = h.leaves[first];
-            const int ld         = v_stride(L0, c);
-            const uint64_t base  = C.varena_size;
-            int row              = 0;
+            int K = 0, ld = 0;
             for (int l : P) {
                 const int kc = terms(h.leaves[l], c);
                 if (kc > 0) {
-                    C.leaf_v_off[l] = base + (uint64_t)row * ld;
-                    row += kc;
+                    if (K == 0)
+                        ld = v_stride(h.leaves[l], c);
+                    C.leaf_v_off[l] = C.varena_size + (uint64_t)K * ld;
+                    K += kc;
                 }
             }
-            C.varena_size    = align_up(base + (uint64_t)K * ld);
-            const int tfirst = t_entry(h, first, c == 0 ? 0 : L0.k_hi);
-            for (int g0 = 0; g0 < K; g0 += h.lay.vrows) {
-                const int kr     = std::min(h.lay.vrows, K - g0);
-                const int cpc    = round_up(std::max(64, h.lay.vchunk_el / kr), CLASS_ALIGN[c]);
-                const int nparts = (ld + cpc - 1) / cpc;
-                const int pbase  = h.part_size;
-                if (nparts > 1)
-                    h.part_size += nparts * kr;
-                for (int q = 0; q < nparts; q++) {
-                    VTile v{};
-                    const int j0 = q * cpc;
-                    v.v_off      = base + (uint64_t)g0 * ld + (uint64_t)j0;
-                    v.ld         = ld;
-                    v.k          = kr;
-                    v.ncols      = std::min(cpc, ld - j0);
-                    v.x_off      = v_first(L0, c) + j0;
-                    v.t_index    = tfirst + g0;
-                    v.part_off   = pbase + q * kr;
-                    v.part_base  = pbase;
-                    v.nparts     = nparts;
-                    v.cls        = c;
-                    if (nparts > 1 && q == 0)
-                        h.multi_tiles.push_back(v);
-                    const int64_t el   = (int64_t)v.k * v.ncols;
-                    const int in_group = (int)h.vtiles.size() - h.vg_off.back();
-                    if (in_group > 0 && (group_el + el > h.lay.vgroup_el || in_group >= h.lay.vgroup_tiles)) {
-                        h.vg_off.push_back((int)h.vtiles.size());
-                        group_el = 0;
+            if (K > 0)
+                C.varena_size = align_up(C.varena_size + (uint64_t)K * ld);
+        }
+    }
+    // V chunks, class 0 then class 1, grouped per wavefront, in both cuts. A
+    // panel's stacked rows are cut into groups of at most vrows rows; in cut 0
+    // (and for a panel wider than vchunk_el in cut 1) each group's columns are
+    // cut into chunks of about vchunk_el entries, in cut 1 a narrower panel's
+    // groups keep all columns and take at most vchunk_el entries.
+    h.part_size = 0;
+    for (int cut = 0; cut < 2; cut++) {
+        VTileList &T = h.vt[cut];
+        T.tiles.clear();
+        T.multi.clear();
+        T.vg_off.assign(1, 0);
+        int part_size    = 0;
+        int64_t group_el = 0;
+        for (int c = 0; c < 2; c++) {
+            ClassLayout &C = h.cls[c];
+            if (cut == 0)
+                C.nvtiles = 0;
+            for (const auto &P : h.panels) {
+                int K = 0, first = -1;
+                for (int l : P)
+                    if (terms(h.leaves[l], c) > 0) {
+                        if (first < 0)
+                            first = l;
+                        K += terms(h.leaves[l], c);
                     }
-                    h.vtiles.push_back(v);
-                    C.nvtiles++;
-                    group_el += el;
+                if (K == 0)
+                    continue;
+                const Leaf &L0       = h.leaves[first];
+                const int ld         = v_stride(L0, c);
+                const uint64_t base  = C.leaf_v_off[first];
+                const int tfirst     = t_entry(h, first, c == 0 ? 0 : L0.k_hi);
+                const bool whole     = cut == 1 && ld <= h.lay.vchunk_el;
+                const int rows_g     = whole ? std::max(1, std::min(h.lay.vrows, h.lay.vchunk_el / ld)) : h.lay.vrows;
+                for (int g0 = 0; g0 < K; g0 += rows_g) {
+                    const int kr     = std::min(rows_g, K - g0);
+                    const int cpc    = whole ? ld : round_up(std::max(64, h.lay.vchunk_el / kr), CLASS_ALIGN[c]);
+                    const int nparts = (ld + cpc - 1) / cpc;
+                    const int pbase  = part_size;
+                    if (nparts > 1)
+                        part_size += nparts * kr;
+                    for (int q = 0; q < nparts; q++) {
+                        VTile v{};
+                        const int j0 = q * cpc;
+                        v.v_off      = base + (uint64_t)g0 * ld + (uint64_t)j0;
+                        v.ld         = ld;
+                        v.k          = kr;
+                        v.ncols      = std::min(cpc, ld - j0);
+                        v.x_off      = v_first(L0, c) + j0;
+                        v.t_index    = tfirst + g0;
+                        v.part_off   = pbase + q * kr;
+                        v.part_base  = pbase;
+                        v.nparts     = nparts;
+                        v.cls        = c;
+                        if (nparts > 1 && q == 0)
+                            T.multi.push_back(v);
+                        const int64_t el   = (int64_t)v.k * v.ncols;
+                        const int in_group = (int)T.tiles.size() - T.vg_off.back();
+                        if (in_group > 0 && (group_el + el > h.lay.vgroup_el || in_group >= h.lay.vgroup_tiles)) {
+                            T.vg_off.push_back((int)T.tiles.size());
+                            group_el = 0;
+                        }
+                        T.tiles.push_back(v);
+                        if (cut == 0)
+                            C.nvtiles++;
+                        group_el += el;
+                    }
                 }
             }
         }
+        if ((int)T.tiles.size() > T.vg_off.back())
+            T.vg_off.push_back((int)T.tiles.size());
+        h.part_size = std::max(h.part_size, part_size); // the cuts never run at once
     }
-    if ((int)h.vtiles.size() > h.vg_off.back())
-        h.vg_off.push_back((int)h.vtiles.size());
 }
 
 void upload_layout(htg_hmatrix &h) {
@@ -359,9 +389,11 @@ void upload_layout(htg_hmatrix &h) {
         C.d_rb_aoff.upload(C.rb_aoff);
         C.d_colidx.upload(C.colidx);
     }
-    h.d_vtiles.upload(h.vtiles);
-    h.d_vg_off.upload(h.vg_off);
-    h.d_multi_tiles.upload(h.multi_tiles);
+    for (VTileList &T : h.vt) {
+        T.d_tiles.upload(T.tiles);
+        T.d_vg_off.upload(T.vg_off);
+        T.d_multi.upload(T.multi);
+    }
     if (h.scaled)
         h.d_tscale.upload(h.tscale);
     // z = [x (nc entries, zero padded to tpad) | t], per RHS; zeroed once so the
@@ -433,8 +465,9 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
         if (ev)
             HTG_HIP(hipEventRecord(ev[1], st));
         const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
-        launch_vx<TS0, TS1, TC>(h.d_vg_off.p, (int)h.vg_off.size() - 1, h.d_vtiles.p, h.d_multi_tiles.p,
-                                (int)h.multi_tiles.size(), arena_of<TS0>(C0, true),
+        const VTileList &T = h.vt[MU == 1 && std::is_same<TC, float>::value ? 1 : 0];
+        launch_vx<TS0, TS1, TC>(T.d_vg_off.p, (int)T.vg_off.size() - 1, T.d_tiles.p, T.d_multi.p,
+                                (int)T.multi.size(), arena_of<TS0>(C0, true),
                                 has1 ? arena_of<TS1>(C1, true) : nullptr, z, h.d_part.p, tscale, MU, st);
         if (ev)
             HTG_HIP(hipEventRecord(ev[2], st));
@@ -465,8 +498,9 @@ void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream
         if (ev)
             HTG_HIP(hipEventRecord(ev[1], st));
         const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
-        launch_vx_c<TS0, TS1, R>(h.d_vg_off.p, (int)h.vg_off.size() - 1, h.d_vtiles.p, h.d_multi_tiles.p,
-                                 (int)h.multi_tiles.size(), arena_of<TS0>(C0, true),
+        const VTileList &T = h.vt[0];
+        launch_vx_c<TS0, TS1, R>(T.d_vg_off.p, (int)T.vg_off.size() - 1, T.d_tiles.p, T.d_multi.p,
+                                 (int)T.multi.size(), arena_of<TS0>(C0, true),
                                  has1 ? arena_of<TS1>(C1, true) : nullptr, z, h.d_part.p, tscale, MU, st);
         if (ev)
             HTG_HIP(hipEventRecord(ev[2], st));
@@ -1186,7 +1220,7 @@ int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info) {
         r.compression_ratio = stored > 0 ? double(h->nr) * double(h->nc) / stored : 0.0;
         r.n_row_blocks      = (int)h->rb_row0.size() - 1;
         r.n_columns         = (int)(h->cls[0].colidx.size() + h->cls[1].colidx.size());
-        r.n_vtiles          = (int)h->vtiles.size();
+        r.n_vtiles          = (int)h->vt[0].tiles.size();
         r.build_seconds     = h->build_seconds;
         r.n_aca             = (int)h->aca_terms.size();
         r.n_aca_retry       = h->aca_retry;

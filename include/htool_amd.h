@@ -138,8 +138,9 @@ typedef struct htg_info {
 
 /* Work decomposition of the product (no counterpart in the reference): rows
  * per row block of the near-field/U phase (16..128), target entries per V
- * chunk and per phase-1 wavefront, most chunks per wavefront, and stacked V
- * rows per chunk (the rank bucket of a source-cluster panel). */
+ * chunk and per phase-1 wavefront, most chunks per wavefront, and most stacked
+ * V rows per chunk (the rank bucket of a source-cluster panel; a panel no
+ * wider than vchunk_el takes fewer rows per chunk instead of a column cut). */
 typedef struct htg_layout {
     int rb_max;
     int vchunk_el;
