@@ -31,6 +31,10 @@ inline int v_first(const Leaf &L, int c) { return L.oj & ~(CLASS_ALIGN[c] - 1); 
 inline int v_stride(const Leaf &L, int c) { return round_up(L.oj + L.n, CLASS_ALIGN[c]) - v_first(L, c); }
 inline int terms(const Leaf &L, int c) { return L.k <= 0 ? 0 : (c == 0 ? L.k_hi : L.k - L.k_hi); }
 
+// Rows per workgroup of the single-RHS row phase with two precision classes (a
+// row block of up to rb_max rows is cut into parts of at most this many).
+constexpr int ROWS_SPLIT = 32;
+
 // V chunks grouped per wavefront, and the first chunk of every row group cut
 // into several chunks (its partial sums are added by a second pass).
 struct VTileList {
@@ -93,6 +97,9 @@ struct htg_hmatrix {
     bool scaled = false; // some low-rank terms carry scales (recompressed)
 
     std::vector<int> rb_row0, rb_order;
+    // single-RHS row phase with two precision classes: rb_order with every row
+    // block cut into parts of at most ROWS_SPLIT rows (entry = rb | p << 24 | np << 28)
+    std::vector<int> rb_parts;
     // per leaf, per covered row block: (row block, class-0 tile offset, class-1 tile offset)
     std::vector<std::vector<std::tuple<int, uint64_t, uint64_t>>> leaf_tiles;
     std::vector<std::vector<int>> panels; // low-rank leaves per source cluster
@@ -107,7 +114,7 @@ struct htg_hmatrix {
     // instead (single right-hand side in FP32 arithmetic).
     VTileList vt[2];
 
-    DevArray<int> d_perm_t, d_perm_s, d_rb_row0, d_rb_order;
+    DevArray<int> d_perm_t, d_perm_s, d_rb_row0, d_rb_order, d_rb_parts;
     DevArray<double> d_tscale;
     DevArray<double> d_z, d_part; // z: cluster-ordered x (tpad entries) then t, per RHS
     DevArray<double> d_hx, d_hy;  // staging for host-pointer products
@@ -158,6 +165,8 @@ void build_layout(htg_hmatrix &h) {
     }
     h.rb_row0.push_back(h.nr);
     const int nrb = (int)h.rb_row0.size() - 1;
+    if (nrb > 0xffffff)
+        throw Error(HTG_ERR_UNSUPPORTED, "more than 2^24 row blocks");
     h.tpad        = round_up(h.nc, 8);
 
     // Low-rank leaves sharing a source cluster (same column range) form one
@@ -282,6 +291,15 @@ void build_layout(htg_hmatrix &h) {
         return (int64_t)(h.rb_row0[r + 1] - h.rb_row0[r]) * cols;
     };
     std::stable_sort(h.rb_order.begin(), h.rb_order.end(), [&](int a, int b) { return work(a) > work(b); });
+    // Single right-hand side with two precision classes: more, shorter
+    // workgroups balance the phase better (a row block's columns are staged
+    // once per part).
+    h.rb_parts.clear();
+    for (int r : h.rb_order) {
+        const int np = (h.rb_row0[r + 1] - h.rb_row0[r] + ROWS_SPLIT - 1) / ROWS_SPLIT;
+        for (int p = 0; p < np; p++)
+            h.rb_parts.push_back(r | p << 24 | (np > 1 ? np : 0) << 28);
+    }
 
     // V arena: per class, panel by panel, the stacked V rows of the panel's
     // leaves with a common stride.
@@ -384,6 +402,7 @@ void upload_layout(htg_hmatrix &h) {
     h.d_perm_s.upload(h.perm_s);
     h.d_rb_row0.upload(h.rb_row0);
     h.d_rb_order.upload(h.rb_order);
+    h.d_rb_parts.upload(h.rb_parts);
     for (ClassLayout &C : h.cls) {
         C.d_rb_col0.upload(C.rb_col0);
         C.d_rb_aoff.upload(C.rb_aoff);
@@ -471,7 +490,11 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
                                 has1 ? arena_of<TS1>(C1, true) : nullptr, z, h.d_part.p, tscale, MU, st);
         if (ev)
             HTG_HIP(hipEventRecord(ev[2], st));
-        launch_rows<TS0, TS1, TC>(h.d_rb_order.p, h.d_rb_row0.p, (int)h.rb_row0.size() - 1, rc0,
+        // row-block parts: measured faster with two precision classes, slower
+        // with FP32 alone (one staging pass streams more bytes there)
+        const bool parts = MU == 1 && std::is_same<TC, float>::value && has1;
+        launch_rows<TS0, TS1, TC>(parts ? h.d_rb_parts.p : h.d_rb_order.p, h.d_rb_row0.p,
+                                  parts ? (int)h.rb_parts.size() : (int)h.rb_row0.size() - 1, rc0,
                                   arena_of<TS0>(C0, false), rc1, has1 ? arena_of<TS1>(C1, false) : nullptr, z,
                                   h.d_perm_t.p, h.nr, done, MU, d_y, st);
         if (ev)

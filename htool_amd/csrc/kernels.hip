@@ -1517,15 +1517,15 @@ __global__ void __launch_bounds__(WG) k_vx_sum(const VTile *first, int nmulti,
 // column) is staged through LDS. Both classes add into the same FP64 row
 // sums, which are scattered to original order.
 template <typename TS, typename TC, int MU>
-__device__ __forceinline__ void rows_class(const TS *arena, const RowClass &rc, int rb, int R, const TC *z, TC *s_z,
-                                           double *s_acc) {
+__device__ __forceinline__ void rows_class(const TS *arena, const RowClass &rc, int rb, int lo, int R, const TC *z,
+                                           TC *s_z, double *s_acc) {
     constexpr int ZCOLS = ZCAP / MU;
     constexpr int W     = Vec<TS>::W;
     const int *colidx   = rc.colidx;
     const int c0 = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - c0;
     const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
     const uint64_t *rb_aoff = rc.rb_aoff;
-    const TS *A  = arena + rb_aoff[rb];
+    const TS *A  = arena + rb_aoff[rb] + (size_t)lo * Cp;
     constexpr int ZPT = ZCAP / WG; // staged entries per thread
     for (int cc = 0; cc < Cp; cc += ZCOLS) {
         const int nz = min(ZCOLS, Cp - cc);
@@ -1567,13 +1567,18 @@ __global__ void __launch_bounds__(WG) k_rows(const int *rb_order, const int *rb_
                                              int mu_base, double *y) {
     __shared__ TC s_z[ZCAP];
     __shared__ double s_acc[ROWMAX * MU];
-    const int rb = rb_order[blockIdx.x];
-    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
+    // entry: row block (bits 0-23); a nonzero part count np (bits 28-31)
+    // selects rows [R p / np, R (p + 1) / np) of it, p in bits 24-27
+    const int ent = rb_order[blockIdx.x];
+    const int rb = ent & 0xffffff, p = (ent >> 24) & 15, np = (int)((unsigned)ent >> 28);
+    const int rf0 = rb_row0[rb], Rf = rb_row0[rb + 1] - rf0;
+    const int lo = np ? Rf * p / np : 0, hi = np ? Rf * (p + 1) / np : Rf;
+    const int r0 = rf0 + lo, R = hi - lo;
     for (int e = threadIdx.x; e < R * MU; e += WG)
         s_acc[e] = 0;
-    rows_class<TS0, TC, MU>(arena0, c0, rb, R, z, s_z, s_acc);
+    rows_class<TS0, TC, MU>(arena0, c0, rb, lo, R, z, s_z, s_acc);
     if (arena1)
-        rows_class<TS1, TC, MU>(arena1, c1, rb, R, z, s_z, s_acc);
+        rows_class<TS1, TC, MU>(arena1, c1, rb, lo, R, z, s_z, s_acc);
     __syncthreads();
     for (int e = threadIdx.x; e < R * MU; e += WG) {
         const int r = e / MU, s = e % MU;
