@@ -125,3 +125,25 @@ def test_multi_rhs_matrix_core_path(htg, storage, bound):
         assert rel(Y[:, c], H @ X[:, c]) < 1e-6, c
     for c in range(40):
         assert rel(Y[:, c], Y64[:, c]) < bound, c
+
+
+@pytest.mark.parametrize("vchunk", [256, 1024, 8192])
+def test_single_rhs_chunk_cuts(htg, vchunk):
+    """Single-RHS products in FP32 arithmetic walk their own V chunk list
+    (panels no wider than vchunk_el whole, wider ones cut by columns and summed
+    in a second pass) and, with two precision classes, run row blocks in parts;
+    multi-RHS products use the 16-row chunks and whole row blocks. Every cut
+    gives the multi-RHS columns (FP32 arithmetic, other summation order) and the
+    FP64 product within the storage's budget."""
+    x, diag = sphere_case(7000, 12)
+    H = build(htg, x, diag, rc=2e-6, lp=5e-7)
+    X = np.random.default_rng(13).standard_normal((x.shape[0], 3))
+    Y64 = H.mvprod_global_to_global(X)
+    H.set_layout(vchunk_el=vchunk)
+    for storage, bound in (("fp32", 1e-6), ("fp32_fp16", 1e-6), ("fp32_bf16", 8e-6)):
+        H.set_storage(storage)
+        Y = H.mvprod_global_to_global(X)
+        for c in range(3):
+            y1 = H @ X[:, c]
+            assert rel(y1, Y[:, c]) < 1e-6, (storage, c)
+            assert rel(y1, Y64[:, c]) < bound, (storage, c)
