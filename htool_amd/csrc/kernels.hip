@@ -1606,6 +1606,90 @@ constexpr int VPASS32 = 32; // V columns per pass, 32-row tiles
 // Output row of accumulator i of lane l in a 32x32 tile (v_mfma_f32_32x32x2f32).
 __device__ __forceinline__ int mfma32_row(int i, int lane) { return (i & 3) + 4 * (lane >> 5) + 8 * (i >> 2); }
 
+// 16-bit matrix-core products for 16-bit stored entries: v_mfma_f32_32x32x16
+// and 16x16x32 (f16 or bf16), eight entries of one row per lane, the lane's
+// 16-byte load as it is. The FP32 vector entries are split into 16-bit terms
+// whose sum keeps their precision: FP16 entries times a power-of-two pass
+// scale s = h1 + h2 (s brings the pass's largest magnitude into [2^14, 2^15),
+// inside FP16's range), BF16 entries = b1 + b2 + b3 (BF16 has FP32's range, no
+// scale). Each term is one MFMA into the same FP32 accumulators; a pass's sums
+// are multiplied by 1 / s when added into the FP64 totals. Output layouts are
+// those of the FP32 forms (32x32: mfma32_row; 16x16: row 4 (l / 16) + i). The
+// eight k slots of a lane may be any eight columns as long as A and B agree.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+template <typename TS>
+struct Mma16 {
+    static constexpr bool on = false, scaled = false;
+};
+template <>
+struct Mma16<__half> {
+    static constexpr bool on = true, scaled = true;
+    static constexpr int NB = 2;
+    using V                 = f16x8;
+    __device__ static V a(const __half (&x)[8], bool in) {
+        V v;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            v[j] = in ? __builtin_bit_cast(_Float16, x[j]) : (_Float16)0.0f;
+        return v;
+    }
+    __device__ static void split(const float (&b)[8], float sc, V (&o)[NB]) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const float t    = b[j] * sc;
+            const _Float16 h = (_Float16)t;
+            o[0][j]          = h;
+            o[1][j]          = (_Float16)(t - (float)h);
+        }
+    }
+    __device__ static floatx16 mma32(V a, V b, floatx16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
+    __device__ static floatx4 mma16(V a, V b, floatx4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Mma16<__hip_bfloat16> {
+    static constexpr bool on = true, scaled = false;
+    static constexpr int NB = 3;
+    using V                 = bf16x8;
+    __device__ static V a(const __hip_bfloat16 (&x)[8], bool in) {
+        V v;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            v[j] = in ? __builtin_bit_cast(__bf16, x[j]) : (__bf16)0.0f;
+        return v;
+    }
+    __device__ static void split(const float (&b)[8], float, V (&o)[NB]) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const float t  = b[j];
+            const __bf16 h = (__bf16)t;
+            const float r  = t - (float)h;
+            const __bf16 m = (__bf16)r;
+            o[0][j]        = h;
+            o[1][j]        = m;
+            o[2][j]        = (__bf16)(r - (float)m);
+        }
+    }
+    __device__ static floatx16 mma32(V a, V b, floatx16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+    __device__ static floatx4 mma16(V a, V b, floatx4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+
+// Largest of a float over the wave; power-of-two pass scale for a largest
+// magnitude m (1 when m is zero or not finite: the products then carry the
+// infinity or NaN as FP32 ones would).
+__device__ __forceinline__ float wave_max(float m) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        m = fmaxf(m, __shfl_xor(m, o, 64));
+    return m;
+}
+__device__ __forceinline__ float pass_scale(float m) {
+    return m > 0.0f && m <= 3.0e38f ? ldexpf(1.0f, 14 - ilogbf(m)) : 1.0f;
+}
+__device__ __forceinline__ float abs_max4(float m, float4 v) {
+    return fmaxf(fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y))), fmaxf(fabsf(v.z), fabsf(v.w)));
+}
+
 // Phase 2, one precision class of a row block. The workgroup's waves split as
 // T row tiles of 32 rows times S column slices (T S <= 4): wave (t, q) owns
 // rows 32t..32t+31 and columns [q W, (q+1) W) of every 128-column pass, W =
@@ -1657,10 +1741,20 @@ __device__ __forceinline__ void rows_class_mfma(const TS *arena, const RowClass 
         load_a(0, a);
     for (int cc = 0; cc < Cp; cc += RPASS_COLS) {
         const int nz = min(RPASS_COLS, Cp - cc);
+        float zmax = 0.0f; // largest staged magnitude (16-bit FP16 products: the pass scale)
 #pragma unroll
         for (int i = 0; i < ZPT; i++) {
             const int c      = zc + (WG / MU) * i;
-            s_z[zblk(c, zs)] = cc + c < C ? v[i] : 0.0f;
+            const float vz   = cc + c < C ? v[i] : 0.0f;
+            s_z[zblk(c, zs)] = vz;
+            zmax             = fmaxf(zmax, fabsf(vz));
+        }
+        // the kernel's LDS holds the staged pass and then four floats for the waves' maxima
+        float *s_mx = s_z + RPASS_COLS * MU;
+        if constexpr (Mma16<TS>::on && Mma16<TS>::scaled) {
+            zmax = wave_max(zmax);
+            if ((threadIdx.x & 63) == 0)
+                s_mx[threadIdx.x >> 6] = zmax;
         }
         __syncthreads();
         const bool more = cc + RPASS_COLS < Cp;
@@ -1671,25 +1765,51 @@ __device__ __forceinline__ void rows_class_mfma(const TS *arena, const RowClass 
         }
         if (active) {
             floatx16 acc = {0};
+            double inv   = 1.0;
+            if constexpr (Mma16<TS>::on) {
+                using M  = Mma16<TS>;
+                float sc = 1.0f;
+                if constexpr (M::scaled) {
+                    sc  = pass_scale(fmaxf(fmaxf(s_mx[0], s_mx[1]), fmaxf(s_mx[2], s_mx[3])));
+                    inv = 1.0 / (double)sc;
+                }
 #pragma unroll
-            for (int it = 0; it < NIT; it++) {
-                const int k0 = q * WCOL + 2 * VW * it;
-                if (k0 < nz) {
-                    const int kk = k0 + VW * half;
+                for (int it = 0; it < NIT; it++) {
+                    const int k0 = q * WCOL + 2 * VW * it;
+                    if (k0 < nz) {
+                        const int kk    = k0 + VW * half;
+                        const float4 b0 = *reinterpret_cast<const float4 *>(s_z + zblk(kk, lane & 31));
+                        const float4 b1 = *reinterpret_cast<const float4 *>(s_z + zblk(kk + 4, lane & 31));
+                        const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+                        typename M::V bs[M::NB];
+                        M::split(bb, sc, bs);
+                        const typename M::V av = M::a(a[it], true);
 #pragma unroll
-                    for (int m4 = 0; m4 < VW; m4 += 4) {
-                        const float4 b = *reinterpret_cast<const float4 *>(s_z + zblk(kk + m4, lane & 31));
-                        const float bb[4] = {b.x, b.y, b.z, b.w};
+                        for (int u = 0; u < M::NB; u++)
+                            acc = M::mma32(av, bs[u], acc);
+                    }
+                }
+            } else {
 #pragma unroll
-                        for (int m = 0; m < 4; m++)
-                            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cvt<float>(a[it][m4 + m]), bb[m], acc, 0, 0,
-                                                                       0);
+                for (int it = 0; it < NIT; it++) {
+                    const int k0 = q * WCOL + 2 * VW * it;
+                    if (k0 < nz) {
+                        const int kk = k0 + VW * half;
+#pragma unroll
+                        for (int m4 = 0; m4 < VW; m4 += 4) {
+                            const float4 b = *reinterpret_cast<const float4 *>(s_z + zblk(kk + m4, lane & 31));
+                            const float bb[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+                            for (int m = 0; m < 4; m++)
+                                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cvt<float>(a[it][m4 + m]), bb[m], acc, 0,
+                                                                           0, 0);
+                        }
                     }
                 }
             }
 #pragma unroll
             for (int i = 0; i < 16; i++)
-                accd[i] += (double)acc[i];
+                accd[i] += (double)acc[i] * inv;
         }
         __syncthreads();
         if (more && active) {
@@ -1814,27 +1934,62 @@ __device__ __forceinline__ void vx_chunk_mfma16(const VTile &d, const TS *varena
         if (more)
             load(c0 + VPASS16, an, bn);
         floatx4 acc0 = {0}, acc1 = {0};
+        double inv   = 1.0;
+        if constexpr (Mma16<TS>::on) {
+            using M  = Mma16<TS>;
+            float sc = 1.0f;
+            if constexpr (M::scaled) {
+                float m = 0.0f;
 #pragma unroll
-        for (int it = 0; it < NIT; it++) {
-            if (4 * VW * it < nc) {
-                const bool in = 4 * VW * it + VW * g < nc;
+                for (int it = 0; it < NIT; it++)
 #pragma unroll
-                for (int m4 = 0; m4 < VW / 4; m4++) {
-                    const float b0[4] = {b[it][m4][0].x, b[it][m4][0].y, b[it][m4][0].z, b[it][m4][0].w};
-                    const float b1[4] = {b[it][m4][1].x, b[it][m4][1].y, b[it][m4][1].z, b[it][m4][1].w};
+                    for (int m4 = 0; m4 < VW / 4; m4++)
+                        m = abs_max4(abs_max4(m, b[it][m4][0]), b[it][m4][1]);
+                sc  = pass_scale(wave_max(m));
+                inv = 1.0 / (double)sc;
+            }
 #pragma unroll
-                    for (int m = 0; m < 4; m++) {
-                        const float av = in ? cvt<float>(a[it][4 * m4 + m]) : 0.0f;
-                        acc0           = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b0[m], acc0, 0, 0, 0);
-                        acc1           = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b1[m], acc1, 0, 0, 0);
+            for (int it = 0; it < NIT; it++) {
+                if (4 * VW * it < nc) {
+                    const bool in     = 4 * VW * it + VW * g < nc;
+                    const float b0[8] = {b[it][0][0].x, b[it][0][0].y, b[it][0][0].z, b[it][0][0].w,
+                                         b[it][1][0].x, b[it][1][0].y, b[it][1][0].z, b[it][1][0].w};
+                    const float b1[8] = {b[it][0][1].x, b[it][0][1].y, b[it][0][1].z, b[it][0][1].w,
+                                         b[it][1][1].x, b[it][1][1].y, b[it][1][1].z, b[it][1][1].w};
+                    typename M::V s0[M::NB], s1[M::NB];
+                    M::split(b0, sc, s0);
+                    M::split(b1, sc, s1);
+                    const typename M::V av = M::a(a[it], in);
+#pragma unroll
+                    for (int u = 0; u < M::NB; u++) {
+                        acc0 = M::mma16(av, s0[u], acc0);
+                        acc1 = M::mma16(av, s1[u], acc1);
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                if (4 * VW * it < nc) {
+                    const bool in = 4 * VW * it + VW * g < nc;
+#pragma unroll
+                    for (int m4 = 0; m4 < VW / 4; m4++) {
+                        const float b0[4] = {b[it][m4][0].x, b[it][m4][0].y, b[it][m4][0].z, b[it][m4][0].w};
+                        const float b1[4] = {b[it][m4][1].x, b[it][m4][1].y, b[it][m4][1].z, b[it][m4][1].w};
+#pragma unroll
+                        for (int m = 0; m < 4; m++) {
+                            const float av = in ? cvt<float>(a[it][4 * m4 + m]) : 0.0f;
+                            acc0           = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b0[m], acc0, 0, 0, 0);
+                            acc1           = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b1[m], acc1, 0, 0, 0);
+                        }
                     }
                 }
             }
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            accd[0][i] += (double)acc0[i];
-            accd[1][i] += (double)acc1[i];
+            accd[0][i] += (double)acc0[i] * inv;
+            accd[1][i] += (double)acc1[i] * inv;
         }
         if (more) {
 #pragma unroll
@@ -1894,23 +2049,53 @@ __device__ __forceinline__ void vx_chunk_mfma32(const VTile &d, const TS *varena
             if (more)
                 load(c0 + VPASS32, an, bn);
             floatx16 acc = {0};
+            double inv   = 1.0;
+            if constexpr (Mma16<TS>::on) {
+                using M  = Mma16<TS>;
+                float sc = 1.0f;
+                if constexpr (M::scaled) {
+                    float m = 0.0f;
 #pragma unroll
-            for (int it = 0; it < NIT; it++) {
-                if (2 * VW * it < nc) {
-                    const bool in = 2 * VW * it + VW * half < nc;
+                    for (int it = 0; it < NIT; it++)
 #pragma unroll
-                    for (int m4 = 0; m4 < VW / 4; m4++) {
-                        const float bb[4] = {b[it][m4].x, b[it][m4].y, b[it][m4].z, b[it][m4].w};
+                        for (int m4 = 0; m4 < VW / 4; m4++)
+                            m = abs_max4(m, b[it][m4]);
+                    sc  = pass_scale(wave_max(m));
+                    inv = 1.0 / (double)sc;
+                }
 #pragma unroll
-                        for (int m = 0; m < 4; m++)
-                            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(in ? cvt<float>(a[it][4 * m4 + m]) : 0.0f,
-                                                                       bb[m], acc, 0, 0, 0);
+                for (int it = 0; it < NIT; it++) {
+                    if (2 * VW * it < nc) {
+                        const bool in     = 2 * VW * it + VW * half < nc;
+                        const float bb[8] = {b[it][0].x, b[it][0].y, b[it][0].z, b[it][0].w,
+                                             b[it][1].x, b[it][1].y, b[it][1].z, b[it][1].w};
+                        typename M::V bs[M::NB];
+                        M::split(bb, sc, bs);
+                        const typename M::V av = M::a(a[it], in);
+#pragma unroll
+                        for (int u = 0; u < M::NB; u++)
+                            acc = M::mma32(av, bs[u], acc);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int it = 0; it < NIT; it++) {
+                    if (2 * VW * it < nc) {
+                        const bool in = 2 * VW * it + VW * half < nc;
+#pragma unroll
+                        for (int m4 = 0; m4 < VW / 4; m4++) {
+                            const float bb[4] = {b[it][m4].x, b[it][m4].y, b[it][m4].z, b[it][m4].w};
+#pragma unroll
+                            for (int m = 0; m < 4; m++)
+                                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                                    in ? cvt<float>(a[it][4 * m4 + m]) : 0.0f, bb[m], acc, 0, 0, 0);
+                        }
                     }
                 }
             }
 #pragma unroll
             for (int i = 0; i < 16; i++)
-                accd[i] += (double)acc[i];
+                accd[i] += (double)acc[i] * inv;
             if (more) {
 #pragma unroll
                 for (int it = 0; it < NIT; it++) {
