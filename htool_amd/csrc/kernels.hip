@@ -1627,6 +1627,7 @@ struct Mma16<__half> {
     static constexpr bool on = true, scaled = true;
     static constexpr int NB = 2;
     using V                 = f16x8;
+    __device__ static _Float16 el(__half x, bool in) { return in ? __builtin_bit_cast(_Float16, x) : (_Float16)0.0f; }
     __device__ static V a(const __half (&x)[8], bool in) {
         V v;
 #pragma unroll
@@ -1651,6 +1652,7 @@ struct Mma16<__hip_bfloat16> {
     static constexpr bool on = true, scaled = false;
     static constexpr int NB = 3;
     using V                 = bf16x8;
+    __device__ static __bf16 el(__hip_bfloat16 x, bool in) { return in ? __builtin_bit_cast(__bf16, x) : (__bf16)0.0f; }
     __device__ static V a(const __hip_bfloat16 (&x)[8], bool in) {
         V v;
 #pragma unroll
@@ -1673,6 +1675,22 @@ struct Mma16<__hip_bfloat16> {
     __device__ static floatx16 mma32(V a, V b, floatx16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
     __device__ static floatx4 mma16(V a, V b, floatx4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 };
+
+// Real and imaginary parts of two 16-byte loads of complex 16-bit entries
+// (four (re, im) pairs each) as two 8-entry operands; entries of a load that
+// lies past the pass's columns are zero when its flag is false.
+template <typename TS>
+__device__ __forceinline__ void deinterleave16(const TS (&p)[8], const TS (&q)[8], bool inp, bool inq,
+                                               typename Mma16<TS>::V &re, typename Mma16<TS>::V &im) {
+    using M = Mma16<TS>;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        re[j]     = M::el(p[2 * j], inp);
+        im[j]     = M::el(p[2 * j + 1], inp);
+        re[j + 4] = M::el(q[2 * j], inq);
+        im[j + 4] = M::el(q[2 * j + 1], inq);
+    }
+}
 
 // Largest of a float over the wave; power-of-two pass scale for a largest
 // magnitude m (1 when m is zero or not finite: the products then carry the
@@ -2325,12 +2343,13 @@ __device__ __forceinline__ void load_b_slots(const float *p, float (&b)[VE]) {
 }
 
 // Combines the Re(A) and Im(A) accumulators of a 32x32 tile into slot values.
-__device__ __forceinline__ void combine32(const floatx16 &ar, const floatx16 &ai, int lane, double (&accd)[16]) {
+__device__ __forceinline__ void combine32(const floatx16 &ar, const floatx16 &ai, int lane, double (&accd)[16],
+                                          double inv = 1.0) {
     const bool im_slot = (lane & 31) >= 16;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         const float t = __shfl_xor(ai[i], 16, 64);
-        accd[i] += (double)(im_slot ? ar[i] + t : ar[i] - t);
+        accd[i] += (double)(im_slot ? ar[i] + t : ar[i] - t) * inv;
     }
 }
 
@@ -2379,10 +2398,20 @@ __device__ __forceinline__ void rows_class_mfma_c(const TS *arena, const RowClas
         load_a(0, a);
     for (int cc = 0; cc < Cp; cc += RPASS_C) {
         const int nz = min(RPASS_C, Cp - cc);
+        float zmax = 0.0f; // largest staged magnitude (16-bit FP16 products: the pass scale)
 #pragma unroll
         for (int i = 0; i < ZPT; i++) {
             const int c      = zc + (WG / SL) * i;
-            s_z[zblk(c, zs)] = cc + c < C ? v[i] : 0.0f;
+            const float vz   = cc + c < C ? v[i] : 0.0f;
+            s_z[zblk(c, zs)] = vz;
+            zmax             = fmaxf(zmax, fabsf(vz));
+        }
+        // the kernel's LDS holds the staged pass and then four floats for the waves' maxima
+        float *s_mx = s_z + RPASS_C * SL;
+        if constexpr (Mma16<TS>::on && Mma16<TS>::scaled) {
+            zmax = wave_max(zmax);
+            if ((threadIdx.x & 63) == 0)
+                s_mx[threadIdx.x >> 6] = zmax;
         }
         __syncthreads();
         const bool more = cc + RPASS_C < Cp;
@@ -2393,21 +2422,57 @@ __device__ __forceinline__ void rows_class_mfma_c(const TS *arena, const RowClas
         }
         if (active) {
             floatx16 ar = {0}, ai = {0};
+            double inv  = 1.0;
+            if constexpr (Mma16<TS>::on) {
+                // two loads (eight complex entries) per lane and step: one
+                // 32x32x16 MFMA per 16-bit term for Re(A) and one for Im(A)
+                using M  = Mma16<TS>;
+                float sc = 1.0f;
+                if constexpr (M::scaled) {
+                    sc  = pass_scale(fmaxf(fmaxf(s_mx[0], s_mx[1]), fmaxf(s_mx[2], s_mx[3])));
+                    inv = 1.0 / (double)sc;
+                }
+                static_assert(NIT % 2 == 0, "steps come in pairs");
 #pragma unroll
-            for (int it = 0; it < NIT; it++) {
-                const int k0 = q * WCOL + 2 * VE * it;
-                if (k0 < nz) {
-                    const int kk = k0 + VE * half;
-                    float b[VE];
-                    load_b_slots<VE>(s_z + zblk(kk, lane & 31), b);
+                for (int it = 0; it < NIT; it += 2) {
+                    const int k0 = q * WCOL + 2 * VE * it;
+                    if (k0 < nz) {
+                        const int kk = k0 + VE * half;
+                        float bb[8];
+                        {
+                            const float4 b0 = *reinterpret_cast<const float4 *>(s_z + zblk(kk, lane & 31));
+                            const float4 b1 = *reinterpret_cast<const float4 *>(s_z + zblk(kk + 2 * VE, lane & 31));
+                            bb[0] = b0.x, bb[1] = b0.y, bb[2] = b0.z, bb[3] = b0.w;
+                            bb[4] = b1.x, bb[5] = b1.y, bb[6] = b1.z, bb[7] = b1.w;
+                        }
+                        typename M::V bs[M::NB], are, aim;
+                        M::split(bb, sc, bs);
+                        deinterleave16<TS>(a[it], a[it + 1], true, true, are, aim);
 #pragma unroll
-                    for (int e = 0; e < VE; e++) {
-                        ar = __builtin_amdgcn_mfma_f32_32x32x2f32(cvt<float>(a[it][2 * e]), b[e], ar, 0, 0, 0);
-                        ai = __builtin_amdgcn_mfma_f32_32x32x2f32(cvt<float>(a[it][2 * e + 1]), b[e], ai, 0, 0, 0);
+                        for (int u = 0; u < M::NB; u++) {
+                            ar = M::mma32(are, bs[u], ar);
+                            ai = M::mma32(aim, bs[u], ai);
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int it = 0; it < NIT; it++) {
+                    const int k0 = q * WCOL + 2 * VE * it;
+                    if (k0 < nz) {
+                        const int kk = k0 + VE * half;
+                        float b[VE];
+                        load_b_slots<VE>(s_z + zblk(kk, lane & 31), b);
+#pragma unroll
+                        for (int e = 0; e < VE; e++) {
+                            ar = __builtin_amdgcn_mfma_f32_32x32x2f32(cvt<float>(a[it][2 * e]), b[e], ar, 0, 0, 0);
+                            ai = __builtin_amdgcn_mfma_f32_32x32x2f32(cvt<float>(a[it][2 * e + 1]), b[e], ai, 0, 0,
+                                                                      0);
+                        }
                     }
                 }
             }
-            combine32(ar, ai, lane, accd);
+            combine32(ar, ai, lane, accd, inv);
         }
         __syncthreads();
         if (more && active) {
@@ -2520,25 +2585,65 @@ __device__ __forceinline__ void vx_chunk_mfma16_c(const VTile &d, const TS *vare
             load_b_slots<VE>(z + zblk(j, r + 16), b1[it]);
         }
         floatx4 rr = {0}, ri = {0}, ir = {0}, ii = {0}; // (Re|Im A) x (Re|Im x)
+        double inv = 1.0;
+        if constexpr (Mma16<TS>::on) {
+            // steps in pairs: eight complex entries per lane, one 16x16x32 MFMA
+            // per 16-bit term and (Re|Im A, Re|Im x) pair
+            using M  = Mma16<TS>;
+            float sc = 1.0f;
+            if constexpr (M::scaled) {
+                float m = 0.0f;
 #pragma unroll
-        for (int it = 0; it < NIT; it++) {
-            if (4 * VE * it < nc) {
-                const bool in = 4 * VE * it + VE * g < nc;
+                for (int it = 0; it < NIT; it++)
 #pragma unroll
-                for (int e = 0; e < VE; e++) {
-                    const float xr = cvt<float>(a[it][2 * e]), xi = cvt<float>(a[it][2 * e + 1]);
-                    const float arv = in ? xr : 0.0f, aiv = in ? xi : 0.0f;
-                    rr = __builtin_amdgcn_mfma_f32_16x16x4f32(arv, b0[it][e], rr, 0, 0, 0);
-                    ri = __builtin_amdgcn_mfma_f32_16x16x4f32(arv, b1[it][e], ri, 0, 0, 0);
-                    ir = __builtin_amdgcn_mfma_f32_16x16x4f32(aiv, b0[it][e], ir, 0, 0, 0);
-                    ii = __builtin_amdgcn_mfma_f32_16x16x4f32(aiv, b1[it][e], ii, 0, 0, 0);
+                    for (int e = 0; e < VE; e++)
+                        m = fmaxf(m, fmaxf(fabsf(b0[it][e]), fabsf(b1[it][e])));
+                sc  = pass_scale(wave_max(m));
+                inv = 1.0 / (double)sc;
+            }
+            static_assert(NIT % 2 == 0 && VE == 4, "steps come in pairs of four entries");
+#pragma unroll
+            for (int it = 0; it < NIT; it += 2) {
+                if (4 * VE * it < nc) {
+                    const bool in0 = 4 * VE * it + VE * g < nc, in1 = 4 * VE * (it + 1) + VE * g < nc;
+                    const float x0[8] = {b0[it][0], b0[it][1], b0[it][2], b0[it][3],
+                                         b0[it + 1][0], b0[it + 1][1], b0[it + 1][2], b0[it + 1][3]};
+                    const float x1[8] = {b1[it][0], b1[it][1], b1[it][2], b1[it][3],
+                                         b1[it + 1][0], b1[it + 1][1], b1[it + 1][2], b1[it + 1][3]};
+                    typename M::V s0[M::NB], s1[M::NB], are, aim;
+                    M::split(x0, sc, s0);
+                    M::split(x1, sc, s1);
+                    deinterleave16<TS>(a[it], a[it + 1], in0, in1, are, aim);
+#pragma unroll
+                    for (int u = 0; u < M::NB; u++) {
+                        rr = M::mma16(are, s0[u], rr);
+                        ri = M::mma16(are, s1[u], ri);
+                        ir = M::mma16(aim, s0[u], ir);
+                        ii = M::mma16(aim, s1[u], ii);
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                if (4 * VE * it < nc) {
+                    const bool in = 4 * VE * it + VE * g < nc;
+#pragma unroll
+                    for (int e = 0; e < VE; e++) {
+                        const float xr = cvt<float>(a[it][2 * e]), xi = cvt<float>(a[it][2 * e + 1]);
+                        const float arv = in ? xr : 0.0f, aiv = in ? xi : 0.0f;
+                        rr = __builtin_amdgcn_mfma_f32_16x16x4f32(arv, b0[it][e], rr, 0, 0, 0);
+                        ri = __builtin_amdgcn_mfma_f32_16x16x4f32(arv, b1[it][e], ri, 0, 0, 0);
+                        ir = __builtin_amdgcn_mfma_f32_16x16x4f32(aiv, b0[it][e], ir, 0, 0, 0);
+                        ii = __builtin_amdgcn_mfma_f32_16x16x4f32(aiv, b1[it][e], ii, 0, 0, 0);
+                    }
                 }
             }
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            re[i] += (double)(rr[i] - ii[i]);
-            im[i] += (double)(ri[i] + ir[i]);
+            re[i] += (double)(rr[i] - ii[i]) * inv;
+            im[i] += (double)(ri[i] + ir[i]) * inv;
         }
     }
 #pragma unroll
@@ -2578,20 +2683,53 @@ __device__ __forceinline__ void vx_chunk_mfma32_c(const VTile &d, const TS *vare
                 load_b_slots<VE>(z + zblk((size_t)d.x_off + c0 + kc, c), b[it]);
             }
             floatx16 ar = {0}, ai = {0};
+            double inv  = 1.0;
+            if constexpr (Mma16<TS>::on) {
+                using M  = Mma16<TS>;
+                float sc = 1.0f;
+                if constexpr (M::scaled) {
+                    float m = 0.0f;
 #pragma unroll
-            for (int it = 0; it < NIT; it++) {
-                if (2 * VE * it < nc) {
-                    const bool in = 2 * VE * it + VE * half < nc;
+                    for (int it = 0; it < NIT; it++)
 #pragma unroll
-                    for (int e = 0; e < VE; e++) {
-                        ar = __builtin_amdgcn_mfma_f32_32x32x2f32(in ? cvt<float>(a[it][2 * e]) : 0.0f, b[it][e], ar,
-                                                                  0, 0, 0);
-                        ai = __builtin_amdgcn_mfma_f32_32x32x2f32(in ? cvt<float>(a[it][2 * e + 1]) : 0.0f,
-                                                                  b[it][e], ai, 0, 0, 0);
+                        for (int e = 0; e < VE; e++)
+                            m = fmaxf(m, fabsf(b[it][e]));
+                    sc  = pass_scale(wave_max(m));
+                    inv = 1.0 / (double)sc;
+                }
+                static_assert(NIT % 2 == 0 && VE == 4, "steps come in pairs of four entries");
+#pragma unroll
+                for (int it = 0; it < NIT; it += 2) {
+                    if (2 * VE * it < nc) {
+                        const bool in0 = 2 * VE * it + VE * half < nc, in1 = 2 * VE * (it + 1) + VE * half < nc;
+                        const float xb[8] = {b[it][0], b[it][1], b[it][2], b[it][3],
+                                             b[it + 1][0], b[it + 1][1], b[it + 1][2], b[it + 1][3]};
+                        typename M::V bs[M::NB], are, aim;
+                        M::split(xb, sc, bs);
+                        deinterleave16<TS>(a[it], a[it + 1], in0, in1, are, aim);
+#pragma unroll
+                        for (int u = 0; u < M::NB; u++) {
+                            ar = M::mma32(are, bs[u], ar);
+                            ai = M::mma32(aim, bs[u], ai);
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int it = 0; it < NIT; it++) {
+                    if (2 * VE * it < nc) {
+                        const bool in = 2 * VE * it + VE * half < nc;
+#pragma unroll
+                        for (int e = 0; e < VE; e++) {
+                            ar = __builtin_amdgcn_mfma_f32_32x32x2f32(in ? cvt<float>(a[it][2 * e]) : 0.0f, b[it][e],
+                                                                      ar, 0, 0, 0);
+                            ai = __builtin_amdgcn_mfma_f32_32x32x2f32(in ? cvt<float>(a[it][2 * e + 1]) : 0.0f,
+                                                                      b[it][e], ai, 0, 0, 0);
+                        }
                     }
                 }
             }
-            combine32(ar, ai, lane, accd);
+            combine32(ar, ai, lane, accd, inv);
         }
 #pragma unroll
         for (int i = 0; i < 16; i++) {

@@ -147,3 +147,20 @@ def test_single_rhs_chunk_cuts(htg, vchunk):
             y1 = H @ X[:, c]
             assert rel(y1, Y[:, c]) < 1e-6, (storage, c)
             assert rel(y1, Y64[:, c]) < bound, (storage, c)
+
+
+@pytest.mark.parametrize("scale", [1e-7, 1.0, 1e7])
+def test_matrix_core_16bit_terms_any_magnitude(htg, scale):
+    """32-RHS products of 16-bit stored blocks run on the 16-bit matrix cores
+    with the vector entries split into 16-bit terms (FP16: under a power-of-two
+    pass scale): results stay finite and agree with the single-vector product
+    (FP32 arithmetic) whatever the magnitude of x, far outside FP16's range."""
+    x, diag = sphere_case(7000, 14)
+    H = build(htg, x, diag, rc=2e-6, lp=5e-7)
+    X = np.random.default_rng(15).standard_normal((x.shape[0], 32)) * scale
+    for storage in ("fp32_fp16", "fp32_bf16"):
+        H.set_storage(storage)
+        Y = H.mvprod_global_to_global(X)
+        assert np.isfinite(Y).all(), storage
+        for c in (0, 17, 31):
+            assert rel(Y[:, c], H @ X[:, c]) < 1e-6, (storage, c)
