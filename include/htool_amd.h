@@ -144,7 +144,9 @@ typedef struct htg_info {
  * per row block of the near-field/U phase (16..128), target entries per V
  * chunk and per phase-1 wavefront, most chunks per wavefront, and most stacked
  * V rows per chunk (the rank bucket of a source-cluster panel; a panel no
- * wider than vchunk_el takes fewer rows per chunk instead of a column cut). */
+ * wider than vchunk_el takes fewer rows per chunk instead of a column cut).
+ * A build starts from {128, 8192, 2048, 4, 16}, with vrows 32 when it puts
+ * terms in the 16-bit class. */
 typedef struct htg_layout {
     int rb_max;
     int vchunk_el;
