@@ -107,12 +107,14 @@ struct htg_hmatrix {
     std::vector<double> tscale; // per z entry (x part unused)
     int t_size = 0, part_size = 0;
     ClassLayout cls[2];
-    // V chunks of both classes, grouped per wavefront, in two cuts of the same
-    // V arena: vt[0] cuts every panel into row groups of vrows rows and wide
-    // groups into column chunks (matrix-core tiles, FP64 rows); vt[1] keeps
-    // every panel no wider than vchunk_el whole and shrinks its row groups
-    // instead (single right-hand side in FP32 arithmetic).
-    VTileList vt[2];
+    // V chunks of both classes, grouped per wavefront, in three cuts of the
+    // same V arena: vt[0] cuts every panel into row groups of vrows rows and
+    // wide groups into column chunks of about vchunk_el entries (single right-
+    // hand side in FP64); vt[1] keeps every panel no wider than vchunk_el whole
+    // and shrinks its row groups instead (single right-hand side in FP32
+    // arithmetic); vt[2] is vt[0] with chunks of twice vchunk_el entries
+    // (several right-hand sides: longer matrix-core passes per chunk).
+    VTileList vt[3];
 
     DevArray<int> d_perm_t, d_perm_s, d_rb_row0, d_rb_order, d_rb_parts;
     DevArray<double> d_tscale;
@@ -322,13 +324,14 @@ void build_layout(htg_hmatrix &h) {
                 C.varena_size = align_up(C.varena_size + (uint64_t)K * ld);
         }
     }
-    // V chunks, class 0 then class 1, grouped per wavefront, in both cuts. A
-    // panel's stacked rows are cut into groups of at most vrows rows; in cut 0
-    // (and for a panel wider than vchunk_el in cut 1) each group's columns are
-    // cut into chunks of about vchunk_el entries, in cut 1 a narrower panel's
-    // groups keep all columns and take at most vchunk_el entries.
+    // V chunks, class 0 then class 1, grouped per wavefront, in the three
+    // cuts. A panel's stacked rows are cut into groups of at most vrows rows;
+    // in cuts 0 and 2 (and for a panel wider than vchunk_el in cut 1) each
+    // group's columns are cut into chunks of about vchunk_el entries (twice
+    // that in cut 2), in cut 1 a narrower panel's groups keep all columns and
+    // take at most vchunk_el entries.
     h.part_size = 0;
-    for (int cut = 0; cut < 2; cut++) {
+    for (int cut = 0; cut < 3; cut++) {
         VTileList &T = h.vt[cut];
         T.tiles.clear();
         T.multi.clear();
@@ -354,10 +357,11 @@ void build_layout(htg_hmatrix &h) {
                 const uint64_t base  = C.leaf_v_off[first];
                 const int tfirst     = t_entry(h, first, c == 0 ? 0 : L0.k_hi);
                 const bool whole     = cut == 1 && ld <= h.lay.vchunk_el;
+                const int chunk_el   = cut == 2 ? 2 * h.lay.vchunk_el : h.lay.vchunk_el;
                 const int rows_g     = whole ? std::max(1, std::min(h.lay.vrows, h.lay.vchunk_el / ld)) : h.lay.vrows;
                 for (int g0 = 0; g0 < K; g0 += rows_g) {
                     const int kr     = std::min(rows_g, K - g0);
-                    const int cpc    = whole ? ld : round_up(std::max(64, h.lay.vchunk_el / kr), CLASS_ALIGN[c]);
+                    const int cpc    = whole ? ld : round_up(std::max(64, chunk_el / kr), CLASS_ALIGN[c]);
                     const int nparts = (ld + cpc - 1) / cpc;
                     const int pbase  = part_size;
                     if (nparts > 1)
@@ -484,7 +488,7 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
         if (ev)
             HTG_HIP(hipEventRecord(ev[1], st));
         const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
-        const VTileList &T = h.vt[MU == 1 && std::is_same<TC, float>::value ? 1 : 0];
+        const VTileList &T = h.vt[MU > 1 ? 2 : std::is_same<TC, float>::value ? 1 : 0];
         launch_vx<TS0, TS1, TC>(T.d_vg_off.p, (int)T.vg_off.size() - 1, T.d_tiles.p, T.d_multi.p,
                                 (int)T.multi.size(), arena_of<TS0>(C0, true),
                                 has1 ? arena_of<TS1>(C1, true) : nullptr, z, h.d_part.p, tscale, MU, st);
@@ -521,7 +525,8 @@ void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream
         if (ev)
             HTG_HIP(hipEventRecord(ev[1], st));
         const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
-        const VTileList &T = h.vt[0];
+        // longer chunks measured faster for the FP32-arithmetic passes only
+        const VTileList &T = h.vt[MU > 1 && std::is_same<R, float>::value ? 2 : 0];
         launch_vx_c<TS0, TS1, R>(T.d_vg_off.p, (int)T.vg_off.size() - 1, T.d_tiles.p, T.d_multi.p,
                                  (int)T.multi.size(), arena_of<TS0>(C0, true),
                                  has1 ? arena_of<TS1>(C1, true) : nullptr, z, h.d_part.p, tscale, MU, st);
