@@ -386,6 +386,21 @@ def main():
     err = {s: float(np.linalg.norm(ys[s][rows] - yd) / np.linalg.norm(yd)) for s in ys}
     err_vs_fp64 = {s: float(np.linalg.norm(ys[s] - ys["fp64"]) / np.linalg.norm(ys["fp64"])) for s in ys}
 
+    # The same mixed product from host buffers (x copied in and y out over PCIe
+    # by the library, pageable memory): reported beside the value, never as it.
+    from htool_amd import capi
+    xh = np.ascontiguousarray(x)
+    yh = np.zeros(n)
+    for _ in range(3):
+        capi.check(capi.lib().htg_hmatrix_mvprod(Hm._handle(), capi.ptr(xh), capi.ptr(yh), 1))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        capi.check(capi.lib().htg_hmatrix_mvprod(Hm._handle(), capi.ptr(xh), capi.ptr(yh), 1))
+    host_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    host = {"ms": round(host_ms, 4), "gbps": round(res["mixed"]["bytes"] / (host_ms * 1e-3) / 1e9, 2),
+            "rel_diff_vs_device": float(np.linalg.norm(yh - ys["mixed"]) / np.linalg.norm(ys["mixed"]))}
+    log(f"mixed from host buffers: {host}")
+
     mixed = res["mixed"]
     value = mixed["bytes"] / (mixed["ms"] * 1e-3) / 1e9
     # dominant kernel of the headline step and its own achieved bandwidth
@@ -444,6 +459,7 @@ def main():
         "fp32": entry("fp32"),
         "mixed": entry("mixed"),
         "speedup_mixed_vs_fp64": round(res["fp64"]["ms"] / mixed["ms"], 3),
+        "mixed_host_buffers_pcie": host,
         "build_s": round(build_s, 3),
         "build_mixed_s": round(build_m_s, 3),
         "leaves": {"dense": info.n_dense, "lowrank": info.n_lowrank, "false_positive": info.n_false_positive,
