@@ -1072,6 +1072,24 @@ __global__ void k_scatter_x32(const double *x, const int *iperm_s, int nc, int m
     o[1]      = make_float4(v[4], v[5], v[6], v[7]);
 }
 
+// FP64 32-wide passes keep z RHS-interleaved, the staging layout itself: one
+// coalesced step, RHS s0..s0+7 of original entry i to z[iperm[i] * 32 + s0..],
+// and zeros for the padding entries [nc, tpad).
+__global__ void k_scatter_x64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, double *z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tpad)
+        return;
+    const int s0 = 8 * blockIdx.y;
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+        v[u] = i < nc ? x[i + (size_t)nc * (mu_base + s0 + u)] : 0.0;
+    double2 *o = reinterpret_cast<double2 *>(z + (size_t)(i < nc ? iperm_s[i] : i) * 32 + s0);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+        o[u] = make_double2(v[2 * u], v[2 * u + 1]);
+}
+
 __global__ void k_block_x32(const float *stage, int nc, int tpad, float *z) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x; // (group, RHS)
     if (e >= tpad * 8)
@@ -2833,6 +2851,25 @@ struct cpair {
     R re, im;
 };
 
+// FP64 complex 16-wide passes: RHS s0..s0+7 of original entry i to
+// z[(iperm[i] * 16 + s) * 2 + {re, im}] (128 contiguous bytes per thread,
+// coalesced reads of consecutive i), zeros for the padding entries [nc, tpad).
+__global__ void k_scatter_xc64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, double *z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tpad)
+        return;
+    const int s0 = 8 * blockIdx.y;
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+        v[u] = i < nc ? *reinterpret_cast<const double2 *>(x + 2 * (i + (size_t)nc * (mu_base + s0 + u)))
+                      : make_double2(0.0, 0.0);
+    double2 *o = reinterpret_cast<double2 *>(z) + (size_t)(i < nc ? iperm_s[i] : i) * 16 + s0;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+        o[u] = v[u];
+}
+
 template <typename R, int MU>
 __global__ void k_gather_xc(const double *x, const int *perm_s, int nc, int tpad, int mu_base, R *z) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3337,6 +3374,11 @@ void launch_gather_x(const double *x, const int *perm_s, int nc, int tpad, int m
 #undef HTG_GATHER
 }
 
+void launch_gather_x64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, double *z, hipStream_t st) {
+    hipLaunchKernelGGL(k_scatter_x64, dim3((tpad + WG - 1) / WG, MFMA_MU / 8), dim3(WG), 0, st, x, iperm_s, nc, tpad,
+                       mu_base, z);
+}
+
 void launch_gather_x32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *stage, float *z,
                        hipStream_t st) {
     hipLaunchKernelGGL(k_scatter_x32, dim3((nc + WG - 1) / WG, MFMA_MU / 8), dim3(WG), 0, st, x, iperm_s, nc, mu_base,
@@ -3434,6 +3476,12 @@ void launch_gather_xc(const double *x, const int *perm_s, int nc, int tpad, int 
 #define HTG_GATHER_C(M) hipLaunchKernelGGL((k_gather_xc<R, M>), grid, dim3(WG), 0, st, x, perm_s, nc, tpad, mu_base, z)
     HTG_MU_SWITCH_C(MU, HTG_GATHER_C)
 #undef HTG_GATHER_C
+}
+
+void launch_gather_xc64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, double *z,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_scatter_xc64, dim3((tpad + WG - 1) / WG, MU_C64 / 8), dim3(WG), 0, st, x, iperm_s, nc, tpad,
+                       mu_base, z);
 }
 
 template <typename TS0, typename TS1, typename R>
