@@ -524,14 +524,18 @@ void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream
         const int MU = pick_mu_chunk(mu - done, 16);
         if (ev)
             HTG_HIP(hipEventRecord(ev[0], st));
+        if (MU == 16 && !h.d_iperm_s.p) {
+            std::vector<int> ip(h.nc);
+            for (int j = 0; j < h.nc; j++)
+                ip[h.perm_s[j]] = j;
+            h.d_iperm_s.upload(ip);
+        }
         if (std::is_same<R, double>::value && MU == 16) {
-            if (!h.d_iperm_s.p) {
-                std::vector<int> ip(h.nc);
-                for (int j = 0; j < h.nc; j++)
-                    ip[h.perm_s[j]] = j;
-                h.d_iperm_s.upload(ip);
-            }
             launch_gather_xc64(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, reinterpret_cast<double *>(z), st);
+        } else if (std::is_same<R, float>::value && MU == 16) {
+            if (!h.d_xstage.p)
+                h.d_xstage.alloc((size_t)h.nc * 32);
+            launch_gather_xc32(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, reinterpret_cast<float *>(z), st);
         } else {
             launch_gather_xc<R>(d_x, h.d_perm_s.p, h.nc, h.tpad, done, MU, z, st);
         }

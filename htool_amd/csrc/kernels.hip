@@ -2799,6 +2799,28 @@ __global__ void k_gather_xc_blk(const double *x, const int *perm_s, int nc, int 
     z[zblk(j, s + 16)] = im;
 }
 
+// The same in two coalesced steps (as for the real 32-wide passes): complex RHS
+// s0..s0+7 of original entry i to slots s0.. (Re) and 16 + s0.. (Im) of row
+// iperm[i] of an interleaved staging copy, then k_block_x32 into the blocks.
+__global__ void k_scatter_xc32(const double *x, const int *iperm_s, int nc, int mu_base, float *stage) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    const int s0 = 8 * blockIdx.y;
+    float re[8], im[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const double2 v = *reinterpret_cast<const double2 *>(x + 2 * (i + (size_t)nc * (mu_base + s0 + u)));
+        re[u]           = (float)v.x;
+        im[u]           = (float)v.y;
+    }
+    float4 *o = reinterpret_cast<float4 *>(stage + (size_t)iperm_s[i] * 32 + s0);
+    o[0]      = make_float4(re[0], re[1], re[2], re[3]);
+    o[1]      = make_float4(re[4], re[5], re[6], re[7]);
+    o[4]      = make_float4(im[0], im[1], im[2], im[3]); // slot 16 + s0 is 16 floats on
+    o[5]      = make_float4(im[4], im[5], im[6], im[7]);
+}
+
 template <typename TS>
 __global__ void k_convert(const double *src, TS *dst, uint64_t n) {
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x)
@@ -3476,6 +3498,13 @@ void launch_gather_xc(const double *x, const int *perm_s, int nc, int tpad, int 
 #define HTG_GATHER_C(M) hipLaunchKernelGGL((k_gather_xc<R, M>), grid, dim3(WG), 0, st, x, perm_s, nc, tpad, mu_base, z)
     HTG_MU_SWITCH_C(MU, HTG_GATHER_C)
 #undef HTG_GATHER_C
+}
+
+void launch_gather_xc32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *stage, float *z,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_scatter_xc32, dim3((nc + WG - 1) / WG, MFMA_MU_C / 8), dim3(WG), 0, st, x, iperm_s, nc,
+                       mu_base, stage);
+    hipLaunchKernelGGL(k_block_x32, dim3((tpad * 8 + WG - 1) / WG), dim3(WG), 0, st, stage, nc, tpad, z);
 }
 
 void launch_gather_xc64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, double *z,
