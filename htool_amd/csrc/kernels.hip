@@ -1879,8 +1879,11 @@ template <typename TS0, typename TS1>
 __global__ void __launch_bounds__(WG) k_rows_mfma(const int *rb_order, const int *rb_row0, RowClass c0,
                                                   const TS0 *arena0, RowClass c1, const TS1 *arena1, const float *z,
                                                   const int *perm_t, int nr, int mu_base, double *y) {
-    // LDS: the staged pass (16 KB), then the column slices' partial row sums (24 KB)
+    // LDS: the staged pass (16 KB) and four floats of pass maxima, then the
+    // column slices' partial row sums (24 KB)
     __shared__ __align__(16) double smem[3 * 16 * 64];
+    static_assert(RPASS_COLS * MFMA_MU * sizeof(float) + 4 * sizeof(float) <= sizeof(smem),
+                  "staged pass and the waves' maxima fit the workgroup's LDS");
     float *s_z   = reinterpret_cast<float *>(smem);
     const int rb = rb_order[blockIdx.x];
     const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
@@ -2526,6 +2529,8 @@ __global__ void __launch_bounds__(WG) k_rows_mfma_c(const int *rb_order, const i
                                                     const float *z, const int *perm_t, int nr, int mu_base,
                                                     double *y) {
     __shared__ __align__(16) double smem[3 * 16 * 64];
+    static_assert(RPASS_C * 32 * sizeof(float) + 4 * sizeof(float) <= sizeof(smem),
+                  "staged pass and the waves' maxima fit the workgroup's LDS");
     float *s_z   = reinterpret_cast<float *>(smem);
     const int rb = rb_order[blockIdx.x];
     const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
