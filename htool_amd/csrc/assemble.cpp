@@ -21,6 +21,8 @@ constexpr int CAP_FIRST = 64;                       // ACA terms reserved per bl
 constexpr int CAP_MAX   = 512;                      // must match the ACA kernel's CAPMAX
 constexpr size_t WS_MAX = size_t(1) << 30;          // doubles of ACA workspace per sub-batch (8 GiB)
 constexpr int ACAW_MIN_DEFAULT = 16384;             // M + N from which a block runs the multi-workgroup ACA
+constexpr int ACAW_MIN_DEFAULT_C = 4096;            // the same for complex entries (costlier per entry: the
+                                                    // one-workgroup kernel's largest blocks set its length)
 
 void PointsOnDevice::upload(const double *pts, const std::vector<int> &perm_h) {
     const size_t n = perm_h.size();
@@ -300,7 +302,7 @@ struct Assembler {
     // with M + N >= aca_wide_min on the multi-workgroup path, the others one
     // workgroup each. Ranks in block order.
     std::vector<int> run_aca(const std::vector<AcaBlock> &blocks, const std::vector<uint64_t> &vis_off) {
-        const int wmin = prm.aca_wide_min == 0 ? ACAW_MIN_DEFAULT : prm.aca_wide_min;
+        const int wmin = prm.aca_wide_min != 0 ? prm.aca_wide_min : esz == 2 ? ACAW_MIN_DEFAULT_C : ACAW_MIN_DEFAULT;
         std::vector<int> narrow, wide;
         for (size_t i = 0; i < blocks.size(); i++)
             ((wmin > 0 && blocks[i].M + blocks[i].N >= wmin) ? wide : narrow).push_back((int)i);

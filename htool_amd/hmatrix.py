@@ -132,7 +132,8 @@ class HMatrix:
 
     def set_aca_wide_min(self, m: int):
         """Blocks with M + N >= m run the multi-workgroup ACA (0: the library
-        default, 16384; negative: never). Factors and ranks do not change."""
+        default, 16384 for real and 4096 for complex entries; negative: never).
+        Factors and ranks do not change."""
         self.aca_wide_min = int(m)
 
     # -- assembly ---------------------------------------------------------------

@@ -104,7 +104,8 @@ typedef struct htg_params {
                                        (no counterpart in the reference); the ranks do not change */
     double predictor_margin;        /* room above the prediction: cap = ceil((1 + margin) t) + 2 */
     int aca_wide_min;               /* blocks with M + N >= this run the multi-workgroup ACA (same
-                                       factors and ranks); 0: 16384, < 0: never */
+                                       factors and ranks); 0: 16384 (real entries) or 4096
+                                       (complex entries), < 0: never */
     int symmetry;                   /* 'N' (or 0): general; 'S': symmetric storage, one triangle of
                                        blocks kept (HMatrix(..., 'S', UPLO), hmatrix.hpp:115-122);
                                        'H' is rejected (no Hermitian kernel family) */
