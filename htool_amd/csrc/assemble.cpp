@@ -83,7 +83,6 @@ struct Assembler {
     std::vector<Leaf> out;
     int false_positive = 0;
     // compressed store: U (M x k col-major) then V (k x N row-major) per low-rank leaf
-    std::vector<DevArray<double> *> store_chunks;
     DevArray<double> store;
     uint64_t store_used = 0;     // entries (complex entries for the complex families)
     Assembly *rec = nullptr; // ACA records and counters
@@ -421,7 +420,9 @@ struct Assembler {
     void grow_store(uint64_t need) { // need: entries
         if (need * esz <= store.n)
             return;
-        uint64_t cap = std::max<uint64_t>(need * esz, store.n + store.n / 2 + (1u << 20));
+        // doubling: few reallocations (each frees a large buffer, which later
+        // builds in the same process pay for)
+        uint64_t cap = std::max<uint64_t>(need * esz, std::max<uint64_t>(2 * store.n, uint64_t(1) << 20));
         double *np   = nullptr;
         HTG_HIP(hipMalloc(&np, cap * sizeof(double)));
         if (store.p && store_used)
@@ -456,7 +457,7 @@ void assemble(const htg_params &prm, const htg_generator &gen, int nt, const dou
     res.delay_dense = prm.delay_dense != 0;
     if (res.cplx && prm.recompress_tol > 0)
         throw Error(HTG_ERR_UNSUPPORTED, "QR+SVD recompression is implemented for the real kernel families only");
-    Assembler as{T, S, prm, res.kd, res.dT(), res.dS(), {}, {}, {}, 0, {}, {}, 0, &res, res.cplx ? 2 : 1};
+    Assembler as{T, S, prm, res.kd, res.dT(), res.dS(), {}, {}, {}, 0, {}, 0, &res, res.cplx ? 2 : 1};
     const char uplo = sym_uplo(prm);
     as.sym          = uplo != 'N';
     const std::vector<BlockTask> tasks = build_block_tasks(T, S, prm.eta, (size_t)prm.maxblocksize, uplo);
