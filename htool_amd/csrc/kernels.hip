@@ -1507,6 +1507,33 @@ constexpr int vx_sum_groups_per_block() {
     return MU == 1 ? NWAVE : 1;
 }
 
+// In-order sum of n partial values (E doubles each, stride apart), with up to
+// eight loads in flight: the sum's latency is not one load per chunk.
+template <int E>
+__device__ __forceinline__ void sum_parts(const double *p, size_t stride, int n, double (&v)[E]) {
+    constexpr int U = 8;
+    int c = 0;
+    for (; c + U <= n; c += U) {
+        double w[U][E];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if constexpr (E == 1)
+                w[u][0] = p[(size_t)(c + u) * stride];
+            else
+                Vec<double>::load(p + (size_t)(c + u) * stride, w[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int i = 0; i < E; i++)
+                v[i] += w[u][i];
+    }
+    for (; c < n; c++)
+#pragma unroll
+        for (int i = 0; i < E; i++)
+            v[i] += p[(size_t)c * stride + i];
+}
+
 template <typename TC, int MU, bool BLK = false>
 __global__ void __launch_bounds__(WG) k_vx_sum(const VTile *first, int nmulti,
                                                const double *part, TC *z, const double *tscale) {
@@ -1519,12 +1546,11 @@ __global__ void __launch_bounds__(WG) k_vx_sum(const VTile *first, int nmulti,
     const int km     = d.k * MU;
     const double *pb = part + (size_t)d.part_base * MU;
     for (int e = threadIdx.x % TPG; e < km; e += TPG) {
-        double v = 0;
-        for (int c = 0; c < d.nparts; c++)
-            v += pb[(size_t)c * km + e];
+        double v[1] = {0};
+        sum_parts<1>(pb + e, km, d.nparts, v);
         const int r = e / MU, s = e % MU;
         z[BLK ? zblk(d.t_index + r, s) : (size_t)(d.t_index + r) * MU + s] =
-            (TC)(tscale ? v * tscale[d.t_index + r] : v);
+            (TC)(tscale ? v[0] * tscale[d.t_index + r] : v[0]);
     }
 }
 
@@ -3013,15 +3039,12 @@ __global__ void __launch_bounds__(WG) k_vx_sum_c(const VTile *first, int nmulti,
     const int km     = d.k * MU;
     const double *pb = part + 2 * (size_t)d.part_base * MU;
     for (int e = threadIdx.x % TPG; e < km; e += TPG) {
-        double re = 0, im = 0;
-        for (int c = 0; c < d.nparts; c++) {
-            re += pb[2 * ((size_t)c * km + e)];
-            im += pb[2 * ((size_t)c * km + e) + 1];
-        }
+        double v[2] = {0, 0};
+        sum_parts<2>(pb + 2 * (size_t)e, 2 * (size_t)km, d.nparts, v);
         const double sc = tscale ? tscale[d.t_index + e / MU] : 1.0;
         R *o            = z + 2 * ((size_t)d.t_index * MU + e);
-        o[0]            = (R)(re * sc);
-        o[1]            = (R)(im * sc);
+        o[0]            = (R)(v[0] * sc);
+        o[1]            = (R)(v[1] * sc);
     }
 }
 
