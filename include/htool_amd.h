@@ -165,7 +165,9 @@ const char *htg_status_string(int status);
 /* Replaces HMatrix<T>(t, s, eps, eta) + set_compression + build(A, xt, xs)
  * (include/htool/types/hmatrix.hpp:115-122, 220, 285-341) for the built-in
  * kernel families: cluster and block trees on the host, kernel entries and ACA
- * on the current HIP device. xs == NULL (or gen->same_points) uses the target points. */
+ * on the current HIP device. xs == NULL (or gen->same_points) uses the target points.
+ * The ACA workspace stays allocated per device between builds (it only grows);
+ * builds on one device from several host threads run one at a time. */
 int htg_hmatrix_build(const htg_params *params, const htg_generator *gen, int nt, const double *xt, int ns,
                       const double *xs, htg_hmatrix **out);
 
