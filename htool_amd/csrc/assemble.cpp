@@ -28,7 +28,9 @@ struct AcaScratch {
     DevArray<unsigned char> vis;
 };
 AcaScratch &aca_scratch() {
-    static AcaScratch per_device[64];
+    // never destroyed: freeing device memory from a static destructor can run
+    // after the HIP runtime has shut down; the process exit releases it
+    static AcaScratch *per_device = new AcaScratch[64];
     int dev = 0;
     HTG_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64)
