@@ -40,7 +40,7 @@ class htg_params(C.Structure):
                 ("minclustersize", C.c_int), ("compressor", C.c_int), ("reqrank", C.c_int),
                 ("lowprec_all", C.c_int), ("recompress_tol", C.c_double), ("lowprec_tol", C.c_double),
                 ("predictor", C.c_void_p), ("predictor_margin", C.c_double), ("aca_wide_min", C.c_int),
-                ("symmetry", C.c_int), ("uplo", C.c_int), ("delay_dense", C.c_int)]
+                ("symmetry", C.c_int), ("uplo", C.c_int), ("delay_dense", C.c_int), ("ref_int_overflow", C.c_int)]
 
 
 class htg_layout(C.Structure):
@@ -59,7 +59,7 @@ class htg_info(C.Structure):
                 ("n_aca_retry", C.c_int), ("aca_ws_entries", C.c_int64), ("aca_need_entries", C.c_int64),
                 ("aca_seconds", C.c_double), ("is_complex", C.c_int), ("n_aca_wide", C.c_int),
                 ("symmetry", C.c_int), ("uplo", C.c_int), ("mirrored_entries", C.c_int64),
-                ("mirrored_lowprec_entries", C.c_int64)]
+                ("mirrored_lowprec_entries", C.c_int64), ("n_aca_int_wrap", C.c_int)]
 
 
 class HtgError(RuntimeError):

@@ -40,7 +40,8 @@ AcaScratch &aca_scratch() {
 
 constexpr int CAP_FIRST = 64;                       // ACA terms reserved per block in the first attempt
 constexpr int CAP_MAX   = 512;                      // must match the ACA kernel's CAPMAX
-constexpr size_t WS_MAX = size_t(1) << 30;          // doubles of ACA workspace per sub-batch (8 GiB)
+constexpr size_t WS_MAX = size_t(1) << 30;          // entries of ACA workspace per sub-batch (8 GiB of
+                                                    // real entries, 16 GiB of complex ones)
 constexpr int ACAW_MIN_DEFAULT = 16384;             // M + N from which a block runs the multi-workgroup ACA
 constexpr int ACAW_MIN_DEFAULT_C = 4096;            // the same for complex entries (costlier per entry: the
                                                     // one-workgroup kernel's largest blocks set its length)
@@ -60,7 +61,7 @@ void PointsOnDevice::upload(const double *pts, const std::vector<int> &perm_h) {
 }
 
 KernelDesc kernel_desc(const htg_generator &g, bool same) {
-    if (g.kernel < 0 || g.kernel > HTG_KERNEL_COMPLEX_TEST)
+    if (g.kernel < 0 || g.kernel > HTG_KERNEL_HERMITIAN_TEST)
         throw Error(HTG_ERR_UNSUPPORTED, "unknown kernel family " + std::to_string(g.kernel));
     return KernelDesc{g.kernel, same ? 1 : 0, g.delta, g.diag, g.wavenumber};
 }
@@ -234,6 +235,7 @@ struct Assembler {
                     B.N     = b.size;
                     B.swap  = (prm.compressor == HTG_SYM_PARTIAL_ACA && a.offset < b.offset) ? 1 : 0;
                     B.cap   = caps[w];
+                    B.mn    = aca_size_bound(B.M, B.N, prm.ref_int_overflow != 0);
                     const int n1 = B.swap ? B.N : B.M;
                     B.ws1 = ws_used;
                     B.ws2 = ws_used + (uint64_t)n1 * B.cap;
@@ -273,6 +275,9 @@ struct Assembler {
                     }
                     const int need               = aca_terms_needed(B.M, B.N, k);
                     rec->aca_terms[rec0 + w]     = need;
+                    if (prm.ref_int_overflow ? (k == -1 && B.mn < (int64_t)B.M * B.N && B.M + B.N <= (int64_t)B.M * B.N)
+                                             : aca_int32_wrap_differs(B.M, B.N, k))
+                        rec->aca_int_wrap++;
                     rec->aca_need_entries += (int64_t)(B.M + B.N) * need;
                     if (k == 0) { // fixed rank 0: a zero block with no factors (lrmat.hpp:37-44)
                         lr_out.push_back(Leaf{B.off_t, B.M, B.off_s, B.N, 0});
@@ -606,3 +611,12 @@ void recompress_and_split(const htg_params &prm, std::vector<Leaf> &leaves, DevA
 }
 
 } // namespace htg
+
+extern "C" int htg_release_workspace(void) {
+    return htg::guard([&] {
+        htg::AcaScratch &s = htg::aca_scratch();
+        std::lock_guard<std::mutex> hold(s.lock); // a build on this device holds it until it ends
+        s.ws.reset();
+        s.vis.reset();
+    });
+}

@@ -33,7 +33,7 @@ struct Assembly {
     // first workspace terms; retries and workspace totals.
     std::vector<float> aca_feat;
     std::vector<int> aca_terms, aca_cap;
-    int aca_retry = 0, aca_wide = 0;
+    int aca_retry = 0, aca_wide = 0, aca_int_wrap = 0;
     int64_t aca_ws_entries = 0, aca_need_entries = 0;
     double aca_seconds = 0;
     DevPoints dT() const { return pT.view(); }
@@ -42,10 +42,15 @@ struct Assembly {
 
 KernelDesc kernel_desc(const htg_generator &g, bool same);
 
-// Kept triangle of a symmetric build ('L' or 'U'), 'N' for a general one.
+// Kept triangle of a symmetric or Hermitian build ('L' or 'U'), 'N' for a general one.
 inline char sym_uplo(const htg_params &p) {
     const int s = p.symmetry == 0 ? 'N' : p.symmetry;
     return s == 'N' ? 'N' : (char)p.uplo;
+}
+// Storage kind: 'N', 'S' or 'H' (the block tree and assembly of 'H' are those of 'S').
+inline char sym_kind(const htg_params &p) {
+    const int s = p.symmetry == 0 ? 'N' : p.symmetry;
+    return s == 'H' ? 'H' : (s == 'S' ? 'S' : 'N');
 }
 
 // HMatrix::build (reference: include/htool/types/hmatrix.hpp:285-341, 422-607).

@@ -3,11 +3,13 @@
 #include "assemble.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -89,7 +91,7 @@ struct htg_hmatrix {
     // ACA runs of the build (first attempts): features, terms needed, first workspace terms
     std::vector<float> aca_feat;
     std::vector<int> aca_terms, aca_cap;
-    int aca_retry = 0, aca_wide = 0;
+    int aca_retry = 0, aca_wide = 0, aca_int_wrap = 0;
     int64_t aca_ws_entries = 0, aca_need_entries = 0;
     double aca_seconds = 0;
     char symmetry = 'N', uplo = 'N'; // symmetric storage keeps one triangle; `leaves` adds mirrored copies
@@ -124,6 +126,12 @@ struct htg_hmatrix {
     DevArray<float> d_xstage;     // 32-RHS gather staging, nc x 32
     int storage = HTG_STORAGE_FP64;
     bool cplx   = false; // complex<double> entries, x and y
+    // Layout generation: changes whenever device buffers a captured product
+    // reads are replaced (layout rebuilds, storage changes); a graph keeps the
+    // value it was captured at, and the token is set to ~0 when the H-matrix
+    // is destroyed.
+    std::shared_ptr<std::atomic<uint64_t>> generation = std::make_shared<std::atomic<uint64_t>>(0);
+    ~htg_hmatrix() { generation->store(~uint64_t(0)); }
     int esz() const { return cplx ? 2 : 1; }
 };
 
@@ -424,6 +432,14 @@ void upload_layout(htg_hmatrix &h) {
     h.d_z.alloc((size_t)(h.tpad + h.t_size) * 64);
     h.d_z.zero();
     h.d_part.alloc((size_t)std::max(h.part_size, 1) * 64);
+    // multi-RHS gathers: the inverse column permutation and an nc x 32 FP32
+    // staging buffer, allocated here so that products allocate nothing
+    std::vector<int> ip(h.nc);
+    for (int j = 0; j < h.nc; j++)
+        ip[h.perm_s[j]] = j;
+    h.d_iperm_s.upload(ip);
+    h.d_xstage.alloc((size_t)h.nc * 32);
+    h.generation->fetch_add(1); // graphs captured before read freed buffers
 }
 
 // FP64 master arenas of both classes, zero filled (row padding stays zero).
@@ -473,15 +489,7 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
         const int MU = pick_mu_chunk(mu - done, 32);
         if (ev)
             HTG_HIP(hipEventRecord(ev[0], st));
-        if (MU == 32 && !h.d_iperm_s.p) {
-            std::vector<int> ip(h.nc);
-            for (int j = 0; j < h.nc; j++)
-                ip[h.perm_s[j]] = j;
-            h.d_iperm_s.upload(ip);
-        }
         if (std::is_same<TC, float>::value && MU == 32) {
-            if (!h.d_xstage.p)
-                h.d_xstage.alloc((size_t)h.nc * 32);
             launch_gather_x32(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, reinterpret_cast<float *>(z), st);
         } else if (std::is_same<TC, double>::value && MU == 32) {
             launch_gather_x64(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, reinterpret_cast<double *>(z), st);
@@ -524,17 +532,9 @@ void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream
         const int MU = pick_mu_chunk(mu - done, 16);
         if (ev)
             HTG_HIP(hipEventRecord(ev[0], st));
-        if (MU == 16 && !h.d_iperm_s.p) {
-            std::vector<int> ip(h.nc);
-            for (int j = 0; j < h.nc; j++)
-                ip[h.perm_s[j]] = j;
-            h.d_iperm_s.upload(ip);
-        }
         if (std::is_same<R, double>::value && MU == 16) {
             launch_gather_xc64(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, reinterpret_cast<double *>(z), st);
         } else if (std::is_same<R, float>::value && MU == 16) {
-            if (!h.d_xstage.p)
-                h.d_xstage.alloc((size_t)h.nc * 32);
             launch_gather_xc32(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, reinterpret_cast<float *>(z), st);
         } else {
             launch_gather_xc<R>(d_x, h.d_perm_s.p, h.nc, h.tpad, done, MU, z, st);
@@ -613,7 +613,10 @@ namespace {
 // into the FP64 master arenas.
 void fill_from_assembly(htg_hmatrix &h, const Part &part) {
     const Assembly &A = *part.A;
-    const DevPoints rowsP = part.transposed ? A.dS() : A.dT(), colsP = part.transposed ? A.dT() : A.dS();
+    // the generator always sees (target point, source point) of the assembly:
+    // tiles of a transposed part evaluate G(x_t[col], x_s[row]) (FillTile::tr),
+    // which needs no symmetry of the kernel function
+    const DevPoints rowsP = A.dT(), colsP = A.dS();
     std::vector<FillTile> fills;
     std::vector<LrPack> packs[2];
     std::vector<UTileDst> udst[2];
@@ -624,7 +627,7 @@ void fill_from_assembly(htg_hmatrix &h, const Part &part) {
             for (const auto &rt : h.leaf_tiles[l]) {
                 const int rb = std::get<0>(rt);
                 fills.push_back(FillTile{std::get<1>(rt), h.rb_row0[rb] - part.shift, row_block_rows(h, rb),
-                                         L.oj - part.shift, L.n, row_stride(h, 0, rb)});
+                                         L.oj - part.shift, L.n, row_stride(h, 0, rb), part.transposed ? 1 : 0});
             }
             continue;
         }
@@ -644,6 +647,7 @@ void fill_from_assembly(htg_hmatrix &h, const Part &part) {
             pk.dst_ldv     = v_stride(L, c);
             pk.piece_first = (int)udst[c].size();
             pk.npieces     = (int)h.leaf_tiles[l].size();
+            pk.conj        = L.mirror && h.symmetry == 'H' && h.cplx; // K^H of the kept leaf K
             for (const auto &rt : h.leaf_tiles[l]) {
                 const int rb = std::get<0>(rt);
                 udst[c].push_back(UTileDst{c == 0 ? std::get<1>(rt) : std::get<2>(rt), h.rb_row0[rb] - L.oi,
@@ -692,6 +696,15 @@ void ensure_copy(DevArray<TS> &dst, const DevArray<double> &src, uint64_t n) {
     HTG_HIP(hipGetLastError());
 }
 
+// Converts an existing storage copy again from the FP64 master, in place.
+template <typename TS>
+void reconvert(DevArray<TS> &dst, const DevArray<double> &src) {
+    if (!dst.p)
+        return;
+    launch_convert<TS>(src.p, dst.p, std::min<uint64_t>(dst.n, src.n), nullptr);
+    HTG_HIP(hipGetLastError());
+}
+
 // Adds the storage variant's copies of both classes (converted from FP64).
 void apply_storage(htg_hmatrix &h, int storage) {
     for (int c = 0; c < 2; c++) {
@@ -713,6 +726,8 @@ void apply_storage(htg_hmatrix &h, int storage) {
         }
     }
     HTG_HIP(hipDeviceSynchronize());
+    if (h.storage != storage)
+        h.generation->fetch_add(1);
     h.storage = storage;
 }
 
@@ -750,8 +765,8 @@ void write_user_dense(htg_hmatrix &h) {
                 for (int j = 0; j < L.n; j++) {
                     // kept leaf: column-major m x n; mirror: the kept n x m leaf's (j, i)
                     const size_t s = L.mirror ? (size_t)j + (size_t)L.n * (i0 + i) : (size_t)(i0 + i) + (size_t)L.m * j;
-                    for (int u = 0; u < e; u++)
-                        dst[e * (i * ld + j) + u] = d[e * s + u];
+                    for (int u = 0; u < e; u++) // Hermitian storage: the mirror is the conjugate transpose
+                        dst[e * (i * ld + j) + u] = (u == 1 && L.mirror && h.symmetry == 'H') ? -d[e * s + u] : d[e * s + u];
                 }
         }
     }
@@ -835,6 +850,7 @@ void take_aca_stats(htg_hmatrix &h, Assembly &A) {
     h.aca_cap.insert(h.aca_cap.end(), A.aca_cap.begin(), A.aca_cap.end());
     h.aca_retry += A.aca_retry;
     h.aca_wide += A.aca_wide;
+    h.aca_int_wrap += A.aca_int_wrap;
     h.aca_ws_entries += A.aca_ws_entries;
     h.aca_need_entries += A.aca_need_entries;
     h.aca_seconds += A.aca_seconds;
@@ -848,14 +864,21 @@ void take_aca_stats(htg_hmatrix &h, Assembly &A) {
 
 namespace htg {
 namespace {
+// Hermitian storage applies the conjugate transpose of each kept block: the
+// kernel family must be Hermitian (every real family is symmetric, so 'H' is
+// 'S' there, as the reference's real HMatrix with 'H' is).
+void check_hermitian(char symmetry, const htg_generator &g) {
+    if (symmetry == 'H' && !kernel_is_hermitian(g.kernel))
+        throw Error(HTG_ERR_UNSUPPORTED, "Hermitian storage needs a Hermitian kernel family (HTG_KERNEL_HERMITIAN_TEST "
+                                         "or a real family)");
+}
+
 void validate_params(const htg_params &prm) {
     // symmetry and storage triangle as HMatrix's constructor checks them (hmatrix.hpp:116-121)
     const int sy = prm.symmetry == 0 ? 'N' : prm.symmetry, up = prm.uplo == 0 ? 'N' : prm.uplo;
     if (!((sy == 'N' || sy == 'S' || sy == 'H') && (up == 'N' || up == 'L' || up == 'U') &&
           ((sy == 'N' && up == 'N') || (sy != 'N' && up != 'N'))))
         throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments to create HMatrix");
-    if (sy == 'H')
-        throw Error(HTG_ERR_UNSUPPORTED, "Hermitian storage: none of the kernel families is Hermitian");
     if (prm.minclustersize <= 0)
         throw Error(HTG_ERR_INVALID, "[Htool error] MinClusterSize parameter cannot be zero");
     if (prm.maxblocksize < 1)
@@ -871,6 +894,8 @@ void validate_params(const htg_params &prm) {
 struct htg_graph {
     hipGraph_t graph    = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::shared_ptr<std::atomic<uint64_t>> generation; // the H-matrix's token
+    uint64_t captured_at = 0;
     ~htg_graph() {
         if (exec)
             (void)hipGraphExecDestroy(exec);
@@ -878,6 +903,167 @@ struct htg_graph {
             (void)hipGraphDestroy(graph);
     }
 };
+
+namespace htg {
+namespace {
+
+// Arguments of htg_aca_batch (host arrays; the
+// meaning of each is in include/htool_amd.h).
+struct AcaBatchArgs {
+    const htg_generator *gen;
+    int compressor;
+    double epsilon;
+    int reqrank, nt;
+    const double *xt;
+    int ns;
+    const double *xs;
+    int nblocks;
+    const int *M, *N, *rows, *row_off, *cols, *col_off;
+    const double *ctr_t, *ctr_s;
+    const int *off_t, *off_s;
+    int cap;
+    int *ranks;
+    double *U;
+    const int64_t *u_off;
+    double *V;
+    const int64_t *v_off;
+};
+
+// Host side of one batched compression: each block's points gathered into
+// contiguous stretches (the kernel's cluster order), the block records, and
+// where each device array sits in one byte buffer (256-byte aligned parts).
+struct AcaBatchPlan {
+    std::vector<int> rp, cp;
+    std::vector<AcaBlock> blocks;
+    std::vector<uint64_t> vis_off;
+    uint64_t ws = 0, vis = 0;
+    size_t off_pt[4] = {}, off_ps[4] = {}, off_blocks = 0, off_visoff = 0, off_ranks = 0, off_ws = 0, off_vis = 0;
+    size_t bytes = 0;
+};
+
+AcaBatchPlan plan_aca_batch(const AcaBatchArgs &a) {
+    if (!a.gen || !a.xt || a.nblocks < 0 || !a.M || !a.N || !a.rows || !a.row_off || !a.cols || !a.col_off ||
+        !a.ctr_t || !a.ctr_s || !a.off_t || !a.off_s || !a.ranks || !a.U || !a.u_off || !a.V || !a.v_off || a.cap <= 0)
+        throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments to htg_aca_batch");
+    if (a.cap > CAP_MAX)
+        throw Error(HTG_ERR_INVALID, "cap above " + std::to_string(CAP_MAX));
+    if (kernel_is_complex(a.gen->kernel))
+        throw Error(HTG_ERR_UNSUPPORTED, "htg_aca_batch compresses real kernel families only");
+    const bool same = a.gen->same_points || a.xs == nullptr;
+    AcaBatchPlan P;
+    P.blocks.resize(a.nblocks);
+    P.vis_off.resize(a.nblocks);
+    for (int b = 0; b < a.nblocks; b++) {
+        if (a.M[b] <= 0 || a.N[b] <= 0)
+            throw Error(HTG_ERR_INVALID, "empty block");
+        AcaBlock &B = P.blocks[b];
+        B.off_t     = (int)P.rp.size();
+        B.M         = a.M[b];
+        B.off_s     = (int)P.cp.size();
+        B.N         = a.N[b];
+        for (int i = 0; i < a.M[b]; i++) {
+            const int r = a.rows[a.row_off[b] + i];
+            if (r < 0 || r >= a.nt)
+                throw Error(HTG_ERR_INVALID, "row index out of range");
+            P.rp.push_back(r);
+        }
+        for (int j = 0; j < a.N[b]; j++) {
+            const int c = a.cols[a.col_off[b] + j];
+            if (c < 0 || c >= (same ? a.nt : a.ns))
+                throw Error(HTG_ERR_INVALID, "column index out of range");
+            P.cp.push_back(c);
+        }
+        B.swap = (a.compressor == HTG_SYM_PARTIAL_ACA && a.off_t[b] < a.off_s[b]) ? 1 : 0;
+        B.cap  = a.cap;
+        B.mn   = aca_size_bound(B.M, B.N, false);
+        const int n1 = B.swap ? B.N : B.M;
+        B.ws1        = P.ws;
+        B.ws2        = P.ws + (uint64_t)n1 * a.cap;
+        P.ws += (uint64_t)(B.M + B.N) * a.cap;
+        const double *c = B.swap ? a.ctr_s + 3 * b : a.ctr_t + 3 * b;
+        B.ctr[0] = c[0], B.ctr[1] = c[1], B.ctr[2] = c[2];
+        P.vis_off[b] = P.vis;
+        P.vis += (uint64_t)(B.M + B.N);
+    }
+    size_t o = 0;
+    auto take = [&](size_t n) {
+        const size_t at = o;
+        o               = (o + std::max<size_t>(n, 1) + 255) / 256 * 256;
+        return at;
+    };
+    for (int d = 0; d < 3; d++)
+        P.off_pt[d] = take(P.rp.size() * sizeof(double));
+    P.off_pt[3] = take(P.rp.size() * sizeof(int));
+    for (int d = 0; d < 3; d++)
+        P.off_ps[d] = take(P.cp.size() * sizeof(double));
+    P.off_ps[3]  = take(P.cp.size() * sizeof(int));
+    P.off_blocks = take(P.blocks.size() * sizeof(AcaBlock));
+    P.off_visoff = take(P.vis_off.size() * sizeof(uint64_t));
+    P.off_ranks  = take((size_t)a.nblocks * sizeof(int));
+    P.off_ws     = take(P.ws * sizeof(double));
+    P.off_vis    = take(P.vis);
+    P.bytes      = o;
+    return P;
+}
+
+// Uploads the plan into `dev` (P.bytes of device memory), runs the batched ACA
+// and copies ranks and factors out. Synchronous.
+void run_aca_batch(const AcaBatchArgs &a, const AcaBatchPlan &P, unsigned char *dev) {
+    require_device();
+    const bool same    = a.gen->same_points || a.xs == nullptr;
+    const double *xsrc = same ? a.xt : a.xs;
+    auto put_points = [&](const double *pts, const std::vector<int> &idx, const size_t *off) {
+        std::vector<double> c(idx.size());
+        for (int d = 0; d < 3; d++) {
+            for (size_t i = 0; i < idx.size(); i++)
+                c[i] = pts[3 * (size_t)idx[i] + d];
+            if (!c.empty())
+                HTG_HIP(hipMemcpy(dev + off[d], c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        if (!idx.empty())
+            HTG_HIP(hipMemcpy(dev + off[3], idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+        return DevPoints{reinterpret_cast<const double *>(dev + off[0]), reinterpret_cast<const double *>(dev + off[1]),
+                         reinterpret_cast<const double *>(dev + off[2]), reinterpret_cast<const int *>(dev + off[3])};
+    };
+    const DevPoints pT = put_points(a.xt, P.rp, P.off_pt), pS = put_points(xsrc, P.cp, P.off_ps);
+    if (a.nblocks > 0) {
+        HTG_HIP(hipMemcpy(dev + P.off_blocks, P.blocks.data(), P.blocks.size() * sizeof(AcaBlock),
+                          hipMemcpyHostToDevice));
+        HTG_HIP(hipMemcpy(dev + P.off_visoff, P.vis_off.data(), P.vis_off.size() * sizeof(uint64_t),
+                          hipMemcpyHostToDevice));
+    }
+    const KernelDesc kd = kernel_desc(*a.gen, same);
+    double *ws          = reinterpret_cast<double *>(dev + P.off_ws);
+    int *d_ranks        = reinterpret_cast<int *>(dev + P.off_ranks);
+    // the kernel clears each block's visited flags itself; the workspace is
+    // written before it is read
+    launch_aca(kd, pT, pS, reinterpret_cast<const AcaBlock *>(dev + P.off_blocks), a.nblocks, a.epsilon, a.reqrank,
+               ws, dev + P.off_vis, reinterpret_cast<const uint64_t *>(dev + P.off_visoff), d_ranks, nullptr);
+    HTG_HIP(hipGetLastError());
+    if (a.nblocks == 0)
+        return;
+    HTG_HIP(hipMemcpy(a.ranks, d_ranks, a.nblocks * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<double> hws(P.ws);
+    HTG_HIP(hipMemcpy(hws.data(), ws, P.ws * sizeof(double), hipMemcpyDeviceToHost));
+    for (int b = 0; b < a.nblocks; b++) {
+        const AcaBlock &B = P.blocks[b];
+        if (a.ranks[b] == -3)
+            throw Error(HTG_ERR_RANK_CAP, "block " + std::to_string(b) + " needs more than cap terms");
+        const int k = a.ranks[b];
+        if (k <= 0)
+            continue;
+        const double *su = hws.data() + (B.swap ? B.ws2 : B.ws1);
+        const double *sv = hws.data() + (B.swap ? B.ws1 : B.ws2);
+        std::copy(su, su + (size_t)B.M * k, a.U + a.u_off[b]);
+        double *vo = a.V + a.v_off[b];
+        for (int r = 0; r < k; r++)
+            for (int j = 0; j < B.N; j++)
+                vo[r + (size_t)k * j] = sv[(size_t)r * B.N + j];
+    }
+}
+
+} // namespace
+} // namespace htg
 
 extern "C" {
 
@@ -901,6 +1087,7 @@ int htg_hmatrix_build(const htg_params *prm, const htg_generator *gen, int nt, c
         if (!prm || !gen || !xt || !out || nt <= 0)
             throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments to build HMatrix");
         validate_params(*prm);
+        check_hermitian(sym_kind(*prm), *gen);
         const bool same = gen->same_points || xs == nullptr;
         if (sym_uplo(*prm) != 'N' && !same)
             throw Error(HTG_ERR_INVALID, "[Htool error] symmetric storage needs a single point set");
@@ -922,7 +1109,7 @@ int htg_hmatrix_build(const htg_params *prm, const htg_generator *gen, int nt, c
         h->perm_s = A.perm_s;
         h->leaves = A.leaves;
         h->uplo     = sym_uplo(*prm);
-        h->symmetry = h->uplo == 'N' ? 'N' : 'S';
+        h->symmetry = sym_kind(*prm);
         if (h->uplo != 'N')
             add_mirrors(h->leaves, 0);
         take_aca_stats(*h, A);
@@ -1033,6 +1220,8 @@ int htg_hmatrix_build_batch(const htg_params *prm, int count, const htg_generato
         if (total > (1LL << 30))
             throw Error(HTG_ERR_INVALID, "batch too large");
         validate_params(*prm);
+        for (int i = 0; i < count; i++)
+            check_hermitian(sym_kind(*prm), gens[i]);
         require_device();
         const auto t0 = std::chrono::steady_clock::now();
         std::unique_ptr<htg_hmatrix> h(new htg_hmatrix);
@@ -1068,7 +1257,7 @@ int htg_hmatrix_build_batch(const htg_params *prm, int count, const htg_generato
         }
         h->nr = h->nc = shift;
         h->uplo     = sym_uplo(*prm);
-        h->symmetry = h->uplo == 'N' ? 'N' : 'S';
+        h->symmetry = sym_kind(*prm);
         sort_leaves(h->leaves);
         for (int i = 0; i < count; i++)
             locate_part(*h, h->parts[i], n[i]);
@@ -1098,6 +1287,7 @@ int htg_hmatrix_transpose(const htg_hmatrix *h, htg_hmatrix **out) {
         t->perm_t         = h->perm_s;
         t->perm_s         = h->perm_t;
         t->false_positive = h->false_positive;
+        t->aca_int_wrap   = h->aca_int_wrap;
         t->symmetry       = h->symmetry;
         t->uplo           = h->uplo == 'L' ? 'U' : (h->uplo == 'U' ? 'L' : 'N');
         t->leaves.reserve(h->leaves.size());
@@ -1116,11 +1306,39 @@ int htg_hmatrix_transpose(const htg_hmatrix *h, htg_hmatrix **out) {
             locate_part(*t, q, q.transposed ? p.A->nc : p.A->nr);
             t->parts.push_back(std::move(q));
         }
+        // Dense blocks the caller supplied to h (set_dense_blocks) become t's
+        // supplied blocks: each kept dense leaf of t is the transpose of a kept
+        // dense leaf of h (no conjugation), in t's get_leaves order.
+        if (!h->user_dense.empty()) {
+            const int e = h->esz();
+            std::map<std::pair<int, int>, size_t> where; // (off_i, off_j) of h's kept dense leaf -> data offset
+            size_t pos = 0;
+            for (const Leaf &L : h->leaves) {
+                if (L.mirror || L.k >= 0)
+                    continue;
+                where[{L.oi, L.oj}] = pos;
+                pos += (size_t)L.m * L.n * e;
+            }
+            t->user_dense.reserve(pos);
+            for (const Leaf &L : t->leaves) {
+                if (L.mirror || L.k >= 0)
+                    continue;
+                const auto it = where.find({L.oj, L.oi});
+                if (it == where.end())
+                    throw Error(HTG_ERR_INVALID, "transposed dense leaf without supplied data");
+                const double *d = h->user_dense.data() + it->second; // h's n x m leaf, column-major
+                for (int j = 0; j < L.n; j++)
+                    for (int i = 0; i < L.m; i++)
+                        for (int u = 0; u < e; u++)
+                            t->user_dense.push_back(d[e * ((size_t)j + (size_t)L.n * i) + u]);
+            }
+        }
         build_layout(*t);
         upload_layout(*t);
         alloc_master(*t);
         for (const Part &part : t->parts)
             fill_from_assembly(*t, part);
+        write_user_dense(*t);
         *out = t.release();
     });
 }
@@ -1230,7 +1448,9 @@ int htg_graph_capture(htg_hmatrix *h, const double *d_x, double *d_y, int mu, vo
         }
         HTG_HIP(hipStreamEndCapture(st, &g->graph));
         HTG_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
-        *out = g.release();
+        g->generation  = h->generation;
+        g->captured_at = h->generation->load();
+        *out           = g.release();
     });
 }
 
@@ -1238,6 +1458,9 @@ int htg_graph_launch(htg_graph *g, void *stream) {
     return guard([&] {
         if (!g || !g->exec)
             throw Error(HTG_ERR_INVALID, "null graph");
+        if (!g->generation || g->generation->load() != g->captured_at)
+            throw Error(HTG_ERR_INVALID, "stale graph: the H-matrix was destroyed, or its layout, storage or "
+                                         "buffers changed after the capture (capture again)");
         HTG_HIP(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
     });
 }
@@ -1293,6 +1516,7 @@ int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info) {
         r.n_aca_wide        = h->aca_wide;
         r.symmetry          = h->symmetry;
         r.uplo              = h->uplo;
+        r.n_aca_int_wrap    = h->aca_int_wrap;
         *info               = r;
     });
 }
@@ -1409,9 +1633,14 @@ int htg_hmatrix_set_dense_blocks(htg_hmatrix *h, const double *data) {
                 n += (size_t)L.m * L.n * h->esz();
         h->user_dense.assign(data, data + n);
         write_user_dense(*h);
-        for (ClassLayout &C : h->cls) // storage copies are converted again from the new FP64 entries
-            C.a32.reset(), C.a16.reset(), C.ab16.reset();
-        apply_storage(*h, h->storage);
+        // storage copies are converted again from the new FP64 entries, in
+        // place: captured graphs keep valid pointers and see the new blocks
+        for (ClassLayout &C : h->cls) {
+            reconvert(C.a32, C.a64);
+            reconvert(C.a16, C.a64);
+            reconvert(C.ab16, C.a64);
+        }
+        HTG_HIP(hipDeviceSynchronize());
     });
 }
 
@@ -1509,85 +1738,14 @@ int htg_aca_batch(const htg_generator *gen, int compressor, double epsilon, int 
                   const int *off_t, const int *off_s, int cap, int *ranks, double *U, const int64_t *u_off, double *V,
                   const int64_t *v_off) {
     return guard([&] {
-        if (!gen || !xt || nblocks < 0 || !M || !N || !rows || !row_off || !cols || !col_off || !ctr_t || !ctr_s ||
-            !off_t || !off_s || !ranks || !U || !u_off || !V || !v_off || cap <= 0)
-            throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments to htg_aca_batch");
-        if (cap > CAP_MAX)
-            throw Error(HTG_ERR_INVALID, "cap above " + std::to_string(CAP_MAX));
-        require_device();
-        const bool same = gen->same_points || xs == nullptr;
-        const double *xsrc = same ? xt : xs;
-        // Gather each block's points into contiguous stretches (the kernel's cluster order).
-        std::vector<int> rp, cp;
-        std::vector<AcaBlock> blocks(nblocks);
-        std::vector<uint64_t> vis_off(nblocks);
-        uint64_t ws = 0, vis = 0;
-        for (int b = 0; b < nblocks; b++) {
-            if (M[b] <= 0 || N[b] <= 0)
-                throw Error(HTG_ERR_INVALID, "empty block");
-            AcaBlock &B = blocks[b];
-            B.off_t     = (int)rp.size();
-            B.M         = M[b];
-            B.off_s     = (int)cp.size();
-            B.N         = N[b];
-            for (int i = 0; i < M[b]; i++) {
-                const int r = rows[row_off[b] + i];
-                if (r < 0 || r >= nt)
-                    throw Error(HTG_ERR_INVALID, "row index out of range");
-                rp.push_back(r);
-            }
-            for (int j = 0; j < N[b]; j++) {
-                const int c = cols[col_off[b] + j];
-                if (c < 0 || c >= (same ? nt : ns))
-                    throw Error(HTG_ERR_INVALID, "column index out of range");
-                cp.push_back(c);
-            }
-            B.swap = (compressor == HTG_SYM_PARTIAL_ACA && off_t[b] < off_s[b]) ? 1 : 0;
-            B.cap  = cap;
-            const int n1 = B.swap ? B.N : B.M;
-            B.ws1        = ws;
-            B.ws2        = ws + (uint64_t)n1 * cap;
-            ws += (uint64_t)(B.M + B.N) * cap;
-            const double *c = B.swap ? ctr_s + 3 * b : ctr_t + 3 * b;
-            B.ctr[0] = c[0], B.ctr[1] = c[1], B.ctr[2] = c[2];
-            vis_off[b] = vis;
-            vis += (uint64_t)(B.M + B.N);
-        }
-        PointsOnDevice pT, pS;
-        pT.upload(xt, rp);
-        pS.upload(xsrc, cp);
-        const KernelDesc kd = kernel_desc(*gen, same);
-        DevArray<AcaBlock> d_blocks;
-        DevArray<uint64_t> d_vis_off;
-        DevArray<double> d_ws;
-        DevArray<unsigned char> d_vis;
-        DevArray<int> d_ranks;
-        d_blocks.upload(blocks);
-        d_vis_off.upload(vis_off);
-        d_ws.alloc(ws);
-        d_vis.alloc(vis);
-        d_ranks.alloc(nblocks);
-        launch_aca(kd, pT.view(), pS.view(), d_blocks.p, nblocks, epsilon, reqrank, d_ws.p, d_vis.p, d_vis_off.p,
-                   d_ranks.p, nullptr);
-        HTG_HIP(hipGetLastError());
-        HTG_HIP(hipMemcpy(ranks, d_ranks.p, nblocks * sizeof(int), hipMemcpyDeviceToHost));
-        std::vector<double> hws(ws);
-        HTG_HIP(hipMemcpy(hws.data(), d_ws.p, ws * sizeof(double), hipMemcpyDeviceToHost));
-        for (int b = 0; b < nblocks; b++) {
-            const AcaBlock &B = blocks[b];
-            if (ranks[b] == -3)
-                throw Error(HTG_ERR_RANK_CAP, "block " + std::to_string(b) + " needs more than cap terms");
-            const int k = ranks[b];
-            if (k <= 0)
-                continue;
-            const double *su = hws.data() + (B.swap ? B.ws2 : B.ws1);
-            const double *sv = hws.data() + (B.swap ? B.ws1 : B.ws2);
-            std::copy(su, su + (size_t)B.M * k, U + u_off[b]);
-            double *vo = V + v_off[b];
-            for (int r = 0; r < k; r++)
-                for (int j = 0; j < B.N; j++)
-                    vo[r + (size_t)k * j] = sv[(size_t)r * B.N + j];
-        }
+        AcaBatchArgs a{gen,    compressor, epsilon, reqrank, nt,    xt,    ns,    xs,    nblocks, M,     N,
+                       rows,   row_off,    cols,    col_off, ctr_t, ctr_s, off_t, off_s, cap,     ranks, U,
+                       u_off,  V,          v_off};
+        AcaBatchPlan plan = plan_aca_batch(a);
+        // device buffers of this call, allocated here and freed on return
+        DevArray<unsigned char> buf;
+        buf.alloc(plan.bytes);
+        run_aca_batch(a, plan, buf.p);
     });
 }
 

@@ -39,6 +39,8 @@ __device__ __forceinline__ double kernel_entry(const KernelDesc &k, const DevPoi
         return 1. / (k.delta + sqrt(r2));
     case 2:
         return 1. / (k.delta + 4 * M_PI * sqrt(r2));
+    case HTG_KERNEL_ZERO:
+        return 0.0;
     default:
         return 1. / (4 * M_PI * sqrt(k.delta + r2));
     }
@@ -66,6 +68,14 @@ __device__ __forceinline__ cdbl kernel_entry_c(const KernelDesc &k, const DevPoi
         sincos(k.wavenumber * r, &sn, &cs);
         const double d = 4 * M_PI * r;
         return cdbl{cs / d, sn / d};
+    }
+    if (k.kind == HTG_KERNEL_HERMITIAN_TEST) {
+        // (1 + sign(x_i - x_j) i) / (1e-5 + 4 pi r): the complex division by a
+        // real denominator divides both parts
+        const double dx = T.x[i] - S.x[j];
+        const double sg = dx > 0 ? 1.0 : (dx < 0 ? -1.0 : 0.0);
+        const double d  = 1e-5 + 4 * M_PI * r;
+        return cdbl{1. / d, sg / d};
     }
     const double d = 4 * M_PI * r;
     return cdbl{1. / d, 1. / d};
@@ -197,7 +207,8 @@ __global__ void __launch_bounds__(WG) k_dense_fill(KernelDesc k, DevPoints T, De
     const int total   = ft.R * ft.ncols;
     for (int e = threadIdx.x; e < total; e += WG) {
         const int i = e / ft.ncols, j = e % ft.ncols;
-        arena[ft.off + (size_t)i * ft.ld + j] = kernel_entry(k, T, ft.row0 + i, S, ft.col0 + j);
+        arena[ft.off + (size_t)i * ft.ld + j] =
+            ft.tr ? kernel_entry(k, T, ft.col0 + j, S, ft.row0 + i) : kernel_entry(k, T, ft.row0 + i, S, ft.col0 + j);
     }
 }
 
@@ -207,7 +218,8 @@ __global__ void __launch_bounds__(WG) k_dense_fill_c(KernelDesc k, DevPoints T, 
     const int total   = ft.R * ft.ncols;
     for (int e = threadIdx.x; e < total; e += WG) {
         const int i = e / ft.ncols, j = e % ft.ncols;
-        arena[ft.off + (size_t)i * ft.ld + j] = kernel_entry_c(k, T, ft.row0 + i, S, ft.col0 + j);
+        arena[ft.off + (size_t)i * ft.ld + j] = ft.tr ? kernel_entry_c(k, T, ft.col0 + j, S, ft.row0 + i)
+                                                      : kernel_entry_c(k, T, ft.row0 + i, S, ft.col0 + j);
     }
 }
 
@@ -317,7 +329,7 @@ __global__ void __launch_bounds__(WG) k_aca(KernelDesc kd, DevPoints T, DevPoint
     int I2 = 0, q = 0;
     double frob = 0, aux = 0;
     bool overflow = false;
-    const long long MN = (long long)M * N;
+    const long long MN = B.mn;
     while (((reqrank > 0) && (q < min(reqrank, min(M, N)))) || ((reqrank < 0) && (q == 0 || sqrt(aux / frob) > eps))) {
         q += 1;
         if ((long long)q * (M + N) > MN) {
@@ -432,7 +444,7 @@ __device__ void acaw_enter(AcaWide &st, const AcaBlock &B, int reqrank, double e
     st.active = 0;
     if (!go) {
         st.rank = p;
-    } else if ((long long)(p + 1) * (B.M + B.N) > (long long)B.M * B.N) {
+    } else if ((long long)(p + 1) * (B.M + B.N) > B.mn) {
         st.rank = -1;
     } else if (p + 1 > B.cap || p + 1 > CAPMAX) {
         st.rank = -3;
@@ -709,6 +721,9 @@ __global__ void __launch_bounds__(WG) k_acaw_fin(const AcaBlock *blocks, AcaWide
 // ---------------------------------------------------------------------------
 // Copies compressed leaves (from the ACA workspace or the compressed store)
 // into another layout: V row by row, U into one tile per destination.
+__device__ __forceinline__ double conj_if(double v, int) { return v; }
+__device__ __forceinline__ cdbl conj_if(cdbl v, int c) { return c ? cdbl{v.re, -v.im} : v; }
+
 template <typename E>
 __global__ void __launch_bounds__(WG) k_lr_pack(const LrPack *packs, const UTileDst *utiles, const E *ws, E *arena,
                                                 E *varena) {
@@ -716,14 +731,15 @@ __global__ void __launch_bounds__(WG) k_lr_pack(const LrPack *packs, const UTile
     const size_t nv = (size_t)p.k * p.N;
     for (size_t e = threadIdx.x; e < nv; e += WG) {
         const size_t r = e / p.N, j = e % p.N;
-        varena[p.dst_v + r * p.dst_ldv + j] = ws[p.src_v + r * p.src_ldv + j];
+        varena[p.dst_v + r * p.dst_ldv + j] = conj_if(ws[p.src_v + r * p.src_ldv + j], p.conj);
     }
     for (int t = 0; t < p.npieces; t++) {
         const UTileDst d = utiles[p.piece_first + t];
         const int tot    = d.R * p.k;
         for (int e = threadIdx.x; e < tot; e += WG) {
             const int i = e / p.k, r = e % p.k;
-            arena[d.off + (size_t)i * d.si + (size_t)r * d.sr] = ws[p.src_u + (size_t)r * p.M + d.row0 + i];
+            arena[d.off + (size_t)i * d.si + (size_t)r * d.sr] =
+                conj_if(ws[p.src_u + (size_t)r * p.M + d.row0 + i], p.conj);
         }
     }
 }

@@ -22,7 +22,12 @@ struct KernelDesc {
     double wavenumber; // Helmholtz
 };
 
-inline bool kernel_is_complex(int kind) { return kind == HTG_KERNEL_HELMHOLTZ || kind == HTG_KERNEL_COMPLEX_TEST; }
+inline bool kernel_is_complex(int kind) {
+    return kind == HTG_KERNEL_HELMHOLTZ || kind == HTG_KERNEL_COMPLEX_TEST || kind == HTG_KERNEL_HERMITIAN_TEST;
+}
+// Kernel families whose matrix is Hermitian (A(j, i) = conj A(i, j), bitwise
+// as evaluated) on one point set; every real family is symmetric.
+inline bool kernel_is_hermitian(int kind) { return !kernel_is_complex(kind) || kind == HTG_KERNEL_HERMITIAN_TEST; }
 
 // complex<double> in device memory and registers: re then im.
 struct cdbl {
@@ -40,6 +45,7 @@ struct DevPoints {
 struct FillTile {
     uint64_t off;
     int row0, R, col0, ncols, ld;
+    int tr; // 1: a tile of the transpose, entry (i, j) = G(target col0 + j, source row0 + i)
 };
 
 // One block handed to the batched ACA kernel.
@@ -49,7 +55,25 @@ struct AcaBlock {
     int cap;                // room for this many cross terms in the workspace
     uint64_t ws1, ws2;      // workspace offsets: side-1 vectors (n1*cap), side-2 vectors (n2*cap)
     double ctr[3];          // centre of the cluster whose points choose the first pivot
+    int64_t mn;             // right side of the q (M + N) > MN exit: M*N, or the reference's 32-bit
+                            // wrap of it (aca_size_bound)
 };
+
+// Right side of partialACA's "next rank not advantageous" test q (M + N) > M N
+// (partialACA.hpp:74, sympartialACA.hpp:98). The reference evaluates it in
+// int, so for M N > 2^31 - 1 its product wraps modulo 2^32 (to a negative
+// value below 2^32: the test then fires at q = 1 and the block is split);
+// int32_wrap reproduces that, otherwise the exact 64-bit product is used.
+inline int64_t aca_size_bound(int M, int N, bool int32_wrap) {
+    const int64_t mn = (int64_t)M * N;
+    return int32_wrap ? (int64_t)(int32_t)(uint32_t)(uint64_t)mn : mn;
+}
+// Whether the reference's int test and the 64-bit one part ways for a block
+// whose ACA ran k >= 1 cross terms: the 64-bit test let every q <= k pass,
+// the 32-bit one fails at the latest at q = k.
+inline bool aca_int32_wrap_differs(int M, int N, int k) {
+    return k >= 1 && (int64_t)k * (M + N) > aca_size_bound(M, N, true) && (int64_t)k * (M + N) <= (int64_t)M * N;
+}
 
 // Column chunk of a low-rank leaf's V (k x n, row-major, row stride ld). Each
 // stored row covers the global columns [oj & ~(a-1), round_a(oj + n)), zero
@@ -83,6 +107,7 @@ struct LrPack {
     int M, N, k, src_ldv, dst_ldv;
     int piece_first; // first U tile in the U-tile list
     int npieces;
+    int conj;        // complex entries: store the conjugates (mirrored copies of Hermitian storage)
 };
 
 // U rows [row0, row0+R) of a leaf: entry (i, r) goes to off + i*si + r*sr.

@@ -19,6 +19,9 @@ KERNEL_INV_DELTA_4PIR = 2   # 1/(delta + 4 pi r)   (GeneratorTestDoubleSymmetric
 KERNEL_SMOOTH = 3           # 1/(4 pi sqrt(delta + r^2))   (examples/smallest_example.cpp)
 KERNEL_HELMHOLTZ = 4        # e^{ikr}/(4 pi r), complex; `diag` on the diagonal of a square same-point matrix
 KERNEL_COMPLEX_TEST = 5     # (1 + i)/(4 pi r), complex   (GeneratorTestComplex)
+KERNEL_ZERO = 6             # every entry 0   (GeneratorTestZero, test_hmat_zero.hpp)
+KERNEL_HERMITIAN_TEST = 7   # (1 + sign(x_i - x_j) i)/(1e-5 + 4 pi r), complex   (GeneratorTestComplexHermitian)
+COMPLEX_KERNELS = (KERNEL_HELMHOLTZ, KERNEL_COMPLEX_TEST, KERNEL_HERMITIAN_TEST)
 
 
 def cylinder(n: int, seed: int = 0, radius: float = 1.0, height: float = 1.0) -> np.ndarray:

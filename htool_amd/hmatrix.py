@@ -65,6 +65,7 @@ class HMatrix:
         self.predictor_margin = 0.25
         self.aca_wide_min = 0
         self.delay_dense = False
+        self.ref_int_overflow = False
         self._h = None
         self._storage = "fp64"
 
@@ -124,11 +125,20 @@ class HMatrix:
         data = np.concatenate([np.asarray(p, dtype=dt).ravel(order="F") for p in ptr]) if ptr else np.zeros(0, dt)
         data = np.ascontiguousarray(data)
         capi.check(capi.lib().htg_hmatrix_set_dense_blocks(self._handle(), capi.ptr(data.view(np.float64))))
+        self._transposed = None  # made from the previous dense blocks
 
     def save_plot(self, outputname: str) -> str:
         """HMatrix::save_plot (hmatrix.hpp:1695-1712): the leaf list CSV `<outputname>_0.csv`."""
         from .io import save_plot
         return save_plot(self, outputname)
+
+    def set_ref_int_overflow(self, on: bool = True):
+        """Evaluate the ACA's q (M + N) > M N exit in 32-bit int, as the reference
+        does (partialACA.hpp:74, sympartialACA.hpp:98): blocks with M N > 2^31 - 1
+        then return rank -1 and are split, giving the reference's leaf list.
+        Default off: the exact 64-bit test (get_info().n_aca_int_wrap counts
+        the blocks where the two differ)."""
+        self.ref_int_overflow = bool(on)
 
     def set_aca_wide_min(self, m: int):
         """Blocks with M + N >= m run the multi-workgroup ACA (0: the library
@@ -159,7 +169,7 @@ class HMatrix:
                                self.reqrank, int(self.lowprec_all), self.recompress_tol, self.lowprec_tol,
                                self.predictor.handle if self.predictor is not None else None,
                                self.predictor_margin, self.aca_wide_min, ord(self.symmetry), ord(self.UPLO),
-                               int(self.delay_dense))
+                               int(self.delay_dense), int(self.ref_int_overflow))
 
     def build_batch(self, gens, point_sets) -> "HMatrix":
         """Many independent square H-matrices as one block-diagonal H-matrix

@@ -48,8 +48,12 @@ typedef enum htg_kernel {
     /* Complex families: the H-matrix, x and y are complex<double> (interleaved re, im). */
     HTG_KERNEL_HELMHOLTZ      = 4, /* e^{i k r}/(4 pi r), k = `wavenumber`; `diag` on the diagonal when the
                                       point sets coincide */
-    HTG_KERNEL_COMPLEX_TEST   = 5  /* (1 + i)/(4 pi r): GeneratorTestComplex,
+    HTG_KERNEL_COMPLEX_TEST   = 5, /* (1 + i)/(4 pi r): GeneratorTestComplex,
                                       include/htool/testing/generator_test.hpp:110-120 */
+    HTG_KERNEL_ZERO           = 6, /* real, every entry 0: GeneratorTestZero,
+                                      tests/functional_tests/types/virtual_hmatrix/hmatrix/test_hmat_zero.hpp:18-65 */
+    HTG_KERNEL_HERMITIAN_TEST = 7  /* complex, (1 + sign(x_i - x_j) i)/(1e-5 + 4 pi r) (x_i the first coordinate):
+                                      GeneratorTestComplexHermitian, include/htool/testing/generator_test.hpp:165-180 */
 } htg_kernel;
 
 typedef enum htg_compressor {
@@ -107,11 +111,20 @@ typedef struct htg_params {
                                        factors and ranks); 0: 16384 (real entries) or 4096
                                        (complex entries), < 0: never */
     int symmetry;                   /* 'N' (or 0): general; 'S': symmetric storage, one triangle of
-                                       blocks kept (HMatrix(..., 'S', UPLO), hmatrix.hpp:115-122);
-                                       'H' is rejected (no Hermitian kernel family) */
-    int uplo;                       /* 'L' or 'U' with 'S' (the kept triangle); 'N' (or 0) otherwise */
+                                       blocks kept (HMatrix(..., 'S', UPLO), hmatrix.hpp:115-122); 'H':
+                                       Hermitian storage, the same triangle, the product applying the
+                                       conjugate transpose of each kept off-diagonal block
+                                       (hmatrix.hpp:864-904); needs HTG_KERNEL_HERMITIAN_TEST or a real
+                                       family (HTG_ERR_UNSUPPORTED otherwise) */
+    int uplo;                       /* 'L' or 'U' with 'S' / 'H' (the kept triangle); 'N' (or 0) otherwise */
     int delay_dense;                /* 1: dense leaves left zero at build, filled later by
                                        htg_hmatrix_set_dense_blocks (set_delay_dense_computation) */
+    int ref_int_overflow;           /* 0: the ACA's "next rank not advantageous" exit q (M + N) > M N
+                                       (partialACA.hpp:74, sympartialACA.hpp:98) in exact 64-bit
+                                       arithmetic; 1: in 32-bit int as the reference evaluates it,
+                                       where M N > 2^31 - 1 wraps and the block returns rank -1 (and
+                                       is split) at once. htg_info.n_aca_int_wrap counts the blocks
+                                       on which the two differ */
 } htg_params;
 
 typedef struct htg_info {
@@ -139,6 +152,10 @@ typedef struct htg_info {
                                  off-diagonal leaves that the product also streams (the counts
                                  above are of the kept leaves, as the reference's) */
     int64_t mirrored_lowprec_entries; /* the part of mirrored_entries in class 1 */
+    int n_aca_int_wrap;       /* ACA runs of blocks with M N > 2^31 - 1 on which the reference's 32-bit
+                                 q (M + N) > M N test and the 64-bit one differ: with ref_int_overflow 0
+                                 these blocks are compressed (the reference splits them), with 1 they
+                                 are split as in the reference */
 } htg_info;
 
 /* Work decomposition of the product (no counterpart in the reference): rows
@@ -212,8 +229,11 @@ int htg_hmatrix_autotune(htg_hmatrix *h, int mu, int reps, htg_layout *best, dou
 /* Captures one product of device buffers d_x -> d_y (mu right-hand sides, the
  * current storage) into a HIP graph on `stream` (a hipStream_t, not NULL);
  * htg_graph_launch replays it on a stream. The graph reads the H-matrix's
- * device data: destroy it before the H-matrix, and capture again after
- * htg_hmatrix_set_storage / _set_layout. */
+ * device data: capture again after htg_hmatrix_set_storage (to another mode),
+ * _set_layout or _autotune (both rebuild the layout). htg_graph_launch refuses
+ * (HTG_ERR_INVALID) a graph captured before one of these calls, or whose
+ * H-matrix was destroyed. htg_hmatrix_set_dense_blocks writes in place: a graph
+ * stays valid and sees the new blocks. */
 int htg_graph_capture(htg_hmatrix *h, const double *d_x, double *d_y, int mu, void *stream, htg_graph **out);
 int htg_graph_launch(htg_graph *g, void *stream);
 int htg_graph_destroy(htg_graph *g);
@@ -260,6 +280,12 @@ int htg_hmatrix_mvprod_device(htg_hmatrix *h, const double *d_x, double *d_y, in
 /* Times each product kernel with HIP events over `reps` products of device
  * buffers on `stream`; ms[0..3] = average ms of (gather, V x, row apply, whole product). */
 int htg_hmatrix_profile(htg_hmatrix *h, const double *d_x, double *d_y, int mu, int reps, void *stream, double *ms);
+
+/* Frees the current device's ACA workspace and visited flags, which builds
+ * keep allocated between builds (they only grow); the next build allocates
+ * them again. Waits for a build running on the device to finish. No
+ * counterpart in the reference (which allocates per block). */
+int htg_release_workspace(void);
 
 /* Batched stand-alone leaf compression, replaces calling
  * VirtualLowRankGenerator::copy_low_rank_approximation once per block

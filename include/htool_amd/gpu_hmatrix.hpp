@@ -58,6 +58,7 @@ class GpuHMatrix {
         prm_.symmetry         = symmetry;
         prm_.uplo             = UPLO;
         prm_.delay_dense      = 0;
+        prm_.ref_int_overflow = 0;
     }
     GpuHMatrix(const GpuHMatrix &) = delete;
     GpuHMatrix &operator=(const GpuHMatrix &) = delete;
@@ -81,9 +82,17 @@ class GpuHMatrix {
     }
     void set_compression(htg_compressor c) { prm_.compressor = c; }
     void set_delay_dense_computation(bool d) { prm_.delay_dense = d ? 1 : 0; }
+    // The ACA's q (M + N) > M N exit in 32-bit int as the reference computes it
+    // (partialACA.hpp:74): blocks with M N > 2^31 - 1 are then split as there.
+    void set_ref_int_overflow(bool on) { prm_.ref_int_overflow = on ? 1 : 0; }
     // build_dense_blocks (hmatrix.hpp:404-418): every kept dense leaf, in
     // get_leaves order, m x n column-major in cluster ordering
-    void build_dense_blocks(const double *data) { check(htg_hmatrix_set_dense_blocks(handle(), data)); }
+    // (the transpose, built from the blocks present when it was made, is dropped)
+    void build_dense_blocks(const double *data) {
+        check(htg_hmatrix_set_dense_blocks(handle(), data));
+        htg_hmatrix_destroy(ht_);
+        ht_ = nullptr;
+    }
     // Build-time options of the mixed-precision product (no reference counterpart).
     void set_recompression(double tol) { prm_.recompress_tol = tol; }
     void set_lowprec(double tol, bool all_terms = false) {

@@ -41,6 +41,9 @@ def lib():
             "orc_leaf_mvprod_sym": (None, [_I, _VP, _I, _VP, _VP, _VP, _VP, _I, _I, _I]),
             "orc_dense_rows_c": (None, [_I, _D, _D, _I, _VP, _VP, _I, _I, _VP, _VP, _I, _VP]),
             "orc_mlp_predict": (None, [_VP, _VP, _VP, _VP, _I, _VP]),
+            "orc_set_ref_int_overflow": (None, [_I]),
+            "orc_leaf_mvprod_herm": (None, [_I, _VP, _I, _VP, _VP, _VP, _VP, _I, _I]),
+            "orc_aca_size_bound": (C.c_longlong, [_I, _I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -52,6 +55,28 @@ def lib():
 
 def _p(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class ref_int_overflow:
+    """Context manager: inside it, the oracle's ACA evaluates q (M + N) > M N in
+    the reference's 32-bit int arithmetic (partialACA.hpp:74), outside in exact
+    64-bit arithmetic."""
+
+    def __init__(self, on=True):
+        self.on = bool(on)
+
+    def __enter__(self):
+        lib().orc_set_ref_int_overflow(int(self.on))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_ref_int_overflow(0)
+        return False
+
+
+def aca_size_bound(M, N):
+    """Right side of the q (M + N) > M N test under the current arithmetic."""
+    return int(lib().orc_aca_size_bound(int(M), int(N)))
 
 
 def kernel_block(kernel, delta, diag, xt, xs, rows, cols, same=False):
@@ -111,19 +136,23 @@ class OracleHMatrix:
     data: np.ndarray
     false_positive: int
     uplo: str = "N"
+    symmetry: str = "S"  # with uplo 'L' / 'U': 'S' symmetric or 'H' Hermitian storage
 
     def mvprod(self, x):
+        if self.uplo != "N" and self.symmetry == "H" and np.iscomplexobj(self.data):
+            return leaf_mvprod_herm(self.nr, self.perm_t, self.leaves, self.data, x, self.uplo)
         if self.uplo != "N":
             return leaf_mvprod_sym(self.nr, self.perm_t, self.leaves, self.data, x, self.uplo)
         return leaf_mvprod(self.nr, self.nc, self.perm_t, self.perm_s, self.leaves, self.data, x)
 
 
 def hmat_build(xt, xs=None, *, kernel=0, delta=0.0, diag=0.0, minclustersize=10, eta=10.0, eps=1e-6, sym=False,
-               reqrank=-1, maxblocksize=1e6, wavenumber=0.0, uplo="N") -> OracleHMatrix:
-    """Complex kernel families (kernel 4, 5): data is complex128 and mvprod takes
+               reqrank=-1, maxblocksize=1e6, wavenumber=0.0, uplo="N", symmetry="S") -> OracleHMatrix:
+    """Complex kernel families (kernel 4, 5, 7): data is complex128 and mvprod takes
     complex vectors; the Helmholtz wavenumber travels in `delta`. uplo 'L' / 'U':
-    symmetric storage (HMatrix(..., 'S', uplo), one point set): the leaves of one
-    triangle, and a product that also applies their transposes."""
+    symmetric storage (HMatrix(..., symmetry, uplo), one point set): the leaves of
+    one triangle (the same for 'S' and 'H'), and a product that also applies
+    their transposes ('S') or conjugate transposes ('H')."""
     if uplo != "N" and xs is not None:
         raise ValueError("symmetric storage needs one point set")
     if kernel == 4 and wavenumber:
@@ -150,7 +179,7 @@ def hmat_build(xt, xs=None, *, kernel=0, delta=0.0, diag=0.0, minclustersize=10,
         fp = L.orc_hmat_false_positives(h)
     finally:
         L.orc_hmat_free(h)
-    return OracleHMatrix(nt, ns, pt, ps, leaves, data, fp, uplo)
+    return OracleHMatrix(nt, ns, pt, ps, leaves, data, fp, uplo, symmetry)
 
 
 def leaf_mvprod(nr, nc, perm_t, perm_s, leaves, data, x):
@@ -185,6 +214,22 @@ def leaf_mvprod_sym(n, perm, leaves, data, x, uplo):
     perm = np.ascontiguousarray(perm, np.int32)
     lib().orc_leaf_mvprod_sym(n, _p(perm), leaves.shape[0], _p(leaves), _p(data), _p(xm), _p(y), mu, ord(uplo),
                               int(cplx))
+    return y[:, 0].copy() if vec else np.ascontiguousarray(y)
+
+
+def leaf_mvprod_herm(n, perm, leaves, data, x, uplo):
+    """Product of a Hermitian-storage leaf list (one triangle, complex): kept
+    leaves, conjugate transposes of the off-diagonal ones, hemv on the
+    diagonal leaves."""
+    x = np.asarray(x, np.complex128)
+    vec = x.ndim == 1
+    xm = np.asfortranarray(x.reshape(n, -1))
+    mu = xm.shape[1]
+    y = np.zeros((n, mu), dtype=np.complex128, order="F")
+    leaves = np.ascontiguousarray(leaves, np.int32)
+    data = np.ascontiguousarray(data, np.complex128)
+    perm = np.ascontiguousarray(perm, np.int32)
+    lib().orc_leaf_mvprod_herm(n, _p(perm), leaves.shape[0], _p(leaves), _p(data), _p(xm), _p(y), mu, ord(uplo))
     return y[:, 0].copy() if vec else np.ascontiguousarray(y)
 
 

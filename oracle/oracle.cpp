@@ -25,6 +25,20 @@
 
 namespace {
 
+// Arithmetic of partialACA's "next rank not advantageous" test q (M + N) > M N
+// (partialACA.hpp:74, sympartialACA.hpp:98). Both sides are int there, so for
+// M N > 2^31 - 1 the right side wraps modulo 2^32; g_int_wrap = 1 evaluates it
+// that way (two's complement, the reference's behaviour on x86-64), 0 in
+// exact 64-bit arithmetic (the library's default).
+int g_int_wrap = 0;
+long long aca_mn(int M, int N) {
+    const long long mn = (long long)M * N;
+    if (!g_int_wrap)
+        return mn;
+    const unsigned long long low = (unsigned long long)mn & 0xffffffffull;
+    return low >= 0x80000000ull ? (long long)low - 0x100000000ll : (long long)low;
+}
+
 // ----------------------------------------------------------------------------
 // Kernel entries (H/testing/generator_test.hpp:93-103, 135-145;
 // examples/smallest_example.cpp:16-27 for the smooth kernel).
@@ -49,13 +63,17 @@ struct Kernel {
             return 1. / (delta + std::sqrt(r2));
         case 2:
             return 1. / (delta + 4 * M_PI * std::sqrt(r2));
+        case 6: // GeneratorTestZero (tests/functional_tests/types/virtual_hmatrix/hmatrix/test_hmat_zero.hpp:34)
+            return 0.0;
         default:
             return 1. / (4 * M_PI * std::sqrt(delta + r2));
         }
     }
     // Complex families: 4 = e^{ikr}/(4 pi r) (k = `delta`; `diag` on the
     // diagonal of a square matrix), 5 = (1 + i)/(4 pi r)
-    // (GeneratorTestComplex, H/testing/generator_test.hpp:110-120).
+    // (GeneratorTestComplex, H/testing/generator_test.hpp:110-120),
+    // 7 = (1 + sign(x_i - x_j) i)/(1e-5 + 4 pi r)
+    // (GeneratorTestComplexHermitian, H/testing/generator_test.hpp:165-180).
     std::complex<double> c(int i, int j) const {
         const double *a = xt + 3 * i, *b = xs + 3 * j;
         double r2 = 0;
@@ -68,6 +86,11 @@ struct Kernel {
             if (same && i == j)
                 return {diag, 0.0};
             return {std::cos(delta * r) / den, std::sin(delta * r) / den};
+        }
+        if (kind == 7) {
+            const double dx = a[0] - b[0], sg = dx > 0 ? 1.0 : (dx < 0 ? -1.0 : 0.0);
+            const double dh = 1e-5 + 4 * M_PI * r;
+            return {1. / dh, sg / dh};
         }
         return {1. / den, 1. / den};
     }
@@ -344,7 +367,7 @@ int aca(bool swap, double epsilon, int M, int N, const int *rows, const int *col
     std::vector<T> u1(n2), u2(n1);
     while (((reqrank > 0) && (q < std::min(reqrank, std::min(M, N)))) || ((reqrank < 0) && (q == 0 || std::sqrt(aux / frob) > epsilon))) {
         q += 1;
-        if ((long long)q * (M + N) > (long long)M * N) { // 64-bit: the int product overflows past 46341^2
+        if ((long long)q * (M + N) > aca_mn(M, N)) { // exact, or the int wrap past 46341^2 (g_int_wrap)
             q = -1;
             break;
         }
@@ -705,11 +728,18 @@ cd load<cd>(const double *p, size_t i) {
 // LowRankMatrix::add_mvprod_row_major (lrmat.hpp:89-100), one RHS column at a
 // time. T = cd: data, x and y hold (re, im) pairs. uplo 'L' / 'U' (symmetric
 // storage, hmatrix.hpp:864-904): every off-diagonal leaf also adds its
-// transpose (no conjugation, 'S'), a diagonal dense leaf is applied as symv
-// on its `uplo` triangle (matrix.hpp:386-447).
+// transpose (no conjugation, 'S'; conjugated, 'H'), a diagonal dense leaf is
+// applied as symv / hemv on its `uplo` triangle (matrix.hpp:386-447; hemv
+// reads only the real part of the diagonal entries).
+inline double conj_t(double v) { return v; }
+inline cd conj_t(cd v) { return std::conj(v); }
+inline double real_t(double v) { return v; }
+inline cd real_t(cd v) { return cd(v.real(), 0.0); }
+
 template <typename T>
 void leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int nleaves, const int *leaves5,
-                 const double *data, const double *x, double *y, int mu, char uplo = 'N') {
+                 const double *data, const double *x, double *y, int mu, char uplo = 'N', char symmetry = 'S') {
+    const bool herm = symmetry == 'H';
     std::vector<size_t> off(nleaves);
     size_t p = 0;
     for (int l = 0; l < nleaves; l++) {
@@ -730,25 +760,28 @@ void leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int nleav
                 for (int i = 0; i < m; i++)
                     for (int j = 0; j < m; j++) {
                         const bool keep = uplo == 'L' ? i >= j : i <= j; // entry (i, j) of the kept triangle
-                        const T a = keep ? load<T>(data, d + i + size_t(m) * j) : load<T>(data, d + j + size_t(m) * i);
+                        T a = keep ? load<T>(data, d + i + size_t(m) * j) : load<T>(data, d + j + size_t(m) * i);
+                        if (herm)
+                            a = i == j ? real_t(a) : (keep ? a : conj_t(a));
                         yout[oi + i] += a * xin[oj + j];
                     }
                 continue;
             }
-            if (uplo != 'N') { // the transpose of an off-diagonal leaf: y_J += A^T x_I
+            if (uplo != 'N') { // the transpose of an off-diagonal leaf: y_J += A^T x_I ('H': A^H x_I)
+                auto op = [&](T a) { return herm ? conj_t(a) : a; };
                 if (k < 0) {
                     for (int j = 0; j < n; j++)
                         for (int i = 0; i < m; i++)
-                            yout[oj + j] += load<T>(data, d + i + size_t(m) * j) * xin[oi + i];
+                            yout[oj + j] += op(load<T>(data, d + i + size_t(m) * j)) * xin[oi + i];
                 } else {
                     const size_t U = d, V = d + size_t(m) * k;
                     t.assign(k, T(0));
                     for (int r = 0; r < k; r++)
                         for (int i = 0; i < m; i++)
-                            t[r] += load<T>(data, U + i + size_t(m) * r) * xin[oi + i];
+                            t[r] += op(load<T>(data, U + i + size_t(m) * r)) * xin[oi + i];
                     for (int j = 0; j < n; j++)
                         for (int r = 0; r < k; r++)
-                            yout[oj + j] += load<T>(data, V + r + size_t(k) * j) * t[r];
+                            yout[oj + j] += op(load<T>(data, V + r + size_t(k) * j)) * t[r];
                 }
             }
             if (k < 0) {
@@ -779,6 +812,9 @@ void leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int nleav
 } // namespace
 
 extern "C" {
+
+void orc_set_ref_int_overflow(int on) { g_int_wrap = on != 0; }
+long long orc_aca_size_bound(int M, int N) { return aca_mn(M, N); }
 
 void orc_kernel_block(int kernel, double delta, double diag, int same_points, const double *xt, const double *xs,
                       int M, int N, const int *rows, const int *cols, double *out) {
@@ -858,7 +894,7 @@ orc_hmat *orc_hmat_build(int nt, const double *xt, int ns, const double *xs, int
             return S[a.second].offset < S[b.second].offset;
         return T[a.first].offset < T[b.first].offset;
     });
-    h->cplx = kernel >= 4;
+    h->cplx = kernel == 4 || kernel == 5 || kernel == 7;
     if (h->cplx) {
         Builder<cd> BC{T, S, h->perm_t, h->perm_s, A, eta, eps, (size_t)maxblocksize, sym_aca, reqrank, h->uplo};
         compute_blocks(BC, local, h->cleaves, h->false_positive);
@@ -931,6 +967,11 @@ void orc_leaf_mvprod_sym(int n, const int *perm, int nleaves, const int *leaves5
         leaf_mvprod<cd>(n, n, perm, perm, nleaves, leaves5, data, x, y, mu, (char)uplo);
     else
         leaf_mvprod<double>(n, n, perm, perm, nleaves, leaves5, data, x, y, mu, (char)uplo);
+}
+
+void orc_leaf_mvprod_herm(int n, const int *perm, int nleaves, const int *leaves5, const double *data,
+                          const double *x, double *y, int mu, int uplo) {
+    leaf_mvprod<cd>(n, n, perm, perm, nleaves, leaves5, data, x, y, mu, (char)uplo, 'H');
 }
 
 void orc_hmat_mvprod(const orc_hmat *h, const double *x, double *y, int mu) {

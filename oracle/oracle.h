@@ -12,6 +12,8 @@
  *   orc_kernel_block       closed forms; bitwise vs the reference driver's dense leaves
  *   orc_cluster_build      permutation bitwise vs the reference driver; invariants
  *                          (contiguous sons, every point once)
+ *   orc_aca_size_bound     the reference driver's rank -1 / compressed outcome on blocks
+ *                          either side of M N = 2^31 (46340^2, 46341^2)
  *   orc_aca                ranks/U/V vs the reference driver; cross interpolation
  *                          property; space-saving bands of the reference's lrmat tests;
  *                          Eckart-Young floor vs numpy SVD
@@ -29,8 +31,15 @@ extern "C" {
 
 typedef struct orc_hmat orc_hmat;
 
+/* Arithmetic of partialACA's q (M + N) > M N exit (partialACA.hpp:74,
+ * sympartialACA.hpp:98) for every later orc_aca / orc_hmat_build: 0 (default)
+ * exact 64-bit; 1 the reference's int arithmetic, where M N > 2^31 - 1 wraps
+ * modulo 2^32. orc_aca_size_bound returns the right side in use. */
+void orc_set_ref_int_overflow(int on);
+long long orc_aca_size_bound(int M, int N);
+
 /* One kernel entry family: 0 = 1/(4 pi r) (diag on i==j when same points),
- * 1 = 1/(delta+r), 2 = 1/(delta+4 pi r), 3 = 1/(4 pi sqrt(delta+r^2)). */
+ * 1 = 1/(delta+r), 2 = 1/(delta+4 pi r), 3 = 1/(4 pi sqrt(delta+r^2)), 6 = 0. */
 void orc_kernel_block(int kernel, double delta, double diag, int same_points, const double *xt,
                       const double *xs, int M, int N, const int *rows, const int *cols, double *out);
 
@@ -77,6 +86,11 @@ void orc_leaf_mvprod_c(int nr, int nc, const int *perm_t, const int *perm_s, int
  * triangle of the diagonal leaves; cplx: complex data and vectors. */
 void orc_leaf_mvprod_sym(int n, const int *perm, int nleaves, const int *leaves5, const double *data,
                          const double *x, double *y, int mu, int uplo, int cplx);
+/* Hermitian storage ('H', complex): the kept leaves, the conjugate transposes
+ * of the off-diagonal ones, hemv on the kept triangle of the diagonal leaves
+ * (the real part of their diagonal entries), hmatrix.hpp:864-904. */
+void orc_leaf_mvprod_herm(int n, const int *perm, int nleaves, const int *leaves5, const double *data,
+                          const double *x, double *y, int mu, int uplo);
 
 /* Brute-force dense rows, complex kernels, (re, im) pairs. */
 void orc_dense_rows_c(int kernel, double wavenumber, double diag, int same_points, const double *xt, const double *xs,

@@ -18,15 +18,19 @@
 //   int32  mode (0 = H-matrix build + product, 1 = one leaf compression of the whole t x s block)
 //   int32  nt, ns, same_points, kernel, minclustersize, compressor (0 partialACA, 1 sympartialACA)
 //   int32  mu, nreps, dump_data, reqrank, uplo (0: general; 'L' / 'U': HMatrix(..., 'S', uplo)
-//          built with the single-point-set build(A, xt), hmatrix.hpp:343-400)
+//          built with the single-point-set build(A, xt), hmatrix.hpp:343-400; 'H' << 8 added:
+//          HMatrix(..., 'H', uplo))
 //   double eta, eps, delta, diag, maxblocksize
 //   double xt[3 nt]; double xs[3 ns] (absent when same_points); double x[ns mu] (column-major)
 // Complex kernel families (kernel 4: e^{ikr}/(4 pi r) with k = `delta`, `diag`
-// on the diagonal; kernel 5: (1 + i)/(4 pi r)) run HMatrix<std::complex<double>>:
-// x, y and the leaf data are then complex<double> (re, im interleaved), mode 0 only.
+// on the diagonal; kernel 5: (1 + i)/(4 pi r); kernel 7: the reference's own
+// GeneratorTestComplexHermitian) run HMatrix<std::complex<double>>: x, y and the
+// leaf data are then complex<double> (re, im interleaved), mode 0 only. Kernel 6
+// is the reference's GeneratorTestZero (every entry 0).
 #include <htool/clustering/pca.hpp>
 #include <htool/lrmat/partialACA.hpp>
 #include <htool/lrmat/sympartialACA.hpp>
+#include <htool/testing/generator_test.hpp>
 #include <htool/types/hmatrix.hpp>
 
 #include <chrono>
@@ -69,6 +73,8 @@ struct KernelGen final : public VirtualGenerator<double> {
             return 1. / (delta + std::sqrt(r2));
         case 2:
             return 1. / (delta + 4 * M_PI * std::sqrt(r2));
+        case 6:
+            return 0.0;
         default:
             return 1. / (4 * M_PI * std::sqrt(delta + r2));
         }
@@ -142,8 +148,8 @@ template <typename T>
 void run_hmatrix(std::shared_ptr<Cluster<PCAGeometricClustering>> t, std::shared_ptr<Cluster<PCAGeometricClustering>> s,
                  VirtualGenerator<T> &A, std::shared_ptr<VirtualLowRankGenerator<T>> comp, const double *xt,
                  const double *pxs, std::vector<T> &x, int nt, int mu, int nreps, int dump, int reqrank, double eps,
-                 double eta, double maxblocksize, const std::string &out, std::FILE *info, char uplo) {
-    HMatrix<T> H(t, s, eps, eta, uplo == 'N' ? 'N' : 'S', uplo, reqrank);
+                 double eta, double maxblocksize, const std::string &out, std::FILE *info, char uplo, char symmetry) {
+    HMatrix<T> H(t, s, eps, eta, uplo == 'N' ? 'N' : symmetry, uplo, reqrank);
     H.set_compression(comp);
     H.set_maxblocksize(static_cast<unsigned int>(maxblocksize));
     double t0 = now();
@@ -218,9 +224,12 @@ int main(int argc, char **argv) {
     }
     const int mode = hdr[2], nt = hdr[3], ns = hdr[4], same = hdr[5], kernel = hdr[6], mcs = hdr[7];
     const int compressor = hdr[8], mu = hdr[9], nreps = hdr[10], dump = hdr[11], reqrank = hdr[12];
-    const char uplo = (hdr[13] == 'L' || hdr[13] == 'U') ? (char)hdr[13] : 'N';
+    const int up0 = hdr[13] & 0xff;
+    const char uplo = (up0 == 'L' || up0 == 'U') ? (char)up0 : 'N';
+    const char symmetry = (hdr[13] >> 8) == 'H' ? 'H' : 'S';
+    const bool cplx = kernel == 4 || kernel == 5 || kernel == 7;
     const double eta = prm[0], eps = prm[1], delta = prm[2], diag = prm[3], maxblocksize = prm[4];
-    std::vector<double> xt(3 * size_t(nt)), xs, x(size_t(ns) * mu * (kernel >= 4 ? 2 : 1));
+    std::vector<double> xt(3 * size_t(nt)), xs, x(size_t(ns) * mu * (cplx ? 2 : 1));
     in.read(reinterpret_cast<char *>(xt.data()), xt.size() * sizeof(double));
     if (!same) {
         xs.resize(3 * size_t(ns));
@@ -280,8 +289,12 @@ int main(int argc, char **argv) {
         return 0;
     }
 
-    if (kernel >= 4) {
+    if (cplx) {
         KernelGenC Ac(nt, ns, xt.data(), pxs, kernel, same != 0, delta, diag);
+        // kernel 7 is the reference's own generator on the same points (3-d)
+        std::vector<double> p1(xt), p2(same ? xt : xs);
+        GeneratorTestComplexHermitian Ah(3, nt, ns, p1, p2);
+        VirtualGenerator<std::complex<double>> &Ause = kernel == 7 ? static_cast<VirtualGenerator<std::complex<double>> &>(Ah) : Ac;
         std::vector<std::complex<double>> xc(size_t(ns) * mu);
         for (size_t e = 0; e < xc.size(); e++)
             xc[e] = {x[2 * e], x[2 * e + 1]};
@@ -289,11 +302,11 @@ int main(int argc, char **argv) {
                          ? std::shared_ptr<VirtualLowRankGenerator<std::complex<double>>>(
                                std::make_shared<partialACA<std::complex<double>>>())
                          : std::make_shared<sympartialACA<std::complex<double>>>();
-        run_hmatrix<std::complex<double>>(t, s, Ac, compc, xt.data(), pxs, xc, nt, mu, nreps, dump, reqrank, eps, eta,
-                                          maxblocksize, out, info, uplo);
+        run_hmatrix<std::complex<double>>(t, s, Ause, compc, xt.data(), pxs, xc, nt, mu, nreps, dump, reqrank, eps, eta,
+                                          maxblocksize, out, info, uplo, symmetry);
     } else {
         run_hmatrix<double>(t, s, A, comp, xt.data(), pxs, x, nt, mu, nreps, dump, reqrank, eps, eta, maxblocksize,
-                            out, info, uplo);
+                            out, info, uplo, symmetry);
     }
     MPI_Finalize();
     return 0;

@@ -50,7 +50,7 @@ class RefResult:
 
 def run(xt, xs=None, x=None, *, kernel=0, eta=10.0, eps=1e-6, minclustersize=10, compressor=0,
         delta=0.0, diag=0.0, maxblocksize=1e6, mu=1, nreps=0, dump_data=True, reqrank=-1, mode=0,
-        threads=None, timeout=600, wavenumber=0.0, uplo="N") -> RefResult:
+        threads=None, timeout=600, wavenumber=0.0, uplo="N", symmetry="S") -> RefResult:
     """kernel 4 (Helmholtz, wavenumber travels in the case file's delta) and 5
     run the reference's HMatrix<std::complex<double>>. uplo 'L' / 'U': the
     reference's symmetric storage HMatrix(..., 'S', uplo) (one point set)."""
@@ -62,14 +62,15 @@ def run(xt, xs=None, x=None, *, kernel=0, eta=10.0, eps=1e-6, minclustersize=10,
     same = xs is None
     nt = xt.shape[0]
     ns = nt if same else xs.shape[0]
-    cplx = kernel >= 4  # complex families: x, y and the leaf data are complex128
+    cplx = kernel in (4, 5, 7)  # complex families: x, y and the leaf data are complex128
     if x is None:
         x = np.zeros((ns, mu))
     x = np.asarray(x, dtype=np.complex128 if cplx else np.float64).reshape(ns, mu)
     if uplo != "N" and not same:
         raise ValueError("symmetric storage needs one point set")
     hdr = struct.pack("<14i", 0x48545246, 1, mode, nt, ns, int(same), kernel, minclustersize, compressor,
-                      mu, nreps, int(dump_data), reqrank, 0 if uplo == "N" else ord(uplo))
+                      mu, nreps, int(dump_data), reqrank,
+                      0 if uplo == "N" else ord(uplo) | (ord("H") << 8 if symmetry == "H" else 0))
     prm = struct.pack("<5d", eta, eps, delta, diag, maxblocksize)
     with tempfile.TemporaryDirectory() as td:
         case = os.path.join(td, "case.bin")

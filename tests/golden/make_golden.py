@@ -38,11 +38,11 @@ def leaf_slices(leaves):
 ONLY = set(sys.argv[1:])
 
 
-def save(name, *, xt, xs, kernel, delta, diag, mcs, eta, eps, sym, x, sample=None, uplo="N", **kw):
+def save(name, *, xt, xs, kernel, delta, diag, mcs, eta, eps, sym, x, sample=None, uplo="N", symmetry="S", **kw):
     if ONLY and name not in ONLY:
         return
     r = refcase.run(xt, xs, x=x, kernel=kernel, eta=eta, eps=eps, minclustersize=mcs, compressor=int(sym),
-                    delta=delta, diag=diag, mu=x.shape[1], nreps=0, dump_data=True, uplo=uplo)
+                    delta=delta, diag=diag, mu=x.shape[1], nreps=0, dump_data=True, uplo=uplo, symmetry=symmetry)
     order = np.lexsort((r.leaves[:, 2], r.leaves[:, 0]))
     sl = leaf_slices(r.leaves)
     leaves = r.leaves[order]
@@ -61,7 +61,7 @@ def save(name, *, xt, xs, kernel, delta, diag, mcs, eta, eps, sym, x, sample=Non
                         mcs=np.int32(mcs), eta=eta, eps=eps, sym=np.int32(sym), perm_t=r.perm_t, perm_s=r.perm_s,
                         leaves=leaves, data_leaves=keep.astype(np.int32), data=data, x=x, y=r.y,
                         false_positive=np.int32(r.info.get("false_positive", 0)),
-                        uplo=np.int32(0 if uplo == "N" else ord(uplo)), **kw)
+                        uplo=np.int32(0 if uplo == "N" else ord(uplo)), symmetry=np.int32(ord(symmetry)), **kw)
     print(name, "leaves", len(leaves), "stored leaf data", len(keep), "values", data.size)
 
 
@@ -80,6 +80,31 @@ def save_lrmat(name, *, dist, seed, reqrank, sym):
     np.savez_compressed(os.path.join(HERE, name + ".npz"), xt=xt, xs=xs, perm_t=r.perm_t, perm_s=r.perm_s,
                         rank=np.int32(k), U=u, V=v, reqrank=np.int32(reqrank), sym=np.int32(sym), eps=1e-4)
     print(name, "rank", k)
+
+
+def save_int_wrap(name, *, n, seed, z):
+    """Two disks of n points, z apart: the block tree's admissible blocks are
+    n x ~n/2 with M N > 2^31 - 1, where the reference's int test q (M + N) > M N
+    (partialACA.hpp:74) wraps, returns rank -1 and splits them. Stores the
+    reference's leaf list, false positives and a 500-row sample of y = H x
+    (x = standard normal from default_rng(seed + 2)); the points come from
+    geometry.disk(n, 0, seed) and disk(n, z, seed + 1)."""
+    if ONLY and name not in ONLY:
+        return
+    t = geo.disk(n, 0.0, seed=seed)
+    s = geo.disk(n, float(z), seed=seed + 1)
+    x = np.random.default_rng(seed + 2).standard_normal((n, 1))
+    r = refcase.run(t, s, x=x, kernel=geo.KERNEL_LAPLACE, eta=10.0, eps=1e-6, minclustersize=10, compressor=0,
+                    nreps=0, dump_data=False)
+    order = np.lexsort((r.leaves[:, 2], r.leaves[:, 0]))
+    rows = np.sort(np.random.default_rng(seed + 3).choice(n, 500, replace=False)).astype(np.int32)
+    # the whole n x n block through one leaf compression (mode 1): rank -1 as well
+    r1 = refcase.run(t, s, kernel=geo.KERNEL_LAPLACE, eps=1e-6, minclustersize=10, compressor=0, mode=1,
+                     dump_data=False)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), n=np.int32(n), seed=np.int32(seed), z=float(z),
+                        leaves=r.leaves[order], false_positive=np.int32(r.info.get("false_positive", 0)),
+                        rows=rows, y_rows=r.y[rows, 0], whole_block_rank=np.int32(int(r1.info["rank"])))
+    print(name, "leaves", r.leaves[order].tolist(), "whole-block rank", r1.info["rank"])
 
 
 def main():
@@ -127,6 +152,21 @@ def main():
     save("sphere800_helmholtz_symL", xt=s, xs=None, kernel=4, delta=5.0, diag=geo.laplace_diag(geo.sphere_spacing(n)),
          mcs=10, eta=10.0, eps=1e-6, sym=True, x=rng.standard_normal((n, 1)) + 1j * rng.standard_normal((n, 1)),
          sample=20, uplo="L")
+    # Hermitian storage HMatrix(..., 'H', UPLO) with the reference's GeneratorTestComplexHermitian
+    n = 900
+    s = geo.sphere(n, seed=34)
+    save("sphere900_hermitian_hermL", xt=s, xs=None, kernel=7, delta=0.0, diag=0.0, mcs=10, eta=10.0, eps=1e-6,
+         sym=True, x=rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2)), sample=20, uplo="L",
+         symmetry="H")
+    c = geo.cylinder(700, seed=35)
+    save("cyl700_hermitian_partial_hermU", xt=c, xs=None, kernel=7, delta=0.0, diag=0.0, mcs=10, eta=10.0,
+         eps=1e-6, sym=False, x=rng.standard_normal((700, 1)) + 1j * rng.standard_normal((700, 1)), sample=20, uplo="U",
+         symmetry="H")
+    # the zero generator of test_hmat_zero.hpp:18-65, 92-128: disks of 500 and 400 points 10 apart,
+    # eta 0.1, epsilon 1e-8 (every ACA finds a zero first row: dense leaves of zeros)
+    save("disks500x400_zero", xt=geo.disk(500, 1.0, seed=36), xs=geo.disk(400, 11.0, seed=37), kernel=6, delta=0.0,
+         diag=0.0, mcs=10, eta=0.1, eps=1e-8, sym=False, x=rng.uniform(0, 10000, (400, 1)))
+    save_int_wrap("intwrap_disks70000", n=70000, seed=301, z=5.0)
     for dist, seed in ((15, 100), (40, 200)):
         for reqrank in (-1, 10):
             for sym in (False, True):

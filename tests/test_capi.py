@@ -41,9 +41,15 @@ def test_invalid_arguments_rejected_without_device(htg):
     # symmetry / UPLO as HMatrix's constructor checks them (hmatrix.hpp:116-121)
     prm.compressor = 0
     for sy, up, status in ((ord("S"), ord("N"), capi.HTG_ERR_INVALID), (ord("N"), ord("L"), capi.HTG_ERR_INVALID),
-                           (ord("X"), ord("L"), capi.HTG_ERR_INVALID), (ord("H"), ord("L"), capi.HTG_ERR_UNSUPPORTED)):
+                           (ord("X"), ord("L"), capi.HTG_ERR_INVALID)):
         prm.symmetry, prm.uplo = sy, up
         assert L.htg_hmatrix_build(C.byref(prm), C.byref(gen), 4, capi.ptr(pts), 4, None, C.byref(h)) == status
+    # Hermitian storage needs a Hermitian family (refused before any device is touched)
+    prm.symmetry, prm.uplo = ord("H"), ord("L")
+    helm = capi.htg_generator(4, 1, 0.0, 1.0, 3.0)
+    assert L.htg_hmatrix_build(C.byref(prm), C.byref(helm), 4, capi.ptr(pts), 4, None, C.byref(h)) == \
+        capi.HTG_ERR_UNSUPPORTED
+    assert b"Hermitian" in L.htg_last_error()
     prm.symmetry, prm.uplo = ord("S"), ord("L")
     gen2 = capi.htg_generator(0, 0, 0.0, 0.0)
     assert L.htg_hmatrix_build(C.byref(prm), C.byref(gen2), 4, capi.ptr(pts), 4, capi.ptr(pts), C.byref(h)) == \

@@ -174,3 +174,103 @@ def test_delayed_dense_blocks(htg, symmetric):
     D.set_storage("fp32")
     H.set_storage("fp32")
     assert rel(D @ x, H @ x) < 1e-6
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_transpose_keeps_supplied_dense_blocks(htg, orc, symmetric):
+    """The transpose of an H-matrix whose dense leaves were supplied by
+    build_dense_blocks carries those blocks (transposed): A^T x equals the
+    oracle's product over the transposed leaves (1e-12), for general and
+    symmetric storage, after a delayed build and after replacing the blocks."""
+    print(f"transpose of supplied dense blocks symmetric={symmetric}", flush=True)
+    from test_gpu_transpose import transposed_leaves
+    n = 4000
+    xt = geo.sphere(n, seed=46)
+    diag = geo.laplace_diag(geo.sphere_spacing(n))
+    kw = dict(symmetry="S", UPLO="U") if symmetric else {}
+    D = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32, **kw)
+    D.set_compression("partialACA")
+    D.set_delay_dense_computation(True)
+    D.build(htg.Generator(geo.KERNEL_LAPLACE, 0.0, diag), xt)
+    x = np.random.default_rng(9).standard_normal((n, 2))
+    T0 = D.transpose()  # zero near field on both sides
+    leaves, data = D.get_leaves(with_data=True)
+    pt, ps = D.get_perms()
+    if symmetric:
+        ref0 = orc.leaf_mvprod_sym(n, pt, leaves, data, x, "U")
+    else:
+        lt, dt = transposed_leaves(orc, leaves, data)
+        ref0 = orc.leaf_mvprod(n, n, ps, pt, lt, dt, x)
+    assert rel(T0.mvprod_global_to_global(x), ref0) < 1e-12
+    D.build_dense_blocks(LaplaceBlocks(xt, 2.0 * diag))  # a non-default diagonal: A^T must see the supplied blocks
+    leaves, data = D.get_leaves(with_data=True)
+    pt, ps = D.get_perms()
+    for T in (D.transpose(), None):
+        yt = T.mvprod_global_to_global(x) if T is not None else D.mvprod_transp_global_to_global(x)
+        if symmetric:
+            ref = orc.leaf_mvprod_sym(n, pt, leaves, data, x, "U")  # a symmetric matrix is its own transpose
+        else:
+            lt, dt = transposed_leaves(orc, leaves, data)
+            ref = orc.leaf_mvprod(n, n, ps, pt, lt, dt, x)
+        assert rel(yt, ref) < 1e-12
+    if not symmetric:
+        Tl, Td = D.transpose().get_leaves(with_data=True)
+        lt, dt = transposed_leaves(orc, leaves, data)
+        assert np.array_equal(Tl, lt) and np.array_equal(Td, dt)
+
+
+HERM_FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*_herm[LU].npz")))
+
+
+@pytest.mark.parametrize("name", HERM_FIXTURES)
+def test_hermitian_build_matches_reference_fixture(htg, name):
+    """HMatrix(..., 'H', UPLO) with GeneratorTestComplexHermitian from the
+    reference build: permutation, kept leaf list, false positives and y = H x."""
+    print(f"hermitian fixture {name}", flush=True)
+    g = dict(np.load(os.path.join(GOLDEN, name + ".npz")))
+    uplo = chr(int(g["uplo"]))
+    assert chr(int(g["symmetry"])) == "H"
+    H = htg.HMatrix(epsilon=float(g["eps"]), eta=float(g["eta"]), minclustersize=int(g["mcs"]), symmetry="H",
+                    UPLO=uplo)
+    H.set_compression("sympartialACA" if int(g["sym"]) else "partialACA")
+    H.build(htg.Generator(geo.KERNEL_HERMITIAN_TEST), g["xt"])
+    pt, ps = H.get_perms()
+    assert np.array_equal(pt, g["perm_t"])
+    assert np.array_equal(H.get_leaves(), g["leaves"]), "leaf list or ranks differ"
+    i = H.get_info()
+    assert i.n_false_positive == int(g["false_positive"])
+    assert chr(i.symmetry) == "H" and chr(i.uplo) == uplo and i.is_complex
+    assert rel(H.mvprod_global_to_global(g["x"]), g["y"]) < 1e-12
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_hermitian_matches_oracle(htg, orc, uplo):
+    """The 'H' product over the exported kept leaves (conjugate transposes of the
+    off-diagonal ones, hemv on the diagonal leaves) equals the oracle's at 1e-12
+    for 1, 16 and 64 right-hand sides; FP32 storage within 1e-6; the transpose
+    of a Hermitian matrix is its conjugate; a non-Hermitian kernel is refused."""
+    print(f"hermitian {uplo}", flush=True)
+    n = 5000
+    xt = geo.sphere(n, seed=47)
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32, symmetry="H", UPLO=uplo)
+    H.build(htg.Generator(geo.KERNEL_HERMITIAN_TEST), xt)
+    O = orc.hmat_build(xt, kernel=geo.KERNEL_HERMITIAN_TEST, minclustersize=32, eta=10.0, eps=1e-6, sym=True,
+                       uplo=uplo, symmetry="H")
+    leaves, data = H.get_leaves(with_data=True)
+    assert np.array_equal(leaves, O.leaves)
+    pt, _ = H.get_perms()
+    rng = np.random.default_rng(11)
+    for mu in (1, 16, 64):
+        x = rng.standard_normal((n, mu)) + 1j * rng.standard_normal((n, mu))
+        y = H.mvprod_global_to_global(x)
+        assert rel(y, orc.leaf_mvprod_herm(n, pt, leaves, data, x, uplo)) < 1e-12, mu
+        assert rel(y, O.mvprod(x)) < 1e-9, mu
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = H @ x
+    T = H.transpose()
+    assert rel(T @ x, np.conj(H @ np.conj(x))) < 1e-12
+    H.set_storage("fp32")
+    assert rel(H @ x, y) < 1e-6
+    B = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=32, symmetry="H", UPLO=uplo)
+    with pytest.raises(Exception, match="Hermitian"):
+        B.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, 1.0, 3.0), xt)

@@ -98,6 +98,28 @@ def test_transpose_helmholtz_is_not_conjugated(htg, orc):
     assert rel(y, H.mvprod_global_to_global(v)) < 1e-5
 
 
+@pytest.mark.parametrize("same", [True, False])
+def test_transpose_of_a_kernel_that_is_not_symmetric(htg, orc, same):
+    """GeneratorTestComplexHermitian is not symmetric (A(j, i) = conj A(i, j)): the
+    transposed dense blocks must be the generator's entries with rows and columns
+    exchanged, not the generator evaluated with the point sets exchanged."""
+    t = geo.sphere(3000, seed=85)
+    # (sign(x_i - x_j) is discontinuous: keep the two sets apart in x so that far blocks are low rank)
+    s = None if same else geo.disk(1700, 0.0, seed=86) + np.array([3.5, 0.0, 0.0])
+    H = htg.HMatrix(epsilon=1e-6, eta=10.0, minclustersize=10)
+    H.set_compression("partialACA")
+    H.build(htg.Generator(geo.KERNEL_HERMITIAN_TEST), t, s)
+    leaves, data = H.get_leaves(with_data=True)
+    pt, ps = H.get_perms()
+    lt, dt = transposed_leaves(orc, leaves, data)
+    T = H.transpose()
+    tl, td = T.get_leaves(with_data=True)
+    assert np.array_equal(tl, lt) and np.array_equal(td, dt)
+    rng = np.random.default_rng(6)
+    v = rng.standard_normal((H.nb_rows(), 3)) + 1j * rng.standard_normal((H.nb_rows(), 3))
+    assert rel(T.mvprod_global_to_global(v), orc.leaf_mvprod(H.nb_cols(), H.nb_rows(), ps, pt, lt, dt, v)) < 1e-12
+
+
 def test_transpose_of_a_batch(htg):
     pts = [geo.ellipsoid(n, (1.0, 0.8, 1.5), seed=90 + n) for n in (2500, 4000, 3100)]
     gens = [htg.Generator(geo.KERNEL_INV_DELTA_R, 1e-5) for _ in pts]

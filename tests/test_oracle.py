@@ -31,16 +31,17 @@ def load(name):
 
 
 HMAT_FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                       if not os.path.basename(p).startswith("lrmat"))
+                       if not os.path.basename(p).startswith(("lrmat", "intwrap")))
 LRMAT_FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "lrmat*.npz")))
 
 
 def oracle_from_fixture(orc, g):
     xs = None if int(g["same"]) else g["xs"]
     uplo = chr(int(g["uplo"])) if "uplo" in g and int(g["uplo"]) else "N"
+    symmetry = chr(int(g["symmetry"])) if "symmetry" in g else "S"
     return orc.hmat_build(g["xt"], xs, kernel=int(g["kernel"]), delta=float(g["delta"]), diag=float(g["diag"]),
                           minclustersize=int(g["mcs"]), eta=float(g["eta"]), eps=float(g["eps"]),
-                          sym=bool(int(g["sym"])), uplo=uplo)
+                          sym=bool(int(g["sym"])), uplo=uplo, symmetry=symmetry)
 
 
 @pytest.mark.parametrize("name", HMAT_FIXTURES)
@@ -73,7 +74,11 @@ def test_oracle_matches_reference_fixture(orc, name):
             else:
                 assert np.array_equal(arr[0], u) and np.array_equal(arr[1], v)
     y = O.mvprod(g["x"])
-    assert rel(y, g["y"]) < 1e-13
+    if int(g["kernel"]) == geo.KERNEL_ZERO:  # test_hmat_zero.hpp:163-166
+        assert np.linalg.norm(g["y"]) == 0 and np.linalg.norm(y) < 1e-15
+        assert (O.leaves[:, 4] == -1).all() and O.false_positive > 0  # every ACA found a zero first row
+    else:
+        assert rel(y, g["y"]) < 1e-13
 
 
 @pytest.mark.parametrize("name", LRMAT_FIXTURES)
@@ -115,15 +120,20 @@ def test_symmetric_oracle_matches_reference_driver(orc, ref):
         (geo.sphere(3000, seed=5), geo.KERNEL_LAPLACE, 0.0, geo.laplace_diag(geo.sphere_spacing(3000)), 0.0, 0, "U"),
         (geo.cylinder(2500, seed=6), geo.KERNEL_INV_DELTA_R, 1e-5, 0.0, 0.0, 1, "U"),
         (geo.sphere(2000, seed=7), 4, 0.0, 2.0, 9.0, 1, "L"),
+        # Hermitian storage HMatrix(..., 'H', UPLO) with the reference's GeneratorTestComplexHermitian
+        (geo.sphere(2500, seed=8), 7, 0.0, 0.0, 0.0, 1, "L", "H"),
+        (geo.cylinder(2000, seed=9), 7, 0.0, 0.0, 0.0, 0, "U", "H"),
     ]
-    for xt, kernel, delta, diag, k, comp, uplo in cases:
+    for case in cases:
+        xt, kernel, delta, diag, k, comp, uplo = case[:7]
+        symmetry = case[7] if len(case) > 7 else "S"
         n = xt.shape[0]
         rng = np.random.default_rng(1)
-        x = rng.standard_normal((n, 2)) + (1j * rng.standard_normal((n, 2)) if kernel >= 4 else 0)
+        x = rng.standard_normal((n, 2)) + (1j * rng.standard_normal((n, 2)) if kernel in geo.COMPLEX_KERNELS else 0)
         r = ref.run(xt, None, x=x, kernel=kernel, delta=delta, diag=diag, wavenumber=k, minclustersize=10, eta=10.0,
-                    eps=1e-6, compressor=comp, mu=2, uplo=uplo)
+                    eps=1e-6, compressor=comp, mu=2, uplo=uplo, symmetry=symmetry)
         O = orc.hmat_build(xt, kernel=kernel, delta=delta, diag=diag, wavenumber=k, minclustersize=10, eta=10.0,
-                           eps=1e-6, sym=bool(comp), uplo=uplo)
+                           eps=1e-6, sym=bool(comp), uplo=uplo, symmetry=symmetry)
         order = np.lexsort((r.leaves[:, 2], r.leaves[:, 0]))
         assert np.array_equal(O.leaves, r.leaves[order])
         assert O.false_positive == int(r.info["false_positive"])
@@ -155,7 +165,30 @@ def test_symmetric_storage_is_the_same_matrix(orc, uplo):
     assert (cover == 1).all()
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2, 3])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_hermitian_storage_is_the_same_matrix(orc, uplo):
+    """One triangle of a Hermitian matrix applied with the conjugate transposes
+    of its off-diagonal blocks (and hemv on the diagonal leaves) is the whole
+    matrix: the 'H' product agrees with the general H-matrix and with dense rows
+    to the ACA tolerance; 'S' on the same leaves (plain transposes) does not."""
+    n = 1500
+    xt = geo.sphere(n, seed=10)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    K = geo.KERNEL_HERMITIAN_TEST
+    Oh = orc.hmat_build(xt, kernel=K, minclustersize=10, eps=1e-8, sym=True, uplo=uplo, symmetry="H")
+    Of = orc.hmat_build(xt, kernel=K, minclustersize=10, eps=1e-8, sym=True)
+    yh = Oh.mvprod(x)
+    err = rel(yh, Of.mvprod(x))
+    assert err < 1e-7
+    rows = np.arange(0, n, 7, dtype=np.int32)
+    assert rel(yh[rows], orc.dense_rows_c(xt, None, rows, x, kernel=K, same=True)) < 1e-7
+    Os = orc.hmat_build(xt, kernel=K, minclustersize=10, eps=1e-8, sym=True, uplo=uplo, symmetry="S")
+    # (the 1e5 diagonal dominates the product, so the missing conjugations show at ~1e-4)
+    assert np.array_equal(Os.leaves, Oh.leaves) and rel(Os.mvprod(x), yh) > 100 * err
+
+
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 6])
 def test_kernel_entries_closed_form(orc, kernel):
     xt = geo.sphere(37, seed=1)
     xs = geo.disk(23, 2.0, seed=2)
@@ -164,20 +197,21 @@ def test_kernel_entries_closed_form(orc, kernel):
     A = orc.kernel_block(kernel, 1e-5, 0.0, xt, xs, rows, cols)
     r = np.linalg.norm(xt[rows][:, None, :] - xs[cols][None, :, :], axis=2)
     ref = {0: 1 / (4 * math.pi * r), 1: 1 / (1e-5 + r), 2: 1 / (1e-5 + 4 * math.pi * r),
-           3: 1 / (4 * math.pi * np.sqrt(1e-5 + r * r))}[kernel]
+           3: 1 / (4 * math.pi * np.sqrt(1e-5 + r * r)), 6: np.zeros_like(r)}[kernel]
     assert np.allclose(A, ref, rtol=1e-14, atol=0)
     # diagonal rule of the singular Laplace kernel on a square same-point matrix
     D = orc.kernel_block(0, 0.0, 7.5, xt, None, rows, rows, same=True)
     assert np.all(np.diag(D) == 7.5)
 
 
-@pytest.mark.parametrize("kernel", [4, 5])
+@pytest.mark.parametrize("kernel", [4, 5, 7])
 def test_complex_kernel_entries_closed_form(orc, kernel):
     """Columns of the complex families by brute force against numpy:
-    e^{ikr}/(4 pi r) (diagonal value on a square matrix) and (1 + i)/(4 pi r)."""
+    e^{ikr}/(4 pi r) (diagonal value on a square matrix), (1 + i)/(4 pi r) and
+    (1 + sign(x_i - x_j) i)/(1e-5 + 4 pi r), which is Hermitian entry for entry."""
     xt = geo.sphere(41, seed=3)
     # (1 + i)/(4 pi r) is singular on the diagonal: disjoint point sets, as the reference's tests use
-    xs = None if kernel == 4 else geo.disk(37, 2.0, seed=4)
+    xs = None if kernel in (4, 7) else geo.disk(37, 2.0, seed=4)
     ns = 41 if xs is None else 37
     k, diag = 7.0, 3.25
     rows = np.arange(41, dtype=np.int32)
@@ -188,11 +222,18 @@ def test_complex_kernel_entries_closed_form(orc, kernel):
         A[:, j] = orc.dense_rows_c(xt, xs, rows, e, kernel=kernel, wavenumber=k, diag=diag, same=xs is None)
     r = np.linalg.norm(xt[:, None, :] - (xt if xs is None else xs)[None, :, :], axis=2)
     with np.errstate(divide="ignore", invalid="ignore"):
-        ref = np.exp(1j * k * r) / (4 * math.pi * r) if kernel == 4 else (1 + 1j) / (4 * math.pi * r)
+        if kernel == 4:
+            ref = np.exp(1j * k * r) / (4 * math.pi * r)
+        elif kernel == 5:
+            ref = (1 + 1j) / (4 * math.pi * r)
+        else:
+            ref = (1 + 1j * np.sign(xt[:, None, 0] - xt[None, :, 0])) / (1e-5 + 4 * math.pi * r)
     off = ~np.eye(41, ns, dtype=bool) if xs is None else np.ones((41, ns), bool)
     assert np.allclose(A[off], ref[off], rtol=1e-14, atol=0)
     if kernel == 4:
         assert np.all(np.diag(A) == diag)
+    if kernel == 7:
+        assert np.array_equal(A, A.conj().T) and np.all(np.diag(A) == 1.0 / 1e-5)
 
 
 def test_complex_oracle_matches_reference_driver(orc, ref):
@@ -317,3 +358,58 @@ def test_aca_rejects_incompressible_block(orc):
     xs = geo.sphere(8, seed=4) * 5 + 0.01
     k, U, V = orc.aca(xt, xs, np.arange(8), np.arange(8), kernel=0, eps=1e-14, ctr_t=xt.mean(0), ctr_s=xs.mean(0))
     assert k == -1 and U is None
+
+
+def test_aca_size_test_int_wrap_matches_reference(orc):
+    """partialACA's q (M + N) > M N exit (partialACA.hpp:74) is evaluated in int
+    by the reference: the fixture (written by the reference build) has two disks
+    of 70000 points whose admissible 70000 x ~35000 blocks have M N > 2^31 - 1;
+    the wrapped product is negative, the reference returns rank -1 at q = 1 and
+    splits them (two false positives, four leaves). The oracle's 32-bit mode
+    gives that leaf list and product; its default exact mode compresses the two
+    blocks. The whole 70000 x 70000 block wraps to a positive 605032704, on which
+    the reference's ACA runs normally (rank 14), as the oracle's 32-bit mode does."""
+    g = load("intwrap_disks70000")
+    n, seed, z = int(g["n"]), int(g["seed"]), float(g["z"])
+    t = geo.disk(n, 0.0, seed=seed)
+    s = geo.disk(n, z, seed=seed + 1)
+    x = np.random.default_rng(seed + 2).standard_normal((n, 1))
+    T = orc.cluster_build(t, 10)
+    S = orc.cluster_build(s, 10)
+    with orc.ref_int_overflow(True):
+        assert orc.aca_size_bound(46340, 46340) == 46340 ** 2
+        assert orc.aca_size_bound(46341, 46341) == 46341 ** 2 - 2 ** 32 < 0
+        assert orc.aca_size_bound(n, n) == n * n - 2 ** 32 == 605032704
+        O = orc.hmat_build(t, s, kernel=geo.KERNEL_LAPLACE, minclustersize=10, eta=10.0, eps=1e-6)
+        k, _, _ = orc.aca(t, s, T.perm, S.perm, kernel=geo.KERNEL_LAPLACE, eps=1e-6, ctr_t=T.geom[0, :3],
+                          ctr_s=S.geom[0, :3])
+    assert np.array_equal(O.leaves, g["leaves"]) and O.false_positive == int(g["false_positive"]) == 2
+    assert k == int(g["whole_block_rank"]) == 14
+    assert rel(O.mvprod(x)[g["rows"], 0], g["y_rows"]) < 1e-12
+    assert orc.aca_size_bound(46341, 46341) == 46341 ** 2  # back to exact arithmetic outside the context
+    E = orc.hmat_build(t, s, kernel=geo.KERNEL_LAPLACE, minclustersize=10, eta=10.0, eps=1e-6)
+    assert len(E.leaves) == 2 and E.false_positive == 0
+    assert all(int(m) == n and int(m) * int(nn) > 2 ** 31 - 1 for _, m, _, nn, _ in E.leaves)
+    assert rel(E.mvprod(x)[g["rows"], 0], g["y_rows"]) < 1e-5  # the same matrix within the ACA tolerance
+
+
+def test_aca_size_test_int_wrap_reference_driver(orc, ref):
+    """The reference's own leaf compression either side of M N = 2^31: a
+    46340 x 46340 block compresses, a 46341 x 46341 block returns rank -1 at once;
+    the oracle's 32-bit mode agrees, its exact mode compresses both."""
+    for n, ref_rank in ((46340, None), (46341, -1)):
+        t = geo.disk(n, 0.0, seed=301)
+        s = geo.disk(n, 5.0, seed=302)
+        r = ref.run(t, s, kernel=geo.KERNEL_LAPLACE, eps=1e-6, minclustersize=10, compressor=0, mode=1,
+                    dump_data=False)
+        T = orc.cluster_build(t, 10)
+        S = orc.cluster_build(s, 10)
+        kw = dict(kernel=geo.KERNEL_LAPLACE, eps=1e-6, ctr_t=T.geom[0, :3], ctr_s=S.geom[0, :3])
+        with orc.ref_int_overflow(True):
+            k32, _, _ = orc.aca(t, s, T.perm, S.perm, **kw)
+        k64, _, _ = orc.aca(t, s, T.perm, S.perm, **kw)
+        assert k32 == int(r.info["rank"])
+        if ref_rank is None:
+            assert k32 == k64 > 0
+        else:
+            assert k32 == ref_rank and k64 > 0
