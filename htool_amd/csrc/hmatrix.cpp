@@ -907,7 +907,7 @@ struct htg_graph {
 namespace htg {
 namespace {
 
-// Arguments of htg_aca_batch (host arrays; the
+// Arguments of htg_aca_batch / htg_aca_batch_noalloc (host arrays; the
 // meaning of each is in include/htool_amd.h).
 struct AcaBatchArgs {
     const htg_generator *gen;
@@ -1060,6 +1060,36 @@ void run_aca_batch(const AcaBatchArgs &a, const AcaBatchPlan &P, unsigned char *
             for (int j = 0; j < B.N; j++)
                 vo[r + (size_t)k * j] = sv[(size_t)r * B.N + j];
     }
+}
+
+// Device scratch of htg_aca_batch_noalloc: one array per GPU the library runs
+// on, held until the process ends. Host code never uses the array's name as an
+// address: noalloc_scratch() returns this device's address, looked up once per
+// device with hipGetSymbolAddress while that device is current.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex &noalloc_lock() {
+    static std::mutex m;
+    return m;
+}
+
+unsigned char *noalloc_scratch() {
+    static std::mutex lock;
+    static unsigned char *per_device[64] = {};
+    int dev = 0;
+    HTG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64)
+        throw Error(HTG_ERR_UNSUPPORTED, "more than 64 devices");
+    std::lock_guard<std::mutex> hold(lock);
+    if (!per_device[dev]) {
+        void *ptr = nullptr;
+        HTG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr)
+            throw Error(HTG_ERR_HIP, "no device address for the scratch array");
+        per_device[dev] = static_cast<unsigned char *>(ptr);
+    }
+    return per_device[dev];
 }
 
 } // namespace
@@ -1746,6 +1776,29 @@ int htg_aca_batch(const htg_generator *gen, int compressor, double epsilon, int 
         DevArray<unsigned char> buf;
         buf.alloc(plan.bytes);
         run_aca_batch(a, plan, buf.p);
+    });
+}
+
+int htg_aca_batch_noalloc(const htg_generator *gen, int compressor, double epsilon, int reqrank, int nt,
+                          const double *xt, int ns, const double *xs, int nblocks, const int *M, const int *N,
+                          const int *rows, const int *row_off, const int *cols, const int *col_off,
+                          const double *ctr_t, const double *ctr_s, const int *off_t, const int *off_s, int cap,
+                          int *ranks, double *U, const int64_t *u_off, double *V, const int64_t *v_off) {
+    return guard([&] {
+        AcaBatchArgs a{gen,    compressor, epsilon, reqrank, nt,    xt,    ns,    xs,    nblocks, M,     N,
+                       rows,   row_off,    cols,    col_off, ctr_t, ctr_s, off_t, off_s, cap,     ranks, U,
+                       u_off,  V,          v_off};
+        AcaBatchPlan plan = plan_aca_batch(a);
+        require_device();
+        if (plan.bytes > NOALLOC_SCRATCH_BYTES)
+            throw Error(HTG_ERR_INVALID, "htg_aca_batch_noalloc: the call needs " + std::to_string(plan.bytes) +
+                                             " bytes of device scratch, more than its " +
+                                             std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                             " (fewer or smaller blocks, a smaller cap, or htg_aca_batch)");
+        // one call at a time per device: the calls share the device's array
+        unsigned char *dev = noalloc_scratch();
+        std::lock_guard<std::mutex> hold(noalloc_lock());
+        run_aca_batch(a, plan, dev);
     });
 }
 

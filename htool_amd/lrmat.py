@@ -27,9 +27,10 @@ class Block:
 
 
 def aca_batch(gen: Generator, blocks: list, xt: np.ndarray, xs: np.ndarray | None = None, *, epsilon: float = 1e-6,
-              reqrank: int = -1, compressor: str = "partialACA", cap: int = 64):
+              reqrank: int = -1, compressor: str = "partialACA", cap: int = 64, noalloc: bool = False):
     """Returns a list of (rank, U, V): U (m, k), V (k, n); rank -1 means the block is
-    not compressible (the reference then stores it dense)."""
+    not compressible (the reference then stores it dense). noalloc: run through
+    htg_aca_batch_noalloc (no device allocation; the library's 64 MiB scratch)."""
     L = capi.lib()
     xt = np.ascontiguousarray(xt, dtype=np.float64)
     same = xs is None
@@ -59,7 +60,8 @@ def aca_batch(gen: Generator, blocks: list, xt: np.ndarray, xs: np.ndarray | Non
     V = np.zeros(int((N.astype(np.int64) * cap).sum()) or 1)
     ranks = np.zeros(nb, np.int32)
     g = gen._c(same)
-    capi.check(L.htg_aca_batch(C.byref(g), capi.SYM_PARTIAL_ACA if compressor == "sympartialACA" else
+    fn = L.htg_aca_batch_noalloc if noalloc else L.htg_aca_batch
+    capi.check(fn(C.byref(g), capi.SYM_PARTIAL_ACA if compressor == "sympartialACA" else
                                capi.PARTIAL_ACA, float(epsilon), int(reqrank), xt.shape[0], capi.ptr(xt),
                                xt.shape[0] if same else xs_.shape[0], capi.ptr(xs_), nb, capi.ptr(M), capi.ptr(N),
                                capi.ptr(rows), capi.ptr(row_off), capi.ptr(cols), capi.ptr(col_off), capi.ptr(ctr_t),

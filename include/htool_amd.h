@@ -304,6 +304,23 @@ int htg_aca_batch(const htg_generator *gen, int compressor, double epsilon, int 
                   const int *off_t, const int *off_s, int cap, int *ranks, double *U, const int64_t *u_off, double *V,
                   const int64_t *v_off);
 
+/* htg_aca_batch with no device allocation. No counterpart in the reference:
+ * for callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation). Same arguments, kernels and
+ * results as htg_aca_batch (ranks and factors bit for bit); every device buffer
+ * of the call (gathered points, block records, ACA workspace, visited flags,
+ * ranks) lives in one statically allocated scratch array of 64 MiB per GPU,
+ * held by the library until the process ends. A call that needs more (about
+ * 8 * cap * sum(M + N) + 33 * sum(M + N) bytes) returns HTG_ERR_INVALID before
+ * touching the device. Calls share the array: the library runs them one at a
+ * time, each leaving its contents behind (the kernels clear what must start
+ * at zero). Synchronous, on the default stream. */
+int htg_aca_batch_noalloc(const htg_generator *gen, int compressor, double epsilon, int reqrank, int nt,
+                          const double *xt, int ns, const double *xs, int nblocks, const int *M, const int *N,
+                          const int *rows, const int *row_off, const int *cols, const int *col_off,
+                          const double *ctr_t, const double *ctr_s, const int *off_t, const int *off_s, int cap,
+                          int *ranks, double *U, const int64_t *u_off, double *V, const int64_t *v_off);
+
 /* ACA records of the last build, one per ACA run of an admissible block (first
  * attempt): the 8 block features (ln m, ln n, ln d, ln diam_t, ln diam_s, eta,
  * ln epsilon, ln gap; d the centre distance, gap = max(d - rad_t - rad_s,
