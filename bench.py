@@ -17,6 +17,7 @@ Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -27,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+C_void = ctypes.c_void_p
 RECOMPRESS_TOL = 2e-6  # per-leaf relative Frobenius tail dropped by the FP64 recompression
 LOWPREC_TOL = 4e-6     # per-leaf relative error budget of the FP16-stored trailing terms
 
@@ -46,10 +48,12 @@ def parse():
     ap.add_argument("--no-helmholtz", action="store_true", help="skip the complex Helmholtz N=2e5 workload")
     ap.add_argument("--no-assembly-5e5", action="store_true",
                     help="skip the N=5e5 assembly with and without the learned rank predictor")
-    ap.add_argument("--cpu-reps", type=int, default=10)
+    ap.add_argument("--cpu-reps", type=int, default=20)
     ap.add_argument("--matmat-counters-run", action="store_true",
                     help="short run for rocprofv3 --pmc on the 32-RHS products: three products per storage "
                          "(FP64, FP32, FP32+FP16) of the headline H-matrices; prints nothing else")
+    ap.add_argument("--probe-only", action="store_true",
+                    help="run only the device bandwidth probe (htg_stream_probe) and print its JSON")
     ap.add_argument("--counters-run", action="store_true",
                     help="short run for rocprofv3 --pmc: a 256 MiB device copy of known size, then a few "
                          "mixed-precision and FP64 products; prints nothing else")
@@ -81,13 +85,34 @@ def leaf_bytes(leaves):
     return dense, u, v
 
 
+def host_cpu():
+    """CPU model, logical CPUs of the host and those this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"model": model, "cpu_count": os.cpu_count(), "affinity": allowed,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(pts, diag, x, leaves_entries, args):
     """The reference's own OpenMP CPU product (1 MPI rank) when its driver is
-    built, else the oracle's single-thread restatement, on the same workload."""
+    built, else the oracle's single-thread restatement, on the same workload.
+    Threads: the CPUs this process may use, at most 16 (the host's share for one
+    GPU on the box this runs on; a shared host is not ours to fill)."""
     import numpy as np
     from oracle import refcase
-    threads = os.cpu_count() or 1
-    threads = min(threads, 16)
+    hc = host_cpu()
+    threads = max(1, min(hc["affinity"], 16))
     if refcase.available():
         try:
             t0 = time.time()
@@ -102,10 +127,11 @@ def cpu_baseline(pts, diag, x, leaves_entries, args):
             b = 8 * (dense + lre) + 16 * n + 8 * n
             t = float(r.info["matvec_s"])
             return {"value": round(b / t / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "reference",
-                    "ms_per_matvec": round(t * 1e3, 3),
+                    "ms_per_matvec": round(t * 1e3, 3), "host": hc,
                     "sample": f"full N={n} workload, {args.cpu_reps} timed FP64 matvecs of the reference HMatrix "
-                              f"(OpenMP, MKL sequential) after its own build ({float(r.info['build_s']):.1f}s); "
-                              f"driver wall {wall:.1f}s"}
+                              f"after 3 warm-ups (OpenMP {threads} threads, MKL sequential, 1 MPI rank) after its "
+                              f"own build ({float(r.info['build_s']):.1f}s); driver wall {wall:.1f}s; "
+                              f"host {hc['model']}, {hc['cpu_count']} logical CPUs, {hc['affinity']} usable"}
         except Exception as e:  # fall through to the restatement
             print(f"reference baseline unavailable: {e}", file=sys.stderr)
     from oracle import lib as orc
@@ -118,7 +144,7 @@ def cpu_baseline(pts, diag, x, leaves_entries, args):
     t = (time.perf_counter() - t0) / reps
     b = 8 * leaves_entries + 24 * pts.shape[0]
     return {"value": round(b / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "ms_per_matvec": round(t * 1e3, 3),
+            "ms_per_matvec": round(t * 1e3, 3), "host": hc,
             "sample": f"oracle FP64 leaf loop, {reps} matvecs on the full workload, one thread"}
 
 
@@ -217,7 +243,10 @@ def helmholtz_2e5(torch, log):
     out = {"N": n, "wavenumber": round(k, 4), "h": round(h, 6), "epsilon": 1e-5, "eta": 10.0, "leaf_size": 32,
            "build_s": round(time.time() - t0, 3), "aca_s": round(i.aca_seconds, 3), "aca_wide_blocks": i.n_aca_wide,
            "rank_mean": round(i.rank_mean, 2),
-           "rank_max": i.rank_max, "dense_entries": int(i.dense_entries), "lowrank_entries": int(i.lowrank_entries)}
+           "rank_max": i.rank_max, "dense_entries": int(i.dense_entries), "lowrank_entries": int(i.lowrank_entries),
+           # blocks with M N > 2^31 - 1 that the reference's int test q (M + N) > M N splits and the
+           # exact 64-bit test used here compresses (htg_params.ref_int_overflow follows the reference)
+           "aca_int_wrap_blocks": i.n_aca_int_wrap}
     rng = np.random.default_rng(9)
     X = rng.standard_normal((n, 64)) + 1j * rng.standard_normal((n, 64))
     dX = torch.from_numpy(np.asfortranarray(X).reshape(-1, order="F").copy()).cuda()
@@ -303,6 +332,13 @@ def main():
     from htool_amd import Generator, HMatrix, geometry
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.probe_only:
+        from htool_amd import capi
+        rd, cp, bg = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        capi.check(capi.lib().htg_stream_probe(1 << 30, 20, ctypes.byref(rd), ctypes.byref(cp), ctypes.byref(bg)))
+        print(json.dumps({"read_gbps": round(rd.value, 1), "copy_gbps": round(cp.value, 1),
+                          "read_best_grid": bg.value}), flush=True)
+        return
     n = args.n
     pts = geometry.sphere(n, seed=0)
     h = geometry.sphere_spacing(n)
@@ -331,9 +367,10 @@ def main():
     rng = np.random.default_rng(1)
     x = rng.standard_normal(n)
     if args.counters_run:
+        from htool_amd import capi
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
         b = torch.empty_like(a)
-        b.copy_(a)
+        capi.check(capi.lib().htg_stream_copy(C_void(a.data_ptr()), C_void(b.data_ptr()), 256 << 20))
         torch.cuda.synchronize()
         dx = torch.from_numpy(x).cuda()
         dy = torch.zeros(n, dtype=torch.float64, device="cuda")
@@ -414,6 +451,15 @@ def main():
     dom = max(kern, key=lambda k: kern[k][0])
     achieved = kern[dom][1] / (kern[dom][0] * 1e-3) / 1e9
 
+    # the device's achievable streaming rate, measured with the library's own
+    # read and copy kernels (htg_stream_probe, 1 GiB, 20 launches per grid size)
+    from htool_amd import capi
+    rd, cp, bg = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    capi.check(capi.lib().htg_stream_probe(1 << 30, 20, ctypes.byref(rd), ctypes.byref(cp), ctypes.byref(bg)))
+    measured = {"read_gbps": round(rd.value, 1), "copy_gbps": round(cp.value, 1), "read_best_grid": bg.value,
+                "kernel": "htg_stream_probe (k_stream_read, k_stream_copy), 1 GiB"}
+    log(f"stream probe: {measured}")
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.isfile(tpath):
@@ -459,6 +505,7 @@ def main():
         "fp32": entry("fp32"),
         "mixed": entry("mixed"),
         "speedup_mixed_vs_fp64": round(res["fp64"]["ms"] / mixed["ms"], 3),
+        "speedup_target": 4.0,
         "mixed_host_buffers_pcie": host,
         "build_s": round(build_s, 3),
         "build_mixed_s": round(build_m_s, 3),
@@ -466,11 +513,16 @@ def main():
                    "rank_mean": round(info.rank_mean, 3), "rank_max": info.rank_max,
                    "dense_entries": dense_e, "lowrank_entries": u_e + v_e,
                    "mixed_rank_mean": round(mi.rank_mean, 3), "mixed_lowrank_entries": mi.lowrank_entries,
-                   "mixed_fp16_entries": mi.lowprec_entries},
+                   "mixed_fp16_entries": mi.lowprec_entries, "aca_int_wrap_blocks": info.n_aca_int_wrap},
         "kernels_ms": {k: round(v, 5) for k, v in p.items()},
         "kernel_algorithmic_bytes": {k: int(v[1]) for k, v in kern.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "measured_peak_gbps": measured["read_gbps"],
+                     "frac_of_measured_peak": round(achieved / measured["read_gbps"], 4),
+                     "whole_step_frac": round(value / HBM_PEAK_GBS, 4),
+                     "whole_step_frac_of_measured_peak": round(value / measured["read_gbps"], 4)},
+        "stream_probe": measured,
         "cpu_baseline": cpu,
         "assembly_5e5": asm,
         "batch_64": batch,

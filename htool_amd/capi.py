@@ -93,6 +93,8 @@ _SIGNATURES = {
     "htg_aca_batch_noalloc": (_I, [C.POINTER(htg_generator), _I, _D, _I, _I, _VP, _I, _VP, _I, _VP, _VP, _VP, _VP, _VP,
                                    _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "htg_release_workspace": (_I, []),
+    "htg_stream_probe": (_I, [C.c_uint64, _I, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I)]),
+    "htg_stream_copy": (_I, [_VP, _VP, C.c_uint64]),
     "htg_hmatrix_aca_records": (_I, [_VP, C.POINTER(_I), _VP, _VP, _VP]),
     "htg_hmatrix_build_batch": (_I, [C.POINTER(htg_params), _I, _VP, _VP, _VP, C.POINTER(_VP)]),
     "htg_hmatrix_transpose": (_I, [_VP, C.POINTER(_VP)]),

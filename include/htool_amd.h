@@ -287,6 +287,17 @@ int htg_hmatrix_profile(htg_hmatrix *h, const double *d_x, double *d_y, int mu, 
  * counterpart in the reference (which allocates per block). */
 int htg_release_workspace(void);
 
+/* Device bandwidth probe (no counterpart in the reference): the best rate over
+ * a few grid sizes of a streaming read (16-byte loads, four in flight per lane)
+ * and of a streaming copy of `bytes` (>= 1 MiB), each averaged over `reps`
+ * launches after a warm-up; GB/s = bytes read (copy: read + written) / time.
+ * best_grid (may be NULL): workgroups of the fastest read. */
+int htg_stream_probe(uint64_t bytes, int reps, double *read_gbps, double *copy_gbps, int *best_grid);
+/* One copy of `bytes` (a multiple of 16) between device buffers with the
+ * probe's copy kernel, synchronous: a transfer of known size for scaling HBM
+ * traffic counters. */
+int htg_stream_copy(const void *d_src, void *d_dst, uint64_t bytes);
+
 /* Batched stand-alone leaf compression, replaces calling
  * VirtualLowRankGenerator::copy_low_rank_approximation once per block
  * (include/htool/lrmat/virtual_lrmat_generator.hpp:15, partialACA.hpp:39-171,

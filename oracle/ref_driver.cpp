@@ -185,9 +185,11 @@ void run_hmatrix(std::shared_ptr<Cluster<PCAGeometricClustering>> t, std::shared
     if (dump)
         write_vec(out + "/data.f64", data);
 
-    // Products: one untimed warm-up, then nreps timed ones.
+    // Products: three untimed warm-ups (one when no timing is asked for), then
+    // nreps timed ones.
     std::vector<T> y(size_t(nt) * mu, T(0));
-    H.mvprod_global_to_global(x.data(), y.data(), mu);
+    for (int w = 0; w < (nreps > 0 ? 3 : 1); w++)
+        H.mvprod_global_to_global(x.data(), y.data(), mu);
     double tm = 0;
     for (int r = 0; r < nreps; r++) {
         t0 = now();

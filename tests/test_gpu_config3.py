@@ -1,0 +1,97 @@
+"""GPU parity on the complex Helmholtz workload's geometry and parameters at a
+size the oracle covers: N = 12000 points on the 4:1 prolate spheroid (semi-axes
+1, 1, 4, area-uniform), e^{ikr}/(4 pi r) with k = 2 pi / (10 h) (10 points per
+wavelength, h = sqrt(area / N)), 1/(4 pi h) on the diagonal, epsilon 1e-5,
+eta 10, leaf size 32, partialACA, every low-rank term stored 16-bit with its
+scale (the stored configuration of the N = 2e5 workload).
+
+* the leaf list and false positives equal the oracle's (pinned against the
+  reference's HMatrix<std::complex<double>>);
+* the FP64 product equals the oracle's product of the exported leaves within
+  1e-12 and the oracle's own H-matrix within 1e-10 (its factors round
+  differently), at 1, 16 and 64 right-hand sides, column by column;
+* FP32, FP32+FP16 and FP32+BF16 storage against the oracle's FP64 product of
+  the same exported leaves: within 1e-6, 1e-3 and 1e-2 (relative l2, all 64
+  columns), the 16-bit bounds being the unit roundoffs 2^-11 and 2^-8 times a
+  small factor for the sum over terms;
+* against dense FP64 rows: within 2 epsilon in FP64.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from htool_amd import geometry as geo
+
+pytestmark = pytest.mark.gpu
+
+N = 12000
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.fixture(scope="module")
+def case(htg, orc):
+    pts = geo.ellipsoid(N, (1.0, 1.0, 4.0), seed=5)
+    h = math.sqrt(geo.prolate_spheroid_area(1.0, 4.0) / N)
+    k = 2 * math.pi / (10 * h)
+    diag = geo.laplace_diag(h)
+    H = htg.HMatrix(epsilon=1e-5, eta=10.0, minclustersize=32)
+    H.set_compression("partialACA")
+    H.set_lowprec(0.0, all_terms=True)
+    H.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, diag, k), pts)
+    O = orc.hmat_build(pts, kernel=geo.KERNEL_HELMHOLTZ, wavenumber=k, diag=diag, minclustersize=32, eta=10.0,
+                       eps=1e-5)
+    leaves, data = H.get_leaves(with_data=True)
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((N, 64)) + 1j * rng.standard_normal((N, 64))
+    pt, ps = H.get_perms()
+    Y_leaves = orc.leaf_mvprod(N, N, pt, ps, leaves, data, X)
+    return dict(pts=pts, k=k, diag=diag, H=H, O=O, leaves=leaves, data=data, X=X, Y_leaves=Y_leaves)
+
+
+def test_config3_leaves_match_oracle(case):
+    H, O = case["H"], case["O"]
+    assert np.array_equal(case["leaves"], O.leaves)
+    i = H.get_info()
+    assert i.n_false_positive == O.false_positive
+    assert i.is_complex and i.lowprec_entries == i.lowrank_entries > 0
+    assert i.n_aca_int_wrap == 0  # no block reaches M N > 2^31 - 1 at this size
+
+
+@pytest.mark.parametrize("mu", [1, 16, 64])
+def test_config3_fp64_products(case, orc, mu):
+    print(f"config 3 fp64 mu={mu}", flush=True)
+    H = case["H"]
+    H.set_storage("fp64")
+    X = case["X"][:, :mu]
+    Y = H.mvprod_global_to_global(X)
+    Yo = case["O"].mvprod(X)
+    for c in range(mu):
+        assert rel(Y[:, c], case["Y_leaves"][:, c]) < 1e-12, c
+        assert rel(Y[:, c], Yo[:, c]) < 1e-10, c
+
+
+@pytest.mark.parametrize("storage,bound", [("fp32", 1e-6), ("fp32_fp16", 1e-3), ("fp32_bf16", 1e-2)])
+def test_config3_storages_against_fp64_leaves(case, storage, bound):
+    print(f"config 3 {storage}", flush=True)
+    H = case["H"]
+    H.set_storage(storage)
+    for mu in (1, 64):
+        Y = H.mvprod_global_to_global(case["X"][:, :mu])
+        err = rel(Y, case["Y_leaves"][:, :mu])
+        print(f"  mu={mu} rel err vs FP64 leaves {err:.3e}", flush=True)
+        assert err < bound, (mu, err)
+    H.set_storage("fp64")
+
+
+def test_config3_against_dense(case, orc):
+    H = case["H"]
+    H.set_storage("fp64")
+    rows = np.random.default_rng(10).choice(N, 300, replace=False).astype(np.int32)
+    x = case["X"][:, 0]
+    yd = orc.dense_rows_c(case["pts"], None, rows, x, kernel=geo.KERNEL_HELMHOLTZ, wavenumber=case["k"],
+                          diag=case["diag"], same=True)
+    assert rel((H @ x)[rows], yd) < 2e-5
