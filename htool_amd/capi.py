@@ -40,7 +40,21 @@ class htg_params(C.Structure):
                 ("minclustersize", C.c_int), ("compressor", C.c_int), ("reqrank", C.c_int),
                 ("lowprec_all", C.c_int), ("recompress_tol", C.c_double), ("lowprec_tol", C.c_double),
                 ("predictor", C.c_void_p), ("predictor_margin", C.c_double), ("aca_wide_min", C.c_int),
-                ("symmetry", C.c_int), ("uplo", C.c_int), ("delay_dense", C.c_int), ("ref_int_overflow", C.c_int)]
+                ("symmetry", C.c_int), ("uplo", C.c_int), ("delay_dense", C.c_int), ("mintargetdepth", C.c_int),
+                ("minsourcedepth", C.c_int), ("no_permutation", C.c_int), ("ref_int_overflow", C.c_int)]
+
+
+COPY_SUBMATRIX_FN = C.CFUNCTYPE(None, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_double), C.c_void_p)
+
+
+class htg_host_generator(C.Structure):
+    _fields_ = [("copy_submatrix", COPY_SUBMATRIX_FN), ("ctx", C.c_void_p), ("is_complex", C.c_int)]
+
+
+class htg_cluster_tree(C.Structure):
+    _fields_ = [("n", C.c_int), ("perm", C.c_void_p), ("nnodes", C.c_int), ("nodes_int", C.c_void_p),
+                ("nodes_dbl", C.c_void_p)]
 
 
 class htg_layout(C.Structure):
@@ -59,7 +73,8 @@ class htg_info(C.Structure):
                 ("n_aca_retry", C.c_int), ("aca_ws_entries", C.c_int64), ("aca_need_entries", C.c_int64),
                 ("aca_seconds", C.c_double), ("is_complex", C.c_int), ("n_aca_wide", C.c_int),
                 ("symmetry", C.c_int), ("uplo", C.c_int), ("mirrored_entries", C.c_int64),
-                ("mirrored_lowprec_entries", C.c_int64), ("n_aca_int_wrap", C.c_int)]
+                ("mirrored_lowprec_entries", C.c_int64), ("nb_mat_vec_prod", C.c_int64),
+                ("total_time_mat_vec_prod", C.c_double), ("n_aca_int_wrap", C.c_int)]
 
 
 class HtgError(RuntimeError):
@@ -93,6 +108,14 @@ _SIGNATURES = {
     "htg_aca_batch_noalloc": (_I, [C.POINTER(htg_generator), _I, _D, _I, _I, _VP, _I, _VP, _I, _VP, _VP, _VP, _VP, _VP,
                                    _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "htg_release_workspace": (_I, []),
+    "htg_hmatrix_build_ex": (_I, [C.POINTER(htg_params), C.POINTER(htg_generator), C.POINTER(htg_host_generator), _I,
+                                  _VP, _I, _VP, C.POINTER(htg_cluster_tree), C.POINTER(htg_cluster_tree),
+                                  C.POINTER(_VP)]),
+    "htg_hmatrix_mymvprod_local_to_local_device": (_I, [_VP, _VP, _VP, _I, _VP, _VP]),
+    "htg_hmatrix_mymvprod_local_to_local": (_I, [_VP, _VP, _VP, _I]),
+    "htg_hmatrix_mvprod_subrhs": (_I, [_VP, _VP, _VP, _I, _I, _I, _I]),
+    "htg_hmatrix_get_dense": (_I, [_VP, _VP, _I]),
+    "htg_hmatrix_get_diagonal": (_I, [_VP, _VP, _I]),
     "htg_stream_probe": (_I, [C.c_uint64, _I, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I)]),
     "htg_stream_copy": (_I, [_VP, _VP, C.c_uint64]),
     "htg_hmatrix_aca_records": (_I, [_VP, C.POINTER(_I), _VP, _VP, _VP]),

@@ -97,6 +97,12 @@ struct Assembler {
     DevArray<double> &d_ws        = scratch.ws;
     DevArray<unsigned char> &d_vis = scratch.vis;
     bool sym = false; // symmetric storage (htg_params.symmetry 'S')
+    // A user generator on the host (htg_hmatrix_build_ex): every ACA runs the
+    // multi-workgroup kernels with the generator's rows and columns fetched
+    // through its copy_submatrix between the two halves of each cross term;
+    // prow / pcol give the original index of each cluster position.
+    const htg_host_generator *hg = nullptr;
+    const int *prow = nullptr, *pcol = nullptr;
 
     // Sons of a false positive (hmatrix.hpp:543-606): the larger side splits;
     // in a symmetric build both sides split into every pair of sons
@@ -329,7 +335,8 @@ struct Assembler {
     // with M + N >= aca_wide_min on the multi-workgroup path, the others one
     // workgroup each. Ranks in block order.
     std::vector<int> run_aca(const std::vector<AcaBlock> &blocks, const std::vector<uint64_t> &vis_off) {
-        const int wmin = prm.aca_wide_min != 0 ? prm.aca_wide_min : esz == 2 ? ACAW_MIN_DEFAULT_C : ACAW_MIN_DEFAULT;
+        const int wmin = hg ? 1 : prm.aca_wide_min != 0 ? prm.aca_wide_min : esz == 2 ? ACAW_MIN_DEFAULT_C
+                                                                                           : ACAW_MIN_DEFAULT;
         std::vector<int> narrow, wide;
         for (size_t i = 0; i < blocks.size(); i++)
             ((wmin > 0 && blocks[i].M + blocks[i].N >= wmin) ? wide : narrow).push_back((int)i);
@@ -403,8 +410,10 @@ struct Assembler {
         const int nw = (int)wide.size();
         launch_acaw_init(kd, dT, dS, d_b.p, d_st.p, nw, prm.reqrank, prm.epsilon, d_vis.p, d_v.p, nullptr);
         HTG_HIP(hipGetLastError());
+        if (hg)
+            host_terms(wb, st, d_b, d_st, d_v, d_t1, t1.size(), d_t2, t2.size(), d_pv, d_pi, d_dd, d_xs);
         // one cross term per iteration; every block stops by its cap (<= CAP_MAX terms)
-        for (int it = 0; it <= CAP_MAX; it++) {
+        for (int it = 0; it <= CAP_MAX && !hg; it++) {
             launch_acaw_term(kd, dT, dS, d_b.p, d_st.p, nw, d_t1.p, (int)t1.size(), d_t2.p, (int)t2.size(),
                              prm.reqrank, prm.epsilon, d_ws.p, esz == 2, d_vis.p, d_v.p, d_pv.p, d_pi.p, d_dd.p,
                              d_xs.p, nullptr);
@@ -422,6 +431,58 @@ struct Assembler {
             ranks[wide[k]] = st[k].rank;
         }
         return ranks;
+    }
+
+    // Cross terms of the multi-workgroup ACA with the host generator: before
+    // each half the block states come back, the generator fills the vector
+    // the half needs for every active block (side 2 at the side-1 pivot, then
+    // side 1 at the side-2 pivot; with `swap` the sides are columns and rows),
+    // and the values go to the device.
+    void host_terms(const std::vector<AcaBlock> &wb, std::vector<AcaWide> &st, DevArray<AcaBlock> &d_b,
+                    DevArray<AcaWide> &d_st, DevArray<uint64_t> &d_v, DevArray<AcawTask> &d_t1, size_t n1t,
+                    DevArray<AcawTask> &d_t2, size_t n2t, DevArray<double> &d_pv, DevArray<int> &d_pi,
+                    DevArray<double> &d_dd, DevArray<double> &d_xs) {
+        const int nw = (int)wb.size();
+        std::vector<uint64_t> gv_off(nw);
+        uint64_t tot = 0;
+        for (int k = 0; k < nw; k++) {
+            gv_off[k] = tot;
+            tot += (uint64_t)std::max(wb[k].M, wb[k].N);
+        }
+        std::vector<double> h_gv(tot * esz);
+        DevArray<double> d_gv;
+        DevArray<uint64_t> d_go;
+        d_gv.alloc(tot * esz);
+        d_go.upload(gv_off);
+        for (int it = 0; it <= CAP_MAX; it++) {
+            for (int half = 0; half < 2; half++) {
+                HTG_HIP(hipMemcpy(st.data(), d_st.p, st.size() * sizeof(AcaWide), hipMemcpyDeviceToHost));
+                bool any = false;
+                for (int k = 0; k < nw; k++) {
+                    if (!st[k].active)
+                        continue;
+                    any = true;
+                    const AcaBlock &B = wb[k];
+                    const int *rows = prow + B.off_t, *cols = pcol + B.off_s;
+                    double *dst     = h_gv.data() + esz * gv_off[k];
+                    // half 0: side-2 vector at I1; half 1: side-1 vector at I2
+                    const int piv      = half == 0 ? st[k].I1 : st[k].I2;
+                    const bool row_vec = (half == 0) != (B.swap != 0); // a row of the block: 1 x N
+                    if (row_vec)
+                        hg->copy_submatrix(1, B.N, rows + piv, cols, dst, hg->ctx);
+                    else
+                        hg->copy_submatrix(B.M, 1, rows, cols + piv, dst, hg->ctx);
+                }
+                if (!any)
+                    return;
+                HTG_HIP(hipMemcpy(d_gv.p, h_gv.data(), h_gv.size() * sizeof(double), hipMemcpyHostToDevice));
+                launch_acaw_half(half == 0, dT, dS, d_b.p, d_st.p, nw, half == 0 ? d_t2.p : d_t1.p,
+                                 (int)(half == 0 ? n2t : n1t), half == 0 ? 0 : (int)n2t, prm.reqrank, prm.epsilon,
+                                 d_ws.p, esz == 2, d_vis.p, d_v.p, d_pv.p, d_pi.p, d_dd.p, d_xs.p, d_gv.p, d_go.p,
+                                 nullptr);
+                HTG_HIP(hipGetLastError());
+            }
+        }
     }
 
     void grow_store(uint64_t need) { // need: entries
@@ -444,30 +505,62 @@ struct Assembler {
 
 void assemble(const htg_params &prm, const htg_generator &gen, int nt, const double *xt, int ns, const double *xs,
               Assembly &res) {
-    const bool same = gen.same_points || xs == nullptr;
+    assemble_ex(prm, &gen, nullptr, gen.same_points || xs == nullptr, nt, xt, ns, xs, nullptr, nullptr, res);
+}
+
+void assemble_ex(const htg_params &prm, const htg_generator *gen, const htg_host_generator *hgen, bool same, int nt,
+                 const double *xt, int ns, const double *xs, const ClusterTree *tree_t, const ClusterTree *tree_s,
+                 Assembly &res) {
+    same = same || xs == nullptr;
     ClusterTree T, Sown;
-    build_cluster_tree(nt, xt, prm.minclustersize, T);
-    if (!same)
-        build_cluster_tree(ns, xs, prm.minclustersize, Sown);
+    if (tree_t)
+        T = *tree_t;
+    else
+        build_cluster_tree(nt, xt, prm.minclustersize, T);
+    if (!same) {
+        if (tree_s)
+            Sown = *tree_s;
+        else
+            build_cluster_tree(ns, xs, prm.minclustersize, Sown);
+    }
     const ClusterTree &S = same ? T : Sown;
     res.nr     = nt;
     res.nc     = same ? nt : ns;
     res.perm_t = T.perm;
     res.perm_s = S.perm;
-    res.same   = same;
-    res.kd     = kernel_desc(gen, same);
-    res.pT.upload(xt, T.perm);
+    if (prm.no_permutation) {
+        // set_use_permutation(false) (hmatrix.hpp:218, 306-311): matrix indices,
+        // points and product vectors are in the cluster ordering already
+        for (int i = 0; i < res.nr; i++)
+            res.perm_t[i] = i;
+        for (int j = 0; j < res.nc; j++)
+            res.perm_s[j] = j;
+    }
+    res.same = same;
+    if (hgen) {
+        res.kd      = KernelDesc{-1, same ? 1 : 0, 0.0, 0.0, 0.0}; // no device family: entries from the host
+        res.cplx    = hgen->is_complex != 0;
+        res.host_gen = true;
+    } else {
+        res.kd   = kernel_desc(*gen, same);
+        res.cplx = kernel_is_complex(res.kd.kind);
+    }
+    res.pT.upload(xt, res.perm_t);
     if (!same)
-        res.pS.upload(xs, S.perm);
+        res.pS.upload(xs, res.perm_s);
 
-    res.cplx        = kernel_is_complex(res.kd.kind);
-    res.delay_dense = prm.delay_dense != 0;
+    // dense leaves of a host generator are filled after the build (from the caller's generator)
+    res.delay_dense = prm.delay_dense != 0 || hgen != nullptr;
     if (res.cplx && prm.recompress_tol > 0)
         throw Error(HTG_ERR_UNSUPPORTED, "QR+SVD recompression is implemented for the real kernel families only");
     Assembler as{T, S, prm, res.kd, res.dT(), res.dS(), {}, {}, {}, 0, {}, 0, &res, res.cplx ? 2 : 1};
     const char uplo = sym_uplo(prm);
     as.sym          = uplo != 'N';
-    const std::vector<BlockTask> tasks = build_block_tasks(T, S, prm.eta, (size_t)prm.maxblocksize, uplo);
+    as.hg           = hgen;
+    as.prow         = res.perm_t.data();
+    as.pcol         = res.perm_s.data();
+    const std::vector<BlockTask> tasks = build_block_tasks(T, S, prm.eta, (size_t)prm.maxblocksize, uplo,
+                                                           prm.mintargetdepth, prm.minsourcedepth);
     std::vector<int> roots(tasks.size(), -1);
     for (size_t p = 0; p < tasks.size(); p++) {
         if (!tasks[p].admissible) {

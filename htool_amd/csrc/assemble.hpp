@@ -28,6 +28,7 @@ struct Assembly {
     bool same = true;
     bool cplx = false; // complex kernel family: the store holds complex<double> entries
     bool delay_dense = false; // dense leaves not generated (filled by the caller later)
+    bool host_gen = false;    // entries come from a host generator (no device kernel family)
     PointsOnDevice pT, pS;
     // ACA runs of admissible blocks (first attempts): features, terms needed,
     // first workspace terms; retries and workspace totals.
@@ -56,6 +57,11 @@ inline char sym_kind(const htg_params &p) {
 // HMatrix::build (reference: include/htool/types/hmatrix.hpp:285-341, 422-607).
 void assemble(const htg_params &prm, const htg_generator &gen, int nt, const double *xt, int ns, const double *xs,
               Assembly &out);
+// The same with either a device kernel family (gen) or a host generator
+// (hgen), and optionally the caller's cluster trees (NULL: built from the points).
+void assemble_ex(const htg_params &prm, const htg_generator *gen, const htg_host_generator *hgen, bool same, int nt,
+                 const double *xt, int ns, const double *xs, const ClusterTree *tree_t, const ClusterTree *tree_s,
+                 Assembly &out);
 
 // Recompresses the low-rank leaves in the store (FP64) and sets each leaf's
 // per-term scales; then assigns precision classes (Leaf::k_hi).

@@ -3,6 +3,7 @@
 #include "assemble.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -96,6 +97,10 @@ struct htg_hmatrix {
     double aca_seconds = 0;
     char symmetry = 'N', uplo = 'N'; // symmetric storage keeps one triangle; `leaves` adds mirrored copies
     std::vector<double> user_dense;  // dense leaves supplied by the caller (set_dense_blocks), get_leaves order
+    std::vector<std::array<int, 4>> user_keys; // (off_i, off_j, m, n) of each block of user_dense, in its order
+    // mvprod_subrhs: the H-matrix of the leaves whose columns meet a window,
+    // per (offset, size), made on first use
+    std::map<std::pair<int, int>, std::unique_ptr<htg_hmatrix>> subrhs_views;
     bool scaled = false; // some low-rank terms carry scales (recompressed)
 
     std::vector<int> rb_row0, rb_order;
@@ -126,6 +131,9 @@ struct htg_hmatrix {
     DevArray<float> d_xstage;     // 32-RHS gather staging, nc x 32
     int storage = HTG_STORAGE_FP64;
     bool cplx   = false; // complex<double> entries, x and y
+    // infos["nb_mat_vec_prod"], infos["total_time_mat_vec_prod"] (hmatrix.hpp:1180-1181)
+    int64_t n_products     = 0;
+    double product_seconds = 0;
     // Layout generation: changes whenever device buffers a captured product
     // reads are replaced (layout rebuilds, storage changes); a graph keeps the
     // value it was captured at, and the token is set to ~0 when the H-matrix
@@ -731,6 +739,16 @@ void apply_storage(htg_hmatrix &h, int storage) {
     h.storage = storage;
 }
 
+// (off_i, off_j, m, n) of the kept dense leaves in get_leaves order: the list
+// the caller's blocks of set_dense_blocks follow.
+std::vector<std::array<int, 4>> kept_dense_keys(const htg_hmatrix &h) {
+    std::vector<std::array<int, 4>> k;
+    for (const Leaf &L : h.leaves)
+        if (!L.mirror && L.k < 0)
+            k.push_back({L.oi, L.oj, L.m, L.n});
+    return k;
+}
+
 // Writes the caller's dense leaves (h.user_dense) into the FP64 master arena:
 // each kept dense leaf into its row-block tiles, and its transpose into the
 // tiles of its mirrored copy (symmetric storage).
@@ -741,13 +759,15 @@ void write_user_dense(htg_hmatrix &h) {
     ClassLayout &C = h.cls[0];
     std::vector<double> arena(C.a64.n);
     HTG_HIP(hipMemcpy(arena.data(), C.a64.p, arena.size() * sizeof(double), hipMemcpyDeviceToHost));
-    std::map<std::pair<int, int>, size_t> where; // (off_i, off_j) of a kept dense leaf -> data offset
-    size_t pos = 0;
-    for (const Leaf &L : h.leaves) {
-        if (L.mirror || L.k >= 0)
-            continue;
-        where[{L.oi, L.oj}] = pos;
-        pos += (size_t)L.m * L.n * e;
+    std::map<std::pair<int, int>, size_t> where; // (off_i, off_j) of a supplied dense block -> data offset
+    {
+        size_t pos = 0;
+        for (const auto &key : h.user_keys) {
+            where[{key[0], key[1]}] = pos;
+            pos += (size_t)key[2] * key[3] * e;
+        }
+        if (pos != h.user_dense.size())
+            throw Error(HTG_ERR_INVALID, "supplied dense blocks do not match their list");
     }
     for (size_t l = 0; l < h.leaves.size(); l++) {
         const Leaf &L = h.leaves[l];
@@ -841,6 +861,40 @@ void add_mirrors(std::vector<Leaf> &leaves, int first) {
         M.mirror = true;
         leaves.push_back(std::move(M));
     }
+}
+
+// The H-matrix of the leaves (kept and mirrored) of h whose columns meet
+// [offset, offset + size], sharing h's assemblies and supplied dense blocks:
+// what HMatrix::mvprod_subrhs applies (hmatrix.hpp:1457-1554).
+htg_hmatrix &subrhs_view(htg_hmatrix &h, int offset, int size) {
+    auto it = h.subrhs_views.find({offset, size});
+    if (it != h.subrhs_views.end())
+        return *it->second;
+    if (h.parts.empty())
+        throw Error(HTG_ERR_UNSUPPORTED, "mvprod_subrhs needs an H-matrix assembled by the library");
+    std::unique_ptr<htg_hmatrix> v(new htg_hmatrix);
+    v->nr = h.nr, v->nc = h.nc, v->cplx = h.cplx, v->lay = h.lay;
+    v->perm_t = h.perm_t, v->perm_s = h.perm_s;
+    v->symmetry = h.symmetry, v->uplo = h.uplo;
+    v->user_dense = h.user_dense;
+    v->user_keys  = h.user_keys;
+    for (const Leaf &L : h.leaves)
+        if (L.oj <= offset + size && offset <= L.oj + L.n)
+            v->leaves.push_back(L);
+    for (const Part &p : h.parts) {
+        Part q = p;
+        locate_part(*v, q, p.transposed ? p.A->nc : p.A->nr);
+        v->parts.push_back(std::move(q));
+    }
+    build_layout(*v);
+    upload_layout(*v);
+    alloc_master(*v);
+    for (const Part &part : v->parts)
+        fill_from_assembly(*v, part);
+    write_user_dense(*v);
+    htg_hmatrix &ref = *v;
+    h.subrhs_views[{offset, size}] = std::move(v);
+    return ref;
 }
 
 void take_aca_stats(htg_hmatrix &h, Assembly &A) {
@@ -1095,6 +1149,202 @@ unsigned char *noalloc_scratch() {
 } // namespace
 } // namespace htg
 
+namespace htg {
+namespace {
+
+// A caller's cluster tree (htg_cluster_tree) as the library's, checked: a
+// permutation of [0, n), the root over every point, sons consecutive and
+// tiling their parent's range, the rank-0 clusters of depth 1 covering all.
+ClusterTree tree_from(const htg_cluster_tree &c, int n, const char *which) {
+    const std::string w = which;
+    if (c.n != n || !c.perm || c.nnodes < 1 || !c.nodes_int || !c.nodes_dbl)
+        throw Error(HTG_ERR_INVALID, "[Htool error] " + w + " cluster tree does not match the points");
+    ClusterTree T;
+    T.perm.assign(c.perm, c.perm + n);
+    std::vector<char> seen(n, 0);
+    for (int p : T.perm) {
+        if (p < 0 || p >= n || seen[p])
+            throw Error(HTG_ERR_INVALID, "[Htool error] " + w + " cluster permutation is not a permutation");
+        seen[p] = 1;
+    }
+    T.nodes.resize(c.nnodes);
+    for (int i = 0; i < c.nnodes; i++) {
+        const int *v = c.nodes_int + 6 * i;
+        ClusterNode &d = T.nodes[i];
+        d.offset = v[0], d.size = v[1], d.depth = v[2], d.rank = v[3], d.nb_sons = v[4], d.first_son = v[5];
+        for (int k = 0; k < 3; k++)
+            d.ctr[k] = c.nodes_dbl[4 * i + k];
+        d.rad = c.nodes_dbl[4 * i + 3];
+    }
+    auto bad = [&](const std::string &m) { throw Error(HTG_ERR_INVALID, "[Htool error] " + w + " cluster tree: " + m); };
+    if (T.nodes[0].offset != 0 || T.nodes[0].size != n)
+        bad("the root does not cover every point");
+    long long rank0 = 0;
+    for (int i = 0; i < c.nnodes; i++) {
+        const ClusterNode &d = T.nodes[i];
+        if (d.size <= 0 || d.offset < 0 || d.offset + d.size > n || d.nb_sons < 0)
+            bad("node " + std::to_string(i) + " out of range");
+        if (d.depth == 1 && d.rank == 0)
+            rank0 += d.size;
+        if (d.nb_sons == 0)
+            continue;
+        if (d.first_son <= i || d.first_son + d.nb_sons > c.nnodes)
+            bad("node " + std::to_string(i) + " has sons outside the node list");
+        int o = d.offset;
+        for (int p = 0; p < d.nb_sons; p++) {
+            const ClusterNode &sn = T.nodes[d.first_son + p];
+            if (sn.offset != o || sn.depth != d.depth + 1)
+                bad("the sons of node " + std::to_string(i) + " do not tile it");
+            o += sn.size;
+        }
+        if (o != d.offset + d.size)
+            bad("the sons of node " + std::to_string(i) + " do not tile it");
+    }
+    if (rank0 != n)
+        bad("its rank-0 clusters do not cover every point (a tree of one process is needed)");
+    return T;
+}
+
+// The caller's dense leaves from the host generator, in get_leaves order.
+void host_dense_blocks(htg_hmatrix &h, const htg_host_generator &g) {
+    const int e = h.esz();
+    h.user_dense.clear();
+    for (const Leaf &L : h.leaves) {
+        if (L.mirror || L.k >= 0)
+            continue;
+        const size_t at = h.user_dense.size();
+        h.user_dense.resize(at + (size_t)L.m * L.n * e);
+        g.copy_submatrix(L.m, L.n, h.perm_t.data() + L.oi, h.perm_s.data() + L.oj, h.user_dense.data() + at, g.ctx);
+    }
+    h.user_keys = kept_dense_keys(h);
+}
+
+// Calls f(i, j, v) with v the e doubles of entry (i, j) (cluster ordering) of
+// every leaf, kept and mirrored, whose rows and columns meet `want(L)`, from
+// the FP64 master arenas; low-rank entries are formed from U, the scales and V.
+template <typename Want, typename F>
+void for_leaf_entries(const htg_hmatrix &h, Want want, F f) {
+    const int e = h.esz();
+    std::vector<double> arena[2], varena[2];
+    for (int c = 0; c < 2; c++) {
+        arena[c].resize(h.cls[c].a64.n);
+        varena[c].resize(h.cls[c].v64.n);
+        HTG_HIP(hipMemcpy(arena[c].data(), h.cls[c].a64.p, arena[c].size() * sizeof(double), hipMemcpyDeviceToHost));
+        HTG_HIP(hipMemcpy(varena[c].data(), h.cls[c].v64.p, varena[c].size() * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    }
+    double v[2];
+    for (size_t l = 0; l < h.leaves.size(); l++) {
+        const Leaf &L = h.leaves[l];
+        if (L.k == 0 || !want(L))
+            continue;
+        for (const auto &rt : h.leaf_tiles[l]) {
+            const int rb = std::get<0>(rt), R = row_block_rows(h, rb), i0 = h.rb_row0[rb] - L.oi;
+            for (int i = 0; i < R; i++) {
+                const int gi = L.oi + i0 + i;
+                for (int j = 0; j < L.n; j++) {
+                    if (!want.entry(gi, L.oj + j))
+                        continue;
+                    if (L.k < 0) {
+                        const double *src = arena[0].data() + e * (std::get<1>(rt) + (size_t)i * row_stride(h, 0, rb) + j);
+                        for (int u = 0; u < e; u++)
+                            v[u] = src[u];
+                    } else {
+                        v[0] = v[1] = 0;
+                        for (int r = 0; r < L.k; r++) {
+                            const int c = r < L.k_hi ? 0 : 1, rc = c == 0 ? r : r - L.k_hi;
+                            const double *us = arena[c].data() + e * ((c == 0 ? std::get<1>(rt) : std::get<2>(rt)) +
+                                                                      (size_t)i * row_stride(h, c, rb) + rc);
+                            const double sc  = L.sig.empty() ? 1.0 : L.sig[r];
+                            const double *vs = varena[c].data() + e * (h.cls[c].leaf_v_off[l] + (L.oj - v_first(L, c)) +
+                                                                       (size_t)rc * v_stride(L, c) + j);
+                            if (e == 1) {
+                                v[0] += us[0] * sc * vs[0];
+                            } else {
+                                const double ar = us[0] * sc, ai = us[1] * sc;
+                                v[0] += ar * vs[0] - ai * vs[1];
+                                v[1] += ar * vs[1] + ai * vs[0];
+                            }
+                        }
+                    }
+                    f(gi, L.oj + j, v);
+                }
+            }
+        }
+    }
+}
+
+struct AllEntries {
+    bool operator()(const Leaf &) const { return true; }
+    bool entry(int, int) const { return true; }
+};
+struct DiagonalEntries {
+    bool operator()(const Leaf &L) const { return L.oi < L.oj + L.n && L.oj < L.oi + L.m; }
+    bool entry(int i, int j) const { return i == j; }
+};
+
+// HMatrix::build(A, xt, xs) for htg_hmatrix_build and htg_hmatrix_build_ex.
+void build_one(const htg_params &prm, const htg_generator *gen, const htg_host_generator *hgen, int nt,
+               const double *xt, int ns, const double *xs, const htg_cluster_tree *tree_t,
+               const htg_cluster_tree *tree_s, htg_hmatrix **out) {
+    validate_params(prm);
+    if (gen)
+        check_hermitian(sym_kind(prm), *gen);
+    const bool same = (gen && gen->same_points) || xs == nullptr;
+    if (sym_uplo(prm) != 'N' && !same)
+        throw Error(HTG_ERR_INVALID, "[Htool error] symmetric storage needs a single point set");
+    if (same && ns != nt && ns > 0)
+        throw Error(HTG_ERR_INVALID, "[Htool error] Invalid size in arguments for building HMatrix");
+    if (!same && ns <= 0)
+        throw Error(HTG_ERR_INVALID, "[Htool error] Invalid size in arguments for building HMatrix");
+    if (prm.no_permutation && !tree_t)
+        throw Error(HTG_ERR_INVALID, "[Htool error] no_permutation needs the caller's cluster trees (their "
+                                     "ordering is the one of the points and vectors)");
+    ClusterTree Tt, Ts;
+    if (tree_t)
+        Tt = tree_from(*tree_t, nt, "target");
+    if (tree_s && !same)
+        Ts = tree_from(*tree_s, ns, "source");
+    if (prm.no_permutation && !same && !tree_s)
+        throw Error(HTG_ERR_INVALID, "[Htool error] no_permutation needs the source cluster tree too");
+    require_device();
+    const auto t0 = std::chrono::steady_clock::now();
+    std::unique_ptr<htg_hmatrix> h(new htg_hmatrix);
+    Part part;
+    part.A.reset(new Assembly);
+    Assembly &A = *part.A;
+    assemble_ex(prm, gen, hgen, same, nt, xt, ns, xs, tree_t ? &Tt : nullptr, tree_s && !same ? &Ts : nullptr, A);
+    h->nr       = A.nr;
+    h->nc       = A.nc;
+    h->cplx     = A.cplx;
+    h->perm_t   = A.perm_t;
+    h->perm_s   = A.perm_s;
+    h->leaves   = A.leaves;
+    h->uplo     = sym_uplo(prm);
+    h->symmetry = sym_kind(prm);
+    if (h->uplo != 'N')
+        add_mirrors(h->leaves, 0);
+    take_aca_stats(*h, A);
+    sort_leaves(h->leaves);
+    part.leaf_begin = 0;
+    part.leaf_end   = h->leaves.size();
+    h->parts.push_back(std::move(part));
+    pick_default_layout(*h);
+    build_layout(*h);
+    upload_layout(*h);
+    alloc_master(*h);
+    fill_from_assembly(*h, h->parts[0]);
+    if (hgen && !prm.delay_dense) {
+        host_dense_blocks(*h, *hgen);
+        write_user_dense(*h);
+    }
+    h->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    *out             = h.release();
+}
+
+} // namespace
+} // namespace htg
+
 extern "C" {
 
 const char *htg_last_error(void) { return htg::last_error().c_str(); }
@@ -1116,44 +1366,19 @@ int htg_hmatrix_build(const htg_params *prm, const htg_generator *gen, int nt, c
     return guard([&] {
         if (!prm || !gen || !xt || !out || nt <= 0)
             throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments to build HMatrix");
-        validate_params(*prm);
-        check_hermitian(sym_kind(*prm), *gen);
-        const bool same = gen->same_points || xs == nullptr;
-        if (sym_uplo(*prm) != 'N' && !same)
-            throw Error(HTG_ERR_INVALID, "[Htool error] symmetric storage needs a single point set");
-        if (same && ns != nt && ns > 0)
-            throw Error(HTG_ERR_INVALID, "[Htool error] Invalid size in arguments for building HMatrix");
-        if (!same && ns <= 0)
-            throw Error(HTG_ERR_INVALID, "[Htool error] Invalid size in arguments for building HMatrix");
-        require_device();
-        const auto t0 = std::chrono::steady_clock::now();
-        std::unique_ptr<htg_hmatrix> h(new htg_hmatrix);
-        Part part;
-        part.A.reset(new Assembly);
-        Assembly &A = *part.A;
-        assemble(*prm, *gen, nt, xt, ns, xs, A);
-        h->nr     = A.nr;
-        h->nc     = A.nc;
-        h->cplx   = A.cplx;
-        h->perm_t = A.perm_t;
-        h->perm_s = A.perm_s;
-        h->leaves = A.leaves;
-        h->uplo     = sym_uplo(*prm);
-        h->symmetry = sym_kind(*prm);
-        if (h->uplo != 'N')
-            add_mirrors(h->leaves, 0);
-        take_aca_stats(*h, A);
-        sort_leaves(h->leaves);
-        part.leaf_begin = 0;
-        part.leaf_end   = h->leaves.size();
-        h->parts.push_back(std::move(part));
-        pick_default_layout(*h);
-        build_layout(*h);
-        upload_layout(*h);
-        alloc_master(*h);
-        fill_from_assembly(*h, h->parts[0]);
-        h->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        *out             = h.release();
+        build_one(*prm, gen, nullptr, nt, xt, ns, xs, nullptr, nullptr, out);
+    });
+}
+
+int htg_hmatrix_build_ex(const htg_params *prm, const htg_generator *gen, const htg_host_generator *host_gen,
+                         int nt, const double *xt, int ns, const double *xs, const htg_cluster_tree *tree_t,
+                         const htg_cluster_tree *tree_s, htg_hmatrix **out) {
+    return guard([&] {
+        if (!prm || (!gen == !host_gen) || !xt || !out || nt <= 0)
+            throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments to build HMatrix (one generator, points)");
+        if (host_gen && !host_gen->copy_submatrix)
+            throw Error(HTG_ERR_INVALID, "[Htool error] host generator without copy_submatrix");
+        build_one(*prm, gen, host_gen, nt, xt, ns, xs, tree_t, tree_s, out);
     });
 }
 
@@ -1362,6 +1587,7 @@ int htg_hmatrix_transpose(const htg_hmatrix *h, htg_hmatrix **out) {
                         for (int u = 0; u < e; u++)
                             t->user_dense.push_back(d[e * ((size_t)j + (size_t)L.n * i) + u]);
             }
+            t->user_keys = kept_dense_keys(*t);
         }
         build_layout(*t);
         upload_layout(*t);
@@ -1547,6 +1773,8 @@ int htg_hmatrix_get_info(const htg_hmatrix *h, htg_info *info) {
         r.symmetry          = h->symmetry;
         r.uplo              = h->uplo;
         r.n_aca_int_wrap    = h->aca_int_wrap;
+        r.nb_mat_vec_prod   = h->n_products;
+        r.total_time_mat_vec_prod = h->product_seconds;
         *info               = r;
     });
 }
@@ -1662,6 +1890,8 @@ int htg_hmatrix_set_dense_blocks(htg_hmatrix *h, const double *data) {
             if (!L.mirror && L.k < 0)
                 n += (size_t)L.m * L.n * h->esz();
         h->user_dense.assign(data, data + n);
+        h->user_keys = kept_dense_keys(*h);
+        h->subrhs_views.clear();
         write_user_dense(*h);
         // storage copies are converted again from the new FP64 entries, in
         // place: captured graphs keep valid pointers and see the new blocks
@@ -1718,6 +1948,7 @@ int htg_hmatrix_mvprod_device(htg_hmatrix *h, const double *d_x, double *d_y, in
         if (!h || !d_x || !d_y)
             throw Error(HTG_ERR_INVALID, "null argument");
         run_product(*h, d_x, d_y, mu, static_cast<hipStream_t>(stream));
+        h->n_products++;
     });
 }
 
@@ -1725,6 +1956,7 @@ int htg_hmatrix_mvprod(htg_hmatrix *h, const double *x, double *y, int mu) {
     return guard([&] {
         if (!h || !x || !y || mu <= 0)
             throw Error(HTG_ERR_INVALID, "null argument");
+        const auto t0 = std::chrono::steady_clock::now();
         const size_t nx = (size_t)h->nc * mu * h->esz(), ny = (size_t)h->nr * mu * h->esz();
         if (h->d_hx.n < nx)
             h->d_hx.alloc(nx);
@@ -1733,6 +1965,111 @@ int htg_hmatrix_mvprod(htg_hmatrix *h, const double *x, double *y, int mu) {
         HTG_HIP(hipMemcpy(h->d_hx.p, x, nx * sizeof(double), hipMemcpyHostToDevice));
         run_product(*h, h->d_hx.p, h->d_hy.p, mu, nullptr);
         HTG_HIP(hipMemcpy(y, h->d_hy.p, ny * sizeof(double), hipMemcpyDeviceToHost));
+        h->n_products++;
+        h->product_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+int htg_hmatrix_mymvprod_local_to_local_device(htg_hmatrix *h, const double *d_in, double *d_out, int mu,
+                                               double *d_work, void *stream) {
+    return guard([&] {
+        if (!h || !d_in || !d_out || mu <= 0)
+            throw Error(HTG_ERR_INVALID, "null argument");
+        hipStream_t st  = static_cast<hipStream_t>(stream);
+        const int e     = h->esz();
+        const size_t nx = (size_t)h->nc * mu * e, ny = (size_t)h->nr * mu * e;
+        DevArray<double> own; // the reference allocates `work` when the caller passes none (hmatrix.hpp:981-985)
+        if (!d_work) {
+            own.alloc(nx + ny);
+            d_work = own.p;
+        }
+        launch_to_original(d_in, h->d_perm_s.p, h->nc, mu, e, d_work, st);
+        run_product(*h, d_work, d_work + nx, mu, st);
+        launch_to_cluster(d_work + nx, h->d_perm_t.p, h->nr, mu, e, d_out, st);
+        HTG_HIP(hipGetLastError());
+        if (own.p)
+            HTG_HIP(hipStreamSynchronize(st)); // before the per-call workspace is freed
+        h->n_products++;
+    });
+}
+
+int htg_hmatrix_mymvprod_local_to_local(htg_hmatrix *h, const double *in, double *out, int mu) {
+    return guard([&] {
+        if (!h || !in || !out || mu <= 0)
+            throw Error(HTG_ERR_INVALID, "null argument");
+        const auto t0   = std::chrono::steady_clock::now();
+        const int e     = h->esz();
+        const size_t nx = (size_t)h->nc * mu * e, ny = (size_t)h->nr * mu * e;
+        if (h->d_hx.n < 2 * nx)
+            h->d_hx.alloc(2 * nx);
+        if (h->d_hy.n < 2 * ny)
+            h->d_hy.alloc(2 * ny);
+        HTG_HIP(hipMemcpy(h->d_hx.p + nx, in, nx * sizeof(double), hipMemcpyHostToDevice));
+        launch_to_original(h->d_hx.p + nx, h->d_perm_s.p, h->nc, mu, e, h->d_hx.p, nullptr);
+        run_product(*h, h->d_hx.p, h->d_hy.p, mu, nullptr);
+        launch_to_cluster(h->d_hy.p, h->d_perm_t.p, h->nr, mu, e, h->d_hy.p + ny, nullptr);
+        HTG_HIP(hipGetLastError());
+        HTG_HIP(hipMemcpy(out, h->d_hy.p + ny, ny * sizeof(double), hipMemcpyDeviceToHost));
+        h->n_products++;
+        h->product_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+int htg_hmatrix_mvprod_subrhs(htg_hmatrix *h, const double *in, double *out, int mu, int offset, int size,
+                              int margin) {
+    return guard([&] {
+        if (!h || !in || !out || mu <= 0 || size < 0 || margin < 0 || offset < 0 || offset > h->nc)
+            throw Error(HTG_ERR_INVALID, "invalid mvprod_subrhs arguments");
+        // in holds the cluster-ordered entries [offset - margin, offset + size + margin)
+        // (interleaved); every leaf whose columns meet [offset, offset + size] reads its
+        // columns from there (hmatrix.hpp:1457-1554), the others are not applied: the
+        // product of the H-matrix of those leaves (made once per window) with in
+        // placed in a vector of zeros.
+        htg_hmatrix &v = subrhs_view(*h, offset, size);
+        apply_storage(v, h->storage);
+        const int e = h->esz();
+        std::vector<double> x((size_t)h->nc * mu * e, 0.0);
+        const int lo = std::max(0, offset - margin), hi = std::min(h->nc, offset + size + margin);
+        for (int j = lo; j < hi; j++)
+            std::copy(in + (size_t)(j - offset + margin) * mu * e, in + (size_t)(j - offset + margin + 1) * mu * e,
+                      x.data() + (size_t)j * mu * e);
+        const int st = htg_hmatrix_mymvprod_local_to_local(&v, x.data(), out, mu);
+        if (st != HTG_OK)
+            throw Error(st, last_error());
+        h->n_products++;
+        h->product_seconds += v.product_seconds;
+        v.product_seconds = 0;
+    });
+}
+
+int htg_hmatrix_get_dense(const htg_hmatrix *h, double *out, int original_order) {
+    return guard([&] {
+        if (!h || !out)
+            throw Error(HTG_ERR_INVALID, "null argument");
+        const int e = h->esz();
+        std::fill(out, out + (size_t)h->nr * h->nc * e, 0.0);
+        // get_local_dense (cluster ordering) or copy_local_dense_perm (original ordering)
+        for_leaf_entries(*h, AllEntries{}, [&](int i, int j, const double *v) {
+            const size_t at = original_order ? (size_t)h->perm_t[i] + (size_t)h->nr * h->perm_s[j]
+                                             : (size_t)i + (size_t)h->nr * j;
+            for (int u = 0; u < e; u++)
+                out[e * at + u] = v[u];
+        });
+    });
+}
+
+int htg_hmatrix_get_diagonal(const htg_hmatrix *h, double *diag, int original_order) {
+    return guard([&] {
+        if (!h || !diag)
+            throw Error(HTG_ERR_INVALID, "null argument");
+        if (h->nr != h->nc)
+            throw Error(HTG_ERR_INVALID, "[Htool error] the diagonal needs a square matrix");
+        const int e = h->esz();
+        std::fill(diag, diag + (size_t)h->nr * e, 0.0);
+        for_leaf_entries(*h, DiagonalEntries{}, [&](int i, int, const double *v) {
+            for (int u = 0; u < e; u++)
+                diag[e * (size_t)(original_order ? h->perm_t[i] : i) + u] = v[u];
+        });
     });
 }
 

@@ -519,12 +519,15 @@ __global__ void __launch_bounds__(WG) k_acaw_init(const AcaBlock *blocks, AcaWid
 }
 
 // Residual row (side 2, kRow) or scaled residual column (side 1) of the current
-// cross term over one chunk, with the chunk's partial results at `slot`.
+// cross term over one chunk, with the chunk's partial results at `slot`. The
+// generator entries come from the device kernel family, or, with gv, from
+// values the host generator wrote for this step: entry e of the block's
+// vector at gv[gv_off[block] + e].
 template <typename E, bool kRow>
 __global__ void __launch_bounds__(WG) k_acaw_vec(KernelDesc kd, DevPoints T, DevPoints S, const AcaBlock *blocks,
                                                  const AcaWide *states, const AcawTask *tasks, int slot0, E *ws,
                                                  const unsigned char *vis_ws, const uint64_t *vis_off, double *pv,
-                                                 int *pi, double *dd, E *xs) {
+                                                 int *pi, double *dd, E *xs, const E *gv, const uint64_t *gv_off) {
     __shared__ E s_coef[CAPMAX];
     __shared__ E s_u[ACAW_CHUNK];
     __shared__ double s_v[NWAVE];
@@ -556,11 +559,13 @@ __global__ void __launch_bounds__(WG) k_acaw_vec(KernelDesc kd, DevPoints T, Dev
         g = E{st.gre, st.gim};
     double best = -1.0, dsq = 0;
     int bi = -1;
+    const E *hv = gv ? gv + gv_off[tk.block] : nullptr;
     for (int e = e0 + tid; e < e1; e += WG) {
-        E val = kRow ? (swap ? entry_s<E>(kd, T, B.off_t + e, S, B.off_s + st.I1)
-                             : entry_s<E>(kd, T, B.off_t + st.I1, S, B.off_s + e))
-                     : (swap ? entry_s<E>(kd, T, B.off_t + st.I2, S, B.off_s + e)
-                             : entry_s<E>(kd, T, B.off_t + e, S, B.off_s + st.I2));
+        E val = hv ? hv[e]
+                   : kRow ? (swap ? entry_s<E>(kd, T, B.off_t + e, S, B.off_s + st.I1)
+                                  : entry_s<E>(kd, T, B.off_t + st.I1, S, B.off_s + e))
+                          : (swap ? entry_s<E>(kd, T, B.off_t + st.I2, S, B.off_s + e)
+                                  : entry_s<E>(kd, T, B.off_t + e, S, B.off_s + st.I2));
         for (int j = 0; j < p; j++)
             val = axpy_s(s_coef[j], prev[(size_t)j * len + e], val);
         if (!kRow)
@@ -3376,13 +3381,50 @@ static void acaw_term(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBl
                       int reqrank, double eps, E *ws, unsigned char *vis_ws, const uint64_t *vis_off, double *pv,
                       int *pi, double *dd, E *xs, hipStream_t s) {
     hipLaunchKernelGGL((k_acaw_vec<E, true>), dim3(ntasks2), dim3(WG), 0, s, k, T, S, blocks, st, tasks2, 0, ws,
-                       vis_ws, vis_off, pv, pi, dd, xs);
+                       vis_ws, vis_off, pv, pi, dd, xs, (const E *)nullptr, (const uint64_t *)nullptr);
     hipLaunchKernelGGL(k_acaw_piv<E>, dim3((nblocks + WG - 1) / WG), dim3(WG), 0, s, blocks, st, nblocks, ws, vis_ws,
                        vis_off, pv, pi);
     hipLaunchKernelGGL((k_acaw_vec<E, false>), dim3(ntasks1), dim3(WG), 0, s, k, T, S, blocks, st, tasks1, ntasks2, ws,
-                       vis_ws, vis_off, pv, pi, dd, xs);
+                       vis_ws, vis_off, pv, pi, dd, xs, (const E *)nullptr, (const uint64_t *)nullptr);
     hipLaunchKernelGGL(k_acaw_fin<E>, dim3(nblocks), dim3(WG), 0, s, blocks, st, reqrank, eps, vis_ws, vis_off, pv, pi,
                        dd, xs);
+}
+
+// The two halves of a cross term with generator values from the host (a user
+// generator): the residual row and pivot column from gv, then the scaled
+// residual column, next pivot row and stopping test from gv.
+template <typename E>
+static void acaw_half(bool row, DevPoints T, DevPoints S, const AcaBlock *blocks, AcaWide *st, int nblocks,
+                      const AcawTask *tasks, int ntasks, int slot0, int reqrank, double eps, E *ws,
+                      unsigned char *vis_ws, const uint64_t *vis_off, double *pv, int *pi, double *dd, E *xs,
+                      const E *gv, const uint64_t *gv_off, hipStream_t s) {
+    const KernelDesc k{};
+    if (row) {
+        hipLaunchKernelGGL((k_acaw_vec<E, true>), dim3(ntasks), dim3(WG), 0, s, k, T, S, blocks, st, tasks, slot0, ws,
+                           vis_ws, vis_off, pv, pi, dd, xs, gv, gv_off);
+        hipLaunchKernelGGL(k_acaw_piv<E>, dim3((nblocks + WG - 1) / WG), dim3(WG), 0, s, blocks, st, nblocks, ws,
+                           vis_ws, vis_off, pv, pi);
+    } else {
+        hipLaunchKernelGGL((k_acaw_vec<E, false>), dim3(ntasks), dim3(WG), 0, s, k, T, S, blocks, st, tasks, slot0,
+                           ws, vis_ws, vis_off, pv, pi, dd, xs, gv, gv_off);
+        hipLaunchKernelGGL(k_acaw_fin<E>, dim3(nblocks), dim3(WG), 0, s, blocks, st, reqrank, eps, vis_ws, vis_off, pv,
+                           pi, dd, xs);
+    }
+}
+
+void launch_acaw_half(bool row, DevPoints T, DevPoints S, const AcaBlock *blocks, AcaWide *st, int nblocks,
+                      const AcawTask *tasks, int ntasks, int slot0, int reqrank, double eps, void *ws, bool cplx,
+                      unsigned char *vis_ws, const uint64_t *vis_off, double *pv, int *pi, double *dd, void *xs,
+                      const void *gv, const uint64_t *gv_off, hipStream_t s) {
+    if (nblocks <= 0)
+        return;
+    if (cplx)
+        acaw_half<cdbl>(row, T, S, blocks, st, nblocks, tasks, ntasks, slot0, reqrank, eps, static_cast<cdbl *>(ws),
+                        vis_ws, vis_off, pv, pi, dd, static_cast<cdbl *>(xs), static_cast<const cdbl *>(gv), gv_off, s);
+    else
+        acaw_half<double>(row, T, S, blocks, st, nblocks, tasks, ntasks, slot0, reqrank, eps,
+                          static_cast<double *>(ws), vis_ws, vis_off, pv, pi, dd, static_cast<double *>(xs),
+                          static_cast<const double *>(gv), gv_off, s);
 }
 
 void launch_acaw_term(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBlock *blocks, AcaWide *st,
@@ -3431,6 +3473,39 @@ void launch_recompress(const RcLeaf *leaves, int nleaves, double *store, double 
                        double *sigma, hipStream_t st) {
     if (nleaves > 0)
         hipLaunchKernelGGL(k_recompress, dim3(nleaves), dim3(WG), 0, st, leaves, store, tol, rank, ok, sigma);
+}
+
+// Layout changes between the cluster ordering with interleaved right-hand sides
+// (entry (j, s) at j * mu + s, the layout of mymvprod_local_to_local,
+// hmatrix.hpp:1117-1131) and the original ordering, column-major (entry
+// (perm[j], s) at perm[j] + n * s); e doubles per entry.
+__global__ void k_to_original(const double *in, const int *perm, int n, int mu, int e, double *out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * mu)
+        return;
+    const int j = (int)(t / mu), s = (int)(t % mu);
+    for (int u = 0; u < e; u++)
+        out[e * ((size_t)perm[j] + (size_t)n * s) + u] = in[e * t + u];
+}
+__global__ void k_to_cluster(const double *in, const int *perm, int n, int mu, int e, double *out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * mu)
+        return;
+    const int j = (int)(t / mu), s = (int)(t % mu);
+    for (int u = 0; u < e; u++)
+        out[e * t + u] = in[e * ((size_t)perm[j] + (size_t)n * s) + u];
+}
+void launch_to_original(const double *in, const int *perm, int n, int mu, int e, double *out, hipStream_t st) {
+    const size_t tot = (size_t)n * mu;
+    if (tot)
+        hipLaunchKernelGGL(k_to_original, dim3((unsigned)((tot + WG - 1) / WG)), dim3(WG), 0, st, in, perm, n, mu, e,
+                           out);
+}
+void launch_to_cluster(const double *in, const int *perm, int n, int mu, int e, double *out, hipStream_t st) {
+    const size_t tot = (size_t)n * mu;
+    if (tot)
+        hipLaunchKernelGGL(k_to_cluster, dim3((unsigned)((tot + WG - 1) / WG)), dim3(WG), 0, st, in, perm, n, mu, e,
+                           out);
 }
 
 template <typename TS>

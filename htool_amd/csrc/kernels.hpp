@@ -167,6 +167,9 @@ void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, 
 // FP64 master copy -> storage type (round to nearest).
 template <typename TS>
 void launch_convert(const double *src, TS *dst, uint64_t n, hipStream_t st);
+// Cluster-ordered, RHS-interleaved (j * mu + s) <-> original, column-major (perm[j] + n * s).
+void launch_to_original(const double *in, const int *perm, int n, int mu, int e, double *out, hipStream_t st);
+void launch_to_cluster(const double *in, const int *perm, int n, int mu, int e, double *out, hipStream_t st);
 
 // Multi-workgroup partial ACA of very large blocks: per-block state between
 // the launches of one cross term (cluster-order row/column indices local to the
@@ -194,6 +197,13 @@ void launch_acaw_term(const KernelDesc &k, DevPoints T, DevPoints S, const AcaBl
                       int nblocks, const AcawTask *tasks1, int ntasks1, const AcawTask *tasks2, int ntasks2,
                       int reqrank, double eps, void *ws, bool cplx, unsigned char *vis_ws, const uint64_t *vis_off,
                       double *pv, int *pi, double *dd, void *xs, hipStream_t s);
+// Half a cross term (row: residual row + pivot column; else residual column +
+// next pivot row and stopping test) with the generator values of every active
+// block supplied by the host in gv (block k's vector at gv_off[k]).
+void launch_acaw_half(bool row, DevPoints T, DevPoints S, const AcaBlock *blocks, AcaWide *st, int nblocks,
+                      const AcawTask *tasks, int ntasks, int slot0, int reqrank, double eps, void *ws, bool cplx,
+                      unsigned char *vis_ws, const uint64_t *vis_off, double *pv, int *pi, double *dd, void *xs,
+                      const void *gv, const uint64_t *gv_off, hipStream_t s);
 
 // Complex kernel families.
 void launch_dense_fill_c(const KernelDesc &k, DevPoints T, DevPoints S, const FillTile *tiles, int ntiles, cdbl *arena,

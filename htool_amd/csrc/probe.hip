@@ -1,6 +1,6 @@
 // Bandwidth probe of the device (no counterpart in the reference): a
-// streaming read and a streaming copy of a large buffer with 16-byte loads per
-// lane, four in flight per thread, timed with HIP events. It gives the
+// streaming read (non-temporal 16-byte loads) and a streaming copy of a large
+// buffer, four 16-byte words in flight per thread, timed with HIP events. It gives the
 // achievable HBM rate the product kernels are compared with, beside the
 // vendor's 8 TB/s, and a copy of known size for calibrating the FETCH_SIZE /
 // WRITE_SIZE counters.
@@ -12,15 +12,20 @@ namespace {
 
 constexpr int PWG = 256;
 
-// Reads n16 16-byte words; the XOR of every word decides one store that a
-// real run never makes (acc equal to the magic value), so the loads stay.
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
+// Reads n16 16-byte words (non-temporal loads: the fastest form measured on
+// MI355X); the XOR of every word decides one store that a real run never
+// makes (acc equal to the magic value), so the loads stay.
 __global__ void __launch_bounds__(PWG) k_stream_read(const uint4 *__restrict__ src, size_t n16,
                                                      unsigned *__restrict__ sink, unsigned magic) {
     const size_t stride = (size_t)gridDim.x * PWG;
     size_t i            = (size_t)blockIdx.x * PWG + threadIdx.x;
     unsigned acc        = 0;
+    const u4v *s        = reinterpret_cast<const u4v *>(src);
     for (; i + 3 * stride < n16; i += 4 * stride) {
-        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        const u4v a = __builtin_nontemporal_load(s + i), b = __builtin_nontemporal_load(s + i + stride);
+        const u4v c = __builtin_nontemporal_load(s + i + 2 * stride), d = __builtin_nontemporal_load(s + i + 3 * stride);
         acc ^= a.x ^ a.y ^ a.z ^ a.w ^ b.x ^ b.y ^ b.z ^ b.w ^ c.x ^ c.y ^ c.z ^ c.w ^ d.x ^ d.y ^ d.z ^ d.w;
     }
     for (; i < n16; i += stride) {
@@ -73,8 +78,8 @@ extern "C" int htg_stream_probe(uint64_t bytes, int reps, double *read_gbps, dou
         src.alloc(n16);
         dst.alloc(n16);
         src.zero();
-        const int grids[] = {1024, 2048, 4096, 8192, 16384};
-        sink.alloc(16384);
+        const int grids[] = {4096, 8192, 16384, 32768, 65536};
+        sink.alloc(65536);
         Events ev;
         double best_r = 0, best_c = 0;
         int bg = 0;

@@ -200,13 +200,15 @@ struct TaskBuilder {
     double eta;
     std::size_t maxblocksize;
     std::vector<BlockTask> tasks;
+    int mintd = 0, minsd = 0;
+    bool deep(const ClusterNode &a, const ClusterNode &b) const { return a.depth >= mintd && b.depth >= minsd; }
 
     // Returns true when (t, s) was not turned into tasks (it may be merged into
     // a larger dense block by its parent).
     bool visit(int t, int s) {
         const ClusterNode &a = T.nodes[t], &b = S.nodes[s];
         const bool adm       = admissible(a, b, eta);
-        if (adm && a.rank >= 0) {
+        if (adm && a.rank >= 0 && deep(a, b)) {
             tasks.push_back({t, s, true});
             return false;
         }
@@ -227,7 +229,7 @@ struct TaskBuilder {
             all_not_pushed = all_not_pushed && np[p];
         }
         const std::size_t bsize = std::size_t(a.size) * std::size_t(b.size);
-        if (bsize <= maxblocksize && all_not_pushed && a.rank >= 0)
+        if (bsize <= maxblocksize && all_not_pushed && a.rank >= 0 && deep(a, b))
             return true;
         for (int p = 0; p < owner.nb_sons; p++) {
             if (!np[p])
@@ -248,11 +250,13 @@ struct SymTaskBuilder {
     double eta;
     std::size_t maxblocksize;
     std::vector<BlockTask> tasks;
+    int mintd = 0, minsd = 0;
+    bool deep(const ClusterNode &a, const ClusterNode &b) const { return a.depth >= mintd && b.depth >= minsd; }
 
     bool visit(int t, int s) {
         const ClusterNode &a = T.nodes[t], &b = S.nodes[s];
         const bool ok        = sym_compatible(a, b);
-        if (admissible(a, b, eta) && a.rank >= 0 && ok) {
+        if (admissible(a, b, eta) && a.rank >= 0 && deep(a, b) && ok) {
             tasks.push_back({t, s, true});
             return false;
         }
@@ -277,7 +281,7 @@ struct SymTaskBuilder {
             all_not_pushed = all_not_pushed && np[k];
         }
         const std::size_t bsize = std::size_t(a.size) * std::size_t(b.size);
-        if (bsize <= maxblocksize && all_not_pushed && a.rank >= 0 && ok)
+        if (bsize <= maxblocksize && all_not_pushed && a.rank >= 0 && deep(a, b) && ok)
             return true;
         for (size_t k = 0; k < kids.size(); k++)
             if (np[k])
@@ -296,17 +300,17 @@ bool sym_compatible(const ClusterNode &t, const ClusterNode &s) {
 }
 
 std::vector<BlockTask> build_block_tasks(const ClusterTree &T, const ClusterTree &S, double eta,
-                                         std::size_t maxblocksize, char uplo) {
+                                         std::size_t maxblocksize, char uplo, int mintargetdepth, int minsourcedepth) {
     if (maxblocksize == 0)
         throw std::invalid_argument("maxblocksize cannot be zero");
     std::vector<BlockTask> all;
     if (uplo == 'L' || uplo == 'U') {
-        SymTaskBuilder tb{T, S, eta, maxblocksize, {}};
+        SymTaskBuilder tb{T, S, eta, maxblocksize, {}, mintargetdepth, minsourcedepth};
         if (tb.visit(0, 0))
             tb.tasks.push_back({0, 0, admissible(T.nodes[0], S.nodes[0], eta)});
         all.swap(tb.tasks);
     } else {
-        TaskBuilder tb{T, S, eta, maxblocksize, {}};
+        TaskBuilder tb{T, S, eta, maxblocksize, {}, mintargetdepth, minsourcedepth};
         if (tb.visit(0, 0))
             tb.tasks.push_back({0, 0, admissible(T.nodes[0], S.nodes[0], eta)});
         all.swap(tb.tasks);

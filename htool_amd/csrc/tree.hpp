@@ -45,8 +45,11 @@ struct BlockTask {
 // not overlap, and both sides split at once) and keeps the tasks of one
 // triangle (s.offset <= t.offset for 'L', >= for 'U'); any other value the
 // ordinary traversal.
+// mintargetdepth / minsourcedepth: a block is a task or a merged dense block
+// only when its clusters are at least that deep (blocks.hpp:51, 68, 154, 169).
 std::vector<BlockTask> build_block_tasks(const ClusterTree &T, const ClusterTree &S, double eta,
-                                         std::size_t maxblocksize, char uplo = 'N');
+                                         std::size_t maxblocksize, char uplo = 'N', int mintargetdepth = 0,
+                                         int minsourcedepth = 0);
 
 // Both clusters are the same, or their index ranges are disjoint: the blocks
 // a symmetric block tree may keep (blocks.hpp:152, 170).

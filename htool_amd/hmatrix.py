@@ -66,6 +66,9 @@ class HMatrix:
         self.aca_wide_min = 0
         self.delay_dense = False
         self.ref_int_overflow = False
+        self.mintargetdepth = 0
+        self.minsourcedepth = 0
+        self.use_permutation = True
         self._h = None
         self._storage = "fp64"
 
@@ -83,6 +86,25 @@ class HMatrix:
 
     def set_compression(self, name: str):
         self.compressor = _COMPRESSORS[name]
+
+    def set_mintargetdepth(self, d: int):
+        """hmatrix.hpp:216 (blocks.hpp:51): blocks only from target clusters at least d deep."""
+        self.mintargetdepth = int(d)
+
+    def set_minsourcedepth(self, d: int):
+        """hmatrix.hpp:215: blocks only from source clusters at least d deep."""
+        self.minsourcedepth = int(d)
+
+    def get_mintargetdepth(self) -> int:
+        return self.mintargetdepth
+
+    def get_minsourcedepth(self) -> int:
+        return self.minsourcedepth
+
+    def set_use_permutation(self, choice: bool):
+        """hmatrix.hpp:218, 306-311: False means the generator, the points and the product
+        vectors are in the cluster ordering of the trees passed to `build`."""
+        self.use_permutation = bool(choice)
 
     # build-time options of the mixed-precision product (no reference counterpart)
     def set_recompression(self, tol: float):
@@ -147,18 +169,42 @@ class HMatrix:
         self.aca_wide_min = int(m)
 
     # -- assembly ---------------------------------------------------------------
-    def build(self, gen: Generator, xt: np.ndarray, xs: np.ndarray | None = None) -> "HMatrix":
-        """HMatrix::build(A, xt, xs) (include/htool/types/hmatrix.hpp:285-341)."""
+    def build(self, gen, xt: np.ndarray, xs: np.ndarray | None = None, cluster_t=None, cluster_s=None) -> "HMatrix":
+        """HMatrix::build(A, xt, xs) (include/htool/types/hmatrix.hpp:285-341).
+
+        gen: a Generator (a device kernel family), or any object with
+        copy_submatrix(M, N, rows, cols, ptr) filling the (M, N) array ptr with
+        A[rows[i], cols[j]] (a VirtualGenerator, virtual_generator.hpp:22; attribute
+        is_complex for complex entries), evaluated on the host and fed to the GPU
+        ACA row by row and column by column. cluster_t / cluster_s: the caller's
+        cluster trees (io.ClusterTree, e.g. from io.read_cluster), else built from
+        the points."""
         L = capi.lib()
         xt = np.ascontiguousarray(xt, dtype=np.float64)
         same = xs is None
         if not same:
             xs = np.ascontiguousarray(xs, dtype=np.float64)
         prm = self._params()
-        g = gen._c(same)
         h = C.c_void_p()
-        capi.check(L.htg_hmatrix_build(C.byref(prm), C.byref(g), xt.shape[0], capi.ptr(xt),
-                                       xt.shape[0] if same else xs.shape[0], capi.ptr(xs), C.byref(h)))
+        host = not isinstance(gen, Generator)
+        if not host and cluster_t is None and cluster_s is None:
+            g = gen._c(same)
+            capi.check(L.htg_hmatrix_build(C.byref(prm), C.byref(g), xt.shape[0], capi.ptr(xt),
+                                           xt.shape[0] if same else xs.shape[0], capi.ptr(xs), C.byref(h)))
+        else:
+            keep = []
+            tt = _tree_c(cluster_t, keep)
+            ts = _tree_c(cluster_s, keep)
+            g = None if host else gen._c(same)
+            hg, errors = (_host_gen_c(gen, keep) if host else (None, []))
+            capi.check(L.htg_hmatrix_build_ex(C.byref(prm), None if g is None else C.byref(g),
+                                              None if hg is None else C.byref(hg), xt.shape[0], capi.ptr(xt),
+                                              xt.shape[0] if same else xs.shape[0], capi.ptr(xs),
+                                              None if tt is None else C.byref(tt),
+                                              None if ts is None else C.byref(ts), C.byref(h)))
+            if errors:
+                capi.lib().htg_hmatrix_destroy(h)
+                raise errors[0]
         self._close()
         self._h = h
         self._storage = "fp64"
@@ -169,7 +215,8 @@ class HMatrix:
                                self.reqrank, int(self.lowprec_all), self.recompress_tol, self.lowprec_tol,
                                self.predictor.handle if self.predictor is not None else None,
                                self.predictor_margin, self.aca_wide_min, ord(self.symmetry), ord(self.UPLO),
-                               int(self.delay_dense), int(self.ref_int_overflow))
+                               int(self.delay_dense), self.mintargetdepth, self.minsourcedepth,
+                               int(not self.use_permutation), int(self.ref_int_overflow))
 
     def build_batch(self, gens, point_sets) -> "HMatrix":
         """Many independent square H-matrices as one block-diagonal H-matrix
@@ -301,6 +348,66 @@ class HMatrix:
         """operator* (include/htool/types/hmatrix.hpp:1596-1602)."""
         return self.mvprod_global_to_global(x)
 
+    def mvprod_local_to_local(self, x: np.ndarray, mu: int = 1) -> np.ndarray:
+        """hmatrix.hpp:1184-1274: with one process the local vectors are the global
+        ones (original ordering), so this is mvprod_global_to_global."""
+        return self.mvprod_global_to_global(x, mu)
+
+    def mymvprod_local_to_local(self, x: np.ndarray, mu: int = 1) -> np.ndarray:
+        """hmatrix.hpp:979-996: x (nc * mu) in the cluster ordering with interleaved
+        right-hand sides (entry (j, s) at j * mu + s); returns y (nr * mu) likewise."""
+        info = self.get_info()
+        dt = np.complex128 if info.is_complex else np.float64
+        x = np.ascontiguousarray(np.asarray(x, dtype=dt).reshape(-1))
+        if x.size != info.nc * mu:
+            raise ValueError("x must hold nc * mu entries")
+        y = np.zeros(info.nr * mu, dtype=dt)
+        capi.check(capi.lib().htg_hmatrix_mymvprod_local_to_local(self._handle(), capi.ptr(x), capi.ptr(y), mu))
+        return y
+
+    def mymvprod_local_to_local_device(self, d_in: int, d_out: int, mu: int, d_work: int = 0,
+                                       stream: int = 0) -> None:
+        """The same on device pointers (cluster ordering, interleaved), enqueued on
+        `stream`; d_work: (nr + nc) * mu entries of device workspace, or 0."""
+        capi.check(capi.lib().htg_hmatrix_mymvprod_local_to_local_device(
+            self._handle(), C.c_void_p(d_in), C.c_void_p(d_out), mu, C.c_void_p(d_work or None),
+            C.c_void_p(stream)))
+
+    def mvprod_subrhs(self, x: np.ndarray, mu: int, offset: int, size: int, margin: int) -> np.ndarray:
+        """hmatrix.hpp:1457-1554: x holds the cluster-ordered source entries
+        [offset - margin, offset + size + margin) (interleaved); returns nr * mu."""
+        info = self.get_info()
+        dt = np.complex128 if info.is_complex else np.float64
+        x = np.ascontiguousarray(np.asarray(x, dtype=dt).reshape(-1))
+        if x.size < (size + 2 * margin) * mu:
+            raise ValueError("x must hold (size + 2 margin) * mu entries")
+        y = np.zeros(info.nr * mu, dtype=dt)
+        capi.check(capi.lib().htg_hmatrix_mvprod_subrhs(self._handle(), capi.ptr(x), capi.ptr(y), mu, offset, size,
+                                                        margin))
+        return y
+
+    def _dense(self, original: bool) -> np.ndarray:
+        info = self.get_info()
+        dt = np.complex128 if info.is_complex else np.float64
+        A = np.zeros((info.nr, info.nc), dtype=dt, order="F")
+        capi.check(capi.lib().htg_hmatrix_get_dense(self._handle(), capi.ptr(A), int(original)))
+        return A
+
+    def get_local_dense(self) -> np.ndarray:
+        """hmatrix.hpp:1728-1757: the dense matrix in the cluster ordering."""
+        return self._dense(False)
+
+    def get_local_dense_perm(self) -> np.ndarray:
+        """copy_local_dense_perm: the dense matrix in the original ordering."""
+        return self._dense(True)
+
+    def get_local_diagonal(self, cluster_ordering: bool = True) -> np.ndarray:
+        """get_local_diagonal(permutation) of virtual_hmatrix.hpp: the diagonal entries."""
+        info = self.get_info()
+        d = np.zeros(info.nr, dtype=np.complex128 if info.is_complex else np.float64)
+        capi.check(capi.lib().htg_hmatrix_get_diagonal(self._handle(), capi.ptr(d), int(not cluster_ordering)))
+        return d
+
     def mvprod_device(self, d_x: int, d_y: int, mu: int, stream: int = 0) -> None:
         """Same product on device pointers (e.g. torch tensors' data_ptr()), column-major
         nc x mu / nr x mu, enqueued on `stream`."""
@@ -325,7 +432,8 @@ class HMatrix:
         return {"Number_of_lrmat": i.n_lowrank, "Number_of_dmat": i.n_dense,
                 "Number_of_false_positive": i.n_false_positive, "Rank_min": i.rank_min, "Rank_max": i.rank_max,
                 "Rank_mean": i.rank_mean, "Compression_ratio": i.compression_ratio,
-                "Target_size": i.nr, "Source_size": i.nc, "Blocks_max": i.build_seconds}
+                "Target_size": i.nr, "Source_size": i.nc, "Blocks_max": i.build_seconds,
+                "nb_mat_vec_prod": i.nb_mat_vec_prod, "total_time_mat_vec_prod": i.total_time_mat_vec_prod}
 
     def nb_rows(self) -> int:
         return self.get_info().nr
@@ -380,6 +488,44 @@ class HMatrix:
     def flops(self, mu: int = 1) -> int:
         i = self.get_info()
         return int((8 if i.is_complex else 2) * mu * (i.dense_entries + i.lowrank_entries + i.mirrored_entries))
+
+
+def _tree_c(tree, keep):
+    """An io.ClusterTree as the C ABI's htg_cluster_tree (arrays kept alive in `keep`)."""
+    if tree is None:
+        return None
+    perm = np.ascontiguousarray(tree.perm, dtype=np.int32)
+    ni = np.ascontiguousarray(np.stack([tree.offset, tree.size, tree.depth, tree.rank, tree.nb_sons,
+                                        tree.first_son], axis=1), dtype=np.int32)
+    nd = np.ascontiguousarray(np.concatenate([np.asarray(tree.ctr, np.float64).reshape(-1, 3),
+                                              np.asarray(tree.rad, np.float64).reshape(-1, 1)], axis=1))
+    keep.extend([perm, ni, nd])
+    return capi.htg_cluster_tree(perm.shape[0], perm.ctypes.data, ni.shape[0], ni.ctypes.data, nd.ctypes.data)
+
+
+def _host_gen_c(gen, keep):
+    """A Python generator (copy_submatrix(M, N, rows, cols, ptr) on an (M, N) array)
+    behind the C ABI's callback; exceptions are collected and raised after the build."""
+    cplx = bool(getattr(gen, "is_complex", False))
+    dt = np.complex128 if cplx else np.float64
+    errors = []
+
+    def cb(M, N, rows, cols, ptr, ctx):
+        if errors:
+            return
+        try:
+            r = np.ctypeslib.as_array(rows, shape=(M,)).copy()
+            c = np.ctypeslib.as_array(cols, shape=(N,)).copy()
+            out = np.zeros((M, N), dtype=dt, order="F")
+            gen.copy_submatrix(M, N, r, c, out)
+            flat = np.ctypeslib.as_array(ptr, shape=(M * N * (2 if cplx else 1),))
+            flat[:] = np.asfortranarray(out, dtype=dt).reshape(-1, order="F").view(np.float64)
+        except Exception as e:  # noqa: BLE001 - re-raised by the caller after the build
+            errors.append(e)
+
+    fn = capi.COPY_SUBMATRIX_FN(cb)
+    keep.append(fn)
+    return capi.htg_host_generator(fn, None, int(cplx)), errors
 
 
 class ProductGraph:

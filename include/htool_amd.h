@@ -85,6 +85,36 @@ typedef struct htg_generator {
     double wavenumber; /* HTG_KERNEL_HELMHOLTZ */
 } htg_generator;
 
+/* A user generator evaluated on the host, for htg_hmatrix_build_ex: the C form
+ * of VirtualGenerator<T>::copy_submatrix(M, N, rows, cols, ptr)
+ * (include/htool/types/virtual_generator.hpp:22). It writes ptr[i + M*j] =
+ * A(rows[i], cols[j]) (original indices, or cluster positions with
+ * htg_params.no_permutation); complex entries are (re, im) pairs. The library
+ * calls it from the building thread only, one call at a time: for every dense
+ * leaf (m x n) and, per ACA cross term, one row (1 x n) or column (m x 1) of
+ * each block being compressed. */
+typedef void (*htg_copy_submatrix_fn)(int M, int N, const int *rows, const int *cols, double *ptr, void *ctx);
+typedef struct htg_host_generator {
+    htg_copy_submatrix_fn copy_submatrix;
+    void *ctx;      /* passed back to every call */
+    int is_complex; /* 1: complex<double> entries */
+} htg_host_generator;
+
+/* A cluster tree given by the caller (the layout htg_cluster_build writes:
+ * Cluster::save_cluster's tree, cluster.hpp:433-484, or read_cluster's,
+ * 486-583): n points, perm[cluster position] = original index, nnodes nodes
+ * with nodes_int 6 per node (offset, size, depth, rank, nb_sons, first_son or
+ * -1; node 0 the root, the sons of a node consecutive and covering its range)
+ * and nodes_dbl 4 per node (centre x, y, z, radius). The tree of a single
+ * process: the rank-0 clusters cover every point. */
+typedef struct htg_cluster_tree {
+    int n;
+    const int *perm;
+    int nnodes;
+    const int *nodes_int;
+    const double *nodes_dbl;
+} htg_cluster_tree;
+
 /* Parameters of HMatrix<T>(t, s, epsilon, eta, 'N', 'N', reqrank) plus the
  * setters the hot path reads (include/htool/types/hmatrix.hpp:115, 213-220)
  * and Cluster::set_minclustersize (include/htool/clustering/cluster.hpp:334). */
@@ -119,6 +149,14 @@ typedef struct htg_params {
     int uplo;                       /* 'L' or 'U' with 'S' / 'H' (the kept triangle); 'N' (or 0) otherwise */
     int delay_dense;                /* 1: dense leaves left zero at build, filled later by
                                        htg_hmatrix_set_dense_blocks (set_delay_dense_computation) */
+    int mintargetdepth, minsourcedepth; /* HMatrix::set_mintargetdepth / set_minsourcedepth
+                                       (hmatrix.hpp:215-216; blocks.hpp:51): a block is a task or a merged
+                                       dense block only when its target / source cluster is at least this
+                                       deep (default 0) */
+    int no_permutation;             /* 1: set_use_permutation(false) (hmatrix.hpp:218, 306-311): the
+                                       generator's indices, the points and the product vectors are all in
+                                       the cluster ordering (use with the caller's cluster trees, see
+                                       htg_hmatrix_build_ex); 0 (default): original ordering */
     int ref_int_overflow;           /* 0: the ACA's "next rank not advantageous" exit q (M + N) > M N
                                        (partialACA.hpp:74, sympartialACA.hpp:98) in exact 64-bit
                                        arithmetic; 1: in 32-bit int as the reference evaluates it,
@@ -152,6 +190,11 @@ typedef struct htg_info {
                                  off-diagonal leaves that the product also streams (the counts
                                  above are of the kept leaves, as the reference's) */
     int64_t mirrored_lowprec_entries; /* the part of mirrored_entries in class 1 */
+    int64_t nb_mat_vec_prod;  /* products made through this handle, any entry point (the reference's
+                                 infos["nb_mat_vec_prod"], hmatrix.hpp:1180) */
+    double total_time_mat_vec_prod; /* wall seconds of the host-pointer products among them
+                                 (infos["total_time_mat_vec_prod"], hmatrix.hpp:1181); device-pointer
+                                 products are enqueued and return at once, adding no time */
     int n_aca_int_wrap;       /* ACA runs of blocks with M N > 2^31 - 1 on which the reference's 32-bit
                                  q (M + N) > M N test and the 64-bit one differ: with ref_int_overflow 0
                                  these blocks are compressed (the reference splits them), with 1 they
@@ -187,6 +230,20 @@ const char *htg_status_string(int status);
  * builds on one device from several host threads run one at a time. */
 int htg_hmatrix_build(const htg_params *params, const htg_generator *gen, int nt, const double *xt, int ns,
                       const double *xs, htg_hmatrix **out);
+
+/* HMatrix(t, s, ...) + build(A, xt, xs) in full: the generator is a built-in
+ * device family (gen) or a user generator on the host (host_gen; exactly one of
+ * the two, the other NULL), and the cluster trees are the caller's (tree_t,
+ * tree_s; NULL: built from the points as htg_hmatrix_build does; tree_s is
+ * ignored when the point sets coincide). With a host generator every ACA runs
+ * the multi-workgroup kernels fed with the generator's rows and columns, so
+ * ranks, pivots and factors are those of a device family that computes the
+ * same entries, and its dense leaves are filled through it after the ACA (not
+ * with delay_dense, which leaves them to htg_hmatrix_set_dense_blocks). Points
+ * are needed for the first ACA pivots (partialACA.hpp:42-50). */
+int htg_hmatrix_build_ex(const htg_params *params, const htg_generator *gen, const htg_host_generator *host_gen,
+                         int nt, const double *xt, int ns, const double *xs, const htg_cluster_tree *tree_t,
+                         const htg_cluster_tree *tree_s, htg_hmatrix **out);
 
 /* Builds the product structures from leaves produced elsewhere (dense
  * SubMatrix blocks and LowRankMatrix U, V factors; the layout of the header
@@ -277,6 +334,37 @@ int htg_hmatrix_mvprod(htg_hmatrix *h, const double *x, double *y, int mu);
  * for the default stream); returns without synchronising. */
 int htg_hmatrix_mvprod_device(htg_hmatrix *h, const double *d_x, double *d_y, int mu, void *stream);
 
+/* y = H x in the cluster ordering with right-hand sides interleaved (entry
+ * (j, s) at j * mu + s, as the reference's mymvprod_local_to_local takes and
+ * returns them, hmatrix.hpp:979-996, layout 1117-1131; with one process local
+ * = global), the product HPDDM's GMV calls (wrappers/wrapper_hpddm.hpp:41-60).
+ * Device pointers, enqueued on `stream`; d_work: the caller's device workspace
+ * of (nr + nc) * mu entries, or NULL for one allocated for the call (the
+ * reference allocates `work` when given none; the call then synchronises the
+ * stream before freeing it). */
+int htg_hmatrix_mymvprod_local_to_local_device(htg_hmatrix *h, const double *d_in, double *d_out, int mu,
+                                               double *d_work, void *stream);
+/* The same on host pointers (copies over PCIe; synchronous). */
+int htg_hmatrix_mymvprod_local_to_local(htg_hmatrix *h, const double *in, double *out, int mu);
+
+/* HMatrix::mvprod_subrhs(in, out, mu, offset, size, margin)
+ * (hmatrix.hpp:1457-1554): in holds the cluster-ordered source entries
+ * [offset - margin, offset + size + margin) with interleaved right-hand sides;
+ * every leaf whose columns meet [offset, offset + size] is applied, reading its
+ * columns from in; out (nr * mu, cluster ordering, interleaved) is overwritten.
+ * Host pointers. */
+int htg_hmatrix_mvprod_subrhs(htg_hmatrix *h, const double *in, double *out, int mu, int offset, int size,
+                              int margin);
+
+/* The dense matrix of the leaves (nr x nc column-major): in the cluster
+ * ordering (HMatrix::get_local_dense, hmatrix.hpp:1728-1757) or, with
+ * original_order, in the original one (copy_local_dense_perm). Symmetric and
+ * Hermitian storage: both triangles. Host memory of nr * nc entries. */
+int htg_hmatrix_get_dense(const htg_hmatrix *h, double *out, int original_order);
+/* Diagonal entries of a square H-matrix (get_local_diagonal / copy_local_diagonal):
+ * cluster ordering, or original with original_order. */
+int htg_hmatrix_get_diagonal(const htg_hmatrix *h, double *diag, int original_order);
+
 /* Times each product kernel with HIP events over `reps` products of device
  * buffers on `stream`; ms[0..3] = average ms of (gather, V x, row apply, whole product). */
 int htg_hmatrix_profile(htg_hmatrix *h, const double *d_x, double *d_y, int mu, int reps, void *stream, double *ms);
@@ -288,7 +376,7 @@ int htg_hmatrix_profile(htg_hmatrix *h, const double *d_x, double *d_y, int mu, 
 int htg_release_workspace(void);
 
 /* Device bandwidth probe (no counterpart in the reference): the best rate over
- * a few grid sizes of a streaming read (16-byte loads, four in flight per lane)
+ * a few grid sizes of a streaming read (non-temporal 16-byte loads, four in flight per lane)
  * and of a streaming copy of `bytes` (>= 1 MiB), each averaged over `reps`
  * launches after a warm-up; GB/s = bytes read (copy: read + written) / time.
  * best_grid (may be NULL): workgroups of the fastest read. */

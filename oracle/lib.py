@@ -42,6 +42,7 @@ def lib():
             "orc_dense_rows_c": (None, [_I, _D, _D, _I, _VP, _VP, _I, _I, _VP, _VP, _I, _VP]),
             "orc_mlp_predict": (None, [_VP, _VP, _VP, _VP, _I, _VP]),
             "orc_set_ref_int_overflow": (None, [_I]),
+            "orc_set_min_depths": (None, [_I, _I]),
             "orc_leaf_mvprod_herm": (None, [_I, _VP, _I, _VP, _VP, _VP, _VP, _I, _I]),
             "orc_aca_size_bound": (C.c_longlong, [_I, _I]),
         }
@@ -71,6 +72,22 @@ class ref_int_overflow:
 
     def __exit__(self, *exc):
         lib().orc_set_ref_int_overflow(0)
+        return False
+
+
+class min_depths:
+    """Context manager: block trees of builds inside it use set_mintargetdepth /
+    set_minsourcedepth (hmatrix.hpp:215-216)."""
+
+    def __init__(self, mintargetdepth=0, minsourcedepth=0):
+        self.d = (int(mintargetdepth), int(minsourcedepth))
+
+    def __enter__(self):
+        lib().orc_set_min_depths(*self.d)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_min_depths(0, 0)
         return False
 
 

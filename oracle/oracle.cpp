@@ -31,6 +31,9 @@ namespace {
 // that way (two's complement, the reference's behaviour on x86-64), 0 in
 // exact 64-bit arithmetic (the library's default).
 int g_int_wrap = 0;
+// HMatrix::set_mintargetdepth / set_minsourcedepth (hmatrix.hpp:215-216) for
+// the block trees of every later orc_hmat_build (blocks.hpp:51, 68, 154, 169).
+int g_mintd = 0, g_minsd = 0;
 long long aca_mn(int M, int N) {
     const long long mn = (long long)M * N;
     if (!g_int_wrap)
@@ -481,7 +484,7 @@ struct Builder {
     bool block_tree(int t, int s, std::vector<std::pair<int, int>> &tasks) const {
         const Node &a = Tn[t], &b = Sn[s];
         size_t bsize  = size_t(a.size) * size_t(b.size);
-        if (admissible(t, s) && a.rank >= 0) {
+        if (admissible(t, s) && a.rank >= 0 && a.depth >= g_mintd && b.depth >= g_minsd) {
             tasks.emplace_back(t, s);
             return false;
         }
@@ -499,7 +502,8 @@ struct Builder {
         std::vector<bool> not_pushed(sons.size());
         for (size_t p = 0; p < sons.size(); p++)
             not_pushed[p] = split_t ? block_tree(sons[p], s, tasks) : block_tree(t, sons[p], tasks);
-        if (bsize <= maxblocksize && std::all_of(not_pushed.begin(), not_pushed.end(), [](bool v) { return v; }) && a.rank >= 0)
+        if (bsize <= maxblocksize && std::all_of(not_pushed.begin(), not_pushed.end(), [](bool v) { return v; }) && a.rank >= 0 &&
+            a.depth >= g_mintd && b.depth >= g_minsd)
             return true;
         for (size_t p = 0; p < sons.size(); p++)
             if (not_pushed[p])
@@ -539,7 +543,7 @@ struct Builder {
     bool sym_block_tree(int t, int s, std::vector<std::pair<int, int>> &tasks) const {
         const Node &a = Tn[t], &b = Sn[s];
         size_t bsize  = size_t(a.size) * size_t(b.size);
-        if (admissible(t, s) && a.rank >= 0 && sym_ok(t, s)) {
+        if (admissible(t, s) && a.rank >= 0 && a.depth >= g_mintd && b.depth >= g_minsd && sym_ok(t, s)) {
             tasks.emplace_back(t, s);
             return false;
         }
@@ -550,7 +554,7 @@ struct Builder {
         for (size_t p = 0; p < kids.size(); p++)
             not_pushed[p] = sym_block_tree(kids[p].first, kids[p].second, tasks);
         if (bsize <= maxblocksize && std::all_of(not_pushed.begin(), not_pushed.end(), [](bool v) { return v; }) &&
-            a.rank >= 0 && sym_ok(t, s))
+            a.rank >= 0 && a.depth >= g_mintd && b.depth >= g_minsd && sym_ok(t, s))
             return true;
         for (size_t p = 0; p < kids.size(); p++)
             if (not_pushed[p])
@@ -814,6 +818,10 @@ void leaf_mvprod(int nr, int nc, const int *perm_t, const int *perm_s, int nleav
 extern "C" {
 
 void orc_set_ref_int_overflow(int on) { g_int_wrap = on != 0; }
+void orc_set_min_depths(int mintargetdepth, int minsourcedepth) {
+    g_mintd = mintargetdepth;
+    g_minsd = minsourcedepth;
+}
 long long orc_aca_size_bound(int M, int N) { return aca_mn(M, N); }
 
 void orc_kernel_block(int kernel, double delta, double diag, int same_points, const double *xt, const double *xs,

@@ -36,6 +36,8 @@ typedef struct orc_hmat orc_hmat;
  * exact 64-bit; 1 the reference's int arithmetic, where M N > 2^31 - 1 wraps
  * modulo 2^32. orc_aca_size_bound returns the right side in use. */
 void orc_set_ref_int_overflow(int on);
+/* mintargetdepth / minsourcedepth of the block trees of later builds (default 0). */
+void orc_set_min_depths(int mintargetdepth, int minsourcedepth);
 long long orc_aca_size_bound(int M, int N);
 
 /* One kernel entry family: 0 = 1/(4 pi r) (diag on i==j when same points),

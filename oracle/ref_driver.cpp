@@ -20,6 +20,8 @@
 //   int32  mu, nreps, dump_data, reqrank, uplo (0: general; 'L' / 'U': HMatrix(..., 'S', uplo)
 //          built with the single-point-set build(A, xt), hmatrix.hpp:343-400; 'H' << 8 added:
 //          HMatrix(..., 'H', uplo))
+//   (version 2 adds) int32 mintargetdepth, minsourcedepth (HMatrix::set_mintargetdepth /
+//          set_minsourcedepth, hmatrix.hpp:215-216)
 //   double eta, eps, delta, diag, maxblocksize
 //   double xt[3 nt]; double xs[3 ns] (absent when same_points); double x[ns mu] (column-major)
 // Complex kernel families (kernel 4: e^{ikr}/(4 pi r) with k = `delta`, `diag`
@@ -148,8 +150,11 @@ template <typename T>
 void run_hmatrix(std::shared_ptr<Cluster<PCAGeometricClustering>> t, std::shared_ptr<Cluster<PCAGeometricClustering>> s,
                  VirtualGenerator<T> &A, std::shared_ptr<VirtualLowRankGenerator<T>> comp, const double *xt,
                  const double *pxs, std::vector<T> &x, int nt, int mu, int nreps, int dump, int reqrank, double eps,
-                 double eta, double maxblocksize, const std::string &out, std::FILE *info, char uplo, char symmetry) {
+                 double eta, double maxblocksize, const std::string &out, std::FILE *info, char uplo, char symmetry,
+                 int mintd, int minsd) {
     HMatrix<T> H(t, s, eps, eta, uplo == 'N' ? 'N' : symmetry, uplo, reqrank);
+    H.set_mintargetdepth(mintd);
+    H.set_minsourcedepth(minsd);
     H.set_compression(comp);
     H.set_maxblocksize(static_cast<unsigned int>(maxblocksize));
     double t0 = now();
@@ -216,11 +221,13 @@ int main(int argc, char **argv) {
     }
     std::ifstream in(argv[1], std::ios::binary);
     std::string out = argv[2];
-    int hdr[14];
+    int hdr[16] = {};
     double prm[5];
-    in.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
+    in.read(reinterpret_cast<char *>(hdr), 14 * sizeof(int));
+    if (in && hdr[1] == 2)
+        in.read(reinterpret_cast<char *>(hdr + 14), 2 * sizeof(int));
     in.read(reinterpret_cast<char *>(prm), sizeof(prm));
-    if (!in || hdr[0] != 0x48545246 || hdr[1] != 1) {
+    if (!in || hdr[0] != 0x48545246 || (hdr[1] != 1 && hdr[1] != 2)) {
         std::fprintf(stderr, "bad case file\n");
         return 2;
     }
@@ -230,6 +237,7 @@ int main(int argc, char **argv) {
     const char uplo = (up0 == 'L' || up0 == 'U') ? (char)up0 : 'N';
     const char symmetry = (hdr[13] >> 8) == 'H' ? 'H' : 'S';
     const bool cplx = kernel == 4 || kernel == 5 || kernel == 7;
+    const int mintd = hdr[14], minsd = hdr[15];
     const double eta = prm[0], eps = prm[1], delta = prm[2], diag = prm[3], maxblocksize = prm[4];
     std::vector<double> xt(3 * size_t(nt)), xs, x(size_t(ns) * mu * (cplx ? 2 : 1));
     in.read(reinterpret_cast<char *>(xt.data()), xt.size() * sizeof(double));
@@ -305,10 +313,10 @@ int main(int argc, char **argv) {
                                std::make_shared<partialACA<std::complex<double>>>())
                          : std::make_shared<sympartialACA<std::complex<double>>>();
         run_hmatrix<std::complex<double>>(t, s, Ause, compc, xt.data(), pxs, xc, nt, mu, nreps, dump, reqrank, eps, eta,
-                                          maxblocksize, out, info, uplo, symmetry);
+                                          maxblocksize, out, info, uplo, symmetry, mintd, minsd);
     } else {
         run_hmatrix<double>(t, s, A, comp, xt.data(), pxs, x, nt, mu, nreps, dump, reqrank, eps, eta, maxblocksize,
-                            out, info, uplo, symmetry);
+                            out, info, uplo, symmetry, mintd, minsd);
     }
     MPI_Finalize();
     return 0;

@@ -50,7 +50,8 @@ class RefResult:
 
 def run(xt, xs=None, x=None, *, kernel=0, eta=10.0, eps=1e-6, minclustersize=10, compressor=0,
         delta=0.0, diag=0.0, maxblocksize=1e6, mu=1, nreps=0, dump_data=True, reqrank=-1, mode=0,
-        threads=None, timeout=600, wavenumber=0.0, uplo="N", symmetry="S") -> RefResult:
+        threads=None, timeout=600, wavenumber=0.0, uplo="N", symmetry="S", mintargetdepth=0,
+        minsourcedepth=0) -> RefResult:
     """kernel 4 (Helmholtz, wavenumber travels in the case file's delta) and 5
     run the reference's HMatrix<std::complex<double>>. uplo 'L' / 'U': the
     reference's symmetric storage HMatrix(..., 'S', uplo) (one point set)."""
@@ -68,9 +69,10 @@ def run(xt, xs=None, x=None, *, kernel=0, eta=10.0, eps=1e-6, minclustersize=10,
     x = np.asarray(x, dtype=np.complex128 if cplx else np.float64).reshape(ns, mu)
     if uplo != "N" and not same:
         raise ValueError("symmetric storage needs one point set")
-    hdr = struct.pack("<14i", 0x48545246, 1, mode, nt, ns, int(same), kernel, minclustersize, compressor,
+    hdr = struct.pack("<16i", 0x48545246, 2, mode, nt, ns, int(same), kernel, minclustersize, compressor,
                       mu, nreps, int(dump_data), reqrank,
-                      0 if uplo == "N" else ord(uplo) | (ord("H") << 8 if symmetry == "H" else 0))
+                      0 if uplo == "N" else ord(uplo) | (ord("H") << 8 if symmetry == "H" else 0),
+                      int(mintargetdepth), int(minsourcedepth))
     prm = struct.pack("<5d", eta, eps, delta, diag, maxblocksize)
     with tempfile.TemporaryDirectory() as td:
         case = os.path.join(td, "case.bin")

@@ -413,3 +413,27 @@ def test_aca_size_test_int_wrap_reference_driver(orc, ref):
             assert k32 == k64 > 0
         else:
             assert k32 == ref_rank and k64 > 0
+
+
+def test_min_depths_match_reference_driver(orc, ref):
+    """HMatrix::set_mintargetdepth / set_minsourcedepth (hmatrix.hpp:215-216,
+    blocks.hpp:51, 68): the reference's leaf list and product for depths (7, 2),
+    (2, 7) and (6, 6), in the general and the symmetric block tree; the depths
+    change the trees of this case (more, smaller blocks)."""
+    n = 3000
+    xt = geo.sphere(n, seed=12)
+    diag = geo.laplace_diag(geo.sphere_spacing(n))
+    x = np.random.default_rng(4).standard_normal((n, 1))
+    counts = {}
+    for uplo in ("N", "L"):
+        for mt, ms in ((0, 0), (7, 2), (2, 7), (6, 6)):
+            r = ref.run(xt, None, x=x, kernel=geo.KERNEL_LAPLACE, diag=diag, minclustersize=10, eta=10.0, eps=1e-6,
+                        compressor=1, uplo=uplo, mintargetdepth=mt, minsourcedepth=ms)
+            with orc.min_depths(mt, ms):
+                O = orc.hmat_build(xt, kernel=geo.KERNEL_LAPLACE, diag=diag, minclustersize=10, eta=10.0, eps=1e-6,
+                                   sym=True, uplo=uplo)
+            order = np.lexsort((r.leaves[:, 2], r.leaves[:, 0]))
+            assert np.array_equal(O.leaves, r.leaves[order]), (uplo, mt, ms)
+            assert rel(O.mvprod(x), r.y) < 1e-13
+            counts[uplo, mt, ms] = len(O.leaves)
+        assert counts[uplo, 6, 6] > counts[uplo, 0, 0] and counts[uplo, 7, 2] > counts[uplo, 6, 6]
