@@ -116,6 +116,7 @@ _SIGNATURES = {
     "htg_hmatrix_mvprod_subrhs": (_I, [_VP, _VP, _VP, _I, _I, _I, _I]),
     "htg_hmatrix_get_dense": (_I, [_VP, _VP, _I]),
     "htg_hmatrix_get_diagonal": (_I, [_VP, _VP, _I]),
+    "htg_generator_block": (_I, [C.POINTER(htg_generator), _I, _VP, _I, _VP, _I, _I, _VP, _VP, _VP]),
     "htg_stream_probe": (_I, [C.c_uint64, _I, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I)]),
     "htg_stream_copy": (_I, [_VP, _VP, C.c_uint64]),
     "htg_hmatrix_aca_records": (_I, [_VP, C.POINTER(_I), _VP, _VP, _VP]),

@@ -2139,4 +2139,50 @@ int htg_aca_batch_noalloc(const htg_generator *gen, int compressor, double epsil
     });
 }
 
+int htg_generator_block(const htg_generator *gen, int nt, const double *xt, int ns, const double *xs, int M, int N,
+                        const int *rows, const int *cols, double *out) {
+    return guard([&] {
+        if (!gen || !xt || nt <= 0 || M < 0 || N < 0 || ((M > 0 && N > 0) && (!rows || !cols || !out)))
+            throw Error(HTG_ERR_INVALID, "[Htool error] Invalid arguments to copy_submatrix");
+        if (M == 0 || N == 0)
+            return;
+        const bool same = gen->same_points || xs == nullptr;
+        const double *xsrc = same ? xt : xs;
+        const int nsrc     = same ? nt : ns;
+        std::vector<int> r(rows, rows + M), c(cols, cols + N);
+        for (int v : r)
+            if (v < 0 || v >= nt)
+                throw Error(HTG_ERR_INVALID, "row index out of range");
+        for (int v : c)
+            if (v < 0 || v >= nsrc)
+                throw Error(HTG_ERR_INVALID, "column index out of range");
+        require_device();
+        const KernelDesc kd = kernel_desc(*gen, same);
+        const bool cplx     = kernel_is_complex(kd.kind);
+        const int e         = cplx ? 2 : 1;
+        PointsOnDevice pT, pS;
+        pT.upload(xt, r); // point i of the tile is xt[rows[i]], its perm entry rows[i] (the diagonal rule)
+        pS.upload(xsrc, c);
+        std::vector<FillTile> tiles;
+        for (int i0 = 0; i0 < M; i0 += 64)
+            tiles.push_back(FillTile{(uint64_t)i0 * N, i0, std::min(64, M - i0), 0, N, N, 0});
+        DevArray<FillTile> d_tiles;
+        DevArray<double> d_out;
+        d_tiles.upload(tiles);
+        d_out.alloc((size_t)M * N * e);
+        if (cplx)
+            launch_dense_fill_c(kd, pT.view(), pS.view(), d_tiles.p, (int)tiles.size(), reinterpret_cast<cdbl *>(d_out.p),
+                                nullptr);
+        else
+            launch_dense_fill(kd, pT.view(), pS.view(), d_tiles.p, (int)tiles.size(), d_out.p, nullptr);
+        HTG_HIP(hipGetLastError());
+        std::vector<double> h((size_t)M * N * e);
+        HTG_HIP(hipMemcpy(h.data(), d_out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < M; i++) // row-major tile -> column-major ptr[i + M j]
+            for (int j = 0; j < N; j++)
+                for (int u = 0; u < e; u++)
+                    out[e * ((size_t)i + (size_t)M * j) + u] = h[e * ((size_t)i * N + j) + u];
+    });
+}
+
 } // extern "C"

@@ -27,6 +27,29 @@ class Generator:
         return capi.htg_generator(self.kernel, 1 if same else 0, float(self.delta), float(self.diag),
                                   float(self.wavenumber))
 
+    @property
+    def is_complex(self) -> bool:
+        return self.kernel in _geo.COMPLEX_KERNELS
+
+    def copy_submatrix(self, xt, xs, rows, cols) -> np.ndarray:
+        """VirtualGenerator::copy_submatrix (virtual_generator.hpp:22): the (M, N)
+        block A[rows, cols] of this family on the points xt (rows) and xs (columns;
+        None: the same points), computed on the device."""
+        xt = np.ascontiguousarray(xt, dtype=np.float64)
+        same = xs is None
+        xs_ = None if same else np.ascontiguousarray(xs, dtype=np.float64)
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        M, N = rows.shape[0], cols.shape[0]
+        dt = np.complex128 if self.is_complex else np.float64
+        out = np.zeros((M, N), dtype=dt, order="F")
+        g = self._c(same)
+        capi.check(capi.lib().htg_generator_block(C.byref(g), xt.shape[0], capi.ptr(xt),
+                                                  xt.shape[0] if same else xs_.shape[0], capi.ptr(xs_), M, N,
+                                                  capi.ptr(rows), capi.ptr(cols), capi.ptr(out.reshape(-1, order="F")
+                                                                                        .view(np.float64))))
+        return out
+
 
 _COMPRESSORS = {"partialACA": capi.PARTIAL_ACA, "sympartialACA": capi.SYM_PARTIAL_ACA}
 _STORAGE = {"fp64": capi.STORAGE_FP64, "fp32": capi.STORAGE_FP32, "fp32_fp16": capi.STORAGE_FP32_FP16,
@@ -401,11 +424,12 @@ class HMatrix:
         """copy_local_dense_perm: the dense matrix in the original ordering."""
         return self._dense(True)
 
-    def get_local_diagonal(self, cluster_ordering: bool = True) -> np.ndarray:
-        """get_local_diagonal(permutation) of virtual_hmatrix.hpp: the diagonal entries."""
+    def get_local_diagonal(self, permutation: bool = True) -> np.ndarray:
+        """get_local_diagonal(permutation) (hmatrix.hpp:1842-1874): the diagonal entries
+        in the original ordering (permutation, the default) or the cluster one."""
         info = self.get_info()
         d = np.zeros(info.nr, dtype=np.complex128 if info.is_complex else np.float64)
-        capi.check(capi.lib().htg_hmatrix_get_diagonal(self._handle(), capi.ptr(d), int(not cluster_ordering)))
+        capi.check(capi.lib().htg_hmatrix_get_diagonal(self._handle(), capi.ptr(d), int(bool(permutation))))
         return d
 
     def mvprod_device(self, d_x: int, d_y: int, mu: int, stream: int = 0) -> None:

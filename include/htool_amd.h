@@ -245,6 +245,16 @@ int htg_hmatrix_build_ex(const htg_params *params, const htg_generator *gen, con
                          int nt, const double *xt, int ns, const double *xs, const htg_cluster_tree *tree_t,
                          const htg_cluster_tree *tree_s, htg_hmatrix **out);
 
+/* VirtualGenerator::copy_submatrix(M, N, rows, cols, ptr) of a built-in family
+ * (virtual_generator.hpp:22), its entries computed on the device:
+ * ptr[i + M*j] = A(rows[i], cols[j]) (original indices into xt / xs; xs NULL
+ * or gen->same_points: one point set, where the diagonal rule applies);
+ * complex families write (re, im) pairs. For API compatibility (a generator
+ * handed to code that calls copy_submatrix); builds evaluate the family inside
+ * their own kernels. */
+int htg_generator_block(const htg_generator *gen, int nt, const double *xt, int ns, const double *xs, int M, int N,
+                        const int *rows, const int *cols, double *out);
+
 /* Builds the product structures from leaves produced elsewhere (dense
  * SubMatrix blocks and LowRankMatrix U, V factors; the layout of the header
  * comment), e.g. the leaves of a reference HMatrix: replaces the leaf list that

@@ -208,8 +208,8 @@ def test_local_products_subrhs_dense_and_infos(htg, orc, kind):
     Ao = H.get_local_dense_perm()
     assert rel(Ao[np.ix_(pt, ps)], Ac) == 0.0
     assert rel(Ao @ X, Y) < 1e-12
-    assert np.array_equal(H.get_local_diagonal(True), np.diag(Ac))
-    assert np.array_equal(H.get_local_diagonal(False)[pt], np.diag(Ac))
+    assert np.array_equal(H.get_local_diagonal(False), np.diag(Ac))  # cluster ordering
+    assert np.array_equal(H.get_local_diagonal()[pt], np.diag(Ac))   # original ordering (permutation=True)
     # mvprod_subrhs: leaves meeting the column window [off, off + size], in from the window
     off, size, margin = 700, 400, 60
     win = np.ascontiguousarray(X[ps, :][off - margin:off + size + margin]).reshape(-1)
