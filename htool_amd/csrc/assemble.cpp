@@ -551,8 +551,6 @@ void assemble_ex(const htg_params &prm, const htg_generator *gen, const htg_host
 
     // dense leaves of a host generator are filled after the build (from the caller's generator)
     res.delay_dense = prm.delay_dense != 0 || hgen != nullptr;
-    if (res.cplx && prm.recompress_tol > 0)
-        throw Error(HTG_ERR_UNSUPPORTED, "QR+SVD recompression is implemented for the real kernel families only");
     Assembler as{T, S, prm, res.kd, res.dT(), res.dS(), {}, {}, {}, 0, {}, 0, &res, res.cplx ? 2 : 1};
     const char uplo = sym_uplo(prm);
     as.sym          = uplo != 'N';
@@ -645,7 +643,7 @@ void recompress_and_split(const htg_params &prm, std::vector<Leaf> &leaves, DevA
         std::vector<int> which;
         for (size_t l = 0; l < leaves.size(); l++) {
             const Leaf &L = leaves[l];
-            if (L.k >= 2 && L.k <= RC_KMAX) {
+            if (L.k >= 2 && L.k <= (cplx ? RC_KMAX_C : RC_KMAX)) {
                 rc.push_back(RcLeaf{L.src_u, L.src_v, L.m, L.n, L.k, 0});
                 which.push_back((int)l);
             }
@@ -658,7 +656,12 @@ void recompress_and_split(const htg_params &prm, std::vector<Leaf> &leaves, DevA
             d_rank.alloc(rc.size());
             d_ok.alloc(rc.size());
             d_sig.alloc(rc.size() * RC_KMAX);
-            launch_recompress(d_rc.p, (int)rc.size(), store.p, prm.recompress_tol, d_rank.p, d_ok.p, d_sig.p, nullptr);
+            if (cplx)
+                launch_recompress_c(d_rc.p, (int)rc.size(), reinterpret_cast<cdbl *>(store.p), prm.recompress_tol,
+                                    d_rank.p, d_ok.p, d_sig.p, nullptr);
+            else
+                launch_recompress(d_rc.p, (int)rc.size(), store.p, prm.recompress_tol, d_rank.p, d_ok.p, d_sig.p,
+                                  nullptr);
             HTG_HIP(hipGetLastError());
             std::vector<int> rank(rc.size()), ok(rc.size());
             std::vector<double> sig(rc.size() * RC_KMAX);

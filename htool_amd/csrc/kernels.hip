@@ -1050,6 +1050,330 @@ __global__ void __launch_bounds__(WG) k_recompress(const RcLeaf *leaves, double 
 
 
 // ---------------------------------------------------------------------------
+// Complex FP64 recompression of one low-rank leaf per workgroup: the real
+// kernel's steps with Hermitian Grams and unitary rotations. A = U V with
+// U^H U = Ru^H Ru and V V^H = Rv^H Rv (Cholesky of the scaled Grams), so
+// A = (U Ru^-1)(Ru Rv^H)(Rv^-H V) with orthonormal outer factors; C = Ru Rv^H =
+// P S Q^H by one-sided Jacobi (each pair of columns first turned so that their
+// inner product is real, then the real rotation), U' = U Ru^-1 P_r and
+// V' = (Rv^-1 Q_r)^H V: unit columns / rows, the kept singular values as
+// scales, r the smallest rank whose dropped tail is <= tol * ||S||.
+namespace rcc {
+constexpr int K  = RC_KMAX_C;
+constexpr int CH = 32;
+
+__device__ __forceinline__ cdbl cj(cdbl a) { return cdbl{a.re, -a.im}; }
+__device__ __forceinline__ cdbl cmul(cdbl a, cdbl b) {
+    return cdbl{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cdbl csub(cdbl a, cdbl b) { return cdbl{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cdbl cadd(cdbl a, cdbl b) { return cdbl{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cdbl cdiv_real(cdbl a, double d) { return cdbl{a.re / d, a.im / d}; }
+// a / b for b != 0 (Smith's method, as the ACA's reciprocal)
+__device__ __forceinline__ cdbl cdiv(cdbl a, cdbl b) {
+    if (fabs(b.re) >= fabs(b.im)) {
+        const double r = b.im / b.re, d = b.re + b.im * r;
+        return cdbl{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+    }
+    const double r = b.re / b.im, d = b.re * r + b.im;
+    return cdbl{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+
+// The Hermitian Gram of k vectors of length len, vector a at X[a * ld + i]:
+// G[a][b] = sum_i conj(x_a[i]) x_b[i] for U's columns (U^H U), or
+// sum_i x_a[i] conj(x_b[i]) for V's rows (V V^H, vrows).
+__device__ void gram(const cdbl *X, size_t ld, int len, int k, bool vrows, cdbl *G, cdbl *sT) {
+    constexpr int NQ = (K * K + WG - 1) / WG;
+    cdbl acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+        acc[q] = cdbl{0, 0};
+    const int nent = k * k;
+    for (int i0 = 0; i0 < len; i0 += CH) {
+        const int nb = min(CH, len - i0);
+        for (int e = threadIdx.x; e < CH * k; e += WG) {
+            const int ii = e % CH, a = e / CH;
+            sT[ii * K + a] = ii < nb ? X[(size_t)a * ld + i0 + ii] : cdbl{0, 0};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const int e = threadIdx.x + q * WG;
+            if (e < nent) {
+                const int a = e / k, b = e % k;
+                cdbl s{0, 0};
+                for (int ii = 0; ii < CH; ii++)
+                    s = cadd(s, vrows ? cmul(sT[ii * K + a], cj(sT[ii * K + b]))
+                                      : cmul(cj(sT[ii * K + a]), sT[ii * K + b]));
+                acc[q] = cadd(acc[q], s);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int e = threadIdx.x + q * WG;
+        if (e < nent)
+            G[(e / k) * K + e % k] = acc[q];
+    }
+    __syncthreads();
+}
+
+// In place: Hermitian positive definite G -> upper triangular R with
+// R^H R = D^-1 G D^-1, then R <- R D (D = sqrt(diag G), real). false when not
+// positive definite.
+__device__ bool cholesky_scaled(cdbl *G, int k, double *sd, int *s_ok) {
+    if (threadIdx.x < k)
+        sd[threadIdx.x] = sqrt(G[threadIdx.x * K + threadIdx.x].re);
+    if (threadIdx.x == 0)
+        *s_ok = 1;
+    __syncthreads();
+    for (int e = threadIdx.x; e < k * k; e += WG) {
+        const int a = e / k, b = e % k;
+        G[a * K + b] = cdiv_real(G[a * K + b], sd[a] * sd[b]);
+    }
+    __syncthreads();
+    for (int j = 0; j < k; j++) {
+        if (threadIdx.x == 0) {
+            double d = G[j * K + j].re;
+            for (int i = 0; i < j; i++)
+                d -= G[i * K + j].re * G[i * K + j].re + G[i * K + j].im * G[i * K + j].im;
+            if (!(d > 1e-13))
+                *s_ok = 0;
+            G[j * K + j] = cdbl{sqrt(fmax(d, 1e-300)), 0.0};
+        }
+        __syncthreads();
+        const int c = j + 1 + threadIdx.x;
+        if (c < k) {
+            cdbl v = G[j * K + c];
+            for (int i = 0; i < j; i++)
+                v = csub(v, cmul(cj(G[i * K + j]), G[i * K + c]));
+            G[j * K + c] = cdiv_real(v, G[j * K + j].re);
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < k * k; e += WG) {
+        const int a = e / k, b = e % k;
+        G[a * K + b] = b >= a ? cdbl{G[a * K + b].re * sd[b], G[a * K + b].im * sd[b]} : cdbl{0, 0};
+    }
+    const bool ok = *s_ok != 0;
+    __syncthreads();
+    return ok;
+}
+
+// R X = B in place in B (R upper triangular k x k, B k x r).
+__device__ void upper_solve(const cdbl *R, cdbl *B, int k, int r) {
+    const int c = threadIdx.x;
+    if (c < r) {
+        for (int i = k - 1; i >= 0; i--) {
+            cdbl v = B[i * K + c];
+            for (int l = i + 1; l < k; l++)
+                v = csub(v, cmul(R[i * K + l], B[l * K + c]));
+            B[i * K + c] = cdiv(v, R[i * K + i]);
+        }
+    }
+    __syncthreads();
+}
+} // namespace rcc
+
+__global__ void __launch_bounds__(WG) k_recompress_c(const RcLeaf *leaves, cdbl *store, double tol, int *rank_out,
+                                                     int *ok_out, double *sigma_out) {
+    using namespace rcc;
+    __shared__ cdbl sRu[K * K], sRv[K * K], sW[K * K], sQ[K * K], sA[K * K], sT[CH * K];
+    __shared__ double s_sig[K], s_d[K];
+    __shared__ int s_idx[K], s_ok, s_rot, s_r;
+    const RcLeaf L = leaves[blockIdx.x];
+    const int k    = L.k;
+    cdbl *U        = store + L.u_off;
+    cdbl *V        = store + L.v_off;
+    if (k < 2 || k > K) {
+        if (threadIdx.x == 0) {
+            rank_out[blockIdx.x] = k;
+            ok_out[blockIdx.x]   = 0;
+        }
+        return;
+    }
+    gram(U, (size_t)L.M, L.M, k, false, sRu, sT); // U^H U
+    gram(V, (size_t)L.N, L.N, k, true, sRv, sT);  // V V^H
+    const bool oku = cholesky_scaled(sRu, k, s_d, &s_ok);
+    const bool okv = cholesky_scaled(sRv, k, s_d, &s_ok);
+    if (!oku || !okv) {
+        if (threadIdx.x == 0) {
+            rank_out[blockIdx.x] = k;
+            ok_out[blockIdx.x]   = 0;
+        }
+        return;
+    }
+    // C = Ru Rv^H into sW; sQ = identity.
+    for (int e = threadIdx.x; e < k * k; e += WG) {
+        const int a = e / k, b = e % k;
+        cdbl v{0, 0};
+        for (int i = max(a, b); i < k; i++)
+            v = cadd(v, cmul(sRu[a * K + i], cj(sRv[b * K + i])));
+        sW[a * K + b] = v;
+        sQ[a * K + b] = cdbl{a == b ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();
+    const int kk = k + (k & 1);
+    if (k & 1) {
+        for (int a = threadIdx.x; a < k; a += WG) {
+            sW[a * K + k] = cdbl{0, 0};
+            sQ[a * K + k] = cdbl{0, 0};
+        }
+        if (threadIdx.x == 0)
+            sQ[k * K + k] = cdbl{1.0, 0.0};
+        for (int b = threadIdx.x; b < k; b += WG)
+            sQ[k * K + b] = cdbl{0, 0};
+        __syncthreads();
+    }
+    const int rows = k;
+    for (int sweep = 0; sweep < 40; sweep++) {
+        if (threadIdx.x == 0)
+            s_rot = 0;
+        __syncthreads();
+        for (int round = 0; round < kk - 1; round++) {
+            const int t = threadIdx.x;
+            if (t < kk / 2) {
+                auto at = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + round) % (kk - 1); };
+                int p = at(t), q = at(kk - 1 - t);
+                if (p > q) {
+                    const int tmp = p;
+                    p = q, q = tmp;
+                }
+                double al = 0, be = 0;
+                cdbl ga{0, 0}; // w_p^H w_q
+                for (int a = 0; a < rows; a++) {
+                    const cdbl wp = sW[a * K + p], wq = sW[a * K + q];
+                    al += wp.re * wp.re + wp.im * wp.im;
+                    be += wq.re * wq.re + wq.im * wq.im;
+                    ga = cadd(ga, cmul(cj(wp), wq));
+                }
+                const double g = sqrt(ga.re * ga.re + ga.im * ga.im);
+                if (g > 1e-15 * sqrt(al * be) && g != 0.0) {
+                    // column q times e^{-i arg ga}: the pair's inner product becomes g (real)
+                    const cdbl ph{ga.re / g, -ga.im / g};
+                    const double zeta = (be - al) / (2 * g);
+                    const double tt   = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+                    const double c = 1 / sqrt(1 + tt * tt), sn = c * tt;
+                    for (int a = 0; a < rows; a++) {
+                        const cdbl wp = sW[a * K + p], wq = cmul(sW[a * K + q], ph);
+                        sW[a * K + p] = cdbl{c * wp.re - sn * wq.re, c * wp.im - sn * wq.im};
+                        sW[a * K + q] = cdbl{sn * wp.re + c * wq.re, sn * wp.im + c * wq.im};
+                    }
+                    for (int a = 0; a < kk; a++) {
+                        const cdbl vp = sQ[a * K + p], vq = cmul(sQ[a * K + q], ph);
+                        sQ[a * K + p] = cdbl{c * vp.re - sn * vq.re, c * vp.im - sn * vq.im};
+                        sQ[a * K + q] = cdbl{sn * vp.re + c * vq.re, sn * vp.im + c * vq.im};
+                    }
+                    s_rot = 1;
+                }
+            }
+            __syncthreads();
+        }
+        if (!s_rot)
+            break;
+        __syncthreads();
+    }
+    if (threadIdx.x < kk) {
+        double v = 0;
+        for (int a = 0; a < rows; a++)
+            v += sW[a * K + threadIdx.x].re * sW[a * K + threadIdx.x].re +
+                 sW[a * K + threadIdx.x].im * sW[a * K + threadIdx.x].im;
+        s_d[threadIdx.x] = sqrt(v);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 0; j < kk; j++)
+            s_idx[j] = j;
+        for (int j = 1; j < kk; j++) {
+            const int id = s_idx[j];
+            int l        = j - 1;
+            while (l >= 0 && s_d[s_idx[l]] < s_d[id]) {
+                s_idx[l + 1] = s_idx[l];
+                l--;
+            }
+            s_idx[l + 1] = id;
+        }
+        double tot = 0;
+        for (int j = 0; j < k; j++)
+            tot += s_d[s_idx[j]] * s_d[s_idx[j]];
+        int r       = k;
+        double tail = 0;
+        for (int j = k - 1; j >= 1; j--) {
+            tail += s_d[s_idx[j]] * s_d[s_idx[j]];
+            if (tail <= tol * tol * tot)
+                r = j;
+            else
+                break;
+        }
+        for (int j = 0; j < k; j++)
+            s_sig[j] = s_d[s_idx[j]];
+        s_r = r;
+    }
+    __syncthreads();
+    const int r = s_r;
+    // A = Ru^-1 P_r (P_r: W's columns over sigma, sorted); B = Rv^-1 Q_r written
+    // over W once A has been read out of it.
+    for (int e = threadIdx.x; e < k * r; e += WG) {
+        const int a = e / r, c = e % r;
+        sA[a * K + c] = cdiv_real(sW[a * K + s_idx[c]], s_sig[c]);
+    }
+    __syncthreads();
+    cdbl *sB = sW;
+    for (int e = threadIdx.x; e < k * r; e += WG) {
+        const int a = e / r, c = e % r;
+        sB[a * K + c] = sQ[a * K + s_idx[c]];
+    }
+    __syncthreads();
+    upper_solve(sRu, sA, k, r);
+    upper_solve(sRv, sB, k, r);
+    // U <- U A, V <- B^H V.
+    for (int i0 = 0; i0 < L.M; i0 += CH) {
+        const int nb = min(CH, L.M - i0);
+        for (int e = threadIdx.x; e < CH * k; e += WG) {
+            const int ii = e % CH, a = e / CH;
+            sT[ii * K + a] = ii < nb ? U[(size_t)a * L.M + i0 + ii] : cdbl{0, 0};
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < CH * r; e += WG) {
+            const int ii = e % CH, c = e / CH;
+            if (ii < nb) {
+                cdbl v{0, 0};
+                for (int a = 0; a < k; a++)
+                    v = cadd(v, cmul(sT[ii * K + a], sA[a * K + c]));
+                U[(size_t)c * L.M + i0 + ii] = v;
+            }
+        }
+        __syncthreads();
+    }
+    for (int j0 = 0; j0 < L.N; j0 += CH) {
+        const int nb = min(CH, L.N - j0);
+        for (int e = threadIdx.x; e < CH * k; e += WG) {
+            const int jj = e % CH, a = e / CH;
+            sT[jj * K + a] = jj < nb ? V[(size_t)a * L.N + j0 + jj] : cdbl{0, 0};
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < CH * r; e += WG) {
+            const int jj = e % CH, c = e / CH;
+            if (jj < nb) {
+                cdbl v{0, 0};
+                for (int a = 0; a < k; a++)
+                    v = cadd(v, cmul(cj(sB[a * K + c]), sT[jj * K + a]));
+                V[(size_t)c * L.N + j0 + jj] = v;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        rank_out[blockIdx.x] = r;
+        ok_out[blockIdx.x]   = 1;
+    }
+    for (int j = threadIdx.x; j < r; j += WG)
+        sigma_out[(size_t)blockIdx.x * RC_KMAX + j] = s_sig[j];
+}
+
+
+// ---------------------------------------------------------------------------
 // Product phase 0: x (nc x mu column-major, original order) -> x part of z
 // (cluster order, compute type) for RHS [mu_base, mu_base+MU). Entries
 // [nc, tpad) are written as zeros on every call: they meet the zero padding of
@@ -3473,6 +3797,12 @@ void launch_recompress(const RcLeaf *leaves, int nleaves, double *store, double 
                        double *sigma, hipStream_t st) {
     if (nleaves > 0)
         hipLaunchKernelGGL(k_recompress, dim3(nleaves), dim3(WG), 0, st, leaves, store, tol, rank, ok, sigma);
+}
+
+void launch_recompress_c(const RcLeaf *leaves, int nleaves, cdbl *store, double tol, int *rank, int *ok,
+                         double *sigma, hipStream_t st) {
+    if (nleaves > 0)
+        hipLaunchKernelGGL(k_recompress_c, dim3(nleaves), dim3(WG), 0, st, leaves, store, tol, rank, ok, sigma);
 }
 
 // Layout changes between the cluster ordering with interleaved right-hand sides

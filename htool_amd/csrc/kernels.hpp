@@ -98,7 +98,8 @@ struct RcLeaf {
     uint64_t u_off, v_off;
     int M, N, k, pad;
 };
-constexpr int RC_KMAX = 48; // largest rank the recompression kernel handles
+constexpr int RC_KMAX = 48;   // largest rank the recompression kernel handles
+constexpr int RC_KMAX_C = 36; // the same for complex leaves (twice the LDS per entry)
 
 // Copy of one compressed leaf: U (M x k column-major at src_u) into U tiles, V
 // (k rows of N entries, row strides src_ldv / dst_ldv) from src_v to dst_v.
@@ -128,6 +129,8 @@ void launch_lr_pack(const LrPack *packs, int npacks, const UTileDst *utiles, con
 // recompressed (0: kept as is), sigma[b*RC_KMAX + j] = the kept singular values.
 void launch_recompress(const RcLeaf *leaves, int nleaves, double *store, double tol, int *rank, int *ok,
                        double *sigma, hipStream_t st);
+void launch_recompress_c(const RcLeaf *leaves, int nleaves, cdbl *store, double tol, int *rank, int *ok,
+                         double *sigma, hipStream_t st);
 
 // Product phases. TS = storage type, TC = compute type inside a tile. z holds
 // the cluster-ordered x (nc RHS vectors) followed by t (one RHS vector per

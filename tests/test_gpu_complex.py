@@ -128,3 +128,50 @@ def test_complex_matrix_core_passes_match_single_rhs(htg, storage, bound):
     Y = H.mvprod_global_to_global(V)
     for c in (0, 5, 15, 16, 31, 32, 39):
         assert rel(Y[:, c], H @ V[:, c]) < bound, c
+
+
+def test_complex_recompression_ranks_match_numpy_svd(htg, orc):
+    """FP64 QR+SVD recompression of complex leaves (Hermitian Grams, unitary
+    Jacobi rotations): the kept rank of every leaf of rank 2..36 is, within one,
+    the smallest r whose dropped singular values have a Frobenius norm below
+    tol ||A_leaf|| (the truncation of the reference's SVD compressor,
+    lrmat/SVD.hpp:46-55, applied to the leaf); the recompressed leaves' product
+    is the H-matrix's product (1e-12) and within the tolerance of the ACA leaves'."""
+    x, k, diag = helmholtz_case(5000, 81)
+    tol = 3e-6
+    H0 = htg.HMatrix(epsilon=1e-5, eta=10.0, minclustersize=32)
+    H0.set_compression("partialACA")
+    H0.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, diag, k), x)
+    H1 = htg.HMatrix(epsilon=1e-5, eta=10.0, minclustersize=32)
+    H1.set_compression("partialACA")
+    H1.set_recompression(tol)
+    H1.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, diag, k), x)
+    L0, D0 = H0.get_leaves(with_data=True)
+    L1, D1 = H1.get_leaves(with_data=True)
+    assert np.array_equal(L0[:, :4], L1[:, :4])
+    checked = 0
+    for (oi, m, oj, n, kk, a), k1 in zip(orc.iter_leaf_data(L0, D0), L1[:, 4]):
+        if kk < 0:
+            continue
+        if 2 <= kk <= 36:
+            s = np.linalg.svd(a[0] @ a[1], compute_uv=False)[:kk]
+            norm = np.sqrt((s ** 2).sum())
+            j, tail = len(s), 0.0
+            while True:  # SVD.hpp: drop from the smallest while sqrt(tail) / norm < tol
+                j -= 1
+                tail += s[j] ** 2
+                if not (j > 0 and np.sqrt(tail) / norm < tol):
+                    break
+            r = j + 1
+            assert abs(int(k1) - r) <= 1, (oi, oj, kk, k1, r)
+            checked += 1
+        else:
+            assert k1 == kk
+    assert checked > 100
+    i0, i1 = H0.get_info(), H1.get_info()
+    assert i1.lowrank_entries < i0.lowrank_entries
+    v = cvec(5000, 2, 9)
+    y1 = H1.mvprod_global_to_global(v)
+    pt, ps = H1.get_perms()
+    assert rel(y1, orc.leaf_mvprod(5000, 5000, pt, ps, L1, D1, v)) < 1e-12
+    assert rel(y1, H0.mvprod_global_to_global(v)) < 10 * tol

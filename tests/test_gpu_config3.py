@@ -95,3 +95,31 @@ def test_config3_against_dense(case, orc):
     yd = orc.dense_rows_c(case["pts"], None, rows, x, kernel=geo.KERNEL_HELMHOLTZ, wavenumber=case["k"],
                           diag=case["diag"], same=True)
     assert rel((H @ x)[rows], yd) < 2e-5
+
+
+@pytest.mark.parametrize("storage,bound", [("fp32", 2e-5), ("fp32_fp16", 2e-5), ("fp32_bf16", 4e-5)])
+def test_config3_recompressed_mixed_storage_against_dense(htg, orc, case, storage, bound):
+    """With complex FP64 QR+SVD recompression (tolerance 3e-6) and the trailing
+    terms in 16 bits within a 5e-6 budget, every storage stays within 2
+    epsilon of dense FP64 (BF16: 4 epsilon), where storing every term in 16 bits
+    does not (test_config3_storages_against_fp64_leaves: 1e-3)."""
+    print(f"config 3 recompressed {storage}", flush=True)
+    pts, k, diag = case["pts"], case["k"], case["diag"]
+    H = htg.HMatrix(epsilon=1e-5, eta=10.0, minclustersize=32)
+    H.set_compression("partialACA")
+    H.set_recompression(3e-6)
+    H.set_lowprec(5e-6)
+    H.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, diag, k), pts)
+    i = H.get_info()
+    assert 0 < i.lowprec_entries < i.lowrank_entries
+    assert np.array_equal(H.get_leaves()[:, :4], case["leaves"][:, :4])
+    rows = np.random.default_rng(10).choice(N, 300, replace=False).astype(np.int32)
+    x = case["X"][:, 0]
+    yd = orc.dense_rows_c(pts, None, rows, x, kernel=geo.KERNEL_HELMHOLTZ, wavenumber=k, diag=diag, same=True)
+    H.set_storage(storage)
+    err = rel((H @ x)[rows], yd)
+    print(f"  rel err vs dense {err:.3e}", flush=True)
+    assert err < bound
+    Y = H.mvprod_global_to_global(case["X"])
+    H.set_storage("fp64")
+    assert rel(Y, H.mvprod_global_to_global(case["X"])) < (1e-6 if storage == "fp32" else bound)
