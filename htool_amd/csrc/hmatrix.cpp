@@ -19,19 +19,26 @@ namespace htg {
 namespace {
 
 
-constexpr int ALIGN        = 32;   // block alignment in elements
 // Default work decomposition (htg_layout): rows per row block, target entries
 // per V chunk and per phase-1 wavefront, most chunks per wavefront, stacked V
 // rows per chunk (one matrix-core tile).
 constexpr htg_layout DEFAULT_LAYOUT = {128, 8192, 2048, 4, 16};
 constexpr int CLASS_ALIGN[2] = {4, 8}; // entries per aligned group: class 1 may hold 16-bit values
+// Row strides and block bases in entries: every stored row (a row of a row
+// block, a V row) starts on a 128-byte line at every storage width of its
+// class (32-bit class 0: 32 entries; 16-bit class 1: 64), so no line holds the
+// end of one row and the start of the next: a row read in several passes
+// fetches each line once.
+constexpr int ROW_ALIGN[2] = {32, 64};
 constexpr int CAP_MAX        = 512;    // must match the ACA kernel's CAPMAX
 
-inline uint64_t align_up(uint64_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
+inline uint64_t align_to(uint64_t v, int a) { return (v + a - 1) / a * a; }
 inline int round_up(int v, int a) { return (v + a - 1) / a * a; }
-// V row of a low-rank leaf in class c: global columns [oj & ~(a-1), round_a(oj + n)).
+// V row of a low-rank leaf in class c: global columns [oj & ~(a-1), round_a(oj + n))
+// (v_extent entries, read by the products), stored with stride v_stride.
 inline int v_first(const Leaf &L, int c) { return L.oj & ~(CLASS_ALIGN[c] - 1); }
-inline int v_stride(const Leaf &L, int c) { return round_up(L.oj + L.n, CLASS_ALIGN[c]) - v_first(L, c); }
+inline int v_extent(const Leaf &L, int c) { return round_up(L.oj + L.n, CLASS_ALIGN[c]) - v_first(L, c); }
+inline int v_stride(const Leaf &L, int c) { return round_up(v_extent(L, c), ROW_ALIGN[c]); }
 inline int terms(const Leaf &L, int c) { return L.k <= 0 ? 0 : (c == 0 ? L.k_hi : L.k - L.k_hi); }
 
 // Rows per workgroup of the single-RHS row phase with two precision classes (a
@@ -295,7 +302,7 @@ void build_layout(htg_hmatrix &h) {
         for (int c = 0; c < 2; c++) {
             ClassLayout &C   = h.cls[c];
             C.rb_col0[r + 1] = C.rb_col0[r] + col[c];
-            C.arena_size     = align_up(C.arena_size + (uint64_t)R * round_up(col[c], CLASS_ALIGN[c]));
+            C.arena_size     = align_to(C.arena_size + (uint64_t)R * round_up(col[c], ROW_ALIGN[c]), ROW_ALIGN[c]);
         }
     }
     // Heaviest row blocks are dispatched first.
@@ -313,10 +320,28 @@ void build_layout(htg_hmatrix &h) {
     // workgroups balance the phase better (a row block's columns are staged
     // once per part).
     h.rb_parts.clear();
-    for (int r : h.rb_order) {
-        const int np = (h.rb_row0[r + 1] - h.rb_row0[r] + ROWS_SPLIT - 1) / ROWS_SPLIT;
-        for (int p = 0; p < np; p++)
-            h.rb_parts.push_back(r | p << 24 | (np > 1 ? np : 0) << 28);
+    {
+        // all parts of a row block on one XCD (workgroup b runs on XCD b % 8),
+        // so its column indices and vector entries are fetched into one L2:
+        // whole row blocks, heaviest first, to the least loaded XCD; list j
+        // goes to positions 8 i + j, shorter lists padded with empty entries
+        constexpr int NX = 8;
+        std::vector<std::vector<int>> lst(NX);
+        std::vector<int64_t> load(NX, 0);
+        for (int r : h.rb_order) {
+            const int np = (h.rb_row0[r + 1] - h.rb_row0[r] + ROWS_SPLIT - 1) / ROWS_SPLIT;
+            const int j  = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            for (int p = 0; p < np; p++)
+                lst[j].push_back(r | p << 24 | (np > 1 ? np : 0) << 28);
+            load[j] += work(r);
+        }
+        size_t len = 0;
+        for (auto &l : lst)
+            len = std::max(len, l.size());
+        h.rb_parts.assign(len * NX, 0xffffff);
+        for (int j = 0; j < NX; j++)
+            for (size_t i = 0; i < lst[j].size(); i++)
+                h.rb_parts[i * NX + j] = lst[j][i];
     }
 
     // V arena: per class, panel by panel, the stacked V rows of the panel's
@@ -337,7 +362,7 @@ void build_layout(htg_hmatrix &h) {
                 }
             }
             if (K > 0)
-                C.varena_size = align_up(C.varena_size + (uint64_t)K * ld);
+                C.varena_size = align_to(C.varena_size + (uint64_t)K * ld, ROW_ALIGN[c]);
         }
     }
     // V chunks, class 0 then class 1, grouped per wavefront, in the three
@@ -370,15 +395,16 @@ void build_layout(htg_hmatrix &h) {
                     continue;
                 const Leaf &L0       = h.leaves[first];
                 const int ld         = v_stride(L0, c);
+                const int ext        = v_extent(L0, c);
                 const uint64_t base  = C.leaf_v_off[first];
                 const int tfirst     = t_entry(h, first, c == 0 ? 0 : L0.k_hi);
-                const bool whole     = cut == 1 && ld <= h.lay.vchunk_el;
+                const bool whole     = cut == 1 && ext <= h.lay.vchunk_el;
                 const int chunk_el   = cut == 2 ? 2 * h.lay.vchunk_el : h.lay.vchunk_el;
-                const int rows_g     = whole ? std::max(1, std::min(h.lay.vrows, h.lay.vchunk_el / ld)) : h.lay.vrows;
+                const int rows_g     = whole ? std::max(1, std::min(h.lay.vrows, h.lay.vchunk_el / ext)) : h.lay.vrows;
                 for (int g0 = 0; g0 < K; g0 += rows_g) {
                     const int kr     = std::min(rows_g, K - g0);
-                    const int cpc    = whole ? ld : round_up(std::max(64, chunk_el / kr), CLASS_ALIGN[c]);
-                    const int nparts = (ld + cpc - 1) / cpc;
+                    const int cpc    = whole ? ext : round_up(std::max(64, chunk_el / kr), ROW_ALIGN[c]);
+                    const int nparts = (ext + cpc - 1) / cpc;
                     const int pbase  = part_size;
                     if (nparts > 1)
                         part_size += nparts * kr;
@@ -388,7 +414,7 @@ void build_layout(htg_hmatrix &h) {
                         v.v_off      = base + (uint64_t)g0 * ld + (uint64_t)j0;
                         v.ld         = ld;
                         v.k          = kr;
-                        v.ncols      = std::min(cpc, ld - j0);
+                        v.ncols      = std::min(cpc, ext - j0);
                         v.x_off      = v_first(L0, c) + j0;
                         v.t_index    = tfirst + g0;
                         v.part_off   = pbase + q * kr;
@@ -413,6 +439,28 @@ void build_layout(htg_hmatrix &h) {
         }
         if ((int)T.tiles.size() > T.vg_off.back())
             T.vg_off.push_back((int)T.tiles.size());
+        {
+            // wavefront groups in decreasing work (16-bit rows count half):
+            // the long groups start first and the short ones fill the end
+            const int ng = (int)T.vg_off.size() - 1;
+            std::vector<int64_t> work(ng, 0);
+            for (int g = 0; g < ng; g++)
+                for (int t = T.vg_off[g]; t < T.vg_off[g + 1]; t++)
+                    work[g] += (int64_t)T.tiles[t].k * T.tiles[t].ncols * (T.tiles[t].cls == 0 ? 2 : 1);
+            std::vector<int> ord(ng);
+            for (int g = 0; g < ng; g++)
+                ord[g] = g;
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return work[a] > work[b]; });
+            std::vector<VTile> nt;
+            std::vector<int> no(1, 0);
+            for (int g : ord) {
+                for (int t = T.vg_off[g]; t < T.vg_off[g + 1]; t++)
+                    nt.push_back(T.tiles[t]);
+                no.push_back((int)nt.size());
+            }
+            T.tiles.swap(nt);
+            T.vg_off.swap(no);
+        }
         h.part_size = std::max(h.part_size, part_size); // the cuts never run at once
     }
 }
@@ -462,7 +510,7 @@ void alloc_master(htg_hmatrix &h) {
 
 int row_block_rows(const htg_hmatrix &h, int rb) { return h.rb_row0[rb + 1] - h.rb_row0[rb]; }
 int row_stride(const htg_hmatrix &h, int c, int rb) {
-    return round_up(h.cls[c].rb_col0[rb + 1] - h.cls[c].rb_col0[rb], CLASS_ALIGN[c]);
+    return round_up(h.cls[c].rb_col0[rb + 1] - h.cls[c].rb_col0[rb], ROW_ALIGN[c]);
 }
 
 // Right-hand sides per pass: at most 32, the width of the matrix-core passes
@@ -490,8 +538,8 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
     TC *z = reinterpret_cast<TC *>(h.d_z.p);
     const double *tscale = h.scaled ? h.d_tscale.p : nullptr;
     const bool has1      = !h.cls[1].empty();
-    const RowClass rc0{h.cls[0].d_rb_col0.p, h.cls[0].d_rb_aoff.p, h.cls[0].d_colidx.p, CLASS_ALIGN[0]};
-    const RowClass rc1{h.cls[1].d_rb_col0.p, h.cls[1].d_rb_aoff.p, h.cls[1].d_colidx.p, CLASS_ALIGN[1]};
+    const RowClass rc0{h.cls[0].d_rb_col0.p, h.cls[0].d_rb_aoff.p, h.cls[0].d_colidx.p, CLASS_ALIGN[0], ROW_ALIGN[0]};
+    const RowClass rc1{h.cls[1].d_rb_col0.p, h.cls[1].d_rb_aoff.p, h.cls[1].d_colidx.p, CLASS_ALIGN[1], ROW_ALIGN[1]};
     int done = 0;
     while (done < mu) {
         const int MU = pick_mu_chunk(mu - done, 32);
@@ -533,8 +581,8 @@ void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream
     R *z                 = reinterpret_cast<R *>(h.d_z.p);
     const double *tscale = h.scaled ? h.d_tscale.p : nullptr;
     const bool has1      = !h.cls[1].empty();
-    const RowClass rc0{h.cls[0].d_rb_col0.p, h.cls[0].d_rb_aoff.p, h.cls[0].d_colidx.p, CLASS_ALIGN[0]};
-    const RowClass rc1{h.cls[1].d_rb_col0.p, h.cls[1].d_rb_aoff.p, h.cls[1].d_colidx.p, CLASS_ALIGN[1]};
+    const RowClass rc0{h.cls[0].d_rb_col0.p, h.cls[0].d_rb_aoff.p, h.cls[0].d_colidx.p, CLASS_ALIGN[0], ROW_ALIGN[0]};
+    const RowClass rc1{h.cls[1].d_rb_col0.p, h.cls[1].d_rb_aoff.p, h.cls[1].d_colidx.p, CLASS_ALIGN[1], ROW_ALIGN[1]};
     int done = 0;
     while (done < mu) {
         const int MU = pick_mu_chunk(mu - done, 16);

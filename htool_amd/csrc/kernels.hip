@@ -1526,6 +1526,43 @@ struct Vec<__hip_bfloat16> {
     }
 };
 
+// Stored entries as raw 16-byte words, converted to FP32 only when used
+// (16-bit formats stay packed two to a register until then).
+template <typename TS>
+struct Raw;
+template <>
+struct Raw<float> {
+    static constexpr int W = 4;
+    __device__ static void cvt(const nt_u4 &v, float (&f)[4]) {
+        f[0] = __uint_as_float(v.x), f[1] = __uint_as_float(v.y), f[2] = __uint_as_float(v.z),
+        f[3] = __uint_as_float(v.w);
+    }
+};
+template <>
+struct Raw<__half> {
+    static constexpr int W = 8;
+    __device__ static void cvt(const nt_u4 &v, float (&f)[8]) {
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            f[2 * k]     = __half2float(__ushort_as_half((unsigned short)(w[k] & 0xffffu)));
+            f[2 * k + 1] = __half2float(__ushort_as_half((unsigned short)(w[k] >> 16)));
+        }
+    }
+};
+template <>
+struct Raw<__hip_bfloat16> {
+    static constexpr int W = 8;
+    __device__ static void cvt(const nt_u4 &v, float (&f)[8]) {
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            f[2 * k]     = __uint_as_float(w[k] << 16);
+            f[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+        }
+    }
+};
+
 // W consecutive entries of the compute type (16-byte aligned), in 16-byte loads.
 template <typename TC, int W>
 __device__ __forceinline__ void load_n(const TC *p, TC (&a)[W]) {
@@ -1564,6 +1601,7 @@ __device__ __forceinline__ int lanes_per_row(int nvec) {
     return lpr;
 }
 
+
 // Row-major block times an LDS-staged vector segment, shared by both product
 // phases. Rows [0, nrows) with row stride ld, nvec 16-byte vectors of columns
 // per row. Each row is taken by a group of lanes of one wave (lanes_per_row),
@@ -1578,7 +1616,70 @@ __device__ __forceinline__ void rows_gemv(const TS *base, size_t ld, int nrows, 
     const int lpr  = lanes_per_row(nvec);
     const int rpw  = 64 / lpr;
     const int sl = lane & (lpr - 1), sub = lane / lpr;
-    if constexpr (MU == 1) {
+    if constexpr (MU == 1 && std::is_same<TC, float>::value) {
+        // Two rows per lane group, software pipelined: the loads of step
+        // it + 1 (one 16-byte vector of both rows per lane) are issued before
+        // the products and the row reduction of step it. Step it takes row
+        // pair it / nseg and vectors [seg lpr, (seg + 1) lpr) of its rows.
+        const int wu      = __builtin_amdgcn_readfirstlane(w);
+        const int nseg    = (nvec + lpr - 1) / lpr;
+        const int rstep   = NWAVE * 2 * rpw;
+        const int r00     = wu * 2 * rpw + sub;
+        const int npair   = r00 - sub < nrows ? (nrows - (r00 - sub) + rstep - 1) / rstep : 0;
+        const int nit     = npair * nseg;
+        const nt_u4 zero4 = {0u, 0u, 0u, 0u};
+        auto issue = [&](int it, nt_u4 &ra, nt_u4 &rb) {
+            const int pr = it / nseg, q = (it - pr * nseg) * lpr + sl;
+            const int r0 = r00 + pr * rstep, r1 = r0 + rpw;
+            ra = zero4;
+            rb = zero4;
+            if (q < nvec && r0 < nrows)
+                ra = nt_load16(base + (size_t)r0 * ld + (size_t)q * W);
+            if (q < nvec && r1 < nrows)
+                rb = nt_load16(base + (size_t)r1 * ld + (size_t)q * W);
+        };
+        float acc0 = 0.f, acc1 = 0.f;
+        auto consume = [&](int it, const nt_u4 &ra, const nt_u4 &rb) {
+            const int pr = it / nseg, sg = it - pr * nseg, q = sg * lpr + sl;
+            float xq[W], fa[W], fb[W];
+#pragma unroll
+            for (int u = 0; u < W; u++)
+                xq[u] = s_z[(q < nvec ? q : 0) * W + u];
+            Raw<TS>::cvt(ra, fa);
+            Raw<TS>::cvt(rb, fb);
+#pragma unroll
+            for (int u = 0; u < W; u++) {
+                acc0 += fa[u] * xq[u];
+                acc1 += fb[u] * xq[u];
+            }
+            if (sg == nseg - 1) {
+                const int r0 = r00 + pr * rstep, r1 = r0 + rpw;
+                double v0 = (double)acc0, v1 = (double)acc1;
+                for (int o = lpr >> 1; o > 0; o >>= 1) {
+                    v0 += __shfl_xor(v0, o, 64);
+                    v1 += __shfl_xor(v1, o, 64);
+                }
+                if (sl == 0 && r0 < nrows)
+                    s_acc[r0] += v0;
+                if (sl == 0 && r1 < nrows)
+                    s_acc[r1] += v1;
+                acc0 = 0.f, acc1 = 0.f;
+            }
+        };
+        nt_u4 a0, b0, a1, b1;
+        if (nit > 0)
+            issue(0, a0, b0);
+        for (int it = 0; it < nit; it += 2) {
+            if (it + 1 < nit)
+                issue(it + 1, a1, b1);
+            consume(it, a0, b0);
+            if (it + 2 < nit)
+                issue(it + 2, a0, b0);
+            if (it + 1 < nit)
+                consume(it + 1, a1, b1);
+        }
+        return;
+    } else if constexpr (MU == 1) {
         // two rows per lane group: one staged vector read serves both
         for (int rbase = w * 2 * rpw; rbase < nrows; rbase += NWAVE * 2 * rpw) {
             const int r0 = rbase + sub, r1 = r0 + rpw;
@@ -1758,6 +1859,89 @@ __device__ __forceinline__ void vx_chunk_rows(const VTile &d, const TS *varena, 
     }
 }
 
+
+// One V chunk, one right-hand side, FP32 arithmetic, rows of at most 256
+// vectors. Every lane keeps 8 16-byte loads in flight per step, as R rows of Q
+// vectors (Q = 1, 2 or 4 vectors of each row per lane, R = 8 / Q rows per lane
+// group), and holds its Q x vectors in registers for the whole chunk. Lane
+// group g (of lpr lanes, rpw groups per wave) takes rows g, g + rpw, ... of the
+// step, so each load instruction reads rpw consecutive rows. The R row sums
+// of a lane are reduced in FP64 by recursive halving (R - 1 shuffles, then
+// log2(lpr / R) more); lane sl ends with row g + rpw (sl / (lpr / R)).
+template <typename TS, int Q>
+__device__ __forceinline__ void vx_chunk_q(const VTile &d, const TS *varena, float *z, double *part,
+                                           const double *tscale, int nvec, int lpr, int lane) {
+    constexpr int W = Raw<TS>::W;
+    constexpr int R = 8 / Q;
+    const int rpw = 64 / lpr, SH = lpr / R;
+    const int sl = lane & (lpr - 1), sub = lane / lpr;
+    const float *xv = z + d.x_off;
+    const TS *vb    = varena + d.v_off;
+    float xq[Q][W];
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const int q = sl + i * lpr;
+        if (q < nvec) {
+            load_n<float, W>(xv + (size_t)q * W, xq[i]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < W; u++)
+                xq[i][u] = 0.f;
+        }
+    }
+    const nt_u4 zero4 = {0u, 0u, 0u, 0u};
+    for (int rbase = 0; rbase < d.k; rbase += R * rpw) {
+        const int rfin   = rbase + sub + rpw * (sl / SH);
+        const bool wr    = sl % SH == 0 && rfin < d.k;
+        const double sc  = (tscale && d.nparts == 1 && wr) ? tscale[d.t_index + rfin] : 1.0; // in flight early
+        nt_u4 ra[R][Q];
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            const int r   = rbase + sub + rpw * j;
+            const TS *row = vb + (size_t)(r < d.k ? r : 0) * d.ld;
+#pragma unroll
+            for (int i = 0; i < Q; i++) {
+                const int q = sl + i * lpr;
+                ra[j][i]    = zero4;
+                if (q < nvec && r < d.k)
+                    ra[j][i] = nt_load16(row + (size_t)q * W);
+            }
+        }
+        double acc[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            float a = 0.f;
+#pragma unroll
+            for (int i = 0; i < Q; i++) {
+                float f[W];
+                Raw<TS>::cvt(ra[j][i], f);
+#pragma unroll
+                for (int u = 0; u < W; u++)
+                    a += f[u] * xq[i][u];
+            }
+            acc[j] = (double)a;
+        }
+#pragma unroll
+        for (int h = R / 2, o = lpr / 2; h >= 1; h >>= 1, o >>= 1) {
+            const bool up = (sl & o) != 0;
+#pragma unroll
+            for (int j = 0; j < h; j++) {
+                const double send = up ? acc[j] : acc[j + h];
+                const double keep = up ? acc[j + h] : acc[j];
+                acc[j]            = keep + __shfl_xor(send, o, 64);
+            }
+        }
+        for (int o = SH / 2; o >= 1; o >>= 1)
+            acc[0] += __shfl_xor(acc[0], o, 64);
+        if (wr) {
+            if (d.nparts > 1)
+                part[(size_t)(d.part_off + rfin)] = acc[0];
+            else
+                z[(size_t)(d.t_index + rfin)] = (float)(acc[0] * sc);
+        }
+    }
+}
+
 // One V chunk: rows taken by lane groups (lanes_per_row), FP64 shuffle reduction.
 template <typename TS, typename TC, int MU>
 __device__ __forceinline__ void vx_chunk(const VTile &d, const TS *varena, TC *z, double *part, const double *tscale,
@@ -1783,8 +1967,18 @@ __device__ __forceinline__ void vx_chunk(const VTile &d, const TS *varena, TC *z
             vx_chunk_rows<TS, TC, 4>(d, varena, z, part, tscale, lane);
         else
             vx_chunk_rows<TS, TC, 2>(d, varena, z, part, tscale, lane);
-    } else if constexpr (MU == 1) {
-        // two rows per lane group: one x load serves both
+    } else if constexpr (MU == 1 && std::is_same<TC, float>::value) {
+        // rows of at most 16 vectors: eight lanes a row, two vectors each;
+        // up to 256 vectors: four vectors a lane (x held in registers)
+        if (nvec <= 16) {
+            vx_chunk_q<TS, 2>(d, varena, z, part, tscale, nvec, 8, lane);
+            return;
+        }
+        if (nvec <= 4 * lpr) {
+            vx_chunk_q<TS, 4>(d, varena, z, part, tscale, nvec, lpr, lane);
+            return;
+        }
+        // longer rows: two rows per lane group, x reloaded per row pair
         for (int rbase = 0; rbase < d.k; rbase += 2 * rpw) {
             const int r0 = rbase + sub, r1 = r0 + rpw;
             const bool ok0 = r0 < d.k, ok1 = r1 < d.k;
@@ -1862,7 +2056,8 @@ __device__ __forceinline__ void vx_chunk(const VTile &d, const TS *varena, TC *z
 template <typename TS0, typename TS1, typename TC, int MU>
 __global__ void __launch_bounds__(WG) k_vx(const int *wv_off, int nwaves, const VTile *tiles, const TS0 *varena0,
                                            const TS1 *varena1, TC *z, double *part, const double *tscale) {
-    const int wv = blockIdx.x * NWAVE + (threadIdx.x >> 6);
+    // wave-uniform: the descriptors of the wave's chunks go to scalar registers
+    const int wv = __builtin_amdgcn_readfirstlane(blockIdx.x * NWAVE + (threadIdx.x >> 6));
     if (wv >= nwaves)
         return;
     const int lane = threadIdx.x & 63;
@@ -1947,9 +2142,10 @@ __device__ __forceinline__ void rows_class(const TS *arena, const RowClass &rc, 
     constexpr int W     = Vec<TS>::W;
     const int *colidx   = rc.colidx;
     const int c0 = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - c0;
-    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);    // columns read (vector width)
+    const int Cs = (C + rc.salign - 1) & ~(rc.salign - 1);  // row stride (128-byte rows)
     const uint64_t *rb_aoff = rc.rb_aoff;
-    const TS *A  = arena + rb_aoff[rb] + (size_t)lo * Cp;
+    const TS *A  = arena + rb_aoff[rb] + (size_t)lo * Cs;
     constexpr int ZPT = ZCAP / WG; // staged entries per thread
     for (int cc = 0; cc < Cp; cc += ZCOLS) {
         const int nz = min(ZCOLS, Cp - cc);
@@ -1980,7 +2176,7 @@ __device__ __forceinline__ void rows_class(const TS *arena, const RowClass &rc, 
             }
         }
         __syncthreads();
-        rows_gemv<TS, TC, MU>(A + cc, (size_t)Cp, R, nz / W, s_z, s_acc);
+        rows_gemv<TS, TC, MU>(A + cc, (size_t)Cs, R, nz / W, s_z, s_acc);
         __syncthreads();
     }
 }
@@ -1994,6 +2190,8 @@ __global__ void __launch_bounds__(WG) k_rows(const int *rb_order, const int *rb_
     // entry: row block (bits 0-23); a nonzero part count np (bits 28-31)
     // selects rows [R p / np, R (p + 1) / np) of it, p in bits 24-27
     const int ent = rb_order[blockIdx.x];
+    if ((ent & 0xffffff) == 0xffffff) // padding entry of an XCD-ordered list
+        return;
     const int rb = ent & 0xffffff, p = (ent >> 24) & 15, np = (int)((unsigned)ent >> 28);
     const int rf0 = rb_row0[rb], Rf = rb_row0[rb + 1] - rf0;
     const int lo = np ? Rf * p / np : 0, hi = np ? Rf * (p + 1) / np : Rf;
@@ -2149,13 +2347,14 @@ __device__ __forceinline__ void rows_class_mfma(const TS *arena, const RowClass 
     constexpr int ZPT  = RPASS_COLS * MU / WG; // staged entries per thread
     constexpr bool PREFETCH_A = S > 1;      // four row tiles: registers too few for a second copy
     const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
-    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);    // columns read (vector width)
+    const int Cs = (C + rc.salign - 1) & ~(rc.salign - 1);  // row stride (128-byte rows)
     if (Cp == 0)
         return;
     const TS *A    = arena + rc.rb_aoff[rb];
     const int lane = threadIdx.x & 63, half = lane >> 5;
     const int row  = 32 * t + (lane & 31);
-    const TS *rowp = A + (size_t)(row < R ? row : 0) * Cp + q * WCOL;
+    const TS *rowp = A + (size_t)(row < R ? row : 0) * Cs + q * WCOL;
     const int zs = threadIdx.x & (MU - 1), zc = threadIdx.x / MU; // staging: RHS, first column
     float v[ZPT];
     TS a[NIT][VW], an[PREFETCH_A ? NIT : 1][VW];
@@ -2619,7 +2818,8 @@ __global__ void __launch_bounds__(WG) k_rows_mfma64(const int *rb_order, const i
         if (!arena)
             continue;
         const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
-        const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+        const int Cp = (C + rc.align - 1) & ~(rc.align - 1);    // columns read (vector width)
+        const int Cs = (C + rc.salign - 1) & ~(rc.salign - 1);  // row stride (128-byte rows)
         const double *A = arena + rc.rb_aoff[rb];
         for (int cc = 0; cc < Cp; cc += RPASS64) {
             const int nz = min(RPASS64, Cp - cc);
@@ -2643,7 +2843,7 @@ __global__ void __launch_bounds__(WG) k_rows_mfma64(const int *rb_order, const i
                 const int t = w + 4 * tt;
                 if (t < T16) {
                     const int row   = 16 * t + r16;
-                    const double *rp = A + (size_t)(row < R ? row : 0) * Cp + cc;
+                    const double *rp = A + (size_t)(row < R ? row : 0) * Cs + cc;
                     double a[NIT][2];
 #pragma unroll
                     for (int it = 0; it < NIT; it++) {
@@ -2791,13 +2991,14 @@ __device__ __forceinline__ void rows_class_mfma_c(const TS *arena, const RowClas
     constexpr int ZPT  = RPASS_C * SL / WG;
     constexpr bool PREFETCH_A = S == 4;
     const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
-    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);    // columns read (vector width)
+    const int Cs = (C + rc.salign - 1) & ~(rc.salign - 1);  // row stride (128-byte rows)
     if (Cp == 0)
         return;
     const TS *A    = arena + 2 * rc.rb_aoff[rb];
     const int lane = threadIdx.x & 63, half = lane >> 5;
     const int row  = 32 * t + (lane & 31);
-    const TS *rowp = A + 2 * ((size_t)(row < R ? row : 0) * Cp + q * WCOL);
+    const TS *rowp = A + 2 * ((size_t)(row < R ? row : 0) * Cs + q * WCOL);
     const int zs = threadIdx.x & (SL - 1), zc = threadIdx.x / SL;
     float v[ZPT];
     TS a[NIT][VW], an[PREFETCH_A ? NIT : 1][VW];
@@ -3436,7 +3637,8 @@ __device__ __forceinline__ void rows_class_c(const TS *arena, const RowClass &rc
     constexpr int W     = Vec<TS>::W;
     constexpr int ZCOLS = ZCAP_C / MU;
     const int c0 = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - c0;
-    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);    // columns read (vector width)
+    const int Cs = (C + rc.salign - 1) & ~(rc.salign - 1);  // row stride (128-byte rows)
     const TS *A  = arena + 2 * rc.rb_aoff[rb];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int cc = 0; cc < Cp; cc += ZCOLS) {
@@ -3464,7 +3666,7 @@ __device__ __forceinline__ void rows_class_c(const TS *arena, const RowClass &rc
             for (int s = 0; s < MU; s++)
                 acc[s] = cpair<R>{0, 0};
             if (r < R_) {
-                const TS *row = A + 2 * ((size_t)r * Cp + cc);
+                const TS *row = A + 2 * ((size_t)r * Cs + cc);
                 for (int q = sl; q < nvec; q += lpr)
                     cmac_vec<TS, R, MU>(row + (size_t)q * W, s_z + (size_t)q * (W / 2) * MU * 2, acc);
             }
@@ -3522,7 +3724,8 @@ __global__ void __launch_bounds__(WG) k_rows_mfma64c(const int *rb_order, const 
         if (!arena)
             continue;
         const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
-        const int Cp = (C + rc.align - 1) & ~(rc.align - 1);
+        const int Cp = (C + rc.align - 1) & ~(rc.align - 1);    // columns read (vector width)
+        const int Cs = (C + rc.salign - 1) & ~(rc.salign - 1);  // row stride (128-byte rows)
         const double *A = arena + 2 * rc.rb_aoff[rb];
         for (int cc = 0; cc < Cp; cc += RPASS_C64) {
             const int nz = min(RPASS_C64, Cp - cc);
@@ -3549,7 +3752,7 @@ __global__ void __launch_bounds__(WG) k_rows_mfma64c(const int *rb_order, const 
                 const int t = w + 4 * tt;
                 if (t < T16) {
                     const int row    = 16 * t + r16;
-                    const double *rp = A + 2 * ((size_t)(row < R ? row : 0) * Cp + cc);
+                    const double *rp = A + 2 * ((size_t)(row < R ? row : 0) * Cs + cc);
                     double a[NIT][2];
 #pragma unroll
                     for (int it = 0; it < NIT; it++) {

@@ -160,7 +160,8 @@ struct RowClass {
     const int *rb_col0;
     const uint64_t *rb_aoff;
     const int *colidx;
-    int align; // row stride = columns rounded up to a multiple of align (4, or 8 for class 1)
+    int align;  // columns read: the row's columns rounded up to a multiple of align (4, or 8 for class 1)
+    int salign; // row stride: the columns rounded up to a multiple of salign (rows start on 128-byte lines)
 };
 template <typename TS0, typename TS1, typename TC>
 void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, const TS0 *arena0, RowClass c1,
