@@ -575,7 +575,8 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
     HTG_HIP(hipGetLastError());
 }
 
-// Complex product: the same layout with (re, im) pairs; at most 16 RHS per pass.
+// Complex product: the same layout with (re, im) pairs; at most 32 RHS per pass
+// (FP64 arithmetic) or 16 (FP32 arithmetic).
 template <typename TS0, typename TS1, typename R>
 void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t st, hipEvent_t *ev) {
     R *z                 = reinterpret_cast<R *>(h.d_z.p);
@@ -585,11 +586,12 @@ void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream
     const RowClass rc1{h.cls[1].d_rb_col0.p, h.cls[1].d_rb_aoff.p, h.cls[1].d_colidx.p, CLASS_ALIGN[1], ROW_ALIGN[1]};
     int done = 0;
     while (done < mu) {
-        const int MU = pick_mu_chunk(mu - done, 16);
+        // FP64: 32 complex RHS per pass (every A entry feeds 8 MFMAs); FP32 arithmetic: 16
+        const int MU = pick_mu_chunk(mu - done, std::is_same<R, double>::value ? 32 : 16);
         if (ev)
             HTG_HIP(hipEventRecord(ev[0], st));
-        if (std::is_same<R, double>::value && MU == 16) {
-            launch_gather_xc64(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, reinterpret_cast<double *>(z), st);
+        if (std::is_same<R, double>::value && (MU == 16 || MU == 32)) {
+            launch_gather_xc64(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, MU, reinterpret_cast<double *>(z), st);
         } else if (std::is_same<R, float>::value && MU == 16) {
             launch_gather_xc32(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, reinterpret_cast<float *>(z), st);
         } else {

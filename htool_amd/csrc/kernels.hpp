@@ -142,7 +142,7 @@ void launch_gather_x(const double *x, const int *perm_s, int nc, int tpad, int m
 // the inverse of perm_s, stage holds nc x 32 floats.
 void launch_gather_xc32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *stage, float *z,
                         hipStream_t st);
-void launch_gather_xc64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, double *z,
+void launch_gather_xc64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, int MU, double *z,
                         hipStream_t st);
 void launch_gather_x64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, double *z, hipStream_t st);
 void launch_gather_x32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *stage, float *z,
