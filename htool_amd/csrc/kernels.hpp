@@ -78,9 +78,11 @@ inline bool aca_int32_wrap_differs(int M, int N, int k) {
 // Column chunk of a low-rank leaf's V (k x n, row-major, row stride ld). Each
 // stored row covers the global columns [oj & ~(a-1), round_a(oj + n)), zero
 // outside the leaf (a = 4 for class 0, 8 for class 1), so stored column c
-// multiplies z entry (oj & ~(a-1)) + c and both are 16-byte aligned. A chunk is
-// k sums over ncols (a multiple of a) stored columns starting at z entry x_off;
-// chunks are grouped per wavefront.
+// multiplies z entry (oj & ~(a-1)) + c and both are 16-byte aligned; the row
+// stride is that extent rounded up to 32 (class 0) or 64 (class 1) entries, so
+// every stored row starts on a 128-byte line. A chunk is k sums over ncols (a
+// multiple of a) stored columns starting at z entry x_off; chunks are grouped
+// per wavefront.
 struct VTile {
     uint64_t v_off; // first entry of the chunk in the V arena
     int k, ncols, x_off, ld;

@@ -367,7 +367,7 @@ int htg_hmatrix_mvprod_subrhs(htg_hmatrix *h, const double *in, double *out, int
                               int margin);
 
 /* The dense matrix of the leaves (nr x nc column-major): in the cluster
- * ordering (HMatrix::get_local_dense, hmatrix.hpp:1728-1757) or, with
+ * ordering (HMatrix::get_local_dense, hmatrix.hpp:1729-1758) or, with
  * original_order, in the original one (copy_local_dense_perm). Symmetric and
  * Hermitian storage: both triangles. Host memory of nr * nc entries. */
 int htg_hmatrix_get_dense(const htg_hmatrix *h, double *out, int original_order);
