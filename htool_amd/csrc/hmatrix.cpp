@@ -41,6 +41,10 @@ inline int v_extent(const Leaf &L, int c) { return round_up(L.oj + L.n, CLASS_AL
 inline int v_stride(const Leaf &L, int c) { return round_up(v_extent(L, c), ROW_ALIGN[c]); }
 inline int terms(const Leaf &L, int c) { return L.k <= 0 ? 0 : (c == 0 ? L.k_hi : L.k - L.k_hi); }
 
+// Entries per 32-RHS row-range work item (V phase): the column range of a
+// range of rows is cut so that items hold about this many.
+constexpr int RR_WORK = 32768;
+
 // Rows per workgroup of the single-RHS row phase with two precision classes (a
 // row block of up to rb_max rows is cut into parts of at most this many).
 constexpr int ROWS_SPLIT = 32;
@@ -127,12 +131,16 @@ struct htg_hmatrix {
     // hand side in FP64); vt[1] keeps every panel no wider than vchunk_el whole
     // and shrinks its row groups instead (single right-hand side in FP32
     // arithmetic); vt[2] is vt[0] with chunks of twice vchunk_el entries
-    // (several right-hand sides: longer matrix-core passes per chunk).
-    VTileList vt[3];
+    // (several right-hand sides: longer matrix-core passes per chunk). vt[3]
+    // and vt[4] are the 32-RHS row ranges (FP32 arithmetic, FP64): at most
+    // RR_ITEM_ROWS rows of a panel's class over a column range of about
+    // RR_WORK entries, one workgroup each, heaviest first (vg_off unused).
+    VTileList vt[5];
 
     DevArray<int> d_perm_t, d_perm_s, d_rb_row0, d_rb_order, d_rb_parts;
     DevArray<double> d_tscale;
     DevArray<double> d_z, d_part; // z: cluster-ordered x (tpad entries) then t, per RHS
+    DevArray<float> d_xmax;       // 32-RHS FP32 passes: largest |x| per 32-entry block of z
     DevArray<double> d_hx, d_hy;  // staging for host-pointer products
     DevArray<int> d_iperm_s;      // inverse column permutation (32-RHS gather)
     DevArray<float> d_xstage;     // 32-RHS gather staging, nc x 32
@@ -463,6 +471,66 @@ void build_layout(htg_hmatrix &h) {
         }
         h.part_size = std::max(h.part_size, part_size); // the cuts never run at once
     }
+    // 32-RHS row ranges: rows [r0, r0 + RR_ITEM_ROWS) of a panel's class, their
+    // columns cut into ranges of whole staged chunks holding about RR_WORK
+    // entries each (fewer, longer ranges for fewer rows).
+    for (int cut = 3; cut < 5; cut++) {
+        VTileList &T = h.vt[cut];
+        T.tiles.clear();
+        T.multi.clear();
+        T.vg_off.assign(1, 0);
+        const int cw  = cut == 3 ? RR_ITEM_CHUNK32 : RR_ITEM_CHUNK64;
+        int part_size = 0;
+        for (int c = 0; c < 2; c++) {
+            const ClassLayout &C = h.cls[c];
+            for (const auto &P : h.panels) {
+                int K = 0, first = -1;
+                for (int l : P)
+                    if (terms(h.leaves[l], c) > 0) {
+                        if (first < 0)
+                            first = l;
+                        K += terms(h.leaves[l], c);
+                    }
+                if (K == 0)
+                    continue;
+                const Leaf &L0   = h.leaves[first];
+                const int ext    = v_extent(L0, c);
+                const int ld     = v_stride(L0, c);
+                const int tfirst = t_entry(h, first, c == 0 ? 0 : L0.k_hi);
+                for (int r0 = 0; r0 < K; r0 += RR_ITEM_ROWS) {
+                    const int rows = std::min(RR_ITEM_ROWS, K - r0);
+                    int width      = std::max(cw, round_up((RR_WORK + rows - 1) / rows, cw));
+                    const int np0  = (ext + width - 1) / width;
+                    width          = round_up((ext + np0 - 1) / np0, cw);
+                    const int np   = (ext + width - 1) / width;
+                    const int pbase = part_size;
+                    if (np > 1)
+                        part_size += np * rows;
+                    for (int q = 0; q < np; q++) {
+                        VTile v{};
+                        v.v_off     = C.leaf_v_off[first] + (uint64_t)r0 * ld + (uint64_t)q * width;
+                        v.ld        = ld;
+                        v.k         = rows;
+                        v.ncols     = std::min(width, ext - q * width);
+                        v.x_off     = v_first(L0, c) + q * width;
+                        v.t_index   = tfirst + r0;
+                        v.part_off  = pbase + q * rows;
+                        v.part_base = pbase;
+                        v.nparts    = np;
+                        v.cls       = c;
+                        if (np > 1 && q == 0)
+                            T.multi.push_back(v);
+                        T.tiles.push_back(v);
+                    }
+                }
+            }
+        }
+        // heaviest first (16-bit rows count half)
+        std::stable_sort(T.tiles.begin(), T.tiles.end(), [](const VTile &a, const VTile &b) {
+            return (int64_t)a.k * a.ncols * (a.cls == 0 ? 2 : 1) > (int64_t)b.k * b.ncols * (b.cls == 0 ? 2 : 1);
+        });
+        h.part_size = std::max(h.part_size, part_size);
+    }
 }
 
 void upload_layout(htg_hmatrix &h) {
@@ -488,6 +556,7 @@ void upload_layout(htg_hmatrix &h) {
     h.d_z.alloc((size_t)(h.tpad + h.t_size) * 64);
     h.d_z.zero();
     h.d_part.alloc((size_t)std::max(h.part_size, 1) * 64);
+    h.d_xmax.alloc((size_t)h.tpad / 32 + 1);
     // multi-RHS gathers: the inverse column permutation and an nc x 32 FP32
     // staging buffer, allocated here so that products allocate nothing
     std::vector<int> ip(h.nc);
@@ -546,7 +615,8 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
         if (ev)
             HTG_HIP(hipEventRecord(ev[0], st));
         if (std::is_same<TC, float>::value && MU == 32) {
-            launch_gather_x32(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, reinterpret_cast<float *>(z), st);
+            launch_gather_x32(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, reinterpret_cast<float *>(z),
+                              h.d_xmax.p, st);
         } else if (std::is_same<TC, double>::value && MU == 32) {
             launch_gather_x64(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, reinterpret_cast<double *>(z), st);
         } else {
@@ -555,10 +625,17 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
         if (ev)
             HTG_HIP(hipEventRecord(ev[1], st));
         const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
-        const VTileList &T = h.vt[MU > 1 ? 2 : std::is_same<TC, float>::value ? 1 : 0];
-        launch_vx<TS0, TS1, TC>(T.d_vg_off.p, (int)T.vg_off.size() - 1, T.d_tiles.p, T.d_multi.p,
-                                (int)T.multi.size(), arena_of<TS0>(C0, true),
-                                has1 ? arena_of<TS1>(C1, true) : nullptr, z, h.d_part.p, tscale, MU, st);
+        if (MU == 32) {
+            const VTileList &S = h.vt[std::is_same<TC, float>::value ? 3 : 4];
+            launch_vx_rr<TS0, TS1, TC>(S.d_tiles.p, (int)S.tiles.size(), S.d_multi.p, (int)S.multi.size(),
+                                       arena_of<TS0>(C0, true), has1 ? arena_of<TS1>(C1, true) : nullptr, z,
+                                       h.d_xmax.p, h.d_part.p, tscale, st);
+        } else {
+            const VTileList &T = h.vt[MU > 1 ? 2 : std::is_same<TC, float>::value ? 1 : 0];
+            launch_vx<TS0, TS1, TC>(T.d_vg_off.p, (int)T.vg_off.size() - 1, T.d_tiles.p, T.d_multi.p,
+                                    (int)T.multi.size(), arena_of<TS0>(C0, true),
+                                    has1 ? arena_of<TS1>(C1, true) : nullptr, z, h.d_part.p, tscale, MU, st);
+        }
         if (ev)
             HTG_HIP(hipEventRecord(ev[2], st));
         // row-block parts: measured faster with two precision classes, slower

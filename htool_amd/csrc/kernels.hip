@@ -1435,18 +1435,33 @@ __global__ void k_scatter_x64(const double *x, const int *iperm_s, int nc, int t
         o[u] = make_double2(v[2 * u], v[2 * u + 1]);
 }
 
-__global__ void k_block_x32(const float *stage, int nc, int tpad, float *z) {
+// Workgroup b (256 threads, 32 entries) also writes the largest |x| of its
+// entries to xmax[b] when xmax is given (the FP16 scale of the 32-RHS V phase).
+__global__ void __launch_bounds__(256) k_block_x32(const float *stage, int nc, int tpad, float *z, float *xmax) {
+    __shared__ float s_m[4];
     const int e = blockIdx.x * blockDim.x + threadIdx.x; // (group, RHS)
-    if (e >= tpad * 8)
-        return;
-    const int g = e >> 5, s = e & 31;
-    float v[4];
+    float m     = 0.0f;
+    if (e < tpad * 8) {
+        const int g = e >> 5, s = e & 31;
+        float v[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int j = 4 * g + c;
-        v[c]        = j < nc ? stage[(size_t)j * 32 + s] : 0.0f;
+        for (int c = 0; c < 4; c++) {
+            const int j = 4 * g + c;
+            v[c]        = j < nc ? stage[(size_t)j * 32 + s] : 0.0f;
+            m           = fmaxf(m, fabsf(v[c]));
+        }
+        reinterpret_cast<float4 *>(z)[e] = make_float4(v[0], v[1], v[2], v[3]);
     }
-    reinterpret_cast<float4 *>(z)[e] = make_float4(v[0], v[1], v[2], v[3]);
+    if (xmax) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1)
+            m = fmaxf(m, __shfl_xor(m, o, 64));
+        if ((threadIdx.x & 63) == 0)
+            s_m[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            xmax[blockIdx.x] = fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]));
+    }
 }
 
 // 16-byte vector loads of the stored entries.
@@ -2120,7 +2135,8 @@ __global__ void __launch_bounds__(WG) k_vx_sum(const VTile *first, int nmulti,
     const VTile d    = first[gi];
     const int km     = d.k * MU;
     const double *pb = part + (size_t)d.part_base * MU;
-    for (int e = threadIdx.x % TPG; e < km; e += TPG) {
+    // gridDim.y workgroups share one group's entries
+    for (int e = threadIdx.x % TPG + TPG * blockIdx.y; e < km; e += TPG * gridDim.y) {
         double v[1] = {0};
         sum_parts<1>(pb + e, km, d.nparts, v);
         const int r = e / MU, s = e % MU;
@@ -2249,6 +2265,7 @@ struct Mma16<__half> {
     static constexpr bool on = true, scaled = true;
     static constexpr int NB = 2;
     using V                 = f16x8;
+    using E                 = _Float16;
     __device__ static _Float16 el(__half x, bool in) { return in ? __builtin_bit_cast(_Float16, x) : (_Float16)0.0f; }
     __device__ static V a(const __half (&x)[8], bool in) {
         V v;
@@ -2257,13 +2274,19 @@ struct Mma16<__half> {
             v[j] = in ? __builtin_bit_cast(_Float16, x[j]) : (_Float16)0.0f;
         return v;
     }
+    __device__ static void split1(float b, float sc, E (&o)[NB]) {
+        const float t    = b * sc;
+        const _Float16 h = (_Float16)t;
+        o[0]             = h;
+        o[1]             = (_Float16)(t - (float)h);
+    }
     __device__ static void split(const float (&b)[8], float sc, V (&o)[NB]) {
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const float t    = b[j] * sc;
-            const _Float16 h = (_Float16)t;
-            o[0][j]          = h;
-            o[1][j]          = (_Float16)(t - (float)h);
+            E e[NB];
+            split1(b[j], sc, e);
+            o[0][j] = e[0];
+            o[1][j] = e[1];
         }
     }
     __device__ static floatx16 mma32(V a, V b, floatx16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
@@ -2274,6 +2297,7 @@ struct Mma16<__hip_bfloat16> {
     static constexpr bool on = true, scaled = false;
     static constexpr int NB = 3;
     using V                 = bf16x8;
+    using E                 = __bf16;
     __device__ static __bf16 el(__hip_bfloat16 x, bool in) { return in ? __builtin_bit_cast(__bf16, x) : (__bf16)0.0f; }
     __device__ static V a(const __hip_bfloat16 (&x)[8], bool in) {
         V v;
@@ -2282,16 +2306,22 @@ struct Mma16<__hip_bfloat16> {
             v[j] = in ? __builtin_bit_cast(__bf16, x[j]) : (__bf16)0.0f;
         return v;
     }
+    __device__ static void split1(float t, float, E (&o)[NB]) {
+        const __bf16 h = (__bf16)t;
+        const float r  = t - (float)h;
+        const __bf16 m = (__bf16)r;
+        o[0]           = h;
+        o[1]           = m;
+        o[2]           = (__bf16)(r - (float)m);
+    }
     __device__ static void split(const float (&b)[8], float, V (&o)[NB]) {
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const float t  = b[j];
-            const __bf16 h = (__bf16)t;
-            const float r  = t - (float)h;
-            const __bf16 m = (__bf16)r;
-            o[0][j]        = h;
-            o[1][j]        = m;
-            o[2][j]        = (__bf16)(r - (float)m);
+            E e[NB];
+            split1(b[j], 1.0f, e);
+            o[0][j] = e[0];
+            o[1][j] = e[1];
+            o[2][j] = e[2];
         }
     }
     __device__ static floatx16 mma32(V a, V b, floatx16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
@@ -2941,6 +2971,323 @@ __global__ void __launch_bounds__(WG) k_vx_mfma64(const int *wv_off, int nwaves,
                     vx_store64(d, r, r16, acc0[i], z, part, tscale);
                     vx_store64(d, r, r16 + 16, acc1[i], z, part, tscale);
                 }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 32-RHS V phase by row ranges. A work item is at most RR_ROWS (128) stacked V
+// rows of one panel in one precision class over a range of its columns; one
+// workgroup each, wave w owning the 16-row tiles w and w + 4 of the range. The
+// workgroup walks the range in chunks of RR_CHUNK columns (RR_CHUNK64 in FP64):
+// each chunk's x entries (all 32 right-hand sides) are staged in LDS, double
+// buffered, the next chunk read into registers while the current one is used.
+// A panel stacks the V rows of every low-rank leaf of its source cluster (107
+// on average at N = 1e5), so each x entry is fetched once per 128 rows rather
+// than once per 16. Each wave streams its tiles' rows in units of four column
+// steps on 16x16 matrix-core tiles, the next unit's entries loaded while the
+// current one runs (register ping-pong; every load is issued unconditionally,
+// so the wait before a unit covers that unit's loads only, and the one barrier
+// per chunk waits on LDS alone). The row sums stay in registers over the range
+// and are stored once at the end: the t entries when the range spans the
+// panel's columns, else partial sums that k_vx_sum adds.
+constexpr int RR_ROWS    = RR_ITEM_ROWS;    // rows per work item (one 16-row tile per wave)
+constexpr int RR_CHUNK   = RR_ITEM_CHUNK32; // columns per staged chunk, FP32 arithmetic
+constexpr int RR_CHUNK64 = RR_ITEM_CHUNK64; // columns per staged chunk, FP64
+static_assert(RR_ROWS == NWAVE * 16, "one 16-row tile per wave");
+
+// LDS bytes of one staged chunk in the form class storage TS multiplies: FP32
+// (blocked by 4 columns) or the 16-bit terms of the split (blocked by 8).
+template <typename TS>
+constexpr int rr_stage_bytes() {
+    if constexpr (Mma16<TS>::on)
+        return Mma16<TS>::NB * RR_CHUNK * MFMA_MU * 2;
+    else
+        return RR_CHUNK * MFMA_MU * 4;
+}
+
+// One wave's walk over a work item: chunk 0 staged, then units u = 0 .. U - 1
+// (UPC per chunk), unit u's matrix entries in one register buffer while unit
+// u - 1 is consumed from the other (every load issued unconditionally, so the wait before a unit
+// covers that unit's loads only). At the first unit of chunk j the next
+// chunk's x is read into registers (xload), after its last unit it is stored
+// to the other LDS buffer (xstore) and the workgroup meets at the chunk's one
+// barrier (which waits on LDS, not on the loads in flight).
+template <int UPC, typename Buf, typename Load, typename Run, typename XLoad, typename XStore>
+__device__ __forceinline__ void rr_walk(int nchunk, Load load, Run run, XLoad xload, XStore xstore) {
+    const int U = nchunk * UPC;
+    Buf A, B;
+    // the first unit's entries are requested before chunk 0 is staged: the two
+    // latencies overlap
+    load(0, A);
+    xload(0);
+    xstore(0);
+    __syncthreads();
+    auto half = [&](int u, const Buf &cur, Buf &nxt) {
+        const int j = u / UPC;
+        if (u % UPC == 0)
+            xload(min(j + 1, nchunk - 1));
+        load(min(u + 1, U - 1), nxt);
+        run(u, cur);
+        if (u % UPC == UPC - 1) {
+            if (j + 1 < nchunk)
+                xstore(j + 1);
+            __syncthreads();
+        }
+    };
+    for (int u = 0; u < U; u += 2) {
+        half(u, A, B);
+        if (u + 1 >= U)
+            break;
+        half(u + 1, B, A);
+    }
+}
+
+// Rows [0, d.k) of a work item (<= 64: wave w owns the 16-row tile w), class
+// storage TS, FP32 arithmetic. Steps are 16 columns for FP32 entries
+// (v_mfma_f32_16x16x4f32: lane (r, g) = (l % 16, l / 16) loads row r's columns
+// 4 g .. 4 g + 3 of the step and feeds them to four MFMAs, k = g in each, B =
+// x of the same column from LDS) and 32 columns for 16-bit entries
+// (v_mfma_f32_16x16x32: lane (r, g) loads columns 8 g .. 8 g + 7 as its A
+// operand, B = the staged 16-bit terms, one MFMA per term); two MFMAs per A
+// operand (right-hand sides 0-15 and 16-31). Every step of a unit runs: past
+// the range's columns the staged x is zero and the clamped loads finite. FP16
+// terms carry a power-of-two scale per chunk from xmax (the largest |x| of
+// each 32-column block of z, written by the gather). Unit sums are FP32, then
+// FP64.
+template <typename TS>
+__device__ __forceinline__ void rr_rows32(const VTile &d, const TS *varena, float *z, const float *xmax,
+                                          unsigned char *lds, double *part, const double *tscale) {
+    constexpr bool H16 = Mma16<TS>::on;
+    constexpr int VW   = Vec<TS>::W;           // entries per 16-byte load
+    constexpr int SC   = H16 ? 32 : 16;        // columns per step
+    constexpr int NS   = 4;                    // steps per unit
+    constexpr int UPC  = RR_CHUNK / (SC * NS); // units per chunk (2 or 1)
+    constexpr int BUF  = rr_stage_bytes<TS>();
+    constexpr int XPT  = RR_CHUNK * MFMA_MU / 4 / WG; // staged float4 per thread (4)
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    struct Buf {
+        u4 a[NS];
+    };
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 15, g = lane >> 4;
+    const int K      = d.k;
+    const int nstep  = (d.ncols + SC - 1) / SC;
+    const int nchunk = (d.ncols + RR_CHUNK - 1) / RR_CHUNK;
+    const int ngrp   = (d.ncols + 3) >> 2;       // 4-column groups of the range
+    const bool real  = 16 * w < K;               // the wave's tile holds rows
+    const TS *rp = varena + d.v_off + (size_t)min(16 * w + r, K - 1) * d.ld + VW * g;
+    auto load = [&](int u, Buf &b) {
+#pragma unroll
+        for (int st = 0; st < NS; st++)
+            b.a[st] = nt_load16(rp + (size_t)min(u * NS + st, nstep - 1) * SC);
+    };
+    const float4 *xsrc = reinterpret_cast<const float4 *>(z + (size_t)(d.x_off >> 2) * 128);
+    float4 xv[XPT];
+    auto xload = [&](int j) {
+#pragma unroll
+        for (int i = 0; i < XPT; i++) {
+            const int f = (int)threadIdx.x + WG * i;
+            xv[i]       = xsrc[(size_t)min(j * (RR_CHUNK / 4) + (f >> 5), ngrp - 1) * MFMA_MU + (f & 31)];
+        }
+    };
+    auto chunk_scale = [&](int j) { // FP16 terms: the chunk's power-of-two scale
+        const int c0 = d.x_off + j * RR_CHUNK, c1 = min(c0 + RR_CHUNK, d.x_off + d.ncols) - 1;
+        float m = 0.0f;
+        for (int b = c0 >> 5; b <= (c1 >> 5); b++)
+            m = fmaxf(m, xmax[b]);
+        return pass_scale(m);
+    };
+    auto xstore = [&](int j) {
+        unsigned char *buf = lds + (j & 1) * BUF;
+        float sc = 1.0f;
+        if constexpr (H16 && Mma16<TS>::scaled)
+            sc = chunk_scale(j);
+#pragma unroll
+        for (int i = 0; i < XPT; i++) {
+            const int f = (int)threadIdx.x + WG * i, grp = f >> 5, s = f & 31;
+            const int c = j * RR_CHUNK + grp * 4; // column of the range
+            float4 v    = xv[i];
+            v.x = c + 0 < d.ncols ? v.x : 0.0f;
+            v.y = c + 1 < d.ncols ? v.y : 0.0f;
+            v.z = c + 2 < d.ncols ? v.z : 0.0f;
+            v.w = c + 3 < d.ncols ? v.w : 0.0f;
+            if constexpr (!H16) {
+                reinterpret_cast<float4 *>(buf)[f] = v;
+            } else {
+                using M  = Mma16<TS>;
+                using EL = typename M::E;
+                const float b[4] = {v.x, v.y, v.z, v.w};
+                EL t[M::NB][4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    EL o[M::NB];
+                    M::split1(b[e], sc, o);
+#pragma unroll
+                    for (int u = 0; u < M::NB; u++)
+                        t[u][e] = o[u];
+                }
+#pragma unroll
+                for (int u = 0; u < M::NB; u++) {
+                    // [term][grp / 2][s][8]: this group is entries (grp % 2) * 4 .. + 3
+                    EL *p = reinterpret_cast<EL *>(buf) +
+                            ((size_t)(u * (RR_CHUNK / 8) + (grp >> 1)) * MFMA_MU + s) * 8 + (grp & 1) * 4;
+                    typedef EL el4 __attribute__((ext_vector_type(4)));
+                    el4 wv;
+                    wv[0] = t[u][0], wv[1] = t[u][1], wv[2] = t[u][2], wv[3] = t[u][3];
+                    *reinterpret_cast<el4 *>(p) = wv;
+                }
+            }
+        }
+    };
+    double accd[2][4] = {}; // [RHS half][row]
+    auto run = [&](int u, const Buf &b) {
+        if (!real)
+            return;
+        const int j = u / UPC, s = u % UPC;
+        const unsigned char *buf = lds + (j & 1) * BUF;
+        floatx4 acc0 = {0}, acc1 = {0};
+#pragma unroll
+        for (int st = 0; st < NS; st++) {
+            const int cs = s * NS + st; // step within the chunk
+            if constexpr (H16) {
+                using M = Mma16<TS>;
+                const typename M::V av = __builtin_bit_cast(typename M::V, b.a[st]);
+#pragma unroll
+                for (int t = 0; t < M::NB; t++) {
+                    const typename M::V *bp = reinterpret_cast<const typename M::V *>(buf) +
+                                              (size_t)(t * (RR_CHUNK / 8) + cs * (SC / 8) + g) * MFMA_MU;
+                    acc0 = M::mma16(av, bp[r], acc0);
+                    acc1 = M::mma16(av, bp[r + 16], acc1);
+                }
+            } else {
+                const float4 *bp = reinterpret_cast<const float4 *>(buf) + (size_t)(cs * 4 + g) * MFMA_MU;
+                const float4 b0 = bp[r], b1 = bp[r + 16];
+                const float av[4] = {__uint_as_float(b.a[st].x), __uint_as_float(b.a[st].y),
+                                     __uint_as_float(b.a[st].z), __uint_as_float(b.a[st].w)};
+                const float bl[4] = {b0.x, b0.y, b0.z, b0.w}, bh[4] = {b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[m], bl[m], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[m], bh[m], acc1, 0, 0, 0);
+                }
+            }
+        }
+        double inv = 1.0;
+        if constexpr (H16 && Mma16<TS>::scaled)
+            inv = 1.0 / (double)chunk_scale(j);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            accd[0][i] += (double)acc0[i] * inv;
+            accd[1][i] += (double)acc1[i] * inv;
+        }
+    };
+    rr_walk<UPC, Buf>(nchunk, load, run, xload, xstore);
+    // lane (r, g) holds rows 4 g + i of the tile, right-hand sides r and r + 16
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int row = 16 * w + 4 * g + i;
+        if (row < K) {
+            if (d.nparts > 1) {
+                part[(size_t)(d.part_off + row) * MFMA_MU + r]      = accd[0][i];
+                part[(size_t)(d.part_off + row) * MFMA_MU + r + 16] = accd[1][i];
+            } else {
+                const double f = tscale ? tscale[d.t_index + row] : 1.0;
+                z[zblk(d.t_index + row, r)]      = (float)(accd[0][i] * f);
+                z[zblk(d.t_index + row, r + 16)] = (float)(accd[1][i] * f);
+            }
+        }
+    }
+}
+
+template <typename TS0, typename TS1>
+__global__ void __launch_bounds__(WG) k_vx_rr(const VTile *items, const TS0 *varena0, const TS1 *varena1, float *z,
+                                              const float *xmax, double *part, const double *tscale) {
+    constexpr int B0 = rr_stage_bytes<TS0>(), B1 = rr_stage_bytes<TS1>();
+    __shared__ __align__(16) unsigned char lds[2 * (B0 > B1 ? B0 : B1)];
+    const VTile d = items[blockIdx.x];
+    if (d.cls == 0)
+        rr_rows32<TS0>(d, varena0, z, xmax, lds, part, tscale);
+    else
+        rr_rows32<TS1>(d, varena1, z, xmax, lds, part, tscale);
+}
+
+// FP64 work items: x staged RHS-interleaved as in z ([column][32]), chunks of
+// RR_CHUNK64 columns; units of four 8-column steps (v_mfma_f64_16x16x4f64:
+// lane (r, g) loads columns 2 g, 2 g + 1 of the step, MFMA m takes column
+// 2 g + m as k = g; result register i of lane (r, g) holds row 4 i + g). Sums
+// are FP64 throughout; the walk is the FP32 one.
+__global__ void __launch_bounds__(WG) k_vx_rr64(const VTile *items, const double *varena0, const double *varena1,
+                                                double *z, double *part, const double *tscale) {
+    constexpr int MU = 32, NS = 4, SC = 8, UPC = RR_CHUNK64 / (SC * NS), XPT = RR_CHUNK64 * MU / 2 / WG;
+    __shared__ __align__(16) double s_x[2][RR_CHUNK64 * MU];
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    struct Buf {
+        u4 a[NS];
+    };
+    const VTile d = items[blockIdx.x];
+    const double *varena = d.cls == 0 ? varena0 : varena1;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 15, g = lane >> 4;
+    const int K      = d.k;
+    const int nstep  = (d.ncols + SC - 1) / SC;
+    const int nchunk = (d.ncols + RR_CHUNK64 - 1) / RR_CHUNK64;
+    const bool real  = 16 * w < K;
+    const double *rp = varena + d.v_off + (size_t)min(16 * w + r, K - 1) * d.ld + 2 * g;
+    auto load = [&](int u, Buf &b) {
+#pragma unroll
+        for (int st = 0; st < NS; st++)
+            b.a[st] = nt_load16(rp + (size_t)min(u * NS + st, nstep - 1) * SC);
+    };
+    const double2 *xsrc = reinterpret_cast<const double2 *>(z + (size_t)d.x_off * MU);
+    double2 xv[XPT];
+    auto xload = [&](int j) {
+#pragma unroll
+        for (int i = 0; i < XPT; i++) {
+            const int f = (int)threadIdx.x + WG * i; // column j * RR_CHUNK64 + f / 16 of the range, RHS pair f % 16
+            xv[i] = xsrc[(size_t)min(j * RR_CHUNK64 + (f >> 4), d.ncols - 1) * (MU / 2) + (f & 15)];
+        }
+    };
+    auto xstore = [&](int j) {
+        double2 *buf = reinterpret_cast<double2 *>(s_x[j & 1]);
+#pragma unroll
+        for (int i = 0; i < XPT; i++) {
+            const int f = (int)threadIdx.x + WG * i;
+            buf[f] = j * RR_CHUNK64 + (f >> 4) < d.ncols ? xv[i] : make_double2(0.0, 0.0);
+        }
+    };
+    doublex4 acc0 = {0}, acc1 = {0};
+    auto run = [&](int u, const Buf &b) {
+        if (!real)
+            return;
+        const int j = u / UPC, s = u % UPC;
+        const double *buf = s_x[j & 1];
+#pragma unroll
+        for (int st = 0; st < NS; st++) {
+            const int cs = s * NS + st;
+            const double av[2] = {__hiloint2double((int)b.a[st].y, (int)b.a[st].x),
+                                  __hiloint2double((int)b.a[st].w, (int)b.a[st].z)};
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                const double *bp = buf + (size_t)(cs * SC + 2 * g + m) * MU;
+                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bp[r], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bp[r + 16], acc1, 0, 0, 0);
+            }
+        }
+    };
+    rr_walk<UPC, Buf>(nchunk, load, run, xload, xstore);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int row = 16 * w + 4 * i + g;
+        if (row < K) {
+            if (d.nparts > 1) {
+                part[(size_t)(d.part_off + row) * MU + r]      = acc0[i];
+                part[(size_t)(d.part_off + row) * MU + r + 16] = acc1[i];
+            } else {
+                const double f = tscale ? tscale[d.t_index + row] : 1.0;
+                z[(size_t)(d.t_index + row) * MU + r]      = acc0[i] * f;
+                z[(size_t)(d.t_index + row) * MU + r + 16] = acc1[i] * f;
             }
         }
     }
@@ -4223,10 +4570,10 @@ void launch_gather_x64(const double *x, const int *iperm_s, int nc, int tpad, in
 }
 
 void launch_gather_x32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *stage, float *z,
-                       hipStream_t st) {
+                       float *xmax, hipStream_t st) {
     hipLaunchKernelGGL(k_scatter_x32, dim3((nc + WG - 1) / WG, MFMA_MU / 8), dim3(WG), 0, st, x, iperm_s, nc, mu_base,
                        stage);
-    hipLaunchKernelGGL(k_block_x32, dim3((tpad * 8 + WG - 1) / WG), dim3(WG), 0, st, stage, nc, tpad, z);
+    hipLaunchKernelGGL(k_block_x32, dim3((tpad * 8 + WG - 1) / WG), dim3(WG), 0, st, stage, nc, tpad, z, xmax);
 }
 
 template <typename TS0, typename TS1, typename TC>
@@ -4267,6 +4614,26 @@ void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const VTile *
     } while (0)
     HTG_MU_SWITCH(MU, HTG_VX)
 #undef HTG_VX
+}
+
+template <typename TS0, typename TS1, typename TC>
+void launch_vx_rr(const VTile *items, int nitems, const VTile *multi_tiles, int nmulti, const TS0 *varena0,
+                  const TS1 *varena1, TC *z, const float *xmax, double *part, const double *tscale, hipStream_t st) {
+    if (nitems <= 0)
+        return;
+    if constexpr (std::is_same<TC, double>::value) {
+        static_assert(std::is_same<TS0, double>::value && std::is_same<TS1, double>::value, "FP64 items");
+        hipLaunchKernelGGL(k_vx_rr64, dim3(nitems), dim3(WG), 0, st, items, varena0, varena1, z, part, tscale);
+        if (nmulti > 0) // a range's rows x 32 sums (<= 4096) over four workgroups
+            hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU>), dim3(nmulti, 4), dim3(WG), 0, st, multi_tiles, nmulti, part,
+                               z, tscale);
+    } else {
+        hipLaunchKernelGGL((k_vx_rr<TS0, TS1>), dim3(nitems), dim3(WG), 0, st, items, varena0, varena1, z, xmax,
+                           part, tscale);
+        if (nmulti > 0)
+            hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU, true>), dim3(nmulti, 4), dim3(WG), 0, st, multi_tiles, nmulti,
+                               part, z, tscale);
+    }
 }
 
 template <typename TS0, typename TS1, typename TC>
@@ -4325,7 +4692,7 @@ void launch_gather_xc32(const double *x, const int *iperm_s, int nc, int tpad, i
                         hipStream_t st) {
     hipLaunchKernelGGL(k_scatter_xc32, dim3((nc + WG - 1) / WG, MFMA_MU_C / 8), dim3(WG), 0, st, x, iperm_s, nc,
                        mu_base, stage);
-    hipLaunchKernelGGL(k_block_x32, dim3((tpad * 8 + WG - 1) / WG), dim3(WG), 0, st, stage, nc, tpad, z);
+    hipLaunchKernelGGL(k_block_x32, dim3((tpad * 8 + WG - 1) / WG), dim3(WG), 0, st, stage, nc, tpad, z, nullptr);
 }
 
 void launch_gather_xc64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, int MU, double *z,
@@ -4445,6 +4812,17 @@ template void launch_vx<float, __half, float>(const int *, int, const VTile *, c
 template void launch_vx<float, __hip_bfloat16, float>(const int *, int, const VTile *, const VTile *, int,
                                                       const float *, const __hip_bfloat16 *, float *, double *,
                                                       const double *, int, hipStream_t);
+template void launch_vx_rr<double, double, double>(const VTile *, int, const VTile *, int, const double *,
+                                                   const double *, double *, const float *, double *, const double *,
+                                                   hipStream_t);
+template void launch_vx_rr<float, float, float>(const VTile *, int, const VTile *, int, const float *, const float *,
+                                                float *, const float *, double *, const double *, hipStream_t);
+template void launch_vx_rr<float, __half, float>(const VTile *, int, const VTile *, int, const float *,
+                                                 const __half *, float *, const float *, double *, const double *,
+                                                 hipStream_t);
+template void launch_vx_rr<float, __hip_bfloat16, float>(const VTile *, int, const VTile *, int, const float *,
+                                                         const __hip_bfloat16 *, float *, const float *, double *,
+                                                         const double *, hipStream_t);
 template void launch_rows<double, double, double>(const int *, const int *, int, RowClass, const double *, RowClass,
                                                   const double *, const double *, const int *, int, int, int,
                                                   double *, hipStream_t);

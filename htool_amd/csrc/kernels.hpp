@@ -147,13 +147,26 @@ void launch_gather_xc32(const double *x, const int *iperm_s, int nc, int tpad, i
 void launch_gather_xc64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, int MU, double *z,
                         hipStream_t st);
 void launch_gather_x64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, double *z, hipStream_t st);
+// xmax (tpad / 32 + 1 floats): the largest |x| of each 32-entry block of z.
 void launch_gather_x32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *stage, float *z,
-                       hipStream_t st);
+                       float *xmax, hipStream_t st);
 // Both precision classes in one launch: chunk d reads varena0 or varena1 by d.cls.
 template <typename TS0, typename TS1, typename TC>
 void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const VTile *multi_tiles, int nmulti,
                const TS0 *varena0, const TS1 *varena1, TC *z, double *part, const double *tscale, int MU,
                hipStream_t st);
+
+// 32-RHS V phase by row ranges (one workgroup each: at most RR_ITEM_ROWS rows of
+// a panel's class over a range of its columns, a multiple of RR_ITEM_CHUNK32
+// (FP32 arithmetic) or RR_ITEM_CHUNK64 (FP64) columns long except at the
+// panel's end), then the partial sums of ranges that split a panel's columns.
+// xmax: the largest |x| of each 32-entry block of z (launch_gather_x32).
+template <typename TS0, typename TS1, typename TC>
+void launch_vx_rr(const VTile *items, int nitems, const VTile *multi_tiles, int nmulti, const TS0 *varena0,
+                  const TS1 *varena1, TC *z, const float *xmax, double *part, const double *tscale, hipStream_t st);
+constexpr int RR_ITEM_ROWS    = 64;
+constexpr int RR_ITEM_CHUNK32 = 128;
+constexpr int RR_ITEM_CHUNK64 = 64;
 
 // One precision class of the row blocks: per row block its first column in
 // colidx and the offset of its row-major block (row stride = columns rounded
