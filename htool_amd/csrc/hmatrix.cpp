@@ -118,6 +118,9 @@ struct htg_hmatrix {
     // single-RHS row phase with two precision classes: rb_order with every row
     // block cut into parts of at most ROWS_SPLIT rows (entry = rb | p << 24 | np << 28)
     std::vector<int> rb_parts;
+    // 32-RHS row phase: rb_order with every row block cut into parts of at
+    // most RR_ITEM_ROWS rows (entry = rb | p << 24)
+    std::vector<int> rb_items;
     // per leaf, per covered row block: (row block, class-0 tile offset, class-1 tile offset)
     std::vector<std::vector<std::tuple<int, uint64_t, uint64_t>>> leaf_tiles;
     std::vector<std::vector<int>> panels; // low-rank leaves per source cluster
@@ -137,13 +140,14 @@ struct htg_hmatrix {
     // RR_WORK entries, one workgroup each, heaviest first (vg_off unused).
     VTileList vt[5];
 
-    DevArray<int> d_perm_t, d_perm_s, d_rb_row0, d_rb_order, d_rb_parts;
+    DevArray<int> d_perm_t, d_perm_s, d_rb_row0, d_rb_order, d_rb_parts, d_rb_items;
     DevArray<double> d_tscale;
     DevArray<double> d_z, d_part; // z: cluster-ordered x (tpad entries) then t, per RHS
     DevArray<float> d_xmax;       // 32-RHS FP32 passes: largest |x| per 32-entry block of z
+    DevArray<unsigned> d_tmax;    // 32-RHS FP32 passes: largest class-1 |t| (FP32 bits)
     DevArray<double> d_hx, d_hy;  // staging for host-pointer products
     DevArray<int> d_iperm_s;      // inverse column permutation (32-RHS gather)
-    DevArray<float> d_xstage;     // 32-RHS gather staging, nc x 32
+    DevArray<float> d_xstage;     // 32-RHS FP32 z, RHS-interleaved: x (gather staging) then t, (tpad + t) x 32
     int storage = HTG_STORAGE_FP64;
     bool cplx   = false; // complex<double> entries, x and y
     // infos["nb_mat_vec_prod"], infos["total_time_mat_vec_prod"] (hmatrix.hpp:1180-1181)
@@ -327,6 +331,10 @@ void build_layout(htg_hmatrix &h) {
     // Single right-hand side with two precision classes: more, shorter
     // workgroups balance the phase better (a row block's columns are staged
     // once per part).
+    h.rb_items.clear();
+    for (int r : h.rb_order)
+        for (int p = 0; p * RR_ITEM_ROWS < h.rb_row0[r + 1] - h.rb_row0[r]; p++)
+            h.rb_items.push_back(r | p << 24);
     h.rb_parts.clear();
     {
         // all parts of a row block on one XCD (workgroup b runs on XCD b % 8),
@@ -539,6 +547,7 @@ void upload_layout(htg_hmatrix &h) {
     h.d_rb_row0.upload(h.rb_row0);
     h.d_rb_order.upload(h.rb_order);
     h.d_rb_parts.upload(h.rb_parts);
+    h.d_rb_items.upload(h.rb_items);
     for (ClassLayout &C : h.cls) {
         C.d_rb_col0.upload(C.rb_col0);
         C.d_rb_aoff.upload(C.rb_aoff);
@@ -563,7 +572,9 @@ void upload_layout(htg_hmatrix &h) {
     for (int j = 0; j < h.nc; j++)
         ip[h.perm_s[j]] = j;
     h.d_iperm_s.upload(ip);
-    h.d_xstage.alloc((size_t)h.nc * 32);
+    h.d_xstage.alloc((size_t)(h.tpad + h.t_size) * 32);
+    h.d_xstage.zero(); // padding entries [nc, tpad) stay zero
+    h.d_tmax.alloc(TMAX_SLOTS);
     h.generation->fetch_add(1); // graphs captured before read freed buffers
 }
 
@@ -616,7 +627,7 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
             HTG_HIP(hipEventRecord(ev[0], st));
         if (std::is_same<TC, float>::value && MU == 32) {
             launch_gather_x32(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, reinterpret_cast<float *>(z),
-                              h.d_xmax.p, st);
+                              h.d_xmax.p, h.d_tmax.p, st);
         } else if (std::is_same<TC, double>::value && MU == 32) {
             launch_gather_x64(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, reinterpret_cast<double *>(z), st);
         } else {
@@ -629,7 +640,7 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
             const VTileList &S = h.vt[std::is_same<TC, float>::value ? 3 : 4];
             launch_vx_rr<TS0, TS1, TC>(S.d_tiles.p, (int)S.tiles.size(), S.d_multi.p, (int)S.multi.size(),
                                        arena_of<TS0>(C0, true), has1 ? arena_of<TS1>(C1, true) : nullptr, z,
-                                       h.d_xmax.p, h.d_part.p, tscale, st);
+                                       h.d_xstage.p, h.d_xmax.p, h.d_tmax.p, h.d_part.p, tscale, st);
         } else {
             const VTileList &T = h.vt[MU > 1 ? 2 : std::is_same<TC, float>::value ? 1 : 0];
             launch_vx<TS0, TS1, TC>(T.d_vg_off.p, (int)T.vg_off.size() - 1, T.d_tiles.p, T.d_multi.p,
@@ -638,13 +649,19 @@ void product(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream_t
         }
         if (ev)
             HTG_HIP(hipEventRecord(ev[2], st));
-        // row-block parts: measured faster with two precision classes, slower
-        // with FP32 alone (one staging pass streams more bytes there)
-        const bool parts = MU == 1 && std::is_same<TC, float>::value && has1;
-        launch_rows<TS0, TS1, TC>(parts ? h.d_rb_parts.p : h.d_rb_order.p, h.d_rb_row0.p,
-                                  parts ? (int)h.rb_parts.size() : (int)h.rb_row0.size() - 1, rc0,
-                                  arena_of<TS0>(C0, false), rc1, has1 ? arena_of<TS1>(C1, false) : nullptr, z,
-                                  h.d_perm_t.p, h.nr, done, MU, d_y, st);
+        if (MU == 32) {
+            launch_rows_rr<TS0, TS1, TC>(h.d_rb_items.p, (int)h.rb_items.size(), h.d_rb_row0.p, rc0,
+                                         arena_of<TS0>(C0, false), rc1, has1 ? arena_of<TS1>(C1, false) : nullptr, z,
+                                         h.d_xstage.p, h.d_tmax.p, h.d_perm_t.p, h.nr, done, d_y, st);
+        } else {
+            // row-block parts: measured faster with two precision classes, slower
+            // with FP32 alone (one staging pass streams more bytes there)
+            const bool parts = MU == 1 && std::is_same<TC, float>::value && has1;
+            launch_rows<TS0, TS1, TC>(parts ? h.d_rb_parts.p : h.d_rb_order.p, h.d_rb_row0.p,
+                                      parts ? (int)h.rb_parts.size() : (int)h.rb_row0.size() - 1, rc0,
+                                      arena_of<TS0>(C0, false), rc1, has1 ? arena_of<TS1>(C1, false) : nullptr, z,
+                                      h.d_perm_t.p, h.nr, done, MU, d_y, st);
+        }
         if (ev)
             HTG_HIP(hipEventRecord(ev[3], st));
         done += MU;

@@ -128,6 +128,13 @@ __device__ __forceinline__ T zero_s() {
     return T{};
 }
 
+// Largest of a float over the wave.
+__device__ __forceinline__ float wave_max(float m) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        m = fmaxf(m, __shfl_xor(m, o, 64));
+    return m;
+}
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
@@ -1436,10 +1443,14 @@ __global__ void k_scatter_x64(const double *x, const int *iperm_s, int nc, int t
 }
 
 // Workgroup b (256 threads, 32 entries) also writes the largest |x| of its
-// entries to xmax[b] when xmax is given (the FP16 scale of the 32-RHS V phase).
-__global__ void __launch_bounds__(256) k_block_x32(const float *stage, int nc, int tpad, float *z, float *xmax) {
+// entries to xmax[b] when xmax is given (the FP16 scale of the 32-RHS V phase),
+// and workgroup 0 clears tmax.
+__global__ void __launch_bounds__(256) k_block_x32(const float *stage, int nc, int tpad, float *z, float *xmax,
+                                                   unsigned *tmax) {
     __shared__ float s_m[4];
     const int e = blockIdx.x * blockDim.x + threadIdx.x; // (group, RHS)
+    if (tmax && e < TMAX_SLOTS)
+        tmax[e] = 0u; // the V phase that follows keeps its largest class-1 |t| here
     float m     = 0.0f;
     if (e < tpad * 8) {
         const int g = e >> 5, s = e & 31;
@@ -2126,7 +2137,8 @@ __device__ __forceinline__ void sum_parts(const double *p, size_t stride, int n,
 
 template <typename TC, int MU, bool BLK = false>
 __global__ void __launch_bounds__(WG) k_vx_sum(const VTile *first, int nmulti,
-                                               const double *part, TC *z, const double *tscale) {
+                                               const double *part, TC *z, const double *tscale,
+                                               unsigned *tmax = nullptr) {
     constexpr int GPB = vx_sum_groups_per_block<MU>();
     constexpr int TPG = WG / GPB; // threads per group
     const int gi      = blockIdx.x * GPB + (int)threadIdx.x / TPG;
@@ -2136,12 +2148,19 @@ __global__ void __launch_bounds__(WG) k_vx_sum(const VTile *first, int nmulti,
     const int km     = d.k * MU;
     const double *pb = part + (size_t)d.part_base * MU;
     // gridDim.y workgroups share one group's entries
+    float m = 0.0f;
     for (int e = threadIdx.x % TPG + TPG * blockIdx.y; e < km; e += TPG * gridDim.y) {
         double v[1] = {0};
         sum_parts<1>(pb + e, km, d.nparts, v);
         const int r = e / MU, s = e % MU;
-        z[BLK ? zblk(d.t_index + r, s) : (size_t)(d.t_index + r) * MU + s] =
-            (TC)(tscale ? v[0] * tscale[d.t_index + r] : v[0]);
+        const TC o  = (TC)(tscale ? v[0] * tscale[d.t_index + r] : v[0]);
+        z[BLK ? zblk(d.t_index + r, s) : (size_t)(d.t_index + r) * MU + s] = o;
+        m = fmaxf(m, fabsf((float)o));
+    }
+    if (tmax && d.cls == 1) { // one group per workgroup here (MU = 32)
+        m = wave_max(m);
+        if ((threadIdx.x & 63) == 0)
+            atomicMax(tmax + ((blockIdx.x + blockIdx.y) & (TMAX_SLOTS - 1)), __float_as_uint(m));
     }
 }
 
@@ -2344,15 +2363,8 @@ __device__ __forceinline__ void deinterleave16(const TS (&p)[8], const TS (&q)[8
     }
 }
 
-// Largest of a float over the wave; power-of-two pass scale for a largest
-// magnitude m (1 when m is zero or not finite: the products then carry the
+// Power-of-two pass scale for a largest magnitude m (1 when m is zero or not finite: the products then carry the
 // infinity or NaN as FP32 ones would).
-__device__ __forceinline__ float wave_max(float m) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1)
-        m = fmaxf(m, __shfl_xor(m, o, 64));
-    return m;
-}
 __device__ __forceinline__ float pass_scale(float m) {
     return m > 0.0f && m <= 3.0e38f ? ldexpf(1.0f, 14 - ilogbf(m)) : 1.0f;
 }
@@ -2993,6 +3005,7 @@ __global__ void __launch_bounds__(WG) k_vx_mfma64(const int *wv_off, int nwaves,
 // and are stored once at the end: the t entries when the range spans the
 // panel's columns, else partial sums that k_vx_sum adds.
 constexpr int RR_ROWS    = RR_ITEM_ROWS;    // rows per work item (one 16-row tile per wave)
+static_assert((TMAX_SLOTS & (TMAX_SLOTS - 1)) == 0, "slots of the largest class-1 |t|: a power of two");
 constexpr int RR_CHUNK   = RR_ITEM_CHUNK32; // columns per staged chunk, FP32 arithmetic
 constexpr int RR_CHUNK64 = RR_ITEM_CHUNK64; // columns per staged chunk, FP64
 static_assert(RR_ROWS == NWAVE * 16, "one 16-row tile per wave");
@@ -3057,8 +3070,9 @@ __device__ __forceinline__ void rr_walk(int nchunk, Load load, Run run, XLoad xl
 // each 32-column block of z, written by the gather). Unit sums are FP32, then
 // FP64.
 template <typename TS>
-__device__ __forceinline__ void rr_rows32(const VTile &d, const TS *varena, float *z, const float *xmax,
-                                          unsigned char *lds, double *part, const double *tscale) {
+__device__ __forceinline__ void rr_rows32(const VTile &d, const TS *varena, const float *z, float *zil,
+                                          const float *xmax, unsigned *tmax, unsigned char *lds, double *part,
+                                          const double *tscale) {
     constexpr bool H16 = Mma16<TS>::on;
     constexpr int VW   = Vec<TS>::W;           // entries per 16-byte load
     constexpr int SC   = H16 ? 32 : 16;        // columns per step
@@ -3184,7 +3198,9 @@ __device__ __forceinline__ void rr_rows32(const VTile &d, const TS *varena, floa
         }
     };
     rr_walk<UPC, Buf>(nchunk, load, run, xload, xstore);
-    // lane (r, g) holds rows 4 g + i of the tile, right-hand sides r and r + 16
+    // lane (r, g) holds rows 4 g + i of the tile, right-hand sides r and r + 16;
+    // t entries go to the RHS-interleaved z the row phase gathers from
+    float m = 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int row = 16 * w + 4 * g + i;
@@ -3194,23 +3210,31 @@ __device__ __forceinline__ void rr_rows32(const VTile &d, const TS *varena, floa
                 part[(size_t)(d.part_off + row) * MFMA_MU + r + 16] = accd[1][i];
             } else {
                 const double f = tscale ? tscale[d.t_index + row] : 1.0;
-                z[zblk(d.t_index + row, r)]      = (float)(accd[0][i] * f);
-                z[zblk(d.t_index + row, r + 16)] = (float)(accd[1][i] * f);
+                const float v0 = (float)(accd[0][i] * f), v1 = (float)(accd[1][i] * f);
+                zil[(size_t)(d.t_index + row) * MFMA_MU + r]      = v0;
+                zil[(size_t)(d.t_index + row) * MFMA_MU + r + 16] = v1;
+                m = fmaxf(m, fmaxf(fabsf(v0), fabsf(v1)));
             }
         }
+    }
+    if (d.cls == 1 && d.nparts == 1) { // the class-1 scale of the row phase
+        m = wave_max(m);
+        if (lane == 0)
+            atomicMax(tmax + (blockIdx.x & (TMAX_SLOTS - 1)), __float_as_uint(m));
     }
 }
 
 template <typename TS0, typename TS1>
-__global__ void __launch_bounds__(WG) k_vx_rr(const VTile *items, const TS0 *varena0, const TS1 *varena1, float *z,
-                                              const float *xmax, double *part, const double *tscale) {
+__global__ void __launch_bounds__(WG) k_vx_rr(const VTile *items, const TS0 *varena0, const TS1 *varena1,
+                                              const float *z, float *zil, const float *xmax, unsigned *tmax,
+                                              double *part, const double *tscale) {
     constexpr int B0 = rr_stage_bytes<TS0>(), B1 = rr_stage_bytes<TS1>();
     __shared__ __align__(16) unsigned char lds[2 * (B0 > B1 ? B0 : B1)];
     const VTile d = items[blockIdx.x];
     if (d.cls == 0)
-        rr_rows32<TS0>(d, varena0, z, xmax, lds, part, tscale);
+        rr_rows32<TS0>(d, varena0, z, zil, xmax, tmax, lds, part, tscale);
     else
-        rr_rows32<TS1>(d, varena1, z, xmax, lds, part, tscale);
+        rr_rows32<TS1>(d, varena1, z, zil, xmax, tmax, lds, part, tscale);
 }
 
 // FP64 work items: x staged RHS-interleaved as in z ([column][32]), chunks of
@@ -3289,6 +3313,248 @@ __global__ void __launch_bounds__(WG) k_vx_rr64(const VTile *items, const double
                 z[(size_t)(d.t_index + row) * MU + r]      = acc0[i] * f;
                 z[(size_t)(d.t_index + row) * MU + r + 16] = acc1[i] * f;
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 32-RHS row phase by row-block parts of at most RR_ROWS rows (wave w owns
+// rows 16 w .. 16 w + 15). Per precision class the part's columns are walked
+// in chunks as in the V phase (rr_walk): the chunk's vector entries are
+// gathered through the column indices from the RHS-interleaved z (16 bytes =
+// four right-hand sides of one entry per load) and staged in LDS in the form
+// the class multiplies, double buffered; the matrix rows stream in pipelined
+// units; both classes add into the same FP64 row sums, stored once to y in
+// original order.
+template <typename TS>
+__device__ __forceinline__ void rb_class32(const RowClass &rc, const TS *arena, int rb, int roff, int R,
+                                           const float *zil, float tsc, unsigned char *lds, double (&accd)[2][4]) {
+    const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
+    if (C == 0)
+        return;
+    constexpr bool H16 = Mma16<TS>::on;
+    constexpr int VW   = Vec<TS>::W;
+    constexpr int SC   = H16 ? 32 : 16;
+    constexpr int NS   = 4;
+    constexpr int UPC  = RR_CHUNK / (SC * NS);
+    constexpr int BUF  = rr_stage_bytes<TS>();
+    constexpr int XPT  = RR_CHUNK * MFMA_MU / 4 / WG; // gathered float4 per thread (4)
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    struct Buf {
+        u4 a[NS];
+    };
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 15, g = lane >> 4;
+    const int Cs     = (C + rc.salign - 1) & ~(rc.salign - 1); // row stride
+    const int nstep  = (C + SC - 1) / SC;
+    const int nchunk = (C + RR_CHUNK - 1) / RR_CHUNK;
+    const bool real  = 16 * w < R;
+    const TS *rp = arena + rc.rb_aoff[rb] + (size_t)(roff + min(16 * w + r, R - 1)) * Cs + VW * g;
+    auto load = [&](int u, Buf &b) {
+#pragma unroll
+        for (int st = 0; st < NS; st++)
+            b.a[st] = nt_load16(rp + (size_t)min(u * NS + st, nstep - 1) * SC);
+    };
+    const float4 *zf = reinterpret_cast<const float4 *>(zil);
+    const int *ci    = rc.colidx + cb;
+    float4 xv[XPT];
+    auto xload = [&](int j) { // float4 f: column f / 8 of the chunk, right-hand sides 4 (f % 8) ..
+        int idx[XPT];
+#pragma unroll
+        for (int i = 0; i < XPT; i++)
+            idx[i] = ci[min(j * RR_CHUNK + (((int)threadIdx.x + WG * i) >> 3), C - 1)];
+#pragma unroll
+        for (int i = 0; i < XPT; i++)
+            xv[i] = zf[(size_t)idx[i] * (MFMA_MU / 4) + (((int)threadIdx.x + WG * i) & 7)];
+    };
+    auto xstore = [&](int j) {
+        unsigned char *buf = lds + (j & 1) * BUF;
+#pragma unroll
+        for (int i = 0; i < XPT; i++) {
+            const int f = (int)threadIdx.x + WG * i, cl = f >> 3, q = f & 7;
+            const bool in = j * RR_CHUNK + cl < C;
+            const float b[4] = {in ? xv[i].x : 0.0f, in ? xv[i].y : 0.0f, in ? xv[i].z : 0.0f, in ? xv[i].w : 0.0f};
+            if constexpr (!H16) {
+                float *bf = reinterpret_cast<float *>(buf);
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    bf[(cl >> 2) * (4 * MFMA_MU) + (4 * q + e) * 4 + (cl & 3)] = b[e];
+            } else {
+                using M  = Mma16<TS>;
+                using EL = typename M::E;
+                EL *be   = reinterpret_cast<EL *>(buf);
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    EL o[M::NB];
+                    M::split1(b[e], tsc, o);
+#pragma unroll
+                    for (int u = 0; u < M::NB; u++)
+                        be[((size_t)(u * (RR_CHUNK / 8) + (cl >> 3)) * MFMA_MU + 4 * q + e) * 8 + (cl & 7)] = o[u];
+                }
+            }
+        }
+    };
+    const double inv = 1.0 / (double)tsc;
+    auto run = [&](int u, const Buf &b) {
+        if (!real)
+            return;
+        const int j = u / UPC, s = u % UPC;
+        const unsigned char *buf = lds + (j & 1) * BUF;
+        floatx4 acc0 = {0}, acc1 = {0};
+#pragma unroll
+        for (int st = 0; st < NS; st++) {
+            const int cs = s * NS + st;
+            if constexpr (H16) {
+                using M = Mma16<TS>;
+                const typename M::V av = __builtin_bit_cast(typename M::V, b.a[st]);
+#pragma unroll
+                for (int t = 0; t < M::NB; t++) {
+                    const typename M::V *bp = reinterpret_cast<const typename M::V *>(buf) +
+                                              (size_t)(t * (RR_CHUNK / 8) + cs * (SC / 8) + g) * MFMA_MU;
+                    acc0 = M::mma16(av, bp[r], acc0);
+                    acc1 = M::mma16(av, bp[r + 16], acc1);
+                }
+            } else {
+                const float4 *bp = reinterpret_cast<const float4 *>(buf) + (size_t)(cs * 4 + g) * MFMA_MU;
+                const float4 b0 = bp[r], b1 = bp[r + 16];
+                const float av[4] = {__uint_as_float(b.a[st].x), __uint_as_float(b.a[st].y),
+                                     __uint_as_float(b.a[st].z), __uint_as_float(b.a[st].w)};
+                const float bl[4] = {b0.x, b0.y, b0.z, b0.w}, bh[4] = {b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[m], bl[m], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[m], bh[m], acc1, 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            accd[0][i] += (double)acc0[i] * inv;
+            accd[1][i] += (double)acc1[i] * inv;
+        }
+    };
+    rr_walk<UPC, Buf>(nchunk, load, run, xload, xstore);
+}
+
+// items[b] = row block | part << 24; the class-1 (16-bit FP16) vector entries
+// carry one power-of-two scale from tmax (the largest class-1 |t|, kept by the
+// V phase).
+template <typename TS0, typename TS1>
+__global__ void __launch_bounds__(WG) k_rows_rr(const int *items, const int *rb_row0, RowClass c0, const TS0 *arena0,
+                                                RowClass c1, const TS1 *arena1, const float *zil,
+                                                const unsigned *tmax, const int *perm_t, int nr, int mu_base,
+                                                double *y) {
+    constexpr int B0 = rr_stage_bytes<TS0>(), B1 = rr_stage_bytes<TS1>();
+    __shared__ __align__(16) unsigned char lds[2 * (B0 > B1 ? B0 : B1)];
+    const int item = items[blockIdx.x], rb = item & 0xffffff, roff = RR_ROWS * (item >> 24);
+    const int r0 = rb_row0[rb] + roff, R = min(RR_ROWS, rb_row0[rb + 1] - r0);
+    double accd[2][4] = {};
+    rb_class32<TS0>(c0, arena0, rb, roff, R, zil, 1.0f, lds, accd);
+    if (arena1) {
+        float tsc = 1.0f;
+        if constexpr (Mma16<TS1>::on && Mma16<TS1>::scaled) {
+            float m = 0.0f; // every wave reduces the slots (lane l: slots l, l + 64, ...)
+            for (int i = threadIdx.x & 63; i < TMAX_SLOTS; i += 64)
+                m = fmaxf(m, __uint_as_float(tmax[i]));
+            tsc = pass_scale(wave_max(m));
+        }
+        rb_class32<TS1>(c1, arena1, rb, roff, R, zil, tsc, lds, accd);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < 4; i++) { // lane (r, g) holds rows 4 g + i, right-hand sides r and r + 16
+        const int row = 16 * w + 4 * g + i;
+        if (row < R) {
+            double *yo = y + perm_t[r0 + row] + (size_t)nr * mu_base;
+            yo[(size_t)nr * r]        = accd[0][i];
+            yo[(size_t)nr * (r + 16)] = accd[1][i];
+        }
+    }
+}
+
+// FP64 row-block parts: entries gathered from the RHS-interleaved FP64 z
+// (16 bytes = two right-hand sides per load), staged [column][32] as in the
+// FP64 V phase; units of four 8-column steps; FP64 sums.
+__global__ void __launch_bounds__(WG) k_rows_rr64(const int *items, const int *rb_row0, RowClass c0,
+                                                  const double *arena0, RowClass c1, const double *arena1,
+                                                  const double *z, const int *perm_t, int nr, int mu_base,
+                                                  double *y) {
+    constexpr int MU = 32, NS = 4, SC = 8, UPC = RR_CHUNK64 / (SC * NS), XPT = RR_CHUNK64 * MU / 2 / WG;
+    __shared__ __align__(16) double s_x[2][RR_CHUNK64 * MU];
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    struct Buf {
+        u4 a[NS];
+    };
+    const int item = items[blockIdx.x], rb = item & 0xffffff, roff = RR_ROWS * (item >> 24);
+    const int r0 = rb_row0[rb] + roff, R = min(RR_ROWS, rb_row0[rb + 1] - r0);
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 15, g = lane >> 4;
+    const bool real = 16 * w < R;
+    const double2 *z2 = reinterpret_cast<const double2 *>(z);
+    doublex4 acc0 = {0}, acc1 = {0};
+    for (int cls = 0; cls < 2; cls++) {
+        const RowClass &rc = cls == 0 ? c0 : c1;
+        const double *arena = cls == 0 ? arena0 : arena1;
+        if (!arena)
+            continue;
+        const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
+        if (C == 0)
+            continue;
+        const int Cs     = (C + rc.salign - 1) & ~(rc.salign - 1);
+        const int nstep  = (C + SC - 1) / SC;
+        const int nchunk = (C + RR_CHUNK64 - 1) / RR_CHUNK64;
+        const double *rp = arena + rc.rb_aoff[rb] + (size_t)(roff + min(16 * w + r, R - 1)) * Cs + 2 * g;
+        const int *ci    = rc.colidx + cb;
+        auto load = [&](int u, Buf &b) {
+#pragma unroll
+            for (int st = 0; st < NS; st++)
+                b.a[st] = nt_load16(rp + (size_t)min(u * NS + st, nstep - 1) * SC);
+        };
+        double2 xv[XPT];
+        auto xload = [&](int j) { // double2 f: column f / 16 of the chunk, right-hand sides 2 (f % 16) ..
+            int idx[XPT];
+#pragma unroll
+            for (int i = 0; i < XPT; i++)
+                idx[i] = ci[min(j * RR_CHUNK64 + (((int)threadIdx.x + WG * i) >> 4), C - 1)];
+#pragma unroll
+            for (int i = 0; i < XPT; i++)
+                xv[i] = z2[(size_t)idx[i] * (MU / 2) + (((int)threadIdx.x + WG * i) & 15)];
+        };
+        auto xstore = [&](int j) {
+            double2 *buf = reinterpret_cast<double2 *>(s_x[j & 1]);
+#pragma unroll
+            for (int i = 0; i < XPT; i++) {
+                const int f = (int)threadIdx.x + WG * i;
+                buf[f] = j * RR_CHUNK64 + (f >> 4) < C ? xv[i] : make_double2(0.0, 0.0);
+            }
+        };
+        auto run = [&](int u, const Buf &b) {
+            if (!real)
+                return;
+            const int j = u / UPC, s = u % UPC;
+            const double *buf = s_x[j & 1];
+#pragma unroll
+            for (int st = 0; st < NS; st++) {
+                const int cs = s * NS + st;
+                const double av[2] = {__hiloint2double((int)b.a[st].y, (int)b.a[st].x),
+                                      __hiloint2double((int)b.a[st].w, (int)b.a[st].z)};
+#pragma unroll
+                for (int m = 0; m < 2; m++) {
+                    const double *bp = buf + (size_t)(cs * SC + 2 * g + m) * MU;
+                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bp[r], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bp[r + 16], acc1, 0, 0, 0);
+                }
+            }
+        };
+        rr_walk<UPC, Buf>(nchunk, load, run, xload, xstore);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) { // result register i of lane (r, g): row 4 i + g
+        const int row = 16 * w + 4 * i + g;
+        if (row < R) {
+            double *yo = y + perm_t[r0 + row] + (size_t)nr * mu_base;
+            yo[(size_t)nr * r]        = acc0[i];
+            yo[(size_t)nr * (r + 16)] = acc1[i];
         }
     }
 }
@@ -4570,10 +4836,10 @@ void launch_gather_x64(const double *x, const int *iperm_s, int nc, int tpad, in
 }
 
 void launch_gather_x32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *stage, float *z,
-                       float *xmax, hipStream_t st) {
+                       float *xmax, unsigned *tmax, hipStream_t st) {
     hipLaunchKernelGGL(k_scatter_x32, dim3((nc + WG - 1) / WG, MFMA_MU / 8), dim3(WG), 0, st, x, iperm_s, nc, mu_base,
                        stage);
-    hipLaunchKernelGGL(k_block_x32, dim3((tpad * 8 + WG - 1) / WG), dim3(WG), 0, st, stage, nc, tpad, z, xmax);
+    hipLaunchKernelGGL(k_block_x32, dim3((tpad * 8 + WG - 1) / WG), dim3(WG), 0, st, stage, nc, tpad, z, xmax, tmax);
 }
 
 template <typename TS0, typename TS1, typename TC>
@@ -4618,7 +4884,8 @@ void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const VTile *
 
 template <typename TS0, typename TS1, typename TC>
 void launch_vx_rr(const VTile *items, int nitems, const VTile *multi_tiles, int nmulti, const TS0 *varena0,
-                  const TS1 *varena1, TC *z, const float *xmax, double *part, const double *tscale, hipStream_t st) {
+                  const TS1 *varena1, TC *z, float *zil, const float *xmax, unsigned *tmax, double *part,
+                  const double *tscale, hipStream_t st) {
     if (nitems <= 0)
         return;
     if constexpr (std::is_same<TC, double>::value) {
@@ -4628,11 +4895,27 @@ void launch_vx_rr(const VTile *items, int nitems, const VTile *multi_tiles, int 
             hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU>), dim3(nmulti, 4), dim3(WG), 0, st, multi_tiles, nmulti, part,
                                z, tscale);
     } else {
-        hipLaunchKernelGGL((k_vx_rr<TS0, TS1>), dim3(nitems), dim3(WG), 0, st, items, varena0, varena1, z, xmax,
-                           part, tscale);
+        hipLaunchKernelGGL((k_vx_rr<TS0, TS1>), dim3(nitems), dim3(WG), 0, st, items, varena0, varena1,
+                           (const float *)z, zil, xmax, tmax, part, tscale);
         if (nmulti > 0)
-            hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU, true>), dim3(nmulti, 4), dim3(WG), 0, st, multi_tiles, nmulti,
-                               part, z, tscale);
+            hipLaunchKernelGGL((k_vx_sum<float, MFMA_MU, false>), dim3(nmulti, 4), dim3(WG), 0, st, multi_tiles,
+                               nmulti, part, zil, tscale, tmax);
+    }
+}
+
+template <typename TS0, typename TS1, typename TC>
+void launch_rows_rr(const int *items, int nitems, const int *rb_row0, RowClass c0, const TS0 *arena0, RowClass c1,
+                    const TS1 *arena1, const TC *z, const float *zil, const unsigned *tmax, const int *perm_t, int nr,
+                    int mu_base, double *y, hipStream_t st) {
+    if (nitems <= 0)
+        return;
+    if constexpr (std::is_same<TC, double>::value) {
+        static_assert(std::is_same<TS0, double>::value && std::is_same<TS1, double>::value, "FP64 parts");
+        hipLaunchKernelGGL(k_rows_rr64, dim3(nitems), dim3(WG), 0, st, items, rb_row0, c0, arena0, c1, arena1, z,
+                           perm_t, nr, mu_base, y);
+    } else {
+        hipLaunchKernelGGL((k_rows_rr<TS0, TS1>), dim3(nitems), dim3(WG), 0, st, items, rb_row0, c0, arena0, c1,
+                           arena1, zil, tmax, perm_t, nr, mu_base, y);
     }
 }
 
@@ -4692,7 +4975,8 @@ void launch_gather_xc32(const double *x, const int *iperm_s, int nc, int tpad, i
                         hipStream_t st) {
     hipLaunchKernelGGL(k_scatter_xc32, dim3((nc + WG - 1) / WG, MFMA_MU_C / 8), dim3(WG), 0, st, x, iperm_s, nc,
                        mu_base, stage);
-    hipLaunchKernelGGL(k_block_x32, dim3((tpad * 8 + WG - 1) / WG), dim3(WG), 0, st, stage, nc, tpad, z, nullptr);
+    hipLaunchKernelGGL(k_block_x32, dim3((tpad * 8 + WG - 1) / WG), dim3(WG), 0, st, stage, nc, tpad, z, nullptr,
+                       nullptr);
 }
 
 void launch_gather_xc64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, int MU, double *z,
@@ -4812,17 +5096,18 @@ template void launch_vx<float, __half, float>(const int *, int, const VTile *, c
 template void launch_vx<float, __hip_bfloat16, float>(const int *, int, const VTile *, const VTile *, int,
                                                       const float *, const __hip_bfloat16 *, float *, double *,
                                                       const double *, int, hipStream_t);
-template void launch_vx_rr<double, double, double>(const VTile *, int, const VTile *, int, const double *,
-                                                   const double *, double *, const float *, double *, const double *,
-                                                   hipStream_t);
-template void launch_vx_rr<float, float, float>(const VTile *, int, const VTile *, int, const float *, const float *,
-                                                float *, const float *, double *, const double *, hipStream_t);
-template void launch_vx_rr<float, __half, float>(const VTile *, int, const VTile *, int, const float *,
-                                                 const __half *, float *, const float *, double *, const double *,
-                                                 hipStream_t);
-template void launch_vx_rr<float, __hip_bfloat16, float>(const VTile *, int, const VTile *, int, const float *,
-                                                         const __hip_bfloat16 *, float *, const float *, double *,
-                                                         const double *, hipStream_t);
+#define HTG_RR_INST(TS0, TS1, TC)                                                                                \
+    template void launch_vx_rr<TS0, TS1, TC>(const VTile *, int, const VTile *, int, const TS0 *, const TS1 *, TC *, \
+                                             float *, const float *, unsigned *, double *, const double *,           \
+                                             hipStream_t);                                                            \
+    template void launch_rows_rr<TS0, TS1, TC>(const int *, int, const int *, RowClass, const TS0 *, RowClass,      \
+                                               const TS1 *, const TC *, const float *, const unsigned *, const int *, \
+                                               int, int, double *, hipStream_t);
+HTG_RR_INST(double, double, double)
+HTG_RR_INST(float, float, float)
+HTG_RR_INST(float, __half, float)
+HTG_RR_INST(float, __hip_bfloat16, float)
+#undef HTG_RR_INST
 template void launch_rows<double, double, double>(const int *, const int *, int, RowClass, const double *, RowClass,
                                                   const double *, const double *, const int *, int, int, int,
                                                   double *, hipStream_t);

@@ -149,7 +149,7 @@ void launch_gather_xc64(const double *x, const int *iperm_s, int nc, int tpad, i
 void launch_gather_x64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, double *z, hipStream_t st);
 // xmax (tpad / 32 + 1 floats): the largest |x| of each 32-entry block of z.
 void launch_gather_x32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *stage, float *z,
-                       float *xmax, hipStream_t st);
+                       float *xmax, unsigned *tmax, hipStream_t st);
 // Both precision classes in one launch: chunk d reads varena0 or varena1 by d.cls.
 template <typename TS0, typename TS1, typename TC>
 void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const VTile *multi_tiles, int nmulti,
@@ -161,10 +161,17 @@ void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const VTile *
 // (FP32 arithmetic) or RR_ITEM_CHUNK64 (FP64) columns long except at the
 // panel's end), then the partial sums of ranges that split a panel's columns.
 // xmax: the largest |x| of each 32-entry block of z (launch_gather_x32).
+// FP32 arithmetic: x is read from the blocked z, the t entries are written to
+// zil (RHS-interleaved z, FP32, (tpad + t) x 32) and the largest class-1 |t|
+// to tmax (TMAX_SLOTS slots, cleared by launch_gather_x32).
 template <typename TS0, typename TS1, typename TC>
 void launch_vx_rr(const VTile *items, int nitems, const VTile *multi_tiles, int nmulti, const TS0 *varena0,
-                  const TS1 *varena1, TC *z, const float *xmax, double *part, const double *tscale, hipStream_t st);
+                  const TS1 *varena1, TC *z, float *zil, const float *xmax, unsigned *tmax, double *part,
+                  const double *tscale, hipStream_t st);
 constexpr int RR_ITEM_ROWS    = 64;
+// The largest class-1 |t| is kept in this many slots (atomic maxima spread
+// over addresses; the row phase takes the largest): tmax holds TMAX_SLOTS.
+constexpr int TMAX_SLOTS = 256;
 constexpr int RR_ITEM_CHUNK32 = 128;
 constexpr int RR_ITEM_CHUNK64 = 64;
 
@@ -182,6 +189,13 @@ template <typename TS0, typename TS1, typename TC>
 void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, const TS0 *arena0, RowClass c1,
                  const TS1 *arena1, const TC *z, const int *perm_t, int nr, int mu_base, int MU, double *y,
                  hipStream_t st);
+
+// 32-RHS row phase by row-block parts of at most RR_ITEM_ROWS rows: items[b] =
+// row block | part << 24. FP32 arithmetic reads zil, FP64 the interleaved z.
+template <typename TS0, typename TS1, typename TC>
+void launch_rows_rr(const int *items, int nitems, const int *rb_row0, RowClass c0, const TS0 *arena0,
+                    RowClass c1, const TS1 *arena1, const TC *z, const float *zil, const unsigned *tmax,
+                    const int *perm_t, int nr, int mu_base, double *y, hipStream_t st);
 
 // FP64 master copy -> storage type (round to nearest).
 template <typename TS>
