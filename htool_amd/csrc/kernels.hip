@@ -2256,9 +2256,8 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int MFMA_MU    = 32;  // right-hand sides of one matrix-core pass
-constexpr int RPASS_COLS = 128; // row-block columns staged in LDS per pass
-constexpr int VPASS16 = 64; // V columns per pass, 16-row tiles
-constexpr int VPASS32 = 32; // V columns per pass, 32-row tiles
+constexpr int VPASS16 = 64; // V columns per pass, 16-row tiles (complex passes)
+constexpr int VPASS32 = 32; // V columns per pass, 32-row tiles (complex passes)
 
 // Output row of accumulator i of lane l in a 32x32 tile (v_mfma_f32_32x32x2f32).
 __device__ __forceinline__ int mfma32_row(int i, int lane) { return (i & 3) + 4 * (lane >> 5) + 8 * (i >> 2); }
@@ -2368,625 +2367,8 @@ __device__ __forceinline__ void deinterleave16(const TS (&p)[8], const TS (&q)[8
 __device__ __forceinline__ float pass_scale(float m) {
     return m > 0.0f && m <= 3.0e38f ? ldexpf(1.0f, 14 - ilogbf(m)) : 1.0f;
 }
-__device__ __forceinline__ float abs_max4(float m, float4 v) {
-    return fmaxf(fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y))), fmaxf(fabsf(v.z), fabsf(v.w)));
-}
 
-// Phase 2, one precision class of a row block. The workgroup's waves split as
-// T row tiles of 32 rows times S column slices (T S <= 4): wave (t, q) owns
-// rows 32t..32t+31 and columns [q W, (q+1) W) of every 128-column pass, W =
-// 128 / S. Lane l takes row l % 32 and, in MFMA m of step it, column
-// q W + 2 VW it + VW (l / 32) + m. The vector entries of a pass are staged in
-// LDS (blocked by 4 columns); the next pass's entries and matrix rows are
-// loaded while the current pass runs on the matrix cores.
-template <typename TS, int S>
-__device__ __forceinline__ void rows_class_mfma(const TS *arena, const RowClass &rc, int rb, int R, const float *z,
-                                                float *s_z, int t, int q, bool active, double (&accd)[16]) {
-    constexpr int MU   = MFMA_MU;
-    constexpr int VW   = Vec<TS>::W;
-    constexpr int WCOL = RPASS_COLS / S;
-    constexpr int NIT  = WCOL / (2 * VW);
-    constexpr int ZPT  = RPASS_COLS * MU / WG; // staged entries per thread
-    constexpr bool PREFETCH_A = S > 1;      // four row tiles: registers too few for a second copy
-    const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
-    const int Cp = (C + rc.align - 1) & ~(rc.align - 1);    // columns read (vector width)
-    const int Cs = (C + rc.salign - 1) & ~(rc.salign - 1);  // row stride (128-byte rows)
-    if (Cp == 0)
-        return;
-    const TS *A    = arena + rc.rb_aoff[rb];
-    const int lane = threadIdx.x & 63, half = lane >> 5;
-    const int row  = 32 * t + (lane & 31);
-    const TS *rowp = A + (size_t)(row < R ? row : 0) * Cs + q * WCOL;
-    const int zs = threadIdx.x & (MU - 1), zc = threadIdx.x / MU; // staging: RHS, first column
-    float v[ZPT];
-    TS a[NIT][VW], an[PREFETCH_A ? NIT : 1][VW];
-    auto load_z = [&](int cc) {
-        int idx[ZPT];
-#pragma unroll
-        for (int i = 0; i < ZPT; i++) {
-            const int c = cc + zc + (WG / MU) * i;
-            idx[i]      = rc.colidx[cb + (c < C ? c : 0)];
-        }
-#pragma unroll
-        for (int i = 0; i < ZPT; i++)
-            v[i] = z[zblk(idx[i], zs)];
-    };
-    auto load_a = [&](int cc, auto &dst) {
-        const int nz = min(RPASS_COLS, Cp - cc);
-#pragma unroll
-        for (int it = 0; it < NIT; it++) {
-            const int kk = q * WCOL + 2 * VW * it + VW * half;
-            Vec<TS>::load(rowp + cc + (kk < nz ? kk - q * WCOL : -q * WCOL), dst[it]);
-        }
-    };
-    load_z(0);
-    if (active)
-        load_a(0, a);
-    for (int cc = 0; cc < Cp; cc += RPASS_COLS) {
-        const int nz = min(RPASS_COLS, Cp - cc);
-        float zmax = 0.0f; // largest staged magnitude (16-bit FP16 products: the pass scale)
-#pragma unroll
-        for (int i = 0; i < ZPT; i++) {
-            const int c      = zc + (WG / MU) * i;
-            const float vz   = cc + c < C ? v[i] : 0.0f;
-            s_z[zblk(c, zs)] = vz;
-            zmax             = fmaxf(zmax, fabsf(vz));
-        }
-        // the kernel's LDS holds the staged pass and then four floats for the waves' maxima
-        float *s_mx = s_z + RPASS_COLS * MU;
-        if constexpr (Mma16<TS>::on && Mma16<TS>::scaled) {
-            zmax = wave_max(zmax);
-            if ((threadIdx.x & 63) == 0)
-                s_mx[threadIdx.x >> 6] = zmax;
-        }
-        __syncthreads();
-        const bool more = cc + RPASS_COLS < Cp;
-        if (more) {
-            load_z(cc + RPASS_COLS);
-            if (PREFETCH_A && active)
-                load_a(cc + RPASS_COLS, an);
-        }
-        if (active) {
-            floatx16 acc = {0};
-            double inv   = 1.0;
-            if constexpr (Mma16<TS>::on) {
-                using M  = Mma16<TS>;
-                float sc = 1.0f;
-                if constexpr (M::scaled) {
-                    sc  = pass_scale(fmaxf(fmaxf(s_mx[0], s_mx[1]), fmaxf(s_mx[2], s_mx[3])));
-                    inv = 1.0 / (double)sc;
-                }
-#pragma unroll
-                for (int it = 0; it < NIT; it++) {
-                    const int k0 = q * WCOL + 2 * VW * it;
-                    if (k0 < nz) {
-                        const int kk    = k0 + VW * half;
-                        const float4 b0 = *reinterpret_cast<const float4 *>(s_z + zblk(kk, lane & 31));
-                        const float4 b1 = *reinterpret_cast<const float4 *>(s_z + zblk(kk + 4, lane & 31));
-                        const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-                        typename M::V bs[M::NB];
-                        M::split(bb, sc, bs);
-                        const typename M::V av = M::a(a[it], true);
-#pragma unroll
-                        for (int u = 0; u < M::NB; u++)
-                            acc = M::mma32(av, bs[u], acc);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int it = 0; it < NIT; it++) {
-                    const int k0 = q * WCOL + 2 * VW * it;
-                    if (k0 < nz) {
-                        const int kk = k0 + VW * half;
-#pragma unroll
-                        for (int m4 = 0; m4 < VW; m4 += 4) {
-                            const float4 b = *reinterpret_cast<const float4 *>(s_z + zblk(kk + m4, lane & 31));
-                            const float bb[4] = {b.x, b.y, b.z, b.w};
-#pragma unroll
-                            for (int m = 0; m < 4; m++)
-                                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cvt<float>(a[it][m4 + m]), bb[m], acc, 0,
-                                                                           0, 0);
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < 16; i++)
-                accd[i] += (double)acc[i] * inv;
-        }
-        __syncthreads();
-        if (more && active) {
-            if constexpr (PREFETCH_A) {
-#pragma unroll
-                for (int it = 0; it < NIT; it++)
-#pragma unroll
-                    for (int u = 0; u < VW; u++)
-                        a[it][u] = an[it][u];
-            } else {
-                load_a(cc + RPASS_COLS, a);
-            }
-        }
-    }
-}
-
-template <typename TS0, typename TS1, int S>
-__device__ __forceinline__ void rows_mfma_split(const int rb, const int R, const RowClass &c0, const TS0 *arena0,
-                                                const RowClass &c1, const TS1 *arena1, const float *z, float *s_z,
-                                                double (&accd)[16], int &q, bool &active) {
-    const int w = threadIdx.x >> 6, T = (R + 31) / 32;
-    const int t = w / S;
-    q           = w % S;
-    active      = t < T;
-    rows_class_mfma<TS0, S>(arena0, c0, rb, R, z, s_z, t, q, active, accd);
-    if (arena1)
-        rows_class_mfma<TS1, S>(arena1, c1, rb, R, z, s_z, t, q, active, accd);
-}
-
-template <typename TS0, typename TS1>
-__global__ void __launch_bounds__(WG) k_rows_mfma(const int *rb_order, const int *rb_row0, RowClass c0,
-                                                  const TS0 *arena0, RowClass c1, const TS1 *arena1, const float *z,
-                                                  const int *perm_t, int nr, int mu_base, double *y) {
-    // LDS: the staged pass (16 KB) and four floats of pass maxima, then the
-    // column slices' partial row sums (24 KB)
-    __shared__ __align__(16) double smem[3 * 16 * 64];
-    static_assert(RPASS_COLS * MFMA_MU * sizeof(float) + 4 * sizeof(float) <= sizeof(smem),
-                  "staged pass and the waves' maxima fit the workgroup's LDS");
-    float *s_z   = reinterpret_cast<float *>(smem);
-    const int rb = rb_order[blockIdx.x];
-    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int T = (R + 31) / 32;
-    const int S = T == 1 ? 4 : (T == 2 ? 2 : 1);
-    double accd[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++)
-        accd[i] = 0;
-    int q       = 0;
-    bool active = false;
-    if (S == 4)
-        rows_mfma_split<TS0, TS1, 4>(rb, R, c0, arena0, c1, arena1, z, s_z, accd, q, active);
-    else if (S == 2)
-        rows_mfma_split<TS0, TS1, 2>(rb, R, c0, arena0, c1, arena1, z, s_z, accd, q, active);
-    else
-        rows_mfma_split<TS0, TS1, 1>(rb, R, c0, arena0, c1, arena1, z, s_z, accd, q, active);
-    const int t = w / S;
-    if (S > 1) {
-        // every wave has passed the last barrier of the pass loop: the LDS is
-        // free. Slice q > 0 of tile t leaves its sums in slot t (S - 1) + q - 1.
-        if (active && q > 0)
-#pragma unroll
-            for (int i = 0; i < 16; i++)
-                smem[((t * (S - 1) + q - 1) * 16 + i) * 64 + lane] = accd[i];
-        __syncthreads();
-        if (active && q == 0)
-            for (int qq = 1; qq < S; qq++)
-#pragma unroll
-                for (int i = 0; i < 16; i++)
-                    accd[i] += smem[((t * (S - 1) + qq - 1) * 16 + i) * 64 + lane];
-    }
-    if (active && q == 0) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const int orow = 32 * t + mfma32_row(i, lane);
-            if (orow < R)
-                y[(size_t)perm_t[r0 + orow] + (size_t)nr * (mu_base + (lane & 31))] = accd[i];
-        }
-    }
-}
-
-// Stores row r of a chunk's V x for right-hand side s (scaled when the leaf is
-// one chunk; a partial sum otherwise).
-__device__ __forceinline__ void vx_store_mfma(const VTile &d, int r, int s, double v, float *z, double *part,
-                                              const double *tscale) {
-    if (d.nparts > 1)
-        part[(size_t)(d.part_off + r) * MFMA_MU + s] = v;
-    else
-        z[zblk(d.t_index + r, s)] = (float)(tscale ? v * tscale[d.t_index + r] : v);
-}
-
-// Phase 1 for a chunk of at most 16 rows: v_mfma_f32_16x16x4f32 on a 16-row
-// tile, two MFMAs (right-hand sides 0-15 and 16-31) per A value; lane l takes
-// row l % 16 and, in MFMA m of step it, column 4 VW it + VW (l / 16) + m. The
-// next pass's V and x entries are loaded while the current pass runs.
-template <typename TS>
-__device__ __forceinline__ void vx_chunk_mfma16(const VTile &d, const TS *varena, float *z, double *part,
-                                                const double *tscale, int lane) {
-    constexpr int VW  = Vec<TS>::W;
-    constexpr int NIT = VPASS16 / (4 * VW);
-    const int g = lane >> 4, r = lane & 15;
-    const TS *rowp = varena + d.v_off + (size_t)(r < d.k ? r : 0) * d.ld;
-    TS a[NIT][VW], an[NIT][VW];
-    float4 b[NIT][VW / 4][2], bn[NIT][VW / 4][2];
-    auto load = [&](int c0, TS (&da)[NIT][VW], float4 (&db)[NIT][VW / 4][2]) {
-        const int nc = min(VPASS16, d.ncols - c0);
-#pragma unroll
-        for (int it = 0; it < NIT; it++) {
-            const int kk = 4 * VW * it + VW * g;
-            const int kc = kk < nc ? kk : 0;
-            Vec<TS>::load(rowp + c0 + kc, da[it]);
-#pragma unroll
-            for (int m4 = 0; m4 < VW / 4; m4++) {
-                const size_t j = (size_t)d.x_off + c0 + kc + 4 * m4;
-                db[it][m4][0]  = *reinterpret_cast<const float4 *>(z + zblk(j, r));
-                db[it][m4][1]  = *reinterpret_cast<const float4 *>(z + zblk(j, r + 16));
-            }
-        }
-    };
-    double accd[2][4] = {};
-    load(0, a, b);
-    for (int c0 = 0; c0 < d.ncols; c0 += VPASS16) {
-        const int nc    = min(VPASS16, d.ncols - c0);
-        const bool more = c0 + VPASS16 < d.ncols;
-        if (more)
-            load(c0 + VPASS16, an, bn);
-        floatx4 acc0 = {0}, acc1 = {0};
-        double inv   = 1.0;
-        if constexpr (Mma16<TS>::on) {
-            using M  = Mma16<TS>;
-            float sc = 1.0f;
-            if constexpr (M::scaled) {
-                float m = 0.0f;
-#pragma unroll
-                for (int it = 0; it < NIT; it++)
-#pragma unroll
-                    for (int m4 = 0; m4 < VW / 4; m4++)
-                        m = abs_max4(abs_max4(m, b[it][m4][0]), b[it][m4][1]);
-                sc  = pass_scale(wave_max(m));
-                inv = 1.0 / (double)sc;
-            }
-#pragma unroll
-            for (int it = 0; it < NIT; it++) {
-                if (4 * VW * it < nc) {
-                    const bool in     = 4 * VW * it + VW * g < nc;
-                    const float b0[8] = {b[it][0][0].x, b[it][0][0].y, b[it][0][0].z, b[it][0][0].w,
-                                         b[it][1][0].x, b[it][1][0].y, b[it][1][0].z, b[it][1][0].w};
-                    const float b1[8] = {b[it][0][1].x, b[it][0][1].y, b[it][0][1].z, b[it][0][1].w,
-                                         b[it][1][1].x, b[it][1][1].y, b[it][1][1].z, b[it][1][1].w};
-                    typename M::V s0[M::NB], s1[M::NB];
-                    M::split(b0, sc, s0);
-                    M::split(b1, sc, s1);
-                    const typename M::V av = M::a(a[it], in);
-#pragma unroll
-                    for (int u = 0; u < M::NB; u++) {
-                        acc0 = M::mma16(av, s0[u], acc0);
-                        acc1 = M::mma16(av, s1[u], acc1);
-                    }
-                }
-            }
-        } else {
-#pragma unroll
-            for (int it = 0; it < NIT; it++) {
-                if (4 * VW * it < nc) {
-                    const bool in = 4 * VW * it + VW * g < nc;
-#pragma unroll
-                    for (int m4 = 0; m4 < VW / 4; m4++) {
-                        const float b0[4] = {b[it][m4][0].x, b[it][m4][0].y, b[it][m4][0].z, b[it][m4][0].w};
-                        const float b1[4] = {b[it][m4][1].x, b[it][m4][1].y, b[it][m4][1].z, b[it][m4][1].w};
-#pragma unroll
-                        for (int m = 0; m < 4; m++) {
-                            const float av = in ? cvt<float>(a[it][4 * m4 + m]) : 0.0f;
-                            acc0           = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b0[m], acc0, 0, 0, 0);
-                            acc1           = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b1[m], acc1, 0, 0, 0);
-                        }
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            accd[0][i] += (double)acc0[i] * inv;
-            accd[1][i] += (double)acc1[i] * inv;
-        }
-        if (more) {
-#pragma unroll
-            for (int it = 0; it < NIT; it++) {
-#pragma unroll
-                for (int u = 0; u < VW; u++)
-                    a[it][u] = an[it][u];
-#pragma unroll
-                for (int m4 = 0; m4 < VW / 4; m4++)
-                    b[it][m4][0] = bn[it][m4][0], b[it][m4][1] = bn[it][m4][1];
-            }
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int row = 4 * g + i;
-        if (row < d.k) {
-            vx_store_mfma(d, row, r, accd[0][i], z, part, tscale);
-            vx_store_mfma(d, row, r + 16, accd[1][i], z, part, tscale);
-        }
-    }
-}
-
-// Phase 1 for a chunk of more than 16 rows: v_mfma_f32_32x32x2f32 over 32-row
-// tiles, next pass prefetched as above.
-template <typename TS>
-__device__ __forceinline__ void vx_chunk_mfma32(const VTile &d, const TS *varena, float *z, double *part,
-                                                const double *tscale, int lane) {
-    constexpr int VW  = Vec<TS>::W;
-    constexpr int NIT = VPASS32 / (2 * VW);
-    const int half = lane >> 5, s = lane & 31;
-    for (int rt = 0; rt < d.k; rt += 32) {
-        const int row  = rt + s;
-        const TS *rowp = varena + d.v_off + (size_t)(row < d.k ? row : 0) * d.ld;
-        TS a[NIT][VW], an[NIT][VW];
-        float4 b[NIT][VW / 4], bn[NIT][VW / 4];
-        auto load = [&](int c0, TS (&da)[NIT][VW], float4 (&db)[NIT][VW / 4]) {
-            const int nc = min(VPASS32, d.ncols - c0);
-#pragma unroll
-            for (int it = 0; it < NIT; it++) {
-                const int kk = 2 * VW * it + VW * half;
-                const int kc = kk < nc ? kk : 0;
-                Vec<TS>::load(rowp + c0 + kc, da[it]);
-#pragma unroll
-                for (int m4 = 0; m4 < VW / 4; m4++)
-                    db[it][m4] = *reinterpret_cast<const float4 *>(z + zblk((size_t)d.x_off + c0 + kc + 4 * m4, s));
-            }
-        };
-        double accd[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++)
-            accd[i] = 0;
-        load(0, a, b);
-        for (int c0 = 0; c0 < d.ncols; c0 += VPASS32) {
-            const int nc    = min(VPASS32, d.ncols - c0);
-            const bool more = c0 + VPASS32 < d.ncols;
-            if (more)
-                load(c0 + VPASS32, an, bn);
-            floatx16 acc = {0};
-            double inv   = 1.0;
-            if constexpr (Mma16<TS>::on) {
-                using M  = Mma16<TS>;
-                float sc = 1.0f;
-                if constexpr (M::scaled) {
-                    float m = 0.0f;
-#pragma unroll
-                    for (int it = 0; it < NIT; it++)
-#pragma unroll
-                        for (int m4 = 0; m4 < VW / 4; m4++)
-                            m = abs_max4(m, b[it][m4]);
-                    sc  = pass_scale(wave_max(m));
-                    inv = 1.0 / (double)sc;
-                }
-#pragma unroll
-                for (int it = 0; it < NIT; it++) {
-                    if (2 * VW * it < nc) {
-                        const bool in     = 2 * VW * it + VW * half < nc;
-                        const float bb[8] = {b[it][0].x, b[it][0].y, b[it][0].z, b[it][0].w,
-                                             b[it][1].x, b[it][1].y, b[it][1].z, b[it][1].w};
-                        typename M::V bs[M::NB];
-                        M::split(bb, sc, bs);
-                        const typename M::V av = M::a(a[it], in);
-#pragma unroll
-                        for (int u = 0; u < M::NB; u++)
-                            acc = M::mma32(av, bs[u], acc);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int it = 0; it < NIT; it++) {
-                    if (2 * VW * it < nc) {
-                        const bool in = 2 * VW * it + VW * half < nc;
-#pragma unroll
-                        for (int m4 = 0; m4 < VW / 4; m4++) {
-                            const float bb[4] = {b[it][m4].x, b[it][m4].y, b[it][m4].z, b[it][m4].w};
-#pragma unroll
-                            for (int m = 0; m < 4; m++)
-                                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(
-                                    in ? cvt<float>(a[it][4 * m4 + m]) : 0.0f, bb[m], acc, 0, 0, 0);
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < 16; i++)
-                accd[i] += (double)acc[i] * inv;
-            if (more) {
-#pragma unroll
-                for (int it = 0; it < NIT; it++) {
-#pragma unroll
-                    for (int u = 0; u < VW; u++)
-                        a[it][u] = an[it][u];
-#pragma unroll
-                    for (int m4 = 0; m4 < VW / 4; m4++)
-                        b[it][m4] = bn[it][m4];
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const int r = rt + mfma32_row(i, lane);
-            if (r < d.k)
-                vx_store_mfma(d, r, s, accd[i], z, part, tscale);
-        }
-    }
-}
-
-template <typename TS>
-__device__ __forceinline__ void vx_chunk_mfma(const VTile &d, const TS *varena, float *z, double *part,
-                                              const double *tscale, int lane) {
-    if (d.k <= 16)
-        vx_chunk_mfma16<TS>(d, varena, z, part, tscale, lane);
-    else
-        vx_chunk_mfma32<TS>(d, varena, z, part, tscale, lane);
-}
-
-template <typename TS0, typename TS1>
-__global__ void __launch_bounds__(WG) k_vx_mfma(const int *wv_off, int nwaves, const VTile *tiles, const TS0 *varena0,
-                                                const TS1 *varena1, float *z, double *part, const double *tscale) {
-    const int wv = blockIdx.x * NWAVE + (threadIdx.x >> 6);
-    if (wv >= nwaves)
-        return;
-    const int lane = threadIdx.x & 63;
-    for (int ti = wv_off[wv]; ti < wv_off[wv + 1]; ti++) {
-        const VTile d = tiles[ti];
-        if (d.cls == 0)
-            vx_chunk_mfma<TS0>(d, varena0, z, part, tscale, lane);
-        else
-            vx_chunk_mfma<TS1>(d, varena1, z, part, tscale, lane);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// FP64 32-RHS products on the matrix cores (v_mfma_f64_16x16x4f64): 16-row
-// tiles; lane l = (g, r) = (l / 16, l % 16) takes row r of the tile and, in
-// MFMA m of step it, column 8 it + 2 g + m (its 16-byte load holds both m);
-// two MFMAs per A value, right-hand sides 0-15 and 16-31. Result register i
-// of lane (g, r) holds row 4 i + g, right-hand side r (the FP32 16x16x4 form
-// holds row 4 g + i). z stays RHS-interleaved (z[j * 32 + s]); sums are FP64
-// throughout.
-typedef double doublex4 __attribute__((ext_vector_type(4)));
-constexpr int RPASS64 = 64; // row-block columns staged per pass
-
-__global__ void __launch_bounds__(WG) k_rows_mfma64(const int *rb_order, const int *rb_row0, RowClass c0,
-                                                    const double *arena0, RowClass c1, const double *arena1,
-                                                    const double *z, const int *perm_t, int nr, int mu_base,
-                                                    double *y) {
-    constexpr int MU  = 32;
-    constexpr int NIT = RPASS64 / 8;
-    constexpr int ZPT = RPASS64 * MU / WG;
-    __shared__ double s_z[RPASS64 * MU];
-    const int rb = rb_order[blockIdx.x];
-    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int g = lane >> 4, r16 = lane & 15;
-    const int T16 = (R + 15) / 16; // <= 8 tiles; wave w takes tiles w and w + 4
-    doublex4 acc[2][2] = {};
-    const int zs = threadIdx.x & (MU - 1), zc = threadIdx.x / MU;
-    for (int cls = 0; cls < 2; cls++) {
-        const RowClass &rc = cls == 0 ? c0 : c1;
-        const double *arena = cls == 0 ? arena0 : arena1;
-        if (!arena)
-            continue;
-        const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
-        const int Cp = (C + rc.align - 1) & ~(rc.align - 1);    // columns read (vector width)
-        const int Cs = (C + rc.salign - 1) & ~(rc.salign - 1);  // row stride (128-byte rows)
-        const double *A = arena + rc.rb_aoff[rb];
-        for (int cc = 0; cc < Cp; cc += RPASS64) {
-            const int nz = min(RPASS64, Cp - cc);
-            {
-                int idx[ZPT];
-#pragma unroll
-                for (int i = 0; i < ZPT; i++) {
-                    const int c = cc + zc + (WG / MU) * i;
-                    idx[i]      = rc.colidx[cb + (c < C ? c : 0)];
-                }
-#pragma unroll
-                for (int i = 0; i < ZPT; i++) {
-                    const int c           = zc + (WG / MU) * i;
-                    const double v        = z[(size_t)idx[i] * MU + zs];
-                    s_z[c * MU + zs]      = cc + c < C ? v : 0.0;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int tt = 0; tt < 2; tt++) {
-                const int t = w + 4 * tt;
-                if (t < T16) {
-                    const int row   = 16 * t + r16;
-                    const double *rp = A + (size_t)(row < R ? row : 0) * Cs + cc;
-                    double a[NIT][2];
-#pragma unroll
-                    for (int it = 0; it < NIT; it++) {
-                        const int kk = 8 * it + 2 * g;
-                        Vec<double>::load(rp + (kk < nz ? kk : 0), a[it]);
-                    }
-#pragma unroll
-                    for (int it = 0; it < NIT; it++) {
-                        if (8 * it < nz) {
-                            const int kk = 8 * it + 2 * g;
-#pragma unroll
-                            for (int m = 0; m < 2; m++) {
-                                const double av = kk < nz ? a[it][m] : 0.0;
-                                const double *bz = s_z + (kk + m) * MU;
-                                acc[tt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bz[r16], acc[tt][0], 0, 0, 0);
-                                acc[tt][1] =
-                                    __builtin_amdgcn_mfma_f64_16x16x4f64(av, bz[r16 + 16], acc[tt][1], 0, 0, 0);
-                            }
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-#pragma unroll
-    for (int tt = 0; tt < 2; tt++) {
-        const int t = w + 4 * tt;
-        if (t < T16) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int orow = 16 * t + 4 * i + g;
-                if (orow < R) {
-                    double *o = y + (size_t)perm_t[r0 + orow] + (size_t)nr * mu_base;
-                    o[(size_t)nr * r16]        = acc[tt][0][i];
-                    o[(size_t)nr * (r16 + 16)] = acc[tt][1][i];
-                }
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ void vx_store64(const VTile &d, int r, int s, double v, double *z, double *part,
-                                           const double *tscale) {
-    if (d.nparts > 1)
-        part[(size_t)(d.part_off + r) * 32 + s] = v;
-    else
-        z[(size_t)(d.t_index + r) * 32 + s] = tscale ? v * tscale[d.t_index + r] : v;
-}
-
-__global__ void __launch_bounds__(WG) k_vx_mfma64(const int *wv_off, int nwaves, const VTile *tiles,
-                                                  const double *varena0, const double *varena1, double *z,
-                                                  double *part, const double *tscale) {
-    constexpr int MU  = 32;
-    constexpr int NIT = 8; // 64 columns per pass
-    const int wv      = blockIdx.x * NWAVE + (threadIdx.x >> 6);
-    if (wv >= nwaves)
-        return;
-    const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
-    for (int ti = wv_off[wv]; ti < wv_off[wv + 1]; ti++) {
-        const VTile d       = tiles[ti];
-        const double *vare = d.cls == 0 ? varena0 : varena1;
-        const double *xv    = z + (size_t)d.x_off * MU;
-        for (int rt = 0; rt < d.k; rt += 16) {
-            const int row    = rt + r16;
-            const double *rp = vare + d.v_off + (size_t)(row < d.k ? row : 0) * d.ld;
-            doublex4 acc0 = {0}, acc1 = {0};
-            for (int c0 = 0; c0 < d.ncols; c0 += 8 * NIT) {
-                const int nc = min(8 * NIT, d.ncols - c0);
-                double a[NIT][2];
-#pragma unroll
-                for (int it = 0; it < NIT; it++) {
-                    const int kk = 8 * it + 2 * g;
-                    Vec<double>::load(rp + c0 + (kk < nc ? kk : 0), a[it]);
-                }
-#pragma unroll
-                for (int it = 0; it < NIT; it++) {
-                    if (8 * it < nc) {
-                        const int kk = 8 * it + 2 * g;
-#pragma unroll
-                        for (int m = 0; m < 2; m++) {
-                            const bool in    = kk < nc;
-                            const double *bz = xv + (size_t)(c0 + (in ? kk : 0) + m) * MU;
-                            const double av  = in ? a[it][m] : 0.0;
-                            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bz[r16], acc0, 0, 0, 0);
-                            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bz[r16 + 16], acc1, 0, 0, 0);
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int r = rt + 4 * i + g;
-                if (r < d.k) {
-                    vx_store64(d, r, r16, acc0[i], z, part, tscale);
-                    vx_store64(d, r, r16 + 16, acc1[i], z, part, tscale);
-                }
-            }
-        }
-    }
-}
+typedef double doublex4 __attribute__((ext_vector_type(4))); // v_mfma_f64_16x16x4f64 accumulators
 
 // ---------------------------------------------------------------------------
 // 32-RHS V phase by row ranges. A work item is at most RR_ROWS (128) stacked V
@@ -4848,27 +4230,6 @@ void launch_vx(const int *vg_off, int ngroups, const VTile *tiles, const VTile *
                hipStream_t st) {
     if (ngroups <= 0)
         return;
-    if constexpr (std::is_same<TC, double>::value && std::is_same<TS0, double>::value &&
-                  std::is_same<TS1, double>::value) {
-        if (MU == MFMA_MU) {
-            hipLaunchKernelGGL(k_vx_mfma64, dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, ngroups,
-                               tiles, varena0, varena1, z, part, tscale);
-            if (nmulti > 0)
-                hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU>), dim3(nmulti), dim3(WG), 0, st, multi_tiles,
-                                   nmulti, part, z, tscale);
-            return;
-        }
-    }
-    if constexpr (std::is_same<TC, float>::value) {
-        if (MU == MFMA_MU) {
-            hipLaunchKernelGGL((k_vx_mfma<TS0, TS1>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off,
-                               ngroups, tiles, varena0, varena1, z, part, tscale);
-            if (nmulti > 0)
-                hipLaunchKernelGGL((k_vx_sum<TC, MFMA_MU, true>), dim3(nmulti), dim3(WG), 0, st, multi_tiles,
-                                   nmulti, part, z, tscale);
-            return;
-        }
-    }
 #define HTG_VX(M)                                                                                          \
     do {                                                                                                   \
         hipLaunchKernelGGL((k_vx<TS0, TS1, TC, M>), dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, \
@@ -4925,21 +4286,6 @@ void launch_rows(const int *rb_order, const int *rb_row0, int nrb, RowClass c0, 
                  hipStream_t st) {
     if (nrb <= 0)
         return;
-    if constexpr (std::is_same<TC, double>::value && std::is_same<TS0, double>::value &&
-                  std::is_same<TS1, double>::value) {
-        if (MU == MFMA_MU) {
-            hipLaunchKernelGGL(k_rows_mfma64, dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0, c1, arena1,
-                               z, perm_t, nr, mu_base, y);
-            return;
-        }
-    }
-    if constexpr (std::is_same<TC, float>::value) {
-        if (MU == MFMA_MU) {
-            hipLaunchKernelGGL((k_rows_mfma<TS0, TS1>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0,
-                               c1, arena1, z, perm_t, nr, mu_base, y);
-            return;
-        }
-    }
 #define HTG_ROWS(M)                                                                                          \
     hipLaunchKernelGGL((k_rows<TS0, TS1, TC, M>), dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0, c1, \
                        arena1, z, perm_t, nr, mu_base, y)
