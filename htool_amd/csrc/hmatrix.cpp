@@ -694,6 +694,18 @@ void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream
         if (ev)
             HTG_HIP(hipEventRecord(ev[1], st));
         const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
+        if (std::is_same<R, double>::value && MU == 32) {
+            const VTileList &S = h.vt[4];
+            launch_rrc64(S.d_tiles.p, (int)S.tiles.size(), S.d_multi.p, (int)S.multi.size(), h.d_rb_items.p,
+                         (int)h.rb_items.size(), h.d_rb_row0.p, rc0, arena_of<double>(C0, false), rc1,
+                         has1 ? arena_of<double>(C1, false) : nullptr, arena_of<double>(C0, true),
+                         has1 ? arena_of<double>(C1, true) : nullptr, reinterpret_cast<double *>(z), h.d_part.p,
+                         tscale, h.d_perm_t.p, h.nr, done, d_y, st, ev ? ev[2] : nullptr);
+            if (ev)
+                HTG_HIP(hipEventRecord(ev[3], st));
+            done += MU;
+            continue;
+        }
         // longer chunks measured faster for the FP32-arithmetic passes only
         const VTileList &T = h.vt[MU > 1 && std::is_same<R, float>::value ? 2 : 0];
         launch_vx_c<TS0, TS1, R>(T.d_vg_off.p, (int)T.vg_off.size() - 1, T.d_tiles.p, T.d_multi.p,

@@ -3975,6 +3975,170 @@ __global__ void __launch_bounds__(WG) k_vx_mfma64c32(const int *wv_off, int nwav
     }
 }
 
+// ---------------------------------------------------------------------------
+// Complex FP64 32-RHS products by row ranges: the real FP64 walk (rr_walk)
+// with (re, im) entries. A chunk is RRC_CHUNK columns of the vector entries
+// for 32 complex right-hand sides (512 bytes per column), staged in LDS as in
+// z ([column][32][re, im]); a step is four columns (lane (r, g) loads the
+// complex entry of row r, column 4 step + g, as one 16-byte load) and feeds
+// three real MFMAs per 16 right-hand sides (Gauss: K1 += br (ar + ai),
+// K2 += ar (bi - br), K3 += ai (br + bi); re = K1 - K3, im = K1 + K2).
+constexpr int RRC_CHUNK = 32;
+typedef unsigned u4x __attribute__((ext_vector_type(4)));
+struct RrcUnit { // one unit's raw 16-byte loads (four steps)
+    u4x a[4];
+};
+
+template <typename Load, typename XLoad, typename XStore>
+__device__ __forceinline__ void rrc_walk(int nchunk, int nstep, bool real, const double *s_x, Load load, XLoad xload,
+                                         XStore xstore, doublex4 (&acc)[2][3]) {
+    constexpr int MU = 32, NS = 4, UPC = RRC_CHUNK / (4 * NS);
+    using Buf = RrcUnit;
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    auto run = [&](int u, const Buf &b) {
+        if (!real)
+            return;
+        const int j = u / UPC, s = u % UPC;
+        const double2 *buf = reinterpret_cast<const double2 *>(s_x + (size_t)(j & 1) * RRC_CHUNK * MU * 2);
+#pragma unroll
+        for (int st = 0; st < NS; st++) {
+            const int cs = s * NS + st; // step within the chunk
+            const double ar = __hiloint2double((int)b.a[st].y, (int)b.a[st].x);
+            const double ai = __hiloint2double((int)b.a[st].w, (int)b.a[st].z);
+            const double as = ar + ai;
+            const double2 *bp = buf + (size_t)(4 * cs + g) * MU;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const double2 x = bp[16 * h + r];
+                acc[h][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(as, x.x, acc[h][0], 0, 0, 0);
+                acc[h][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, x.y - x.x, acc[h][1], 0, 0, 0);
+                acc[h][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, x.x + x.y, acc[h][2], 0, 0, 0);
+            }
+        }
+    };
+    rr_walk<UPC, Buf>(nchunk, load, run, xload, xstore);
+}
+
+// V phase: at most RR_ROWS stacked complex V rows of one panel class over a
+// column range (VTile), one workgroup each; x staged from the interleaved z.
+__global__ void __launch_bounds__(WG) k_vx_rrc64(const VTile *items, const double *varena0, const double *varena1,
+                                                 double *z, double *part, const double *tscale) {
+    constexpr int MU = 32, NS = 4, SC = 4, XPT = RRC_CHUNK * MU / WG; // complex entries staged per thread (4)
+    __shared__ __align__(16) double s_x[2 * RRC_CHUNK * MU * 2];
+    const VTile d = items[blockIdx.x];
+    const double *varena = d.cls == 0 ? varena0 : varena1;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 15, g = lane >> 4;
+    const int K      = d.k;
+    const int nstep  = (d.ncols + SC - 1) / SC;
+    const int nchunk = (d.ncols + RRC_CHUNK - 1) / RRC_CHUNK;
+    const double *rp = varena + 2 * (d.v_off + (size_t)min(16 * w + r, K - 1) * d.ld + g);
+    auto load = [&](int u, RrcUnit &b) {
+#pragma unroll
+        for (int st = 0; st < NS; st++)
+            b.a[st] = nt_load16(rp + 2 * (size_t)min(u * NS + st, nstep - 1) * SC);
+    };
+    const double2 *xsrc = reinterpret_cast<const double2 *>(z) + (size_t)d.x_off * MU;
+    double2 xv[XPT];
+    auto xload = [&](int j) { // complex entry f: column f / 32 of the chunk, RHS f % 32
+#pragma unroll
+        for (int i = 0; i < XPT; i++) {
+            const int f = (int)threadIdx.x + WG * i;
+            xv[i] = xsrc[(size_t)min(j * RRC_CHUNK + (f >> 5), d.ncols - 1) * MU + (f & 31)];
+        }
+    };
+    auto xstore = [&](int j) {
+        double2 *buf = reinterpret_cast<double2 *>(s_x) + (size_t)(j & 1) * RRC_CHUNK * MU;
+#pragma unroll
+        for (int i = 0; i < XPT; i++) {
+            const int f = (int)threadIdx.x + WG * i;
+            buf[f] = j * RRC_CHUNK + (f >> 5) < d.ncols ? xv[i] : make_double2(0.0, 0.0);
+        }
+    };
+    doublex4 acc[2][3] = {};
+    rrc_walk(nchunk, nstep, 16 * w < K, s_x, load, xload, xstore, acc);
+#pragma unroll
+    for (int i = 0; i < 4; i++) { // result register i of lane (r, g): row 4 i + g, RHS 16 h + r
+        const int row = 16 * w + 4 * i + g;
+        if (row < K) {
+            const double f = (tscale && d.nparts == 1) ? tscale[d.t_index + row] : 1.0;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                double2 *o = reinterpret_cast<double2 *>(d.nparts > 1 ? part : z) +
+                             (size_t)((d.nparts > 1 ? d.part_off : d.t_index) + row) * MU + 16 * h + r;
+                *o = make_double2((acc[h][0][i] - acc[h][2][i]) * f, (acc[h][0][i] + acc[h][1][i]) * f);
+            }
+        }
+    }
+}
+
+// Row phase: row-block parts of at most RR_ROWS rows (items as in the real
+// 32-RHS rows), vector entries gathered through the column indices from the
+// interleaved z, both classes into the same sums.
+__global__ void __launch_bounds__(WG) k_rows_rrc64(const int *items, const int *rb_row0, RowClass c0,
+                                                   const double *arena0, RowClass c1, const double *arena1,
+                                                   const double *z, const int *perm_t, int nr, int mu_base,
+                                                   double *y) {
+    constexpr int MU = 32, NS = 4, SC = 4, XPT = RRC_CHUNK * MU / WG;
+    __shared__ __align__(16) double s_x[2 * RRC_CHUNK * MU * 2];
+    const int item = items[blockIdx.x], rb = item & 0xffffff, roff = RR_ROWS * (item >> 24);
+    const int r0 = rb_row0[rb] + roff, R = min(RR_ROWS, rb_row0[rb + 1] - r0);
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 15, g = lane >> 4;
+    const double2 *z2 = reinterpret_cast<const double2 *>(z);
+    doublex4 acc[2][3] = {};
+    for (int cls = 0; cls < 2; cls++) {
+        const RowClass &rc  = cls == 0 ? c0 : c1;
+        const double *arena = cls == 0 ? arena0 : arena1;
+        if (!arena)
+            continue;
+        const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
+        if (C == 0)
+            continue;
+        const int Cs     = (C + rc.salign - 1) & ~(rc.salign - 1);
+        const int nstep  = (C + SC - 1) / SC;
+        const int nchunk = (C + RRC_CHUNK - 1) / RRC_CHUNK;
+        const double *rp = arena + 2 * (rc.rb_aoff[rb] + (size_t)(roff + min(16 * w + r, R - 1)) * Cs + g);
+        const int *ci    = rc.colidx + cb;
+        auto load = [&](int u, RrcUnit &b) {
+#pragma unroll
+            for (int st = 0; st < NS; st++)
+                b.a[st] = nt_load16(rp + 2 * (size_t)min(u * NS + st, nstep - 1) * SC);
+        };
+        double2 xv[XPT];
+        auto xload = [&](int j) {
+            int idx[XPT];
+#pragma unroll
+            for (int i = 0; i < XPT; i++)
+                idx[i] = ci[min(j * RRC_CHUNK + (((int)threadIdx.x + WG * i) >> 5), C - 1)];
+#pragma unroll
+            for (int i = 0; i < XPT; i++)
+                xv[i] = z2[(size_t)idx[i] * MU + (((int)threadIdx.x + WG * i) & 31)];
+        };
+        auto xstore = [&](int j) {
+            double2 *buf = reinterpret_cast<double2 *>(s_x) + (size_t)(j & 1) * RRC_CHUNK * MU;
+#pragma unroll
+            for (int i = 0; i < XPT; i++) {
+                const int f = (int)threadIdx.x + WG * i;
+                buf[f] = j * RRC_CHUNK + (f >> 5) < C ? xv[i] : make_double2(0.0, 0.0);
+            }
+        };
+        rrc_walk(nchunk, nstep, 16 * w < R, s_x, load, xload, xstore, acc);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int row = 16 * w + 4 * i + g;
+        if (row < R) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                double *o = y + 2 * ((size_t)perm_t[r0 + row] + (size_t)nr * (mu_base + 16 * h + r));
+                o[0]      = acc[h][0][i] - acc[h][2][i];
+                o[1]      = acc[h][0][i] + acc[h][1][i];
+            }
+        }
+    }
+}
+
 template <typename TS0, typename TS1, typename R, int MU>
 __global__ void __launch_bounds__(WG) k_rows_c(const int *rb_order, const int *rb_row0, RowClass c0, const TS0 *arena0,
                                                RowClass c1, const TS1 *arena1, const R *z, const int *perm_t, int nr,
@@ -4333,6 +4497,24 @@ void launch_gather_xc64(const double *x, const int *iperm_s, int nc, int tpad, i
     else
         hipLaunchKernelGGL(k_scatter_xc64<16>, dim3((tpad + WG - 1) / WG, 2), dim3(WG), 0, st, x, iperm_s, nc, tpad,
                            mu_base, z);
+}
+
+void launch_rrc64(const VTile *items, int nitems, const VTile *multi_tiles, int nmulti, const int *rb_items,
+                  int nrb_items, const int *rb_row0, RowClass c0, const double *arena0, RowClass c1,
+                  const double *arena1, const double *varena0, const double *varena1, double *z, double *part,
+                  const double *tscale, const int *perm_t, int nr, int mu_base, double *y, hipStream_t st,
+                  hipEvent_t ev_vx_done) {
+    if (nitems > 0) {
+        hipLaunchKernelGGL(k_vx_rrc64, dim3(nitems), dim3(WG), 0, st, items, varena0, varena1, z, part, tscale);
+        if (nmulti > 0)
+            hipLaunchKernelGGL((k_vx_sum_c<double, 32>), dim3(nmulti), dim3(WG), 0, st, multi_tiles, nmulti, part, z,
+                               tscale);
+    }
+    if (ev_vx_done)
+        (void)hipEventRecord(ev_vx_done, st);
+    if (nrb_items > 0)
+        hipLaunchKernelGGL(k_rows_rrc64, dim3(nrb_items), dim3(WG), 0, st, rb_items, rb_row0, c0, arena0, c1, arena1,
+                           z, perm_t, nr, mu_base, y);
 }
 
 template <typename TS0, typename TS1, typename R>

@@ -197,6 +197,16 @@ void launch_rows_rr(const int *items, int nitems, const int *rb_row0, RowClass c
                     RowClass c1, const TS1 *arena1, const TC *z, const float *zil, const unsigned *tmax,
                     const int *perm_t, int nr, int mu_base, double *y, hipStream_t st);
 
+// Complex FP64 product, 32 right-hand sides (z interleaved, (re, im) pairs): the
+// V phase by row ranges (items of the FP64 cut, multi = their partial sums),
+// an event recorded when it is done (if given), then the row phase by
+// row-block parts.
+void launch_rrc64(const VTile *items, int nitems, const VTile *multi_tiles, int nmulti, const int *rb_items,
+                  int nrb_items, const int *rb_row0, RowClass c0, const double *arena0, RowClass c1,
+                  const double *arena1, const double *varena0, const double *varena1, double *z, double *part,
+                  const double *tscale, const int *perm_t, int nr, int mu_base, double *y, hipStream_t st,
+                  hipEvent_t ev_vx_done);
+
 // FP64 master copy -> storage type (round to nearest).
 template <typename TS>
 void launch_convert(const double *src, TS *dst, uint64_t n, hipStream_t st);
