@@ -572,7 +572,7 @@ void upload_layout(htg_hmatrix &h) {
     for (int j = 0; j < h.nc; j++)
         ip[h.perm_s[j]] = j;
     h.d_iperm_s.upload(ip);
-    h.d_xstage.alloc((size_t)(h.tpad + h.t_size) * 32);
+    h.d_xstage.alloc((size_t)(h.tpad + h.t_size) * 32 * h.esz()); // complex: (re, im) pairs
     h.d_xstage.zero(); // padding entries [nc, tpad) stay zero
     h.d_tmax.alloc(TMAX_SLOTS);
     h.generation->fetch_add(1); // graphs captured before read freed buffers
@@ -680,10 +680,27 @@ void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream
     const RowClass rc1{h.cls[1].d_rb_col0.p, h.cls[1].d_rb_aoff.p, h.cls[1].d_colidx.p, CLASS_ALIGN[1], ROW_ALIGN[1]};
     int done = 0;
     while (done < mu) {
-        // FP64: 32 complex RHS per pass (every A entry feeds 8 MFMAs); FP32 arithmetic: 16
-        const int MU = pick_mu_chunk(mu - done, std::is_same<R, double>::value ? 32 : 16);
+        // 32 complex RHS per pass (the row-range kernels), fewer for the remainder
+        const int MU = pick_mu_chunk(mu - done, 32);
         if (ev)
             HTG_HIP(hipEventRecord(ev[0], st));
+        if constexpr (std::is_same<R, float>::value) {
+            if (MU == 32) {
+                const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
+                const VTileList &S = h.vt[4];
+                launch_rrc32<TS0, TS1>(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, h.d_xmax.p, h.d_tmax.p,
+                                       S.d_tiles.p, (int)S.tiles.size(), S.d_multi.p, (int)S.multi.size(),
+                                       h.d_rb_items.p, (int)h.rb_items.size(), h.d_rb_row0.p, rc0,
+                                       arena_of<TS0>(C0, false), rc1, has1 ? arena_of<TS1>(C1, false) : nullptr,
+                                       arena_of<TS0>(C0, true), has1 ? arena_of<TS1>(C1, true) : nullptr, h.d_part.p,
+                                       tscale, h.d_perm_t.p, h.nr, d_y, st, ev ? ev[1] : nullptr,
+                                       ev ? ev[2] : nullptr);
+                if (ev)
+                    HTG_HIP(hipEventRecord(ev[3], st));
+                done += MU;
+                continue;
+            }
+        }
         if (std::is_same<R, double>::value && (MU == 16 || MU == 32)) {
             launch_gather_xc64(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, MU, reinterpret_cast<double *>(z), st);
         } else if (std::is_same<R, float>::value && MU == 16) {

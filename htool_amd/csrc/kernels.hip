@@ -3606,7 +3606,8 @@ __global__ void __launch_bounds__(WG) k_vx_c(const int *wv_off, int nwaves, cons
 
 template <typename R, int MU>
 __global__ void __launch_bounds__(WG) k_vx_sum_c(const VTile *first, int nmulti,
-                                                 const double *part, R *z, const double *tscale) {
+                                                 const double *part, R *z, const double *tscale,
+                                                 unsigned *tmax = nullptr) {
     constexpr int GPB = vx_sum_groups_per_block<MU>();
     constexpr int TPG = WG / GPB;
     const int gi      = blockIdx.x * GPB + (int)threadIdx.x / TPG;
@@ -3615,6 +3616,7 @@ __global__ void __launch_bounds__(WG) k_vx_sum_c(const VTile *first, int nmulti,
     const VTile d    = first[gi];
     const int km     = d.k * MU;
     const double *pb = part + 2 * (size_t)d.part_base * MU;
+    float m = 0.0f;
     for (int e = threadIdx.x % TPG; e < km; e += TPG) {
         double v[2] = {0, 0};
         sum_parts<2>(pb + 2 * (size_t)e, 2 * (size_t)km, d.nparts, v);
@@ -3622,6 +3624,12 @@ __global__ void __launch_bounds__(WG) k_vx_sum_c(const VTile *first, int nmulti,
         R *o            = z + 2 * ((size_t)d.t_index * MU + e);
         o[0]            = (R)(v[0] * sc);
         o[1]            = (R)(v[1] * sc);
+        m = fmaxf(m, fmaxf(fabsf((float)o[0]), fabsf((float)o[1])));
+    }
+    if (tmax && d.cls == 1) { // complex 32-RHS FP32 passes: one group per workgroup
+        m = wave_max(m);
+        if ((threadIdx.x & 63) == 0)
+            atomicMax(tmax + (blockIdx.x & (TMAX_SLOTS - 1)), __float_as_uint(m));
     }
 }
 
@@ -4139,6 +4147,336 @@ __global__ void __launch_bounds__(WG) k_rows_rrc64(const int *items, const int *
     }
 }
 
+// ---------------------------------------------------------------------------
+// Complex 32-RHS products in FP32 arithmetic by row ranges (the FP64 walk
+// above with FP32 tiles). The vector entries are RHS-interleaved (re, im)
+// FP32 pairs, 256 bytes per entry (zc: x from the gather, t from the V phase).
+// A chunk is RRC32_CHUNK columns. FP32 entries: a lane's 16-byte load is two
+// complex entries (columns 2 g, 2 g + 1 of an 8-column step, MFMA m taking
+// column 2 g + m as k = g), three v_mfma_f32_16x16x4f32 per 16 right-hand
+// sides (Gauss, as in FP64), x staged as in zc. 16-bit entries: two loads
+// give eight complex entries (columns 8 g .. 8 g + 7 of a 32-column step),
+// deinterleaved into Re and Im operands of v_mfma_f32_16x16x32, x staged split
+// into 16-bit terms per part ([term][re, im][column / 8][RHS][column % 8]);
+// four products per term (Re Re, Im Im, Re Im, Im Re: no sum of two 16-bit
+// entries is formed, so their rounding stays the storage's). Unit sums FP32,
+// then FP64.
+constexpr int RRC32_CHUNK = 64;
+
+template <typename TS>
+constexpr int rrc32_stage_bytes() {
+    if constexpr (Mma16<TS>::on)
+        return Mma16<TS>::NB * 2 * RRC32_CHUNK * 32 * 2;
+    else
+        return RRC32_CHUNK * 32 * 8;
+}
+
+// The staged chunk: v[i] = the (re, im) pairs of right-hand sides 2 q, 2 q + 1
+// of chunk column f / 16 (q = f % 16, f = thread + WG i), zero past the
+// columns; FP16 terms times sc.
+template <typename TS>
+__device__ __forceinline__ void rrc32_store(unsigned char *buf, const float4 (&v)[4], int j, int ncols, float sc) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int f = (int)threadIdx.x + WG * i, cl = f >> 4, q = f & 15;
+        const bool in = j * RRC32_CHUNK + cl < ncols;
+        const float4 x = in ? v[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if constexpr (!Mma16<TS>::on) {
+            reinterpret_cast<float4 *>(buf)[f] = x;
+        } else {
+            using M  = Mma16<TS>;
+            using EL = typename M::E;
+            EL *be   = reinterpret_cast<EL *>(buf);
+            const float xs[2][2] = {{x.x, x.y}, {x.z, x.w}}; // [RHS 2q + e][re, im]
+#pragma unroll
+            for (int e = 0; e < 2; e++)
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    EL o[M::NB];
+                    M::split1(xs[e][p], sc, o);
+#pragma unroll
+                    for (int t = 0; t < M::NB; t++)
+                        be[(((size_t)(t * 2 + p) * (RRC32_CHUNK / 8) + (cl >> 3)) * 32 + 2 * q + e) * 8 + (cl & 7)] =
+                            o[t];
+                }
+        }
+    }
+}
+
+// One work item's rows (wave w: rows 16 w + r) over the columns walked by
+// rr_walk; acc: [RHS half][re, im][row] FP64 sums.
+template <typename TS, typename Load, typename XLoad, typename XStore>
+__device__ __forceinline__ void rrc32_walk(int nchunk, bool real, const unsigned char *lds, float sc, Load load,
+                                           XLoad xload, XStore xstore, double (&accd)[2][2][4]) {
+    constexpr bool H16 = Mma16<TS>::on;
+    constexpr int NS   = H16 ? 2 : 4;                       // steps per unit
+    constexpr int SC   = H16 ? 32 : 8;                      // columns per step
+    constexpr int UPC  = RRC32_CHUNK / (SC * NS);           // units per chunk (1 or 2)
+    constexpr int BUF  = rrc32_stage_bytes<TS>();
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const double inv = 1.0 / (double)sc;
+    auto run = [&](int u, const RrcUnit &b) {
+        if (!real)
+            return;
+        const int j = u / UPC, s = u % UPC;
+        const unsigned char *buf = lds + (j & 1) * BUF;
+        if constexpr (!H16) {
+            floatx4 K[2][3] = {};
+#pragma unroll
+            for (int st = 0; st < NS; st++) {
+                const int cs = s * NS + st;
+                const float a[4] = {__uint_as_float(b.a[st].x), __uint_as_float(b.a[st].y),
+                                    __uint_as_float(b.a[st].z), __uint_as_float(b.a[st].w)};
+#pragma unroll
+                for (int m = 0; m < 2; m++) {
+                    const float ar = a[2 * m], ai = a[2 * m + 1], as = ar + ai;
+                    const float2 *bp = reinterpret_cast<const float2 *>(buf) + (size_t)(cs * 8 + 2 * g + m) * 32;
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const float2 x = bp[16 * h + r];
+                        K[h][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(as, x.x, K[h][0], 0, 0, 0);
+                        K[h][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, x.y - x.x, K[h][1], 0, 0, 0);
+                        K[h][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, x.x + x.y, K[h][2], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    accd[h][0][i] += (double)K[h][0][i] - (double)K[h][2][i];
+                    accd[h][1][i] += (double)K[h][0][i] + (double)K[h][1][i];
+                }
+        } else {
+            using M = Mma16<TS>;
+            floatx4 P[2][4] = {}; // [half][re re, im im, re im, im re]
+#pragma unroll
+            for (int st = 0; st < NS; st++) {
+                const int cs = s * NS + st;
+                TS p0[8], p1[8];
+                const u4x w0 = b.a[2 * st], w1 = b.a[2 * st + 1];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const unsigned u0 = w0[e], u1 = w1[e];
+                    p0[2 * e]     = __builtin_bit_cast(TS, (unsigned short)(u0 & 0xffffu));
+                    p0[2 * e + 1] = __builtin_bit_cast(TS, (unsigned short)(u0 >> 16));
+                    p1[2 * e]     = __builtin_bit_cast(TS, (unsigned short)(u1 & 0xffffu));
+                    p1[2 * e + 1] = __builtin_bit_cast(TS, (unsigned short)(u1 >> 16));
+                }
+                typename M::V are, aim;
+                deinterleave16<TS>(p0, p1, true, true, are, aim);
+                const typename M::V *bp = reinterpret_cast<const typename M::V *>(buf);
+#pragma unroll
+                for (int t = 0; t < M::NB; t++) {
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const typename M::V xr = bp[((size_t)(t * 2 + 0) * (RRC32_CHUNK / 8) + cs * 4 + g) * 32 + 16 * h + r];
+                        const typename M::V xi = bp[((size_t)(t * 2 + 1) * (RRC32_CHUNK / 8) + cs * 4 + g) * 32 + 16 * h + r];
+                        P[h][0] = M::mma16(are, xr, P[h][0]);
+                        P[h][1] = M::mma16(aim, xi, P[h][1]);
+                        P[h][2] = M::mma16(are, xi, P[h][2]);
+                        P[h][3] = M::mma16(aim, xr, P[h][3]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    accd[h][0][i] += ((double)P[h][0][i] - (double)P[h][1][i]) * inv;
+                    accd[h][1][i] += ((double)P[h][2][i] + (double)P[h][3][i]) * inv;
+                }
+        }
+    };
+    rr_walk<UPC, RrcUnit>(nchunk, load, run, xload, xstore);
+}
+
+// The largest |re|, |im| of the x entries of each 32-column block of zc (the
+// FP16 scale of the complex 32-RHS V phase): one workgroup per block.
+__global__ void __launch_bounds__(WG) k_colmax_c32(const float *zc, int nc, float *xmax) {
+    __shared__ float s_m[NWAVE];
+    const int b = blockIdx.x;
+    const float4 *src = reinterpret_cast<const float4 *>(zc) + (size_t)b * 32 * 16; // 32 columns x 16 float4
+    float m = 0.0f;
+    for (int f = threadIdx.x; f < 32 * 16; f += WG)
+        if (b * 32 + (f >> 4) < nc) {
+            const float4 v = src[f];
+            m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+        }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0)
+        s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        xmax[b] = fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]));
+}
+
+// Complex RHS s0 .. s0 + 7 of original entry i to row iperm[i] of zc as FP32
+// (re, im) pairs (64 contiguous bytes per thread); workgroup 0 also clears the
+// TMAX_SLOTS maxima of the class-1 t entries.
+__global__ void k_scatter_xc32w(const double *x, const int *iperm_s, int nc, int mu_base, float *zc,
+                                unsigned *tmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.y == 0 && i < TMAX_SLOTS)
+        tmax[i] = 0u;
+    if (i >= nc)
+        return;
+    const int s0 = 8 * blockIdx.y;
+    float4 *o = reinterpret_cast<float4 *>(zc + ((size_t)iperm_s[i] * 32 + s0) * 2);
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+        const double2 a = *reinterpret_cast<const double2 *>(x + 2 * (i + (size_t)nc * (mu_base + s0 + u)));
+        const double2 c = *reinterpret_cast<const double2 *>(x + 2 * (i + (size_t)nc * (mu_base + s0 + u + 1)));
+        o[u / 2] = make_float4((float)a.x, (float)a.y, (float)c.x, (float)c.y);
+    }
+}
+
+template <typename TS0, typename TS1>
+__global__ void __launch_bounds__(WG) k_vx_rrc32(const VTile *items, const TS0 *varena0, const TS1 *varena1,
+                                                 float *zc, const float *xmax, unsigned *tmax, double *part,
+                                                 const double *tscale) {
+    constexpr int B0 = rrc32_stage_bytes<TS0>(), B1 = rrc32_stage_bytes<TS1>();
+    __shared__ __align__(16) unsigned char lds[2 * (B0 > B1 ? B0 : B1)];
+    const VTile d = items[blockIdx.x];
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 15, g = lane >> 4;
+    const int K = d.k, nchunk = (d.ncols + RRC32_CHUNK - 1) / RRC32_CHUNK;
+    const float4 *xsrc = reinterpret_cast<const float4 *>(zc) + (size_t)d.x_off * 16;
+    float4 xv[4];
+    auto xload = [&](int j) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int f = (int)threadIdx.x + WG * i;
+            xv[i] = xsrc[(size_t)min(j * RRC32_CHUNK + (f >> 4), d.ncols - 1) * 16 + (f & 15)];
+        }
+    };
+    double accd[2][2][4] = {};
+    auto go = [&](auto tag, const auto *varena) {
+        using TS = decltype(tag);
+        constexpr bool H16 = Mma16<TS>::on;
+        constexpr int NS = H16 ? 2 : 4, SC = H16 ? 32 : 8, LPS = H16 ? 2 : 1; // loads per step
+        const int nstep = (d.ncols + SC - 1) / SC;
+        // lane (r, g): row 16 w + r, columns (2 or 8) g .. of each step, 8 bytes per FP32 complex entry
+        const TS *rp = varena + 2 * (d.v_off + (size_t)min(16 * w + r, K - 1) * d.ld) + (H16 ? 16 : 4) * g;
+        auto load = [&](int u, RrcUnit &b) {
+#pragma unroll
+            for (int st = 0; st < NS; st++)
+#pragma unroll
+                for (int l = 0; l < LPS; l++)
+                    b.a[st * LPS + l] = nt_load16(rp + 2 * (size_t)min(u * NS + st, nstep - 1) * SC + 8 * l);
+        };
+        // FP16: one scale for the item (the largest over its chunks), so the unit sums share it
+        float sc = 1.0f;
+        if constexpr (H16 && Mma16<TS>::scaled) {
+            const int c0 = d.x_off, c1 = d.x_off + d.ncols - 1;
+            float m = 0.0f;
+            for (int b = c0 >> 5; b <= (c1 >> 5); b++)
+                m = fmaxf(m, xmax[b]);
+            sc = pass_scale(m);
+        }
+        auto xstore = [&](int j) { rrc32_store<TS>(lds + (j & 1) * rrc32_stage_bytes<TS>(), xv, j, d.ncols, sc); };
+        rrc32_walk<TS>(nchunk, 16 * w < K, lds, sc, load, xload, xstore, accd);
+    };
+    if (d.cls == 0)
+        go(TS0{}, varena0);
+    else
+        go(TS1{}, varena1);
+    float m = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) { // lane (r, g): rows 4 g + i, RHS 16 h + r
+        const int row = 16 * w + 4 * g + i;
+        if (row < K) {
+            const double f = (tscale && d.nparts == 1) ? tscale[d.t_index + row] : 1.0;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (d.nparts > 1) {
+                    reinterpret_cast<double2 *>(part)[(size_t)(d.part_off + row) * 32 + 16 * h + r] =
+                        make_double2(accd[h][0][i], accd[h][1][i]);
+                } else {
+                    const float re = (float)(accd[h][0][i] * f), im = (float)(accd[h][1][i] * f);
+                    reinterpret_cast<float2 *>(zc)[(size_t)(d.t_index + row) * 32 + 16 * h + r] = make_float2(re, im);
+                    m = fmaxf(m, fmaxf(fabsf(re), fabsf(im)));
+                }
+            }
+        }
+    }
+    if (d.cls == 1 && d.nparts == 1) {
+        m = wave_max(m);
+        if (lane == 0)
+            atomicMax(tmax + (blockIdx.x & (TMAX_SLOTS - 1)), __float_as_uint(m));
+    }
+}
+
+template <typename TS0, typename TS1>
+__global__ void __launch_bounds__(WG) k_rows_rrc32(const int *items, const int *rb_row0, RowClass c0,
+                                                   const TS0 *arena0, RowClass c1, const TS1 *arena1, const float *zc,
+                                                   const unsigned *tmax, const int *perm_t, int nr, int mu_base,
+                                                   double *y) {
+    constexpr int B0 = rrc32_stage_bytes<TS0>(), B1 = rrc32_stage_bytes<TS1>();
+    __shared__ __align__(16) unsigned char lds[2 * (B0 > B1 ? B0 : B1)];
+    const int item = items[blockIdx.x], rb = item & 0xffffff, roff = RR_ROWS * (item >> 24);
+    const int r0 = rb_row0[rb] + roff, R = min(RR_ROWS, rb_row0[rb + 1] - r0);
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = lane & 15, g = lane >> 4;
+    const float4 *zf = reinterpret_cast<const float4 *>(zc);
+    double accd[2][2][4] = {};
+    auto go = [&](auto tag, const RowClass &rc, const auto *arena, float sc) {
+        using TS = decltype(tag);
+        constexpr bool H16 = Mma16<TS>::on;
+        constexpr int NS = H16 ? 2 : 4, SC = H16 ? 32 : 8, LPS = H16 ? 2 : 1;
+        const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
+        if (C == 0)
+            return;
+        const int Cs     = (C + rc.salign - 1) & ~(rc.salign - 1);
+        const int nstep  = (C + SC - 1) / SC;
+        const int nchunk = (C + RRC32_CHUNK - 1) / RRC32_CHUNK;
+        const TS *rp = arena + 2 * (rc.rb_aoff[rb] + (size_t)(roff + min(16 * w + r, R - 1)) * Cs) + (H16 ? 16 : 4) * g;
+        const int *ci = rc.colidx + cb;
+        auto load = [&](int u, RrcUnit &b) {
+#pragma unroll
+            for (int st = 0; st < NS; st++)
+#pragma unroll
+                for (int l = 0; l < LPS; l++)
+                    b.a[st * LPS + l] = nt_load16(rp + 2 * (size_t)min(u * NS + st, nstep - 1) * SC + 8 * l);
+        };
+        float4 xv[4];
+        auto xload = [&](int j) {
+            int idx[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                idx[i] = ci[min(j * RRC32_CHUNK + (((int)threadIdx.x + WG * i) >> 4), C - 1)];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                xv[i] = zf[(size_t)idx[i] * 16 + (((int)threadIdx.x + WG * i) & 15)];
+        };
+        auto xstore = [&](int j) { rrc32_store<TS>(lds + (j & 1) * rrc32_stage_bytes<TS>(), xv, j, C, sc); };
+        rrc32_walk<TS>(nchunk, 16 * w < R, lds, sc, load, xload, xstore, accd);
+    };
+    go(TS0{}, c0, arena0, 1.0f);
+    if (arena1) {
+        float sc = 1.0f;
+        if constexpr (Mma16<TS1>::on && Mma16<TS1>::scaled) {
+            float m = 0.0f;
+            for (int i = threadIdx.x & 63; i < TMAX_SLOTS; i += 64)
+                m = fmaxf(m, __uint_as_float(tmax[i]));
+            sc = pass_scale(wave_max(m));
+        }
+        go(TS1{}, c1, arena1, sc);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int row = 16 * w + 4 * g + i;
+        if (row < R) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                double *o = y + 2 * ((size_t)perm_t[r0 + row] + (size_t)nr * (mu_base + 16 * h + r));
+                o[0]      = accd[h][0][i];
+                o[1]      = accd[h][1][i];
+            }
+        }
+    }
+}
+
 template <typename TS0, typename TS1, typename R, int MU>
 __global__ void __launch_bounds__(WG) k_rows_c(const int *rb_order, const int *rb_row0, RowClass c0, const TS0 *arena0,
                                                RowClass c1, const TS1 *arena1, const R *z, const int *perm_t, int nr,
@@ -4516,6 +4854,42 @@ void launch_rrc64(const VTile *items, int nitems, const VTile *multi_tiles, int 
         hipLaunchKernelGGL(k_rows_rrc64, dim3(nrb_items), dim3(WG), 0, st, rb_items, rb_row0, c0, arena0, c1, arena1,
                            z, perm_t, nr, mu_base, y);
 }
+
+template <typename TS0, typename TS1>
+void launch_rrc32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *zc, float *xmax,
+                  unsigned *tmax, const VTile *items, int nitems, const VTile *multi_tiles, int nmulti,
+                  const int *rb_items, int nrb_items, const int *rb_row0, RowClass c0, const TS0 *arena0, RowClass c1,
+                  const TS1 *arena1, const TS0 *varena0, const TS1 *varena1, double *part, const double *tscale,
+                  const int *perm_t, int nr, double *y, hipStream_t st, hipEvent_t ev_gather, hipEvent_t ev_vx) {
+    hipLaunchKernelGGL(k_scatter_xc32w, dim3((max(nc, TMAX_SLOTS) + WG - 1) / WG, 4), dim3(WG), 0, st, x, iperm_s, nc,
+                       mu_base, zc, tmax);
+    if (Mma16<TS0>::scaled || Mma16<TS1>::scaled)
+        hipLaunchKernelGGL(k_colmax_c32, dim3((tpad + 31) / 32), dim3(WG), 0, st, zc, nc, xmax);
+    if (ev_gather)
+        (void)hipEventRecord(ev_gather, st);
+    if (nitems > 0) {
+        hipLaunchKernelGGL((k_vx_rrc32<TS0, TS1>), dim3(nitems), dim3(WG), 0, st, items, varena0, varena1, zc, xmax,
+                           tmax, part, tscale);
+        if (nmulti > 0)
+            hipLaunchKernelGGL((k_vx_sum_c<float, 32>), dim3(nmulti), dim3(WG), 0, st, multi_tiles, nmulti, part, zc,
+                               tscale, tmax);
+    }
+    if (ev_vx)
+        (void)hipEventRecord(ev_vx, st);
+    if (nrb_items > 0)
+        hipLaunchKernelGGL((k_rows_rrc32<TS0, TS1>), dim3(nrb_items), dim3(WG), 0, st, rb_items, rb_row0, c0, arena0,
+                           c1, arena1, zc, tmax, perm_t, nr, mu_base, y);
+}
+#define HTG_RRC32_INST(TS0, TS1)                                                                                  \
+    template void launch_rrc32<TS0, TS1>(const double *, const int *, int, int, int, float *, float *, unsigned *, \
+                                         const VTile *, int, const VTile *, int, const int *, int, const int *,    \
+                                         RowClass, const TS0 *, RowClass, const TS1 *, const TS0 *, const TS1 *,   \
+                                         double *, const double *, const int *, int, double *, hipStream_t,        \
+                                         hipEvent_t, hipEvent_t);
+HTG_RRC32_INST(float, float)
+HTG_RRC32_INST(float, __half)
+HTG_RRC32_INST(float, __hip_bfloat16)
+#undef HTG_RRC32_INST
 
 template <typename TS0, typename TS1, typename R>
 void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const VTile *multi_tiles, int nmulti,
