@@ -3858,131 +3858,6 @@ __global__ void __launch_bounds__(WG) k_vx_mfma64c16(const int *wv_off, int nwav
 // 16-wide RHS tiles, three MFMAs per complex product). The z block of 32
 // complex RHS is far larger than the caches, which caps this form below the
 // matrix-core roofline; at 16 RHS the LDS-staged kernels above are faster.
-constexpr int MU_C64W = 32;
-
-// One 16-row tile of a row-major complex FP64 block (rows row0 + r16, stride
-// ld, rows past nrows read as zero) times the vector entries of columns
-// [0, ncols) for the 32 RHS, into three Gauss accumulators per RHS tile j:
-// K1 += br (ar + ai), K2 += ar (bi - br), K3 += ai (br + bi), so that
-// re = K1 - K3, im = K1 + K2. The vector entry of column c for RHS s is
-// z[zidx(c) 32 + s] (re, im). Columns go in blocks of 16 (four k-steps).
-template <class ZIdx>
-__device__ __forceinline__ void tile_mfma64c(const double *A, size_t ld, int row0, int nrows, int ncols,
-                                             const double *z, ZIdx zidx, doublex4 (&acc)[2][3]) {
-    constexpr int MU = MU_C64W, RT = MU / 16, NK = 4;
-    const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
-    const int row    = row0 + r16;
-    const bool rok   = row < nrows;
-    const double *rp = A + 2 * (size_t)(rok ? row : 0) * ld;
-    for (int c0 = 0; c0 < ncols; c0 += 4 * NK) {
-        int ci[NK];
-#pragma unroll
-        for (int it = 0; it < NK; it++) {
-            const int c = c0 + 4 * it + g;
-            ci[it]      = zidx(c < ncols ? c : 0);
-        }
-        double a[NK][2], x[NK][RT][2];
-#pragma unroll
-        for (int it = 0; it < NK; it++) {
-            const int c = c0 + 4 * it + g;
-            a[it][0] = a[it][1] = 0.0;
-            if (rok && c < ncols)
-                Vec<double>::stream(rp + 2 * c, a[it]);
-#pragma unroll
-            for (int j = 0; j < RT; j++)
-                Vec<double>::load(z + 2 * ((size_t)ci[it] * MU + 16 * j + r16), x[it][j]);
-        }
-#pragma unroll
-        for (int it = 0; it < NK; it++) {
-            const double as = a[it][0] + a[it][1];
-#pragma unroll
-            for (int j = 0; j < RT; j++) {
-                const double bd = x[it][j][1] - x[it][j][0], bs = x[it][j][0] + x[it][j][1];
-                acc[j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(as, x[it][j][0], acc[j][0], 0, 0, 0);
-                acc[j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[it][0], bd, acc[j][1], 0, 0, 0);
-                acc[j][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[it][1], bs, acc[j][2], 0, 0, 0);
-            }
-        }
-    }
-}
-
-// Row phase: one workgroup per row block, its waves take 16-row tiles in turn
-// over both classes' columns (vector entries gathered through the column
-// indices).
-__global__ void __launch_bounds__(WG) k_rows_mfma64c32(const int *rb_order, const int *rb_row0, RowClass c0,
-                                                       const double *arena0, RowClass c1, const double *arena1,
-                                                       const double *z, const int *perm_t, int nr, int mu_base,
-                                                       double *y) {
-    constexpr int MU = MU_C64W, RT = MU / 16;
-    const int rb = rb_order[blockIdx.x];
-    const int r0 = rb_row0[rb], R = rb_row0[rb + 1] - r0;
-    const int w    = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
-    for (int rw = 16 * w; rw < R; rw += 16 * NWAVE) {
-        doublex4 acc[RT][3] = {};
-        for (int cls = 0; cls < 2; cls++) {
-            const RowClass &rc  = cls == 0 ? c0 : c1;
-            const double *arena = cls == 0 ? arena0 : arena1;
-            if (!arena)
-                continue;
-            const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
-            const int Cp = (C + rc.align - 1) & ~(rc.align - 1);    // columns read (vector width)
-            const int Cs = (C + rc.salign - 1) & ~(rc.salign - 1);  // row stride (128-byte rows)
-            if (C == 0)
-                continue;
-            const int *ci = rc.colidx + cb;
-            tile_mfma64c(arena + 2 * rc.rb_aoff[rb], (size_t)Cs, rw, R, Cp, z,
-                         [&](int c) { return ci[c < C ? c : 0]; }, acc);
-        }
-#pragma unroll
-        for (int j = 0; j < RT; j++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int orow = rw + 4 * i + g;
-                if (orow < R) {
-                    double *o = y + 2 * ((size_t)perm_t[r0 + orow] + (size_t)nr * (mu_base + 16 * j + r16));
-                    o[0]      = acc[j][0][i] - acc[j][2][i];
-                    o[1]      = acc[j][0][i] + acc[j][1][i];
-                }
-            }
-    }
-}
-
-// V x: a wave takes 16 V rows of its chunks at a time (the chunk's x slice
-// read from L2).
-__global__ void __launch_bounds__(WG) k_vx_mfma64c32(const int *wv_off, int nwaves, const VTile *tiles,
-                                                     const double *varena0, const double *varena1, double *z,
-                                                     double *part, const double *tscale) {
-    constexpr int MU = MU_C64W, RT = MU / 16;
-    const int wv     = __builtin_amdgcn_readfirstlane(blockIdx.x * NWAVE + (threadIdx.x >> 6));
-    if (wv >= nwaves)
-        return;
-    const int lane = threadIdx.x & 63, g = lane >> 4, r16 = lane & 15;
-    for (int ti = wv_off[wv]; ti < wv_off[wv + 1]; ti++) {
-        const VTile d      = tiles[ti];
-        const double *vare = d.cls == 0 ? varena0 : varena1;
-        const int x0       = d.x_off;
-        for (int rt = 0; rt < d.k; rt += 16) {
-            doublex4 acc[RT][3] = {};
-            tile_mfma64c(vare + 2 * d.v_off, (size_t)d.ld, rt, d.k, d.ncols, z, [&](int c) { return x0 + c; }, acc);
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int r = rt + 4 * i + g;
-                if (r < d.k) {
-                    const double sc = (tscale && d.nparts == 1) ? tscale[d.t_index + r] : 1.0;
-#pragma unroll
-                    for (int j = 0; j < RT; j++) {
-                        double *o = d.nparts > 1 ? part + 2 * ((size_t)(d.part_off + r) * MU + 16 * j + r16)
-                                                 : z + 2 * ((size_t)(d.t_index + r) * MU + 16 * j + r16);
-                        o[0]      = (acc[j][0][i] - acc[j][2][i]) * sc;
-                        o[1]      = (acc[j][0][i] + acc[j][1][i]) * sc;
-                    }
-                }
-            }
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Complex FP64 32-RHS products by row ranges: the real FP64 walk (rr_walk)
 // with (re, im) entries. A chunk is RRC_CHUNK columns of the vector entries
@@ -4899,22 +4774,12 @@ void launch_vx_c(const int *vg_off, int ngroups, const VTile *tiles, const VTile
         return;
     if constexpr (std::is_same<R, double>::value && std::is_same<TS0, double>::value &&
                   std::is_same<TS1, double>::value) {
-        if (MU == MU_C64W || MU == MU_C64) {
-            const dim3 grid((ngroups + NWAVE - 1) / NWAVE);
-            if (MU == MU_C64W)
-                hipLaunchKernelGGL(k_vx_mfma64c32, grid, dim3(WG), 0, st, vg_off, ngroups, tiles, varena0, varena1, z,
-                                   part, tscale);
-            else
-                hipLaunchKernelGGL(k_vx_mfma64c16, grid, dim3(WG), 0, st, vg_off, ngroups, tiles, varena0, varena1, z,
-                                   part, tscale);
-            if (nmulti > 0) {
-                if (MU == 32)
-                    hipLaunchKernelGGL((k_vx_sum_c<R, 32>), dim3(nmulti), dim3(WG), 0, st, multi_tiles, nmulti, part,
-                                       z, tscale);
-                else
-                    hipLaunchKernelGGL((k_vx_sum_c<R, 16>), dim3(nmulti), dim3(WG), 0, st, multi_tiles, nmulti, part,
-                                       z, tscale);
-            }
+        if (MU == MU_C64) {
+            hipLaunchKernelGGL(k_vx_mfma64c16, dim3((ngroups + NWAVE - 1) / NWAVE), dim3(WG), 0, st, vg_off, ngroups,
+                               tiles, varena0, varena1, z, part, tscale);
+            if (nmulti > 0)
+                hipLaunchKernelGGL((k_vx_sum_c<R, 16>), dim3(nmulti), dim3(WG), 0, st, multi_tiles, nmulti, part, z,
+                                   tscale);
             return;
         }
     }
@@ -4949,11 +4814,6 @@ void launch_rows_c(const int *rb_order, const int *rb_row0, int nrb, RowClass c0
         return;
     if constexpr (std::is_same<R, double>::value && std::is_same<TS0, double>::value &&
                   std::is_same<TS1, double>::value) {
-        if (MU == MU_C64W) {
-            hipLaunchKernelGGL(k_rows_mfma64c32, dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0, c1,
-                               arena1, z, perm_t, nr, mu_base, y);
-            return;
-        }
         if (MU == MU_C64) {
             hipLaunchKernelGGL(k_rows_mfma64c16, dim3(nrb), dim3(WG), 0, st, rb_order, rb_row0, c0, arena0, c1,
                                arena1, z, perm_t, nr, mu_base, y);
