@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--matmat-counters-run", action="store_true",
                     help="short run for rocprofv3 --pmc on the 32-RHS products: three products per storage "
                          "(FP64, FP32, FP32+FP16) of the headline H-matrices; prints nothing else")
+    ap.add_argument("--helmholtz-counters-run", action="store_true",
+                    help="short run for rocprofv3 on the complex 64-RHS products of the Helmholtz N=2e5 workload: "
+                         "one product per storage (FP64, FP32, FP32+FP16, FP32+BF16); prints nothing else")
     ap.add_argument("--probe-only", action="store_true",
                     help="run only the device bandwidth probe (htg_stream_probe) and print its JSON")
     ap.add_argument("--counters-run", action="store_true",
@@ -308,6 +311,35 @@ def helmholtz_2e5(torch, log):
     return out
 
 
+def helmholtz_products_only(torch, log):
+    """The Helmholtz N=2e5 workload of helmholtz_2e5 (every low-rank term
+    16-bit in the mixed storages), one 64-RHS product per storage after a
+    256 MiB copy that calibrates the traffic counters."""
+    import numpy as np
+    from htool_amd import Generator, HMatrix, capi, geometry
+    n = 200000
+    pts = geometry.ellipsoid(n, (1.0, 1.0, 4.0), seed=5)
+    h = math.sqrt(geometry.prolate_spheroid_area(1.0, 4.0) / n)
+    H = HMatrix(epsilon=1e-5, eta=10.0, minclustersize=32)
+    H.set_compression("partialACA")
+    H.set_lowprec(0.0, all_terms=True)
+    H.build(Generator(geometry.KERNEL_HELMHOLTZ, 0.0, geometry.laplace_diag(h), 2 * math.pi / (10 * h)), pts)
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    capi.check(capi.lib().htg_stream_copy(C_void(a.data_ptr()), C_void(b.data_ptr()), 256 << 20))
+    del a, b
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((n, 64)) + 1j * rng.standard_normal((n, 64))
+    dX = torch.from_numpy(np.asfortranarray(X).reshape(-1, order="F").copy()).cuda()
+    dY = torch.zeros_like(dX)
+    sp = torch.cuda.current_stream().cuda_stream
+    for storage in ("fp64", "fp32", "fp32_fp16", "fp32_bf16"):
+        H.set_storage(storage)
+        H.mvprod_device(dX.data_ptr(), dY.data_ptr(), 64, sp)
+    torch.cuda.synchronize()
+    log("helmholtz counters run done")
+
+
 def assembly_5e5(torch, log):
     """N=5e5 unit sphere, Laplace, FP64 ACA at epsilon 1e-6: build without and
     with the trained rank predictor (trained on other sizes and geometries), the
@@ -371,6 +403,9 @@ def main():
         print(json.dumps({"read_gbps": round(rd.value, 1), "copy_gbps": round(cp.value, 1),
                           "read_best_grid": bg.value}), flush=True)
         return
+    if args.helmholtz_counters_run:
+        helmholtz_products_only(torch, log)
+        return
     n = args.n
     pts = geometry.sphere(n, seed=0)
     h = geometry.sphere_spacing(n)
@@ -414,6 +449,11 @@ def main():
         log("counters run done")
         return
     if args.matmat_counters_run:
+        from htool_amd import capi
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB: calibrates FETCH/WRITE_SIZE
+        b = torch.empty_like(a)
+        capi.check(capi.lib().htg_stream_copy(C_void(a.data_ptr()), C_void(b.data_ptr()), 256 << 20))
+        del a, b
         dX = torch.from_numpy(rng.standard_normal(n * 32)).cuda()
         dY = torch.zeros(n * 32, dtype=torch.float64, device="cuda")
         for M, storage in ((H, "fp64"), (H, "fp32"), (Hm, "fp32_fp16")):

@@ -3617,7 +3617,8 @@ __global__ void __launch_bounds__(WG) k_vx_sum_c(const VTile *first, int nmulti,
     const int km     = d.k * MU;
     const double *pb = part + 2 * (size_t)d.part_base * MU;
     float m = 0.0f;
-    for (int e = threadIdx.x % TPG; e < km; e += TPG) {
+    // gridDim.y workgroups share one group's entries
+    for (int e = threadIdx.x % TPG + TPG * blockIdx.y; e < km; e += TPG * gridDim.y) {
         double v[2] = {0, 0};
         sum_parts<2>(pb + 2 * (size_t)e, 2 * (size_t)km, d.nparts, v);
         const double sc = tscale ? tscale[d.t_index + e / MU] : 1.0;
@@ -3629,7 +3630,7 @@ __global__ void __launch_bounds__(WG) k_vx_sum_c(const VTile *first, int nmulti,
     if (tmax && d.cls == 1) { // complex 32-RHS FP32 passes: one group per workgroup
         m = wave_max(m);
         if ((threadIdx.x & 63) == 0)
-            atomicMax(tmax + (blockIdx.x & (TMAX_SLOTS - 1)), __float_as_uint(m));
+            atomicMax(tmax + ((blockIdx.x + blockIdx.y) & (TMAX_SLOTS - 1)), __float_as_uint(m));
     }
 }
 
@@ -4719,9 +4720,9 @@ void launch_rrc64(const VTile *items, int nitems, const VTile *multi_tiles, int 
                   hipEvent_t ev_vx_done) {
     if (nitems > 0) {
         hipLaunchKernelGGL(k_vx_rrc64, dim3(nitems), dim3(WG), 0, st, items, varena0, varena1, z, part, tscale);
-        if (nmulti > 0)
-            hipLaunchKernelGGL((k_vx_sum_c<double, 32>), dim3(nmulti), dim3(WG), 0, st, multi_tiles, nmulti, part, z,
-                               tscale);
+        if (nmulti > 0) // a range's rows x 32 sums (<= 2048) over four workgroups
+            hipLaunchKernelGGL((k_vx_sum_c<double, 32>), dim3(nmulti, 4), dim3(WG), 0, st, multi_tiles, nmulti, part,
+                               z, tscale);
     }
     if (ev_vx_done)
         (void)hipEventRecord(ev_vx_done, st);
@@ -4746,8 +4747,8 @@ void launch_rrc32(const double *x, const int *iperm_s, int nc, int tpad, int mu_
         hipLaunchKernelGGL((k_vx_rrc32<TS0, TS1>), dim3(nitems), dim3(WG), 0, st, items, varena0, varena1, zc, xmax,
                            tmax, part, tscale);
         if (nmulti > 0)
-            hipLaunchKernelGGL((k_vx_sum_c<float, 32>), dim3(nmulti), dim3(WG), 0, st, multi_tiles, nmulti, part, zc,
-                               tscale, tmax);
+            hipLaunchKernelGGL((k_vx_sum_c<float, 32>), dim3(nmulti, 4), dim3(WG), 0, st, multi_tiles, nmulti, part,
+                               zc, tscale, tmax);
     }
     if (ev_vx)
         (void)hipEventRecord(ev_vx, st);
