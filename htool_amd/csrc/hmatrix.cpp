@@ -115,11 +115,13 @@ struct htg_hmatrix {
     bool scaled = false; // some low-rank terms carry scales (recompressed)
 
     std::vector<int> rb_row0, rb_order;
-    // single-RHS row phase with two precision classes: rb_order with every row
-    // block cut into parts of at most ROWS_SPLIT rows (entry = rb | p << 24 | np << 28)
+    // single-RHS row phase with two precision classes: every row block cut into
+    // parts of at most ROWS_SPLIT rows (entry = rb | p << 24 | np << 28), in XCD
+    // order (padding entries 0xffffff)
     std::vector<int> rb_parts;
-    // 32-RHS row phase: rb_order with every row block cut into parts of at
-    // most RR_ITEM_ROWS rows (entry = rb | p << 24)
+    // multi-RHS row-range row phase: every row block cut into parts of at most
+    // RR_ITEM_ROWS rows (entry = rb | p << 24), in XCD order by stripes of
+    // consecutive row blocks (padding entries 0xffffff)
     std::vector<int> rb_items;
     // per leaf, per covered row block: (row block, class-0 tile offset, class-1 tile offset)
     std::vector<std::vector<std::tuple<int, uint64_t, uint64_t>>> leaf_tiles;
@@ -169,6 +171,37 @@ namespace {
 int t_entry(const htg_hmatrix &h, int l, int j) {
     const int kh = h.leaves[l].k_hi;
     return j < kh ? h.tpad + h.t_first[0][l] + j : h.tpad + h.t_first[1][l] + (j - kh);
+}
+
+// A work list in XCD order. Workgroup b runs on XCD b % 8, each XCD with its
+// own L2; a group is work that reads the same vector entries (the parts of a
+// row block, neighbouring row blocks, the row ranges of a V panel), so all of a
+// group goes to one XCD, its entries adjacent so that they run at about the
+// same time. Groups, heaviest first, go to the least loaded XCD; XCD j's list
+// takes positions 8 i + j, shorter lists padded with `pad` (an entry the
+// kernels skip).
+template <typename E>
+std::vector<E> xcd_order(const std::vector<std::vector<E>> &groups, const std::vector<int64_t> &weight, const E &pad) {
+    constexpr int NX = 8;
+    std::vector<int> order(groups.size());
+    for (size_t g = 0; g < groups.size(); g++)
+        order[g] = (int)g;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return weight[a] > weight[b]; });
+    std::vector<std::vector<E>> lst(NX);
+    std::vector<int64_t> load(NX, 0);
+    for (int g : order) {
+        const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lst[j].insert(lst[j].end(), groups[g].begin(), groups[g].end());
+        load[j] += weight[g];
+    }
+    size_t len = 0;
+    for (const auto &l : lst)
+        len = std::max(len, l.size());
+    std::vector<E> out(len * NX, pad);
+    for (int j = 0; j < NX; j++)
+        for (size_t i = 0; i < lst[j].size(); i++)
+            out[i * NX + j] = lst[j][i];
+    return out;
 }
 
 // Product layout.
@@ -328,36 +361,42 @@ void build_layout(htg_hmatrix &h) {
         return (int64_t)(h.rb_row0[r + 1] - h.rb_row0[r]) * cols;
     };
     std::stable_sort(h.rb_order.begin(), h.rb_order.end(), [&](int a, int b) { return work(a) > work(b); });
+    // Multi-RHS row phase: parts of RR_ITEM_ROWS rows, in stripes of
+    // consecutive row blocks (rows under the same leaves read the same t
+    // entries, neighbours share near-field columns) of about 1/128 of the work,
+    // each stripe on one XCD.
+    {
+        constexpr int NSTRIPE = 128;
+        int64_t total = 0;
+        for (int r = 0; r < nrb; r++)
+            total += work(r);
+        const int64_t cap = std::max<int64_t>(1, total / NSTRIPE);
+        std::vector<std::vector<int>> stripes;
+        std::vector<int64_t> sw;
+        for (int r = 0; r < nrb; r++) {
+            if (stripes.empty() || sw.back() >= cap) {
+                stripes.emplace_back();
+                sw.push_back(0);
+            }
+            for (int p = 0; p * RR_ITEM_ROWS < h.rb_row0[r + 1] - h.rb_row0[r]; p++)
+                stripes.back().push_back(r | p << 24);
+            sw.back() += work(r);
+        }
+        h.rb_items = xcd_order(stripes, sw, 0xffffff);
+    }
     // Single right-hand side with two precision classes: more, shorter
     // workgroups balance the phase better (a row block's columns are staged
-    // once per part).
-    h.rb_items.clear();
-    for (int r : h.rb_order)
-        for (int p = 0; p * RR_ITEM_ROWS < h.rb_row0[r + 1] - h.rb_row0[r]; p++)
-            h.rb_items.push_back(r | p << 24);
-    h.rb_parts.clear();
+    // once per part); all parts of a row block on one XCD.
     {
-        // all parts of a row block on one XCD (workgroup b runs on XCD b % 8),
-        // so its column indices and vector entries are fetched into one L2:
-        // whole row blocks, heaviest first, to the least loaded XCD; list j
-        // goes to positions 8 i + j, shorter lists padded with empty entries
-        constexpr int NX = 8;
-        std::vector<std::vector<int>> lst(NX);
-        std::vector<int64_t> load(NX, 0);
-        for (int r : h.rb_order) {
+        std::vector<std::vector<int>> blocks(nrb);
+        std::vector<int64_t> bw(nrb);
+        for (int r = 0; r < nrb; r++) {
             const int np = (h.rb_row0[r + 1] - h.rb_row0[r] + ROWS_SPLIT - 1) / ROWS_SPLIT;
-            const int j  = (int)(std::min_element(load.begin(), load.end()) - load.begin());
             for (int p = 0; p < np; p++)
-                lst[j].push_back(r | p << 24 | (np > 1 ? np : 0) << 28);
-            load[j] += work(r);
+                blocks[r].push_back(r | p << 24 | (np > 1 ? np : 0) << 28);
+            bw[r] = work(r);
         }
-        size_t len = 0;
-        for (auto &l : lst)
-            len = std::max(len, l.size());
-        h.rb_parts.assign(len * NX, 0xffffff);
-        for (int j = 0; j < NX; j++)
-            for (size_t i = 0; i < lst[j].size(); i++)
-                h.rb_parts[i * NX + j] = lst[j][i];
+        h.rb_parts = xcd_order(blocks, bw, 0xffffff);
     }
 
     // V arena: per class, panel by panel, the stacked V rows of the panel's
@@ -479,9 +518,9 @@ void build_layout(htg_hmatrix &h) {
         }
         h.part_size = std::max(h.part_size, part_size); // the cuts never run at once
     }
-    // 32-RHS row ranges: rows [r0, r0 + RR_ITEM_ROWS) of a panel's class, their
-    // columns cut into ranges of whole staged chunks holding about RR_WORK
-    // entries each (fewer, longer ranges for fewer rows).
+    // Multi-RHS row ranges: rows [r0, r0 + RR_ITEM_ROWS) of a panel's class,
+    // their columns cut into ranges of whole staged chunks holding about RR_WORK
+    // entries each (fewer, longer ranges for fewer rows), in XCD order by panel.
     for (int cut = 3; cut < 5; cut++) {
         VTileList &T = h.vt[cut];
         T.tiles.clear();
@@ -489,9 +528,11 @@ void build_layout(htg_hmatrix &h) {
         T.vg_off.assign(1, 0);
         const int cw  = cut == 3 ? RR_ITEM_CHUNK32 : RR_ITEM_CHUNK64;
         int part_size = 0;
+        std::vector<std::vector<VTile>> groups(h.panels.size()); // per panel, both classes
         for (int c = 0; c < 2; c++) {
             const ClassLayout &C = h.cls[c];
-            for (const auto &P : h.panels) {
+            for (size_t pi = 0; pi < h.panels.size(); pi++) {
+                const auto &P = h.panels[pi];
                 int K = 0, first = -1;
                 for (int l : P)
                     if (terms(h.leaves[l], c) > 0) {
@@ -528,15 +569,22 @@ void build_layout(htg_hmatrix &h) {
                         v.cls       = c;
                         if (np > 1 && q == 0)
                             T.multi.push_back(v);
-                        T.tiles.push_back(v);
+                        groups[pi].push_back(v);
                     }
                 }
             }
         }
-        // heaviest first (16-bit rows count half)
-        std::stable_sort(T.tiles.begin(), T.tiles.end(), [](const VTile &a, const VTile &b) {
-            return (int64_t)a.k * a.ncols * (a.cls == 0 ? 2 : 1) > (int64_t)b.k * b.ncols * (b.cls == 0 ? 2 : 1);
-        });
+        // a panel's items on one XCD, those reading the same x columns adjacent;
+        // weights: entries, 16-bit rows counting half
+        std::vector<int64_t> gw(groups.size(), 0);
+        for (size_t pi = 0; pi < groups.size(); pi++) {
+            std::stable_sort(groups[pi].begin(), groups[pi].end(), [](const VTile &a, const VTile &b) {
+                return a.cls != b.cls ? a.cls < b.cls : a.x_off < b.x_off;
+            });
+            for (const VTile &v : groups[pi])
+                gw[pi] += (int64_t)v.k * v.ncols * (v.cls == 0 ? 2 : 1);
+        }
+        T.tiles = xcd_order(groups, gw, VTile{}); // padding: k = 0
         h.part_size = std::max(h.part_size, part_size);
     }
 }
@@ -564,7 +612,7 @@ void upload_layout(htg_hmatrix &h) {
     // padding entries are always finite zeros.
     h.d_z.alloc((size_t)(h.tpad + h.t_size) * 64);
     h.d_z.zero();
-    h.d_part.alloc((size_t)std::max(h.part_size, 1) * 64);
+    h.d_part.alloc((size_t)std::max(h.part_size, 1) * 64 * h.esz()); // complex FP32 passes: 64 (re, im) per row
     h.d_xmax.alloc((size_t)h.tpad / 32 + 1);
     // multi-RHS gathers: the inverse column permutation and an nc x 32 FP32
     // staging buffer, allocated here so that products allocate nothing
@@ -572,7 +620,7 @@ void upload_layout(htg_hmatrix &h) {
     for (int j = 0; j < h.nc; j++)
         ip[h.perm_s[j]] = j;
     h.d_iperm_s.upload(ip);
-    h.d_xstage.alloc((size_t)(h.tpad + h.t_size) * 32 * h.esz()); // complex: (re, im) pairs
+    h.d_xstage.alloc((size_t)(h.tpad + h.t_size) * 32 * (h.cplx ? 4 : 1)); // complex: 64 (re, im) pairs
     h.d_xstage.zero(); // padding entries [nc, tpad) stay zero
     h.d_tmax.alloc(TMAX_SLOTS);
     h.generation->fetch_add(1); // graphs captured before read freed buffers
@@ -680,15 +728,16 @@ void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream
     const RowClass rc1{h.cls[1].d_rb_col0.p, h.cls[1].d_rb_aoff.p, h.cls[1].d_colidx.p, CLASS_ALIGN[1], ROW_ALIGN[1]};
     int done = 0;
     while (done < mu) {
-        // 32 complex RHS per pass (the row-range kernels), fewer for the remainder
-        const int MU = pick_mu_chunk(mu - done, 32);
+        // complex RHS per pass: 32 (FP64) or 64 (FP32 arithmetic) on the row-range
+        // kernels, fewer for the remainder
+        const int MU = pick_mu_chunk(mu - done, std::is_same<R, float>::value ? 64 : 32);
         if (ev)
             HTG_HIP(hipEventRecord(ev[0], st));
         if constexpr (std::is_same<R, float>::value) {
-            if (MU == 32) {
+            if (MU >= 32) {
                 const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
                 const VTileList &S = h.vt[4];
-                launch_rrc32<TS0, TS1>(d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, h.d_xmax.p, h.d_tmax.p,
+                launch_rrc32<TS0, TS1>(MU, d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, h.d_xmax.p, h.d_tmax.p,
                                        S.d_tiles.p, (int)S.tiles.size(), S.d_multi.p, (int)S.multi.size(),
                                        h.d_rb_items.p, (int)h.rb_items.size(), h.d_rb_row0.p, rc0,
                                        arena_of<TS0>(C0, false), rc1, has1 ? arena_of<TS1>(C1, false) : nullptr,

@@ -2439,6 +2439,52 @@ __device__ __forceinline__ void rr_walk(int nchunk, Load load, Run run, XLoad xl
     }
 }
 
+// rr_walk for one unit per chunk with every load two units ahead: while unit
+// u runs, the matrix entries of units u + 1 and u + 2 and the x of chunks
+// u + 1 (in registers until its store to LDS after unit u) and u + 2 are in
+// flight, so each load's latency is covered by two units of work. Three
+// register buffers for the entries, two for x (xload / xstore take the x
+// buffer); the loop is unrolled to six units so that every buffer is a fixed
+// set of registers.
+template <typename Buf, typename XS, typename Load, typename Run, typename XLoad, typename XStore>
+__device__ __forceinline__ void rr_walk2(int nchunk, Load load, Run run, XLoad xload, XStore xstore) {
+    const int U = nchunk;
+    Buf A0, A1, A2;
+    XS X0, X1;
+    load(0, A0);
+    xload(0, X0);
+    load(min(1, U - 1), A1);
+    xload(min(1, U - 1), X1);
+    xstore(0, X0);
+    __syncthreads();
+    auto step = [&](int u, const Buf &cur, Buf &nxt2, XS &xl, const XS &xs) {
+        xload(min(u + 2, U - 1), xl);
+        load(min(u + 2, U - 1), nxt2);
+        run(u, cur);
+        if (u + 1 < U)
+            xstore(u + 1, xs);
+        __syncthreads();
+    };
+    for (int u = 0; u < U; u += 6) {
+        step(u, A0, A2, X0, X1);
+        if (u + 1 >= U)
+            break;
+        step(u + 1, A1, A0, X1, X0);
+        if (u + 2 >= U)
+            break;
+        step(u + 2, A2, A1, X0, X1);
+        if (u + 3 >= U)
+            break;
+        step(u + 3, A0, A2, X1, X0);
+        if (u + 4 >= U)
+            break;
+        step(u + 4, A1, A0, X0, X1);
+        if (u + 5 >= U)
+            break;
+        step(u + 5, A2, A1, X1, X0);
+    }
+}
+
 // Rows [0, d.k) of a work item (<= 64: wave w owns the 16-row tile w), class
 // storage TS, FP32 arithmetic. Steps are 16 columns for FP32 entries
 // (v_mfma_f32_16x16x4f32: lane (r, g) = (l % 16, l / 16) loads row r's columns
@@ -2613,6 +2659,8 @@ __global__ void __launch_bounds__(WG) k_vx_rr(const VTile *items, const TS0 *var
     constexpr int B0 = rr_stage_bytes<TS0>(), B1 = rr_stage_bytes<TS1>();
     __shared__ __align__(16) unsigned char lds[2 * (B0 > B1 ? B0 : B1)];
     const VTile d = items[blockIdx.x];
+    if (d.k == 0) // padding entry of an XCD-ordered list
+        return;
     if (d.cls == 0)
         rr_rows32<TS0>(d, varena0, z, zil, xmax, tmax, lds, part, tscale);
     else
@@ -2633,6 +2681,8 @@ __global__ void __launch_bounds__(WG) k_vx_rr64(const VTile *items, const double
         u4 a[NS];
     };
     const VTile d = items[blockIdx.x];
+    if (d.k == 0) // padding entry of an XCD-ordered list
+        return;
     const double *varena = d.cls == 0 ? varena0 : varena1;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
@@ -2829,6 +2879,8 @@ __global__ void __launch_bounds__(WG) k_rows_rr(const int *items, const int *rb_
     constexpr int B0 = rr_stage_bytes<TS0>(), B1 = rr_stage_bytes<TS1>();
     __shared__ __align__(16) unsigned char lds[2 * (B0 > B1 ? B0 : B1)];
     const int item = items[blockIdx.x], rb = item & 0xffffff, roff = RR_ROWS * (item >> 24);
+    if (rb == 0xffffff) // padding entry of an XCD-ordered list
+        return;
     const int r0 = rb_row0[rb] + roff, R = min(RR_ROWS, rb_row0[rb + 1] - r0);
     double accd[2][4] = {};
     rb_class32<TS0>(c0, arena0, rb, roff, R, zil, 1.0f, lds, accd);
@@ -2868,6 +2920,8 @@ __global__ void __launch_bounds__(WG) k_rows_rr64(const int *items, const int *r
         u4 a[NS];
     };
     const int item = items[blockIdx.x], rb = item & 0xffffff, roff = RR_ROWS * (item >> 24);
+    if (rb == 0xffffff) // padding entry of an XCD-ordered list
+        return;
     const int r0 = rb_row0[rb] + roff, R = min(RR_ROWS, rb_row0[rb + 1] - r0);
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
@@ -3910,6 +3964,8 @@ __global__ void __launch_bounds__(WG) k_vx_rrc64(const VTile *items, const doubl
     constexpr int MU = 32, NS = 4, SC = 4, XPT = RRC_CHUNK * MU / WG; // complex entries staged per thread (4)
     __shared__ __align__(16) double s_x[2 * RRC_CHUNK * MU * 2];
     const VTile d = items[blockIdx.x];
+    if (d.k == 0) // padding entry of an XCD-ordered list
+        return;
     const double *varena = d.cls == 0 ? varena0 : varena1;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
@@ -3966,6 +4022,8 @@ __global__ void __launch_bounds__(WG) k_rows_rrc64(const int *items, const int *
     constexpr int MU = 32, NS = 4, SC = 4, XPT = RRC_CHUNK * MU / WG;
     __shared__ __align__(16) double s_x[2 * RRC_CHUNK * MU * 2];
     const int item = items[blockIdx.x], rb = item & 0xffffff, roff = RR_ROWS * (item >> 24);
+    if (rb == 0xffffff) // padding entry of an XCD-ordered list
+        return;
     const int r0 = rb_row0[rb] + roff, R = min(RR_ROWS, rb_row0[rb + 1] - r0);
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
@@ -4024,80 +4082,144 @@ __global__ void __launch_bounds__(WG) k_rows_rrc64(const int *items, const int *
 }
 
 // ---------------------------------------------------------------------------
-// Complex 32-RHS products in FP32 arithmetic by row ranges (the FP64 walk
-// above with FP32 tiles). The vector entries are RHS-interleaved (re, im)
-// FP32 pairs, 256 bytes per entry (zc: x from the gather, t from the V phase).
-// A chunk is RRC32_CHUNK columns. FP32 entries: a lane's 16-byte load is two
-// complex entries (columns 2 g, 2 g + 1 of an 8-column step, MFMA m taking
-// column 2 g + m as k = g), three v_mfma_f32_16x16x4f32 per 16 right-hand
-// sides (Gauss, as in FP64), x staged as in zc. 16-bit entries: two loads
-// give eight complex entries (columns 8 g .. 8 g + 7 of a 32-column step),
-// deinterleaved into Re and Im operands of v_mfma_f32_16x16x32, x staged split
-// into 16-bit terms per part ([term][re, im][column / 8][RHS][column % 8]);
-// four products per term (Re Re, Im Im, Re Im, Im Re: no sum of two 16-bit
-// entries is formed, so their rounding stays the storage's). Unit sums FP32,
-// then FP64.
-constexpr int RRC32_CHUNK = 64;
-
-template <typename TS>
-constexpr int rrc32_stage_bytes() {
-    if constexpr (Mma16<TS>::on)
-        return Mma16<TS>::NB * 2 * RRC32_CHUNK * 32 * 2;
-    else
-        return RRC32_CHUNK * 32 * 8;
+// Complex products in FP32 arithmetic by row ranges, MUC = 32 or 64 complex
+// right-hand sides per pass (the FP64 walk above with FP32 tiles). The vector
+// entries are RHS-interleaved (re, im) FP32 pairs, 8 MUC bytes per entry (zc:
+// x from the gather, t from the V phase). A chunk is rrc_chunk<MUC> columns
+// (64 for 32 RHS, 32 for 64: 16 KB of FP32 x either way). FP32 entries: a
+// lane's 16-byte load is two complex entries (columns 2 g, 2 g + 1 of an
+// 8-column step, MFMA m taking column 2 g + m as k = g), three
+// v_mfma_f32_16x16x4f32 per 16 right-hand sides (Gauss, as in FP64), x staged
+// as in zc. 16-bit entries: two loads give eight complex entries (columns
+// 8 g .. 8 g + 7 of a 32-column step), deinterleaved into Re and Im operands of
+// v_mfma_f32_16x16x32, x staged split into 16-bit terms per part ([term][re,
+// im][column / 8][RHS][column % 8]); four products per term (Re Re, Im Im,
+// Re Im, Im Re: no sum of two 16-bit entries is formed, so their rounding
+// stays the storage's). Unit sums FP32, then FP64. 64 right-hand sides read
+// every stored entry once per 64 instead of per 32.
+template <int MUC>
+constexpr int rrc_chunk() {
+    return MUC == 32 ? 64 : 32;
 }
 
-// The staged chunk: v[i] = the (re, im) pairs of right-hand sides 2 q, 2 q + 1
-// of chunk column f / 16 (q = f % 16, f = thread + WG i), zero past the
-// columns; FP16 terms times sc.
-template <typename TS>
-__device__ __forceinline__ void rrc32_store(unsigned char *buf, const float4 (&v)[4], int j, int ncols, float sc) {
+template <typename TS, int MUC>
+constexpr int rrc32_stage_bytes() {
+    if constexpr (Mma16<TS>::on)
+        return Mma16<TS>::NB * 2 * rrc_chunk<MUC>() * MUC * 2;
+    else
+        return rrc_chunk<MUC>() * MUC * 8;
+}
+
+// x of a chunk in registers between its loads and its store to LDS: FP32
+// entries (float4 = two right-hand sides of a column), 16-bit entries (float2 =
+// one right-hand side of a column).
+struct RrcX32 {
+    float4 v[4];
+};
+struct RrcX16 {
+    float2 v[8];
+};
+
+// The staged chunk, FP32 entries: v[i] = the (re, im) pairs of right-hand
+// sides 2 q, 2 q + 1 of chunk column f / (MUC / 2) (q = f % (MUC / 2), f =
+// thread + WG i), zero past the columns, stored as loaded.
+template <int MUC>
+__device__ __forceinline__ void rrc32_store(unsigned char *buf, const float4 (&v)[4], int j, int ncols) {
+    constexpr int CH = rrc_chunk<MUC>(), QP = MUC / 2;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        const int f = (int)threadIdx.x + WG * i, cl = f >> 4, q = f & 15;
-        const bool in = j * RRC32_CHUNK + cl < ncols;
-        const float4 x = in ? v[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if constexpr (!Mma16<TS>::on) {
-            reinterpret_cast<float4 *>(buf)[f] = x;
+        const int f = (int)threadIdx.x + WG * i;
+        reinterpret_cast<float4 *>(buf)[f] = j * CH + f / QP < ncols ? v[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// The staged chunk, 16-bit entries: thread (s, cg) = (thread % MUC, thread /
+// MUC) holds v[c] = the (re, im) pair of right-hand side s at chunk column
+// 8 cg + c (zero past the columns: rrc16_load) and writes each 16-bit term of
+// the eight re and of the eight im parts (FP16: times sc) as one 16-byte
+// vector, in the layout [term][re, im][column / 8][RHS][column % 8]: a wave
+// writes 1 KB contiguous.
+template <typename TS, int MUC>
+__device__ __forceinline__ void rrc16_store(unsigned char *buf, const float2 (&v)[8], float sc) {
+    using M            = Mma16<TS>;
+    constexpr int CH   = rrc_chunk<MUC>();
+    const int s = (int)threadIdx.x % MUC, cg = (int)threadIdx.x / MUC;
+    float re[8], im[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        re[c] = v[c].x;
+        im[c] = v[c].y;
+    }
+    typename M::V ore[M::NB], oim[M::NB];
+    M::split(re, sc, ore);
+    M::split(im, sc, oim);
+    typename M::V *bv = reinterpret_cast<typename M::V *>(buf);
+#pragma unroll
+    for (int t = 0; t < M::NB; t++) {
+        bv[((size_t)(t * 2 + 0) * (CH / 8) + cg) * MUC + s] = ore[t];
+        bv[((size_t)(t * 2 + 1) * (CH / 8) + cg) * MUC + s] = oim[t];
+    }
+}
+
+// Thread (s, cg)'s x of chunk j for rrc16_store: v[c] = entry col(8 cg + c)
+// of right-hand side s (z2: (re, im) pairs, MUC per entry), zero for chunk
+// columns at or past ncols; col(i) = the z entry of chunk column i, or, for an
+// int col, col + i.
+template <int MUC, typename Col>
+__device__ __forceinline__ void rrc16_load(float2 (&v)[8], const float2 *z2, int j, int ncols, Col col) {
+    constexpr int CH = rrc_chunk<MUC>();
+    const int s = (int)threadIdx.x % MUC, c0 = j * CH + 8 * ((int)threadIdx.x / MUC);
+    if (c0 + 8 <= ncols) {
+        if constexpr (std::is_same<Col, int>::value) { // entries col + i: one address, immediate offsets
+            const float2 *p = z2 + ((size_t)col + c0) * MUC + s;
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                v[c] = p[c * MUC];
         } else {
-            using M  = Mma16<TS>;
-            using EL = typename M::E;
-            EL *be   = reinterpret_cast<EL *>(buf);
-            const float xs[2][2] = {{x.x, x.y}, {x.z, x.w}}; // [RHS 2q + e][re, im]
+            int e[8];
 #pragma unroll
-            for (int e = 0; e < 2; e++)
+            for (int c = 0; c < 8; c++)
+                e[c] = col(c0 + c);
 #pragma unroll
-                for (int p = 0; p < 2; p++) {
-                    EL o[M::NB];
-                    M::split1(xs[e][p], sc, o);
-#pragma unroll
-                    for (int t = 0; t < M::NB; t++)
-                        be[(((size_t)(t * 2 + p) * (RRC32_CHUNK / 8) + (cl >> 3)) * 32 + 2 * q + e) * 8 + (cl & 7)] =
-                            o[t];
-                }
+            for (int c = 0; c < 8; c++)
+                v[c] = z2[(size_t)e[c] * MUC + s];
         }
+    } else if constexpr (std::is_same<Col, int>::value) {
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            v[c] = c0 + c < ncols ? z2[((size_t)col + c0 + c) * MUC + s] : make_float2(0.0f, 0.0f);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            v[c] = c0 + c < ncols ? z2[(size_t)col(c0 + c) * MUC + s] : make_float2(0.0f, 0.0f);
     }
 }
 
 // One work item's rows (wave w: rows 16 w + r) over the columns walked by
-// rr_walk; acc: [RHS half][re, im][row] FP64 sums.
-template <typename TS, typename Load, typename XLoad, typename XStore>
+// rr_walk / rr_walk2; acc: [RHS sixteen][re, im][row] FP64 sums, fed by FP32
+// sums over one unit (FP32 entries) or two (16-bit entries).
+template <typename TS, int MUC, typename XS, typename Load, typename XLoad, typename XStore>
 __device__ __forceinline__ void rrc32_walk(int nchunk, bool real, const unsigned char *lds, float sc, Load load,
-                                           XLoad xload, XStore xstore, double (&accd)[2][2][4]) {
+                                           XLoad xload, XStore xstore, double (&accd)[MUC / 16][2][4]) {
     constexpr bool H16 = Mma16<TS>::on;
-    constexpr int NS   = H16 ? 2 : 4;                       // steps per unit
-    constexpr int SC   = H16 ? 32 : 8;                      // columns per step
-    constexpr int UPC  = RRC32_CHUNK / (SC * NS);           // units per chunk (1 or 2)
-    constexpr int BUF  = rrc32_stage_bytes<TS>();
+    constexpr int CH   = rrc_chunk<MUC>();
+    constexpr int NS   = H16 ? (MUC == 32 ? 2 : 1) : 4; // steps per unit
+    constexpr int SC   = H16 ? 32 : 8;                  // columns per step
+    constexpr int UPC  = CH / (SC * NS);                // units per chunk (1 or 2)
+    constexpr int BUF  = rrc32_stage_bytes<TS, MUC>();
+    constexpr int NH   = MUC / 16;
+    static_assert(UPC >= 1 && UPC * SC * NS == CH, "whole units per chunk");
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int U = nchunk * UPC;
     const double inv = 1.0 / (double)sc;
+    floatx4 P16[NH][2] = {}; // 16-bit entries: [sixteen][re, im] FP32 sums of the current two units
     auto run = [&](int u, const RrcUnit &b) {
         if (!real)
             return;
         const int j = u / UPC, s = u % UPC;
         const unsigned char *buf = lds + (j & 1) * BUF;
         if constexpr (!H16) {
-            floatx4 K[2][3] = {};
+            floatx4 K[NH][3] = {};
 #pragma unroll
             for (int st = 0; st < NS; st++) {
                 const int cs = s * NS + st;
@@ -4106,9 +4228,9 @@ __device__ __forceinline__ void rrc32_walk(int nchunk, bool real, const unsigned
 #pragma unroll
                 for (int m = 0; m < 2; m++) {
                     const float ar = a[2 * m], ai = a[2 * m + 1], as = ar + ai;
-                    const float2 *bp = reinterpret_cast<const float2 *>(buf) + (size_t)(cs * 8 + 2 * g + m) * 32;
+                    const float2 *bp = reinterpret_cast<const float2 *>(buf) + (size_t)(cs * 8 + 2 * g + m) * MUC;
 #pragma unroll
-                    for (int h = 0; h < 2; h++) {
+                    for (int h = 0; h < NH; h++) {
                         const float2 x = bp[16 * h + r];
                         K[h][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(as, x.x, K[h][0], 0, 0, 0);
                         K[h][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, x.y - x.x, K[h][1], 0, 0, 0);
@@ -4116,19 +4238,20 @@ __device__ __forceinline__ void rrc32_walk(int nchunk, bool real, const unsigned
                     }
                 }
             }
+            // re = K0 - K2, im = K0 + K1 formed in FP32: their rounding (2^-24 of the
+            // result) is below that of the FP32 sums K themselves
 #pragma unroll
-            for (int h = 0; h < 2; h++)
+            for (int h = 0; h < NH; h++)
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    accd[h][0][i] += (double)K[h][0][i] - (double)K[h][2][i];
-                    accd[h][1][i] += (double)K[h][0][i] + (double)K[h][1][i];
+                    accd[h][0][i] += (double)(K[h][0][i] - K[h][2][i]);
+                    accd[h][1][i] += (double)(K[h][0][i] + K[h][1][i]);
                 }
         } else {
             using M = Mma16<TS>;
-            floatx4 P[2][4] = {}; // [half][re re, im im, re im, im re]
+            typename M::V are[NS], aim[NS], aimn[NS]; // aimn = -aim (sign bits flipped)
 #pragma unroll
             for (int st = 0; st < NS; st++) {
-                const int cs = s * NS + st;
                 TS p0[8], p1[8];
                 const u4x w0 = b.a[2 * st], w1 = b.a[2 * st + 1];
 #pragma unroll
@@ -4139,43 +4262,60 @@ __device__ __forceinline__ void rrc32_walk(int nchunk, bool real, const unsigned
                     p1[2 * e]     = __builtin_bit_cast(TS, (unsigned short)(u1 & 0xffffu));
                     p1[2 * e + 1] = __builtin_bit_cast(TS, (unsigned short)(u1 >> 16));
                 }
-                typename M::V are, aim;
-                deinterleave16<TS>(p0, p1, true, true, are, aim);
-                const typename M::V *bp = reinterpret_cast<const typename M::V *>(buf);
+                deinterleave16<TS>(p0, p1, true, true, are[st], aim[st]);
+                aimn[st] = __builtin_bit_cast(typename M::V, __builtin_bit_cast(u4x, aim[st]) ^ 0x80008000u);
+            }
+            const typename M::V *bp = reinterpret_cast<const typename M::V *>(buf);
+            // re += Re a Re x - Im a Im x, im += Re a Im x + Im a Re x (two MFMAs each)
 #pragma unroll
-                for (int t = 0; t < M::NB; t++) {
+            for (int h = 0; h < NH; h++)
 #pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        const typename M::V xr = bp[((size_t)(t * 2 + 0) * (RRC32_CHUNK / 8) + cs * 4 + g) * 32 + 16 * h + r];
-                        const typename M::V xi = bp[((size_t)(t * 2 + 1) * (RRC32_CHUNK / 8) + cs * 4 + g) * 32 + 16 * h + r];
-                        P[h][0] = M::mma16(are, xr, P[h][0]);
-                        P[h][1] = M::mma16(aim, xi, P[h][1]);
-                        P[h][2] = M::mma16(are, xi, P[h][2]);
-                        P[h][3] = M::mma16(aim, xr, P[h][3]);
+                for (int st = 0; st < NS; st++) {
+                    const int cs = s * NS + st;
+#pragma unroll
+                    for (int t = 0; t < M::NB; t++) {
+                        const typename M::V xr = bp[((size_t)(t * 2 + 0) * (CH / 8) + cs * 4 + g) * MUC + 16 * h + r];
+                        const typename M::V xi = bp[((size_t)(t * 2 + 1) * (CH / 8) + cs * 4 + g) * MUC + 16 * h + r];
+                        P16[h][0] = M::mma16(are[st], xr, P16[h][0]);
+                        P16[h][0] = M::mma16(aimn[st], xi, P16[h][0]);
+                        P16[h][1] = M::mma16(are[st], xi, P16[h][1]);
+                        P16[h][1] = M::mma16(aim[st], xr, P16[h][1]);
                     }
                 }
+            // FP32 sums over two units, then FP64
+            if ((u & 1) || u == U - 1) {
+#pragma unroll
+                for (int h = 0; h < NH; h++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        accd[h][0][i] += (double)P16[h][0][i] * inv;
+                        accd[h][1][i] += (double)P16[h][1][i] * inv;
+                        P16[h][0][i] = 0.0f;
+                        P16[h][1][i] = 0.0f;
+                    }
             }
-#pragma unroll
-            for (int h = 0; h < 2; h++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    accd[h][0][i] += ((double)P[h][0][i] - (double)P[h][1][i]) * inv;
-                    accd[h][1][i] += ((double)P[h][2][i] + (double)P[h][3][i]) * inv;
-                }
         }
     };
-    rr_walk<UPC, RrcUnit>(nchunk, load, run, xload, xstore);
+    if constexpr (UPC == 1) {
+        rr_walk2<RrcUnit, XS>(nchunk, load, run, xload, xstore);
+    } else {
+        XS x;
+        rr_walk<UPC, RrcUnit>(
+            nchunk, load, run, [&](int j) { xload(j, x); }, [&](int j) { xstore(j, x); });
+    }
 }
 
 // The largest |re|, |im| of the x entries of each 32-column block of zc (the
-// FP16 scale of the complex 32-RHS V phase): one workgroup per block.
+// FP16 scale of the complex V phase): one workgroup per block.
+template <int MUC>
 __global__ void __launch_bounds__(WG) k_colmax_c32(const float *zc, int nc, float *xmax) {
+    constexpr int FPC = MUC / 2; // float4 per column
     __shared__ float s_m[NWAVE];
     const int b = blockIdx.x;
-    const float4 *src = reinterpret_cast<const float4 *>(zc) + (size_t)b * 32 * 16; // 32 columns x 16 float4
+    const float4 *src = reinterpret_cast<const float4 *>(zc) + (size_t)b * 32 * FPC;
     float m = 0.0f;
-    for (int f = threadIdx.x; f < 32 * 16; f += WG)
-        if (b * 32 + (f >> 4) < nc) {
+    for (int f = threadIdx.x; f < 32 * FPC; f += WG)
+        if (b * 32 + f / FPC < nc) {
             const float4 v = src[f];
             m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
         }
@@ -4190,6 +4330,7 @@ __global__ void __launch_bounds__(WG) k_colmax_c32(const float *zc, int nc, floa
 // Complex RHS s0 .. s0 + 7 of original entry i to row iperm[i] of zc as FP32
 // (re, im) pairs (64 contiguous bytes per thread); workgroup 0 also clears the
 // TMAX_SLOTS maxima of the class-1 t entries.
+template <int MUC>
 __global__ void k_scatter_xc32w(const double *x, const int *iperm_s, int nc, int mu_base, float *zc,
                                 unsigned *tmax) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -4198,7 +4339,7 @@ __global__ void k_scatter_xc32w(const double *x, const int *iperm_s, int nc, int
     if (i >= nc)
         return;
     const int s0 = 8 * blockIdx.y;
-    float4 *o = reinterpret_cast<float4 *>(zc + ((size_t)iperm_s[i] * 32 + s0) * 2);
+    float4 *o = reinterpret_cast<float4 *>(zc + ((size_t)iperm_s[i] * MUC + s0) * 2);
 #pragma unroll
     for (int u = 0; u < 8; u += 2) {
         const double2 a = *reinterpret_cast<const double2 *>(x + 2 * (i + (size_t)nc * (mu_base + s0 + u)));
@@ -4207,32 +4348,34 @@ __global__ void k_scatter_xc32w(const double *x, const int *iperm_s, int nc, int
     }
 }
 
-template <typename TS0, typename TS1>
+template <typename TS0, typename TS1, int MUC>
 __global__ void __launch_bounds__(WG) k_vx_rrc32(const VTile *items, const TS0 *varena0, const TS1 *varena1,
                                                  float *zc, const float *xmax, unsigned *tmax, double *part,
                                                  const double *tscale) {
-    constexpr int B0 = rrc32_stage_bytes<TS0>(), B1 = rrc32_stage_bytes<TS1>();
+    constexpr int CH = rrc_chunk<MUC>(), QP = MUC / 2, NH = MUC / 16;
+    constexpr int B0 = rrc32_stage_bytes<TS0, MUC>(), B1 = rrc32_stage_bytes<TS1, MUC>();
     __shared__ __align__(16) unsigned char lds[2 * (B0 > B1 ? B0 : B1)];
     const VTile d = items[blockIdx.x];
+    if (d.k == 0) // padding entry of an XCD-ordered list
+        return;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
-    const int K = d.k, nchunk = (d.ncols + RRC32_CHUNK - 1) / RRC32_CHUNK;
-    const float4 *xsrc = reinterpret_cast<const float4 *>(zc) + (size_t)d.x_off * 16;
-    float4 xv[4];
-    auto xload = [&](int j) {
+    const int K = d.k, nchunk = (d.ncols + CH - 1) / CH;
+    const float4 *xsrc = reinterpret_cast<const float4 *>(zc) + (size_t)d.x_off * QP;
+    auto xload = [&](int j, RrcX32 &x) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int f = (int)threadIdx.x + WG * i;
-            xv[i] = xsrc[(size_t)min(j * RRC32_CHUNK + (f >> 4), d.ncols - 1) * 16 + (f & 15)];
+            x.v[i] = xsrc[(size_t)min(j * CH + f / QP, d.ncols - 1) * QP + f % QP];
         }
     };
-    double accd[2][2][4] = {};
+    double accd[NH][2][4] = {};
     auto go = [&](auto tag, const auto *varena) {
         using TS = decltype(tag);
         constexpr bool H16 = Mma16<TS>::on;
-        constexpr int NS = H16 ? 2 : 4, SC = H16 ? 32 : 8, LPS = H16 ? 2 : 1; // loads per step
+        constexpr int NS = H16 ? (MUC == 32 ? 2 : 1) : 4, SC = H16 ? 32 : 8, LPS = H16 ? 2 : 1; // loads per step
         const int nstep = (d.ncols + SC - 1) / SC;
-        // lane (r, g): row 16 w + r, columns (2 or 8) g .. of each step, 8 bytes per FP32 complex entry
+        // lane (r, g): row 16 w + r, columns (2 or 8) g .. of each step
         const TS *rp = varena + 2 * (d.v_off + (size_t)min(16 * w + r, K - 1) * d.ld) + (H16 ? 16 : 4) * g;
         auto load = [&](int u, RrcUnit &b) {
 #pragma unroll
@@ -4250,8 +4393,21 @@ __global__ void __launch_bounds__(WG) k_vx_rrc32(const VTile *items, const TS0 *
                 m = fmaxf(m, xmax[b]);
             sc = pass_scale(m);
         }
-        auto xstore = [&](int j) { rrc32_store<TS>(lds + (j & 1) * rrc32_stage_bytes<TS>(), xv, j, d.ncols, sc); };
-        rrc32_walk<TS>(nchunk, 16 * w < K, lds, sc, load, xload, xstore, accd);
+        unsigned char *buf = lds;
+        if constexpr (H16) {
+            // thread (s, cg): right-hand side s of chunk columns 8 cg .. 8 cg + 7
+            const float2 *z2 = reinterpret_cast<const float2 *>(zc);
+            auto xload16 = [&](int j, RrcX16 &x) { rrc16_load<MUC>(x.v, z2, j, d.ncols, d.x_off); };
+            auto xstore16 = [&](int j, const RrcX16 &x) {
+                rrc16_store<TS, MUC>(buf + (j & 1) * rrc32_stage_bytes<TS, MUC>(), x.v, sc);
+            };
+            rrc32_walk<TS, MUC, RrcX16>(nchunk, 16 * w < K, lds, sc, load, xload16, xstore16, accd);
+        } else {
+            auto xstore = [&](int j, const RrcX32 &x) {
+                rrc32_store<MUC>(buf + (j & 1) * rrc32_stage_bytes<TS, MUC>(), x.v, j, d.ncols);
+            };
+            rrc32_walk<TS, MUC, RrcX32>(nchunk, 16 * w < K, lds, sc, load, xload, xstore, accd);
+        }
     };
     if (d.cls == 0)
         go(TS0{}, varena0);
@@ -4264,13 +4420,13 @@ __global__ void __launch_bounds__(WG) k_vx_rrc32(const VTile *items, const TS0 *
         if (row < K) {
             const double f = (tscale && d.nparts == 1) ? tscale[d.t_index + row] : 1.0;
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < NH; h++) {
                 if (d.nparts > 1) {
-                    reinterpret_cast<double2 *>(part)[(size_t)(d.part_off + row) * 32 + 16 * h + r] =
+                    reinterpret_cast<double2 *>(part)[(size_t)(d.part_off + row) * MUC + 16 * h + r] =
                         make_double2(accd[h][0][i], accd[h][1][i]);
                 } else {
                     const float re = (float)(accd[h][0][i] * f), im = (float)(accd[h][1][i] * f);
-                    reinterpret_cast<float2 *>(zc)[(size_t)(d.t_index + row) * 32 + 16 * h + r] = make_float2(re, im);
+                    reinterpret_cast<float2 *>(zc)[(size_t)(d.t_index + row) * MUC + 16 * h + r] = make_float2(re, im);
                     m = fmaxf(m, fmaxf(fabsf(re), fabsf(im)));
                 }
             }
@@ -4283,29 +4439,32 @@ __global__ void __launch_bounds__(WG) k_vx_rrc32(const VTile *items, const TS0 *
     }
 }
 
-template <typename TS0, typename TS1>
+template <typename TS0, typename TS1, int MUC>
 __global__ void __launch_bounds__(WG) k_rows_rrc32(const int *items, const int *rb_row0, RowClass c0,
                                                    const TS0 *arena0, RowClass c1, const TS1 *arena1, const float *zc,
                                                    const unsigned *tmax, const int *perm_t, int nr, int mu_base,
                                                    double *y) {
-    constexpr int B0 = rrc32_stage_bytes<TS0>(), B1 = rrc32_stage_bytes<TS1>();
+    constexpr int CH = rrc_chunk<MUC>(), QP = MUC / 2, NH = MUC / 16;
+    constexpr int B0 = rrc32_stage_bytes<TS0, MUC>(), B1 = rrc32_stage_bytes<TS1, MUC>();
     __shared__ __align__(16) unsigned char lds[2 * (B0 > B1 ? B0 : B1)];
     const int item = items[blockIdx.x], rb = item & 0xffffff, roff = RR_ROWS * (item >> 24);
+    if (rb == 0xffffff) // padding entry of an XCD-ordered list
+        return;
     const int r0 = rb_row0[rb] + roff, R = min(RR_ROWS, rb_row0[rb + 1] - r0);
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
     const float4 *zf = reinterpret_cast<const float4 *>(zc);
-    double accd[2][2][4] = {};
+    double accd[NH][2][4] = {};
     auto go = [&](auto tag, const RowClass &rc, const auto *arena, float sc) {
         using TS = decltype(tag);
         constexpr bool H16 = Mma16<TS>::on;
-        constexpr int NS = H16 ? 2 : 4, SC = H16 ? 32 : 8, LPS = H16 ? 2 : 1;
+        constexpr int NS = H16 ? (MUC == 32 ? 2 : 1) : 4, SC = H16 ? 32 : 8, LPS = H16 ? 2 : 1;
         const int cb = rc.rb_col0[rb], C = rc.rb_col0[rb + 1] - cb;
         if (C == 0)
             return;
         const int Cs     = (C + rc.salign - 1) & ~(rc.salign - 1);
         const int nstep  = (C + SC - 1) / SC;
-        const int nchunk = (C + RRC32_CHUNK - 1) / RRC32_CHUNK;
+        const int nchunk = (C + CH - 1) / CH;
         const TS *rp = arena + 2 * (rc.rb_aoff[rb] + (size_t)(roff + min(16 * w + r, R - 1)) * Cs) + (H16 ? 16 : 4) * g;
         const int *ci = rc.colidx + cb;
         auto load = [&](int u, RrcUnit &b) {
@@ -4315,18 +4474,29 @@ __global__ void __launch_bounds__(WG) k_rows_rrc32(const int *items, const int *
                 for (int l = 0; l < LPS; l++)
                     b.a[st * LPS + l] = nt_load16(rp + 2 * (size_t)min(u * NS + st, nstep - 1) * SC + 8 * l);
         };
-        float4 xv[4];
-        auto xload = [&](int j) {
-            int idx[4];
+        if constexpr (H16) {
+            // thread (s, cg): right-hand side s of chunk columns 8 cg .. 8 cg + 7
+            const float2 *z2 = reinterpret_cast<const float2 *>(zc);
+            auto xload16 = [&](int j, RrcX16 &x) { rrc16_load<MUC>(x.v, z2, j, C, [&](int i) { return ci[i]; }); };
+            auto xstore16 = [&](int j, const RrcX16 &x) {
+                rrc16_store<TS, MUC>(lds + (j & 1) * rrc32_stage_bytes<TS, MUC>(), x.v, sc);
+            };
+            rrc32_walk<TS, MUC, RrcX16>(nchunk, 16 * w < R, lds, sc, load, xload16, xstore16, accd);
+        } else {
+            auto xload = [&](int j, RrcX32 &x) {
+                int idx[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                idx[i] = ci[min(j * RRC32_CHUNK + (((int)threadIdx.x + WG * i) >> 4), C - 1)];
+                for (int i = 0; i < 4; i++)
+                    idx[i] = ci[min(j * CH + ((int)threadIdx.x + WG * i) / QP, C - 1)];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                xv[i] = zf[(size_t)idx[i] * 16 + (((int)threadIdx.x + WG * i) & 15)];
-        };
-        auto xstore = [&](int j) { rrc32_store<TS>(lds + (j & 1) * rrc32_stage_bytes<TS>(), xv, j, C, sc); };
-        rrc32_walk<TS>(nchunk, 16 * w < R, lds, sc, load, xload, xstore, accd);
+                for (int i = 0; i < 4; i++)
+                    x.v[i] = zf[(size_t)idx[i] * QP + ((int)threadIdx.x + WG * i) % QP];
+            };
+            auto xstore = [&](int j, const RrcX32 &x) {
+                rrc32_store<MUC>(lds + (j & 1) * rrc32_stage_bytes<TS, MUC>(), x.v, j, C);
+            };
+            rrc32_walk<TS, MUC, RrcX32>(nchunk, 16 * w < R, lds, sc, load, xload, xstore, accd);
+        }
     };
     go(TS0{}, c0, arena0, 1.0f);
     if (arena1) {
@@ -4344,7 +4514,7 @@ __global__ void __launch_bounds__(WG) k_rows_rrc32(const int *items, const int *
         const int row = 16 * w + 4 * g + i;
         if (row < R) {
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < NH; h++) {
                 double *o = y + 2 * ((size_t)perm_t[r0 + row] + (size_t)nr * (mu_base + 16 * h + r));
                 o[0]      = accd[h][0][i];
                 o[1]      = accd[h][1][i];
@@ -4732,36 +4902,43 @@ void launch_rrc64(const VTile *items, int nitems, const VTile *multi_tiles, int 
 }
 
 template <typename TS0, typename TS1>
-void launch_rrc32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *zc, float *xmax,
-                  unsigned *tmax, const VTile *items, int nitems, const VTile *multi_tiles, int nmulti,
+void launch_rrc32(int muc, const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *zc,
+                  float *xmax, unsigned *tmax, const VTile *items, int nitems, const VTile *multi_tiles, int nmulti,
                   const int *rb_items, int nrb_items, const int *rb_row0, RowClass c0, const TS0 *arena0, RowClass c1,
                   const TS1 *arena1, const TS0 *varena0, const TS1 *varena1, double *part, const double *tscale,
                   const int *perm_t, int nr, double *y, hipStream_t st, hipEvent_t ev_gather, hipEvent_t ev_vx) {
-    hipLaunchKernelGGL(k_scatter_xc32w, dim3((max(nc, TMAX_SLOTS) + WG - 1) / WG, 4), dim3(WG), 0, st, x, iperm_s, nc,
-                       mu_base, zc, tmax);
-    if (Mma16<TS0>::scaled || Mma16<TS1>::scaled)
-        hipLaunchKernelGGL(k_colmax_c32, dim3((tpad + 31) / 32), dim3(WG), 0, st, zc, nc, xmax);
-    if (ev_gather)
-        (void)hipEventRecord(ev_gather, st);
-    if (nitems > 0) {
-        hipLaunchKernelGGL((k_vx_rrc32<TS0, TS1>), dim3(nitems), dim3(WG), 0, st, items, varena0, varena1, zc, xmax,
-                           tmax, part, tscale);
-        if (nmulti > 0)
-            hipLaunchKernelGGL((k_vx_sum_c<float, 32>), dim3(nmulti, 4), dim3(WG), 0, st, multi_tiles, nmulti, part,
-                               zc, tscale, tmax);
-    }
-    if (ev_vx)
-        (void)hipEventRecord(ev_vx, st);
-    if (nrb_items > 0)
-        hipLaunchKernelGGL((k_rows_rrc32<TS0, TS1>), dim3(nrb_items), dim3(WG), 0, st, rb_items, rb_row0, c0, arena0,
-                           c1, arena1, zc, tmax, perm_t, nr, mu_base, y);
+    auto run = [&](auto muc_tag) {
+        constexpr int MUC = decltype(muc_tag)::value;
+        hipLaunchKernelGGL(k_scatter_xc32w<MUC>, dim3((max(nc, TMAX_SLOTS) + WG - 1) / WG, MUC / 8), dim3(WG), 0, st,
+                           x, iperm_s, nc, mu_base, zc, tmax);
+        if (Mma16<TS0>::scaled || Mma16<TS1>::scaled)
+            hipLaunchKernelGGL(k_colmax_c32<MUC>, dim3((tpad + 31) / 32), dim3(WG), 0, st, zc, nc, xmax);
+        if (ev_gather)
+            (void)hipEventRecord(ev_gather, st);
+        if (nitems > 0) {
+            hipLaunchKernelGGL((k_vx_rrc32<TS0, TS1, MUC>), dim3(nitems), dim3(WG), 0, st, items, varena0, varena1, zc,
+                               xmax, tmax, part, tscale);
+            if (nmulti > 0)
+                hipLaunchKernelGGL((k_vx_sum_c<float, MUC>), dim3(nmulti, 4), dim3(WG), 0, st, multi_tiles, nmulti,
+                                   part, zc, tscale, tmax);
+        }
+        if (ev_vx)
+            (void)hipEventRecord(ev_vx, st);
+        if (nrb_items > 0)
+            hipLaunchKernelGGL((k_rows_rrc32<TS0, TS1, MUC>), dim3(nrb_items), dim3(WG), 0, st, rb_items, rb_row0, c0,
+                               arena0, c1, arena1, zc, tmax, perm_t, nr, mu_base, y);
+    };
+    if (muc == 64)
+        run(std::integral_constant<int, 64>{});
+    else
+        run(std::integral_constant<int, 32>{});
 }
-#define HTG_RRC32_INST(TS0, TS1)                                                                                  \
-    template void launch_rrc32<TS0, TS1>(const double *, const int *, int, int, int, float *, float *, unsigned *, \
-                                         const VTile *, int, const VTile *, int, const int *, int, const int *,    \
-                                         RowClass, const TS0 *, RowClass, const TS1 *, const TS0 *, const TS1 *,   \
-                                         double *, const double *, const int *, int, double *, hipStream_t,        \
-                                         hipEvent_t, hipEvent_t);
+#define HTG_RRC32_INST(TS0, TS1)                                                                                   \
+    template void launch_rrc32<TS0, TS1>(int, const double *, const int *, int, int, int, float *, float *,         \
+                                         unsigned *, const VTile *, int, const VTile *, int, const int *, int,      \
+                                         const int *, RowClass, const TS0 *, RowClass, const TS1 *, const TS0 *,    \
+                                         const TS1 *, double *, const double *, const int *, int, double *,         \
+                                         hipStream_t, hipEvent_t, hipEvent_t);
 HTG_RRC32_INST(float, float)
 HTG_RRC32_INST(float, __half)
 HTG_RRC32_INST(float, __hip_bfloat16)

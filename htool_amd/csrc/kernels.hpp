@@ -207,13 +207,14 @@ void launch_rrc64(const VTile *items, int nitems, const VTile *multi_tiles, int 
                   const double *tscale, const int *perm_t, int nr, int mu_base, double *y, hipStream_t st,
                   hipEvent_t ev_vx_done);
 
-// Complex product in FP32 arithmetic, 32 right-hand sides: the gather into zc
-// (RHS-interleaved (re, im) FP32 pairs, (tpad + t) x 32), the per-block |x|
-// maxima, the V phase by row ranges (items of the FP64 cut), its partial sums,
-// an event after each of the gather and the V phase (if given), then the row
-// phase by row-block parts. tmax holds TMAX_SLOTS.
+// Complex product in FP32 arithmetic, muc = 32 or 64 right-hand sides: the
+// gather into zc (RHS-interleaved (re, im) FP32 pairs, (tpad + t) x muc), the
+// per-block |x| maxima, the V phase by row ranges (items of the FP64 cut), its
+// partial sums (part: muc complex entries per row), an event after each of the
+// gather and the V phase (if given), then the row phase by row-block parts.
+// tmax holds TMAX_SLOTS.
 template <typename TS0, typename TS1>
-void launch_rrc32(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *zc, float *xmax,
+void launch_rrc32(int muc, const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *zc, float *xmax,
                   unsigned *tmax, const VTile *items, int nitems, const VTile *multi_tiles, int nmulti,
                   const int *rb_items, int nrb_items, const int *rb_row0, RowClass c0, const TS0 *arena0, RowClass c1,
                   const TS1 *arena1, const TS0 *varena0, const TS1 *varena1, double *part, const double *tscale,

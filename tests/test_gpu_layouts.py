@@ -69,11 +69,13 @@ def test_all_paths_against_fp64(htg, geom, n, eta, lowprec):
 
 @pytest.mark.parametrize("all_terms", [False, True])
 def test_complex_paths_against_fp64(htg, all_terms):
-    """Complex Helmholtz H-matrix: single-vector, narrow multi-RHS, 16-wide and
-    32-wide matrix-core passes (FP32, and FP16/BF16 on the 16-bit cores; 40
-    RHS = one 32-wide pass and an 8-wide one) against the complex FP64 product,
-    for three work decompositions; columns of a 40-RHS product match
-    single-vector products of the same storage, FP64 included."""
+    """Complex Helmholtz H-matrix: single-vector, narrow multi-RHS, 16-wide,
+    32-wide and (FP32 arithmetic) 64-wide matrix-core passes (FP32, and
+    FP16/BF16 on the 16-bit cores; 40 RHS = one 32-wide pass and an 8-wide one,
+    72 = a 64-wide pass and an 8-wide one, or two 32-wide and an 8-wide in FP64)
+    against the complex FP64 product, for three work decompositions; columns of
+    a 72-RHS product match single-vector products of the same storage, FP64
+    included."""
     print(f"complex layouts all_terms={all_terms}", flush=True)
     n = 6000
     x = geo.ellipsoid(n, (1.0, 1.0, 4.0), seed=31)
@@ -85,10 +87,10 @@ def test_complex_paths_against_fp64(htg, all_terms):
     H.build(htg.Generator(geo.KERNEL_HELMHOLTZ, 0.0, geo.laplace_diag(h), k), x)
     assert H.get_info().lowprec_entries > 0
     rng = np.random.default_rng(32)
-    X = rng.standard_normal((n, 40)) + 1j * rng.standard_normal((n, 40))
+    X = rng.standard_normal((n, 72)) + 1j * rng.standard_normal((n, 72))
     H.set_storage("fp64")
     Y64 = H.mvprod_global_to_global(X)
-    for c in (0, 17, 31, 33, 39):
+    for c in (0, 17, 31, 33, 39, 63, 71):
         assert rel(Y64[:, c], H.mvprod_global_to_global(X[:, c])) < 1e-12, c
     bounds = {"fp32": 2e-6, "fp32_fp16": 2e-3 if all_terms else 2e-5, "fp32_bf16": 2e-2 if all_terms else 2e-4}
     for li, lay in enumerate([dict(), dict(vchunk_el=256, vgroup_el=256, vgroup_tiles=1), dict(vrows=32, rb_max=64)]):
@@ -96,9 +98,10 @@ def test_complex_paths_against_fp64(htg, all_terms):
             H.set_layout(**lay)
         for storage, bound in bounds.items():
             H.set_storage(storage)
-            for mu in (1, 3, 16, 20, 32, 40):
+            for mu in (1, 3, 16, 20, 32, 40, 64):
                 e = rel(H.mvprod_global_to_global(X[:, :mu]), Y64[:, :mu])
                 assert e < bound, (li, storage, mu, e)
             Y = H.mvprod_global_to_global(X)
-            for c in (0, 15, 17, 31, 39):
+            assert rel(Y, Y64) < bound, (li, storage)
+            for c in (0, 15, 17, 31, 39, 47, 63, 64, 71):
                 assert rel(Y[:, c], H.mvprod_global_to_global(X[:, c])) < 1e-6, (li, storage, c)
