@@ -53,6 +53,7 @@ constexpr int ROWS_SPLIT = 32;
 // into several chunks (its partial sums are added by a second pass).
 struct VTileList {
     std::vector<VTile> tiles, multi;
+    int n0 = 0; // row-range lists: tiles [0, n0) are class 0 (XCD-ordered), the rest class 1
     std::vector<int> vg_off;
     DevArray<VTile> d_tiles, d_multi;
     DevArray<int> d_vg_off;
@@ -528,7 +529,9 @@ void build_layout(htg_hmatrix &h) {
         T.vg_off.assign(1, 0);
         const int cw  = cut == 3 ? RR_ITEM_CHUNK32 : RR_ITEM_CHUNK64;
         int part_size = 0;
-        std::vector<std::vector<VTile>> groups(h.panels.size()); // per panel, both classes
+        std::vector<std::vector<VTile>> groups[2]; // per class, per panel
+        groups[0].resize(h.panels.size());
+        groups[1].resize(h.panels.size());
         for (int c = 0; c < 2; c++) {
             const ClassLayout &C = h.cls[c];
             for (size_t pi = 0; pi < h.panels.size(); pi++) {
@@ -569,22 +572,27 @@ void build_layout(htg_hmatrix &h) {
                         v.cls       = c;
                         if (np > 1 && q == 0)
                             T.multi.push_back(v);
-                        groups[pi].push_back(v);
+                        groups[c][pi].push_back(v);
                     }
                 }
             }
         }
-        // a panel's items on one XCD, those reading the same x columns adjacent;
-        // weights: entries, 16-bit rows counting half
-        std::vector<int64_t> gw(groups.size(), 0);
-        for (size_t pi = 0; pi < groups.size(); pi++) {
-            std::stable_sort(groups[pi].begin(), groups[pi].end(), [](const VTile &a, const VTile &b) {
-                return a.cls != b.cls ? a.cls < b.cls : a.x_off < b.x_off;
-            });
-            for (const VTile &v : groups[pi])
-                gw[pi] += (int64_t)v.k * v.ncols * (v.cls == 0 ? 2 : 1);
+        // per class (class 0 first; their lists can run as separate launches), a
+        // panel's items on one XCD, those reading the same x columns adjacent
+        T.tiles.clear();
+        for (int c = 0; c < 2; c++) {
+            std::vector<int64_t> gw(groups[c].size(), 0);
+            for (size_t pi = 0; pi < groups[c].size(); pi++) {
+                std::stable_sort(groups[c][pi].begin(), groups[c][pi].end(),
+                                 [](const VTile &a, const VTile &b) { return a.x_off < b.x_off; });
+                for (const VTile &v : groups[c][pi])
+                    gw[pi] += (int64_t)v.k * v.ncols;
+            }
+            const std::vector<VTile> l = xcd_order(groups[c], gw, VTile{}); // padding: k = 0
+            T.tiles.insert(T.tiles.end(), l.begin(), l.end());
+            if (c == 0)
+                T.n0 = (int)T.tiles.size();
         }
-        T.tiles = xcd_order(groups, gw, VTile{}); // padding: k = 0
         h.part_size = std::max(h.part_size, part_size);
     }
 }
@@ -738,7 +746,7 @@ void product_c(htg_hmatrix &h, const double *d_x, double *d_y, int mu, hipStream
                 const ClassLayout &C0 = h.cls[0], &C1 = h.cls[1];
                 const VTileList &S = h.vt[4];
                 launch_rrc32<TS0, TS1>(MU, d_x, h.d_iperm_s.p, h.nc, h.tpad, done, h.d_xstage.p, h.d_xmax.p, h.d_tmax.p,
-                                       S.d_tiles.p, (int)S.tiles.size(), S.d_multi.p, (int)S.multi.size(),
+                                       S.d_tiles.p, (int)S.tiles.size(), S.n0, S.d_multi.p, (int)S.multi.size(),
                                        h.d_rb_items.p, (int)h.rb_items.size(), h.d_rb_row0.p, rc0,
                                        arena_of<TS0>(C0, false), rc1, has1 ? arena_of<TS1>(C1, false) : nullptr,
                                        arena_of<TS0>(C0, true), has1 ? arena_of<TS1>(C1, true) : nullptr, h.d_part.p,

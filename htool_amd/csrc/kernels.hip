@@ -4197,8 +4197,10 @@ __device__ __forceinline__ void rrc16_load(float2 (&v)[8], const float2 *z2, int
 
 // One work item's rows (wave w: rows 16 w + r) over the columns walked by
 // rr_walk / rr_walk2; acc: [RHS sixteen][re, im][row] FP64 sums, fed by FP32
-// sums over one unit (FP32 entries) or two (16-bit entries).
-template <typename TS, int MUC, typename XS, typename Load, typename XLoad, typename XStore>
+// sums over one unit (FP32 entries) or two (16-bit entries; ACC32: over the
+// whole walk, for 16-bit entries whose own rounding dwarfs that of FP32 sums,
+// which keeps the FP64 sums out of the loop's registers).
+template <typename TS, int MUC, typename XS, bool ACC32, typename Load, typename XLoad, typename XStore>
 __device__ __forceinline__ void rrc32_walk(int nchunk, bool real, const unsigned char *lds, float sc, Load load,
                                            XLoad xload, XStore xstore, double (&accd)[MUC / 16][2][4]) {
     constexpr bool H16 = Mma16<TS>::on;
@@ -4282,17 +4284,19 @@ __device__ __forceinline__ void rrc32_walk(int nchunk, bool real, const unsigned
                         P16[h][1] = M::mma16(aim[st], xr, P16[h][1]);
                     }
                 }
-            // FP32 sums over two units, then FP64
-            if ((u & 1) || u == U - 1) {
+            // FP32 sums over two units, then FP64 (ACC32: over the whole walk)
+            if constexpr (!ACC32) {
+                if ((u & 1) || u == U - 1) {
 #pragma unroll
-                for (int h = 0; h < NH; h++)
+                    for (int h = 0; h < NH; h++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        accd[h][0][i] += (double)P16[h][0][i] * inv;
-                        accd[h][1][i] += (double)P16[h][1][i] * inv;
-                        P16[h][0][i] = 0.0f;
-                        P16[h][1][i] = 0.0f;
-                    }
+                        for (int i = 0; i < 4; i++) {
+                            accd[h][0][i] += (double)P16[h][0][i] * inv;
+                            accd[h][1][i] += (double)P16[h][1][i] * inv;
+                            P16[h][0][i] = 0.0f;
+                            P16[h][1][i] = 0.0f;
+                        }
+                }
             }
         }
     };
@@ -4302,6 +4306,17 @@ __device__ __forceinline__ void rrc32_walk(int nchunk, bool real, const unsigned
         XS x;
         rr_walk<UPC, RrcUnit>(
             nchunk, load, run, [&](int j) { xload(j, x); }, [&](int j) { xstore(j, x); });
+    }
+    if constexpr (H16 && ACC32) {
+        if (real) {
+#pragma unroll
+            for (int h = 0; h < NH; h++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    accd[h][0][i] += (double)P16[h][0][i] * inv;
+                    accd[h][1][i] += (double)P16[h][1][i] * inv;
+                }
+        }
     }
 }
 
@@ -4348,71 +4363,62 @@ __global__ void k_scatter_xc32w(const double *x, const int *iperm_s, int nc, int
     }
 }
 
-template <typename TS0, typename TS1, int MUC>
-__global__ void __launch_bounds__(WG) k_vx_rrc32(const VTile *items, const TS0 *varena0, const TS1 *varena1,
+// The V phase of one launch: items of storage TS (d.cls picks the arena);
+// ACC32 for 16-bit storage (FP32 sums over the item).
+template <typename TS, int MUC, bool ACC32>
+__global__ void __launch_bounds__(WG) k_vx_rrc32(const VTile *items, const TS *varena0, const TS *varena1,
                                                  float *zc, const float *xmax, unsigned *tmax, double *part,
                                                  const double *tscale) {
     constexpr int CH = rrc_chunk<MUC>(), QP = MUC / 2, NH = MUC / 16;
-    constexpr int B0 = rrc32_stage_bytes<TS0, MUC>(), B1 = rrc32_stage_bytes<TS1, MUC>();
-    __shared__ __align__(16) unsigned char lds[2 * (B0 > B1 ? B0 : B1)];
+    constexpr bool H16 = Mma16<TS>::on;
+    constexpr int BUF = rrc32_stage_bytes<TS, MUC>();
+    __shared__ __align__(16) unsigned char lds[2 * BUF];
     const VTile d = items[blockIdx.x];
     if (d.k == 0) // padding entry of an XCD-ordered list
         return;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = lane & 15, g = lane >> 4;
     const int K = d.k, nchunk = (d.ncols + CH - 1) / CH;
-    const float4 *xsrc = reinterpret_cast<const float4 *>(zc) + (size_t)d.x_off * QP;
-    auto xload = [&](int j, RrcX32 &x) {
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int f = (int)threadIdx.x + WG * i;
-            x.v[i] = xsrc[(size_t)min(j * CH + f / QP, d.ncols - 1) * QP + f % QP];
-        }
-    };
+    const TS *varena = d.cls == 0 ? varena0 : varena1;
     double accd[NH][2][4] = {};
-    auto go = [&](auto tag, const auto *varena) {
-        using TS = decltype(tag);
-        constexpr bool H16 = Mma16<TS>::on;
-        constexpr int NS = H16 ? (MUC == 32 ? 2 : 1) : 4, SC = H16 ? 32 : 8, LPS = H16 ? 2 : 1; // loads per step
-        const int nstep = (d.ncols + SC - 1) / SC;
-        // lane (r, g): row 16 w + r, columns (2 or 8) g .. of each step
-        const TS *rp = varena + 2 * (d.v_off + (size_t)min(16 * w + r, K - 1) * d.ld) + (H16 ? 16 : 4) * g;
-        auto load = [&](int u, RrcUnit &b) {
+    constexpr int NS = H16 ? (MUC == 32 ? 2 : 1) : 4, SC = H16 ? 32 : 8, LPS = H16 ? 2 : 1; // loads per step
+    const int nstep = (d.ncols + SC - 1) / SC;
+    // lane (r, g): row 16 w + r, columns (2 or 8) g .. of each step
+    const TS *rp = varena + 2 * (d.v_off + (size_t)min(16 * w + r, K - 1) * d.ld) + (H16 ? 16 : 4) * g;
+    auto load = [&](int u, RrcUnit &b) {
 #pragma unroll
-            for (int st = 0; st < NS; st++)
+        for (int st = 0; st < NS; st++)
 #pragma unroll
-                for (int l = 0; l < LPS; l++)
-                    b.a[st * LPS + l] = nt_load16(rp + 2 * (size_t)min(u * NS + st, nstep - 1) * SC + 8 * l);
-        };
-        // FP16: one scale for the item (the largest over its chunks), so the unit sums share it
-        float sc = 1.0f;
-        if constexpr (H16 && Mma16<TS>::scaled) {
-            const int c0 = d.x_off, c1 = d.x_off + d.ncols - 1;
-            float m = 0.0f;
-            for (int b = c0 >> 5; b <= (c1 >> 5); b++)
-                m = fmaxf(m, xmax[b]);
-            sc = pass_scale(m);
-        }
-        unsigned char *buf = lds;
-        if constexpr (H16) {
-            // thread (s, cg): right-hand side s of chunk columns 8 cg .. 8 cg + 7
-            const float2 *z2 = reinterpret_cast<const float2 *>(zc);
-            auto xload16 = [&](int j, RrcX16 &x) { rrc16_load<MUC>(x.v, z2, j, d.ncols, d.x_off); };
-            auto xstore16 = [&](int j, const RrcX16 &x) {
-                rrc16_store<TS, MUC>(buf + (j & 1) * rrc32_stage_bytes<TS, MUC>(), x.v, sc);
-            };
-            rrc32_walk<TS, MUC, RrcX16>(nchunk, 16 * w < K, lds, sc, load, xload16, xstore16, accd);
-        } else {
-            auto xstore = [&](int j, const RrcX32 &x) {
-                rrc32_store<MUC>(buf + (j & 1) * rrc32_stage_bytes<TS, MUC>(), x.v, j, d.ncols);
-            };
-            rrc32_walk<TS, MUC, RrcX32>(nchunk, 16 * w < K, lds, sc, load, xload, xstore, accd);
-        }
+            for (int l = 0; l < LPS; l++)
+                b.a[st * LPS + l] = nt_load16(rp + 2 * (size_t)min(u * NS + st, nstep - 1) * SC + 8 * l);
     };
-    if (d.cls == 0)
-        go(TS0{}, varena0);
-    else
-        go(TS1{}, varena1);
+    // FP16: one scale for the item (the largest over its chunks), so the unit sums share it
+    float sc = 1.0f;
+    if constexpr (H16 && Mma16<TS>::scaled) {
+        const int c0 = d.x_off, c1 = d.x_off + d.ncols - 1;
+        float m = 0.0f;
+        for (int b = c0 >> 5; b <= (c1 >> 5); b++)
+            m = fmaxf(m, xmax[b]);
+        sc = pass_scale(m);
+    }
+    if constexpr (H16) {
+        // thread (s, cg): right-hand side s of chunk columns 8 cg .. 8 cg + 7
+        const float2 *z2 = reinterpret_cast<const float2 *>(zc);
+        auto xload16 = [&](int j, RrcX16 &x) { rrc16_load<MUC>(x.v, z2, j, d.ncols, d.x_off); };
+        auto xstore16 = [&](int j, const RrcX16 &x) { rrc16_store<TS, MUC>(lds + (j & 1) * BUF, x.v, sc); };
+        rrc32_walk<TS, MUC, RrcX16, ACC32>(nchunk, 16 * w < K, lds, sc, load, xload16, xstore16, accd);
+    } else {
+        const float4 *xsrc = reinterpret_cast<const float4 *>(zc) + (size_t)d.x_off * QP;
+        auto xload = [&](int j, RrcX32 &x) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int f = (int)threadIdx.x + WG * i;
+                x.v[i] = xsrc[(size_t)min(j * CH + f / QP, d.ncols - 1) * QP + f % QP];
+            }
+        };
+        auto xstore = [&](int j, const RrcX32 &x) { rrc32_store<MUC>(lds + (j & 1) * BUF, x.v, j, d.ncols); };
+        rrc32_walk<TS, MUC, RrcX32, false>(nchunk, 16 * w < K, lds, sc, load, xload, xstore, accd);
+    }
     float m = 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; i++) { // lane (r, g): rows 4 g + i, RHS 16 h + r
@@ -4481,7 +4487,7 @@ __global__ void __launch_bounds__(WG) k_rows_rrc32(const int *items, const int *
             auto xstore16 = [&](int j, const RrcX16 &x) {
                 rrc16_store<TS, MUC>(lds + (j & 1) * rrc32_stage_bytes<TS, MUC>(), x.v, sc);
             };
-            rrc32_walk<TS, MUC, RrcX16>(nchunk, 16 * w < R, lds, sc, load, xload16, xstore16, accd);
+            rrc32_walk<TS, MUC, RrcX16, true>(nchunk, 16 * w < R, lds, sc, load, xload16, xstore16, accd);
         } else {
             auto xload = [&](int j, RrcX32 &x) {
                 int idx[4];
@@ -4495,7 +4501,7 @@ __global__ void __launch_bounds__(WG) k_rows_rrc32(const int *items, const int *
             auto xstore = [&](int j, const RrcX32 &x) {
                 rrc32_store<MUC>(lds + (j & 1) * rrc32_stage_bytes<TS, MUC>(), x.v, j, C);
             };
-            rrc32_walk<TS, MUC, RrcX32>(nchunk, 16 * w < R, lds, sc, load, xload, xstore, accd);
+            rrc32_walk<TS, MUC, RrcX32, false>(nchunk, 16 * w < R, lds, sc, load, xload, xstore, accd);
         }
     };
     go(TS0{}, c0, arena0, 1.0f);
@@ -4903,7 +4909,8 @@ void launch_rrc64(const VTile *items, int nitems, const VTile *multi_tiles, int 
 
 template <typename TS0, typename TS1>
 void launch_rrc32(int muc, const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *zc,
-                  float *xmax, unsigned *tmax, const VTile *items, int nitems, const VTile *multi_tiles, int nmulti,
+                  float *xmax, unsigned *tmax, const VTile *items, int nitems, int nitems0, const VTile *multi_tiles,
+                  int nmulti,
                   const int *rb_items, int nrb_items, const int *rb_row0, RowClass c0, const TS0 *arena0, RowClass c1,
                   const TS1 *arena1, const TS0 *varena0, const TS1 *varena1, double *part, const double *tscale,
                   const int *perm_t, int nr, double *y, hipStream_t st, hipEvent_t ev_gather, hipEvent_t ev_vx) {
@@ -4916,8 +4923,17 @@ void launch_rrc32(int muc, const double *x, const int *iperm_s, int nc, int tpad
         if (ev_gather)
             (void)hipEventRecord(ev_gather, st);
         if (nitems > 0) {
-            hipLaunchKernelGGL((k_vx_rrc32<TS0, TS1, MUC>), dim3(nitems), dim3(WG), 0, st, items, varena0, varena1, zc,
-                               xmax, tmax, part, tscale);
+            if constexpr (std::is_same<TS0, TS1>::value) {
+                hipLaunchKernelGGL((k_vx_rrc32<TS0, MUC, false>), dim3(nitems), dim3(WG), 0, st, items, varena0,
+                                   varena1, zc, xmax, tmax, part, tscale);
+            } else { // items [0, nitems0) are class 0, the rest class 1
+                if (nitems0 > 0)
+                    hipLaunchKernelGGL((k_vx_rrc32<TS0, MUC, false>), dim3(nitems0), dim3(WG), 0, st, items, varena0,
+                                       varena0, zc, xmax, tmax, part, tscale);
+                if (nitems > nitems0)
+                    hipLaunchKernelGGL((k_vx_rrc32<TS1, MUC, true>), dim3(nitems - nitems0), dim3(WG), 0, st,
+                                       items + nitems0, varena1, varena1, zc, xmax, tmax, part, tscale);
+            }
             if (nmulti > 0)
                 hipLaunchKernelGGL((k_vx_sum_c<float, MUC>), dim3(nmulti, 4), dim3(WG), 0, st, multi_tiles, nmulti,
                                    part, zc, tscale, tmax);
@@ -4935,7 +4951,7 @@ void launch_rrc32(int muc, const double *x, const int *iperm_s, int nc, int tpad
 }
 #define HTG_RRC32_INST(TS0, TS1)                                                                                   \
     template void launch_rrc32<TS0, TS1>(int, const double *, const int *, int, int, int, float *, float *,         \
-                                         unsigned *, const VTile *, int, const VTile *, int, const int *, int,      \
+                                         unsigned *, const VTile *, int, int, const VTile *, int, const int *, int, \
                                          const int *, RowClass, const TS0 *, RowClass, const TS1 *, const TS0 *,    \
                                          const TS1 *, double *, const double *, const int *, int, double *,         \
                                          hipStream_t, hipEvent_t, hipEvent_t);

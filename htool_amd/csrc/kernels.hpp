@@ -212,10 +212,11 @@ void launch_rrc64(const VTile *items, int nitems, const VTile *multi_tiles, int 
 // per-block |x| maxima, the V phase by row ranges (items of the FP64 cut), its
 // partial sums (part: muc complex entries per row), an event after each of the
 // gather and the V phase (if given), then the row phase by row-block parts.
-// tmax holds TMAX_SLOTS.
+// tmax holds TMAX_SLOTS. With two storage types, items [0, nitems0) are the
+// class-0 items and the rest class 1 (one V launch each).
 template <typename TS0, typename TS1>
 void launch_rrc32(int muc, const double *x, const int *iperm_s, int nc, int tpad, int mu_base, float *zc, float *xmax,
-                  unsigned *tmax, const VTile *items, int nitems, const VTile *multi_tiles, int nmulti,
+                  unsigned *tmax, const VTile *items, int nitems, int nitems0, const VTile *multi_tiles, int nmulti,
                   const int *rb_items, int nrb_items, const int *rb_row0, RowClass c0, const TS0 *arena0, RowClass c1,
                   const TS1 *arena1, const TS0 *varena0, const TS1 *varena1, double *part, const double *tscale,
                   const int *perm_t, int nr, double *y, hipStream_t st, hipEvent_t ev_gather, hipEvent_t ev_vx);
