@@ -41,9 +41,12 @@ inline int v_extent(const Leaf &L, int c) { return round_up(L.oj + L.n, CLASS_AL
 inline int v_stride(const Leaf &L, int c) { return round_up(v_extent(L, c), ROW_ALIGN[c]); }
 inline int terms(const Leaf &L, int c) { return L.k <= 0 ? 0 : (c == 0 ? L.k_hi : L.k - L.k_hi); }
 
-// Entries per 32-RHS row-range work item (V phase): the column range of a
-// range of rows is cut so that items hold about this many.
-constexpr int RR_WORK = 32768;
+// Entries per multi-RHS row-range work item (V phase): the column range of a
+// range of rows is cut so that items hold about this many. Each cut costs a
+// partial sum of rows x RHS FP64 entries written and read back (for 64
+// complex RHS, 1 KB per row against 4 bytes per FP16 entry), so items are
+// large; the lists still hold thousands of items.
+constexpr int RR_WORK = 131072;
 
 // Rows per workgroup of the single-RHS row phase with two precision classes (a
 // row block of up to rb_max rows is cut into parts of at most this many).
@@ -140,7 +143,7 @@ struct htg_hmatrix {
     // (several right-hand sides: longer matrix-core passes per chunk). vt[3]
     // and vt[4] are the 32-RHS row ranges (FP32 arithmetic, FP64): at most
     // RR_ITEM_ROWS rows of a panel's class over a column range of about
-    // RR_WORK entries, one workgroup each, heaviest first (vg_off unused).
+    // RR_WORK entries, one workgroup each, in XCD order (vg_off unused).
     VTileList vt[5];
 
     DevArray<int> d_perm_t, d_perm_s, d_rb_row0, d_rb_order, d_rb_parts, d_rb_items;
