@@ -3534,15 +3534,19 @@ struct cpair {
     R re, im;
 };
 
-// FP64 complex MU-wide passes: RHS s0..s0+7 of original entry i to
-// z[(iperm[i] * MU + s) * 2 + {re, im}] (128 contiguous bytes per thread,
-// coalesced reads of consecutive i), zeros for the padding entries [nc, tpad).
+// FP64 complex MU-wide passes: z[(iperm[i] * MU + s) * 2 + {re, im}] = RHS s
+// of original entry i, zeros for the padding entries [nc, tpad). A workgroup
+// covers WG / (MU / 8) consecutive entries whole: thread (e, sg) reads RHS
+// 8 sg .. 8 sg + 7 of its entry (consecutive entries: coalesced) and writes
+// them as 128 contiguous bytes, so each entry's row is written by one
+// workgroup in full cache lines.
 template <int MU>
-__global__ void k_scatter_xc64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, double *z) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(WG) k_scatter_xc64(const double *x, const int *iperm_s, int nc, int tpad,
+                                                     int mu_base, double *z) {
+    constexpr int E = WG / (MU / 8); // entries per workgroup
+    const int i = blockIdx.x * E + (int)threadIdx.x % E, s0 = 8 * ((int)threadIdx.x / E);
     if (i >= tpad)
         return;
-    const int s0 = 8 * blockIdx.y;
     double2 v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++)
@@ -4342,18 +4346,22 @@ __global__ void __launch_bounds__(WG) k_colmax_c32(const float *zc, int nc, floa
         xmax[b] = fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]));
 }
 
-// Complex RHS s0 .. s0 + 7 of original entry i to row iperm[i] of zc as FP32
-// (re, im) pairs (64 contiguous bytes per thread); workgroup 0 also clears the
-// TMAX_SLOTS maxima of the class-1 t entries.
+// zc[(iperm[i] * MUC + s) * 2 + {re, im}] = RHS s of original entry i as FP32.
+// A workgroup covers WG / (MUC / 8) consecutive entries whole: thread (e, sg)
+// reads RHS 8 sg .. 8 sg + 7 of its entry (consecutive entries: coalesced) and
+// writes them as 64 contiguous bytes, so each entry's row is written by one
+// workgroup in full cache lines. Workgroup 0 also clears the TMAX_SLOTS maxima
+// of the class-1 t entries.
 template <int MUC>
-__global__ void k_scatter_xc32w(const double *x, const int *iperm_s, int nc, int mu_base, float *zc,
-                                unsigned *tmax) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.y == 0 && i < TMAX_SLOTS)
-        tmax[i] = 0u;
+__global__ void __launch_bounds__(WG) k_scatter_xc32w(const double *x, const int *iperm_s, int nc, int mu_base,
+                                                      float *zc, unsigned *tmax) {
+    static_assert(TMAX_SLOTS == WG, "one slot per thread of workgroup 0");
+    constexpr int E = WG / (MUC / 8); // entries per workgroup
+    if (blockIdx.x == 0)
+        tmax[threadIdx.x] = 0u;
+    const int i = blockIdx.x * E + (int)threadIdx.x % E, s0 = 8 * ((int)threadIdx.x / E);
     if (i >= nc)
         return;
-    const int s0 = 8 * blockIdx.y;
     float4 *o = reinterpret_cast<float4 *>(zc + ((size_t)iperm_s[i] * MUC + s0) * 2);
 #pragma unroll
     for (int u = 0; u < 8; u += 2) {
@@ -4363,8 +4371,6 @@ __global__ void k_scatter_xc32w(const double *x, const int *iperm_s, int nc, int
     }
 }
 
-// The V phase of one launch: items of storage TS (d.cls picks the arena);
-// ACC32 for 16-bit storage (FP32 sums over the item).
 template <typename TS, int MUC, bool ACC32>
 __global__ void __launch_bounds__(WG) k_vx_rrc32(const VTile *items, const TS *varena0, const TS *varena1,
                                                  float *zc, const float *xmax, unsigned *tmax, double *part,
@@ -4882,10 +4888,10 @@ void launch_gather_xc32(const double *x, const int *iperm_s, int nc, int tpad, i
 void launch_gather_xc64(const double *x, const int *iperm_s, int nc, int tpad, int mu_base, int MU, double *z,
                         hipStream_t st) {
     if (MU == 32)
-        hipLaunchKernelGGL(k_scatter_xc64<32>, dim3((tpad + WG - 1) / WG, 4), dim3(WG), 0, st, x, iperm_s, nc, tpad,
+        hipLaunchKernelGGL(k_scatter_xc64<32>, dim3((tpad + WG / 4 - 1) / (WG / 4)), dim3(WG), 0, st, x, iperm_s, nc, tpad,
                            mu_base, z);
     else
-        hipLaunchKernelGGL(k_scatter_xc64<16>, dim3((tpad + WG - 1) / WG, 2), dim3(WG), 0, st, x, iperm_s, nc, tpad,
+        hipLaunchKernelGGL(k_scatter_xc64<16>, dim3((tpad + WG / 2 - 1) / (WG / 2)), dim3(WG), 0, st, x, iperm_s, nc, tpad,
                            mu_base, z);
 }
 
@@ -4916,7 +4922,7 @@ void launch_rrc32(int muc, const double *x, const int *iperm_s, int nc, int tpad
                   const int *perm_t, int nr, double *y, hipStream_t st, hipEvent_t ev_gather, hipEvent_t ev_vx) {
     auto run = [&](auto muc_tag) {
         constexpr int MUC = decltype(muc_tag)::value;
-        hipLaunchKernelGGL(k_scatter_xc32w<MUC>, dim3((max(nc, TMAX_SLOTS) + WG - 1) / WG, MUC / 8), dim3(WG), 0, st,
+        hipLaunchKernelGGL(k_scatter_xc32w<MUC>, dim3((nc + WG / (MUC / 8) - 1) / (WG / (MUC / 8))), dim3(WG), 0, st,
                            x, iperm_s, nc, mu_base, zc, tmax);
         if (Mma16<TS0>::scaled || Mma16<TS1>::scaled)
             hipLaunchKernelGGL(k_colmax_c32<MUC>, dim3((tpad + 31) / 32), dim3(WG), 0, st, zc, nc, xmax);
