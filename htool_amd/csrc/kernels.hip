@@ -128,6 +128,39 @@ __device__ __forceinline__ T zero_s() {
     return T{};
 }
 
+// A double from another lane through DPP (no LDS round trip): CTRL is the DPP
+// control (quad_perm, row_mirror, row_half_mirror).
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (unsigned long long)(unsigned)lo | (unsigned long long)(unsigned)hi << 32);
+}
+
+// The partner's value for a butterfly stage of offset o (o a power of two,
+// wave-uniform) over groups of lanes: o = 1, 2: lane ^ o (DPP quad_perm); o =
+// 4, 8: the mirrored lane of the 8- or 16-lane group (DPP row_half_mirror,
+// row_mirror), which has the opposite bit o and the same higher bits, so a
+// butterfly over offsets from high to low still sums every lane of the group
+// exactly once (and leaves each lane's row selection unchanged); larger o:
+// ds_bpermute. Cuts the reduction's LDS round trips in the single-vector
+// kernels.
+__device__ __forceinline__ double shfl_pair(double v, int o) {
+    switch (o) {
+    case 1:
+        return dpp_double<0xB1>(v); // quad_perm [1, 0, 3, 2]
+    case 2:
+        return dpp_double<0x4E>(v); // quad_perm [2, 3, 0, 1]
+    case 4:
+        return dpp_double<0x141>(v); // row_half_mirror
+    case 8:
+        return dpp_double<0x140>(v); // row_mirror
+    default:
+        return __shfl_xor(v, o, 64);
+    }
+}
+
 // Largest of a float over the wave.
 __device__ __forceinline__ float wave_max(float m) {
 #pragma unroll
@@ -1954,11 +1987,11 @@ __device__ __forceinline__ void vx_chunk_q(const VTile &d, const TS *varena, flo
             for (int j = 0; j < h; j++) {
                 const double send = up ? acc[j] : acc[j + h];
                 const double keep = up ? acc[j + h] : acc[j];
-                acc[j]            = keep + __shfl_xor(send, o, 64);
+                acc[j]            = keep + shfl_pair(send, o);
             }
         }
         for (int o = SH / 2; o >= 1; o >>= 1)
-            acc[0] += __shfl_xor(acc[0], o, 64);
+            acc[0] += shfl_pair(acc[0], o);
         if (wr) {
             if (d.nparts > 1)
                 part[(size_t)(d.part_off + rfin)] = acc[0];
