@@ -66,10 +66,11 @@ typedef enum htg_compressor {
  * (the low-rank terms assigned to 16-bit storage at build time, see
  * htg_params.lowprec_tol / lowprec_all; empty otherwise). Arithmetic inside a
  * tile is FP32 except in FP64 mode; the sum over tiles and the output are FP64.
- * Multi-RHS products (32 real or 16 complex right-hand sides per pass) run a
- * 16-bit class on the 16-bit matrix cores, with the FP32 vector entries split
- * into 16-bit terms (FP16: two, under a power-of-two scale per pass; BF16:
- * three) and FP32 accumulation. */
+ * Multi-RHS products (32 real right-hand sides per pass; complex: 64 per pass
+ * in FP32 arithmetic, 32 in FP64) run a 16-bit class on the 16-bit matrix
+ * cores, with the FP32 vector entries split into 16-bit terms (FP16: two,
+ * under a power-of-two scale per work item or pass; BF16: three) and FP32
+ * accumulation. */
 typedef enum htg_storage {
     HTG_STORAGE_FP64      = 0, /* class 0 and 1 in FP64 */
     HTG_STORAGE_FP32      = 1, /* class 0 and 1 in FP32 */
