@@ -4205,7 +4205,10 @@ __device__ __forceinline__ void rrc16_store(unsigned char *buf, const float2 (&v
 template <int MUC, typename Col>
 __device__ __forceinline__ void rrc16_load(float2 (&v)[8], const float2 *z2, int j, int ncols, Col col) {
     constexpr int CH = rrc_chunk<MUC>();
-    const int s = (int)threadIdx.x % MUC, c0 = j * CH + 8 * ((int)threadIdx.x / MUC);
+    // 64 RHS: the column group is the wave's (made wave-uniform, so the column
+    // indices are scalar loads)
+    const int cg = MUC == 64 ? __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) : (int)threadIdx.x / MUC;
+    const int s = (int)threadIdx.x % MUC, c0 = j * CH + 8 * cg;
     if (c0 + 8 <= ncols) {
         if constexpr (std::is_same<Col, int>::value) { // entries col + i: one address, immediate offsets
             const float2 *p = z2 + ((size_t)col + c0) * MUC + s;
