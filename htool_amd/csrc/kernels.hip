@@ -2823,14 +2823,22 @@ __device__ __forceinline__ void rb_class32(const RowClass &rc, const TS *arena, 
     const float4 *zf = reinterpret_cast<const float4 *>(zil);
     const int *ci    = rc.colidx + cb;
     float4 xv[XPT];
-    auto xload = [&](int j) { // float4 f: column f / 8 of the chunk, right-hand sides 4 (f % 8) ..
-        int idx[XPT];
+    // column indices one chunk ahead: the walk reads chunks 0, 1, 2, ... in
+    // order (the last one twice, unused), so xload(j) finds chunk j's indices
+    // in registers, loaded with the previous chunk's entries, and issues the
+    // vector loads at once rather than behind a wait on the index loads
+    int idx[XPT];
+    auto iload = [&](int j) {
 #pragma unroll
         for (int i = 0; i < XPT; i++)
             idx[i] = ci[min(j * RR_CHUNK + (((int)threadIdx.x + WG * i) >> 3), C - 1)];
+    };
+    iload(0);
+    auto xload = [&](int j) { // float4 f: column f / 8 of the chunk, right-hand sides 4 (f % 8) ..
 #pragma unroll
         for (int i = 0; i < XPT; i++)
             xv[i] = zf[(size_t)idx[i] * (MFMA_MU / 4) + (((int)threadIdx.x + WG * i) & 7)];
+        iload(j + 1);
     };
     auto xstore = [&](int j) {
         unsigned char *buf = lds + (j & 1) * BUF;
@@ -2980,14 +2988,18 @@ __global__ void __launch_bounds__(WG) k_rows_rr64(const int *items, const int *r
                 b.a[st] = nt_load16(rp + (size_t)min(u * NS + st, nstep - 1) * SC);
         };
         double2 xv[XPT];
-        auto xload = [&](int j) { // double2 f: column f / 16 of the chunk, right-hand sides 2 (f % 16) ..
-            int idx[XPT];
+        int idx[XPT]; // column indices one chunk ahead, as in rb_class32
+        auto iload = [&](int j) {
 #pragma unroll
             for (int i = 0; i < XPT; i++)
                 idx[i] = ci[min(j * RR_CHUNK64 + (((int)threadIdx.x + WG * i) >> 4), C - 1)];
+        };
+        iload(0);
+        auto xload = [&](int j) { // double2 f: column f / 16 of the chunk, right-hand sides 2 (f % 16) ..
 #pragma unroll
             for (int i = 0; i < XPT; i++)
                 xv[i] = z2[(size_t)idx[i] * (MU / 2) + (((int)threadIdx.x + WG * i) & 15)];
+            iload(j + 1);
         };
         auto xstore = [&](int j) {
             double2 *buf = reinterpret_cast<double2 *>(s_x[j & 1]);
@@ -4085,14 +4097,18 @@ __global__ void __launch_bounds__(WG) k_rows_rrc64(const int *items, const int *
                 b.a[st] = nt_load16(rp + 2 * (size_t)min(u * NS + st, nstep - 1) * SC);
         };
         double2 xv[XPT];
-        auto xload = [&](int j) {
-            int idx[XPT];
+        int idx[XPT]; // column indices one chunk ahead, as in rb_class32
+        auto iload = [&](int j) {
 #pragma unroll
             for (int i = 0; i < XPT; i++)
                 idx[i] = ci[min(j * RRC_CHUNK + (((int)threadIdx.x + WG * i) >> 5), C - 1)];
+        };
+        iload(0);
+        auto xload = [&](int j) {
 #pragma unroll
             for (int i = 0; i < XPT; i++)
                 xv[i] = z2[(size_t)idx[i] * MU + (((int)threadIdx.x + WG * i) & 31)];
+            iload(j + 1);
         };
         auto xstore = [&](int j) {
             double2 *buf = reinterpret_cast<double2 *>(s_x) + (size_t)(j & 1) * RRC_CHUNK * MU;
