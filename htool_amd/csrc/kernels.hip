@@ -1661,15 +1661,17 @@ __device__ __forceinline__ int lanes_per_row(int nvec) {
 }
 
 
-// Row-major block times an LDS-staged vector segment, shared by both product
-// phases. Rows [0, nrows) with row stride ld, nvec 16-byte vectors of columns
-// per row. Each row is taken by a group of lanes of one wave (lanes_per_row),
-// so short rows still fill the wave; the group sums its products in TC,
-// reduces in FP64 with shuffles, and its first lane adds the row total into
-// s_acc[row*MU + s].
-template <typename TS, typename TC, int MU>
+// Row-major block times an LDS-staged vector segment. Rows [0, nrows) with
+// row stride ld, nvec 16-byte vectors of columns per row. Each row is taken by
+// a group of lanes of one wave (lanes_per_row), so short rows still fill the
+// wave; the group sums its products in TC, reduces in FP64 with shuffles, and
+// its first lane adds the row total into s_acc[row*MU + s]. stage() fills s_z
+// (every thread calls it once; it ends with a workgroup barrier): for one FP32
+// right-hand side the first matrix vectors are requested before it, so their
+// latency overlaps the staging's.
+template <typename TS, typename TC, int MU, typename Stage>
 __device__ __forceinline__ void rows_gemv(const TS *base, size_t ld, int nrows, int nvec, const TC *s_z,
-                                          double *s_acc) {
+                                          double *s_acc, Stage stage) {
     constexpr int W = Vec<TS>::W;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int lpr  = lanes_per_row(nvec);
@@ -1728,17 +1730,23 @@ __device__ __forceinline__ void rows_gemv(const TS *base, size_t ld, int nrows, 
         nt_u4 a0, b0, a1, b1;
         if (nit > 0)
             issue(0, a0, b0);
+        if (nit > 1)
+            issue(1, a1, b1);
+        stage();
         for (int it = 0; it < nit; it += 2) {
-            if (it + 1 < nit)
-                issue(it + 1, a1, b1);
             consume(it, a0, b0);
             if (it + 2 < nit)
                 issue(it + 2, a0, b0);
-            if (it + 1 < nit)
+            if (it + 1 < nit) {
                 consume(it + 1, a1, b1);
+                if (it + 3 < nit)
+                    issue(it + 3, a1, b1);
+            }
         }
         return;
-    } else if constexpr (MU == 1) {
+    }
+    stage();
+    if constexpr (MU == 1) {
         // two rows per lane group: one staged vector read serves both
         for (int rbase = w * 2 * rpw; rbase < nrows; rbase += NWAVE * 2 * rpw) {
             const int r0 = rbase + sub, r1 = r0 + rpw;
@@ -2217,34 +2225,36 @@ __device__ __forceinline__ void rows_class(const TS *arena, const RowClass &rc, 
     constexpr int ZPT = ZCAP / WG; // staged entries per thread
     for (int cc = 0; cc < Cp; cc += ZCOLS) {
         const int nz = min(ZCOLS, Cp - cc);
-        if constexpr (MU == 1 && sizeof(TC) == 4) {
-            // all column indices of the pass, then all vector entries, in
-            // flight together: two memory latencies per pass (FP32 vectors;
-            // for FP64 the extra registers cost more occupancy than this saves)
-            int idx[ZPT];
+        auto stage = [&]() {
+            if constexpr (MU == 1 && sizeof(TC) == 4) {
+                // all column indices of the pass, then all vector entries, in
+                // flight together: two memory latencies per pass (FP32 vectors;
+                // for FP64 the extra registers cost more occupancy than this saves)
+                int idx[ZPT];
 #pragma unroll
-            for (int i = 0; i < ZPT; i++) {
-                const int c = threadIdx.x + i * WG;
-                idx[i]      = c < nz && cc + c < C ? colidx[c0 + cc + c] : -1;
-            }
-            TC v[ZPT];
+                for (int i = 0; i < ZPT; i++) {
+                    const int c = threadIdx.x + i * WG;
+                    idx[i]      = c < nz && cc + c < C ? colidx[c0 + cc + c] : -1;
+                }
+                TC v[ZPT];
 #pragma unroll
-            for (int i = 0; i < ZPT; i++)
-                v[i] = idx[i] >= 0 ? z[idx[i]] : (TC)0;
+                for (int i = 0; i < ZPT; i++)
+                    v[i] = idx[i] >= 0 ? z[idx[i]] : (TC)0;
 #pragma unroll
-            for (int i = 0; i < ZPT; i++) {
-                const int c = threadIdx.x + i * WG;
-                if (c < nz)
-                    s_z[c] = v[i];
+                for (int i = 0; i < ZPT; i++) {
+                    const int c = threadIdx.x + i * WG;
+                    if (c < nz)
+                        s_z[c] = v[i];
+                }
+            } else {
+                for (int e = threadIdx.x; e < nz * MU; e += WG) {
+                    const int c = cc + e / MU;
+                    s_z[e]      = c < C ? z[(size_t)colidx[c0 + c] * MU + e % MU] : (TC)0;
+                }
             }
-        } else {
-            for (int e = threadIdx.x; e < nz * MU; e += WG) {
-                const int c = cc + e / MU;
-                s_z[e]      = c < C ? z[(size_t)colidx[c0 + c] * MU + e % MU] : (TC)0;
-            }
-        }
-        __syncthreads();
-        rows_gemv<TS, TC, MU>(A + cc, (size_t)Cs, R, nz / W, s_z, s_acc);
+            __syncthreads();
+        };
+        rows_gemv<TS, TC, MU>(A + cc, (size_t)Cs, R, nz / W, s_z, s_acc, stage);
         __syncthreads();
     }
 }
