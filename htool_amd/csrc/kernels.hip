@@ -4551,20 +4551,38 @@ __global__ void __launch_bounds__(WG) k_rows_rrc32(const int *items, const int *
         if constexpr (H16) {
             // thread (s, cg): right-hand side s of chunk columns 8 cg .. 8 cg + 7
             const float2 *z2 = reinterpret_cast<const float2 *>(zc);
-            auto xload16 = [&](int j, RrcX16 &x) { rrc16_load<MUC>(x.v, z2, j, C, [&](int i) { return ci[i]; }); };
+            // column indices one chunk ahead (the walk reads chunks in order,
+            // the last one again unused), as in rb_class32; chunk column
+            // 8 cg + c is entry e[c]
+            const int cg = MUC == 64 ? __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) : (int)threadIdx.x / MUC;
+            int e[8];
+            auto eload = [&](int j) {
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+                    e[c] = ci[min(j * CH + 8 * cg + c, C - 1)];
+            };
+            eload(0);
+            auto xload16 = [&](int j, RrcX16 &x) {
+                rrc16_load<MUC>(x.v, z2, j, C, [&](int i) { return e[i & 7]; });
+                eload(j + 1);
+            };
             auto xstore16 = [&](int j, const RrcX16 &x) {
                 rrc16_store<TS, MUC>(lds + (j & 1) * rrc32_stage_bytes<TS, MUC>(), x.v, sc);
             };
             rrc32_walk<TS, MUC, RrcX16, true>(nchunk, 16 * w < R, lds, sc, load, xload16, xstore16, accd);
         } else {
-            auto xload = [&](int j, RrcX32 &x) {
-                int idx[4];
+            int idx[4]; // column indices one chunk ahead, as above
+            auto iload = [&](int j) {
 #pragma unroll
                 for (int i = 0; i < 4; i++)
                     idx[i] = ci[min(j * CH + ((int)threadIdx.x + WG * i) / QP, C - 1)];
+            };
+            iload(0);
+            auto xload = [&](int j, RrcX32 &x) {
 #pragma unroll
                 for (int i = 0; i < 4; i++)
                     x.v[i] = zf[(size_t)idx[i] * QP + ((int)threadIdx.x + WG * i) % QP];
+                iload(j + 1);
             };
             auto xstore = [&](int j, const RrcX32 &x) {
                 rrc32_store<MUC>(lds + (j & 1) * rrc32_stage_bytes<TS, MUC>(), x.v, j, C);
