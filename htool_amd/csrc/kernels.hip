@@ -2431,6 +2431,7 @@ typedef double doublex4 __attribute__((ext_vector_type(4))); // v_mfma_f64_16x16
 // panel's columns, else partial sums that k_vx_sum adds.
 constexpr int RR_ROWS    = RR_ITEM_ROWS;    // rows per work item (one 16-row tile per wave)
 static_assert((TMAX_SLOTS & (TMAX_SLOTS - 1)) == 0, "slots of the largest class-1 |t|: a power of two");
+static_assert(TMAX_SLOTS % 64 == 0, "the row kernels read the slots 64 to a load");
 constexpr int RR_CHUNK   = RR_ITEM_CHUNK32; // columns per staged chunk, FP32 arithmetic
 constexpr int RR_CHUNK64 = RR_ITEM_CHUNK64; // columns per staged chunk, FP64
 static_assert(RR_ROWS == NWAVE * 16, "one 16-row tile per wave");
@@ -2934,13 +2935,22 @@ __global__ void __launch_bounds__(WG) k_rows_rr(const int *items, const int *rb_
         return;
     const int r0 = rb_row0[rb] + roff, R = min(RR_ROWS, rb_row0[rb + 1] - r0);
     double accd[2][4] = {};
+    // every wave reduces the class-1 scale's slots (lane l: slots l, l + 64,
+    // ...), requested before the class-0 walk and reduced after it
+    constexpr bool SC1 = Mma16<TS1>::on && Mma16<TS1>::scaled;
+    unsigned tv[TMAX_SLOTS / 64] = {};
+    if (SC1 && arena1)
+#pragma unroll
+        for (int i = 0; i < TMAX_SLOTS / 64; i++)
+            tv[i] = tmax[(threadIdx.x & 63) + 64 * i];
     rb_class32<TS0>(c0, arena0, rb, roff, R, zil, 1.0f, lds, accd);
     if (arena1) {
         float tsc = 1.0f;
-        if constexpr (Mma16<TS1>::on && Mma16<TS1>::scaled) {
-            float m = 0.0f; // every wave reduces the slots (lane l: slots l, l + 64, ...)
-            for (int i = threadIdx.x & 63; i < TMAX_SLOTS; i += 64)
-                m = fmaxf(m, __uint_as_float(tmax[i]));
+        if constexpr (SC1) {
+            float m = 0.0f;
+#pragma unroll
+            for (int i = 0; i < TMAX_SLOTS / 64; i++)
+                m = fmaxf(m, __uint_as_float(tv[i]));
             tsc = pass_scale(wave_max(m));
         }
         rb_class32<TS1>(c1, arena1, rb, roff, R, zil, tsc, lds, accd);
@@ -4590,13 +4600,22 @@ __global__ void __launch_bounds__(WG) k_rows_rrc32(const int *items, const int *
             rrc32_walk<TS, MUC, RrcX32, false>(nchunk, 16 * w < R, lds, sc, load, xload, xstore, accd);
         }
     };
+    // the class-1 scale's slots (lane l: slots l, l + 64, ...) are requested
+    // before the class-0 walk and reduced after it
+    constexpr bool SC1 = Mma16<TS1>::on && Mma16<TS1>::scaled;
+    unsigned tv[TMAX_SLOTS / 64] = {};
+    if (SC1 && arena1)
+#pragma unroll
+        for (int i = 0; i < TMAX_SLOTS / 64; i++)
+            tv[i] = tmax[(threadIdx.x & 63) + 64 * i];
     go(TS0{}, c0, arena0, 1.0f);
     if (arena1) {
         float sc = 1.0f;
-        if constexpr (Mma16<TS1>::on && Mma16<TS1>::scaled) {
+        if constexpr (SC1) {
             float m = 0.0f;
-            for (int i = threadIdx.x & 63; i < TMAX_SLOTS; i += 64)
-                m = fmaxf(m, __uint_as_float(tmax[i]));
+#pragma unroll
+            for (int i = 0; i < TMAX_SLOTS / 64; i++)
+                m = fmaxf(m, __uint_as_float(tv[i]));
             sc = pass_scale(wave_max(m));
         }
         go(TS1{}, c1, arena1, sc);
