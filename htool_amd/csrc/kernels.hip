@@ -2571,25 +2571,39 @@ __device__ __forceinline__ void rr_rows32(const VTile &d, const TS *varena, cons
     };
     const float4 *xsrc = reinterpret_cast<const float4 *>(z + (size_t)(d.x_off >> 2) * 128);
     float4 xv[XPT];
+    // FP16 terms: the chunk's power-of-two scale, from the xmax of the (at most
+    // NXB) 32-column blocks it touches. Those are loaded with the chunk's x and
+    // the scale kept in a register from the chunk's store to its last unit: a
+    // load at the store or in a unit would wait on every load issued before it
+    // (loads complete in order), the next unit's matrix entries included.
+    constexpr bool SCL = H16 && Mma16<TS>::scaled;
+    constexpr int NXB  = RR_CHUNK / 32 + 1;
+    float xm[NXB];
+    double inv_cur = 1.0; // 1 / the scale of the chunk in LDS being used
     auto xload = [&](int j) {
 #pragma unroll
         for (int i = 0; i < XPT; i++) {
             const int f = (int)threadIdx.x + WG * i;
             xv[i]       = xsrc[(size_t)min(j * (RR_CHUNK / 4) + (f >> 5), ngrp - 1) * MFMA_MU + (f & 31)];
         }
-    };
-    auto chunk_scale = [&](int j) { // FP16 terms: the chunk's power-of-two scale
-        const int c0 = d.x_off + j * RR_CHUNK, c1 = min(c0 + RR_CHUNK, d.x_off + d.ncols) - 1;
-        float m = 0.0f;
-        for (int b = c0 >> 5; b <= (c1 >> 5); b++)
-            m = fmaxf(m, xmax[b]);
-        return pass_scale(m);
+        if constexpr (SCL) {
+            const int c0 = d.x_off + j * RR_CHUNK, c1 = min(c0 + RR_CHUNK, d.x_off + d.ncols) - 1;
+#pragma unroll
+            for (int k = 0; k < NXB; k++)
+                xm[k] = xmax[min((c0 >> 5) + k, c1 >> 5)];
+        }
     };
     auto xstore = [&](int j) {
         unsigned char *buf = lds + (j & 1) * BUF;
         float sc = 1.0f;
-        if constexpr (H16 && Mma16<TS>::scaled)
-            sc = chunk_scale(j);
+        if constexpr (SCL) {
+            float m = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NXB; k++)
+                m = fmaxf(m, xm[k]);
+            sc      = pass_scale(m);
+            inv_cur = 1.0 / (double)sc;
+        }
 #pragma unroll
         for (int i = 0; i < XPT; i++) {
             const int f = (int)threadIdx.x + WG * i, grp = f >> 5, s = f & 31;
@@ -2660,9 +2674,9 @@ __device__ __forceinline__ void rr_rows32(const VTile &d, const TS *varena, cons
                 }
             }
         }
-        double inv = 1.0;
-        if constexpr (H16 && Mma16<TS>::scaled)
-            inv = 1.0 / (double)chunk_scale(j);
+        // chunk j's store came after every unit of chunk j - 1 and before
+        // this one: inv_cur is chunk j's
+        const double inv = inv_cur;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             accd[0][i] += (double)acc0[i] * inv;
