@@ -199,6 +199,8 @@ def batch_64(torch, log):
                                                      50, 5, s, torch), 4)
         lay, tuned_ms, tried = B.autotune(mu=1, reps=20)
         out["autotune"] = {"layout": lay, "ms": round(tuned_ms, 4), "candidates": tried}
+        out["eager_tuned_layout_ms"] = round(timed(lambda: B.mvprod_device(x.data_ptr(), y.data_ptr(), 1, sp),
+                                                   50, 5, s, torch), 4)
         g = B.capture(x.data_ptr(), y.data_ptr(), 1, sp)
         ms = timed(lambda: g.launch(sp), 50, 5, s, torch)
         out["graph_ms"] = round(ms, 4)

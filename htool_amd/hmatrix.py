@@ -273,8 +273,10 @@ class HMatrix:
         capi.check(capi.lib().htg_hmatrix_set_layout(self._handle(), C.byref(lay)))
 
     def autotune(self, mu: int = 1, reps: int = 10):
-        """Times each candidate layout in the current storage, keeps the fastest;
-        returns (layout dict, ms per product, candidates tried)."""
+        """Times each candidate layout in the current storage, times the three
+        fastest again (2 * reps products) and keeps the fastest of that second
+        timing; returns (layout dict, its second time in ms per product,
+        candidates tried)."""
         lay = capi.htg_layout()
         ms = C.c_double()
         cnt = C.c_int()

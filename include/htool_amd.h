@@ -290,8 +290,10 @@ int htg_hmatrix_set_layout(htg_hmatrix *h, const htg_layout *layout);
 
 /* Autotuner: times `reps` products with `mu` right-hand sides in the current
  * storage for the current layout and 24 candidates (rows per row block x
- * (V chunk size, wavefront share) x stacked V rows), keeps the fastest and
- * returns it with its time (ms per product) and the number of layouts tried. */
+ * (V chunk size, wavefront share) x stacked V rows), times the three fastest
+ * again with 2 * reps products, keeps the fastest of that second timing and
+ * returns it with its second time (ms per product) and the number of layouts
+ * tried. */
 int htg_hmatrix_autotune(htg_hmatrix *h, int mu, int reps, htg_layout *best, double *best_ms, int *ncandidates);
 
 /* Captures one product of device buffers d_x -> d_y (mu right-hand sides, the
