@@ -149,15 +149,21 @@ def test_single_rhs_chunk_cuts(htg, vchunk):
             assert rel(y1, Y64[:, c]) < bound, (storage, c)
 
 
+@pytest.mark.parametrize("spread", [False, True])
 @pytest.mark.parametrize("scale", [1e-7, 1.0, 1e7])
-def test_matrix_core_16bit_terms_any_magnitude(htg, scale):
+def test_matrix_core_16bit_terms_any_magnitude(htg, scale, spread):
     """32-RHS products of 16-bit stored blocks run on the 16-bit matrix cores
     with the vector entries split into 16-bit terms (FP16: under a power-of-two
-    pass scale): results stay finite and agree with the single-vector product
-    (FP32 arithmetic) whatever the magnitude of x, far outside FP16's range."""
+    scale per staged chunk of x, and one for the t entries): results stay
+    finite and agree with the single-vector product (FP32 arithmetic) whatever
+    the magnitude of x, far outside FP16's range, and for x whose entries
+    spread over ten orders of magnitude (chunks of very different scales)."""
     x, diag = sphere_case(7000, 14)
     H = build(htg, x, diag, rc=2e-6, lp=5e-7)
-    X = np.random.default_rng(15).standard_normal((x.shape[0], 32)) * scale
+    rng = np.random.default_rng(15)
+    X = rng.standard_normal((x.shape[0], 32)) * scale
+    if spread:
+        X *= 10.0 ** rng.uniform(-5, 5, (x.shape[0], 1))
     for storage in ("fp32_fp16", "fp32_bf16"):
         H.set_storage(storage)
         Y = H.mvprod_global_to_global(X)
